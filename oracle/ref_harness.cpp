@@ -1,0 +1,306 @@
+// Harness around the compiled reference (barntsen/rockseis lib/*.cpp, built
+// from /root/reference by oracle/Makefile into oracle/_ref/librsref.so).
+//
+// TEST INFRASTRUCTURE ONLY.  It drives the reference's own classes (Der,
+// Pml, Model*, Data*, Waves*, Revolve, FwiAcoustic2D) on caller-provided
+// arrays so tests/golden/make_golden.py can write fixtures and bench.py can
+// time the reference CPU path.  Nothing in the product links it.
+//
+// Every entry point is extern "C" with plain pointers; all arrays are
+// float32, x fastest.
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "der.h"
+#include "pml.h"
+#include "model.h"
+#include "data.h"
+#include "waves.h"
+#include "revolve.h"
+#include "fwi.h"
+#include "image.h"
+
+using namespace rockseis;
+
+namespace {
+
+template <typename P>
+void fill2(std::shared_ptr<Data2D<float>> d, const float *coords, bool src) {
+    Point2D<float> *c = src ? d->getGeom()->getScoords() : d->getGeom()->getGcoords();
+    for (size_t i = 0; i < d->getNtrace(); i++) {
+        c[i].x = coords[2 * i];
+        c[i].y = coords[2 * i + 1];
+    }
+}
+
+void copy_map2(std::shared_ptr<Data2D<float>> d, bool smap, int *map, float *shift) {
+    Point2D<int> *m = smap ? d->getGeom()->getSmap() : d->getGeom()->getGmap();
+    Point2D<float> *s = smap ? d->getGeom()->getSshift() : d->getGeom()->getGshift();
+    for (size_t i = 0; i < d->getNtrace(); i++) {
+        map[2 * i] = m[i].x;
+        map[2 * i + 1] = m[i].y;
+        shift[2 * i] = s[i].x;
+        shift[2 * i + 1] = s[i].y;
+    }
+}
+
+rs_field field_of(int f) {
+    switch (f) {
+    case 1: return VX;
+    case 2: return VY;
+    case 3: return VZ;
+    default: return PRESSURE;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// which: 0 ddx_fw 1 ddy_fw 2 ddz_fw 3 ddx_bw 4 ddy_bw 5 ddz_bw
+void ref_der(const float *f, float *df, int nx, int ny, int nz, float dx, float dy, float dz,
+             int order, int which) {
+    Der<float> der(nx, ny, nz, dx, dy, dz, order);
+    std::vector<float> buf(f, f + (size_t)nx * ny * nz);
+    switch (which) {
+    case 0: der.ddx_fw(buf.data()); break;
+    case 1: der.ddy_fw(buf.data()); break;
+    case 2: der.ddz_fw(buf.data()); break;
+    case 3: der.ddx_bw(buf.data()); break;
+    case 4: der.ddy_bw(buf.data()); break;
+    default: der.ddz_bw(buf.data()); break;
+    }
+    std::memcpy(df, der.getDf(), sizeof(float) * (size_t)nx * ny * nz);
+}
+
+void ref_pml_abc(int lpml, float dt, float smax, int sign, float *tab) {
+    Pml<float> p(lpml, dt);
+    p.setSmax(smax);
+    p.computeABC(sign);
+    float *t[12] = {p.A_ltf, p.B_ltf, p.C_ltf, p.A_ltf_stag, p.B_ltf_stag, p.C_ltf_stag,
+                    p.A_rbb, p.B_rbb, p.C_rbb, p.A_rbb_stag, p.B_rbb_stag, p.C_rbb_stag};
+    for (int j = 0; j < 12; j++) std::memcpy(tab + (size_t)j * lpml, t[j], sizeof(float) * lpml);
+}
+
+void ref_stagger_a2d(const float *vp, const float *rho, int nx, int nz, int lpml, int fs,
+                     float *L, float *Rx, float *Rz) {
+    auto m = std::make_shared<ModelAcoustic2D<float>>(nx, nz, lpml, 10.f, 10.f, 0.f, 0.f, fs != 0);
+    m->createModel();
+    std::memcpy(m->getVp(), vp, sizeof(float) * nx * nz);
+    std::memcpy(m->getR(), rho, sizeof(float) * nx * nz);
+    m->staggerModels();
+    size_t n = (size_t)(nx + 2 * lpml) * (nz + 2 * lpml);
+    std::memcpy(L, m->getL(), sizeof(float) * n);
+    std::memcpy(Rx, m->getRx(), sizeof(float) * n);
+    std::memcpy(Rz, m->getRz(), sizeof(float) * n);
+}
+
+void ref_stagger_e2d(const float *vp, const float *vs, const float *rho, int nx, int nz, int lpml,
+                     int fs, float *L, float *L2M, float *M, float *Rx, float *Rz) {
+    auto m = std::make_shared<ModelElastic2D<float>>(nx, nz, lpml, 10.f, 10.f, 0.f, 0.f, fs != 0);
+    m->createModel();
+    std::memcpy(m->getVp(), vp, sizeof(float) * nx * nz);
+    std::memcpy(m->getVs(), vs, sizeof(float) * nx * nz);
+    std::memcpy(m->getR(), rho, sizeof(float) * nx * nz);
+    m->staggerModels();
+    size_t n = (size_t)(nx + 2 * lpml) * (nz + 2 * lpml);
+    std::memcpy(L, m->getL(), sizeof(float) * n);
+    std::memcpy(L2M, m->getL2M(), sizeof(float) * n);
+    std::memcpy(M, m->getM(), sizeof(float) * n);
+    std::memcpy(Rx, m->getRx(), sizeof(float) * n);
+    std::memcpy(Rz, m->getRz(), sizeof(float) * n);
+}
+
+void ref_stagger_a3d(const float *vp, const float *rho, int nx, int ny, int nz, int lpml, int fs,
+                     float *L, float *Rx, float *Ry, float *Rz) {
+    auto m = std::make_shared<ModelAcoustic3D<float>>(nx, ny, nz, lpml, 10.f, 10.f, 10.f, 0.f, 0.f,
+                                                      0.f, fs != 0);
+    m->createModel();
+    std::memcpy(m->getVp(), vp, sizeof(float) * nx * ny * nz);
+    std::memcpy(m->getR(), rho, sizeof(float) * nx * ny * nz);
+    m->staggerModels();
+    size_t n = (size_t)(nx + 2 * lpml) * (ny + 2 * lpml) * (nz + 2 * lpml);
+    std::memcpy(L, m->getL(), sizeof(float) * n);
+    std::memcpy(Rx, m->getRx(), sizeof(float) * n);
+    std::memcpy(Ry, m->getRy(), sizeof(float) * n);
+    std::memcpy(Rz, m->getRz(), sizeof(float) * n);
+}
+
+// Data2D::makeMap(model geometry, SMAP) for a field (0 P, 1 VX, 3 VZ).
+void ref_makemap2d(const float *coords, int ntrace, int nx, int nz, float dx, float dz, float ox,
+                   float oz, int field, int *map, float *shift) {
+    auto m = std::make_shared<ModelAcoustic2D<float>>(nx, nz, 0, dx, dz, ox, oz, false);
+    auto d = std::make_shared<Data2D<float>>(ntrace, 1, 1.f, 0.f);
+    d->setField(field_of(field));
+    fill2<float>(d, coords, true);
+    d->makeMap(m->getGeom(), SMAP);
+    copy_map2(d, true, map, shift);
+}
+
+// Full acoustic 2D modelling loop (modelling.cpp:242-286 order):
+// velocity, stress, insertSource(it), recordData(it).  Snapshots of P
+// (padded grid) are copied at the steps listed in snap_its.
+void ref_a2d_run(const float *vp, const float *rho, int nx, int nz, int lpml, float dx,
+                 float dz, float dt, int nt, int order, int fs, int srcfield, int nsrc,
+                 const float *scoords, const float *wav, int recfield, int nrec,
+                 const float *gcoords, float *rec, int nsnap, const int *snap_its, float *snaps,
+                 float *Pout, float *Vxout, float *Vzout) {
+    auto model = std::make_shared<ModelAcoustic2D<float>>(nx, nz, lpml, dx, dz, 0.f, 0.f, fs != 0);
+    model->createModel();
+    std::memcpy(model->getVp(), vp, sizeof(float) * nx * nz);
+    std::memcpy(model->getR(), rho, sizeof(float) * nx * nz);
+    auto src = std::make_shared<Data2D<float>>(nsrc, nt, dt, 0.f);
+    src->setField(field_of(srcfield));
+    fill2<float>(src, scoords, true);
+    std::memcpy(src->getData(), wav, sizeof(float) * nsrc * nt);
+    src->makeMap(model->getGeom(), SMAP);
+    auto data = std::make_shared<Data2D<float>>(nrec, nt, dt, 0.f);
+    data->setField(field_of(recfield));
+    fill2<float>(data, gcoords, false);
+    data->makeMap(model->getGeom(), GMAP);
+    model->staggerModels();
+    auto waves = std::make_shared<WavesAcoustic2D<float>>(model, nt, dt, 0.f);
+    auto der = std::make_shared<Der<float>>(waves->getNx_pml(), 1, waves->getNz_pml(),
+                                            waves->getDx(), 1.0, waves->getDz(), order);
+    waves->getPml()->setSmax(SMAX);
+    waves->getPml()->computeABC();
+    size_t np = (size_t)waves->getNx_pml() * waves->getNz_pml();
+    int isnap = 0;
+    for (int it = 0; it < nt; it++) {
+        waves->forwardstepVelocity(model, der);
+        waves->forwardstepStress(model, der);
+        waves->insertSource(model, src, SMAP, it);
+        waves->recordData(data, GMAP, it);
+        if (isnap < nsnap && snap_its[isnap] == it) {
+            std::memcpy(snaps + np * isnap, waves->getP(), sizeof(float) * np);
+            isnap++;
+        }
+    }
+    std::memcpy(rec, data->getData(), sizeof(float) * nrec * nt);
+    std::memcpy(Pout, waves->getP(), sizeof(float) * np);
+    std::memcpy(Vxout, waves->getVx(), sizeof(float) * np);
+    std::memcpy(Vzout, waves->getVz(), sizeof(float) * np);
+}
+
+// Elastic 2D: nt forward steps (velocity, stress, inject, record), then nbw
+// adjoint-state steps (backwardstepVelocity, backwardstepStress) with no
+// source.  recfield 0 P (recip flag), 1 VX, 3 VZ.  fields: Vx,Vz,Sxx,Szz,Sxz.
+void ref_e2d_run(const float *vp, const float *vs, const float *rho, int nx, int nz, int lpml,
+                 float dx, float dz, float dt, int nt, int order, int fs, int srcfield, int nsrc,
+                 const float *scoords, const float *wav, int recfield, int recip, int nrec,
+                 const float *gcoords, float *rec, int nbw, float *fields) {
+    auto model = std::make_shared<ModelElastic2D<float>>(nx, nz, lpml, dx, dz, 0.f, 0.f, fs != 0);
+    model->createModel();
+    std::memcpy(model->getVp(), vp, sizeof(float) * nx * nz);
+    std::memcpy(model->getVs(), vs, sizeof(float) * nx * nz);
+    std::memcpy(model->getR(), rho, sizeof(float) * nx * nz);
+    auto src = std::make_shared<Data2D<float>>(nsrc, nt, dt, 0.f);
+    src->setField(field_of(srcfield));
+    fill2<float>(src, scoords, true);
+    std::memcpy(src->getData(), wav, sizeof(float) * nsrc * nt);
+    src->makeMap(model->getGeom(), SMAP);
+    auto data = std::make_shared<Data2D<float>>(nrec, nt, dt, 0.f);
+    data->setField(field_of(recfield));
+    data->setReciprocity(recip != 0);
+    fill2<float>(data, gcoords, false);
+    data->makeMap(model->getGeom(), GMAP);
+    model->staggerModels();
+    auto waves = std::make_shared<WavesElastic2D<float>>(model, nt, dt, 0.f);
+    auto der = std::make_shared<Der<float>>(waves->getNx_pml(), 1, waves->getNz_pml(),
+                                            waves->getDx(), 1.0, waves->getDz(), order);
+    waves->getPml()->setSmax(SMAX);
+    waves->getPml()->computeABC();
+    for (int it = 0; it < nt; it++) {
+        waves->forwardstepVelocity(model, der);
+        waves->forwardstepStress(model, der);
+        waves->insertSource(model, src, SMAP, it);
+        waves->recordData(model, data, GMAP, it);
+    }
+    for (int ib = 0; ib < nbw; ib++) {
+        waves->backwardstepVelocity(model, der);
+        waves->backwardstepStress(model, der);
+    }
+    size_t np = (size_t)waves->getNx_pml() * waves->getNz_pml();
+    std::memcpy(rec, data->getData(), sizeof(float) * nrec * nt);
+    float *f[5] = {waves->getVx(), waves->getVz(), waves->getSxx(), waves->getSzz(), waves->getSxz()};
+    for (int j = 0; j < 5; j++) std::memcpy(fields + np * j, f[j], sizeof(float) * np);
+}
+
+// Acoustic 3D: coords are (x,y,z) triples.  fields: P,Vx,Vy,Vz.
+void ref_a3d_run(const float *vp, const float *rho, int nx, int ny, int nz, int lpml, float dx,
+                 float dy, float dz, float dt, int nt, int order, int fs, int srcfield, int nsrc,
+                 const float *scoords, const float *wav, int recfield, int nrec,
+                 const float *gcoords, float *rec, float *fields) {
+    auto model = std::make_shared<ModelAcoustic3D<float>>(nx, ny, nz, lpml, dx, dy, dz, 0.f, 0.f,
+                                                          0.f, fs != 0);
+    model->createModel();
+    std::memcpy(model->getVp(), vp, sizeof(float) * nx * ny * nz);
+    std::memcpy(model->getR(), rho, sizeof(float) * nx * ny * nz);
+    auto src = std::make_shared<Data3D<float>>(nsrc, nt, dt, 0.f);
+    src->setField(field_of(srcfield));
+    Point3D<float> *sc = src->getGeom()->getScoords();
+    for (int i = 0; i < nsrc; i++) {
+        sc[i].x = scoords[3 * i];
+        sc[i].y = scoords[3 * i + 1];
+        sc[i].z = scoords[3 * i + 2];
+    }
+    std::memcpy(src->getData(), wav, sizeof(float) * nsrc * nt);
+    src->makeMap(model->getGeom(), SMAP);
+    auto data = std::make_shared<Data3D<float>>(nrec, nt, dt, 0.f);
+    data->setField(field_of(recfield));
+    Point3D<float> *gc = data->getGeom()->getGcoords();
+    for (int i = 0; i < nrec; i++) {
+        gc[i].x = gcoords[3 * i];
+        gc[i].y = gcoords[3 * i + 1];
+        gc[i].z = gcoords[3 * i + 2];
+    }
+    data->makeMap(model->getGeom(), GMAP);
+    model->staggerModels();
+    auto waves = std::make_shared<WavesAcoustic3D<float>>(model, nt, dt, 0.f);
+    auto der = std::make_shared<Der<float>>(waves->getNx_pml(), waves->getNy_pml(),
+                                            waves->getNz_pml(), waves->getDx(), waves->getDy(),
+                                            waves->getDz(), order);
+    waves->getPml()->setSmax(SMAX);
+    waves->getPml()->computeABC();
+    for (int it = 0; it < nt; it++) {
+        waves->forwardstepVelocity(model, der);
+        waves->forwardstepStress(model, der);
+        waves->insertSource(model, src, SMAP, it);
+        waves->recordData(data, GMAP, it);
+    }
+    size_t np = (size_t)waves->getNx_pml() * waves->getNy_pml() * waves->getNz_pml();
+    std::memcpy(rec, data->getData(), sizeof(float) * nrec * nt);
+    float *f[4] = {waves->getP(), waves->getVx(), waves->getVy(), waves->getVz()};
+    for (int j = 0; j < 4; j++) std::memcpy(fields + np * j, f[j], sizeof(float) * np);
+}
+
+// Revolve action sequence: actions[k] (advance=0 takeshot=1 restore=2
+// firsturn=3 youturn=4 terminate=5 error=6) and capo after the call.
+// Returns the number of actions written, or -1 if maxn was too small.
+int ref_revolve(int nt, int snaps, int *actions, int *capos, int *checks, int maxn) {
+    Revolve<float> r(nt, snaps, true);
+    int n = 0;
+    revolve_action a;
+    do {
+        a = r.revolve();
+        if (n >= maxn) return -1;
+        actions[n] = (int)a;
+        capos[n] = r.getCapo();
+        checks[n] = r.getCheck();
+        n++;
+    } while (a != terminate && a != error);
+    return n;
+}
+
+int ref_revolve_numforw(int nt, int snaps) {
+    Revolve<float> r(nt, snaps, true);
+    return r.numforw(nt, snaps);
+}
+
+int ref_revolve_adjust(int nt) {
+    Revolve<float> r(nt, 2, true);
+    return r.adjust(nt);
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+"""Write the golden fixtures under tests/golden/ from the compiled reference.
+
+Run in a container where /root/reference is mounted and built
+(``make -C oracle ref``):  python tests/golden/make_golden.py
+
+Every fixture is a compressed .npz holding the inputs and the reference's
+outputs (float32 unless noted).  The reference itself never travels: only
+these vectors do.  Sizes are kept small (a few MB in total).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle  # noqa: E402
+from oracle import runs  # noqa: E402
+from rockseis_amd import synth  # noqa: E402
+
+
+def save(name, **arrs):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **arrs)
+    print(f"{name}.npz  {os.path.getsize(path) / 1024:.1f} KiB")
+
+
+def der_fixture(R):
+    rng = np.random.default_rng(11)
+    nx, ny, nz = 17, 18, 19
+    f = rng.standard_normal((nz, ny, nx)).astype(np.float32)
+    out = {"f": f, "d": np.float32([10.0, 7.5, 5.0])}
+    cases = [(o, w) for o in (1, 2, 4, 8) for w in range(6)]
+    cases += [(o, w) for o in (3, 5, 6, 7) for w in (0, 5)]
+    for order, which in cases:
+        df = np.zeros_like(f)
+        R.ref_der(f.ravel(), df.reshape(-1), nx, ny, nz, 10.0, 7.5, 5.0, order, which)
+        out[f"o{order}_w{which}"] = df
+    save("der", **out)
+
+
+def pml_fixture(R):
+    out = {}
+    for lpml in (10, 20, 30, 40):
+        for dt in (1e-3, 5e-4, 4.5e-4):
+            for sign in (1, -1):
+                t = np.zeros(12 * lpml, np.float32)
+                R.ref_pml_abc(lpml, dt, 1200.0, sign, t)
+                out[f"L{lpml}_dt{dt:g}_s{sign}"] = t
+    save("pml", **out)
+
+
+def stagger_fixture(R):
+    rng = np.random.default_rng(12)
+    nx, ny, nz, lpml = 23, 11, 17, 6
+    vp = synth.gradient_anomaly_vp(nx, nz, 10.0, 10.0, nanom=4, seed=3)
+    vs = (vp / 1.73).astype(np.float32)
+    rho = synth.gardner(vp)
+    vp3 = (2000 + 800 * rng.random((nz, ny, nx))).astype(np.float32)
+    rho3 = synth.gardner(vp3)
+    out = {"vp": vp, "vs": vs, "rho": rho, "vp3": vp3, "rho3": rho3, "lpml": np.int32(lpml)}
+    shp = (nz + 2 * lpml, nx + 2 * lpml)
+    shp3 = (nz + 2 * lpml, ny + 2 * lpml, nx + 2 * lpml)
+    for fs in (0, 1):
+        a = [np.zeros(shp, np.float32) for _ in range(3)]
+        R.ref_stagger_a2d(vp.ravel(), rho.ravel(), nx, nz, lpml, fs, *(x.reshape(-1) for x in a))
+        out.update({f"a2d_fs{fs}_{k}": v for k, v in zip(("L", "Rx", "Rz"), a)})
+        a = [np.zeros(shp, np.float32) for _ in range(5)]
+        R.ref_stagger_e2d(vp.ravel(), vs.ravel(), rho.ravel(), nx, nz, lpml, fs, *(x.reshape(-1) for x in a))
+        out.update({f"e2d_fs{fs}_{k}": v for k, v in zip(("L", "L2M", "M", "Rx", "Rz"), a)})
+        a = [np.zeros(shp3, np.float32) for _ in range(4)]
+        R.ref_stagger_a3d(vp3.ravel(), rho3.ravel(), nx, ny, nz, lpml, fs, *(x.reshape(-1) for x in a))
+        out.update({f"a3d_fs{fs}_{k}": v for k, v in zip(("L", "Rx", "Ry", "Rz"), a)})
+    save("stagger", **out)
+
+
+def makemap_fixture(R):
+    rng = np.random.default_rng(13)
+    n = 300
+    coords = np.stack([rng.uniform(-30, 430, n), rng.uniform(-30, 330, n)], 1).astype(np.float32)
+    # exact grid points, edges and values a hair either side of a node
+    coords[:12] = np.float32([[0, 0], [10, 10], [390, 290], [395, 295], [400, 300], [5, 5],
+                              [100, 0], [10.000001, 5], [9.99999, 5], [15, 15], [-0.0001, 0],
+                              [389.9999, 289.9999]])
+    out = {"coords": coords}
+    for field in (0, 1, 3):
+        m = np.zeros((n, 2), np.int32)
+        s = np.zeros((n, 2), np.float32)
+        R.ref_makemap2d(coords.ravel(), n, 40, 30, 10.0, 10.0, 0.0, 0.0, field, m.reshape(-1), s.reshape(-1))
+        out[f"map_f{field}"] = m
+        out[f"shift_f{field}"] = s
+    save("makemap", **out)
+
+
+def _model2d(nx, nz, dx, seed):
+    vp = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1800.0, 3200.0, nanom=5, seed=seed,
+                                   sigma=(30.0, 120.0))
+    return vp, (vp / 1.73).astype(np.float32), synth.gardner(vp)
+
+
+A2D_CASES = [  # order, fs, srcfield, recfield
+    (4, 0, "P", "P"), (8, 1, "VX", "VZ"), (2, 0, "VZ", "VX"), (8, 0, "P", "P"), (6, 1, "P", "VX"),
+]
+E2D_CASES = [  # order, fs, srcfield, recfield, recip, nbw
+    (8, 0, "P", "VX", 0, 0), (8, 1, "P", "P", 1, 25), (4, 0, "VX", "VZ", 0, 25), (4, 1, "VZ", "P", 0, 0),
+]
+
+
+def run_fixtures(R):
+    nx, nz, lpml, dx, dt, nt = 60, 50, 10, 10.0, 1e-3, 300
+    vp, vs, rho = _model2d(nx, nz, dx, 5)
+    sc = np.float32([[300.0, 0.0], [297.5, 33.3]])
+    wav = np.stack([synth.ricker(nt, dt, 15.0), 0.5 * synth.ricker(nt, dt, 20.0)])
+    gc = np.stack([np.arange(0, 600, 10.0) + 2.5, np.full(60, 20.0)], 1).astype(np.float32)
+    gc[::7, 1] = 0.0
+    common = dict(vp=vp, vs=vs, rho=rho, scoords=sc, wav=wav, gcoords=gc,
+                  grid=np.float32([nx, nz, lpml, dx, dt, nt]))
+    for k, (order, fs, sf, rf) in enumerate(A2D_CASES):
+        r = runs.ref_a2d_run(vp, rho, lpml, dx, dx, dt, nt, order, fs, sf, sc, wav, rf, gc,
+                             snap_its=(10, 150))
+        save(f"a2d_case{k}", rec=r["rec"], P=r["P"], Vx=r["Vx"], Vz=r["Vz"],
+             snap10=r["snaps"][0], snap150=r["snaps"][1], **common)
+    for k, (order, fs, sf, rf, recip, nbw) in enumerate(E2D_CASES):
+        r = runs.ref_e2d_run(vp, vs, rho, lpml, dx, dx, dt, nt, order, fs, sf, sc, wav, rf, gc,
+                             recip, nbw)
+        save(f"e2d_case{k}", rec=r["rec"], **{f: r[f] for f in runs.E2DState.FIELDS}, **common)
+    # acoustic 3D
+    rng = np.random.default_rng(14)
+    nx3, ny3, nz3, l3, nt3 = 22, 24, 20, 6, 80
+    vp3 = (2000 + 500 * rng.random((nz3, ny3, nx3))).astype(np.float32)
+    rho3 = synth.gardner(vp3)
+    sc3 = np.float32([[100.0, 110.0, 0.0]])
+    w3 = synth.ricker(nt3, dt, 15.0)[None]
+    g = np.stack(np.meshgrid(np.arange(0, 220, 20.0), np.arange(0, 240, 30.0), indexing="ij"), -1).reshape(-1, 2)
+    gc3 = np.concatenate([g, np.full((g.shape[0], 1), 15.0)], 1).astype(np.float32)
+    for k, (order, sf, rf) in enumerate([(4, "P", "P"), (8, "VY", "VZ"), (2, "VX", "P")]):
+        r = runs.ref_a3d_run(vp3, rho3, l3, (10.0, 10.0, 10.0), dt, nt3, order, 0, sf, sc3, w3, rf, gc3)
+        save(f"a3d_case{k}", rec=r["rec"], P=r["P"], Vz=r["Vz"],
+             vp=vp3, rho=rho3, scoords=sc3, wav=w3, gcoords=gc3,
+             grid=np.float32([nx3, ny3, nz3, l3, 10.0, dt, nt3, order]))
+
+
+REVOLVE_CASES = [(10, 3), (100, 5), (7, 2), (50, 60), (1, 2), (257, 4), (3000, 50), (2500, 50), (4000, 50)]
+
+
+def revolve_fixture(R):
+    out = {}
+    for nt, sn in REVOLVE_CASES:
+        a = np.zeros(100000, np.int32)
+        c = a.copy()
+        k = a.copy()
+        n = R.ref_revolve(nt, sn, a, c, k, a.size)
+        out[f"n{nt}_s{sn}"] = np.stack([a[:n], c[:n], k[:n]], 1).astype(np.int32)
+        out[f"n{nt}_s{sn}_numforw"] = np.int32(R.ref_revolve_numforw(nt, sn))
+    for nt in (10, 100, 1000, 3000):
+        out[f"adjust_{nt}"] = np.int32(R.ref_revolve_adjust(nt))
+    save("revolve", **out)
+
+
+if __name__ == "__main__":
+    R = oracle.ref()
+    der_fixture(R)
+    pml_fixture(R)
+    stagger_fixture(R)
+    makemap_fixture(R)
+    run_fixtures(R)
+    revolve_fixture(R)

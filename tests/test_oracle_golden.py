@@ -1,0 +1,104 @@
+"""The oracle (oracle/rsoracle.c, oracle/revolve.py) against the golden
+fixtures written by the compiled reference (tests/golden/make_golden.py).
+fp32 outputs must agree bit for bit; integer maps and schedules exactly."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import revolve as orv
+from oracle import runs
+
+from tests.golden.make_golden import A2D_CASES, E2D_CASES, REVOLVE_CASES
+
+
+def test_der_all_orders_and_axes(golden):
+    g = golden("der")
+    f = g["f"]
+    nz, ny, nx = f.shape
+    d = g["d"]
+    keys = [k for k in g.files if k.startswith("o")]
+    assert len(keys) == 32
+    for k in keys:
+        order, which = int(k[1]), int(k.split("_w")[1])
+        axis, fw = which % 3, int(which < 3)
+        df = np.zeros_like(f)
+        oracle.fn("deriv")(f.ravel(), df.reshape(-1), nx, ny, nz, axis, fw, order, d[axis])
+        assert np.array_equal(df, g[k]), k
+
+
+def test_pml_tables(golden):
+    g = golden("pml")
+    for k in g.files:
+        lpml = int(k[1:].split("_")[0])
+        dt = float(k.split("_dt")[1].split("_")[0])
+        sign = int(k.split("_s")[1])
+        t = runs.pml_tables(lpml, dt, sign=sign)
+        assert np.array_equal(t, g[k]), k
+
+
+def test_stagger(golden):
+    g = golden("stagger")
+    lpml = int(g["lpml"])
+    for fs in (0, 1):
+        for got, key in zip(runs.stagger_a2d(g["vp"], g["rho"], lpml, fs), ("L", "Rx", "Rz")):
+            assert np.array_equal(got, g[f"a2d_fs{fs}_{key}"]), key
+        for got, key in zip(runs.stagger_e2d(g["vp"], g["vs"], g["rho"], lpml, fs),
+                            ("L", "L2M", "M", "Rx", "Rz")):
+            assert np.array_equal(got, g[f"e2d_fs{fs}_{key}"]), key
+        for got, key in zip(runs.stagger_a3d(g["vp3"], g["rho3"], lpml, fs), ("L", "Rx", "Ry", "Rz")):
+            assert np.array_equal(got, g[f"a3d_fs{fs}_{key}"]), key
+
+
+def test_makemap(golden):
+    g = golden("makemap")
+    for field, name in ((0, "P"), (1, "VX"), (3, "VZ")):
+        m, s = runs.makemap2d(g["coords"], 40, 30, 10.0, 10.0, name)
+        assert np.array_equal(m, g[f"map_f{field}"])
+        ok = m >= 0
+        assert np.array_equal(s[ok], g[f"shift_f{field}"][ok])
+
+
+@pytest.mark.parametrize("case", range(len(A2D_CASES)))
+def test_a2d_run(golden, case):
+    g = golden(f"a2d_case{case}")
+    order, fs, sf, rf = A2D_CASES[case]
+    nx, nz, lpml, dx, dt, nt = g["grid"]
+    r = runs.a2d_run(g["vp"], g["rho"], int(lpml), dx, dx, dt, int(nt), order, fs, sf, g["scoords"],
+                     g["wav"], rf, g["gcoords"], snap_its=(10, 150))
+    for k in ("rec", "P", "Vx", "Vz"):
+        assert np.array_equal(r[k], g[k]), k
+    assert np.array_equal(r["snaps"][0], g["snap10"])
+    assert np.array_equal(r["snaps"][1], g["snap150"])
+
+
+@pytest.mark.parametrize("case", range(len(E2D_CASES)))
+def test_e2d_run(golden, case):
+    g = golden(f"e2d_case{case}")
+    order, fs, sf, rf, recip, nbw = E2D_CASES[case]
+    nx, nz, lpml, dx, dt, nt = g["grid"]
+    r = runs.e2d_run(g["vp"], g["vs"], g["rho"], int(lpml), dx, dx, dt, int(nt), order, fs, sf,
+                     g["scoords"], g["wav"], rf, g["gcoords"], recip, nbw)
+    for k in ("rec",) + runs.E2DState.FIELDS:
+        assert np.array_equal(r[k], g[k]), k
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_a3d_run(golden, case):
+    g = golden(f"a3d_case{case}")
+    nx, ny, nz, lpml, dx, dt, nt, order = g["grid"]
+    sf, rf = [("P", "P"), ("VY", "VZ"), ("VX", "P")][case]
+    r = runs.a3d_run(g["vp"], g["rho"], int(lpml), (dx, dx, dx), dt, int(nt), int(order), 0, sf,
+                     g["scoords"], g["wav"], rf, g["gcoords"])
+    for k in ("rec", "P", "Vz"):
+        assert np.array_equal(r[k], g[k]), k
+
+
+def test_revolve(golden):
+    g = golden("revolve")
+    for nt, sn in REVOLVE_CASES:
+        ref = g[f"n{nt}_s{sn}"]
+        got = np.int32(orv.schedule(nt, sn))
+        assert np.array_equal(got, ref), (nt, sn)
+        assert orv.numforw(nt, sn) == int(g[f"n{nt}_s{sn}_numforw"])
+    for nt in (10, 100, 1000, 3000):
+        assert orv.adjust(nt) == int(g[f"adjust_{nt}"])

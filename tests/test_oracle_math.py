@@ -1,0 +1,138 @@
+"""Checks of the oracle against what the mathematics fixes, independent of
+the golden fixtures: stencil moments, summation by parts between the forward
+and backward staggered derivatives, the CFS-PML formula, arithmetic-mean
+staggering, the Revolve forward-step count, and fp32/fp64 agreement of whole
+runs.  Each check is chosen so that a plausible slip (a shifted stencil
+index, a swapped fw/bw sign, a wrong PML exponent) fails it."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import revolve as orv
+from oracle import runs
+from rockseis_amd import synth
+
+
+def _coef(order):
+    c = np.zeros(36, np.float64)
+    oracle.fn("der_coeffs", np.float64)(c)
+    k = order
+    return c[(k - 1) * k // 2:(k - 1) * k // 2 + k]
+
+
+@pytest.mark.parametrize("order", range(1, 9))
+def test_linear_moment(order):
+    """d/dx of f=a*x is a*sum_io c_io*(2io+1) away from the edges, for both
+    the forward (f[i+io+1]-f[i-io]) and backward (f[i+io]-f[i-io-1]) forms."""
+    n, a, dx = 40, 3.0, 2.0
+    f = (a * np.arange(n, dtype=np.float64) * dx)[None, None, :].copy()
+    c = _coef(order)
+    expect = a * np.sum(c * (2 * np.arange(order) + 1))
+    assert abs(np.sum(c * (2 * np.arange(order) + 1)) - 1) < 3e-2  # optimised, near-consistent stencil
+    for fw in (1, 0):
+        df = np.zeros_like(f)
+        oracle.fn("deriv", np.float64)(f.ravel(), df.reshape(-1), n, 1, 1, 0, fw, order, dx)
+        inner = df[0, 0, order:n - order - 1]
+        assert np.allclose(inner, expect, rtol=1e-13)
+    # the staggered point: a quadratic has its exact derivative at i+1/2 (fw) / i-1/2 (bw)
+    x = np.arange(n, dtype=np.float64)
+    f2 = (x * x)[None, None, :].copy()
+    df = np.zeros_like(f2)
+    oracle.fn("deriv", np.float64)(f2.ravel(), df.reshape(-1), n, 1, 1, 0, 1, order, 1.0)
+    i = np.arange(order, n - order - 1)
+    assert np.allclose(df[0, 0, i], 2 * (i + 0.5) * np.sum(c * (2 * np.arange(order) + 1)), rtol=1e-12)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("order", [1, 4, 8])
+def test_summation_by_parts(axis, order):
+    """<D_fw f, g> = -<f, D_bw g> for fields vanishing near the boundary."""
+    rng = np.random.default_rng(axis * 10 + order)
+    n = (21, 22, 23)
+    shape = (n[2], n[1], n[0])
+    f = rng.standard_normal(shape)
+    g = rng.standard_normal(shape)
+    m = np.zeros(shape)
+    m[order + 1:-order - 1, order + 1:-order - 1, order + 1:-order - 1] = 1
+    f *= m
+    g *= m
+    d1 = np.zeros_like(f)
+    d2 = np.zeros_like(g)
+    oracle.fn("deriv", np.float64)(f.ravel(), d1.reshape(-1), *n, axis, 1, order, 1.5)
+    oracle.fn("deriv", np.float64)(g.ravel(), d2.reshape(-1), *n, axis, 0, order, 1.5)
+    lhs = np.sum(d1 * g)
+    rhs = -np.sum(f * d2)
+    assert abs(lhs - rhs) < 1e-12 * max(1.0, abs(lhs))
+
+
+@pytest.mark.parametrize("lpml", [10, 30])
+def test_pml_formula(lpml):
+    """CFS-PML: d=((L-i)/L)^3 (staggered (L-i-1/2)/L, right side (i+1)/L,
+    (i+3/2)/L), k=1+d, s=1200 d, a=0: C=1-1/k, B=exp(-s dt/k), A=s(1-B)/(k s)."""
+    dt = 1e-3
+    t = runs.pml_tables(lpml, dt, real=np.float64).reshape(12, lpml)
+    i = np.arange(lpml, dtype=np.float64)
+    for j0, d in ((0, (lpml - i) / lpml), (3, (lpml - i - 0.5) / lpml),
+                  (6, (i + 1) / lpml), (9, (i + 1.5) / lpml)):
+        d3 = d ** 3
+        k = 1 + d3
+        s = 1200.0 * d3
+        B = np.exp(-(s / k) * dt)
+        assert np.allclose(t[j0 + 2], 1 - 1 / k, rtol=1e-14, atol=0)
+        assert np.allclose(t[j0 + 1], B, rtol=1e-14, atol=0)
+        # 1-B cancels for small s: a 1-ulp difference in B grows to ~1e-12 relative
+        assert np.allclose(t[j0 + 0], s * (1 - B) / (k * s), rtol=1e-10, atol=0)
+
+
+def test_stagger_closed_form():
+    rng = np.random.default_rng(3)
+    nx, nz, lpml = 12, 9, 3
+    vp = (2000 + 1000 * rng.random((nz, nx)))
+    rho = (1.5 + rng.random((nz, nx)))
+    L, Rx, Rz = runs.stagger_a2d(vp, rho, lpml, 0, real=np.float64)
+    rp = np.pad(rho, lpml, mode="edge")
+    vpp = np.pad(vp, lpml, mode="edge")
+    assert np.allclose(L, rp * vpp ** 2, rtol=1e-15)
+    assert np.allclose(Rx[:, :-1], 2.0 / (rp[:, 1:] + rp[:, :-1]), rtol=1e-15)
+    assert np.allclose(Rz[:-1, :], 2.0 / (rp[1:, :] + rp[:-1, :]), rtol=1e-15)
+    assert np.array_equal(Rx[:, -1], Rx[:, -2]) and np.array_equal(Rz[-1], Rz[-2])
+
+
+@pytest.mark.parametrize("nt,snaps", [(100, 5), (257, 4), (3000, 50)])
+def test_revolve_forward_count(nt, snaps):
+    """Steps advanced over the whole schedule equal numforw, every state in
+    the reverse sweep is visited once, and takeshots never exceed snaps."""
+    capo = 0
+    adv = 0
+    turns = []
+    live = 0
+    for a, c, check in orv.schedule(nt, snaps):
+        if a == orv.ADVANCE:
+            adv += c - capo
+        if a in (orv.FIRSTURN, orv.YOUTURN):
+            turns.append(c)
+        assert check + 1 <= snaps
+        capo = c
+    assert adv == orv.numforw(nt, snaps)
+    assert turns == list(range(nt - 1, -1, -1))
+    # closed form: reps*steps - C(reps+s, s)*reps/(s+1)
+    from math import comb
+    reps = 0
+    while comb(reps + snaps, snaps) < nt:
+        reps += 1
+    assert orv.numforw(nt, snaps) == reps * nt - comb(reps + snaps, snaps) * reps // (snaps + 1)
+
+
+def test_fp32_vs_fp64_run_and_linearity():
+    nx, nz, lpml, dx, dt, nt = 50, 40, 10, 10.0, 1e-3, 250
+    vp = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1800, 3000, nanom=4, seed=1)
+    rho = synth.gardner(vp)
+    sc = np.float32([[250.0, 0.0]])
+    w = synth.ricker(nt, dt, 15.0)[None]
+    gc = synth.surface_line(50, 10.0, 20.0)
+    r32 = runs.a2d_run(vp, rho, lpml, dx, dx, dt, nt, 4, 0, "P", sc, w, "P", gc)
+    r64 = runs.a2d_run(vp, rho, lpml, dx, dx, dt, nt, 4, 0, "P", sc, w, "P", gc, real=np.float64)
+    rel = np.linalg.norm(r32["rec"] - r64["rec"]) / np.linalg.norm(r64["rec"])
+    assert rel < 1e-5
+    r2 = runs.a2d_run(vp, rho, lpml, dx, dx, dt, nt, 4, 0, "P", sc, 2 * w, "P", gc, real=np.float64)
+    assert np.allclose(r2["rec"], 2 * r64["rec"], rtol=1e-12, atol=1e-12 * np.abs(r64["rec"]).max())
