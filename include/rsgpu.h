@@ -1,0 +1,163 @@
+/*
+ * rsgpu.h -- C-ABI of librsgpu.so, the MI355X (gfx950) implementation of
+ * RockSeis's staggered-grid finite-difference time step, source injection,
+ * receiver gathers and FWI gradient kernels.
+ *
+ * Plain C: opaque stream pointers (hipStream_t passed as void*), device
+ * pointers and sizes; no HIP or torch types.  Every function returns 0 on
+ * success and a negative code on failure; rsg_last_error() then holds a
+ * message (thread-local).  Host callers map a non-zero return to the
+ * reference's fatal path (rs_error: print and exit, utils.cpp:51-62).
+ *
+ * Layout is the reference's (waves.cpp:4-13): padded grid, x fastest,
+ * I2D(i,j)=j*nx+i, I3D(i,j,k)=k*nx*ny+j*nx+i; PML strips I2D_lr=j*lpml+i,
+ * I2D_tb=j*nx+i, I3D_lr=k*lpml*ny+j*lpml+i, I3D_fb=k*nx*lpml+j*nx+i,
+ * I3D_tb=k*nx*ny+j*nx+i.  Traces are trace-major, nt fastest
+ * (Idat(it,i)=i*nt+it).
+ */
+#ifndef RSGPU_H
+#define RSGPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSG_OK 0
+#define RSG_EINVAL (-1)
+#define RSG_EHIP (-2)
+#define RSG_ENOMEM (-3)
+
+/* Field kinds, numbered like rs_field (utils.h:22). */
+#define RSG_PRESSURE 0
+#define RSG_VX 1
+#define RSG_VY 2
+#define RSG_VZ 3
+
+/* Propagator kinds */
+#define RSG_KIND_A2D 1 /* WavesAcoustic2D */
+#define RSG_KIND_E2D 2 /* WavesElastic2D  */
+#define RSG_KIND_A3D 3 /* WavesAcoustic3D */
+
+/* Grid of one propagator: padded sizes as Der(nx_pml,ny_pml,nz_pml,...)
+ * receives them (modelling.cpp:213), ny=1 in 2D. */
+typedef struct {
+    int nx, ny, nz;
+    int lpml;   /* PML width (all six sides applied)                        */
+    int order;  /* stencil half-width 1..8 (Der clamps, der.cpp:94-95)      */
+    int fs;     /* free surface: Szz row z=lpml zeroed (waves.cpp:1666-1674) */
+    float dx, dy, dz, dt;
+} rsg_grid;
+
+/* PML memory strip slots of rsg_state.psi[] */
+enum {
+    /* WavesAcoustic2D / PmlAcoustic2D (pml.h:103-111) */
+    RSG_A2D_P_LEFT = 0, RSG_A2D_P_RIGHT, RSG_A2D_P_TOP, RSG_A2D_P_BOTTOM,
+    RSG_A2D_VXX_LEFT, RSG_A2D_VXX_RIGHT, RSG_A2D_VZZ_TOP, RSG_A2D_VZZ_BOTTOM,
+    RSG_A2D_NSTRIP,
+    /* WavesElastic2D / PmlElastic2D (pml.h:159-178) */
+    RSG_E2D_SXX_LEFT = 0, RSG_E2D_SXX_RIGHT, RSG_E2D_SXZX_LEFT, RSG_E2D_SXZX_RIGHT,
+    RSG_E2D_SZZ_TOP, RSG_E2D_SZZ_BOTTOM, RSG_E2D_SXZZ_TOP, RSG_E2D_SXZZ_BOTTOM,
+    RSG_E2D_VXX_LEFT, RSG_E2D_VXX_RIGHT, RSG_E2D_VZX_LEFT, RSG_E2D_VZX_RIGHT,
+    RSG_E2D_VZZ_TOP, RSG_E2D_VZZ_BOTTOM, RSG_E2D_VXZ_TOP, RSG_E2D_VXZ_BOTTOM,
+    RSG_E2D_NSTRIP,
+    /* WavesAcoustic3D / PmlAcoustic3D (pml.h:132-149) */
+    RSG_A3D_P_LEFT = 0, RSG_A3D_P_RIGHT, RSG_A3D_P_FRONT, RSG_A3D_P_BACK,
+    RSG_A3D_P_TOP, RSG_A3D_P_BOTTOM, RSG_A3D_VXX_LEFT, RSG_A3D_VXX_RIGHT,
+    RSG_A3D_VYY_FRONT, RSG_A3D_VYY_BACK, RSG_A3D_VZZ_TOP, RSG_A3D_VZZ_BOTTOM,
+    RSG_A3D_NSTRIP
+};
+#define RSG_MAX_STRIPS 16
+
+/* Device pointers of one propagator's state (a Waves<T> object plus the
+ * staggered model arrays it reads).  Members a kind does not use are NULL. */
+typedef struct {
+    float *P, *Vx, *Vy, *Vz, *Sxx, *Szz, *Sxz;   /* fields                        */
+    const float *L, *L2M, *M, *Rx, *Ry, *Rz;    /* staggered model (padded grid) */
+    const float *tab;  /* 12*lpml PML tables in pml.h:60-74 order: A,B,C ltf,
+                          A,B,C ltf_stag, A,B,C rbb, A,B,C rbb_stag           */
+    float *psi[RSG_MAX_STRIPS]; /* memory strips, slots above               */
+    float *wrk;        /* optional scratch nx*ny*nz (unused by the fused kernels) */
+} rsg_state;
+
+/* A set of traces on the grid: a source (data = wavelet) or receivers
+ * (data = record), as Data2D/Data3D plus the Geometry maps of makeMap. */
+typedef struct {
+    int field;           /* RSG_PRESSURE / RSG_VX / RSG_VY / RSG_VZ            */
+    int ntrace;
+    int nt;              /* samples per trace                                  */
+    const int *map;      /* device, ntrace*dim grid indices (unpadded), -1 off  */
+    const float *shift;  /* device, ntrace*dim fractional offsets              */
+    float *data;         /* device, ntrace*nt                                  */
+    const float *factor; /* elastic pressure records: per-trace divisor or NULL */
+} rsg_traces;
+
+/* ---- runtime ---------------------------------------------------------- */
+const char *rsg_last_error(void);
+int rsg_device_count(int *n);
+int rsg_set_device(int dev);
+int rsg_stream_sync(void *stream);
+int rsg_malloc(void **ptr, size_t bytes);
+int rsg_free(void *ptr);
+int rsg_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int rsg_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int rsg_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+/* Zero with a fill kernel (safe inside hipGraph capture). */
+int rsg_zero(void *ptr, size_t bytes, void *stream);
+
+/* ---- host-side set-up, bit-exact with the reference --------------------- */
+/* Pml<T>::computeABC(sign) with Amax=0, Kmax=2 and the given Smax
+ * (pml.cpp:79-126; SMAX=1200 forced by modelling.cpp:216-218). */
+int rsg_pml_tables(int lpml, float dt, float smax, int sign, float *tab_host);
+/* Der<T> coefficient table, 36 entries, order k at k(k-1)/2 (der.cpp:3-53). */
+int rsg_der_coeffs(float *out36);
+/* Model*::staggerModels (model.cpp:1350-1411, 2720-2801, 1965-2040):
+ * unpadded [nz][nx] (or [nz][ny][nx]) host inputs -> padded host outputs. */
+int rsg_stagger_a2d(const float *vp, const float *rho, int nx, int nz, int lpml, int fs,
+                    float *L, float *Rx, float *Rz);
+int rsg_stagger_e2d(const float *vp, const float *vs, const float *rho, int nx, int nz, int lpml,
+                    int fs, float *L, float *L2M, float *M, float *Rx, float *Rz);
+int rsg_stagger_a3d(const float *vp, const float *rho, int nx, int ny, int nz, int lpml, int fs,
+                    float *L, float *Rx, float *Ry, float *Rz);
+/* Geometry2D::makeMap for one coordinate set (geometry.cpp:137-235), with
+ * the field shift of Data2D::makeMap (-0.5 along x for VX, z for VZ). */
+int rsg_makemap2d(const float *coords, int ntrace, int nx, int nz, float dx, float dz, float ox,
+                  float oz, int field, int *map, float *shift);
+int rsg_makemap3d(const float *coords, int ntrace, int nx, int ny, int nz, float dx, float dy,
+                  float dz, float ox, float oy, float oz, int field, int *map, float *shift);
+
+/* ---- one time step (the Waves* member functions) ------------------------ */
+/* WavesAcoustic2D::forwardstepVelocity / forwardstepStress (waves.cpp:217-404) */
+int rsg_a2d_forward_velocity(const rsg_grid *g, const rsg_state *s, void *stream);
+int rsg_a2d_forward_stress(const rsg_grid *g, const rsg_state *s, void *stream);
+/* WavesElastic2D::forwardstep* / backwardstep* (waves.cpp:1416-2080) */
+int rsg_e2d_forward_velocity(const rsg_grid *g, const rsg_state *s, void *stream);
+int rsg_e2d_forward_stress(const rsg_grid *g, const rsg_state *s, void *stream);
+int rsg_e2d_backward_velocity(const rsg_grid *g, const rsg_state *s, void *stream);
+int rsg_e2d_backward_stress(const rsg_grid *g, const rsg_state *s, void *stream);
+/* WavesAcoustic3D::forwardstepVelocity / forwardstepStress (waves.cpp:710-996) */
+int rsg_a3d_forward_velocity(const rsg_grid *g, const rsg_state *s, void *stream);
+int rsg_a3d_forward_stress(const rsg_grid *g, const rsg_state *s, void *stream);
+
+/* insertSource (waves.cpp:407-497, 998-1137, 2085-2171): kind is
+ * 1 acoustic 2D, 2 elastic 2D, 3 acoustic 3D; the field and its modulus are
+ * chosen from src->field as the reference does. */
+int rsg_insert_source(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src,
+                      int it, void *stream);
+/* recordData (waves.cpp:500-595, 1140-1275, 2174-2288) into rec->data. */
+int rsg_record_data(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *rec,
+                    int it, void *stream);
+
+/* ---- whole modelling run (Modelling*::run time loop, modelling.cpp:242-286)
+ * Steps it0..it0+nsteps-1: velocity, stress, insertSource(it), recordData(it)
+ * for each receiver set.  graph_chunk > 0 captures that many steps in one
+ * hipGraph and replays it (time index read from device memory). */
+int rsg_modelling_run(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src,
+                      int nrec, const rsg_traces *recs, int it0, int nsteps, int graph_chunk,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RSGPU_H */

@@ -1,0 +1,112 @@
+"""ctypes binding of librsgpu.so (include/rsgpu.h).
+
+The library is loaded on first use from the package directory; a missing
+library raises immediately -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "librsgpu.so")
+
+PRESSURE, VX, VY, VZ = 0, 1, 2, 3
+KIND_A2D, KIND_E2D, KIND_A3D = 1, 2, 3
+MAX_STRIPS = 16
+
+A2D_STRIPS = ("P_left", "P_right", "P_top", "P_bottom", "Vxx_left", "Vxx_right", "Vzz_top",
+              "Vzz_bottom")
+E2D_STRIPS = ("Sxx_left", "Sxx_right", "Sxzx_left", "Sxzx_right", "Szz_top", "Szz_bottom",
+              "Sxzz_top", "Sxzz_bottom", "Vxx_left", "Vxx_right", "Vzx_left", "Vzx_right",
+              "Vzz_top", "Vzz_bottom", "Vxz_top", "Vxz_bottom")
+A3D_STRIPS = ("P_left", "P_right", "P_front", "P_back", "P_top", "P_bottom", "Vxx_left",
+              "Vxx_right", "Vyy_front", "Vyy_back", "Vzz_top", "Vzz_bottom")
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("lpml", C.c_int),
+                ("order", C.c_int), ("fs", C.c_int), ("dx", C.c_float), ("dy", C.c_float),
+                ("dz", C.c_float), ("dt", C.c_float)]
+
+
+class State(C.Structure):
+    _fields_ = ([(n, C.c_void_p) for n in ("P", "Vx", "Vy", "Vz", "Sxx", "Szz", "Sxz",
+                                           "L", "L2M", "M", "Rx", "Ry", "Rz", "tab")]
+                + [("psi", C.c_void_p * MAX_STRIPS), ("wrk", C.c_void_p)])
+
+
+class Traces(C.Structure):
+    _fields_ = [("field", C.c_int), ("ntrace", C.c_int), ("nt", C.c_int), ("map", C.c_void_p),
+                ("shift", C.c_void_p), ("data", C.c_void_p), ("factor", C.c_void_p)]
+
+
+class RsgError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RsgError(f"librsgpu.so not built ({LIBPATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIBPATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    vp, ip, fp = C.c_void_p, C.c_int, C.c_float
+    GP, SP, TP = C.POINTER(Grid), C.POINTER(State), C.POINTER(Traces)
+    sig = {
+        "rsg_last_error": ([], C.c_char_p),
+        "rsg_device_count": ([C.POINTER(C.c_int)], ip),
+        "rsg_set_device": ([ip], ip),
+        "rsg_stream_sync": ([vp], ip),
+        "rsg_malloc": ([C.POINTER(C.c_void_p), C.c_size_t], ip),
+        "rsg_free": ([vp], ip),
+        "rsg_memcpy_h2d": ([vp, vp, C.c_size_t, vp], ip),
+        "rsg_memcpy_d2h": ([vp, vp, C.c_size_t, vp], ip),
+        "rsg_memcpy_d2d": ([vp, vp, C.c_size_t, vp], ip),
+        "rsg_zero": ([vp, C.c_size_t, vp], ip),
+        "rsg_pml_tables": ([ip, fp, fp, ip, vp], ip),
+        "rsg_der_coeffs": ([vp], ip),
+        "rsg_stagger_a2d": ([vp, vp, ip, ip, ip, ip, vp, vp, vp], ip),
+        "rsg_stagger_e2d": ([vp, vp, vp, ip, ip, ip, ip, vp, vp, vp, vp, vp], ip),
+        "rsg_stagger_a3d": ([vp, vp, ip, ip, ip, ip, ip, vp, vp, vp, vp], ip),
+        "rsg_makemap2d": ([vp, ip, ip, ip, fp, fp, fp, fp, ip, vp, vp], ip),
+        "rsg_makemap3d": ([vp, ip, ip, ip, ip, fp, fp, fp, fp, fp, fp, ip, vp, vp], ip),
+        "rsg_insert_source": ([ip, GP, SP, TP, ip, vp], ip),
+        "rsg_record_data": ([ip, GP, SP, TP, ip, vp], ip),
+        "rsg_modelling_run": ([ip, GP, SP, TP, ip, TP, ip, ip, ip, vp], ip),
+    }
+    for name in ("a2d_forward_velocity", "a2d_forward_stress", "e2d_forward_velocity",
+                 "e2d_forward_stress", "e2d_backward_velocity", "e2d_backward_stress",
+                 "a3d_forward_velocity", "a3d_forward_stress"):
+        sig["rsg_" + name] = ([GP, SP, vp], ip)
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+
+
+# Every symbol include/rsgpu.h declares (checked by tests/test_abi.py).
+EXPORTS = (
+    "rsg_last_error", "rsg_device_count", "rsg_set_device", "rsg_stream_sync", "rsg_malloc",
+    "rsg_free", "rsg_memcpy_h2d", "rsg_memcpy_d2h", "rsg_memcpy_d2d", "rsg_zero",
+    "rsg_pml_tables", "rsg_der_coeffs", "rsg_stagger_a2d", "rsg_stagger_e2d", "rsg_stagger_a3d",
+    "rsg_makemap2d", "rsg_makemap3d", "rsg_a2d_forward_velocity", "rsg_a2d_forward_stress",
+    "rsg_e2d_forward_velocity", "rsg_e2d_forward_stress", "rsg_e2d_backward_velocity",
+    "rsg_e2d_backward_stress", "rsg_a3d_forward_velocity", "rsg_a3d_forward_stress",
+    "rsg_insert_source", "rsg_record_data", "rsg_modelling_run",
+)
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = lib().rsg_last_error()
+        raise RsgError(f"{what}: rsgpu error {rc}: {msg.decode() if msg else ''}")

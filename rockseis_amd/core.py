@@ -1,0 +1,601 @@
+"""Python mirror of RockSeis's modelling interface on the MI355X path.
+
+Class and method names follow the reference (include/der.h, pml.h, model.h,
+data.h, waves.h, modelling.h) so that a caller written against the
+reference reads the same: Model*::staggerModels(), Data2D::makeMap(),
+Waves*::forwardstepVelocity(model, der) / forwardstepStress / insertSource /
+recordData, Modelling*::run().  Arrays are numpy on the host side
+(models, wavelets, records, as the reference's host classes hold them) and
+torch tensors on cuda:0 for the propagator state; all arithmetic runs in
+librsgpu.so.  Domain decomposition (Domain::getStatus()) is not offered:
+one GPU holds a whole shot window.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib as R
+
+SMAP, GMAP = 0, 1          # utils.h:12-13
+SMAX = 1200.0              # pml.h:16
+FIELDS = {"P": R.PRESSURE, "PRESSURE": R.PRESSURE, "VX": R.VX, "VY": R.VY, "VZ": R.VZ}
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        raise R.RsgError("rockseis_amd needs a GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _np_ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _field(f):
+    if isinstance(f, str):
+        return FIELDS[f.upper()]
+    return int(f)
+
+
+# --------------------------------------------------------------------------
+# Der / Pml
+# --------------------------------------------------------------------------
+
+class Der:
+    """Der(nx, ny, nz, dx, dy, dz, order): the stencil description; the
+    derivative itself is evaluated inside the fused step kernels."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, order):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+        self.order = min(max(int(order), 1), 8)   # der.cpp:94-95
+
+    def getOrder(self):
+        return self.order
+
+    @staticmethod
+    def coefficients():
+        out = np.zeros(36, np.float32)
+        R.check(R.lib().rsg_der_coeffs(_np_ptr(out)), "rsg_der_coeffs")
+        return out
+
+
+class Pml:
+    """CFS-PML constants (Pml<T>, pml.h:31-81); tables computed on the host
+    exactly as computeABC() and kept on the device."""
+
+    def __init__(self, lpml, dt):
+        self.lpml, self.dt = int(lpml), float(dt)
+        self.Smax, self.Amax, self.Kmax = SMAX, 0.0, 2.0
+        self.tab_host = np.zeros(12 * self.lpml, np.float32)
+        self.tab = None
+        self.computeABC()
+
+    def setSmax(self, smax):
+        self.Smax = float(smax)
+
+    def computeABC(self, sign=1):
+        R.check(R.lib().rsg_pml_tables(self.lpml, self.dt, self.Smax, sign, _np_ptr(self.tab_host)),
+                "rsg_pml_tables")
+        if torch.cuda.is_available():
+            self.tab = torch.from_numpy(self.tab_host).to(_dev())
+
+    def table(self, name):
+        names = ("A_ltf", "B_ltf", "C_ltf", "A_ltf_stag", "B_ltf_stag", "C_ltf_stag",
+                 "A_rbb", "B_rbb", "C_rbb", "A_rbb_stag", "B_rbb_stag", "C_rbb_stag")
+        j = names.index(name)
+        return self.tab_host[j * self.lpml:(j + 1) * self.lpml]
+
+
+# --------------------------------------------------------------------------
+# Models
+# --------------------------------------------------------------------------
+
+class _Model:
+    dim = 2
+
+    def getLpml(self):
+        return self.lpml
+
+    def getFs(self):
+        return self.fs
+
+    def _upload(self, names):
+        dev = _dev()
+        for n in names:
+            setattr(self, "_d" + n, torch.from_numpy(np.ascontiguousarray(getattr(self, n))).to(dev))
+
+
+class ModelAcoustic2D(_Model):
+    """ModelAcoustic2D(nx, nz, lpml, dx, dz, ox, oz, fs); Vp/R are [nz, nx]."""
+
+    def __init__(self, nx, nz, lpml, dx, dz, ox=0.0, oz=0.0, fs=False):
+        self.nx, self.nz, self.lpml = int(nx), int(nz), int(lpml)
+        self.dx, self.dz, self.ox, self.oz, self.fs = float(dx), float(dz), float(ox), float(oz), bool(fs)
+        self.Vp = np.zeros((nz, nx), np.float32)
+        self.R = np.zeros((nz, nx), np.float32)
+
+    def getNx_pml(self):
+        return self.nx + 2 * self.lpml
+
+    def getNz_pml(self):
+        return self.nz + 2 * self.lpml
+
+    def staggerModels(self):
+        shp = (self.getNz_pml(), self.getNx_pml())
+        self.L, self.Rx, self.Rz = (np.zeros(shp, np.float32) for _ in range(3))
+        vp, rho = (np.ascontiguousarray(a, np.float32) for a in (self.Vp, self.R))
+        R.check(R.lib().rsg_stagger_a2d(_np_ptr(vp), _np_ptr(rho), self.nx, self.nz, self.lpml,
+                                        int(self.fs), _np_ptr(self.L), _np_ptr(self.Rx),
+                                        _np_ptr(self.Rz)), "staggerModels")
+        self._upload(("L", "Rx", "Rz"))
+
+
+class ModelElastic2D(_Model):
+    def __init__(self, nx, nz, lpml, dx, dz, ox=0.0, oz=0.0, fs=False):
+        self.nx, self.nz, self.lpml = int(nx), int(nz), int(lpml)
+        self.dx, self.dz, self.ox, self.oz, self.fs = float(dx), float(dz), float(ox), float(oz), bool(fs)
+        self.Vp = np.zeros((nz, nx), np.float32)
+        self.Vs = np.zeros((nz, nx), np.float32)
+        self.R = np.zeros((nz, nx), np.float32)
+
+    def getNx_pml(self):
+        return self.nx + 2 * self.lpml
+
+    def getNz_pml(self):
+        return self.nz + 2 * self.lpml
+
+    def staggerModels(self):
+        shp = (self.getNz_pml(), self.getNx_pml())
+        self.L, self.L2M, self.M, self.Rx, self.Rz = (np.zeros(shp, np.float32) for _ in range(5))
+        vp, vs, rho = (np.ascontiguousarray(a, np.float32) for a in (self.Vp, self.Vs, self.R))
+        R.check(R.lib().rsg_stagger_e2d(_np_ptr(vp), _np_ptr(vs), _np_ptr(rho), self.nx, self.nz,
+                                        self.lpml, int(self.fs), _np_ptr(self.L), _np_ptr(self.L2M),
+                                        _np_ptr(self.M), _np_ptr(self.Rx), _np_ptr(self.Rz)),
+                "staggerModels")
+        self._upload(("L", "L2M", "M", "Rx", "Rz"))
+
+
+class ModelAcoustic3D(_Model):
+    dim = 3
+
+    def __init__(self, nx, ny, nz, lpml, dx, dy, dz, ox=0.0, oy=0.0, oz=0.0, fs=False):
+        self.nx, self.ny, self.nz, self.lpml = int(nx), int(ny), int(nz), int(lpml)
+        self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+        self.ox, self.oy, self.oz, self.fs = float(ox), float(oy), float(oz), bool(fs)
+        self.Vp = np.zeros((nz, ny, nx), np.float32)
+        self.R = np.zeros((nz, ny, nx), np.float32)
+
+    def getNx_pml(self):
+        return self.nx + 2 * self.lpml
+
+    def getNy_pml(self):
+        return self.ny + 2 * self.lpml
+
+    def getNz_pml(self):
+        return self.nz + 2 * self.lpml
+
+    def staggerModels(self):
+        shp = (self.getNz_pml(), self.getNy_pml(), self.getNx_pml())
+        self.L, self.Rx, self.Ry, self.Rz = (np.zeros(shp, np.float32) for _ in range(4))
+        vp, rho = (np.ascontiguousarray(a, np.float32) for a in (self.Vp, self.R))
+        R.check(R.lib().rsg_stagger_a3d(_np_ptr(vp), _np_ptr(rho), self.nx, self.ny, self.nz,
+                                        self.lpml, int(self.fs), _np_ptr(self.L), _np_ptr(self.Rx),
+                                        _np_ptr(self.Ry), _np_ptr(self.Rz)), "staggerModels")
+        self._upload(("L", "Rx", "Ry", "Rz"))
+
+
+# --------------------------------------------------------------------------
+# Data (traces + maps)
+# --------------------------------------------------------------------------
+
+class Data2D:
+    """Data2D(ntrace, nt, dt, ot): trace-major data [ntrace, nt] with source
+    and receiver coordinates (x, z) and the maps makeMap builds."""
+
+    dim = 2
+
+    def __init__(self, ntrace, nt, dt, ot=0.0, field="P"):
+        self.ntrace, self.nt, self.dt, self.ot = int(ntrace), int(nt), float(dt), float(ot)
+        self.data = np.zeros((self.ntrace, self.nt), np.float32)
+        self.scoords = np.zeros((self.ntrace, self.dim), np.float32)
+        self.gcoords = np.zeros((self.ntrace, self.dim), np.float32)
+        self.field = _field(field)
+        self.reciprocity = False
+        self.smap = self.gmap = None
+
+    def setField(self, f):
+        self.field = _field(f)
+
+    def getField(self):
+        return self.field
+
+    def setReciprocity(self, v):
+        self.reciprocity = bool(v)
+
+    def getData(self):
+        return self.data
+
+    def makeMap(self, model, maptype=SMAP):
+        coords = np.ascontiguousarray(self.scoords if maptype == SMAP else self.gcoords, np.float32)
+        m = np.full((self.ntrace, self.dim), -1, np.int32)
+        s = np.zeros((self.ntrace, self.dim), np.float32)
+        if self.dim == 2:
+            R.check(R.lib().rsg_makemap2d(_np_ptr(coords), self.ntrace, model.nx, model.nz, model.dx,
+                                          model.dz, model.ox, model.oz, self.field, _np_ptr(m),
+                                          _np_ptr(s)), "makeMap")
+        else:
+            R.check(R.lib().rsg_makemap3d(_np_ptr(coords), self.ntrace, model.nx, model.ny, model.nz,
+                                          model.dx, model.dy, model.dz, model.ox, model.oy, model.oz,
+                                          self.field, _np_ptr(m), _np_ptr(s)), "makeMap")
+        if maptype == SMAP:
+            self.smap, self.sshift = m, s
+        else:
+            self.gmap, self.gshift = m, s
+
+    def _device(self, maptype, factor=None, zero=False):
+        """rsg_traces view on the device (map, shift, data, factor tensors kept alive)."""
+        dev = _dev()
+        m, s = (self.smap, self.sshift) if maptype == SMAP else (self.gmap, self.gshift)
+        if m is None:
+            raise R.RsgError("makeMap() must be called before using the traces")
+        t = R.Traces()
+        keep = [torch.from_numpy(m).to(dev), torch.from_numpy(s).to(dev)]
+        d = torch.zeros_like(torch.from_numpy(self.data), device=dev) if zero else \
+            torch.from_numpy(np.ascontiguousarray(self.data)).to(dev)
+        keep.append(d)
+        t.field, t.ntrace, t.nt = self.field, self.ntrace, self.nt
+        t.map, t.shift, t.data = keep[0].data_ptr(), keep[1].data_ptr(), d.data_ptr()
+        if factor is not None:
+            f = torch.from_numpy(np.ascontiguousarray(factor, np.float32)).to(dev)
+            keep.append(f)
+            t.factor = f.data_ptr()
+        return t, keep
+
+
+class Data3D(Data2D):
+    dim = 3
+
+
+# --------------------------------------------------------------------------
+# Waves
+# --------------------------------------------------------------------------
+
+class _Waves:
+    kind = 0
+    strips = ()
+
+    def _alloc(self, shape, fields, lpml, dt, strip_shapes):
+        dev = _dev()
+        self.shape = shape
+        for f in fields:
+            setattr(self, f, torch.zeros(shape, dtype=torch.float32, device=dev))
+        self.Pml = Pml(lpml, dt)
+        self.psi = {n: torch.zeros(strip_shapes[i], dtype=torch.float32, device=dev)
+                    for i, n in enumerate(self.strips)}
+
+    def getPml(self):
+        return self.Pml
+
+    def getLpml(self):
+        return self.lpml
+
+    def getDt(self):
+        return self.dt
+
+    def _state(self, model):
+        s = R.State()
+        for f in ("P", "Vx", "Vy", "Vz", "Sxx", "Szz", "Sxz"):
+            t = getattr(self, f, None)
+            if t is not None:
+                setattr(s, f, t.data_ptr())
+        for f in ("L", "L2M", "M", "Rx", "Ry", "Rz"):
+            t = getattr(model, "_d" + f, None)
+            if t is not None:
+                setattr(s, f, t.data_ptr())
+        if self.Pml.tab is None:
+            self.Pml.computeABC()
+        s.tab = self.Pml.tab.data_ptr()
+        for i, n in enumerate(self.strips):
+            s.psi[i] = self.psi[n].data_ptr()
+        return s
+
+    def _grid(self, der):
+        g = R.Grid()
+        g.nx, g.ny, g.nz = self.nx_pml, self.ny_pml, self.nz_pml
+        g.lpml, g.order, g.fs = self.lpml, der.order, int(self.fs)
+        g.dx, g.dy, g.dz, g.dt = self.dx, self.dy, self.dz, self.dt
+        return g
+
+    def _step(self, fname, model, der):
+        g = self._grid(der)
+        s = self._state(model)
+        R.check(getattr(R.lib(), fname)(C.byref(g), C.byref(s), _stream()), fname)
+
+    def insertSource(self, model, source, maptype, it):
+        g = self._grid(Der(1, 1, 1, 1, 1, 1, 4))
+        t, keep = source._device(maptype)
+        R.check(R.lib().rsg_insert_source(self.kind, C.byref(g), C.byref(self._state(model)),
+                                          C.byref(t), int(it), _stream()), "insertSource")
+        torch.cuda.current_stream().synchronize()
+
+    def recordData(self, *args):
+        """recordData([model,] data, maptype, it): accumulates into data.data
+        (host) through a device round trip of that sample."""
+        if len(args) == 4:
+            model, data, maptype, it = args
+        else:
+            model, (data, maptype, it) = None, args
+        factor = _record_factor(model, data, maptype) if model is not None else None
+        g = self._grid(Der(1, 1, 1, 1, 1, 1, 4))
+        t, keep = data._device(maptype, factor)
+        R.check(R.lib().rsg_record_data(self.kind, C.byref(g), C.byref(self._state(model or self._model)),
+                                        C.byref(t), int(it), _stream()), "recordData")
+        data.data[:, it] = keep[2][:, it].cpu().numpy()
+
+
+def _record_factor(model, data, maptype):
+    """Elastic pressure divisor rho*(Vp^2 - Vs^2) at the receiver cell under
+    reciprocity, else 1 (waves.cpp:2219-2228)."""
+    if not isinstance(model, ModelElastic2D) or data.field != R.PRESSURE:
+        return None
+    m = data.smap if maptype == SMAP else data.gmap
+    f = np.ones(data.ntrace, np.float32)
+    if data.reciprocity:
+        ok = (m[:, 0] >= 0) & (m[:, 1] >= 0)
+        x, z = m[ok, 0], m[ok, 1]
+        r, vp, vs = model.R[z, x], model.Vp[z, x], model.Vs[z, x]
+        f[ok] = r * vp * vp - r * vs * vs
+    return f
+
+
+class WavesAcoustic2D(_Waves):
+    kind = R.KIND_A2D
+    strips = R.A2D_STRIPS
+
+    def __init__(self, model: ModelAcoustic2D, nt, dt, ot=0.0):
+        self.nt, self.dt, self.ot = int(nt), float(dt), float(ot)
+        self.lpml, self.fs = model.lpml, model.fs
+        self.nx_pml, self.ny_pml, self.nz_pml = model.getNx_pml(), 1, model.getNz_pml()
+        self.dx, self.dy, self.dz = model.dx, 1.0, model.dz
+        self._model = model
+        nxp, nzp, L = self.nx_pml, self.nz_pml, self.lpml
+        self._alloc((nzp, nxp), ("P", "Vx", "Vz"), L, dt,
+                    [(nzp, L)] * 2 + [(L, nxp)] * 2 + [(nzp, L)] * 2 + [(L, nxp)] * 2)
+
+    def forwardstepVelocity(self, model, der):
+        self._step("rsg_a2d_forward_velocity", model, der)
+
+    def forwardstepStress(self, model, der):
+        self._step("rsg_a2d_forward_stress", model, der)
+
+    def getP(self):
+        return self.P
+
+    def getVx(self):
+        return self.Vx
+
+    def getVz(self):
+        return self.Vz
+
+    def getNx_pml(self):
+        return self.nx_pml
+
+    def getNz_pml(self):
+        return self.nz_pml
+
+
+class WavesElastic2D(_Waves):
+    kind = R.KIND_E2D
+    strips = R.E2D_STRIPS
+
+    def __init__(self, model: ModelElastic2D, nt, dt, ot=0.0):
+        self.nt, self.dt, self.ot = int(nt), float(dt), float(ot)
+        self.lpml, self.fs = model.lpml, model.fs
+        self.nx_pml, self.ny_pml, self.nz_pml = model.getNx_pml(), 1, model.getNz_pml()
+        self.dx, self.dy, self.dz = model.dx, 1.0, model.dz
+        self._model = model
+        nxp, nzp, L = self.nx_pml, self.nz_pml, self.lpml
+        x, z = (nzp, L), (L, nxp)
+        self._alloc((nzp, nxp), ("Sxx", "Szz", "Sxz", "Vx", "Vz"), L, dt,
+                    [x] * 4 + [z] * 4 + [x] * 4 + [z] * 4)
+
+    def forwardstepVelocity(self, model, der):
+        self._step("rsg_e2d_forward_velocity", model, der)
+
+    def forwardstepStress(self, model, der):
+        self._step("rsg_e2d_forward_stress", model, der)
+
+    def backwardstepVelocity(self, model, der):
+        self._step("rsg_e2d_backward_velocity", model, der)
+
+    def backwardstepStress(self, model, der):
+        self._step("rsg_e2d_backward_stress", model, der)
+
+    def getSxx(self):
+        return self.Sxx
+
+    def getSzz(self):
+        return self.Szz
+
+    def getSxz(self):
+        return self.Sxz
+
+    def getVx(self):
+        return self.Vx
+
+    def getVz(self):
+        return self.Vz
+
+    def getNx_pml(self):
+        return self.nx_pml
+
+    def getNz_pml(self):
+        return self.nz_pml
+
+
+class WavesAcoustic3D(_Waves):
+    kind = R.KIND_A3D
+    strips = R.A3D_STRIPS
+
+    def __init__(self, model: ModelAcoustic3D, nt, dt, ot=0.0):
+        self.nt, self.dt, self.ot = int(nt), float(dt), float(ot)
+        self.lpml, self.fs = model.lpml, model.fs
+        self.nx_pml, self.ny_pml, self.nz_pml = model.getNx_pml(), model.getNy_pml(), model.getNz_pml()
+        self.dx, self.dy, self.dz = model.dx, model.dy, model.dz
+        self._model = model
+        nxp, nyp, nzp, L = self.nx_pml, self.ny_pml, self.nz_pml, self.lpml
+        lr, fb, tb = (nzp, nyp, L), (nzp, L, nxp), (L, nyp, nxp)
+        self._alloc((nzp, nyp, nxp), ("P", "Vx", "Vy", "Vz"), L, dt,
+                    [lr, lr, fb, fb, tb, tb, lr, lr, fb, fb, tb, tb])
+
+    def forwardstepVelocity(self, model, der):
+        self._step("rsg_a3d_forward_velocity", model, der)
+
+    def forwardstepStress(self, model, der):
+        self._step("rsg_a3d_forward_stress", model, der)
+
+    def getP(self):
+        return self.P
+
+    def getVx(self):
+        return self.Vx
+
+    def getVy(self):
+        return self.Vy
+
+    def getVz(self):
+        return self.Vz
+
+    def getNx_pml(self):
+        return self.nx_pml
+
+    def getNy_pml(self):
+        return self.ny_pml
+
+    def getNz_pml(self):
+        return self.nz_pml
+
+
+# --------------------------------------------------------------------------
+# Modelling drivers
+# --------------------------------------------------------------------------
+
+class _Modelling:
+    waves_cls = None
+
+    def __init__(self, model, source, order=4, snapinc=1):
+        self.model, self.source = model, source
+        self.order = int(order) if 1 < int(order) < 9 else 4      # modelling.cpp:17-23
+        self.snapinc = max(int(snapinc), 1)
+        self.recs = {}
+        self.graph_chunk = 0
+        self.waves = None
+
+    def getOrder(self):
+        return self.order
+
+    def setRecP(self, d):
+        self.recs["P"] = d
+
+    def setRecVx(self, d):
+        self.recs["VX"] = d
+
+    def setRecVy(self, d):
+        self.recs["VY"] = d
+
+    def setRecVz(self, d):
+        self.recs["VZ"] = d
+
+    def setGraphChunk(self, n):
+        """Capture n steps per hipGraph (0 = plain per-step launches)."""
+        self.graph_chunk = int(n)
+
+    def _vpmax(self):
+        return float(np.max(self.model.Vp))
+
+    def checkStability(self):
+        raise NotImplementedError
+
+    def prepare(self):
+        """Allocate the propagator and upload traces; returns the launch
+        arguments of rsg_modelling_run (kept so run() can be timed alone)."""
+        if not self.checkStability():
+            raise R.RsgError("Wavelet sampling interval (dt) does not match the stability criteria.")
+        m, src = self.model, self.source
+        self.waves = self.waves_cls(m, src.nt, src.dt, src.ot)
+        w = self.waves
+        self.der = Der(w.nx_pml, w.ny_pml, w.nz_pml, w.dx, w.dy, w.dz, self.order)
+        w.getPml().setSmax(SMAX)
+        w.getPml().computeABC()
+        g = w._grid(self.der)
+        s = w._state(m)
+        t_src, keep = src._device(SMAP)
+        recs = (R.Traces * max(len(self.recs), 1))()
+        self._keep = [keep]
+        self._rec_dev = {}
+        for i, (k, d) in enumerate(self.recs.items()):
+            t, kk = d._device(GMAP, _record_factor(m, d, GMAP), zero=True)
+            recs[i] = t
+            self._keep.append(kk)
+            self._rec_dev[k] = kk[2]
+        self._args = (g, s, t_src, recs, len(self.recs))
+        return self._args
+
+    def launch(self, it0=0, nsteps=None):
+        g, s, t_src, recs, nrec = self._args
+        nsteps = self.source.nt - it0 if nsteps is None else nsteps
+        R.check(R.lib().rsg_modelling_run(self.waves.kind, C.byref(g), C.byref(s), C.byref(t_src), nrec,
+                                          recs, it0, nsteps, self.graph_chunk, _stream()),
+                "rsg_modelling_run")
+
+    def finish(self):
+        torch.cuda.current_stream().synchronize()
+        for k, d in self.recs.items():
+            d.data[...] = self._rec_dev[k].cpu().numpy()
+
+    def run(self):
+        self.prepare()
+        self.launch()
+        self.finish()
+        return 1
+
+
+class ModellingAcoustic2D(_Modelling):
+    waves_cls = WavesAcoustic2D
+
+    def checkStability(self):
+        # modelling.cpp:181-194
+        dx, dz, dt = np.float32(self.model.dx), np.float32(self.model.dz), np.float32(self.source.dt)
+        vpmax = np.float32(self._vpmax())
+        dt_stab = 2.0 / (3.1415 * np.sqrt((1.0 / (dx * dx)) + (1 / (dz * dz))) * vpmax)
+        return bool(dt < np.float32(dt_stab))
+
+
+class ModellingElastic2D(ModellingAcoustic2D):
+    waves_cls = WavesElastic2D
+
+
+class ModellingAcoustic3D(_Modelling):
+    waves_cls = WavesAcoustic3D
+
+    def checkStability(self):
+        # modelling.cpp:346-360
+        m = self.model
+        dx, dy, dz = np.float32(m.dx), np.float32(m.dy), np.float32(m.dz)
+        dt = np.float32(self.source.dt)
+        vpmax = np.float32(self._vpmax())
+        dt_stab = 2.0 / (3.1415 * np.sqrt((1.0 / (dx * dx)) + (1 / (dy * dy)) + (1 / (dz * dz))) * vpmax)
+        return bool(dt < np.float32(dt_stab))

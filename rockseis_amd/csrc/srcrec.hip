@@ -1,0 +1,244 @@
+// Source injection and receiver gathers (WavesAcoustic2D/3D and
+// WavesElastic2D insertSource / recordData, waves.cpp:407-595, 998-1275,
+// 2085-2288): one lane per trace, bilinear (2D) or trilinear (3D) weights
+// xtri = {1-shift, shift} over the 2x2(x2) cells at lpml + map.
+//
+// The time index is either a constant or read from device memory plus an
+// offset, so that a captured hipGraph of several steps can be replayed.
+#include "rsg_common.h"
+#include "rsg_srcrec.h"
+
+namespace rsg {
+
+__device__ __forceinline__ int time_index(const int *it_dev, int it) {
+    return it_dev ? *it_dev + it : it;
+}
+
+// F += scale * Mod * wav * xtri * ytri (Mod null: modulus 1).  Corners of
+// different traces may coincide, so the add is atomic.
+__global__ void inject2d_kernel(float *F, const float *Mod, float scale, int nx, int lpml,
+                                int ntrace, const int *map, const float *shift, const float *wav,
+                                int nt, const int *it_dev, int it0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntrace) return;
+    const int mx = map[2 * i], mz = map[2 * i + 1];
+    if (mx < 0 || mz < 0) return;
+    const int it = time_index(it_dev, it0);
+    const float sx = shift[2 * i], sz = shift[2 * i + 1];
+    const float xtri[2] = {1 - sx, sx}, ytri[2] = {1 - sz, sz};
+    const float w = wav[(size_t)i * nt + it];
+#pragma unroll
+    for (int i1 = 0; i1 < 2; i1++)
+#pragma unroll
+        for (int i2 = 0; i2 < 2; i2++) {
+            const size_t p = (size_t)(lpml + mz + i1) * nx + (lpml + mx + i2);
+            const float m = Mod ? scale * Mod[p] : scale;
+            unsafeAtomicAdd(F + p, m * w * xtri[i2] * ytri[i1]);
+        }
+}
+
+__global__ void inject3d_kernel(float *F, const float *Mod, float scale, int nx, int ny, int lpml,
+                                int ntrace, const int *map, const float *shift, const float *wav,
+                                int nt, const int *it_dev, int it0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntrace) return;
+    const int mx = map[3 * i], my = map[3 * i + 1], mz = map[3 * i + 2];
+    if (mx < 0 || my < 0 || mz < 0) return;
+    const int it = time_index(it_dev, it0);
+    const float sx = shift[3 * i], sy = shift[3 * i + 1], sz = shift[3 * i + 2];
+    const float xtri[2] = {1 - sx, sx}, ytri[2] = {1 - sy, sy}, ztri[2] = {1 - sz, sz};
+    const float w = wav[(size_t)i * nt + it];
+    const size_t nxy = (size_t)nx * ny;
+#pragma unroll
+    for (int i1 = 0; i1 < 2; i1++)
+#pragma unroll
+        for (int i2 = 0; i2 < 2; i2++)
+#pragma unroll
+            for (int i3 = 0; i3 < 2; i3++) {
+                const size_t p = (size_t)(lpml + mz + i1) * nxy + (size_t)(lpml + my + i2) * nx +
+                                 (lpml + mx + i3);
+                const float m = Mod ? scale * Mod[p] : scale;
+                unsafeAtomicAdd(F + p, m * w * xtri[i3] * ytri[i2] * ztri[i1]);
+            }
+}
+
+// data[i*nt+it] += sum xtri*ytri*F over the 2x2 cells, in the reference's
+// corner order.  With F2 set, the elastic pressure 0.5*(F+F2)/factor is
+// accumulated in double as waves.cpp:2234-2237 does.
+__global__ void record2d_kernel(const float *F, const float *F2, const float *factor, int nx,
+                                int lpml, int ntrace, const int *map, const float *shift,
+                                float *data, int nt, const int *it_dev, int it0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntrace) return;
+    const int mx = map[2 * i], mz = map[2 * i + 1];
+    if (mx < 0 || mz < 0) return;
+    const int it = time_index(it_dev, it0);
+    const float sx = shift[2 * i], sz = shift[2 * i + 1];
+    const float xtri[2] = {1 - sx, sx}, ytri[2] = {1 - sz, sz};
+    float *d = data + (size_t)i * nt + it;
+    float acc = *d;
+    if (!F2) {
+#pragma unroll
+        for (int i1 = 0; i1 < 2; i1++)
+#pragma unroll
+            for (int i2 = 0; i2 < 2; i2++)
+                acc += xtri[i2] * ytri[i1] * F[(size_t)(lpml + mz + i1) * nx + (lpml + mx + i2)];
+    } else {
+        const double f = factor ? (double)factor[i] : 1.0;
+#pragma unroll
+        for (int i1 = 0; i1 < 2; i1++)
+#pragma unroll
+            for (int i2 = 0; i2 < 2; i2++) {
+                const size_t p = (size_t)(lpml + mz + i1) * nx + (lpml + mx + i2);
+                const double w = 0.5 * (double)xtri[i2] * (double)ytri[i1];
+                acc = (float)((double)acc + w * (double)F[p] / f);
+                acc = (float)((double)acc + w * (double)F2[p] / f);
+            }
+    }
+    *d = acc;
+}
+
+__global__ void record3d_kernel(const float *F, int nx, int ny, int lpml, int ntrace, const int *map,
+                                const float *shift, float *data, int nt, const int *it_dev, int it0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntrace) return;
+    const int mx = map[3 * i], my = map[3 * i + 1], mz = map[3 * i + 2];
+    if (mx < 0 || my < 0 || mz < 0) return;
+    const int it = time_index(it_dev, it0);
+    const float sx = shift[3 * i], sy = shift[3 * i + 1], sz = shift[3 * i + 2];
+    const float xtri[2] = {1 - sx, sx}, ytri[2] = {1 - sy, sy}, ztri[2] = {1 - sz, sz};
+    const size_t nxy = (size_t)nx * ny;
+    float *d = data + (size_t)i * nt + it;
+    float acc = *d;
+#pragma unroll
+    for (int i1 = 0; i1 < 2; i1++)
+#pragma unroll
+        for (int i2 = 0; i2 < 2; i2++)
+#pragma unroll
+            for (int i3 = 0; i3 < 2; i3++)
+                acc += xtri[i3] * ytri[i2] * ztri[i1] *
+                       F[(size_t)(lpml + mz + i1) * nxy + (size_t)(lpml + my + i2) * nx + (lpml + mx + i3)];
+    *d = acc;
+}
+
+__global__ void add_int_kernel(int *p, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
+}
+__global__ void set_int_kernel(int *p, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// Field and modulus a source of `field` drives, per propagator kind
+// (the switch statements of the insertSource members).
+static int source_target(int kind, const rsg_state *s, int field, float dt, float **F, float **F2,
+                         const float **Mod, float *scale) {
+    *F2 = nullptr;
+    *Mod = nullptr;
+    *scale = dt;
+    if (kind == RSG_KIND_E2D) {
+        switch (field) {
+        case RSG_PRESSURE:  // explosive: Sxx, Szz -= dt*wav*w (no modulus)
+            *F = s->Sxx;
+            *F2 = s->Szz;
+            *scale = -dt;
+            return RSG_OK;
+        case RSG_VX: *F = s->Vx; *Mod = s->Rx; return RSG_OK;
+        case RSG_VZ: *F = s->Vz; *Mod = s->Rz; return RSG_OK;
+        default: return fail(RSG_EINVAL, "elastic 2D source field must be P, VX or VZ");
+        }
+    }
+    switch (field) {
+    case RSG_PRESSURE: *F = s->P; *Mod = s->L; return RSG_OK;
+    case RSG_VX: *F = s->Vx; *Mod = s->Rx; return RSG_OK;
+    case RSG_VY:
+        if (kind != RSG_KIND_A3D) break;
+        *F = s->Vy;
+        *Mod = s->Ry;
+        return RSG_OK;
+    case RSG_VZ: *F = s->Vz; *Mod = s->Rz; return RSG_OK;
+    default: break;
+    }
+    return fail(RSG_EINVAL, "unsupported source field for this propagator");
+}
+
+int insert_source(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src,
+                  const int *it_dev, int it, hipStream_t st) {
+    if (!src || src->ntrace <= 0) return RSG_OK;
+    if (!it_dev && (it < 0 || it >= src->nt)) return fail(RSG_EINVAL, "insertSource: it out of range");
+    float *F, *F2;
+    const float *Mod;
+    float scale;
+    if (int rc = source_target(kind, s, src->field, g->dt, &F, &F2, &Mod, &scale)) return rc;
+    if (!F) return fail(RSG_EINVAL, "insertSource: target field not allocated");
+    const int threads = 64, blocks = (src->ntrace + threads - 1) / threads;
+    if (kind == RSG_KIND_A3D) {
+        inject3d_kernel<<<blocks, threads, 0, st>>>(F, Mod, scale, g->nx, g->ny, g->lpml, src->ntrace,
+                                                    src->map, src->shift, src->data, src->nt, it_dev, it);
+    } else {
+        inject2d_kernel<<<blocks, threads, 0, st>>>(F, Mod, scale, g->nx, g->lpml, src->ntrace, src->map,
+                                                    src->shift, src->data, src->nt, it_dev, it);
+        if (F2)
+            inject2d_kernel<<<blocks, threads, 0, st>>>(F2, Mod, scale, g->nx, g->lpml, src->ntrace,
+                                                        src->map, src->shift, src->data, src->nt, it_dev,
+                                                        it);
+    }
+    RSG_LAUNCH_CHECK("inject kernel");
+    return RSG_OK;
+}
+
+int record_data(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *rec,
+                const int *it_dev, int it, hipStream_t st) {
+    if (!rec || rec->ntrace <= 0) return RSG_OK;
+    if (!it_dev && (it < 0 || it >= rec->nt)) return fail(RSG_EINVAL, "recordData: it out of range");
+    const float *F = nullptr, *F2 = nullptr;
+    switch (rec->field) {
+    case RSG_PRESSURE:
+        if (kind == RSG_KIND_E2D) {
+            F = s->Sxx;
+            F2 = s->Szz;
+        } else {
+            F = s->P;
+        }
+        break;
+    case RSG_VX: F = s->Vx; break;
+    case RSG_VY: F = kind == RSG_KIND_A3D ? s->Vy : nullptr; break;
+    case RSG_VZ: F = s->Vz; break;
+    default: break;
+    }
+    if (!F) return fail(RSG_EINVAL, "recordData: field not available for this propagator");
+    const int threads = 64, blocks = (rec->ntrace + threads - 1) / threads;
+    if (kind == RSG_KIND_A3D)
+        record3d_kernel<<<blocks, threads, 0, st>>>(F, g->nx, g->ny, g->lpml, rec->ntrace, rec->map,
+                                                    rec->shift, rec->data, rec->nt, it_dev, it);
+    else
+        record2d_kernel<<<blocks, threads, 0, st>>>(F, F2, rec->factor, g->nx, g->lpml, rec->ntrace,
+                                                    rec->map, rec->shift, rec->data, rec->nt, it_dev, it);
+    RSG_LAUNCH_CHECK("record kernel");
+    return RSG_OK;
+}
+
+int launch_add_int(int *p, int v, hipStream_t st) {
+    add_int_kernel<<<1, 64, 0, st>>>(p, v);
+    RSG_LAUNCH_CHECK("add_int_kernel");
+    return RSG_OK;
+}
+
+int launch_set_int(int *p, int v, hipStream_t st) {
+    set_int_kernel<<<1, 64, 0, st>>>(p, v);
+    RSG_LAUNCH_CHECK("set_int_kernel");
+    return RSG_OK;
+}
+
+}  // namespace rsg
+
+extern "C" int rsg_insert_source(int kind, const rsg_grid *g, const rsg_state *s,
+                                 const rsg_traces *src, int it, void *stream) {
+    if (!g || !s) return rsg::fail(RSG_EINVAL, "null grid/state");
+    return rsg::insert_source(kind, g, s, src, nullptr, it, (hipStream_t)stream);
+}
+
+extern "C" int rsg_record_data(int kind, const rsg_grid *g, const rsg_state *s,
+                               const rsg_traces *rec, int it, void *stream) {
+    if (!g || !s) return rsg::fail(RSG_EINVAL, "null grid/state");
+    return rsg::record_data(kind, g, s, rec, nullptr, it, (hipStream_t)stream);
+}

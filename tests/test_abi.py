@@ -1,0 +1,122 @@
+"""The C-ABI library: it loads, exports every function include/rsgpu.h
+declares, and its host-side set-up (PML tables, Der coefficients, model
+staggering, coordinate maps) is bit-exact with the reference fixtures.
+No GPU is needed for any of these."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from rockseis_amd import _lib as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "rsgpu.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(rsg_[a-z0-9_]+)\s*\(", txt)))
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def test_exports_every_declared_symbol():
+    L = R.lib()
+    names = _declared()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(R.EXPORTS)
+
+
+def test_error_path_reports_message():
+    L = R.lib()
+    g = R.Grid(nx=10, ny=1, nz=10, lpml=3, order=9, fs=0, dx=1, dy=1, dz=1, dt=1e-3)
+    s = R.State()
+    rc = L.rsg_a2d_forward_velocity(C.byref(g), C.byref(s), None)
+    assert rc == -1
+    assert b"order" in L.rsg_last_error()
+
+
+def test_pml_tables_match_reference(golden):
+    g = golden("pml")
+    for k in g.files:
+        lpml = int(k[1:].split("_")[0])
+        dt = float(k.split("_dt")[1].split("_")[0])
+        sign = int(k.split("_s")[1])
+        t = np.zeros(12 * lpml, np.float32)
+        R.check(R.lib().rsg_pml_tables(lpml, dt, 1200.0, sign, _p(t)))
+        assert np.array_equal(t, g[k]), k
+
+
+def test_der_coeffs():
+    import oracle
+    c = np.zeros(36, np.float32)
+    R.check(R.lib().rsg_der_coeffs(_p(c)))
+    o = np.zeros(36, np.float32)
+    oracle.fn("der_coeffs")(o)
+    assert np.array_equal(c, o)
+
+
+def test_stagger_match_reference(golden):
+    g = golden("stagger")
+    lpml = int(g["lpml"])
+    nz, nx = g["vp"].shape
+    nz3, ny3, nx3 = g["vp3"].shape
+    L = R.lib()
+    for fs in (0, 1):
+        shp = (nz + 2 * lpml, nx + 2 * lpml)
+        out = [np.zeros(shp, np.float32) for _ in range(3)]
+        R.check(L.rsg_stagger_a2d(_p(g["vp"]), _p(g["rho"]), nx, nz, lpml, fs, *map(_p, out)))
+        for o, k in zip(out, ("L", "Rx", "Rz")):
+            assert np.array_equal(o, g[f"a2d_fs{fs}_{k}"]), k
+        out = [np.zeros(shp, np.float32) for _ in range(5)]
+        R.check(L.rsg_stagger_e2d(_p(g["vp"]), _p(g["vs"]), _p(g["rho"]), nx, nz, lpml, fs, *map(_p, out)))
+        for o, k in zip(out, ("L", "L2M", "M", "Rx", "Rz")):
+            assert np.array_equal(o, g[f"e2d_fs{fs}_{k}"]), k
+        shp3 = (nz3 + 2 * lpml, ny3 + 2 * lpml, nx3 + 2 * lpml)
+        out = [np.zeros(shp3, np.float32) for _ in range(4)]
+        R.check(L.rsg_stagger_a3d(_p(g["vp3"]), _p(g["rho3"]), nx3, ny3, nz3, lpml, fs, *map(_p, out)))
+        for o, k in zip(out, ("L", "Rx", "Ry", "Rz")):
+            assert np.array_equal(o, g[f"a3d_fs{fs}_{k}"]), k
+
+
+def test_zero_density_is_an_error():
+    vp = np.full((4, 5), 2000, np.float32)
+    rho = np.ones((4, 5), np.float32)
+    rho[2, :] = 0.0  # staggered averages vanish along this row
+    out = [np.zeros((8, 9), np.float32) for _ in range(3)]
+    rc = R.lib().rsg_stagger_a2d(_p(vp), _p(rho), 5, 4, 2, 0, *map(_p, out))
+    assert rc != 0 and b"Zero density" in R.lib().rsg_last_error()
+
+
+def test_makemap_match_reference(golden):
+    g = golden("makemap")
+    coords = g["coords"]
+    n = coords.shape[0]
+    for field in (0, 1, 3):
+        m = np.full((n, 2), -1, np.int32)
+        s = np.zeros((n, 2), np.float32)
+        R.check(R.lib().rsg_makemap2d(_p(coords), n, 40, 30, 10.0, 10.0, 0.0, 0.0, field, _p(m), _p(s)))
+        assert np.array_equal(m, g[f"map_f{field}"])
+        ok = m >= 0
+        assert np.array_equal(s[ok], g[f"shift_f{field}"][ok])
+
+
+def test_makemap3d_against_oracle():
+    from oracle import runs
+    rng = np.random.default_rng(4)
+    c = np.stack([rng.uniform(-20, 230, 200), rng.uniform(-20, 250, 200), rng.uniform(-20, 210, 200)],
+                 1).astype(np.float32)
+    c[:4] = np.float32([[0, 0, 0], [10, 20, 30], [220, 230, 200], [5, 5, 5]])
+    for name, f in (("P", 0), ("VX", 1), ("VY", 2), ("VZ", 3)):
+        m = np.full((200, 3), -1, np.int32)
+        s = np.zeros((200, 3), np.float32)
+        R.check(R.lib().rsg_makemap3d(_p(c), 200, 22, 24, 20, 10.0, 10.0, 10.0, 0.0, 0.0, 0.0, f, _p(m), _p(s)))
+        mo, so = runs.makemap3d(c, (22, 24, 20), (10.0, 10.0, 10.0), name)
+        assert np.array_equal(m, mo)
+        assert np.array_equal(s[m >= 0], so[mo >= 0])
