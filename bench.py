@@ -1,0 +1,211 @@
+#!/usr/bin/env python
+"""Benchmark of the staggered-grid FD time step on one MI355X.
+
+Default workload: plain elastic 2D modelling, 2001 x 1001 cells at dx = 5 m
+(2081 x 1081 padded with a 40-cell PML), 8th-order stencil, fp32, explosive
+20 Hz Ricker, 2001 receivers recording vx and vz, dt = 0.5 ms, nt = 6000;
+Vp = 1500->4000 m/s linear gradient times 20 seeded Gaussian anomalies,
+Vs = Vp/1.73, Gardner density.  All inputs are synthetic and resident in HBM
+before the timed region.
+
+One "step" is one FD time step of that workload (velocity half step, stress
+half step, source injection, two receiver gathers); --steps K times K
+consecutive steps (K <= nt, default the full nt = 6000) after --warmup W
+untimed steps (state and records are zeroed between warm-up and timing).
+The time loop runs on the device as replayed hipGraph chunks.
+
+Prints one JSON line: value = padded cells x K / wall seconds in
+Gcell-updates/s, the roofline of the dominant kernel (elastic stress half
+step: 44 B/cell + 32*lpml*(nx+nz) B of PML memory traffic per launch, timed
+with HIP events on the launch stream) and the reference CPU path timed on
+this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s (% HBM peak); FWI gradient shots/s, batched adjoint vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--graph-chunk", type=int, default=100)
+    ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=80, help="time steps of the CPU sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def build_workload():
+    import numpy as np
+    from rockseis_amd import core, synth
+
+    nx, nz, lpml, dx, dt, nt, order, f0 = 2001, 1001, 40, 5.0, 5e-4, 6000, 8, 20.0
+    vp = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1500.0, 4000.0, nanom=20, seed=2024)
+    vs = (vp / 1.73).astype(np.float32)
+    rho = synth.gardner(vp)
+    m = core.ModelElastic2D(nx, nz, lpml, dx, dx, 0.0, 0.0, False)
+    m.Vp[:], m.Vs[:], m.R[:] = vp, vs, rho
+    src = core.Data2D(1, nt, dt, 0.0, "P")
+    src.scoords[:] = np.float32([[nx // 2 * dx, 10.0]])
+    src.data[:] = synth.ricker(nt, dt, f0)
+    src.makeMap(m, core.SMAP)
+    recs = []
+    for f in ("VX", "VZ"):
+        r = core.Data2D(nx, nt, dt, 0.0, f)
+        r.gcoords[:] = synth.surface_line(nx, dx, 10.0)
+        r.makeMap(m, core.GMAP)
+        recs.append(r)
+    m.staggerModels()
+    mod = core.ModellingElastic2D(m, src, order, 1)
+    mod.setRecVx(recs[0])
+    mod.setRecVz(recs[1])
+    cfg = dict(nx=nx, nz=nz, lpml=lpml, dx=dx, dt=dt, nt=nt, order=order, f0=f0,
+               nx_pml=nx + 2 * lpml, nz_pml=nz + 2 * lpml)
+    return mod, cfg, (vp, vs, rho, src, recs)
+
+
+def reset(mod):
+    w = mod.waves
+    for f in ("Sxx", "Szz", "Sxz", "Vx", "Vz"):
+        getattr(w, f).zero_()
+    for t in w.psi.values():
+        t.zero_()
+    for t in mod._rec_dev.values():
+        t.zero_()
+
+
+def kernel_time_ms(mod, which, reps):
+    """Average duration of one half-step kernel, from HIP events around
+    `reps` back-to-back launches on the stream the kernels run on."""
+    import torch
+    from rockseis_amd import _lib as R
+    g, s = mod._args[0], mod._args[1]
+    st = torch.cuda.current_stream()
+    lib = R.lib()
+    R.check(lib.rsg_repeat_step(R.KIND_E2D, which, C.byref(g), C.byref(s), 5, C.c_void_p(st.cuda_stream)))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    R.check(lib.rsg_repeat_step(R.KIND_E2D, which, C.byref(g), C.byref(s), reps, C.c_void_p(st.cuda_stream)))
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def cpu_baseline(cfg, inputs, nsteps):
+    """The reference WavesElastic2D (oracle/_ref, compiled from the reference
+    sources, single thread) or, where that build is absent, the oracle's C
+    restatement: full padded grid, `nsteps` time steps; the one-step run is
+    subtracted to remove set-up."""
+    import numpy as np
+    import oracle
+    from oracle import runs
+    vp, vs, rho, src, recs = inputs
+    sc = src.scoords
+    gc = recs[0].gcoords
+    wav = src.data[:, :nsteps + 1]
+    args = (vp, vs, rho, cfg["lpml"], cfg["dx"], cfg["dx"], cfg["dt"])
+    kind = "reference" if oracle.ref_available() else "port"
+    fn = runs.ref_e2d_run if kind == "reference" else runs.e2d_run
+
+    def run(n):
+        t = time.perf_counter()
+        fn(*args, n, cfg["order"], False, "P", sc, np.ascontiguousarray(wav[:, :n]), "VX", gc)
+        return time.perf_counter() - t
+
+    t1 = run(1)
+    tn = run(nsteps + 1)
+    per_step = (tn - t1) / nsteps
+    cells = cfg["nx_pml"] * cfg["nz_pml"]
+    return {"value": round(cells / per_step / 1e9, 6), "unit": "Gcell-updates/s", "cores": 1,
+            "kind": kind,
+            "sample": (f"{nsteps} time steps of the same elastic 2D workload on the full "
+                       f"{cfg['nx_pml']}x{cfg['nz_pml']} padded grid, order {cfg['order']}, "
+                       f"single thread ({'reference WavesElastic2D built from its sources' if kind == 'reference' else 'oracle C restatement'}), "
+                       f"{per_step:.3f} s per step")}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): this path runs "
+              "one shot window per GPU and needs no inter-GPU traffic", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.cuda.set_device(0)
+    mod, cfg, inputs = build_workload()
+    K = min(args.steps, cfg["nt"])
+    W = min(args.warmup, cfg["nt"])
+    mod.setGraphChunk(args.graph_chunk)
+    mod.prepare()
+    if W:
+        mod.launch(0, W)
+    torch.cuda.synchronize()
+    reset(mod)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mod.launch(0, K)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    cells = cfg["nx_pml"] * cfg["nz_pml"]
+    value = cells * K / wall / 1e9
+    # algorithmic bytes (fp32 two-pass leapfrog): velocity 7R+2W, stress 8R+3W per cell;
+    # PML memory variables: 4 per half step on x-side strips and 4 on z-side, RMW
+    strip_cells = cfg["lpml"] * (cfg["nx_pml"] + cfg["nz_pml"]) * 2
+    b_vel = 36 * cells + 2 * 8 * strip_cells
+    b_str = 44 * cells + 2 * 8 * strip_cells
+    t_vel = kernel_time_ms(mod, 0, args.kernel_reps)
+    t_str = kernel_time_ms(mod, 1, args.kernel_reps)
+    step_bytes = b_vel + b_str
+    roof = {"bound": "hbm", "kernel": "e2d_stress_kernel<8>", "achieved": round(b_str / (t_str * 1e-3) / 1e9, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(b_str / (t_str * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": _traffic("e2d_stress"),
+            "bytes_per_launch": b_str, "avg_launch_us": round(t_str * 1e3, 2),
+            "velocity_kernel": {"achieved": round(b_vel / (t_vel * 1e-3) / 1e9, 1),
+                                "avg_launch_us": round(t_vel * 1e3, 2), "bytes_per_launch": b_vel}}
+    step_gbs = step_bytes * K / wall / 1e9
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": round(wall / K * 1e3, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain elastic 2D modelling (ModellingElastic2D::run)",
+                   "grid": f"{cfg['nx']}x{cfg['nz']} + lpml {cfg['lpml']} -> {cfg['nx_pml']}x{cfg['nz_pml']}",
+                   "dx_m": cfg["dx"], "order": cfg["order"], "dt_s": cfg["dt"], "nt": cfg["nt"],
+                   "source": f"explosive {cfg['f0']:g} Hz Ricker", "receivers": "2001 x (vx, vz)",
+                   "graph_chunk": args.graph_chunk, "parallelism": "single GPU"},
+        "hbm_frac_whole_step": round(step_gbs / HBM_PEAK_GBS, 4),
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, inputs, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def _traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE
+    for this kernel at this workload (profiles/<name>_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+if __name__ == "__main__":
+    main()

@@ -140,6 +140,12 @@ int rsg_e2d_backward_stress(const rsg_grid *g, const rsg_state *s, void *stream)
 int rsg_a3d_forward_velocity(const rsg_grid *g, const rsg_state *s, void *stream);
 int rsg_a3d_forward_stress(const rsg_grid *g, const rsg_state *s, void *stream);
 
+/* Launch one half step `n` times back to back (which: 0 forward velocity,
+ * 1 forward stress, 2 adjoint-state velocity, 3 adjoint-state stress), for
+ * timing a kernel with events on `stream` without per-launch host overhead. */
+int rsg_repeat_step(int kind, int which, const rsg_grid *g, const rsg_state *s, int n,
+                    void *stream);
+
 /* insertSource (waves.cpp:407-497, 998-1137, 2085-2171): kind is
  * 1 acoustic 2D, 2 elastic 2D, 3 acoustic 3D; the field and its modulus are
  * chosen from src->field as the reference does. */

@@ -83,6 +83,7 @@ def _declare(L):
         "rsg_insert_source": ([ip, GP, SP, TP, ip, vp], ip),
         "rsg_record_data": ([ip, GP, SP, TP, ip, vp], ip),
         "rsg_modelling_run": ([ip, GP, SP, TP, ip, TP, ip, ip, ip, vp], ip),
+        "rsg_repeat_step": ([ip, ip, GP, SP, ip, vp], ip),
     }
     for name in ("a2d_forward_velocity", "a2d_forward_stress", "e2d_forward_velocity",
                  "e2d_forward_stress", "e2d_backward_velocity", "e2d_backward_stress",
@@ -102,7 +103,7 @@ EXPORTS = (
     "rsg_makemap2d", "rsg_makemap3d", "rsg_a2d_forward_velocity", "rsg_a2d_forward_stress",
     "rsg_e2d_forward_velocity", "rsg_e2d_forward_stress", "rsg_e2d_backward_velocity",
     "rsg_e2d_backward_stress", "rsg_a3d_forward_velocity", "rsg_a3d_forward_stress",
-    "rsg_insert_source", "rsg_record_data", "rsg_modelling_run",
+    "rsg_insert_source", "rsg_record_data", "rsg_modelling_run", "rsg_repeat_step",
 )
 
 

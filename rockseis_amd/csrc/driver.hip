@@ -118,3 +118,26 @@ extern "C" int rsg_modelling_run(int kind, const rsg_grid *g, const rsg_state *s
     if (own) (void)hipStreamDestroy(st);
     return rc;
 }
+
+extern "C" int rsg_repeat_step(int kind, int which, const rsg_grid *g, const rsg_state *s, int n,
+                               void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    for (int i = 0; i < n; i++) {
+        int rc;
+        switch (which) {
+        case 0: rc = step_velocity(kind, g, s, st); break;
+        case 1: rc = step_stress(kind, g, s, st); break;
+        case 2:
+            rc = kind == RSG_KIND_E2D ? rsg_e2d_backward_velocity(g, s, st)
+                                      : fail(RSG_EINVAL, "adjoint-state steps exist for elastic 2D only");
+            break;
+        case 3:
+            rc = kind == RSG_KIND_E2D ? rsg_e2d_backward_stress(g, s, st)
+                                      : fail(RSG_EINVAL, "adjoint-state steps exist for elastic 2D only");
+            break;
+        default: rc = fail(RSG_EINVAL, "rsg_repeat_step: which must be 0..3");
+        }
+        if (rc) return rc;
+    }
+    return RSG_OK;
+}
