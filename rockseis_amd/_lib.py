@@ -84,6 +84,8 @@ def _declare(L):
         "rsg_record_data": ([ip, GP, SP, TP, ip, vp], ip),
         "rsg_modelling_run": ([ip, GP, SP, TP, ip, TP, ip, ip, ip, vp], ip),
         "rsg_repeat_step": ([ip, ip, GP, SP, ip, vp], ip),
+        "rsg_set_tuning": ([C.c_char_p, ip], ip),
+        "rsg_get_tuning": ([C.c_char_p, C.POINTER(C.c_int)], ip),
     }
     for name in ("a2d_forward_velocity", "a2d_forward_stress", "e2d_forward_velocity",
                  "e2d_forward_stress", "e2d_backward_velocity", "e2d_backward_stress",
@@ -104,7 +106,18 @@ EXPORTS = (
     "rsg_e2d_forward_velocity", "rsg_e2d_forward_stress", "rsg_e2d_backward_velocity",
     "rsg_e2d_backward_stress", "rsg_a3d_forward_velocity", "rsg_a3d_forward_stress",
     "rsg_insert_source", "rsg_record_data", "rsg_modelling_run", "rsg_repeat_step",
+    "rsg_set_tuning", "rsg_get_tuning",
 )
+
+
+def set_tuning(key: str, value: int):
+    check(lib().rsg_set_tuning(key.encode(), int(value)), f"rsg_set_tuning({key})")
+
+
+def get_tuning(key: str) -> int:
+    v = C.c_int()
+    check(lib().rsg_get_tuning(key.encode(), C.byref(v)), f"rsg_get_tuning({key})")
+    return v.value
 
 
 def check(rc: int, what: str = ""):

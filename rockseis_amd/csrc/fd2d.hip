@@ -5,26 +5,30 @@
 // One fused kernel per half step.  The reference makes, per derivative, a
 // full-grid Der pass into a scratch array, a full-grid update pass and a PML
 // strip pass (der.cpp, waves.cpp); here each block stages the stencil inputs
-// of a TX x TZ tile (plus halo) in LDS once, evaluates every derivative of the
-// half step from LDS, and applies the field update and the CFS-PML memory
-// update of every cell in registers, so each array crosses HBM once per half
-// step.  The per-cell operation sequence is the reference's: += dt*M*d for
-// the first derivative, its PML correction, then the next derivative.
+// of a 64 x TZ tile (plus halo) in LDS once, evaluates every derivative of
+// the half step from LDS, and applies the field update and the CFS-PML
+// memory update of every cell in registers, so each array crosses HBM once
+// per half step.  The per-cell operation sequence is the reference's: += dt*M*d
+// for the first derivative, its PML correction, then the next derivative.
 //
 // Block: 64 x 4 threads (four wave64 rows); each thread owns RZ consecutive
-// cells of one x column, so z-derivative taps are reused from registers and
-// x-derivative taps are conflict-free LDS row reads.  Cells whose stencil
-// reaches the array edge use the reference's reduced-order coefficient sets.
+// cells of one x column.  The thread's own field and model values are loaded
+// into registers before the tile is staged so both sets of loads are in
+// flight together; x-derivative taps are conflict-free LDS row reads.  Cells
+// whose stencil reaches the array edge use the reference's reduced-order
+// coefficient sets.  Tiles are numbered so that consecutive tiles, which
+// share halos, run on the same XCD (blocks are dealt round-robin to the 8
+// XCDs, each with its own L2).
 #include "rsg_common.h"
 
 namespace rsg {
 
 constexpr int BX = 64;
 constexpr int BZ = 4;
-constexpr int RZ2D = 8;
 
 struct K2D {
     int nx, nz, lpml, fs;
+    int ntx, ntiles;  // tiles along x, total tiles
     float dt, invdx, invdz;
     const float *tab;
     float *P, *Vx, *Vz, *Sxx, *Szz, *Sxz;
@@ -33,39 +37,59 @@ struct K2D {
     Coefs cx, cz;
 };
 
-// Stage rows [z0-HZ, z0+TZ+HZ) x cols [x0-HX, x0+TX+HX) of g into LDS,
+// Tile handled by this block: XCD-contiguous numbering of a 1D grid.
+__device__ __forceinline__ bool tile_of(const K2D &a, int TZ, int &x0, int &z0) {
+    const int nb = gridDim.x;
+    const int per = (nb + 7) / 8;
+    const int t = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (t >= a.ntiles) return false;
+    x0 = (t % a.ntx) * BX;
+    z0 = (t / a.ntx) * TZ;
+    return true;
+}
+
+// Stage rows [z0-HZ, z0+TZ+HZ) x cols [x0-HX, x0+BX+HX) of g into LDS,
 // zero outside the grid (those entries are never read by the edge stencils).
-template <int TX, int TZ, int HX, int HZ>
+template <int TZ, int HX, int HZ>
 __device__ __forceinline__ void load_tile(float *lds, const float *__restrict__ g, int nx, int nz,
                                           int x0, int z0) {
-    constexpr int LD = TX + 2 * HX, N = LD * (TZ + 2 * HZ);
-    for (int i = threadIdx.x + threadIdx.y * BX; i < N; i += BX * BZ) {
-        const int r = i / LD, c = i - r * LD;
-        const int z = z0 - HZ + r, x = x0 - HX + c;
-        float v = 0.f;
-        if (x >= 0 && x < nx && z >= 0 && z < nz) v = g[(size_t)z * nx + x];
-        lds[i] = v;
+    constexpr int LD = BX + 2 * HX, ROWS = TZ + 2 * HZ;
+#pragma unroll
+    for (int r = threadIdx.y; r < ROWS; r += BZ) {
+        const int z = z0 - HZ + r;
+        const bool zin = z >= 0 && z < nz;
+        const float *row = g + (ptrdiff_t)z * nx;
+#pragma unroll
+        for (int c = threadIdx.x; c < LD; c += BX) {
+            const int x = x0 - HX + c;
+            lds[r * LD + c] = (zin && x >= 0 && x < nx) ? row[x] : 0.f;
+        }
     }
 }
 
-// Same, staging a*b + c*d (c may be null: a*b) -- the moduli-weighted
-// stresses of WavesElastic2D::backwardstepVelocity (waves.cpp:1771-1775).
-template <int TX, int TZ, int HX, int HZ>
+// Same, staging a*b + c*d (c null: a*b) -- the moduli-weighted stresses of
+// WavesElastic2D::backwardstepVelocity (waves.cpp:1771-1775).
+template <int TZ, int HX, int HZ>
 __device__ __forceinline__ void load_tile_mix(float *lds, const float *__restrict__ a,
                                               const float *__restrict__ b,
                                               const float *__restrict__ c,
                                               const float *__restrict__ d, int nx, int nz, int x0,
                                               int z0) {
-    constexpr int LD = TX + 2 * HX, N = LD * (TZ + 2 * HZ);
-    for (int i = threadIdx.x + threadIdx.y * BX; i < N; i += BX * BZ) {
-        const int r = i / LD, cc = i - r * LD;
-        const int z = z0 - HZ + r, x = x0 - HX + cc;
-        float v = 0.f;
-        if (x >= 0 && x < nx && z >= 0 && z < nz) {
-            const size_t p = (size_t)z * nx + x;
-            v = c ? a[p] * b[p] + c[p] * d[p] : a[p] * b[p];
+    constexpr int LD = BX + 2 * HX, ROWS = TZ + 2 * HZ;
+#pragma unroll
+    for (int r = threadIdx.y; r < ROWS; r += BZ) {
+        const int z = z0 - HZ + r;
+        const bool zin = z >= 0 && z < nz;
+#pragma unroll
+        for (int cc = threadIdx.x; cc < LD; cc += BX) {
+            const int x = x0 - HX + cc;
+            float v = 0.f;
+            if (zin && x >= 0 && x < nx) {
+                const size_t p = (size_t)z * nx + x;
+                v = c ? a[p] * b[p] + c[p] * d[p] : a[p] * b[p];
+            }
+            lds[r * LD + cc] = v;
         }
-        lds[i] = v;
     }
 }
 
@@ -99,7 +123,7 @@ struct Tab {
     }
 };
 
-// Position of a cell relative to the x-side and z-side PML strips.
+// Position of a cell relative to the strips of one axis.
 struct PmlPos {
     int side;  // -1 none, 0 low (left/top), 1 high (right/bottom)
     int i;     // index into the strip profile (table index)
@@ -117,8 +141,6 @@ __device__ __forceinline__ PmlPos pml_pos(int x, int n, int L) {
 }
 
 // Apply dt*M*d then the strip correction of one derivative term.
-// xs: strip is an x-side strip (index z*L+i), else a z-side strip
-// (index i*nx+x).  lo/hi: the two memory arrays of that term.
 __device__ __forceinline__ void term(float &F, float dtM, float d, const PmlPos &pp, const Tab &tb,
                                      int stag, float *lo, float *hi, size_t sidx) {
     F += dtM * d;
@@ -137,31 +159,46 @@ template <int ORDER, int RZ>
 __global__ __launch_bounds__(BX *BZ) void a2d_velocity_kernel(const K2D a) {
     constexpr int H = ORDER, TZ = BZ * RZ, LD = BX + 2 * H;
     __shared__ float sP[(TZ + 2 * H) * LD];
-    const int x0 = blockIdx.x * BX, z0 = blockIdx.y * TZ;
-    load_tile<BX, TZ, H, H>(sP, a.P, a.nx, a.nz, x0, z0);
+    int x0, z0;
+    if (!tile_of(a, TZ, x0, z0)) return;
+    const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
+    const bool xin = x < nx;
+    float *__restrict__ Vx = a.Vx;
+    float *__restrict__ Vz = a.Vz;
+    const float *__restrict__ Rx = a.Rx;
+    const float *__restrict__ Rz = a.Rz;
+    float vx[RZ], vz[RZ], rx[RZ], rz[RZ];
+#pragma unroll
+    for (int r = 0; r < RZ; r++) {
+        const int z = z0 + ty * RZ + r;
+        const size_t p = (size_t)z * nx + x;
+        const bool in = xin && z < nz;
+        vx[r] = in ? Vx[p] : 0.f;
+        vz[r] = in ? Vz[p] : 0.f;
+        rx[r] = in ? Rx[p] : 0.f;
+        rz[r] = in ? Rz[p] : 0.f;
+    }
+    load_tile<TZ, H, H>(sP, a.P, nx, nz, x0, z0);
     __syncthreads();
-    const int tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    if (x >= a.nx) return;
+    if (!xin) return;
     const Tab tb{a.tab, a.lpml};
-    const PmlPos px = pml_pos(x, a.nx, a.lpml);
-    const int kx = kfw(x, a.nx, ORDER);
+    const PmlPos px = pml_pos(x, nx, a.lpml);
+    const int kx = kfw(x, nx, ORDER);
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
         const int zl = ty * RZ + r, z = z0 + zl;
-        if (z >= a.nz) break;
+        if (z >= nz) break;
         const float *s = sP + (zl + H) * LD + tx + H;
         const float dPx = ddfw<ORDER>(s, 1, a.cx.c, kx, a.invdx);
-        const float dPz = ddfw<ORDER>(s, LD, a.cz.c, kfw(z, a.nz, ORDER), a.invdz);
-        const size_t p = (size_t)z * a.nx + x;
-        const PmlPos pz = pml_pos(z, a.nz, a.lpml);
-        float vx = a.Vx[p];
-        term(vx, a.dt * a.Rx[p], dPx, px, tb, 1, a.psi[RSG_A2D_P_LEFT], a.psi[RSG_A2D_P_RIGHT],
+        const float dPz = ddfw<ORDER>(s, LD, a.cz.c, kfw(z, nz, ORDER), a.invdz);
+        const size_t p = (size_t)z * nx + x;
+        const PmlPos pz = pml_pos(z, nz, a.lpml);
+        term(vx[r], a.dt * rx[r], dPx, px, tb, 1, a.psi[RSG_A2D_P_LEFT], a.psi[RSG_A2D_P_RIGHT],
              (size_t)z * a.lpml + px.i);
-        a.Vx[p] = vx;
-        float vz = a.Vz[p];
-        term(vz, a.dt * a.Rz[p], dPz, pz, tb, 1, a.psi[RSG_A2D_P_TOP], a.psi[RSG_A2D_P_BOTTOM],
-             (size_t)pz.i * a.nx + x);
-        a.Vz[p] = vz;
+        term(vz[r], a.dt * rz[r], dPz, pz, tb, 1, a.psi[RSG_A2D_P_TOP], a.psi[RSG_A2D_P_BOTTOM],
+             (size_t)pz.i * nx + x);
+        Vx[p] = vx[r];
+        Vz[p] = vz[r];
     }
 }
 
@@ -170,31 +207,42 @@ __global__ __launch_bounds__(BX *BZ) void a2d_stress_kernel(const K2D a) {
     constexpr int H = ORDER, TZ = BZ * RZ, LDX = BX + 2 * H, LDZ = BX;
     __shared__ float sVx[TZ * LDX];
     __shared__ float sVz[(TZ + 2 * H) * LDZ];
-    const int x0 = blockIdx.x * BX, z0 = blockIdx.y * TZ;
-    load_tile<BX, TZ, H, 0>(sVx, a.Vx, a.nx, a.nz, x0, z0);
-    load_tile<BX, TZ, 0, H>(sVz, a.Vz, a.nx, a.nz, x0, z0);
+    int x0, z0;
+    if (!tile_of(a, TZ, x0, z0)) return;
+    const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
+    const bool xin = x < nx;
+    float *__restrict__ P = a.P;
+    const float *__restrict__ L = a.L;
+    float pv[RZ], lv[RZ];
+#pragma unroll
+    for (int r = 0; r < RZ; r++) {
+        const int z = z0 + ty * RZ + r;
+        const size_t p = (size_t)z * nx + x;
+        const bool in = xin && z < nz;
+        pv[r] = in ? P[p] : 0.f;
+        lv[r] = in ? L[p] : 0.f;
+    }
+    load_tile<TZ, H, 0>(sVx, a.Vx, nx, nz, x0, z0);
+    load_tile<TZ, 0, H>(sVz, a.Vz, nx, nz, x0, z0);
     __syncthreads();
-    const int tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    if (x >= a.nx) return;
+    if (!xin) return;
     const Tab tb{a.tab, a.lpml};
-    const PmlPos px = pml_pos(x, a.nx, a.lpml);
-    const int kx = kbw(x, a.nx, ORDER);
+    const PmlPos px = pml_pos(x, nx, a.lpml);
+    const int kx = kbw(x, nx, ORDER);
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
         const int zl = ty * RZ + r, z = z0 + zl;
-        if (z >= a.nz) break;
+        if (z >= nz) break;
         const float dVx = ddbw<ORDER>(sVx + zl * LDX + tx + H, 1, a.cx.c, kx, a.invdx);
-        const float dVz = ddbw<ORDER>(sVz + (zl + H) * LDZ + tx, LDZ, a.cz.c, kbw(z, a.nz, ORDER),
+        const float dVz = ddbw<ORDER>(sVz + (zl + H) * LDZ + tx, LDZ, a.cz.c, kbw(z, nz, ORDER),
                                       a.invdz);
-        const size_t p = (size_t)z * a.nx + x;
-        const PmlPos pz = pml_pos(z, a.nz, a.lpml);
-        const float dtL = a.dt * a.L[p];
-        float P = a.P[p];
-        term(P, dtL, dVx, px, tb, 0, a.psi[RSG_A2D_VXX_LEFT], a.psi[RSG_A2D_VXX_RIGHT],
+        const PmlPos pz = pml_pos(z, nz, a.lpml);
+        const float dtL = a.dt * lv[r];
+        term(pv[r], dtL, dVx, px, tb, 0, a.psi[RSG_A2D_VXX_LEFT], a.psi[RSG_A2D_VXX_RIGHT],
              (size_t)z * a.lpml + px.i);
-        term(P, dtL, dVz, pz, tb, 0, a.psi[RSG_A2D_VZZ_TOP], a.psi[RSG_A2D_VZZ_BOTTOM],
-             (size_t)pz.i * a.nx + x);
-        a.P[p] = P;
+        term(pv[r], dtL, dVz, pz, tb, 0, a.psi[RSG_A2D_VZZ_TOP], a.psi[RSG_A2D_VZZ_BOTTOM],
+             (size_t)pz.i * nx + x);
+        P[(size_t)z * nx + x] = pv[r];
     }
 }
 
@@ -214,45 +262,60 @@ __global__ __launch_bounds__(BX *BZ) void e2d_velocity_kernel(const K2D a) {
     __shared__ float sXX[TZ * LDXX];
     __shared__ float sZZ[(TZ + 2 * H) * LDZZ];
     __shared__ float sXZ[(TZ + 2 * H) * LDXZ];
-    const int x0 = blockIdx.x * BX, z0 = blockIdx.y * TZ;
+    int x0, z0;
+    if (!tile_of(a, TZ, x0, z0)) return;
+    const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
+    const bool xin = x < nx;
+    float *__restrict__ Vx = a.Vx;
+    float *__restrict__ Vz = a.Vz;
+    const float *__restrict__ Rx = a.Rx;
+    const float *__restrict__ Rz = a.Rz;
+    float vx[RZ], vz[RZ], rx[RZ], rz[RZ];
+#pragma unroll
+    for (int r = 0; r < RZ; r++) {
+        const int z = z0 + ty * RZ + r;
+        const size_t p = (size_t)z * nx + x;
+        const bool in = xin && z < nz;
+        vx[r] = in ? Vx[p] : 0.f;
+        vz[r] = in ? Vz[p] : 0.f;
+        rx[r] = in ? Rx[p] : 0.f;
+        rz[r] = in ? Rz[p] : 0.f;
+    }
     if (!BW) {
-        load_tile<BX, TZ, H, 0>(sXX, a.Sxx, a.nx, a.nz, x0, z0);
-        load_tile<BX, TZ, 0, H>(sZZ, a.Szz, a.nx, a.nz, x0, z0);
-        load_tile<BX, TZ, H, H>(sXZ, a.Sxz, a.nx, a.nz, x0, z0);
+        load_tile<TZ, H, 0>(sXX, a.Sxx, nx, nz, x0, z0);
+        load_tile<TZ, 0, H>(sZZ, a.Szz, nx, nz, x0, z0);
+        load_tile<TZ, H, H>(sXZ, a.Sxz, nx, nz, x0, z0);
     } else {
-        load_tile_mix<BX, TZ, H, 0>(sXX, a.L2M, a.Sxx, a.L, a.Szz, a.nx, a.nz, x0, z0);
-        load_tile_mix<BX, TZ, 0, H>(sZZ, a.L, a.Sxx, a.L2M, a.Szz, a.nx, a.nz, x0, z0);
-        load_tile_mix<BX, TZ, H, H>(sXZ, a.M, a.Sxz, nullptr, nullptr, a.nx, a.nz, x0, z0);
+        load_tile_mix<TZ, H, 0>(sXX, a.L2M, a.Sxx, a.L, a.Szz, nx, nz, x0, z0);
+        load_tile_mix<TZ, 0, H>(sZZ, a.L, a.Sxx, a.L2M, a.Szz, nx, nz, x0, z0);
+        load_tile_mix<TZ, H, H>(sXZ, a.M, a.Sxz, nullptr, nullptr, nx, nz, x0, z0);
     }
     __syncthreads();
-    const int tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    if (x >= a.nx) return;
+    if (!xin) return;
     const Tab tb{a.tab, a.lpml};
-    const PmlPos px = pml_pos(x, a.nx, a.lpml);
-    const int kxf = kfw(x, a.nx, ORDER), kxb = kbw(x, a.nx, ORDER);
+    const PmlPos px = pml_pos(x, nx, a.lpml);
+    const int kxf = kfw(x, nx, ORDER), kxb = kbw(x, nx, ORDER);
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
         const int zl = ty * RZ + r, z = z0 + zl;
-        if (z >= a.nz) break;
+        if (z >= nz) break;
         const float dSxx = ddfw<ORDER>(sXX + zl * LDXX + tx + H, 1, a.cx.c, kxf, a.invdx);
         const float *sxz = sXZ + (zl + H) * LDXZ + tx + H;
-        const float dSxz_z = ddbw<ORDER>(sxz, LDXZ, a.cz.c, kbw(z, a.nz, ORDER), a.invdz);
+        const float dSxz_z = ddbw<ORDER>(sxz, LDXZ, a.cz.c, kbw(z, nz, ORDER), a.invdz);
         const float dSxz_x = ddbw<ORDER>(sxz, 1, a.cx.c, kxb, a.invdx);
-        const float dSzz = ddfw<ORDER>(sZZ + (zl + H) * LDZZ + tx, LDZZ, a.cz.c,
-                                       kfw(z, a.nz, ORDER), a.invdz);
-        const size_t p = (size_t)z * a.nx + x;
-        const PmlPos pz = pml_pos(z, a.nz, a.lpml);
-        const size_t ix = (size_t)z * a.lpml + px.i, iz = (size_t)pz.i * a.nx + x;
-        const float dtRx = a.dt * a.Rx[p];
-        float vx = a.Vx[p];
-        term(vx, dtRx, dSxx, px, tb, 1, a.psi[RSG_E2D_SXX_LEFT], a.psi[RSG_E2D_SXX_RIGHT], ix);
-        term(vx, dtRx, dSxz_z, pz, tb, 0, a.psi[RSG_E2D_SXZZ_TOP], a.psi[RSG_E2D_SXZZ_BOTTOM], iz);
-        a.Vx[p] = vx;
-        const float dtRz = a.dt * a.Rz[p];
-        float vz = a.Vz[p];
-        term(vz, dtRz, dSxz_x, px, tb, 0, a.psi[RSG_E2D_SXZX_LEFT], a.psi[RSG_E2D_SXZX_RIGHT], ix);
-        term(vz, dtRz, dSzz, pz, tb, 1, a.psi[RSG_E2D_SZZ_TOP], a.psi[RSG_E2D_SZZ_BOTTOM], iz);
-        a.Vz[p] = vz;
+        const float dSzz = ddfw<ORDER>(sZZ + (zl + H) * LDZZ + tx, LDZZ, a.cz.c, kfw(z, nz, ORDER),
+                                       a.invdz);
+        const size_t p = (size_t)z * nx + x;
+        const PmlPos pz = pml_pos(z, nz, a.lpml);
+        const size_t ix = (size_t)z * a.lpml + px.i, iz = (size_t)pz.i * nx + x;
+        const float dtRx = a.dt * rx[r];
+        term(vx[r], dtRx, dSxx, px, tb, 1, a.psi[RSG_E2D_SXX_LEFT], a.psi[RSG_E2D_SXX_RIGHT], ix);
+        term(vx[r], dtRx, dSxz_z, pz, tb, 0, a.psi[RSG_E2D_SXZZ_TOP], a.psi[RSG_E2D_SXZZ_BOTTOM], iz);
+        Vx[p] = vx[r];
+        const float dtRz = a.dt * rz[r];
+        term(vz[r], dtRz, dSxz_x, px, tb, 0, a.psi[RSG_E2D_SXZX_LEFT], a.psi[RSG_E2D_SXZX_RIGHT], ix);
+        term(vz[r], dtRz, dSzz, pz, tb, 1, a.psi[RSG_E2D_SZZ_TOP], a.psi[RSG_E2D_SZZ_BOTTOM], iz);
+        Vz[p] = vz[r];
     }
 }
 
@@ -266,70 +329,89 @@ __global__ __launch_bounds__(BX *BZ) void e2d_stress_kernel(const K2D a) {
     constexpr int H = ORDER, TZ = BZ * RZ, LD = BX + 2 * H;
     __shared__ float sVx[(TZ + 2 * H) * LD];
     __shared__ float sVz[(TZ + 2 * H) * LD];
-    const int x0 = blockIdx.x * BX, z0 = blockIdx.y * TZ;
-    load_tile<BX, TZ, H, H>(sVx, a.Vx, a.nx, a.nz, x0, z0);
-    load_tile<BX, TZ, H, H>(sVz, a.Vz, a.nx, a.nz, x0, z0);
+    int x0, z0;
+    if (!tile_of(a, TZ, x0, z0)) return;
+    const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
+    const bool xin = x < nx;
+    float *__restrict__ Sxx = a.Sxx;
+    float *__restrict__ Szz = a.Szz;
+    float *__restrict__ Sxz = a.Sxz;
+    float sxx[RZ], szz[RZ], sxz[RZ], ml[RZ], ml2m[RZ], mm[RZ];
+#pragma unroll
+    for (int r = 0; r < RZ; r++) {
+        const int z = z0 + ty * RZ + r;
+        const size_t p = (size_t)z * nx + x;
+        const bool in = xin && z < nz;
+        sxx[r] = in ? Sxx[p] : 0.f;
+        szz[r] = in ? Szz[p] : 0.f;
+        sxz[r] = in ? Sxz[p] : 0.f;
+        if (!BW) {
+            ml[r] = in ? a.L[p] : 0.f;
+            ml2m[r] = in ? a.L2M[p] : 0.f;
+            mm[r] = in ? a.M[p] : 0.f;
+        }
+    }
+    load_tile<TZ, H, H>(sVx, a.Vx, nx, nz, x0, z0);
+    load_tile<TZ, H, H>(sVz, a.Vz, nx, nz, x0, z0);
     __syncthreads();
-    const int tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    if (x >= a.nx) return;
+    if (!xin) return;
     const Tab tb{a.tab, a.lpml};
-    const PmlPos px = pml_pos(x, a.nx, a.lpml);
-    const int kxf = kfw(x, a.nx, ORDER), kxb = kbw(x, a.nx, ORDER);
+    const PmlPos px = pml_pos(x, nx, a.lpml);
+    const int kxf = kfw(x, nx, ORDER), kxb = kbw(x, nx, ORDER);
+    const float dt = a.dt;
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
         const int zl = ty * RZ + r, z = z0 + zl;
-        if (z >= a.nz) break;
+        if (z >= nz) break;
         const float *svx = sVx + (zl + H) * LD + tx + H;
         const float *svz = sVz + (zl + H) * LD + tx + H;
-        const int kzf = kfw(z, a.nz, ORDER), kzb = kbw(z, a.nz, ORDER);
+        const int kzf = kfw(z, nz, ORDER), kzb = kbw(z, nz, ORDER);
         const float dVx_x = ddbw<ORDER>(svx, 1, a.cx.c, kxb, a.invdx);
         const float dVz_z = ddbw<ORDER>(svz, LD, a.cz.c, kzb, a.invdz);
         const float dVz_x = ddfw<ORDER>(svz, 1, a.cx.c, kxf, a.invdx);
         const float dVx_z = ddfw<ORDER>(svx, LD, a.cz.c, kzf, a.invdz);
-        const size_t p = (size_t)z * a.nx + x;
-        const PmlPos pz = pml_pos(z, a.nz, a.lpml);
-        const size_t ix = (size_t)z * a.lpml + px.i, iz = (size_t)pz.i * a.nx + x;
-        float sxx = a.Sxx[p], szz = a.Szz[p], sxz = a.Sxz[p];
-        const float dt = a.dt;
+        const size_t p = (size_t)z * nx + x;
+        const PmlPos pz = pml_pos(z, nz, a.lpml);
+        const size_t ix = (size_t)z * a.lpml + px.i, iz = (size_t)pz.i * nx + x;
         if (!BW) {
-            const float dtL2M = dt * a.L2M[p], dtL = dt * a.L[p];
+            const float dtL2M = dt * ml2m[r], dtL = dt * ml[r];
             // d/dx Vx feeds Sxx (L2M) and Szz (L) through one memory variable
-            sxx += dtL2M * dVx_x;
-            szz += dtL * dVx_x;
+            sxx[r] += dtL2M * dVx_x;
+            szz[r] += dtL * dVx_x;
             if (px.side >= 0) {
                 float *psi = a.psi[px.side ? RSG_E2D_VXX_RIGHT : RSG_E2D_VXX_LEFT] + ix;
                 const float A = tb.A(px.side, 0, px.i), B = tb.B(px.side, 0, px.i),
                             C = tb.C(px.side, 0, px.i);
                 const float q = B * (*psi) + A * dVx_x;
                 *psi = q;
-                sxx -= dtL2M * (q + C * dVx_x);
-                szz -= dtL * (q + C * dVx_x);
+                sxx[r] -= dtL2M * (q + C * dVx_x);
+                szz[r] -= dtL * (q + C * dVx_x);
             }
-            sxx += dtL * dVz_z;
-            szz += dtL2M * dVz_z;
+            sxx[r] += dtL * dVz_z;
+            szz[r] += dtL2M * dVz_z;
             if (pz.side >= 0) {
                 float *psi = a.psi[pz.side ? RSG_E2D_VZZ_BOTTOM : RSG_E2D_VZZ_TOP] + iz;
                 const float A = tb.A(pz.side, 0, pz.i), B = tb.B(pz.side, 0, pz.i),
                             C = tb.C(pz.side, 0, pz.i);
                 const float q = B * (*psi) + A * dVz_z;
                 *psi = q;
-                sxx -= dtL * (q + C * dVz_z);
-                szz -= dtL2M * (q + C * dVz_z);
+                sxx[r] -= dtL * (q + C * dVz_z);
+                szz[r] -= dtL2M * (q + C * dVz_z);
             }
-            if (a.fs && z == a.lpml) szz = 0.f;
-            const float dtM = dt * a.M[p];
-            term(sxz, dtM, dVz_x, px, tb, 1, a.psi[RSG_E2D_VZX_LEFT], a.psi[RSG_E2D_VZX_RIGHT], ix);
-            term(sxz, dtM, dVx_z, pz, tb, 1, a.psi[RSG_E2D_VXZ_TOP], a.psi[RSG_E2D_VXZ_BOTTOM], iz);
+            if (a.fs && z == a.lpml) szz[r] = 0.f;
+            const float dtM = dt * mm[r];
+            term(sxz[r], dtM, dVz_x, px, tb, 1, a.psi[RSG_E2D_VZX_LEFT], a.psi[RSG_E2D_VZX_RIGHT], ix);
+            term(sxz[r], dtM, dVx_z, pz, tb, 1, a.psi[RSG_E2D_VXZ_TOP], a.psi[RSG_E2D_VXZ_BOTTOM], iz);
         } else {
-            term(sxx, dt, dVx_x, px, tb, 0, a.psi[RSG_E2D_VXX_LEFT], a.psi[RSG_E2D_VXX_RIGHT], ix);
-            term(szz, dt, dVz_z, pz, tb, 0, a.psi[RSG_E2D_VZZ_TOP], a.psi[RSG_E2D_VZZ_BOTTOM], iz);
-            if (a.fs && z == a.lpml) szz = 0.f;
-            term(sxz, dt, dVz_x, px, tb, 1, a.psi[RSG_E2D_VZX_LEFT], a.psi[RSG_E2D_VZX_RIGHT], ix);
-            term(sxz, dt, dVx_z, pz, tb, 1, a.psi[RSG_E2D_VXZ_TOP], a.psi[RSG_E2D_VXZ_BOTTOM], iz);
+            term(sxx[r], dt, dVx_x, px, tb, 0, a.psi[RSG_E2D_VXX_LEFT], a.psi[RSG_E2D_VXX_RIGHT], ix);
+            term(szz[r], dt, dVz_z, pz, tb, 0, a.psi[RSG_E2D_VZZ_TOP], a.psi[RSG_E2D_VZZ_BOTTOM], iz);
+            if (a.fs && z == a.lpml) szz[r] = 0.f;
+            term(sxz[r], dt, dVz_x, px, tb, 1, a.psi[RSG_E2D_VZX_LEFT], a.psi[RSG_E2D_VZX_RIGHT], ix);
+            term(sxz[r], dt, dVx_z, pz, tb, 1, a.psi[RSG_E2D_VXZ_TOP], a.psi[RSG_E2D_VXZ_BOTTOM], iz);
         }
-        a.Sxx[p] = sxx;
-        a.Szz[p] = szz;
-        a.Sxz[p] = sxz;
+        Sxx[p] = sxx[r];
+        Szz[p] = szz[r];
+        Sxz[p] = sxz[r];
     }
 }
 
@@ -349,12 +431,14 @@ static int check_grid2d(const rsg_grid *g, const rsg_state *s) {
     return RSG_OK;
 }
 
-static K2D make_k2d(const rsg_grid *g, const rsg_state *s) {
+static K2D make_k2d(const rsg_grid *g, const rsg_state *s, int tz) {
     K2D a;
     a.nx = g->nx;
     a.nz = g->nz;
     a.lpml = g->lpml;
     a.fs = g->fs;
+    a.ntx = (g->nx + BX - 1) / BX;
+    a.ntiles = a.ntx * ((g->nz + tz - 1) / tz);
     a.dt = g->dt;
     a.invdx = 1.0f / g->dx;
     a.invdz = 1.0f / g->dz;
@@ -376,32 +460,53 @@ static K2D make_k2d(const rsg_grid *g, const rsg_state *s) {
     return a;
 }
 
-#define RSG_ORDER_CASES(LAUNCH) \
-    switch (g->order) {          \
-    case 1: LAUNCH(1); break;    \
-    case 2: LAUNCH(2); break;    \
-    case 3: LAUNCH(3); break;    \
-    case 4: LAUNCH(4); break;    \
-    case 5: LAUNCH(5); break;    \
-    case 6: LAUNCH(6); break;    \
-    case 7: LAUNCH(7); break;    \
-    default: LAUNCH(8); break;   \
+// Rows per thread from the tuning table (2, 4 or 8).
+static int rz2d() {
+    const int v = tuning().rz2d;
+    return v <= 2 ? 2 : (v <= 4 ? 4 : 8);
+}
+
+static unsigned nblocks(const K2D &a) { return 8u * (unsigned)((a.ntiles + 7) / 8); }
+
+#define RSG_ORDER_CASES(LAUNCH, RZ) \
+    switch (g->order) {              \
+    case 1: LAUNCH(1, RZ); break;    \
+    case 2: LAUNCH(2, RZ); break;    \
+    case 3: LAUNCH(3, RZ); break;    \
+    case 4: LAUNCH(4, RZ); break;    \
+    case 5: LAUNCH(5, RZ); break;    \
+    case 6: LAUNCH(6, RZ); break;    \
+    case 7: LAUNCH(7, RZ); break;    \
+    default: LAUNCH(8, RZ); break;   \
     }
 
-static dim3 grid2d(const rsg_grid *g) {
-    return dim3((g->nx + BX - 1) / BX, (g->nz + BZ * RZ2D - 1) / (BZ * RZ2D), 1);
-}
+#define RSG_DISPATCH(LAUNCH)                      \
+    do {                                          \
+        const int rz_ = rz2d();                   \
+        if (rz_ == 2) {                           \
+            RSG_ORDER_CASES(LAUNCH, 2)            \
+        } else if (rz_ == 4) {                    \
+            RSG_ORDER_CASES(LAUNCH, 4)            \
+        } else {                                  \
+            RSG_ORDER_CASES(LAUNCH, 8)            \
+        }                                         \
+    } while (0)
 
 }  // namespace rsg
 
 using namespace rsg;
 
+#define GO(KERNEL, O, RZ)                                                                         \
+    do {                                                                                          \
+        const K2D a = make_k2d(g, s, BZ * RZ);                                                    \
+        KERNEL<<<nblocks(a), dim3(BX, BZ), 0, (hipStream_t)stream>>>(a);                          \
+    } while (0)
+
 extern "C" int rsg_a2d_forward_velocity(const rsg_grid *g, const rsg_state *s, void *stream) {
     if (int rc = check_grid2d(g, s)) return rc;
     if (!s->P || !s->Vx || !s->Vz || !s->Rx || !s->Rz) return fail(RSG_EINVAL, "a2d: null array");
-    const K2D a = make_k2d(g, s);
-#define L_(O) a2d_velocity_kernel<O, RZ2D><<<grid2d(g), dim3(BX, BZ), 0, (hipStream_t)stream>>>(a)
-    RSG_ORDER_CASES(L_)
+#define L_(O, RZ) GO((a2d_velocity_kernel<O, RZ>), O, RZ)
+    RSG_DISPATCH(L_);
 #undef L_
     RSG_LAUNCH_CHECK("a2d_velocity_kernel");
     return RSG_OK;
@@ -410,9 +515,8 @@ extern "C" int rsg_a2d_forward_velocity(const rsg_grid *g, const rsg_state *s, v
 extern "C" int rsg_a2d_forward_stress(const rsg_grid *g, const rsg_state *s, void *stream) {
     if (int rc = check_grid2d(g, s)) return rc;
     if (!s->P || !s->Vx || !s->Vz || !s->L) return fail(RSG_EINVAL, "a2d: null array");
-    const K2D a = make_k2d(g, s);
-#define L_(O) a2d_stress_kernel<O, RZ2D><<<grid2d(g), dim3(BX, BZ), 0, (hipStream_t)stream>>>(a)
-    RSG_ORDER_CASES(L_)
+#define L_(O, RZ) GO((a2d_stress_kernel<O, RZ>), O, RZ)
+    RSG_DISPATCH(L_);
 #undef L_
     RSG_LAUNCH_CHECK("a2d_stress_kernel");
     return RSG_OK;
@@ -428,10 +532,8 @@ static int e2d_check(const rsg_grid *g, const rsg_state *s) {
 
 extern "C" int rsg_e2d_forward_velocity(const rsg_grid *g, const rsg_state *s, void *stream) {
     if (int rc = e2d_check(g, s)) return rc;
-    const K2D a = make_k2d(g, s);
-#define L_(O) \
-    e2d_velocity_kernel<O, RZ2D, false><<<grid2d(g), dim3(BX, BZ), 0, (hipStream_t)stream>>>(a)
-    RSG_ORDER_CASES(L_)
+#define L_(O, RZ) GO((e2d_velocity_kernel<O, RZ, false>), O, RZ)
+    RSG_DISPATCH(L_);
 #undef L_
     RSG_LAUNCH_CHECK("e2d_velocity_kernel");
     return RSG_OK;
@@ -439,10 +541,8 @@ extern "C" int rsg_e2d_forward_velocity(const rsg_grid *g, const rsg_state *s, v
 
 extern "C" int rsg_e2d_forward_stress(const rsg_grid *g, const rsg_state *s, void *stream) {
     if (int rc = e2d_check(g, s)) return rc;
-    const K2D a = make_k2d(g, s);
-#define L_(O) \
-    e2d_stress_kernel<O, RZ2D, false><<<grid2d(g), dim3(BX, BZ), 0, (hipStream_t)stream>>>(a)
-    RSG_ORDER_CASES(L_)
+#define L_(O, RZ) GO((e2d_stress_kernel<O, RZ, false>), O, RZ)
+    RSG_DISPATCH(L_);
 #undef L_
     RSG_LAUNCH_CHECK("e2d_stress_kernel");
     return RSG_OK;
@@ -450,10 +550,8 @@ extern "C" int rsg_e2d_forward_stress(const rsg_grid *g, const rsg_state *s, voi
 
 extern "C" int rsg_e2d_backward_velocity(const rsg_grid *g, const rsg_state *s, void *stream) {
     if (int rc = e2d_check(g, s)) return rc;
-    const K2D a = make_k2d(g, s);
-#define L_(O) \
-    e2d_velocity_kernel<O, RZ2D, true><<<grid2d(g), dim3(BX, BZ), 0, (hipStream_t)stream>>>(a)
-    RSG_ORDER_CASES(L_)
+#define L_(O, RZ) GO((e2d_velocity_kernel<O, RZ, true>), O, RZ)
+    RSG_DISPATCH(L_);
 #undef L_
     RSG_LAUNCH_CHECK("e2d_velocity_kernel(bw)");
     return RSG_OK;
@@ -461,10 +559,8 @@ extern "C" int rsg_e2d_backward_velocity(const rsg_grid *g, const rsg_state *s, 
 
 extern "C" int rsg_e2d_backward_stress(const rsg_grid *g, const rsg_state *s, void *stream) {
     if (int rc = e2d_check(g, s)) return rc;
-    const K2D a = make_k2d(g, s);
-#define L_(O) \
-    e2d_stress_kernel<O, RZ2D, true><<<grid2d(g), dim3(BX, BZ), 0, (hipStream_t)stream>>>(a)
-    RSG_ORDER_CASES(L_)
+#define L_(O, RZ) GO((e2d_stress_kernel<O, RZ, true>), O, RZ)
+    RSG_DISPATCH(L_);
 #undef L_
     RSG_LAUNCH_CHECK("e2d_stress_kernel(bw)");
     return RSG_OK;

@@ -258,6 +258,7 @@ static K3D make_k3d(const rsg_grid *g, const rsg_state *s) {
 // z-chunk length: enough blocks to fill 256 CUs several times over while
 // keeping the queue warm-up (2*ORDER planes) small against the chunk.
 static int zchunk(const rsg_grid *g) {
+    if (tuning().zc3d > 0) return tuning().zc3d;
     const long cols = (long)((g->nx + B3X - 1) / B3X) * ((g->ny + T3Y - 1) / T3Y);
     int zc = 64;
     while (zc > 16 && cols * ((g->nz + zc - 1) / zc) < 2048) zc /= 2;

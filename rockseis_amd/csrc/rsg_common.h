@@ -111,6 +111,13 @@ __device__ __forceinline__ void pml_corr(float &F, float dtM, float *psi, float 
 // PML table rows (pml.h:60-74)
 enum { T_A = 0, T_B = 1, T_C = 2, T_STAG = 3, T_RBB = 6 };
 
+// Launch-shape knobs (set through rsg_set_tuning; picked by the autotuner).
+struct Tuning {
+    int rz2d = 4;   // rows per thread of the 2D tiles: 2, 4 or 8
+    int zc3d = 0;   // z planes per 3D block (0: automatic)
+};
+Tuning &tuning();
+
 // Grid-stride fill used instead of hipMemsetAsync (graph-capture safe).
 __global__ void fill_zero_kernel(float *p, size_t n);
 

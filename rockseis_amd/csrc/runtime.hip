@@ -21,6 +21,11 @@ int check_hip(hipError_t e, const char *what) {
     return RSG_EHIP;
 }
 
+Tuning &tuning() {
+    static Tuning t;
+    return t;
+}
+
 Coefs scaled_coefs(float d) {
     static const float table[36] = {RSG_DER_TABLE};
     Coefs c;
@@ -120,6 +125,33 @@ int rsg_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
 int rsg_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream) {
     if (!bytes) return RSG_OK;
     RSG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return RSG_OK;
+}
+
+int rsg_set_tuning(const char *key, int value) {
+    if (!key) return fail(RSG_EINVAL, "null key");
+    std::string k(key);
+    if (k == "rz2d") {
+        if (value != 2 && value != 4 && value != 8) return fail(RSG_EINVAL, "rz2d must be 2, 4 or 8");
+        tuning().rz2d = value;
+    } else if (k == "zc3d") {
+        if (value < 0 || value > 4096) return fail(RSG_EINVAL, "zc3d must be 0..4096");
+        tuning().zc3d = value;
+    } else {
+        return fail(RSG_EINVAL, "unknown tuning key " + k);
+    }
+    return RSG_OK;
+}
+
+int rsg_get_tuning(const char *key, int *value) {
+    if (!key || !value) return fail(RSG_EINVAL, "null argument");
+    std::string k(key);
+    if (k == "rz2d")
+        *value = tuning().rz2d;
+    else if (k == "zc3d")
+        *value = tuning().zc3d;
+    else
+        return fail(RSG_EINVAL, "unknown tuning key " + k);
     return RSG_OK;
 }
 
