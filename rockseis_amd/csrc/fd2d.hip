@@ -48,28 +48,37 @@ __device__ __forceinline__ bool tile_of(const K2D &a, int TZ, int &x0, int &z0) 
     return true;
 }
 
+// A tile is "interior" when it holds no PML cell and no free-surface row and
+// its whole halo lies inside the grid: such tiles (most of the grid) take a
+// straight-line path with no bounds, edge-stencil or strip tests.
+__device__ __forceinline__ bool interior_tile(const K2D &a, int x0, int z0, int TZ, int H) {
+    const int lo = a.lpml > H ? a.lpml : H;
+    return x0 >= lo && x0 + BX + H <= a.nx - a.lpml && z0 >= lo && z0 + TZ + H <= a.nz - a.lpml &&
+           !(a.fs && z0 <= a.lpml);
+}
+
 // Stage rows [z0-HZ, z0+TZ+HZ) x cols [x0-HX, x0+BX+HX) of g into LDS,
 // zero outside the grid (those entries are never read by the edge stencils).
-template <int TZ, int HX, int HZ>
+template <int TZ, int HX, int HZ, bool FAST>
 __device__ __forceinline__ void load_tile(float *lds, const float *__restrict__ g, int nx, int nz,
                                           int x0, int z0) {
     constexpr int LD = BX + 2 * HX, ROWS = TZ + 2 * HZ;
 #pragma unroll
     for (int r = threadIdx.y; r < ROWS; r += BZ) {
         const int z = z0 - HZ + r;
-        const bool zin = z >= 0 && z < nz;
-        const float *row = g + (ptrdiff_t)z * nx;
+        const bool zin = FAST || (z >= 0 && z < nz);
+        const float *row = g + (ptrdiff_t)z * nx + (x0 - HX);
 #pragma unroll
         for (int c = threadIdx.x; c < LD; c += BX) {
             const int x = x0 - HX + c;
-            lds[r * LD + c] = (zin && x >= 0 && x < nx) ? row[x] : 0.f;
+            lds[r * LD + c] = (FAST || (zin && x >= 0 && x < nx)) ? row[c] : 0.f;
         }
     }
 }
 
 // Same, staging a*b + c*d (c null: a*b) -- the moduli-weighted stresses of
 // WavesElastic2D::backwardstepVelocity (waves.cpp:1771-1775).
-template <int TZ, int HX, int HZ>
+template <int TZ, int HX, int HZ, bool FAST>
 __device__ __forceinline__ void load_tile_mix(float *lds, const float *__restrict__ a,
                                               const float *__restrict__ b,
                                               const float *__restrict__ c,
@@ -79,12 +88,12 @@ __device__ __forceinline__ void load_tile_mix(float *lds, const float *__restric
 #pragma unroll
     for (int r = threadIdx.y; r < ROWS; r += BZ) {
         const int z = z0 - HZ + r;
-        const bool zin = z >= 0 && z < nz;
+        const bool zin = FAST || (z >= 0 && z < nz);
 #pragma unroll
         for (int cc = threadIdx.x; cc < LD; cc += BX) {
             const int x = x0 - HX + cc;
             float v = 0.f;
-            if (zin && x >= 0 && x < nx) {
+            if (FAST || (zin && x >= 0 && x < nx)) {
                 const size_t p = (size_t)z * nx + x;
                 v = c ? a[p] * b[p] + c[p] * d[p] : a[p] * b[p];
             }
@@ -95,16 +104,16 @@ __device__ __forceinline__ void load_tile_mix(float *lds, const float *__restric
 
 // Derivative of an LDS tile at a cell: compile-time order in the interior,
 // reduced order where the reference's Der shortens the stencil.
-template <int ORDER>
+template <int ORDER, bool FAST>
 __device__ __forceinline__ float ddfw(const float *s, int st, const float *c, int k, float invd) {
     float d = dfw<ORDER>(s, st, c + cind(ORDER));
-    if (k != ORDER) d = dfw_k(s, st, k, invd);
+    if (!FAST && k != ORDER) d = dfw_k(s, st, k, invd);
     return d;
 }
-template <int ORDER>
+template <int ORDER, bool FAST>
 __device__ __forceinline__ float ddbw(const float *s, int st, const float *c, int k, float invd) {
     float d = dbw<ORDER>(s, st, c + cind(ORDER));
-    if (k != ORDER) d = dbw_k(s, st, k, invd);
+    if (!FAST && k != ORDER) d = dbw_k(s, st, k, invd);
     return d;
 }
 
@@ -141,10 +150,11 @@ __device__ __forceinline__ PmlPos pml_pos(int x, int n, int L) {
 }
 
 // Apply dt*M*d then the strip correction of one derivative term.
+template <bool FAST>
 __device__ __forceinline__ void term(float &F, float dtM, float d, const PmlPos &pp, const Tab &tb,
                                      int stag, float *lo, float *hi, size_t sidx) {
     F += dtM * d;
-    if (pp.side >= 0) {
+    if (!FAST && pp.side >= 0) {
         float *psi = (pp.side ? hi : lo) + sidx;
         pml_corr(F, dtM, psi, tb.A(pp.side, stag, pp.i), tb.B(pp.side, stag, pp.i),
                  tb.C(pp.side, stag, pp.i), d);
@@ -155,14 +165,13 @@ __device__ __forceinline__ void term(float &F, float dtM, float d, const PmlPos 
 // Acoustic 2D
 // ---------------------------------------------------------------------------
 
-template <int ORDER, int RZ>
-__global__ __launch_bounds__(BX *BZ) void a2d_velocity_kernel(const K2D a) {
+// Kernel bodies take the LDS buffers from the kernel so the interior (FAST)
+// and boundary instantiations share one allocation.
+template <int ORDER, int RZ, bool FAST>
+__device__ __forceinline__ void a2d_velocity_body(const K2D &a, int x0, int z0, float *sP) {
     constexpr int H = ORDER, TZ = BZ * RZ, LD = BX + 2 * H;
-    __shared__ float sP[(TZ + 2 * H) * LD];
-    int x0, z0;
-    if (!tile_of(a, TZ, x0, z0)) return;
     const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    const bool xin = x < nx;
+    const bool xin = FAST || x < nx;
     float *__restrict__ Vx = a.Vx;
     float *__restrict__ Vz = a.Vz;
     const float *__restrict__ Rx = a.Rx;
@@ -172,33 +181,85 @@ __global__ __launch_bounds__(BX *BZ) void a2d_velocity_kernel(const K2D a) {
     for (int r = 0; r < RZ; r++) {
         const int z = z0 + ty * RZ + r;
         const size_t p = (size_t)z * nx + x;
-        const bool in = xin && z < nz;
+        const bool in = FAST || (xin && z < nz);
         vx[r] = in ? Vx[p] : 0.f;
         vz[r] = in ? Vz[p] : 0.f;
         rx[r] = in ? Rx[p] : 0.f;
         rz[r] = in ? Rz[p] : 0.f;
     }
-    load_tile<TZ, H, H>(sP, a.P, nx, nz, x0, z0);
+    load_tile<TZ, H, H, FAST>(sP, a.P, nx, nz, x0, z0);
     __syncthreads();
     if (!xin) return;
     const Tab tb{a.tab, a.lpml};
-    const PmlPos px = pml_pos(x, nx, a.lpml);
-    const int kx = kfw(x, nx, ORDER);
+    const PmlPos px = FAST ? PmlPos{-1, 0} : pml_pos(x, nx, a.lpml);
+    const int kx = FAST ? ORDER : kfw(x, nx, ORDER);
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
         const int zl = ty * RZ + r, z = z0 + zl;
-        if (z >= nz) break;
+        if (!FAST && z >= nz) break;
         const float *s = sP + (zl + H) * LD + tx + H;
-        const float dPx = ddfw<ORDER>(s, 1, a.cx.c, kx, a.invdx);
-        const float dPz = ddfw<ORDER>(s, LD, a.cz.c, kfw(z, nz, ORDER), a.invdz);
+        const float dPx = ddfw<ORDER, FAST>(s, 1, a.cx.c, kx, a.invdx);
+        const float dPz = ddfw<ORDER, FAST>(s, LD, a.cz.c, FAST ? ORDER : kfw(z, nz, ORDER), a.invdz);
         const size_t p = (size_t)z * nx + x;
-        const PmlPos pz = pml_pos(z, nz, a.lpml);
-        term(vx[r], a.dt * rx[r], dPx, px, tb, 1, a.psi[RSG_A2D_P_LEFT], a.psi[RSG_A2D_P_RIGHT],
-             (size_t)z * a.lpml + px.i);
-        term(vz[r], a.dt * rz[r], dPz, pz, tb, 1, a.psi[RSG_A2D_P_TOP], a.psi[RSG_A2D_P_BOTTOM],
-             (size_t)pz.i * nx + x);
+        const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
+        term<FAST>(vx[r], a.dt * rx[r], dPx, px, tb, 1, a.psi[RSG_A2D_P_LEFT],
+                   a.psi[RSG_A2D_P_RIGHT], (size_t)z * a.lpml + px.i);
+        term<FAST>(vz[r], a.dt * rz[r], dPz, pz, tb, 1, a.psi[RSG_A2D_P_TOP],
+                   a.psi[RSG_A2D_P_BOTTOM], (size_t)pz.i * nx + x);
         Vx[p] = vx[r];
         Vz[p] = vz[r];
+    }
+}
+
+template <int ORDER, int RZ>
+__global__ __launch_bounds__(BX *BZ) void a2d_velocity_kernel(const K2D a) {
+    constexpr int H = ORDER, TZ = BZ * RZ, LD = BX + 2 * H;
+    __shared__ float sP[(TZ + 2 * H) * LD];
+    int x0, z0;
+    if (!tile_of(a, TZ, x0, z0)) return;
+    if (interior_tile(a, x0, z0, TZ, H))
+        a2d_velocity_body<ORDER, RZ, true>(a, x0, z0, sP);
+    else
+        a2d_velocity_body<ORDER, RZ, false>(a, x0, z0, sP);
+}
+
+template <int ORDER, int RZ, bool FAST>
+__device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, float *sVx, float *sVz) {
+    constexpr int H = ORDER, LDX = BX + 2 * H, LDZ = BX, TZ = BZ * RZ;
+    const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
+    const bool xin = FAST || x < nx;
+    float *__restrict__ P = a.P;
+    const float *__restrict__ L = a.L;
+    float pv[RZ], lv[RZ];
+#pragma unroll
+    for (int r = 0; r < RZ; r++) {
+        const int z = z0 + ty * RZ + r;
+        const size_t p = (size_t)z * nx + x;
+        const bool in = FAST || (xin && z < nz);
+        pv[r] = in ? P[p] : 0.f;
+        lv[r] = in ? L[p] : 0.f;
+    }
+    load_tile<TZ, H, 0, FAST>(sVx, a.Vx, nx, nz, x0, z0);
+    load_tile<TZ, 0, H, FAST>(sVz, a.Vz, nx, nz, x0, z0);
+    __syncthreads();
+    if (!xin) return;
+    const Tab tb{a.tab, a.lpml};
+    const PmlPos px = FAST ? PmlPos{-1, 0} : pml_pos(x, nx, a.lpml);
+    const int kx = FAST ? ORDER : kbw(x, nx, ORDER);
+#pragma unroll
+    for (int r = 0; r < RZ; r++) {
+        const int zl = ty * RZ + r, z = z0 + zl;
+        if (!FAST && z >= nz) break;
+        const float dVx = ddbw<ORDER, FAST>(sVx + zl * LDX + tx + H, 1, a.cx.c, kx, a.invdx);
+        const float dVz = ddbw<ORDER, FAST>(sVz + (zl + H) * LDZ + tx, LDZ, a.cz.c,
+                                            FAST ? ORDER : kbw(z, nz, ORDER), a.invdz);
+        const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
+        const float dtL = a.dt * lv[r];
+        term<FAST>(pv[r], dtL, dVx, px, tb, 0, a.psi[RSG_A2D_VXX_LEFT], a.psi[RSG_A2D_VXX_RIGHT],
+                   (size_t)z * a.lpml + px.i);
+        term<FAST>(pv[r], dtL, dVz, pz, tb, 0, a.psi[RSG_A2D_VZZ_TOP], a.psi[RSG_A2D_VZZ_BOTTOM],
+                   (size_t)pz.i * nx + x);
+        P[(size_t)z * nx + x] = pv[r];
     }
 }
 
@@ -209,41 +270,10 @@ __global__ __launch_bounds__(BX *BZ) void a2d_stress_kernel(const K2D a) {
     __shared__ float sVz[(TZ + 2 * H) * LDZ];
     int x0, z0;
     if (!tile_of(a, TZ, x0, z0)) return;
-    const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    const bool xin = x < nx;
-    float *__restrict__ P = a.P;
-    const float *__restrict__ L = a.L;
-    float pv[RZ], lv[RZ];
-#pragma unroll
-    for (int r = 0; r < RZ; r++) {
-        const int z = z0 + ty * RZ + r;
-        const size_t p = (size_t)z * nx + x;
-        const bool in = xin && z < nz;
-        pv[r] = in ? P[p] : 0.f;
-        lv[r] = in ? L[p] : 0.f;
-    }
-    load_tile<TZ, H, 0>(sVx, a.Vx, nx, nz, x0, z0);
-    load_tile<TZ, 0, H>(sVz, a.Vz, nx, nz, x0, z0);
-    __syncthreads();
-    if (!xin) return;
-    const Tab tb{a.tab, a.lpml};
-    const PmlPos px = pml_pos(x, nx, a.lpml);
-    const int kx = kbw(x, nx, ORDER);
-#pragma unroll
-    for (int r = 0; r < RZ; r++) {
-        const int zl = ty * RZ + r, z = z0 + zl;
-        if (z >= nz) break;
-        const float dVx = ddbw<ORDER>(sVx + zl * LDX + tx + H, 1, a.cx.c, kx, a.invdx);
-        const float dVz = ddbw<ORDER>(sVz + (zl + H) * LDZ + tx, LDZ, a.cz.c, kbw(z, nz, ORDER),
-                                      a.invdz);
-        const PmlPos pz = pml_pos(z, nz, a.lpml);
-        const float dtL = a.dt * lv[r];
-        term(pv[r], dtL, dVx, px, tb, 0, a.psi[RSG_A2D_VXX_LEFT], a.psi[RSG_A2D_VXX_RIGHT],
-             (size_t)z * a.lpml + px.i);
-        term(pv[r], dtL, dVz, pz, tb, 0, a.psi[RSG_A2D_VZZ_TOP], a.psi[RSG_A2D_VZZ_BOTTOM],
-             (size_t)pz.i * nx + x);
-        P[(size_t)z * nx + x] = pv[r];
-    }
+    if (interior_tile(a, x0, z0, TZ, H))
+        a2d_stress_body<ORDER, RZ, true>(a, x0, z0, sVx, sVz);
+    else
+        a2d_stress_body<ORDER, RZ, false>(a, x0, z0, sVx, sVz);
 }
 
 // ---------------------------------------------------------------------------
@@ -255,17 +285,13 @@ __global__ __launch_bounds__(BX *BZ) void a2d_stress_kernel(const K2D a) {
 //   Vx += dt*Rx*(ddx_fw(Wxx) + ddz_bw(Wxz)),  Vz += dt*Rz*(ddx_bw(Wxz) + ddz_fw(Wzz))
 // with Wxx=Sxx, Wxz=Sxz, Wzz=Szz (forward) or Wxx=L2M*Sxx+L*Szz, Wxz=M*Sxz,
 // Wzz=L*Sxx+L2M*Szz (adjoint state, waves.cpp:1771-1878).
-template <int ORDER, int RZ, bool BW>
-__global__ __launch_bounds__(BX *BZ) void e2d_velocity_kernel(const K2D a) {
+template <int ORDER, int RZ, bool BW, bool FAST>
+__device__ __forceinline__ void e2d_velocity_body(const K2D &a, int x0, int z0, float *sXX,
+                                                  float *sZZ, float *sXZ) {
     constexpr int H = ORDER, TZ = BZ * RZ;
     constexpr int LDXX = BX + 2 * H, LDZZ = BX, LDXZ = BX + 2 * H;
-    __shared__ float sXX[TZ * LDXX];
-    __shared__ float sZZ[(TZ + 2 * H) * LDZZ];
-    __shared__ float sXZ[(TZ + 2 * H) * LDXZ];
-    int x0, z0;
-    if (!tile_of(a, TZ, x0, z0)) return;
     const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    const bool xin = x < nx;
+    const bool xin = FAST || x < nx;
     float *__restrict__ Vx = a.Vx;
     float *__restrict__ Vz = a.Vz;
     const float *__restrict__ Rx = a.Rx;
@@ -275,48 +301,65 @@ __global__ __launch_bounds__(BX *BZ) void e2d_velocity_kernel(const K2D a) {
     for (int r = 0; r < RZ; r++) {
         const int z = z0 + ty * RZ + r;
         const size_t p = (size_t)z * nx + x;
-        const bool in = xin && z < nz;
+        const bool in = FAST || (xin && z < nz);
         vx[r] = in ? Vx[p] : 0.f;
         vz[r] = in ? Vz[p] : 0.f;
         rx[r] = in ? Rx[p] : 0.f;
         rz[r] = in ? Rz[p] : 0.f;
     }
     if (!BW) {
-        load_tile<TZ, H, 0>(sXX, a.Sxx, nx, nz, x0, z0);
-        load_tile<TZ, 0, H>(sZZ, a.Szz, nx, nz, x0, z0);
-        load_tile<TZ, H, H>(sXZ, a.Sxz, nx, nz, x0, z0);
+        load_tile<TZ, H, 0, FAST>(sXX, a.Sxx, nx, nz, x0, z0);
+        load_tile<TZ, 0, H, FAST>(sZZ, a.Szz, nx, nz, x0, z0);
+        load_tile<TZ, H, H, FAST>(sXZ, a.Sxz, nx, nz, x0, z0);
     } else {
-        load_tile_mix<TZ, H, 0>(sXX, a.L2M, a.Sxx, a.L, a.Szz, nx, nz, x0, z0);
-        load_tile_mix<TZ, 0, H>(sZZ, a.L, a.Sxx, a.L2M, a.Szz, nx, nz, x0, z0);
-        load_tile_mix<TZ, H, H>(sXZ, a.M, a.Sxz, nullptr, nullptr, nx, nz, x0, z0);
+        load_tile_mix<TZ, H, 0, FAST>(sXX, a.L2M, a.Sxx, a.L, a.Szz, nx, nz, x0, z0);
+        load_tile_mix<TZ, 0, H, FAST>(sZZ, a.L, a.Sxx, a.L2M, a.Szz, nx, nz, x0, z0);
+        load_tile_mix<TZ, H, H, FAST>(sXZ, a.M, a.Sxz, nullptr, nullptr, nx, nz, x0, z0);
     }
     __syncthreads();
     if (!xin) return;
     const Tab tb{a.tab, a.lpml};
-    const PmlPos px = pml_pos(x, nx, a.lpml);
-    const int kxf = kfw(x, nx, ORDER), kxb = kbw(x, nx, ORDER);
+    const PmlPos px = FAST ? PmlPos{-1, 0} : pml_pos(x, nx, a.lpml);
+    const int kxf = FAST ? ORDER : kfw(x, nx, ORDER), kxb = FAST ? ORDER : kbw(x, nx, ORDER);
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
         const int zl = ty * RZ + r, z = z0 + zl;
-        if (z >= nz) break;
-        const float dSxx = ddfw<ORDER>(sXX + zl * LDXX + tx + H, 1, a.cx.c, kxf, a.invdx);
+        if (!FAST && z >= nz) break;
+        const float dSxx = ddfw<ORDER, FAST>(sXX + zl * LDXX + tx + H, 1, a.cx.c, kxf, a.invdx);
         const float *sxz = sXZ + (zl + H) * LDXZ + tx + H;
-        const float dSxz_z = ddbw<ORDER>(sxz, LDXZ, a.cz.c, kbw(z, nz, ORDER), a.invdz);
-        const float dSxz_x = ddbw<ORDER>(sxz, 1, a.cx.c, kxb, a.invdx);
-        const float dSzz = ddfw<ORDER>(sZZ + (zl + H) * LDZZ + tx, LDZZ, a.cz.c, kfw(z, nz, ORDER),
-                                       a.invdz);
+        const float dSxz_z = ddbw<ORDER, FAST>(sxz, LDXZ, a.cz.c, FAST ? ORDER : kbw(z, nz, ORDER),
+                                               a.invdz);
+        const float dSxz_x = ddbw<ORDER, FAST>(sxz, 1, a.cx.c, kxb, a.invdx);
+        const float dSzz = ddfw<ORDER, FAST>(sZZ + (zl + H) * LDZZ + tx, LDZZ, a.cz.c,
+                                             FAST ? ORDER : kfw(z, nz, ORDER), a.invdz);
         const size_t p = (size_t)z * nx + x;
-        const PmlPos pz = pml_pos(z, nz, a.lpml);
+        const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
         const size_t ix = (size_t)z * a.lpml + px.i, iz = (size_t)pz.i * nx + x;
         const float dtRx = a.dt * rx[r];
-        term(vx[r], dtRx, dSxx, px, tb, 1, a.psi[RSG_E2D_SXX_LEFT], a.psi[RSG_E2D_SXX_RIGHT], ix);
-        term(vx[r], dtRx, dSxz_z, pz, tb, 0, a.psi[RSG_E2D_SXZZ_TOP], a.psi[RSG_E2D_SXZZ_BOTTOM], iz);
+        term<FAST>(vx[r], dtRx, dSxx, px, tb, 1, a.psi[RSG_E2D_SXX_LEFT], a.psi[RSG_E2D_SXX_RIGHT], ix);
+        term<FAST>(vx[r], dtRx, dSxz_z, pz, tb, 0, a.psi[RSG_E2D_SXZZ_TOP],
+                   a.psi[RSG_E2D_SXZZ_BOTTOM], iz);
         Vx[p] = vx[r];
         const float dtRz = a.dt * rz[r];
-        term(vz[r], dtRz, dSxz_x, px, tb, 0, a.psi[RSG_E2D_SXZX_LEFT], a.psi[RSG_E2D_SXZX_RIGHT], ix);
-        term(vz[r], dtRz, dSzz, pz, tb, 1, a.psi[RSG_E2D_SZZ_TOP], a.psi[RSG_E2D_SZZ_BOTTOM], iz);
+        term<FAST>(vz[r], dtRz, dSxz_x, px, tb, 0, a.psi[RSG_E2D_SXZX_LEFT],
+                   a.psi[RSG_E2D_SXZX_RIGHT], ix);
+        term<FAST>(vz[r], dtRz, dSzz, pz, tb, 1, a.psi[RSG_E2D_SZZ_TOP], a.psi[RSG_E2D_SZZ_BOTTOM], iz);
         Vz[p] = vz[r];
     }
+}
+
+template <int ORDER, int RZ, bool BW>
+__global__ __launch_bounds__(BX *BZ) void e2d_velocity_kernel(const K2D a) {
+    constexpr int H = ORDER, TZ = BZ * RZ;
+    __shared__ float sXX[TZ * (BX + 2 * H)];
+    __shared__ float sZZ[(TZ + 2 * H) * BX];
+    __shared__ float sXZ[(TZ + 2 * H) * (BX + 2 * H)];
+    int x0, z0;
+    if (!tile_of(a, TZ, x0, z0)) return;
+    if (interior_tile(a, x0, z0, TZ, H))
+        e2d_velocity_body<ORDER, RZ, BW, true>(a, x0, z0, sXX, sZZ, sXZ);
+    else
+        e2d_velocity_body<ORDER, RZ, BW, false>(a, x0, z0, sXX, sZZ, sXZ);
 }
 
 // Stress update.  Forward (BW=false, waves.cpp:1571-1735):
@@ -324,15 +367,11 @@ __global__ __launch_bounds__(BX *BZ) void e2d_velocity_kernel(const K2D a) {
 //   free surface Szz(z=lpml)=0, Sxz += dt*M*(ddx_fw(Vz) + ddz_fw(Vx)).
 // Adjoint state (BW=true, waves.cpp:1907-2080): the same derivatives with
 // unit moduli, Sxx only from ddx_bw(Vx) and Szz only from ddz_bw(Vz).
-template <int ORDER, int RZ, bool BW>
-__global__ __launch_bounds__(BX *BZ) void e2d_stress_kernel(const K2D a) {
+template <int ORDER, int RZ, bool BW, bool FAST>
+__device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, float *sVx, float *sVz) {
     constexpr int H = ORDER, TZ = BZ * RZ, LD = BX + 2 * H;
-    __shared__ float sVx[(TZ + 2 * H) * LD];
-    __shared__ float sVz[(TZ + 2 * H) * LD];
-    int x0, z0;
-    if (!tile_of(a, TZ, x0, z0)) return;
     const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    const bool xin = x < nx;
+    const bool xin = FAST || x < nx;
     float *__restrict__ Sxx = a.Sxx;
     float *__restrict__ Szz = a.Szz;
     float *__restrict__ Sxz = a.Sxz;
@@ -341,7 +380,7 @@ __global__ __launch_bounds__(BX *BZ) void e2d_stress_kernel(const K2D a) {
     for (int r = 0; r < RZ; r++) {
         const int z = z0 + ty * RZ + r;
         const size_t p = (size_t)z * nx + x;
-        const bool in = xin && z < nz;
+        const bool in = FAST || (xin && z < nz);
         sxx[r] = in ? Sxx[p] : 0.f;
         szz[r] = in ? Szz[p] : 0.f;
         sxz[r] = in ? Sxz[p] : 0.f;
@@ -351,34 +390,34 @@ __global__ __launch_bounds__(BX *BZ) void e2d_stress_kernel(const K2D a) {
             mm[r] = in ? a.M[p] : 0.f;
         }
     }
-    load_tile<TZ, H, H>(sVx, a.Vx, nx, nz, x0, z0);
-    load_tile<TZ, H, H>(sVz, a.Vz, nx, nz, x0, z0);
+    load_tile<TZ, H, H, FAST>(sVx, a.Vx, nx, nz, x0, z0);
+    load_tile<TZ, H, H, FAST>(sVz, a.Vz, nx, nz, x0, z0);
     __syncthreads();
     if (!xin) return;
     const Tab tb{a.tab, a.lpml};
-    const PmlPos px = pml_pos(x, nx, a.lpml);
-    const int kxf = kfw(x, nx, ORDER), kxb = kbw(x, nx, ORDER);
+    const PmlPos px = FAST ? PmlPos{-1, 0} : pml_pos(x, nx, a.lpml);
+    const int kxf = FAST ? ORDER : kfw(x, nx, ORDER), kxb = FAST ? ORDER : kbw(x, nx, ORDER);
     const float dt = a.dt;
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
         const int zl = ty * RZ + r, z = z0 + zl;
-        if (z >= nz) break;
+        if (!FAST && z >= nz) break;
         const float *svx = sVx + (zl + H) * LD + tx + H;
         const float *svz = sVz + (zl + H) * LD + tx + H;
-        const int kzf = kfw(z, nz, ORDER), kzb = kbw(z, nz, ORDER);
-        const float dVx_x = ddbw<ORDER>(svx, 1, a.cx.c, kxb, a.invdx);
-        const float dVz_z = ddbw<ORDER>(svz, LD, a.cz.c, kzb, a.invdz);
-        const float dVz_x = ddfw<ORDER>(svz, 1, a.cx.c, kxf, a.invdx);
-        const float dVx_z = ddfw<ORDER>(svx, LD, a.cz.c, kzf, a.invdz);
+        const int kzf = FAST ? ORDER : kfw(z, nz, ORDER), kzb = FAST ? ORDER : kbw(z, nz, ORDER);
+        const float dVx_x = ddbw<ORDER, FAST>(svx, 1, a.cx.c, kxb, a.invdx);
+        const float dVz_z = ddbw<ORDER, FAST>(svz, LD, a.cz.c, kzb, a.invdz);
+        const float dVz_x = ddfw<ORDER, FAST>(svz, 1, a.cx.c, kxf, a.invdx);
+        const float dVx_z = ddfw<ORDER, FAST>(svx, LD, a.cz.c, kzf, a.invdz);
         const size_t p = (size_t)z * nx + x;
-        const PmlPos pz = pml_pos(z, nz, a.lpml);
+        const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
         const size_t ix = (size_t)z * a.lpml + px.i, iz = (size_t)pz.i * nx + x;
         if (!BW) {
             const float dtL2M = dt * ml2m[r], dtL = dt * ml[r];
             // d/dx Vx feeds Sxx (L2M) and Szz (L) through one memory variable
             sxx[r] += dtL2M * dVx_x;
             szz[r] += dtL * dVx_x;
-            if (px.side >= 0) {
+            if (!FAST && px.side >= 0) {
                 float *psi = a.psi[px.side ? RSG_E2D_VXX_RIGHT : RSG_E2D_VXX_LEFT] + ix;
                 const float A = tb.A(px.side, 0, px.i), B = tb.B(px.side, 0, px.i),
                             C = tb.C(px.side, 0, px.i);
@@ -389,7 +428,7 @@ __global__ __launch_bounds__(BX *BZ) void e2d_stress_kernel(const K2D a) {
             }
             sxx[r] += dtL * dVz_z;
             szz[r] += dtL2M * dVz_z;
-            if (pz.side >= 0) {
+            if (!FAST && pz.side >= 0) {
                 float *psi = a.psi[pz.side ? RSG_E2D_VZZ_BOTTOM : RSG_E2D_VZZ_TOP] + iz;
                 const float A = tb.A(pz.side, 0, pz.i), B = tb.B(pz.side, 0, pz.i),
                             C = tb.C(pz.side, 0, pz.i);
@@ -398,21 +437,34 @@ __global__ __launch_bounds__(BX *BZ) void e2d_stress_kernel(const K2D a) {
                 sxx[r] -= dtL * (q + C * dVz_z);
                 szz[r] -= dtL2M * (q + C * dVz_z);
             }
-            if (a.fs && z == a.lpml) szz[r] = 0.f;
+            if (!FAST && a.fs && z == a.lpml) szz[r] = 0.f;
             const float dtM = dt * mm[r];
-            term(sxz[r], dtM, dVz_x, px, tb, 1, a.psi[RSG_E2D_VZX_LEFT], a.psi[RSG_E2D_VZX_RIGHT], ix);
-            term(sxz[r], dtM, dVx_z, pz, tb, 1, a.psi[RSG_E2D_VXZ_TOP], a.psi[RSG_E2D_VXZ_BOTTOM], iz);
+            term<FAST>(sxz[r], dtM, dVz_x, px, tb, 1, a.psi[RSG_E2D_VZX_LEFT], a.psi[RSG_E2D_VZX_RIGHT], ix);
+            term<FAST>(sxz[r], dtM, dVx_z, pz, tb, 1, a.psi[RSG_E2D_VXZ_TOP], a.psi[RSG_E2D_VXZ_BOTTOM], iz);
         } else {
-            term(sxx[r], dt, dVx_x, px, tb, 0, a.psi[RSG_E2D_VXX_LEFT], a.psi[RSG_E2D_VXX_RIGHT], ix);
-            term(szz[r], dt, dVz_z, pz, tb, 0, a.psi[RSG_E2D_VZZ_TOP], a.psi[RSG_E2D_VZZ_BOTTOM], iz);
-            if (a.fs && z == a.lpml) szz[r] = 0.f;
-            term(sxz[r], dt, dVz_x, px, tb, 1, a.psi[RSG_E2D_VZX_LEFT], a.psi[RSG_E2D_VZX_RIGHT], ix);
-            term(sxz[r], dt, dVx_z, pz, tb, 1, a.psi[RSG_E2D_VXZ_TOP], a.psi[RSG_E2D_VXZ_BOTTOM], iz);
+            term<FAST>(sxx[r], dt, dVx_x, px, tb, 0, a.psi[RSG_E2D_VXX_LEFT], a.psi[RSG_E2D_VXX_RIGHT], ix);
+            term<FAST>(szz[r], dt, dVz_z, pz, tb, 0, a.psi[RSG_E2D_VZZ_TOP], a.psi[RSG_E2D_VZZ_BOTTOM], iz);
+            if (!FAST && a.fs && z == a.lpml) szz[r] = 0.f;
+            term<FAST>(sxz[r], dt, dVz_x, px, tb, 1, a.psi[RSG_E2D_VZX_LEFT], a.psi[RSG_E2D_VZX_RIGHT], ix);
+            term<FAST>(sxz[r], dt, dVx_z, pz, tb, 1, a.psi[RSG_E2D_VXZ_TOP], a.psi[RSG_E2D_VXZ_BOTTOM], iz);
         }
         Sxx[p] = sxx[r];
         Szz[p] = szz[r];
         Sxz[p] = sxz[r];
     }
+}
+
+template <int ORDER, int RZ, bool BW>
+__global__ __launch_bounds__(BX *BZ) void e2d_stress_kernel(const K2D a) {
+    constexpr int H = ORDER, TZ = BZ * RZ, LD = BX + 2 * H;
+    __shared__ float sVx[(TZ + 2 * H) * LD];
+    __shared__ float sVz[(TZ + 2 * H) * LD];
+    int x0, z0;
+    if (!tile_of(a, TZ, x0, z0)) return;
+    if (interior_tile(a, x0, z0, TZ, H))
+        e2d_stress_body<ORDER, RZ, BW, true>(a, x0, z0, sVx, sVz);
+    else
+        e2d_stress_body<ORDER, RZ, BW, false>(a, x0, z0, sVx, sVz);
 }
 
 // ---------------------------------------------------------------------------
