@@ -57,50 +57,78 @@ __device__ __forceinline__ bool interior_tile(const K2D &a, int x0, int z0, int 
            !(a.fs && z0 <= a.lpml);
 }
 
-// Stage rows [z0-HZ, z0+TZ+HZ) x cols [x0-HX, x0+BX+HX) of g into LDS,
-// zero outside the grid (those entries are never read by the edge stencils).
-template <int TZ, int HX, int HZ, bool FAST>
-__device__ __forceinline__ void load_tile(float *lds, const float *__restrict__ g, int nx, int nz,
-                                          int x0, int z0) {
-    constexpr int LD = BX + 2 * HX, ROWS = TZ + 2 * HZ;
+// Staging of a (TZ+2HZ) x (64+2HX) window into LDS, in two phases so that
+// every global load of a block is in flight before the first LDS write: fetch
+// issues all loads into registers (thread row ty takes window rows ty, ty+4,
+// ...; columns tx and, for halo columns, tx+64), store writes them out.
+// Boundary tiles clamp the address into the grid and select 0 outside it, so
+// the loads stay unconditional; zeros outside the grid are never read by the
+// reduced edge stencils.
+template <int TZ, int HX, int HZ>
+struct Tile {
+    static constexpr int LD = BX + 2 * HX, ROWS = TZ + 2 * HZ, KR = (ROWS + BZ - 1) / BZ;
+    float a[KR], b[KR];
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+template <int TZ, int HX, int HZ, bool FAST, class F>
+__device__ __forceinline__ void tile_fetch(Tile<TZ, HX, HZ> &t, F &&ld, int nx, int nz, int x0, int z0) {
+    using T = Tile<TZ, HX, HZ>;
+    const int tx = threadIdx.x, ty = threadIdx.y;
 #pragma unroll
-    for (int r = threadIdx.y; r < ROWS; r += BZ) {
+    for (int k = 0; k < T::KR; k++) {
+        const int r = ty + k * BZ;
         const int z = z0 - HZ + r;
-        const bool zin = FAST || (z >= 0 && z < nz);
-        const float *row = g + (ptrdiff_t)z * nx + (x0 - HX);
-#pragma unroll
-        for (int c = threadIdx.x; c < LD; c += BX) {
-            const int x = x0 - HX + c;
-            lds[r * LD + c] = (FAST || (zin && x >= 0 && x < nx)) ? row[c] : 0.f;
+        const int x = x0 - HX + tx;
+        if (FAST) {
+            t.a[k] = ld((unsigned)(z * nx + x));
+            if (2 * HX > 0) t.b[k] = tx < 2 * HX ? ld((unsigned)(z * nx + x + BX)) : 0.f;
+        } else {
+            const int zc = clampi(z, 0, nz - 1);
+            const bool zok = z >= 0 && z < nz;
+            const float va = ld((unsigned)(zc * nx + clampi(x, 0, nx - 1)));
+            t.a[k] = (zok && x >= 0 && x < nx) ? va : 0.f;
+            if (2 * HX > 0) {
+                const float vb = ld((unsigned)(zc * nx + clampi(x + BX, 0, nx - 1)));
+                t.b[k] = (zok && x + BX < nx) ? vb : 0.f;
+            }
         }
     }
 }
 
-// Same, staging a*b + c*d (c null: a*b) -- the moduli-weighted stresses of
-// WavesElastic2D::backwardstepVelocity (waves.cpp:1771-1775).
-template <int TZ, int HX, int HZ, bool FAST>
-__device__ __forceinline__ void load_tile_mix(float *lds, const float *__restrict__ a,
-                                              const float *__restrict__ b,
-                                              const float *__restrict__ c,
-                                              const float *__restrict__ d, int nx, int nz, int x0,
-                                              int z0) {
-    constexpr int LD = BX + 2 * HX, ROWS = TZ + 2 * HZ;
+template <int TZ, int HX, int HZ>
+__device__ __forceinline__ void tile_store(const Tile<TZ, HX, HZ> &t, float *lds) {
+    using T = Tile<TZ, HX, HZ>;
+    const int tx = threadIdx.x, ty = threadIdx.y;
 #pragma unroll
-    for (int r = threadIdx.y; r < ROWS; r += BZ) {
-        const int z = z0 - HZ + r;
-        const bool zin = FAST || (z >= 0 && z < nz);
-#pragma unroll
-        for (int cc = threadIdx.x; cc < LD; cc += BX) {
-            const int x = x0 - HX + cc;
-            float v = 0.f;
-            if (FAST || (zin && x >= 0 && x < nx)) {
-                const size_t p = (size_t)z * nx + x;
-                v = c ? a[p] * b[p] + c[p] * d[p] : a[p] * b[p];
-            }
-            lds[r * LD + cc] = v;
+    for (int k = 0; k < T::KR; k++) {
+        const int r = ty + k * BZ;
+        if (T::ROWS % BZ == 0 || r < T::ROWS) {
+            lds[r * T::LD + tx] = t.a[k];
+            if (2 * HX > 0 && tx < 2 * HX) lds[r * T::LD + tx + BX] = t.b[k];
         }
     }
 }
+
+// Loaders: a plain array, or the products a*b + c*d / a*b that
+// WavesElastic2D::backwardstepVelocity differentiates (waves.cpp:1771-1775).
+struct Ld1 {
+    const float *__restrict__ g;
+    __device__ __forceinline__ float operator()(unsigned i) const { return g[i]; }
+};
+struct Ld2 {
+    const float *__restrict__ a;
+    const float *__restrict__ b;
+    __device__ __forceinline__ float operator()(unsigned i) const { return a[i] * b[i]; }
+};
+struct Ld4 {
+    const float *__restrict__ a;
+    const float *__restrict__ b;
+    const float *__restrict__ c;
+    const float *__restrict__ d;
+    __device__ __forceinline__ float operator()(unsigned i) const { return a[i] * b[i] + c[i] * d[i]; }
+};
 
 // Derivative of an LDS tile at a cell: compile-time order in the interior,
 // reduced order where the reference's Der shortens the stencil.
@@ -187,7 +215,9 @@ __device__ __forceinline__ void a2d_velocity_body(const K2D &a, int x0, int z0, 
         rx[r] = in ? Rx[p] : 0.f;
         rz[r] = in ? Rz[p] : 0.f;
     }
-    load_tile<TZ, H, H, FAST>(sP, a.P, nx, nz, x0, z0);
+    Tile<TZ, H, H> tp;
+    tile_fetch<TZ, H, H, FAST>(tp, Ld1{a.P}, nx, nz, x0, z0);
+    tile_store(tp, sP);
     __syncthreads();
     if (!xin) return;
     const Tab tb{a.tab, a.lpml};
@@ -239,8 +269,12 @@ __device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, fl
         pv[r] = in ? P[p] : 0.f;
         lv[r] = in ? L[p] : 0.f;
     }
-    load_tile<TZ, H, 0, FAST>(sVx, a.Vx, nx, nz, x0, z0);
-    load_tile<TZ, 0, H, FAST>(sVz, a.Vz, nx, nz, x0, z0);
+    Tile<TZ, H, 0> tvx;
+    Tile<TZ, 0, H> tvz;
+    tile_fetch<TZ, H, 0, FAST>(tvx, Ld1{a.Vx}, nx, nz, x0, z0);
+    tile_fetch<TZ, 0, H, FAST>(tvz, Ld1{a.Vz}, nx, nz, x0, z0);
+    tile_store(tvx, sVx);
+    tile_store(tvz, sVz);
     __syncthreads();
     if (!xin) return;
     const Tab tb{a.tab, a.lpml};
@@ -307,15 +341,21 @@ __device__ __forceinline__ void e2d_velocity_body(const K2D &a, int x0, int z0, 
         rx[r] = in ? Rx[p] : 0.f;
         rz[r] = in ? Rz[p] : 0.f;
     }
+    Tile<TZ, H, 0> txx;
+    Tile<TZ, 0, H> tzz;
+    Tile<TZ, H, H> txz;
     if (!BW) {
-        load_tile<TZ, H, 0, FAST>(sXX, a.Sxx, nx, nz, x0, z0);
-        load_tile<TZ, 0, H, FAST>(sZZ, a.Szz, nx, nz, x0, z0);
-        load_tile<TZ, H, H, FAST>(sXZ, a.Sxz, nx, nz, x0, z0);
+        tile_fetch<TZ, H, 0, FAST>(txx, Ld1{a.Sxx}, nx, nz, x0, z0);
+        tile_fetch<TZ, 0, H, FAST>(tzz, Ld1{a.Szz}, nx, nz, x0, z0);
+        tile_fetch<TZ, H, H, FAST>(txz, Ld1{a.Sxz}, nx, nz, x0, z0);
     } else {
-        load_tile_mix<TZ, H, 0, FAST>(sXX, a.L2M, a.Sxx, a.L, a.Szz, nx, nz, x0, z0);
-        load_tile_mix<TZ, 0, H, FAST>(sZZ, a.L, a.Sxx, a.L2M, a.Szz, nx, nz, x0, z0);
-        load_tile_mix<TZ, H, H, FAST>(sXZ, a.M, a.Sxz, nullptr, nullptr, nx, nz, x0, z0);
+        tile_fetch<TZ, H, 0, FAST>(txx, Ld4{a.L2M, a.Sxx, a.L, a.Szz}, nx, nz, x0, z0);
+        tile_fetch<TZ, 0, H, FAST>(tzz, Ld4{a.L, a.Sxx, a.L2M, a.Szz}, nx, nz, x0, z0);
+        tile_fetch<TZ, H, H, FAST>(txz, Ld2{a.M, a.Sxz}, nx, nz, x0, z0);
     }
+    tile_store(txx, sXX);
+    tile_store(tzz, sZZ);
+    tile_store(txz, sXZ);
     __syncthreads();
     if (!xin) return;
     const Tab tb{a.tab, a.lpml};
@@ -390,8 +430,11 @@ __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, fl
             mm[r] = in ? a.M[p] : 0.f;
         }
     }
-    load_tile<TZ, H, H, FAST>(sVx, a.Vx, nx, nz, x0, z0);
-    load_tile<TZ, H, H, FAST>(sVz, a.Vz, nx, nz, x0, z0);
+    Tile<TZ, H, H> tvx, tvz;
+    tile_fetch<TZ, H, H, FAST>(tvx, Ld1{a.Vx}, nx, nz, x0, z0);
+    tile_fetch<TZ, H, H, FAST>(tvz, Ld1{a.Vz}, nx, nz, x0, z0);
+    tile_store(tvx, sVx);
+    tile_store(tvz, sVz);
     __syncthreads();
     if (!xin) return;
     const Tab tb{a.tab, a.lpml};
