@@ -37,10 +37,7 @@ static int one_step(int kind, const rsg_grid *g, const rsg_state *s, const rsg_t
                     const rsg_traces *recs, const int *it_dev, int it, hipStream_t st) {
     if (int rc = step_velocity(kind, g, s, st)) return rc;
     if (int rc = step_stress(kind, g, s, st)) return rc;
-    if (int rc = insert_source(kind, g, s, src, it_dev, it, st)) return rc;
-    for (int r = 0; r < nrec; r++)
-        if (int rc = record_data(kind, g, s, &recs[r], it_dev, it, st)) return rc;
-    return RSG_OK;
+    return step_io(kind, g, s, src, nrec, recs, it_dev, it, st);
 }
 
 }  // namespace rsg

@@ -9,6 +9,8 @@ int insert_source(int kind, const rsg_grid *g, const rsg_state *s, const rsg_tra
                   const int *it_dev, int it, hipStream_t st);
 int record_data(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *rec,
                 const int *it_dev, int it, hipStream_t st);
+int step_io(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src, int nrec,
+            const rsg_traces *recs, const int *it_dev, int it, hipStream_t st);
 int launch_add_int(int *p, int v, hipStream_t st);
 int launch_set_int(int *p, int v, hipStream_t st);
 }  // namespace rsg

@@ -16,14 +16,11 @@ __device__ __forceinline__ int time_index(const int *it_dev, int it) {
 
 // F += scale * Mod * wav * xtri * ytri (Mod null: modulus 1).  Corners of
 // different traces may coincide, so the add is atomic.
-__global__ void inject2d_kernel(float *F, const float *Mod, float scale, int nx, int lpml,
-                                int ntrace, const int *map, const float *shift, const float *wav,
-                                int nt, const int *it_dev, int it0) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntrace) return;
+__device__ __forceinline__ void inject2d_trace(int i, float *F, const float *Mod, float scale, int nx,
+                                               int lpml, const int *map, const float *shift,
+                                               const float *wav, int nt, int it) {
     const int mx = map[2 * i], mz = map[2 * i + 1];
     if (mx < 0 || mz < 0) return;
-    const int it = time_index(it_dev, it0);
     const float sx = shift[2 * i], sz = shift[2 * i + 1];
     const float xtri[2] = {1 - sx, sx}, ytri[2] = {1 - sz, sz};
     const float w = wav[(size_t)i * nt + it];
@@ -37,14 +34,19 @@ __global__ void inject2d_kernel(float *F, const float *Mod, float scale, int nx,
         }
 }
 
-__global__ void inject3d_kernel(float *F, const float *Mod, float scale, int nx, int ny, int lpml,
+__global__ void inject2d_kernel(float *F, const float *Mod, float scale, int nx, int lpml,
                                 int ntrace, const int *map, const float *shift, const float *wav,
                                 int nt, const int *it_dev, int it0) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntrace) return;
+    inject2d_trace(i, F, Mod, scale, nx, lpml, map, shift, wav, nt, time_index(it_dev, it0));
+}
+
+__device__ __forceinline__ void inject3d_trace(int i, float *F, const float *Mod, float scale, int nx,
+                                               int ny, int lpml, const int *map, const float *shift,
+                                               const float *wav, int nt, int it) {
     const int mx = map[3 * i], my = map[3 * i + 1], mz = map[3 * i + 2];
     if (mx < 0 || my < 0 || mz < 0) return;
-    const int it = time_index(it_dev, it0);
     const float sx = shift[3 * i], sy = shift[3 * i + 1], sz = shift[3 * i + 2];
     const float xtri[2] = {1 - sx, sx}, ytri[2] = {1 - sy, sy}, ztri[2] = {1 - sz, sz};
     const float w = wav[(size_t)i * nt + it];
@@ -62,17 +64,22 @@ __global__ void inject3d_kernel(float *F, const float *Mod, float scale, int nx,
             }
 }
 
+__global__ void inject3d_kernel(float *F, const float *Mod, float scale, int nx, int ny, int lpml,
+                                int ntrace, const int *map, const float *shift, const float *wav,
+                                int nt, const int *it_dev, int it0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntrace) return;
+    inject3d_trace(i, F, Mod, scale, nx, ny, lpml, map, shift, wav, nt, time_index(it_dev, it0));
+}
+
 // data[i*nt+it] += sum xtri*ytri*F over the 2x2 cells, in the reference's
 // corner order.  With F2 set, the elastic pressure 0.5*(F+F2)/factor is
 // accumulated in double as waves.cpp:2234-2237 does.
-__global__ void record2d_kernel(const float *F, const float *F2, const float *factor, int nx,
-                                int lpml, int ntrace, const int *map, const float *shift,
-                                float *data, int nt, const int *it_dev, int it0) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntrace) return;
+__device__ __forceinline__ void record2d_trace(int i, const float *F, const float *F2,
+                                               const float *factor, int nx, int lpml, const int *map,
+                                               const float *shift, float *data, int nt, int it) {
     const int mx = map[2 * i], mz = map[2 * i + 1];
     if (mx < 0 || mz < 0) return;
-    const int it = time_index(it_dev, it0);
     const float sx = shift[2 * i], sz = shift[2 * i + 1];
     const float xtri[2] = {1 - sx, sx}, ytri[2] = {1 - sz, sz};
     float *d = data + (size_t)i * nt + it;
@@ -98,13 +105,19 @@ __global__ void record2d_kernel(const float *F, const float *F2, const float *fa
     *d = acc;
 }
 
-__global__ void record3d_kernel(const float *F, int nx, int ny, int lpml, int ntrace, const int *map,
-                                const float *shift, float *data, int nt, const int *it_dev, int it0) {
+__global__ void record2d_kernel(const float *F, const float *F2, const float *factor, int nx,
+                                int lpml, int ntrace, const int *map, const float *shift,
+                                float *data, int nt, const int *it_dev, int it0) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntrace) return;
+    record2d_trace(i, F, F2, factor, nx, lpml, map, shift, data, nt, time_index(it_dev, it0));
+}
+
+__device__ __forceinline__ void record3d_trace(int i, const float *F, int nx, int ny, int lpml,
+                                               const int *map, const float *shift, float *data,
+                                               int nt, int it) {
     const int mx = map[3 * i], my = map[3 * i + 1], mz = map[3 * i + 2];
     if (mx < 0 || my < 0 || mz < 0) return;
-    const int it = time_index(it_dev, it0);
     const float sx = shift[3 * i], sy = shift[3 * i + 1], sz = shift[3 * i + 2];
     const float xtri[2] = {1 - sx, sx}, ytri[2] = {1 - sy, sy}, ztri[2] = {1 - sz, sz};
     const size_t nxy = (size_t)nx * ny;
@@ -119,6 +132,67 @@ __global__ void record3d_kernel(const float *F, int nx, int ny, int lpml, int nt
                 acc += xtri[i3] * ytri[i2] * ztri[i1] *
                        F[(size_t)(lpml + mz + i1) * nxy + (size_t)(lpml + my + i2) * nx + (lpml + mx + i3)];
     *d = acc;
+}
+
+__global__ void record3d_kernel(const float *F, int nx, int ny, int lpml, int ntrace, const int *map,
+                                const float *shift, float *data, int nt, const int *it_dev, int it0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntrace) return;
+    record3d_trace(i, F, nx, ny, lpml, map, shift, data, nt, time_index(it_dev, it0));
+}
+
+// One launch for the tail of a modelling step: inject the source of step it,
+// then (after a block barrier, so pressure gathers see the injected stress)
+// gather every receiver set.  A single block serves all traces, which replaces
+// three or more tiny launches per step.
+struct SrcDesc {
+    float *F, *F2;
+    const float *Mod;
+    float scale;
+    int ntrace, nt;
+    const int *map;
+    const float *shift;
+    const float *wav;
+};
+struct RecDesc {
+    const float *F, *F2, *factor;
+    int ntrace, nt;
+    const int *map;
+    const float *shift;
+    float *data;
+};
+constexpr int kMaxFusedRec = 4;
+struct StepIO {
+    int dim, nx, ny, lpml, nrec;
+    SrcDesc src;
+    RecDesc rec[kMaxFusedRec];
+};
+
+__global__ __launch_bounds__(1024) void step_io_kernel(const StepIO io, const int *it_dev, int it0) {
+    const int it = time_index(it_dev, it0);
+    const SrcDesc &sd = io.src;
+    for (int i = threadIdx.x; i < sd.ntrace; i += blockDim.x) {
+        if (io.dim == 3) {
+            inject3d_trace(i, sd.F, sd.Mod, sd.scale, io.nx, io.ny, io.lpml, sd.map, sd.shift, sd.wav, sd.nt, it);
+        } else {
+            inject2d_trace(i, sd.F, sd.Mod, sd.scale, io.nx, io.lpml, sd.map, sd.shift, sd.wav, sd.nt, it);
+            if (sd.F2)
+                inject2d_trace(i, sd.F2, sd.Mod, sd.scale, io.nx, io.lpml, sd.map, sd.shift, sd.wav, sd.nt, it);
+        }
+    }
+    // the injected values (device-scope atomics) must be complete before any
+    // lane of the block gathers them
+    __threadfence();
+    __syncthreads();
+    for (int r = 0; r < io.nrec; r++) {
+        const RecDesc &rd = io.rec[r];
+        for (int i = threadIdx.x; i < rd.ntrace; i += blockDim.x) {
+            if (io.dim == 3)
+                record3d_trace(i, rd.F, io.nx, io.ny, io.lpml, rd.map, rd.shift, rd.data, rd.nt, it);
+            else
+                record2d_trace(i, rd.F, rd.F2, rd.factor, io.nx, io.lpml, rd.map, rd.shift, rd.data, rd.nt, it);
+        }
+    }
 }
 
 __global__ void add_int_kernel(int *p, int v) {
@@ -161,6 +235,71 @@ static int source_target(int kind, const rsg_state *s, int field, float dt, floa
     return fail(RSG_EINVAL, "unsupported source field for this propagator");
 }
 
+static int record_target(int kind, const rsg_state *s, int field, const float **F, const float **F2) {
+    *F = nullptr;
+    *F2 = nullptr;
+    switch (field) {
+    case RSG_PRESSURE:
+        if (kind == RSG_KIND_E2D) {
+            *F = s->Sxx;
+            *F2 = s->Szz;
+        } else {
+            *F = s->P;
+        }
+        break;
+    case RSG_VX: *F = s->Vx; break;
+    case RSG_VY: *F = kind == RSG_KIND_A3D ? s->Vy : nullptr; break;
+    case RSG_VZ: *F = s->Vz; break;
+    default: break;
+    }
+    if (!*F) return fail(RSG_EINVAL, "recordData: field not available for this propagator");
+    return RSG_OK;
+}
+
+// Fused injection + gathers of one step (all sets must have few traces).
+int step_io(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src, int nrec,
+            const rsg_traces *recs, const int *it_dev, int it, hipStream_t st) {
+    if (nrec > kMaxFusedRec) {
+        if (int rc = insert_source(kind, g, s, src, it_dev, it, st)) return rc;
+        for (int r = 0; r < nrec; r++)
+            if (int rc = record_data(kind, g, s, &recs[r], it_dev, it, st)) return rc;
+        return RSG_OK;
+    }
+    StepIO io{};
+    io.dim = kind == RSG_KIND_A3D ? 3 : 2;
+    io.nx = g->nx;
+    io.ny = g->ny;
+    io.lpml = g->lpml;
+    if (src && src->ntrace > 0) {
+        if (!it_dev && (it < 0 || it >= src->nt)) return fail(RSG_EINVAL, "insertSource: it out of range");
+        if (int rc = source_target(kind, s, src->field, g->dt, &io.src.F, &io.src.F2, &io.src.Mod, &io.src.scale))
+            return rc;
+        if (!io.src.F) return fail(RSG_EINVAL, "insertSource: target field not allocated");
+        io.src.ntrace = src->ntrace;
+        io.src.nt = src->nt;
+        io.src.map = src->map;
+        io.src.shift = src->shift;
+        io.src.wav = src->data;
+    }
+    io.nrec = 0;
+    for (int r = 0; r < nrec; r++) {
+        if (recs[r].ntrace <= 0) continue;
+        if (!it_dev && (it < 0 || it >= recs[r].nt)) return fail(RSG_EINVAL, "recordData: it out of range");
+        RecDesc &rd = io.rec[io.nrec++];
+        if (int rc = record_target(kind, s, recs[r].field, &rd.F, &rd.F2)) return rc;
+        rd.factor = recs[r].factor;
+        rd.ntrace = recs[r].ntrace;
+        rd.nt = recs[r].nt;
+        rd.map = recs[r].map;
+        rd.shift = recs[r].shift;
+        rd.data = recs[r].data;
+    }
+    if (io.src.ntrace == 0 && io.nrec == 0) return RSG_OK;
+    step_io_kernel<<<1, 1024, 0, st>>>(io, it_dev, it);
+    RSG_LAUNCH_CHECK("step_io_kernel");
+    return RSG_OK;
+}
+
 int insert_source(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src,
                   const int *it_dev, int it, hipStream_t st) {
     if (!src || src->ntrace <= 0) return RSG_OK;
@@ -190,22 +329,8 @@ int record_data(int kind, const rsg_grid *g, const rsg_state *s, const rsg_trace
                 const int *it_dev, int it, hipStream_t st) {
     if (!rec || rec->ntrace <= 0) return RSG_OK;
     if (!it_dev && (it < 0 || it >= rec->nt)) return fail(RSG_EINVAL, "recordData: it out of range");
-    const float *F = nullptr, *F2 = nullptr;
-    switch (rec->field) {
-    case RSG_PRESSURE:
-        if (kind == RSG_KIND_E2D) {
-            F = s->Sxx;
-            F2 = s->Szz;
-        } else {
-            F = s->P;
-        }
-        break;
-    case RSG_VX: F = s->Vx; break;
-    case RSG_VY: F = kind == RSG_KIND_A3D ? s->Vy : nullptr; break;
-    case RSG_VZ: F = s->Vz; break;
-    default: break;
-    }
-    if (!F) return fail(RSG_EINVAL, "recordData: field not available for this propagator");
+    const float *F, *F2;
+    if (int rc = record_target(kind, s, rec->field, &F, &F2)) return rc;
     const int threads = 64, blocks = (rec->ntrace + threads - 1) / threads;
     if (kind == RSG_KIND_A3D)
         record3d_kernel<<<blocks, threads, 0, st>>>(F, g->nx, g->ny, g->lpml, rec->ntrace, rec->map,
