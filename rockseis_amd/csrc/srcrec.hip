@@ -141,19 +141,8 @@ __global__ void record3d_kernel(const float *F, int nx, int ny, int lpml, int nt
     record3d_trace(i, F, nx, ny, lpml, map, shift, data, nt, time_index(it_dev, it0));
 }
 
-// One launch for the tail of a modelling step: inject the source of step it,
-// then (after a block barrier, so pressure gathers see the injected stress)
-// gather every receiver set.  A single block serves all traces, which replaces
-// three or more tiny launches per step.
-struct SrcDesc {
-    float *F, *F2;
-    const float *Mod;
-    float scale;
-    int ntrace, nt;
-    const int *map;
-    const float *shift;
-    const float *wav;
-};
+// All receiver sets of a step in one launch: lane g of the grid gathers
+// trace g of the concatenated sets (set boundaries in `first`).
 struct RecDesc {
     const float *F, *F2, *factor;
     int ntrace, nt;
@@ -162,37 +151,26 @@ struct RecDesc {
     float *data;
 };
 constexpr int kMaxFusedRec = 4;
-struct StepIO {
+struct RecSets {
     int dim, nx, ny, lpml, nrec;
-    SrcDesc src;
+    int first[kMaxFusedRec + 1];
     RecDesc rec[kMaxFusedRec];
 };
 
-__global__ __launch_bounds__(1024) void step_io_kernel(const StepIO io, const int *it_dev, int it0) {
+__global__ void record_sets_kernel(const RecSets rs, const int *it_dev, int it0) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= rs.first[rs.nrec]) return;
+    int r = 0;
+#pragma unroll
+    for (int k = 1; k < kMaxFusedRec; k++)
+        if (k < rs.nrec && g >= rs.first[k]) r = k;
+    const RecDesc &rd = rs.rec[r];
+    const int i = g - rs.first[r];
     const int it = time_index(it_dev, it0);
-    const SrcDesc &sd = io.src;
-    for (int i = threadIdx.x; i < sd.ntrace; i += blockDim.x) {
-        if (io.dim == 3) {
-            inject3d_trace(i, sd.F, sd.Mod, sd.scale, io.nx, io.ny, io.lpml, sd.map, sd.shift, sd.wav, sd.nt, it);
-        } else {
-            inject2d_trace(i, sd.F, sd.Mod, sd.scale, io.nx, io.lpml, sd.map, sd.shift, sd.wav, sd.nt, it);
-            if (sd.F2)
-                inject2d_trace(i, sd.F2, sd.Mod, sd.scale, io.nx, io.lpml, sd.map, sd.shift, sd.wav, sd.nt, it);
-        }
-    }
-    // the injected values (device-scope atomics) must be complete before any
-    // lane of the block gathers them
-    __threadfence();
-    __syncthreads();
-    for (int r = 0; r < io.nrec; r++) {
-        const RecDesc &rd = io.rec[r];
-        for (int i = threadIdx.x; i < rd.ntrace; i += blockDim.x) {
-            if (io.dim == 3)
-                record3d_trace(i, rd.F, io.nx, io.ny, io.lpml, rd.map, rd.shift, rd.data, rd.nt, it);
-            else
-                record2d_trace(i, rd.F, rd.F2, rd.factor, io.nx, io.lpml, rd.map, rd.shift, rd.data, rd.nt, it);
-        }
-    }
+    if (rs.dim == 3)
+        record3d_trace(i, rd.F, rs.nx, rs.ny, rs.lpml, rd.map, rd.shift, rd.data, rd.nt, it);
+    else
+        record2d_trace(i, rd.F, rd.F2, rd.factor, rs.nx, rs.lpml, rd.map, rd.shift, rd.data, rd.nt, it);
 }
 
 __global__ void add_int_kernel(int *p, int v) {
@@ -256,36 +234,26 @@ static int record_target(int kind, const rsg_state *s, int field, const float **
     return RSG_OK;
 }
 
-// Fused injection + gathers of one step (all sets must have few traces).
+// Tail of one modelling step: insertSource(it), then every receiver set
+// gathered by one launch (sets beyond kMaxFusedRec get their own launches).
 int step_io(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src, int nrec,
             const rsg_traces *recs, const int *it_dev, int it, hipStream_t st) {
-    if (nrec > kMaxFusedRec) {
-        if (int rc = insert_source(kind, g, s, src, it_dev, it, st)) return rc;
-        for (int r = 0; r < nrec; r++)
-            if (int rc = record_data(kind, g, s, &recs[r], it_dev, it, st)) return rc;
-        return RSG_OK;
-    }
-    StepIO io{};
-    io.dim = kind == RSG_KIND_A3D ? 3 : 2;
-    io.nx = g->nx;
-    io.ny = g->ny;
-    io.lpml = g->lpml;
-    if (src && src->ntrace > 0) {
-        if (!it_dev && (it < 0 || it >= src->nt)) return fail(RSG_EINVAL, "insertSource: it out of range");
-        if (int rc = source_target(kind, s, src->field, g->dt, &io.src.F, &io.src.F2, &io.src.Mod, &io.src.scale))
-            return rc;
-        if (!io.src.F) return fail(RSG_EINVAL, "insertSource: target field not allocated");
-        io.src.ntrace = src->ntrace;
-        io.src.nt = src->nt;
-        io.src.map = src->map;
-        io.src.shift = src->shift;
-        io.src.wav = src->data;
-    }
-    io.nrec = 0;
+    if (int rc = insert_source(kind, g, s, src, it_dev, it, st)) return rc;
+    RecSets rs{};
+    rs.dim = kind == RSG_KIND_A3D ? 3 : 2;
+    rs.nx = g->nx;
+    rs.ny = g->ny;
+    rs.lpml = g->lpml;
+    rs.nrec = 0;
+    rs.first[0] = 0;
     for (int r = 0; r < nrec; r++) {
         if (recs[r].ntrace <= 0) continue;
+        if (rs.nrec == kMaxFusedRec) {
+            if (int rc = record_data(kind, g, s, &recs[r], it_dev, it, st)) return rc;
+            continue;
+        }
         if (!it_dev && (it < 0 || it >= recs[r].nt)) return fail(RSG_EINVAL, "recordData: it out of range");
-        RecDesc &rd = io.rec[io.nrec++];
+        RecDesc &rd = rs.rec[rs.nrec];
         if (int rc = record_target(kind, s, recs[r].field, &rd.F, &rd.F2)) return rc;
         rd.factor = recs[r].factor;
         rd.ntrace = recs[r].ntrace;
@@ -293,10 +261,13 @@ int step_io(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *s
         rd.map = recs[r].map;
         rd.shift = recs[r].shift;
         rd.data = recs[r].data;
+        rs.first[rs.nrec + 1] = rs.first[rs.nrec] + recs[r].ntrace;
+        rs.nrec++;
     }
-    if (io.src.ntrace == 0 && io.nrec == 0) return RSG_OK;
-    step_io_kernel<<<1, 1024, 0, st>>>(io, it_dev, it);
-    RSG_LAUNCH_CHECK("step_io_kernel");
+    if (rs.nrec == 0) return RSG_OK;
+    const int total = rs.first[rs.nrec], threads = 128;
+    record_sets_kernel<<<(total + threads - 1) / threads, threads, 0, st>>>(rs, it_dev, it);
+    RSG_LAUNCH_CHECK("record_sets_kernel");
     return RSG_OK;
 }
 
