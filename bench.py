@@ -45,7 +45,30 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=80, help="time steps of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-dispatch", action="store_true",
+                    help="no timing: 100 modelling steps, 10 launches of each half-step kernel and 5 "
+                         "calibration copies (torch.neg of 256 MiB), for rocprofv3 --pmc passes")
     return ap.parse_args()
+
+
+def pmc_dispatch():
+    import torch
+    from rockseis_amd import _lib as R
+    mod, cfg, _ = build_workload()
+    mod.prepare()
+    mod.launch(0, 100)
+    torch.cuda.synchronize()
+    g, s = mod._args[0], mod._args[1]
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for which in (0, 1):
+        R.check(R.lib().rsg_repeat_step(R.KIND_E2D, which, C.byref(g), C.byref(s), 10, st))
+    torch.cuda.synchronize()
+    x = torch.ones(64 * 1024 * 1024, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print("pmc dispatch done", flush=True)
 
 
 def build_workload():
@@ -145,6 +168,9 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if args.pmc_dispatch:
+        pmc_dispatch()
+        return
     mod, cfg, inputs = build_workload()
     K = min(args.steps, cfg["nt"])
     W = min(args.warmup, cfg["nt"])

@@ -34,12 +34,16 @@ __device__ __forceinline__ void inject2d_trace(int i, float *F, const float *Mod
         }
 }
 
-__global__ void inject2d_kernel(float *F, const float *Mod, float scale, int nx, int lpml,
+// F2 (optional) receives the same term: the elastic explosive source drives
+// Sxx and Szz together.
+__global__ void inject2d_kernel(float *F, float *F2, const float *Mod, float scale, int nx, int lpml,
                                 int ntrace, const int *map, const float *shift, const float *wav,
                                 int nt, const int *it_dev, int it0) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntrace) return;
-    inject2d_trace(i, F, Mod, scale, nx, lpml, map, shift, wav, nt, time_index(it_dev, it0));
+    const int it = time_index(it_dev, it0);
+    inject2d_trace(i, F, Mod, scale, nx, lpml, map, shift, wav, nt, it);
+    if (F2) inject2d_trace(i, F2, Mod, scale, nx, lpml, map, shift, wav, nt, it);
 }
 
 __device__ __forceinline__ void inject3d_trace(int i, float *F, const float *Mod, float scale, int nx,
@@ -285,12 +289,8 @@ int insert_source(int kind, const rsg_grid *g, const rsg_state *s, const rsg_tra
         inject3d_kernel<<<blocks, threads, 0, st>>>(F, Mod, scale, g->nx, g->ny, g->lpml, src->ntrace,
                                                     src->map, src->shift, src->data, src->nt, it_dev, it);
     } else {
-        inject2d_kernel<<<blocks, threads, 0, st>>>(F, Mod, scale, g->nx, g->lpml, src->ntrace, src->map,
-                                                    src->shift, src->data, src->nt, it_dev, it);
-        if (F2)
-            inject2d_kernel<<<blocks, threads, 0, st>>>(F2, Mod, scale, g->nx, g->lpml, src->ntrace,
-                                                        src->map, src->shift, src->data, src->nt, it_dev,
-                                                        it);
+        inject2d_kernel<<<blocks, threads, 0, st>>>(F, F2, Mod, scale, g->nx, g->lpml, src->ntrace,
+                                                    src->map, src->shift, src->data, src->nt, it_dev, it);
     }
     RSG_LAUNCH_CHECK("inject kernel");
     return RSG_OK;
