@@ -169,6 +169,32 @@ int rsg_modelling_run(int kind, const rsg_grid *g, const rsg_state *s, const rsg
                       int nrec, const rsg_traces *recs, int it0, int nsteps, int graph_chunk,
                       void *stream);
 
+/* ---- FWI gradient of one shot (FwiAcoustic2D::run_optimal, fwi.cpp:1159-1332) */
+#define RSG_FWI_REVOLVE 0 /* Revolve checkpoints in HBM (ncheck snapshots)       */
+#define RSG_FWI_STORE 1   /* forward pressure of every step kept in HBM          */
+
+typedef struct {
+    const float *L, *Rx, *Rz, *tab; /* staggered model and PML tables (device)    */
+    const float *Vp, *Rho;          /* unpadded model nz*nx (device), for crossCorr */
+    int nx, nz;                     /* unpadded sizes; grid is (nx+2lpml)(nz+2lpml) */
+    rsg_traces src;                 /* PRESSURE source, wavelet in data             */
+    rsg_traces obs;                 /* observed PRESSURE data at the receivers      */
+    float *dmod, *dres;             /* out: modelled data, residuals (ntrace*nt)   */
+    float *wavgrad;                 /* out: adjoint recorded at the source, or NULL */
+    float *vpgrad, *rhograd;        /* in/out: accumulated gradients (nz*nx)        */
+    double *misfit;                 /* out (host): DIFFERENCE misfit 0.5*sum r^2    */
+    int ncheck;                     /* Revolve snapshots (RSG_FWI_REVOLVE)          */
+    int mode;                       /* RSG_FWI_REVOLVE or RSG_FWI_STORE             */
+} rsg_fwi_a2d;
+
+int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, void *stream);
+/* Revolve::revolve() action sequence (revolve.cpp:150-305): advance=0,
+ * takeshot=1, restore=2, firsturn=3, youturn=4, terminate=5, error=6. */
+int rsg_revolve_schedule(int nt, int snaps, int *actions, int *capos, int *checks, int maxn,
+                         int *n);
+/* Revolve::numforw (revolve.cpp:31-62) */
+int rsg_revolve_numforw(int nt, int snaps);
+
 #ifdef __cplusplus
 }
 #endif

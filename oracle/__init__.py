@@ -171,6 +171,9 @@ def _declare_ref(R):
                                                  _f32p, C.c_int, _f32p])
     R.ref_a3d_run.argtypes = ([_f32p, _f32p] + [C.c_int] * 4 + [C.c_float] * 4 + [C.c_int] * 5
                               + [_f32p, _f32p, C.c_int, C.c_int, _f32p, _f32p, _f32p])
+    R.ref_fwi_a2d.argtypes = ([_f32p, _f32p] + [C.c_int] * 3 + [C.c_float] * 3 + [C.c_int] * 5
+                              + [_f32p, _f32p, C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p])
+    R.ref_fwi_a2d.restype = C.c_float
     R.ref_revolve.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, C.c_int]
     R.ref_revolve.restype = C.c_int
     R.ref_revolve_numforw.argtypes = [C.c_int, C.c_int]

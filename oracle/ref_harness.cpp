@@ -275,6 +275,58 @@ void ref_a3d_run(const float *vp, const float *rho, int nx, int ny, int nz, int 
     for (int j = 0; j < 4; j++) std::memcpy(fields + np * j, f[j], sizeof(float) * np);
 }
 
+// FwiAcoustic2D::run_optimal on a staggered (non-domain) model: the
+// reference's gradient of one shot (fwi.cpp:1159-1332) with ncheck in-core
+// Revolve checkpoints and the DIFFERENCE misfit.  Outputs: vpgrad, rhograd
+// [nz*nx], modelled and residual data [nrec*nt], wavelet gradient
+// [nsrc*nt]; returns the misfit.
+float ref_fwi_a2d(const float *vp, const float *rho, int nx, int nz, int lpml, float dx, float dz,
+                  float dt, int nt, int order, int fs, int ncheck, int nsrc, const float *scoords,
+                  const float *wav, int nrec, const float *gcoords, const float *dobs, float *vpgrad,
+                  float *rhograd, float *dmod, float *dres, float *wgrad) {
+    auto model = std::make_shared<ModelAcoustic2D<float>>(nx, nz, lpml, dx, dz, 0.f, 0.f, fs != 0);
+    model->createModel();
+    std::memcpy(model->getVp(), vp, sizeof(float) * nx * nz);
+    std::memcpy(model->getR(), rho, sizeof(float) * nx * nz);
+    auto src = std::make_shared<Data2D<float>>(nsrc, nt, dt, 0.f);
+    src->setField(PRESSURE);
+    fill2<float>(src, scoords, true);
+    std::memcpy(src->getData(), wav, sizeof(float) * nsrc * nt);
+    src->makeMap(model->getGeom(), SMAP);
+    auto mk = [&](const float *d) {
+        auto x = std::make_shared<Data2D<float>>(nrec, nt, dt, 0.f);
+        x->setField(PRESSURE);
+        fill2<float>(x, gcoords, false);
+        if (d) std::memcpy(x->getData(), d, sizeof(float) * nrec * nt);
+        x->makeMap(model->getGeom(), GMAP);
+        return x;
+    };
+    auto obs = mk(dobs), mod = mk(nullptr), res = mk(nullptr);
+    model->staggerModels();
+    auto fwi = std::make_shared<FwiAcoustic2D<float>>(model, src, obs, order, 1);
+    fwi->setDatamodP(mod);
+    fwi->setDataresP(res);
+    auto vg = std::make_shared<Image2D<float>>("vpgrad_unused", model, 1, 1);
+    auto rg = std::make_shared<Image2D<float>>("rhograd_unused", model, 1, 1);
+    fwi->setVpgrad(vg);
+    fwi->setRhograd(rg);
+    auto wg = std::make_shared<Data2D<float>>(nsrc, nt, dt, 0.f);
+    wg->setField(PRESSURE);
+    fill2<float>(wg, scoords, true);
+    wg->makeMap(model->getGeom(), SMAP);
+    fwi->setWavgrad(wg);
+    fwi->setNcheck(ncheck);
+    fwi->setIncore(true);
+    fwi->setMisfit_type(DIFFERENCE);
+    fwi->run_optimal();
+    std::memcpy(vpgrad, vg->getImagedata(), sizeof(float) * nx * nz);
+    std::memcpy(rhograd, rg->getImagedata(), sizeof(float) * nx * nz);
+    std::memcpy(dmod, mod->getData(), sizeof(float) * nrec * nt);
+    std::memcpy(dres, res->getData(), sizeof(float) * nrec * nt);
+    std::memcpy(wgrad, wg->getData(), sizeof(float) * nsrc * nt);
+    return fwi->getMisfit();
+}
+
 // Revolve action sequence: actions[k] (advance=0 takeshot=1 restore=2
 // firsturn=3 youturn=4 terminate=5 error=6) and capo after the call.
 // Returns the number of actions written, or -1 if maxn was too small.

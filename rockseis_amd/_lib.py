@@ -41,6 +41,16 @@ class Traces(C.Structure):
                 ("shift", C.c_void_p), ("data", C.c_void_p), ("factor", C.c_void_p)]
 
 
+class Fwi(C.Structure):
+    _fields_ = ([(n, C.c_void_p) for n in ("L", "Rx", "Rz", "tab", "Vp", "Rho")]
+                + [("nx", C.c_int), ("nz", C.c_int), ("src", Traces), ("obs", Traces)]
+                + [(n, C.c_void_p) for n in ("dmod", "dres", "wavgrad", "vpgrad", "rhograd")]
+                + [("misfit", C.POINTER(C.c_double)), ("ncheck", C.c_int), ("mode", C.c_int)])
+
+
+FWI_REVOLVE, FWI_STORE = 0, 1
+
+
 class RsgError(RuntimeError):
     pass
 
@@ -85,6 +95,9 @@ def _declare(L):
         "rsg_modelling_run": ([ip, GP, SP, TP, ip, TP, ip, ip, ip, vp], ip),
         "rsg_repeat_step": ([ip, ip, GP, SP, ip, vp], ip),
         "rsg_set_tuning": ([C.c_char_p, ip], ip),
+        "rsg_fwi_a2d_gradient": ([GP, C.POINTER(Fwi), vp], ip),
+        "rsg_revolve_schedule": ([ip, ip, vp, vp, vp, ip, C.POINTER(C.c_int)], ip),
+        "rsg_revolve_numforw": ([ip, ip], ip),
         "rsg_get_tuning": ([C.c_char_p, C.POINTER(C.c_int)], ip),
     }
     for name in ("a2d_forward_velocity", "a2d_forward_stress", "e2d_forward_velocity",
@@ -106,7 +119,8 @@ EXPORTS = (
     "rsg_e2d_forward_velocity", "rsg_e2d_forward_stress", "rsg_e2d_backward_velocity",
     "rsg_e2d_backward_stress", "rsg_a3d_forward_velocity", "rsg_a3d_forward_stress",
     "rsg_insert_source", "rsg_record_data", "rsg_modelling_run", "rsg_repeat_step",
-    "rsg_set_tuning", "rsg_get_tuning",
+    "rsg_set_tuning", "rsg_get_tuning", "rsg_fwi_a2d_gradient", "rsg_revolve_schedule",
+    "rsg_revolve_numforw",
 )
 
 

@@ -1,0 +1,405 @@
+// Acoustic 2D FWI gradient of one shot (FwiAcoustic2D::run_optimal,
+// fwi.cpp:1159-1332) on the device.
+//
+// The forward wavefield is stepped with the fused acoustic kernels; the
+// Revolve schedule (revolve.cpp:150-305) runs on the host and its checkpoints
+// (pressure, velocities and the 8 PML strips, the payload of
+// revolve.cpp:648-700) live in HBM.  As in the reference, the adjoint
+// wavefield is advanced with the forward operator and driven by the DIFFERENCE
+// residuals injected at the receivers; every reverse step accumulates the
+// zero-lag cross-correlation of crossCorr (fwi.cpp:511-571) into the Vp and
+// rho gradients.
+//
+// Mode RSG_FWI_STORE keeps the forward pressure of every step in HBM instead
+// (nt * nx_pml * nz_pml floats: 4.8 GB for an 861x461 grid and 3000 steps),
+// so the forward sweep runs once; the gradient is identical because the
+// reference correlates, at reverse step capo, the forward pressure after
+// capo-1 forward steps (after nt-1 steps at the first turn).
+#include <cstring>
+#include <vector>
+
+#include "rsg_common.h"
+#include "rsg_fwi.h"
+#include "rsg_srcrec.h"
+
+namespace rsg {
+
+// ---------------------------------------------------------------------------
+// Revolve schedule (binomial checkpointing, Griewank & Walther, ACM TOMS
+// Algorithm 799): the integer state machine of revolve.cpp:150-305.
+// ---------------------------------------------------------------------------
+
+int Revolve::maxrange(int ss, int tt) {
+    if (tt < 0 || ss < 0) return -1;
+    double res = 1.0;
+    for (int i = 1; i <= tt; i++) {
+        res *= (ss + i);
+        res /= i;
+        if (res > 2147483647.0) return 2147483647;
+    }
+    return (int)res;
+}
+
+int Revolve::adjust(int steps) {
+    int snaps = 1, reps = 1, s = 0;
+    while (maxrange(snaps + s, reps + s) > steps) s--;
+    while (maxrange(snaps + s, reps + s) < steps) s++;
+    snaps += s;
+    reps += s;
+    s = -1;
+    while (maxrange(snaps, reps) >= steps) {
+        if (snaps > reps) {
+            snaps -= 1;
+            s = 0;
+        } else {
+            reps -= 1;
+            s = 1;
+        }
+    }
+    if (s == 0) snaps += 1;
+    if (s == 1) reps += 1;
+    return snaps;
+}
+
+int Revolve::numforw(int steps, int snaps) {
+    if (snaps < 1 || snaps > kCheckUp) return -1;
+    int reps = 0, range = 1;
+    while (range < steps) {
+        reps += 1;
+        range = range * (reps + snaps) / reps;
+    }
+    if (reps > kRepsUp) return -1;
+    return reps * steps - range * reps / (snaps + 1);
+}
+
+Revolve::Revolve(int steps_, int snaps_) : steps(steps_), snaps(snaps_) {
+    if (snaps > steps || snaps < 2) snaps = adjust(steps);
+    capo = 0;
+    fine = steps;
+    check = -1;
+    turn = 0;
+    oldfine = 0;
+    ch.assign(kCheckUp, 0);
+}
+
+Revolve::Action Revolve::next() {
+    if (check < -1 || capo > fine) return ERROR;
+    if (check == -1 && capo < fine) {
+        turn = 0;
+        ch[0] = capo - 1;
+    }
+    const int span = fine - capo;
+    if (span == 0) {
+        if (check == -1 || capo == ch[0]) {
+            check -= 1;
+            return TERMINATE;
+        }
+        capo = ch[check];
+        oldfine = fine;
+        return RESTORE;
+    }
+    if (span == 1) {
+        fine -= 1;
+        if (check >= 0 && ch[check] == capo) check -= 1;
+        oldfine = fine;
+        if (turn == 0) {
+            turn = 1;
+            return FIRSTURN;
+        }
+        return YOUTURN;
+    }
+    if (check == -1 || ch[check] != capo) {
+        check += 1;
+        if (check >= kCheckUp || check + 1 > snaps) return ERROR;
+        ch[check] = capo;
+        oldfine = fine;
+        return TAKESHOT;
+    }
+    if (oldfine < fine && snaps == check + 1) return ERROR;
+    const int ds = snaps - check;
+    if (ds < 1) return ERROR;
+    int reps = 0, range = 1;
+    while (range < span) {
+        reps += 1;
+        range = range * (reps + ds) / reps;
+    }
+    if (reps > kRepsUp) return ERROR;
+    const int b1 = range * reps / (ds + reps);
+    const int b2 = ds > 1 ? b1 * ds / (ds + reps - 1) : 1;
+    const int b3 = ds == 1 ? 0 : (ds > 2 ? b2 * (ds - 1) / (ds + reps - 2) : 1);
+    const int b4 = b2 * (reps - 1) / ds;
+    const int b5 = ds < 3 ? 0 : (ds > 3 ? b3 * (ds - 2) / reps : 1);
+    const int old = capo;
+    if (span <= b1 + b3)
+        capo = capo + b4;
+    else if (span >= range - b5)
+        capo = capo + b1;
+    else
+        capo = fine - b2 - b3;
+    if (capo == old) capo = old + 1;
+    oldfine = fine;
+    return ADVANCE;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// crossCorr for one reverse step, interior ix, iz in [1, n-1) of the
+// unpadded nx x nz image (fwi.cpp:550-569); P_f, Vx_a, Vz_a, Rx, Rz padded
+// by lpml.  The rho-gradient velocity terms keep the reference's double
+// temporaries.
+__global__ void crosscorr_a2d_kernel(const float *__restrict__ Pf, const float *__restrict__ Vxa,
+                                     const float *__restrict__ Vza, const float *__restrict__ Rx,
+                                     const float *__restrict__ Rz, const float *__restrict__ Vp,
+                                     const float *__restrict__ Rho, int nx, int nz, int lpml,
+                                     float dx, float dz, float *__restrict__ vpgrad,
+                                     float *__restrict__ rhograd) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int iz = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (ix >= nx - 1 || iz >= nz - 1) return;
+    const int nxp = nx + 2 * lpml;
+    const size_t m = (size_t)iz * nx + ix;
+    const size_t q = (size_t)(iz + lpml) * nxp + ix + lpml;
+    const float vpscale = (float)(-2.0 / (double)Vp[m]);
+    const float rhoscale = (float)(-1.0 / (double)Rho[m]);
+    const float mrxx = (Vxa[q] - Vxa[q - 1]) / dx;
+    const float mrzz = (Vza[q] - Vza[q - nxp]) / dz;
+    const float ws = Pf[q];
+    float vg = vpgrad[m], rg = rhograd[m];
+    vg -= vpscale * ws * (mrxx + mrzz);
+    rg -= rhoscale * ws * (mrxx + mrzz);
+    float ux = (float)(0.5 * (double)Vxa[q] * (double)Rx[q] * (double)(Pf[q + 1] - ws) / (double)dx);
+    ux = (float)((double)ux + 0.5 * (double)Vxa[q - 1] * (double)Rx[q - 1] * (double)(ws - Pf[q - 1]) / (double)dx);
+    float uz = (float)(0.5 * (double)Vza[q] * (double)Rz[q] * (double)(Pf[q + nxp] - ws) / (double)dz);
+    uz = (float)((double)uz + 0.5 * (double)Vza[q - nxp] * (double)Rz[q - nxp] * (double)(ws - Pf[q - nxp]) / (double)dz);
+    rg += ux + uz;
+    vpgrad[m] = vg;
+    rhograd[m] = rg;
+}
+
+// DIFFERENCE misfit (fwi.cpp:574-608) and residuals (fwi.cpp:772-800).  The
+// misfit is reduced in double; res = scaling*(mod - rec).
+__global__ void misfit_residual_kernel(const float *__restrict__ mod, const float *__restrict__ rec,
+                                       float *__restrict__ res, const int *__restrict__ map,
+                                       int ntr, int nt, int scaling, double *misfit) {
+    __shared__ double part[256];
+    double acc = 0.0;
+    const size_t n = (size_t)ntr * nt;
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+        const float r = mod[p] - rec[p];
+        res[p] = scaling * r;
+        const int tr = (int)(p / nt);
+        if (map[2 * tr] >= 0 && map[2 * tr + 1] >= 0) acc += 0.5 * (double)r * (double)r;
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(misfit, part[0]);
+}
+
+// ---------------------------------------------------------------------------
+// Driver
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct DevBuf {
+    std::vector<void *> ptrs;
+    ~DevBuf() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    int alloc(T **out, size_t n) {
+        void *p = nullptr;
+        if (n == 0) n = 1;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return fail(RSG_ENOMEM, "fwi: device allocation failed");
+        ptrs.push_back(p);
+        *out = (T *)p;
+        return RSG_OK;
+    }
+};
+
+// A WavesAcoustic2D state: P, Vx, Vz and its 8 PML strips, one allocation.
+struct A2DWaves {
+    float *base = nullptr;
+    size_t n = 0, nstrip_x = 0, nstrip_z = 0, total = 0;
+    rsg_state st{};
+    int init(DevBuf &db, const rsg_grid *g, const rsg_fwi_a2d *f, hipStream_t s) {
+        n = (size_t)g->nx * g->nz;
+        nstrip_x = (size_t)g->lpml * g->nz;
+        nstrip_z = (size_t)g->nx * g->lpml;
+        total = 3 * n + 4 * nstrip_x + 4 * nstrip_z;
+        if (int rc = db.alloc(&base, total)) return rc;
+        if (int rc = launch_zero(base, total * sizeof(float), s)) return rc;
+        st = rsg_state{};
+        st.P = base;
+        st.Vx = base + n;
+        st.Vz = base + 2 * n;
+        float *p = base + 3 * n;
+        // strip order of the checkpoint payload: P_left, Vxx_left, P_right,
+        // Vxx_right, P_top, Vzz_top, P_bottom, Vzz_bottom (revolve.cpp:673-700)
+        st.psi[RSG_A2D_P_LEFT] = p;
+        st.psi[RSG_A2D_VXX_LEFT] = p + nstrip_x;
+        st.psi[RSG_A2D_P_RIGHT] = p + 2 * nstrip_x;
+        st.psi[RSG_A2D_VXX_RIGHT] = p + 3 * nstrip_x;
+        p += 4 * nstrip_x;
+        st.psi[RSG_A2D_P_TOP] = p;
+        st.psi[RSG_A2D_VZZ_TOP] = p + nstrip_z;
+        st.psi[RSG_A2D_P_BOTTOM] = p + 2 * nstrip_z;
+        st.psi[RSG_A2D_VZZ_BOTTOM] = p + 3 * nstrip_z;
+        st.L = f->L;
+        st.Rx = f->Rx;
+        st.Rz = f->Rz;
+        st.tab = f->tab;
+        return RSG_OK;
+    }
+};
+
+}  // namespace
+
+static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, hipStream_t st) {
+    const int nt = f->src.nt;
+    if (f->obs.nt != nt || f->obs.field != RSG_PRESSURE || f->src.field != RSG_PRESSURE)
+        return fail(RSG_EINVAL, "fwi: source and data must be PRESSURE traces with equal nt");
+    if (f->nx + 2 * g->lpml != g->nx || f->nz + 2 * g->lpml != g->nz)
+        return fail(RSG_EINVAL, "fwi: image size must be the unpadded grid");
+    DevBuf db;
+    A2DWaves fw, bw;
+    if (int rc = fw.init(db, g, f, st)) return rc;
+    if (int rc = bw.init(db, g, f, st)) return rc;
+    double *misfit_d = nullptr;
+    if (int rc = db.alloc(&misfit_d, 1)) return rc;
+    if (int rc = launch_zero(misfit_d, sizeof(double), st)) return rc;
+    if (int rc = launch_zero(f->dmod, sizeof(float) * f->obs.ntrace * nt, st)) return rc;
+    if (f->wavgrad)
+        if (int rc = launch_zero(f->wavgrad, sizeof(float) * f->src.ntrace * nt, st)) return rc;
+
+    rsg_traces mod = f->obs, res = f->obs, wg = f->src;
+    mod.data = f->dmod;
+    res.data = f->dres;
+    wg.data = f->wavgrad;
+    const dim3 xb(64, 4), xg((f->nx + 63) / 64, (f->nz + 3) / 4);
+
+    auto fstep = [&](int it, bool rec) -> int {
+        if (int rc = rsg_a2d_forward_velocity(g, &fw.st, st)) return rc;
+        if (int rc = rsg_a2d_forward_stress(g, &fw.st, st)) return rc;
+        if (int rc = insert_source(RSG_KIND_A2D, g, &fw.st, &f->src, nullptr, it, st)) return rc;
+        if (rec)
+            if (int rc = record_data(RSG_KIND_A2D, g, &fw.st, &mod, nullptr, it, st)) return rc;
+        return RSG_OK;
+    };
+    auto first_turn = [&]() -> int {
+        const int threads = 256, blocks = 1024;
+        misfit_residual_kernel<<<blocks, threads, 0, st>>>(f->dmod, f->obs.data, f->dres, f->obs.map,
+                                                           f->obs.ntrace, nt, 1, misfit_d);
+        RSG_LAUNCH_CHECK("misfit_residual_kernel");
+        return RSG_OK;
+    };
+    auto reverse_tail = [&](int capo, const float *Pf) -> int {
+        if (int rc = insert_source(RSG_KIND_A2D, g, &bw.st, &res, nullptr, capo, st)) return rc;
+        crosscorr_a2d_kernel<<<xg, xb, 0, st>>>(Pf, bw.st.Vx, bw.st.Vz, f->Rx, f->Rz, f->Vp, f->Rho, f->nx,
+                                                f->nz, g->lpml, g->dx, g->dz, f->vpgrad, f->rhograd);
+        RSG_LAUNCH_CHECK("crosscorr_a2d_kernel");
+        if (f->wavgrad)
+            if (int rc = record_data(RSG_KIND_A2D, g, &bw.st, &wg, nullptr, capo, st)) return rc;
+        return RSG_OK;
+    };
+
+    if (f->mode == RSG_FWI_STORE) {
+        // forward pressure after every step, slot k = state after step k
+        float *hist = nullptr;
+        if (int rc = db.alloc(&hist, fw.n * (size_t)nt)) return rc;
+        for (int it = 0; it < nt; it++) {
+            if (int rc = fstep(it, true)) return rc;
+            RSG_HIP(hipMemcpyAsync(hist + fw.n * it, fw.st.P, fw.n * sizeof(float), hipMemcpyDeviceToDevice, st));
+        }
+        if (int rc = first_turn()) return rc;
+        if (int rc = reverse_tail(nt - 1, hist + fw.n * (nt - 1))) return rc;
+        for (int capo = nt - 2; capo >= 0; capo--) {
+            if (int rc = rsg_a2d_forward_velocity(g, &bw.st, st)) return rc;
+            if (int rc = rsg_a2d_forward_stress(g, &bw.st, st)) return rc;
+            // state after capo-1 steps; before the first step it is zero
+            const float *Pf;
+            if (capo >= 1) {
+                Pf = hist + fw.n * (capo - 1);
+            } else {
+                if (int rc = launch_zero(fw.st.P, fw.n * sizeof(float), st)) return rc;
+                Pf = fw.st.P;
+            }
+            if (int rc = reverse_tail(capo, Pf)) return rc;
+        }
+    } else {
+        Revolve rev(nt, f->ncheck);
+        float *ck = nullptr;
+        if (int rc = db.alloc(&ck, fw.total * (size_t)rev.snaps)) return rc;
+        bool reverse = false;
+        for (;;) {
+            const int oldcapo = rev.capo;
+            const Revolve::Action a = rev.next();
+            const int capo = rev.capo;
+            if (a == Revolve::ADVANCE) {
+                for (int it = oldcapo; it < capo; it++)
+                    if (int rc = fstep(it, !reverse)) return rc;
+            } else if (a == Revolve::FIRSTURN) {
+                if (int rc = fstep(capo, true)) return rc;
+                if (int rc = first_turn()) return rc;
+                if (int rc = reverse_tail(capo, fw.st.P)) return rc;
+                reverse = true;
+            } else if (a == Revolve::YOUTURN) {
+                if (int rc = rsg_a2d_forward_velocity(g, &bw.st, st)) return rc;
+                if (int rc = rsg_a2d_forward_stress(g, &bw.st, st)) return rc;
+                if (int rc = reverse_tail(capo, fw.st.P)) return rc;
+            } else if (a == Revolve::TAKESHOT) {
+                RSG_HIP(hipMemcpyAsync(ck + fw.total * rev.check, fw.base, fw.total * sizeof(float),
+                                       hipMemcpyDeviceToDevice, st));
+            } else if (a == Revolve::RESTORE) {
+                RSG_HIP(hipMemcpyAsync(fw.base, ck + fw.total * rev.check, fw.total * sizeof(float),
+                                       hipMemcpyDeviceToDevice, st));
+            } else if (a == Revolve::TERMINATE) {
+                break;
+            } else {
+                return fail(RSG_EINVAL, "fwi: Revolve returned an error");
+            }
+        }
+    }
+    double m = 0.0;
+    RSG_HIP(hipMemcpyAsync(&m, misfit_d, sizeof(double), hipMemcpyDeviceToHost, st));
+    RSG_HIP(hipStreamSynchronize(st));
+    if (f->misfit) *f->misfit = m;
+    return RSG_OK;
+}
+
+}  // namespace rsg
+
+extern "C" int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, void *stream) {
+    if (!g || !f || !f->L || !f->Rx || !f->Rz || !f->tab || !f->Vp || !f->Rho || !f->vpgrad ||
+        !f->rhograd || !f->dmod || !f->dres || !f->obs.data)
+        return rsg::fail(RSG_EINVAL, "rsg_fwi_a2d_gradient: null argument");
+    if (f->mode != RSG_FWI_REVOLVE && f->mode != RSG_FWI_STORE)
+        return rsg::fail(RSG_EINVAL, "rsg_fwi_a2d_gradient: unknown mode");
+    return rsg::fwi_a2d_impl(g, f, (hipStream_t)stream);
+}
+
+extern "C" int rsg_revolve_schedule(int nt, int snaps, int *actions, int *capos, int *checks, int maxn,
+                                    int *n) {
+    if (!actions || !capos || !checks || !n || nt < 1) return rsg::fail(RSG_EINVAL, "bad arguments");
+    rsg::Revolve r(nt, snaps);
+    int k = 0;
+    for (;;) {
+        const rsg::Revolve::Action a = r.next();
+        if (k >= maxn) return rsg::fail(RSG_EINVAL, "rsg_revolve_schedule: maxn too small");
+        actions[k] = (int)a;
+        capos[k] = r.capo;
+        checks[k] = r.check;
+        k++;
+        if (a == rsg::Revolve::TERMINATE || a == rsg::Revolve::ERROR) break;
+    }
+    *n = k;
+    return RSG_OK;
+}
+
+extern "C" int rsg_revolve_numforw(int nt, int snaps) { return rsg::Revolve::numforw(nt, snaps); }

@@ -143,6 +143,30 @@ def run_fixtures(R):
              grid=np.float32([nx3, ny3, nz3, l3, 10.0, dt, nt3, order]))
 
 
+FWI_CASES = [  # order, fs, ncheck
+    (4, 0, 20), (8, 1, 7),
+]
+
+
+def fwi_fixtures(R):
+    from oracle import fwi
+    nx, nz, lpml, dx, dt, nt = 60, 50, 10, 10.0, 1e-3, 300
+    vtrue = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1800.0, 3000.0, nanom=5, seed=3)
+    rtrue = synth.gardner(vtrue)
+    vp0 = synth.gaussian_smooth(vtrue, 60.0, dx, dx)
+    rho0 = synth.gardner(vp0)
+    sc = np.float32([[300.0, 10.0]])
+    wav = synth.ricker(nt, dt, 15.0)[None]
+    gc = synth.surface_line(nx, dx, 20.0)
+    for k, (order, fs, ncheck) in enumerate(FWI_CASES):
+        obs = runs.ref_a2d_run(vtrue, rtrue, lpml, dx, dx, dt, nt, order, fs, "P", sc, wav, "P", gc)["rec"]
+        r = fwi.ref_fwi_a2d(vp0, rho0, lpml, dx, dx, dt, nt, order, fs, ncheck, sc, wav, gc, obs)
+        save(f"fwi_case{k}", vp=vp0, rho=rho0, scoords=sc, wav=wav, gcoords=gc, dobs=obs,
+             grid=np.float32([nx, nz, lpml, dx, dt, nt]), misfit=np.float32(r["misfit"]),
+             vpgrad=r["vpgrad"], rhograd=r["rhograd"], dmod=r["dmod"], dres=r["dres"],
+             wavgrad=r["wavgrad"])
+
+
 REVOLVE_CASES = [(10, 3), (100, 5), (7, 2), (50, 60), (1, 2), (257, 4), (3000, 50), (2500, 50), (4000, 50)]
 
 
@@ -168,3 +192,4 @@ if __name__ == "__main__":
     makemap_fixture(R)
     run_fixtures(R)
     revolve_fixture(R)
+    fwi_fixtures(R)

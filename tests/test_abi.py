@@ -120,3 +120,19 @@ def test_makemap3d_against_oracle():
         mo, so = runs.makemap3d(c, (22, 24, 20), (10.0, 10.0, 10.0), name)
         assert np.array_equal(m, mo)
         assert np.array_equal(s[m >= 0], so[mo >= 0])
+
+
+def test_revolve_schedule_matches_reference(golden):
+    from tests.golden.make_golden import REVOLVE_CASES
+    g = golden("revolve")
+    L = R.lib()
+    for nt, sn in REVOLVE_CASES:
+        ref = g[f"n{nt}_s{sn}"]
+        a = np.zeros(ref.shape[0] + 8, np.int32)
+        c = a.copy()
+        k = a.copy()
+        n = C.c_int()
+        R.check(L.rsg_revolve_schedule(nt, sn, _p(a), _p(c), _p(k), a.size, C.byref(n)))
+        got = np.stack([a[:n.value], c[:n.value], k[:n.value]], 1)
+        assert np.array_equal(got, ref), (nt, sn)
+        assert L.rsg_revolve_numforw(nt, sn) == int(g[f"n{nt}_s{sn}_numforw"])

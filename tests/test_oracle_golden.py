@@ -102,3 +102,19 @@ def test_revolve(golden):
         assert orv.numforw(nt, sn) == int(g[f"n{nt}_s{sn}_numforw"])
     for nt in (10, 100, 1000, 3000):
         assert orv.adjust(nt) == int(g[f"adjust_{nt}"])
+
+
+@pytest.mark.parametrize("case", range(2))
+def test_fwi_a2d_gradient(golden, case):
+    """FwiAcoustic2D::run_optimal restated (oracle/fwi.py) vs the compiled
+    reference's gradient, modelled data, residuals, wavelet gradient, misfit."""
+    from oracle import fwi
+    from tests.golden.make_golden import FWI_CASES
+    g = golden(f"fwi_case{case}")
+    order, fs, ncheck = FWI_CASES[case]
+    nx, nz, lpml, dx, dt, nt = g["grid"]
+    r = fwi.fwi_a2d(g["vp"], g["rho"], int(lpml), dx, dx, dt, int(nt), order, fs, ncheck,
+                    g["scoords"], g["wav"], g["gcoords"], g["dobs"])
+    for k in ("vpgrad", "rhograd", "dmod", "dres", "wavgrad"):
+        assert np.array_equal(r[k], g[k]), k
+    assert np.float32(r["misfit"]) == g["misfit"]
