@@ -36,7 +36,7 @@ def _run(g, case, mode):
     f.setWavgrad(wgr)
     f.setNcheck(ncheck)
     (f.run_optimal if mode == "revolve" else f.run)()
-    return dict(misfit=f.getMisfit(), vpgrad=vg.imagedata, rhograd=rg.imagedata, dmod=mod.data,
+    return dict(misfit=f.getMisfit(), misfit64=f.misfit64, vpgrad=vg.imagedata, rhograd=rg.imagedata, dmod=mod.data,
                 dres=res.data, wavgrad=wgr.data)
 
 
@@ -49,7 +49,13 @@ def test_fwi_a2d_vs_reference(golden, case):
         e = rel_l2(r[k], g[k])
         print(f"  {k}: rel L2 {e:.2e}")
         assert e <= RTOL_L2, k
-    assert abs(r["misfit"] - float(g["misfit"])) <= RTOL_L2 * float(g["misfit"])
+    # The reference sums 0.5*r^2 into a float (fwi.cpp:597), which drifts by
+    # ~1e-5 relative over 18000 samples; the device reduces in double.  So the
+    # device misfit is checked against the double sum of the reference's own
+    # residuals, and against the reference's float total at 1e-4.
+    exact = 0.5 * np.sum((g["dmod"].astype(np.float64) - g["dobs"].astype(np.float64)) ** 2)
+    assert abs(r["misfit64"] - exact) <= RTOL_L2 * exact
+    assert abs(r["misfit"] - float(g["misfit"])) <= 1e-4 * float(g["misfit"])
     s = _run(g, case, "store")
     for k in ("vpgrad", "rhograd", "dmod", "dres", "wavgrad"):
         assert np.array_equal(s[k], r[k]), k
@@ -67,6 +73,7 @@ def test_autograd_function_matches_driver(golden):
     rho = torch.tensor(g["rho"], device="cuda", requires_grad=True)
     f = fwi.misfit(vp, rho, survey)
     f.backward()
-    assert abs(float(f) - float(g["misfit"])) <= RTOL_L2 * float(g["misfit"])
+    exact = 0.5 * np.sum((g["dmod"].astype(np.float64) - g["dobs"].astype(np.float64)) ** 2)
+    assert abs(float(f) - exact) <= RTOL_L2 * exact
     assert rel_l2(vp.grad.cpu().numpy(), g["vpgrad"]) <= RTOL_L2
     assert rel_l2(rho.grad.cpu().numpy(), g["rhograd"]) <= RTOL_L2
