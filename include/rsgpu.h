@@ -152,6 +152,26 @@ int rsg_a3d_forward_stress(const rsg_grid *g, const rsg_state *s, void *stream);
 int rsg_repeat_step(int kind, int which, const rsg_grid *g, const rsg_state *s, int n,
                     void *stream);
 
+/* Many shots on one grid in one launch: every field and strip of shot k sits
+ * at k * stride from shot 0's pointer in rsg_state; model arrays and PML tables
+ * are shared.  No counterpart in the reference, which runs one shot per MPI
+ * rank (parallel.cpp:690-732); results per shot equal the single-shot calls. */
+typedef struct {
+    int nshot;
+    size_t field_stride;  /* elements between shots of P/Vx/Vz/Sxx/Szz/Sxz   */
+    size_t xstrip_stride; /* between shots of an x-side strip (lpml*nz)      */
+    size_t zstrip_stride; /* between shots of a z-side strip (nx*lpml)       */
+} rsg_batch;
+int rsg_a2d_forward_velocity_batch(const rsg_grid *g, const rsg_state *s, const rsg_batch *b,
+                                   void *stream);
+int rsg_a2d_forward_stress_batch(const rsg_grid *g, const rsg_state *s, const rsg_batch *b,
+                                 void *stream);
+/* adjoint = 0: forwardstep*, 1: backwardstep* */
+int rsg_e2d_velocity_batch(const rsg_grid *g, const rsg_state *s, const rsg_batch *b, int adjoint,
+                           void *stream);
+int rsg_e2d_stress_batch(const rsg_grid *g, const rsg_state *s, const rsg_batch *b, int adjoint,
+                         void *stream);
+
 /* insertSource (waves.cpp:407-497, 998-1137, 2085-2171): kind is
  * 1 acoustic 2D, 2 elastic 2D, 3 acoustic 3D; the field and its modulus are
  * chosen from src->field as the reference does. */
@@ -188,6 +208,12 @@ typedef struct {
 } rsg_fwi_a2d;
 
 int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, void *stream);
+/* nshot shots on one grid in one set of launches per step (the reference
+ * runs one shot per MPI worker, inversion.cpp:332-600, and stacks the shot
+ * gradients, image.cpp:533-600): traces of shot k follow shot k-1 in
+ * src/obs (map, shift, data), dmod/dres/wavgrad are [nshot][ntrace][nt],
+ * vpgrad/rhograd and the misfit are summed over the shots. */
+int rsg_fwi_a2d_gradient_batch(const rsg_grid *g, const rsg_fwi_a2d *f, int nshot, void *stream);
 /* Revolve::revolve() action sequence (revolve.cpp:150-305): advance=0,
  * takeshot=1, restore=2, firsturn=3, youturn=4, terminate=5, error=6. */
 int rsg_revolve_schedule(int nt, int snaps, int *actions, int *capos, int *checks, int maxn,

@@ -41,6 +41,11 @@ class Traces(C.Structure):
                 ("shift", C.c_void_p), ("data", C.c_void_p), ("factor", C.c_void_p)]
 
 
+class Batch(C.Structure):
+    _fields_ = [("nshot", C.c_int), ("field_stride", C.c_size_t), ("xstrip_stride", C.c_size_t),
+                ("zstrip_stride", C.c_size_t)]
+
+
 class Fwi(C.Structure):
     _fields_ = ([(n, C.c_void_p) for n in ("L", "Rx", "Rz", "tab", "Vp", "Rho")]
                 + [("nx", C.c_int), ("nz", C.c_int), ("src", Traces), ("obs", Traces)]
@@ -96,6 +101,11 @@ def _declare(L):
         "rsg_repeat_step": ([ip, ip, GP, SP, ip, vp], ip),
         "rsg_set_tuning": ([C.c_char_p, ip], ip),
         "rsg_fwi_a2d_gradient": ([GP, C.POINTER(Fwi), vp], ip),
+        "rsg_fwi_a2d_gradient_batch": ([GP, C.POINTER(Fwi), ip, vp], ip),
+        "rsg_a2d_forward_velocity_batch": ([GP, SP, C.POINTER(Batch), vp], ip),
+        "rsg_a2d_forward_stress_batch": ([GP, SP, C.POINTER(Batch), vp], ip),
+        "rsg_e2d_velocity_batch": ([GP, SP, C.POINTER(Batch), ip, vp], ip),
+        "rsg_e2d_stress_batch": ([GP, SP, C.POINTER(Batch), ip, vp], ip),
         "rsg_revolve_schedule": ([ip, ip, vp, vp, vp, ip, C.POINTER(C.c_int)], ip),
         "rsg_revolve_numforw": ([ip, ip], ip),
         "rsg_get_tuning": ([C.c_char_p, C.POINTER(C.c_int)], ip),
@@ -120,7 +130,8 @@ EXPORTS = (
     "rsg_e2d_backward_stress", "rsg_a3d_forward_velocity", "rsg_a3d_forward_stress",
     "rsg_insert_source", "rsg_record_data", "rsg_modelling_run", "rsg_repeat_step",
     "rsg_set_tuning", "rsg_get_tuning", "rsg_fwi_a2d_gradient", "rsg_revolve_schedule",
-    "rsg_revolve_numforw",
+    "rsg_revolve_numforw", "rsg_fwi_a2d_gradient_batch", "rsg_a2d_forward_velocity_batch",
+    "rsg_a2d_forward_stress_batch", "rsg_e2d_velocity_batch", "rsg_e2d_stress_batch",
 )
 
 

@@ -29,6 +29,7 @@ constexpr int BZ = 4;
 struct K2D {
     int nx, nz, lpml, fs;
     int ntx, ntiles;  // tiles along x, total tiles
+    long sf, sxs, szs;  // per-shot strides of fields, x-side and z-side strips (blockIdx.y = shot)
     float dt, invdx, invdz;
     const float *tab;
     float *P, *Vx, *Vz, *Sxx, *Szz, *Sxz;
@@ -200,8 +201,10 @@ __device__ __forceinline__ void a2d_velocity_body(const K2D &a, int x0, int z0, 
     constexpr int H = ORDER, TZ = BZ * RZ, LD = BX + 2 * H;
     const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
     const bool xin = FAST || x < nx;
-    float *__restrict__ Vx = a.Vx;
-    float *__restrict__ Vz = a.Vz;
+    const size_t fo = blockIdx.y * (size_t)a.sf, xo = blockIdx.y * (size_t)a.sxs,
+                 zo = blockIdx.y * (size_t)a.szs;
+    float *__restrict__ Vx = a.Vx + fo;
+    float *__restrict__ Vz = a.Vz + fo;
     const float *__restrict__ Rx = a.Rx;
     const float *__restrict__ Rz = a.Rz;
     float vx[RZ], vz[RZ], rx[RZ], rz[RZ];
@@ -216,7 +219,7 @@ __device__ __forceinline__ void a2d_velocity_body(const K2D &a, int x0, int z0, 
         rz[r] = in ? Rz[p] : 0.f;
     }
     Tile<TZ, H, H> tp;
-    tile_fetch<TZ, H, H, FAST>(tp, Ld1{a.P}, nx, nz, x0, z0);
+    tile_fetch<TZ, H, H, FAST>(tp, Ld1{a.P + fo}, nx, nz, x0, z0);
     tile_store(tp, sP);
     __syncthreads();
     if (!xin) return;
@@ -232,10 +235,10 @@ __device__ __forceinline__ void a2d_velocity_body(const K2D &a, int x0, int z0, 
         const float dPz = ddfw<ORDER, FAST>(s, LD, a.cz.c, FAST ? ORDER : kfw(z, nz, ORDER), a.invdz);
         const size_t p = (size_t)z * nx + x;
         const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
-        term<FAST>(vx[r], a.dt * rx[r], dPx, px, tb, 1, a.psi[RSG_A2D_P_LEFT],
-                   a.psi[RSG_A2D_P_RIGHT], (size_t)z * a.lpml + px.i);
-        term<FAST>(vz[r], a.dt * rz[r], dPz, pz, tb, 1, a.psi[RSG_A2D_P_TOP],
-                   a.psi[RSG_A2D_P_BOTTOM], (size_t)pz.i * nx + x);
+        term<FAST>(vx[r], a.dt * rx[r], dPx, px, tb, 1, (a.psi[RSG_A2D_P_LEFT] + xo),
+                   (a.psi[RSG_A2D_P_RIGHT] + xo), (size_t)z * a.lpml + px.i);
+        term<FAST>(vz[r], a.dt * rz[r], dPz, pz, tb, 1, (a.psi[RSG_A2D_P_TOP] + zo),
+                   (a.psi[RSG_A2D_P_BOTTOM] + zo), (size_t)pz.i * nx + x);
         Vx[p] = vx[r];
         Vz[p] = vz[r];
     }
@@ -258,7 +261,9 @@ __device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, fl
     constexpr int H = ORDER, LDX = BX + 2 * H, LDZ = BX, TZ = BZ * RZ;
     const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
     const bool xin = FAST || x < nx;
-    float *__restrict__ P = a.P;
+    const size_t fo = blockIdx.y * (size_t)a.sf, xo = blockIdx.y * (size_t)a.sxs,
+                 zo = blockIdx.y * (size_t)a.szs;
+    float *__restrict__ P = a.P + fo;
     const float *__restrict__ L = a.L;
     float pv[RZ], lv[RZ];
 #pragma unroll
@@ -271,8 +276,8 @@ __device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, fl
     }
     Tile<TZ, H, 0> tvx;
     Tile<TZ, 0, H> tvz;
-    tile_fetch<TZ, H, 0, FAST>(tvx, Ld1{a.Vx}, nx, nz, x0, z0);
-    tile_fetch<TZ, 0, H, FAST>(tvz, Ld1{a.Vz}, nx, nz, x0, z0);
+    tile_fetch<TZ, H, 0, FAST>(tvx, Ld1{a.Vx + fo}, nx, nz, x0, z0);
+    tile_fetch<TZ, 0, H, FAST>(tvz, Ld1{a.Vz + fo}, nx, nz, x0, z0);
     tile_store(tvx, sVx);
     tile_store(tvz, sVz);
     __syncthreads();
@@ -289,9 +294,9 @@ __device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, fl
                                             FAST ? ORDER : kbw(z, nz, ORDER), a.invdz);
         const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
         const float dtL = a.dt * lv[r];
-        term<FAST>(pv[r], dtL, dVx, px, tb, 0, a.psi[RSG_A2D_VXX_LEFT], a.psi[RSG_A2D_VXX_RIGHT],
+        term<FAST>(pv[r], dtL, dVx, px, tb, 0, (a.psi[RSG_A2D_VXX_LEFT] + xo), (a.psi[RSG_A2D_VXX_RIGHT] + xo),
                    (size_t)z * a.lpml + px.i);
-        term<FAST>(pv[r], dtL, dVz, pz, tb, 0, a.psi[RSG_A2D_VZZ_TOP], a.psi[RSG_A2D_VZZ_BOTTOM],
+        term<FAST>(pv[r], dtL, dVz, pz, tb, 0, (a.psi[RSG_A2D_VZZ_TOP] + zo), (a.psi[RSG_A2D_VZZ_BOTTOM] + zo),
                    (size_t)pz.i * nx + x);
         P[(size_t)z * nx + x] = pv[r];
     }
@@ -326,8 +331,10 @@ __device__ __forceinline__ void e2d_velocity_body(const K2D &a, int x0, int z0, 
     constexpr int LDXX = BX + 2 * H, LDZZ = BX, LDXZ = BX + 2 * H;
     const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
     const bool xin = FAST || x < nx;
-    float *__restrict__ Vx = a.Vx;
-    float *__restrict__ Vz = a.Vz;
+    const size_t fo = blockIdx.y * (size_t)a.sf, xo = blockIdx.y * (size_t)a.sxs,
+                 zo = blockIdx.y * (size_t)a.szs;
+    float *__restrict__ Vx = a.Vx + fo;
+    float *__restrict__ Vz = a.Vz + fo;
     const float *__restrict__ Rx = a.Rx;
     const float *__restrict__ Rz = a.Rz;
     float vx[RZ], vz[RZ], rx[RZ], rz[RZ];
@@ -345,13 +352,13 @@ __device__ __forceinline__ void e2d_velocity_body(const K2D &a, int x0, int z0, 
     Tile<TZ, 0, H> tzz;
     Tile<TZ, H, H> txz;
     if (!BW) {
-        tile_fetch<TZ, H, 0, FAST>(txx, Ld1{a.Sxx}, nx, nz, x0, z0);
-        tile_fetch<TZ, 0, H, FAST>(tzz, Ld1{a.Szz}, nx, nz, x0, z0);
-        tile_fetch<TZ, H, H, FAST>(txz, Ld1{a.Sxz}, nx, nz, x0, z0);
+        tile_fetch<TZ, H, 0, FAST>(txx, Ld1{a.Sxx + fo}, nx, nz, x0, z0);
+        tile_fetch<TZ, 0, H, FAST>(tzz, Ld1{a.Szz + fo}, nx, nz, x0, z0);
+        tile_fetch<TZ, H, H, FAST>(txz, Ld1{a.Sxz + fo}, nx, nz, x0, z0);
     } else {
-        tile_fetch<TZ, H, 0, FAST>(txx, Ld4{a.L2M, a.Sxx, a.L, a.Szz}, nx, nz, x0, z0);
-        tile_fetch<TZ, 0, H, FAST>(tzz, Ld4{a.L, a.Sxx, a.L2M, a.Szz}, nx, nz, x0, z0);
-        tile_fetch<TZ, H, H, FAST>(txz, Ld2{a.M, a.Sxz}, nx, nz, x0, z0);
+        tile_fetch<TZ, H, 0, FAST>(txx, Ld4{a.L2M, a.Sxx + fo, a.L, a.Szz + fo}, nx, nz, x0, z0);
+        tile_fetch<TZ, 0, H, FAST>(tzz, Ld4{a.L, a.Sxx + fo, a.L2M, a.Szz + fo}, nx, nz, x0, z0);
+        tile_fetch<TZ, H, H, FAST>(txz, Ld2{a.M, a.Sxz + fo}, nx, nz, x0, z0);
     }
     tile_store(txx, sXX);
     tile_store(tzz, sZZ);
@@ -376,14 +383,14 @@ __device__ __forceinline__ void e2d_velocity_body(const K2D &a, int x0, int z0, 
         const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
         const size_t ix = (size_t)z * a.lpml + px.i, iz = (size_t)pz.i * nx + x;
         const float dtRx = a.dt * rx[r];
-        term<FAST>(vx[r], dtRx, dSxx, px, tb, 1, a.psi[RSG_E2D_SXX_LEFT], a.psi[RSG_E2D_SXX_RIGHT], ix);
-        term<FAST>(vx[r], dtRx, dSxz_z, pz, tb, 0, a.psi[RSG_E2D_SXZZ_TOP],
-                   a.psi[RSG_E2D_SXZZ_BOTTOM], iz);
+        term<FAST>(vx[r], dtRx, dSxx, px, tb, 1, (a.psi[RSG_E2D_SXX_LEFT] + xo), (a.psi[RSG_E2D_SXX_RIGHT] + xo), ix);
+        term<FAST>(vx[r], dtRx, dSxz_z, pz, tb, 0, (a.psi[RSG_E2D_SXZZ_TOP] + zo),
+                   (a.psi[RSG_E2D_SXZZ_BOTTOM] + zo), iz);
         Vx[p] = vx[r];
         const float dtRz = a.dt * rz[r];
-        term<FAST>(vz[r], dtRz, dSxz_x, px, tb, 0, a.psi[RSG_E2D_SXZX_LEFT],
-                   a.psi[RSG_E2D_SXZX_RIGHT], ix);
-        term<FAST>(vz[r], dtRz, dSzz, pz, tb, 1, a.psi[RSG_E2D_SZZ_TOP], a.psi[RSG_E2D_SZZ_BOTTOM], iz);
+        term<FAST>(vz[r], dtRz, dSxz_x, px, tb, 0, (a.psi[RSG_E2D_SXZX_LEFT] + xo),
+                   (a.psi[RSG_E2D_SXZX_RIGHT] + xo), ix);
+        term<FAST>(vz[r], dtRz, dSzz, pz, tb, 1, (a.psi[RSG_E2D_SZZ_TOP] + zo), (a.psi[RSG_E2D_SZZ_BOTTOM] + zo), iz);
         Vz[p] = vz[r];
     }
 }
@@ -412,9 +419,11 @@ __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, fl
     constexpr int H = ORDER, TZ = BZ * RZ, LD = BX + 2 * H;
     const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
     const bool xin = FAST || x < nx;
-    float *__restrict__ Sxx = a.Sxx;
-    float *__restrict__ Szz = a.Szz;
-    float *__restrict__ Sxz = a.Sxz;
+    const size_t fo = blockIdx.y * (size_t)a.sf, xo = blockIdx.y * (size_t)a.sxs,
+                 zo = blockIdx.y * (size_t)a.szs;
+    float *__restrict__ Sxx = a.Sxx + fo;
+    float *__restrict__ Szz = a.Szz + fo;
+    float *__restrict__ Sxz = a.Sxz + fo;
     float sxx[RZ], szz[RZ], sxz[RZ], ml[RZ], ml2m[RZ], mm[RZ];
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
@@ -431,8 +440,8 @@ __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, fl
         }
     }
     Tile<TZ, H, H> tvx, tvz;
-    tile_fetch<TZ, H, H, FAST>(tvx, Ld1{a.Vx}, nx, nz, x0, z0);
-    tile_fetch<TZ, H, H, FAST>(tvz, Ld1{a.Vz}, nx, nz, x0, z0);
+    tile_fetch<TZ, H, H, FAST>(tvx, Ld1{a.Vx + fo}, nx, nz, x0, z0);
+    tile_fetch<TZ, H, H, FAST>(tvz, Ld1{a.Vz + fo}, nx, nz, x0, z0);
     tile_store(tvx, sVx);
     tile_store(tvz, sVz);
     __syncthreads();
@@ -461,7 +470,7 @@ __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, fl
             sxx[r] += dtL2M * dVx_x;
             szz[r] += dtL * dVx_x;
             if (!FAST && px.side >= 0) {
-                float *psi = a.psi[px.side ? RSG_E2D_VXX_RIGHT : RSG_E2D_VXX_LEFT] + ix;
+                float *psi = a.psi[px.side ? RSG_E2D_VXX_RIGHT : RSG_E2D_VXX_LEFT] + xo + ix;
                 const float A = tb.A(px.side, 0, px.i), B = tb.B(px.side, 0, px.i),
                             C = tb.C(px.side, 0, px.i);
                 const float q = B * (*psi) + A * dVx_x;
@@ -472,7 +481,7 @@ __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, fl
             sxx[r] += dtL * dVz_z;
             szz[r] += dtL2M * dVz_z;
             if (!FAST && pz.side >= 0) {
-                float *psi = a.psi[pz.side ? RSG_E2D_VZZ_BOTTOM : RSG_E2D_VZZ_TOP] + iz;
+                float *psi = a.psi[pz.side ? RSG_E2D_VZZ_BOTTOM : RSG_E2D_VZZ_TOP] + zo + iz;
                 const float A = tb.A(pz.side, 0, pz.i), B = tb.B(pz.side, 0, pz.i),
                             C = tb.C(pz.side, 0, pz.i);
                 const float q = B * (*psi) + A * dVz_z;
@@ -482,14 +491,14 @@ __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, fl
             }
             if (!FAST && a.fs && z == a.lpml) szz[r] = 0.f;
             const float dtM = dt * mm[r];
-            term<FAST>(sxz[r], dtM, dVz_x, px, tb, 1, a.psi[RSG_E2D_VZX_LEFT], a.psi[RSG_E2D_VZX_RIGHT], ix);
-            term<FAST>(sxz[r], dtM, dVx_z, pz, tb, 1, a.psi[RSG_E2D_VXZ_TOP], a.psi[RSG_E2D_VXZ_BOTTOM], iz);
+            term<FAST>(sxz[r], dtM, dVz_x, px, tb, 1, (a.psi[RSG_E2D_VZX_LEFT] + xo), (a.psi[RSG_E2D_VZX_RIGHT] + xo), ix);
+            term<FAST>(sxz[r], dtM, dVx_z, pz, tb, 1, (a.psi[RSG_E2D_VXZ_TOP] + zo), (a.psi[RSG_E2D_VXZ_BOTTOM] + zo), iz);
         } else {
-            term<FAST>(sxx[r], dt, dVx_x, px, tb, 0, a.psi[RSG_E2D_VXX_LEFT], a.psi[RSG_E2D_VXX_RIGHT], ix);
-            term<FAST>(szz[r], dt, dVz_z, pz, tb, 0, a.psi[RSG_E2D_VZZ_TOP], a.psi[RSG_E2D_VZZ_BOTTOM], iz);
+            term<FAST>(sxx[r], dt, dVx_x, px, tb, 0, (a.psi[RSG_E2D_VXX_LEFT] + xo), (a.psi[RSG_E2D_VXX_RIGHT] + xo), ix);
+            term<FAST>(szz[r], dt, dVz_z, pz, tb, 0, (a.psi[RSG_E2D_VZZ_TOP] + zo), (a.psi[RSG_E2D_VZZ_BOTTOM] + zo), iz);
             if (!FAST && a.fs && z == a.lpml) szz[r] = 0.f;
-            term<FAST>(sxz[r], dt, dVz_x, px, tb, 1, a.psi[RSG_E2D_VZX_LEFT], a.psi[RSG_E2D_VZX_RIGHT], ix);
-            term<FAST>(sxz[r], dt, dVx_z, pz, tb, 1, a.psi[RSG_E2D_VXZ_TOP], a.psi[RSG_E2D_VXZ_BOTTOM], iz);
+            term<FAST>(sxz[r], dt, dVz_x, px, tb, 1, (a.psi[RSG_E2D_VZX_LEFT] + xo), (a.psi[RSG_E2D_VZX_RIGHT] + xo), ix);
+            term<FAST>(sxz[r], dt, dVx_z, pz, tb, 1, (a.psi[RSG_E2D_VXZ_TOP] + zo), (a.psi[RSG_E2D_VXZ_BOTTOM] + zo), iz);
         }
         Sxx[p] = sxx[r];
         Szz[p] = szz[r];
@@ -534,6 +543,7 @@ static K2D make_k2d(const rsg_grid *g, const rsg_state *s, int tz) {
     a.fs = g->fs;
     a.ntx = (g->nx + BX - 1) / BX;
     a.ntiles = a.ntx * ((g->nz + tz - 1) / tz);
+    a.sf = a.sxs = a.szs = 0;
     a.dt = g->dt;
     a.invdx = 1.0f / g->dx;
     a.invdz = 1.0f / g->dz;
@@ -593,12 +603,24 @@ using namespace rsg;
 
 #define GO(KERNEL, O, RZ)                                                                         \
     do {                                                                                          \
-        const K2D a = make_k2d(g, s, BZ * RZ);                                                    \
-        KERNEL<<<nblocks(a), dim3(BX, BZ), 0, (hipStream_t)stream>>>(a);                          \
+        K2D a = make_k2d(g, s, BZ * RZ);                                                          \
+        a.sf = (long)b->field_stride;                                                             \
+        a.sxs = (long)b->xstrip_stride;                                                           \
+        a.szs = (long)b->zstrip_stride;                                                           \
+        KERNEL<<<dim3(nblocks(a), b->nshot), dim3(BX, BZ), 0, (hipStream_t)stream>>>(a);          \
     } while (0)
 
-extern "C" int rsg_a2d_forward_velocity(const rsg_grid *g, const rsg_state *s, void *stream) {
+static const rsg_batch kOneShot = {1, 0, 0, 0};
+
+static int check_batch(const rsg_batch *b) {
+    if (!b || b->nshot < 1 || b->nshot > 65535) return fail(RSG_EINVAL, "batch: nshot must be 1..65535");
+    return RSG_OK;
+}
+
+extern "C" int rsg_a2d_forward_velocity_batch(const rsg_grid *g, const rsg_state *s, const rsg_batch *b,
+                                              void *stream) {
     if (int rc = check_grid2d(g, s)) return rc;
+    if (int rc = check_batch(b)) return rc;
     if (!s->P || !s->Vx || !s->Vz || !s->Rx || !s->Rz) return fail(RSG_EINVAL, "a2d: null array");
 #define L_(O, RZ) GO((a2d_velocity_kernel<O, RZ>), O, RZ)
     RSG_DISPATCH(L_);
@@ -607,8 +629,10 @@ extern "C" int rsg_a2d_forward_velocity(const rsg_grid *g, const rsg_state *s, v
     return RSG_OK;
 }
 
-extern "C" int rsg_a2d_forward_stress(const rsg_grid *g, const rsg_state *s, void *stream) {
+extern "C" int rsg_a2d_forward_stress_batch(const rsg_grid *g, const rsg_state *s, const rsg_batch *b,
+                                            void *stream) {
     if (int rc = check_grid2d(g, s)) return rc;
+    if (int rc = check_batch(b)) return rc;
     if (!s->P || !s->Vx || !s->Vz || !s->L) return fail(RSG_EINVAL, "a2d: null array");
 #define L_(O, RZ) GO((a2d_stress_kernel<O, RZ>), O, RZ)
     RSG_DISPATCH(L_);
@@ -625,38 +649,55 @@ static int e2d_check(const rsg_grid *g, const rsg_state *s) {
     return RSG_OK;
 }
 
-extern "C" int rsg_e2d_forward_velocity(const rsg_grid *g, const rsg_state *s, void *stream) {
+extern "C" int rsg_e2d_velocity_batch(const rsg_grid *g, const rsg_state *s, const rsg_batch *b,
+                                      int adjoint, void *stream) {
     if (int rc = e2d_check(g, s)) return rc;
-#define L_(O, RZ) GO((e2d_velocity_kernel<O, RZ, false>), O, RZ)
-    RSG_DISPATCH(L_);
+    if (int rc = check_batch(b)) return rc;
+    if (adjoint) {
+#define L_(O, RZ) GO((e2d_velocity_kernel<O, RZ, true>), O, RZ)
+        RSG_DISPATCH(L_);
 #undef L_
+    } else {
+#define L_(O, RZ) GO((e2d_velocity_kernel<O, RZ, false>), O, RZ)
+        RSG_DISPATCH(L_);
+#undef L_
+    }
     RSG_LAUNCH_CHECK("e2d_velocity_kernel");
     return RSG_OK;
 }
 
-extern "C" int rsg_e2d_forward_stress(const rsg_grid *g, const rsg_state *s, void *stream) {
+extern "C" int rsg_e2d_stress_batch(const rsg_grid *g, const rsg_state *s, const rsg_batch *b,
+                                    int adjoint, void *stream) {
     if (int rc = e2d_check(g, s)) return rc;
-#define L_(O, RZ) GO((e2d_stress_kernel<O, RZ, false>), O, RZ)
-    RSG_DISPATCH(L_);
+    if (int rc = check_batch(b)) return rc;
+    if (adjoint) {
+#define L_(O, RZ) GO((e2d_stress_kernel<O, RZ, true>), O, RZ)
+        RSG_DISPATCH(L_);
 #undef L_
+    } else {
+#define L_(O, RZ) GO((e2d_stress_kernel<O, RZ, false>), O, RZ)
+        RSG_DISPATCH(L_);
+#undef L_
+    }
     RSG_LAUNCH_CHECK("e2d_stress_kernel");
     return RSG_OK;
 }
 
-extern "C" int rsg_e2d_backward_velocity(const rsg_grid *g, const rsg_state *s, void *stream) {
-    if (int rc = e2d_check(g, s)) return rc;
-#define L_(O, RZ) GO((e2d_velocity_kernel<O, RZ, true>), O, RZ)
-    RSG_DISPATCH(L_);
-#undef L_
-    RSG_LAUNCH_CHECK("e2d_velocity_kernel(bw)");
-    return RSG_OK;
+extern "C" int rsg_a2d_forward_velocity(const rsg_grid *g, const rsg_state *s, void *stream) {
+    return rsg_a2d_forward_velocity_batch(g, s, &kOneShot, stream);
 }
-
+extern "C" int rsg_a2d_forward_stress(const rsg_grid *g, const rsg_state *s, void *stream) {
+    return rsg_a2d_forward_stress_batch(g, s, &kOneShot, stream);
+}
+extern "C" int rsg_e2d_forward_velocity(const rsg_grid *g, const rsg_state *s, void *stream) {
+    return rsg_e2d_velocity_batch(g, s, &kOneShot, 0, stream);
+}
+extern "C" int rsg_e2d_forward_stress(const rsg_grid *g, const rsg_state *s, void *stream) {
+    return rsg_e2d_stress_batch(g, s, &kOneShot, 0, stream);
+}
+extern "C" int rsg_e2d_backward_velocity(const rsg_grid *g, const rsg_state *s, void *stream) {
+    return rsg_e2d_velocity_batch(g, s, &kOneShot, 1, stream);
+}
 extern "C" int rsg_e2d_backward_stress(const rsg_grid *g, const rsg_state *s, void *stream) {
-    if (int rc = e2d_check(g, s)) return rc;
-#define L_(O, RZ) GO((e2d_stress_kernel<O, RZ, true>), O, RZ)
-    RSG_DISPATCH(L_);
-#undef L_
-    RSG_LAUNCH_CHECK("e2d_stress_kernel(bw)");
-    return RSG_OK;
+    return rsg_e2d_stress_batch(g, s, &kOneShot, 1, stream);
 }

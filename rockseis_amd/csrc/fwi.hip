@@ -148,13 +148,14 @@ Revolve::Action Revolve::next() {
 // crossCorr for one reverse step, interior ix, iz in [1, n-1) of the
 // unpadded nx x nz image (fwi.cpp:550-569); P_f, Vx_a, Vz_a, Rx, Rz padded
 // by lpml.  The rho-gradient velocity terms keep the reference's double
-// temporaries.
+// temporaries.  A batch of nshot shots (fields fstride apart) is summed into
+// the one image, shot 0 first.
 __global__ void crosscorr_a2d_kernel(const float *__restrict__ Pf, const float *__restrict__ Vxa,
                                      const float *__restrict__ Vza, const float *__restrict__ Rx,
                                      const float *__restrict__ Rz, const float *__restrict__ Vp,
                                      const float *__restrict__ Rho, int nx, int nz, int lpml,
                                      float dx, float dz, float *__restrict__ vpgrad,
-                                     float *__restrict__ rhograd) {
+                                     float *__restrict__ rhograd, int nshot, size_t fstride) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int iz = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (ix >= nx - 1 || iz >= nz - 1) return;
@@ -163,17 +164,21 @@ __global__ void crosscorr_a2d_kernel(const float *__restrict__ Pf, const float *
     const size_t q = (size_t)(iz + lpml) * nxp + ix + lpml;
     const float vpscale = (float)(-2.0 / (double)Vp[m]);
     const float rhoscale = (float)(-1.0 / (double)Rho[m]);
-    const float mrxx = (Vxa[q] - Vxa[q - 1]) / dx;
-    const float mrzz = (Vza[q] - Vza[q - nxp]) / dz;
-    const float ws = Pf[q];
+    const double rx0 = Rx[q], rx1 = Rx[q - 1], rz0 = Rz[q], rz1 = Rz[q - nxp];
     float vg = vpgrad[m], rg = rhograd[m];
-    vg -= vpscale * ws * (mrxx + mrzz);
-    rg -= rhoscale * ws * (mrxx + mrzz);
-    float ux = (float)(0.5 * (double)Vxa[q] * (double)Rx[q] * (double)(Pf[q + 1] - ws) / (double)dx);
-    ux = (float)((double)ux + 0.5 * (double)Vxa[q - 1] * (double)Rx[q - 1] * (double)(ws - Pf[q - 1]) / (double)dx);
-    float uz = (float)(0.5 * (double)Vza[q] * (double)Rz[q] * (double)(Pf[q + nxp] - ws) / (double)dz);
-    uz = (float)((double)uz + 0.5 * (double)Vza[q - nxp] * (double)Rz[q - nxp] * (double)(ws - Pf[q - nxp]) / (double)dz);
-    rg += ux + uz;
+    for (int k = 0; k < nshot; k++) {
+        const float *P = Pf + k * fstride, *Vx = Vxa + k * fstride, *Vz = Vza + k * fstride;
+        const float mrxx = (Vx[q] - Vx[q - 1]) / dx;
+        const float mrzz = (Vz[q] - Vz[q - nxp]) / dz;
+        const float ws = P[q];
+        vg -= vpscale * ws * (mrxx + mrzz);
+        rg -= rhoscale * ws * (mrxx + mrzz);
+        float ux = (float)(0.5 * (double)Vx[q] * rx0 * (double)(P[q + 1] - ws) / (double)dx);
+        ux = (float)((double)ux + 0.5 * (double)Vx[q - 1] * rx1 * (double)(ws - P[q - 1]) / (double)dx);
+        float uz = (float)(0.5 * (double)Vz[q] * rz0 * (double)(P[q + nxp] - ws) / (double)dz);
+        uz = (float)((double)uz + 0.5 * (double)Vz[q - nxp] * rz1 * (double)(ws - P[q - nxp]) / (double)dz);
+        rg += ux + uz;
+    }
     vpgrad[m] = vg;
     rhograd[m] = rg;
 }
@@ -189,7 +194,7 @@ __global__ void misfit_residual_kernel(const float *__restrict__ mod, const floa
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
         const float r = mod[p] - rec[p];
         res[p] = scaling * r;
-        const int tr = (int)(p / nt);
+        const size_t tr = p / nt;  // trace of the batch; maps of shot k follow shot k-1
         if (map[2 * tr] >= 0 && map[2 * tr + 1] >= 0) acc += 0.5 * (double)r * (double)r;
     }
     part[threadIdx.x] = acc;
@@ -223,45 +228,46 @@ struct DevBuf {
     }
 };
 
-// A WavesAcoustic2D state: P, Vx, Vz and its 8 PML strips, one allocation.
+// WavesAcoustic2D states of a batch of shots: P, Vx, Vz and the 8 PML
+// strips, one allocation; each array holds all shots back to back.
 struct A2DWaves {
     float *base = nullptr;
     size_t n = 0, nstrip_x = 0, nstrip_z = 0, total = 0;
+    int ns = 1;
     rsg_state st{};
-    int init(DevBuf &db, const rsg_grid *g, const rsg_fwi_a2d *f, hipStream_t s) {
+    rsg_batch b{};
+    int init(DevBuf &db, const rsg_grid *g, const rsg_fwi_a2d *f, int nshot, hipStream_t s) {
+        ns = nshot;
         n = (size_t)g->nx * g->nz;
         nstrip_x = (size_t)g->lpml * g->nz;
         nstrip_z = (size_t)g->nx * g->lpml;
-        total = 3 * n + 4 * nstrip_x + 4 * nstrip_z;
+        total = ns * (3 * n + 4 * nstrip_x + 4 * nstrip_z);
         if (int rc = db.alloc(&base, total)) return rc;
         if (int rc = launch_zero(base, total * sizeof(float), s)) return rc;
         st = rsg_state{};
         st.P = base;
-        st.Vx = base + n;
-        st.Vz = base + 2 * n;
-        float *p = base + 3 * n;
-        // strip order of the checkpoint payload: P_left, Vxx_left, P_right,
-        // Vxx_right, P_top, Vzz_top, P_bottom, Vzz_bottom (revolve.cpp:673-700)
-        st.psi[RSG_A2D_P_LEFT] = p;
-        st.psi[RSG_A2D_VXX_LEFT] = p + nstrip_x;
-        st.psi[RSG_A2D_P_RIGHT] = p + 2 * nstrip_x;
-        st.psi[RSG_A2D_VXX_RIGHT] = p + 3 * nstrip_x;
-        p += 4 * nstrip_x;
-        st.psi[RSG_A2D_P_TOP] = p;
-        st.psi[RSG_A2D_VZZ_TOP] = p + nstrip_z;
-        st.psi[RSG_A2D_P_BOTTOM] = p + 2 * nstrip_z;
-        st.psi[RSG_A2D_VZZ_BOTTOM] = p + 3 * nstrip_z;
+        st.Vx = base + ns * n;
+        st.Vz = base + 2 * ns * n;
+        float *p = base + 3 * ns * n;
+        const int xs[4] = {RSG_A2D_P_LEFT, RSG_A2D_VXX_LEFT, RSG_A2D_P_RIGHT, RSG_A2D_VXX_RIGHT};
+        const int zs[4] = {RSG_A2D_P_TOP, RSG_A2D_VZZ_TOP, RSG_A2D_P_BOTTOM, RSG_A2D_VZZ_BOTTOM};
+        for (int i = 0; i < 4; i++, p += ns * nstrip_x) st.psi[xs[i]] = p;
+        for (int i = 0; i < 4; i++, p += ns * nstrip_z) st.psi[zs[i]] = p;
         st.L = f->L;
         st.Rx = f->Rx;
         st.Rz = f->Rz;
         st.tab = f->tab;
+        b.nshot = ns;
+        b.field_stride = n;
+        b.xstrip_stride = nstrip_x;
+        b.zstrip_stride = nstrip_z;
         return RSG_OK;
     }
 };
 
 }  // namespace
 
-static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, hipStream_t st) {
+static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStream_t st) {
     const int nt = f->src.nt;
     if (f->obs.nt != nt || f->obs.field != RSG_PRESSURE || f->src.field != RSG_PRESSURE)
         return fail(RSG_EINVAL, "fwi: source and data must be PRESSURE traces with equal nt");
@@ -269,65 +275,70 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, hipStream_t st)
         return fail(RSG_EINVAL, "fwi: image size must be the unpadded grid");
     DevBuf db;
     A2DWaves fw, bw;
-    if (int rc = fw.init(db, g, f, st)) return rc;
-    if (int rc = bw.init(db, g, f, st)) return rc;
+    if (int rc = fw.init(db, g, f, ns, st)) return rc;
+    if (int rc = bw.init(db, g, f, ns, st)) return rc;
+    const size_t ndat = (size_t)ns * f->obs.ntrace * nt;
     double *misfit_d = nullptr;
     if (int rc = db.alloc(&misfit_d, 1)) return rc;
     if (int rc = launch_zero(misfit_d, sizeof(double), st)) return rc;
-    if (int rc = launch_zero(f->dmod, sizeof(float) * f->obs.ntrace * nt, st)) return rc;
+    if (int rc = launch_zero(f->dmod, sizeof(float) * ndat, st)) return rc;
     if (f->wavgrad)
-        if (int rc = launch_zero(f->wavgrad, sizeof(float) * f->src.ntrace * nt, st)) return rc;
+        if (int rc = launch_zero(f->wavgrad, sizeof(float) * ns * f->src.ntrace * nt, st)) return rc;
 
     rsg_traces mod = f->obs, res = f->obs, wg = f->src;
     mod.data = f->dmod;
     res.data = f->dres;
     wg.data = f->wavgrad;
     const dim3 xb(64, 4), xg((f->nx + 63) / 64, (f->nz + 3) / 4);
+    const size_t fs = fw.n;
 
     auto fstep = [&](int it, bool rec) -> int {
-        if (int rc = rsg_a2d_forward_velocity(g, &fw.st, st)) return rc;
-        if (int rc = rsg_a2d_forward_stress(g, &fw.st, st)) return rc;
-        if (int rc = insert_source(RSG_KIND_A2D, g, &fw.st, &f->src, nullptr, it, st)) return rc;
+        if (int rc = rsg_a2d_forward_velocity_batch(g, &fw.st, &fw.b, st)) return rc;
+        if (int rc = rsg_a2d_forward_stress_batch(g, &fw.st, &fw.b, st)) return rc;
+        if (int rc = insert_source(RSG_KIND_A2D, g, &fw.st, &f->src, nullptr, it, st, ns, fs)) return rc;
         if (rec)
-            if (int rc = record_data(RSG_KIND_A2D, g, &fw.st, &mod, nullptr, it, st)) return rc;
+            if (int rc = record_data(RSG_KIND_A2D, g, &fw.st, &mod, nullptr, it, st, ns, fs)) return rc;
         return RSG_OK;
     };
+    auto bstep = [&]() -> int {
+        if (int rc = rsg_a2d_forward_velocity_batch(g, &bw.st, &bw.b, st)) return rc;
+        return rsg_a2d_forward_stress_batch(g, &bw.st, &bw.b, st);
+    };
     auto first_turn = [&]() -> int {
-        const int threads = 256, blocks = 1024;
-        misfit_residual_kernel<<<blocks, threads, 0, st>>>(f->dmod, f->obs.data, f->dres, f->obs.map,
-                                                           f->obs.ntrace, nt, 1, misfit_d);
+        misfit_residual_kernel<<<1024, 256, 0, st>>>(f->dmod, f->obs.data, f->dres, f->obs.map,
+                                                     ns * f->obs.ntrace, nt, 1, misfit_d);
         RSG_LAUNCH_CHECK("misfit_residual_kernel");
         return RSG_OK;
     };
     auto reverse_tail = [&](int capo, const float *Pf) -> int {
-        if (int rc = insert_source(RSG_KIND_A2D, g, &bw.st, &res, nullptr, capo, st)) return rc;
+        if (int rc = insert_source(RSG_KIND_A2D, g, &bw.st, &res, nullptr, capo, st, ns, fs)) return rc;
         crosscorr_a2d_kernel<<<xg, xb, 0, st>>>(Pf, bw.st.Vx, bw.st.Vz, f->Rx, f->Rz, f->Vp, f->Rho, f->nx,
-                                                f->nz, g->lpml, g->dx, g->dz, f->vpgrad, f->rhograd);
+                                                f->nz, g->lpml, g->dx, g->dz, f->vpgrad, f->rhograd, ns, fs);
         RSG_LAUNCH_CHECK("crosscorr_a2d_kernel");
         if (f->wavgrad)
-            if (int rc = record_data(RSG_KIND_A2D, g, &bw.st, &wg, nullptr, capo, st)) return rc;
+            if (int rc = record_data(RSG_KIND_A2D, g, &bw.st, &wg, nullptr, capo, st, ns, fs)) return rc;
         return RSG_OK;
     };
 
     if (f->mode == RSG_FWI_STORE) {
-        // forward pressure after every step, slot k = state after step k
+        // forward pressure of every shot after every step: slot k = after step k
+        const size_t slot = (size_t)ns * fw.n;
         float *hist = nullptr;
-        if (int rc = db.alloc(&hist, fw.n * (size_t)nt)) return rc;
+        if (int rc = db.alloc(&hist, slot * (size_t)nt)) return rc;
         for (int it = 0; it < nt; it++) {
             if (int rc = fstep(it, true)) return rc;
-            RSG_HIP(hipMemcpyAsync(hist + fw.n * it, fw.st.P, fw.n * sizeof(float), hipMemcpyDeviceToDevice, st));
+            RSG_HIP(hipMemcpyAsync(hist + slot * it, fw.st.P, slot * sizeof(float), hipMemcpyDeviceToDevice, st));
         }
         if (int rc = first_turn()) return rc;
-        if (int rc = reverse_tail(nt - 1, hist + fw.n * (nt - 1))) return rc;
+        if (int rc = reverse_tail(nt - 1, hist + slot * (nt - 1))) return rc;
         for (int capo = nt - 2; capo >= 0; capo--) {
-            if (int rc = rsg_a2d_forward_velocity(g, &bw.st, st)) return rc;
-            if (int rc = rsg_a2d_forward_stress(g, &bw.st, st)) return rc;
+            if (int rc = bstep()) return rc;
             // state after capo-1 steps; before the first step it is zero
             const float *Pf;
             if (capo >= 1) {
-                Pf = hist + fw.n * (capo - 1);
+                Pf = hist + slot * (capo - 1);
             } else {
-                if (int rc = launch_zero(fw.st.P, fw.n * sizeof(float), st)) return rc;
+                if (int rc = launch_zero(fw.st.P, slot * sizeof(float), st)) return rc;
                 Pf = fw.st.P;
             }
             if (int rc = reverse_tail(capo, Pf)) return rc;
@@ -350,8 +361,7 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, hipStream_t st)
                 if (int rc = reverse_tail(capo, fw.st.P)) return rc;
                 reverse = true;
             } else if (a == Revolve::YOUTURN) {
-                if (int rc = rsg_a2d_forward_velocity(g, &bw.st, st)) return rc;
-                if (int rc = rsg_a2d_forward_stress(g, &bw.st, st)) return rc;
+                if (int rc = bstep()) return rc;
                 if (int rc = reverse_tail(capo, fw.st.P)) return rc;
             } else if (a == Revolve::TAKESHOT) {
                 RSG_HIP(hipMemcpyAsync(ck + fw.total * rev.check, fw.base, fw.total * sizeof(float),
@@ -381,7 +391,18 @@ extern "C" int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, voi
         return rsg::fail(RSG_EINVAL, "rsg_fwi_a2d_gradient: null argument");
     if (f->mode != RSG_FWI_REVOLVE && f->mode != RSG_FWI_STORE)
         return rsg::fail(RSG_EINVAL, "rsg_fwi_a2d_gradient: unknown mode");
-    return rsg::fwi_a2d_impl(g, f, (hipStream_t)stream);
+    return rsg::fwi_a2d_impl(g, f, 1, (hipStream_t)stream);
+}
+
+extern "C" int rsg_fwi_a2d_gradient_batch(const rsg_grid *g, const rsg_fwi_a2d *f, int nshot,
+                                          void *stream) {
+    if (!g || !f || !f->L || !f->Rx || !f->Rz || !f->tab || !f->Vp || !f->Rho || !f->vpgrad ||
+        !f->rhograd || !f->dmod || !f->dres || !f->obs.data)
+        return rsg::fail(RSG_EINVAL, "rsg_fwi_a2d_gradient_batch: null argument");
+    if (nshot < 1 || nshot > 65535) return rsg::fail(RSG_EINVAL, "rsg_fwi_a2d_gradient_batch: nshot 1..65535");
+    if (f->mode != RSG_FWI_REVOLVE && f->mode != RSG_FWI_STORE)
+        return rsg::fail(RSG_EINVAL, "rsg_fwi_a2d_gradient_batch: unknown mode");
+    return rsg::fwi_a2d_impl(g, f, nshot, (hipStream_t)stream);
 }
 
 extern "C" int rsg_revolve_schedule(int nt, int snaps, int *actions, int *capos, int *checks, int maxn,

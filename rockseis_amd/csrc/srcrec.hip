@@ -36,14 +36,20 @@ __device__ __forceinline__ void inject2d_trace(int i, float *F, const float *Mod
 
 // F2 (optional) receives the same term: the elastic explosive source drives
 // Sxx and Szz together.
+// blockIdx.y is the shot of a batch: its field sits at fstride elements
+// from shot 0's and its traces follow shot 0's (maps, wavelets).
 __global__ void inject2d_kernel(float *F, float *F2, const float *Mod, float scale, int nx, int lpml,
                                 int ntrace, const int *map, const float *shift, const float *wav,
-                                int nt, const int *it_dev, int it0) {
+                                int nt, const int *it_dev, int it0, size_t fstride) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntrace) return;
     const int it = time_index(it_dev, it0);
-    inject2d_trace(i, F, Mod, scale, nx, lpml, map, shift, wav, nt, it);
-    if (F2) inject2d_trace(i, F2, Mod, scale, nx, lpml, map, shift, wav, nt, it);
+    const size_t k = blockIdx.y;
+    const int *m = map + k * 2 * ntrace;
+    const float *sh = shift + k * 2 * ntrace;
+    const float *w = wav + k * (size_t)ntrace * nt;
+    inject2d_trace(i, F + k * fstride, Mod, scale, nx, lpml, m, sh, w, nt, it);
+    if (F2) inject2d_trace(i, F2 + k * fstride, Mod, scale, nx, lpml, m, sh, w, nt, it);
 }
 
 __device__ __forceinline__ void inject3d_trace(int i, float *F, const float *Mod, float scale, int nx,
@@ -111,10 +117,13 @@ __device__ __forceinline__ void record2d_trace(int i, const float *F, const floa
 
 __global__ void record2d_kernel(const float *F, const float *F2, const float *factor, int nx,
                                 int lpml, int ntrace, const int *map, const float *shift,
-                                float *data, int nt, const int *it_dev, int it0) {
+                                float *data, int nt, const int *it_dev, int it0, size_t fstride) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntrace) return;
-    record2d_trace(i, F, F2, factor, nx, lpml, map, shift, data, nt, time_index(it_dev, it0));
+    const size_t k = blockIdx.y;
+    record2d_trace(i, F + k * fstride, F2 ? F2 + k * fstride : nullptr, factor ? factor + k * ntrace : nullptr,
+                   nx, lpml, map + k * 2 * ntrace, shift + k * 2 * ntrace, data + k * (size_t)ntrace * nt, nt,
+                   time_index(it_dev, it0));
 }
 
 __device__ __forceinline__ void record3d_trace(int i, const float *F, int nx, int ny, int lpml,
@@ -276,7 +285,7 @@ int step_io(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *s
 }
 
 int insert_source(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src,
-                  const int *it_dev, int it, hipStream_t st) {
+                  const int *it_dev, int it, hipStream_t st, int nshot, size_t fstride) {
     if (!src || src->ntrace <= 0) return RSG_OK;
     if (!it_dev && (it < 0 || it >= src->nt)) return fail(RSG_EINVAL, "insertSource: it out of range");
     float *F, *F2;
@@ -286,29 +295,33 @@ int insert_source(int kind, const rsg_grid *g, const rsg_state *s, const rsg_tra
     if (!F) return fail(RSG_EINVAL, "insertSource: target field not allocated");
     const int threads = 64, blocks = (src->ntrace + threads - 1) / threads;
     if (kind == RSG_KIND_A3D) {
+        if (nshot != 1) return fail(RSG_EINVAL, "3D batches are not supported");
         inject3d_kernel<<<blocks, threads, 0, st>>>(F, Mod, scale, g->nx, g->ny, g->lpml, src->ntrace,
                                                     src->map, src->shift, src->data, src->nt, it_dev, it);
     } else {
-        inject2d_kernel<<<blocks, threads, 0, st>>>(F, F2, Mod, scale, g->nx, g->lpml, src->ntrace,
-                                                    src->map, src->shift, src->data, src->nt, it_dev, it);
+        inject2d_kernel<<<dim3(blocks, nshot), threads, 0, st>>>(F, F2, Mod, scale, g->nx, g->lpml,
+                                                                 src->ntrace, src->map, src->shift, src->data,
+                                                                 src->nt, it_dev, it, fstride);
     }
     RSG_LAUNCH_CHECK("inject kernel");
     return RSG_OK;
 }
 
 int record_data(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *rec,
-                const int *it_dev, int it, hipStream_t st) {
+                const int *it_dev, int it, hipStream_t st, int nshot, size_t fstride) {
     if (!rec || rec->ntrace <= 0) return RSG_OK;
     if (!it_dev && (it < 0 || it >= rec->nt)) return fail(RSG_EINVAL, "recordData: it out of range");
     const float *F, *F2;
     if (int rc = record_target(kind, s, rec->field, &F, &F2)) return rc;
     const int threads = 64, blocks = (rec->ntrace + threads - 1) / threads;
+    if (kind == RSG_KIND_A3D && nshot != 1) return fail(RSG_EINVAL, "3D batches are not supported");
     if (kind == RSG_KIND_A3D)
         record3d_kernel<<<blocks, threads, 0, st>>>(F, g->nx, g->ny, g->lpml, rec->ntrace, rec->map,
                                                     rec->shift, rec->data, rec->nt, it_dev, it);
     else
-        record2d_kernel<<<blocks, threads, 0, st>>>(F, F2, rec->factor, g->nx, g->lpml, rec->ntrace,
-                                                    rec->map, rec->shift, rec->data, rec->nt, it_dev, it);
+        record2d_kernel<<<dim3(blocks, nshot), threads, 0, st>>>(F, F2, rec->factor, g->nx, g->lpml,
+                                                                 rec->ntrace, rec->map, rec->shift, rec->data,
+                                                                 rec->nt, it_dev, it, fstride);
     RSG_LAUNCH_CHECK("record kernel");
     return RSG_OK;
 }

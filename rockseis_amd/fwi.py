@@ -139,8 +139,10 @@ class Shot:
 
 
 class Survey:
-    def __init__(self, shots, lpml, dx, dz, dt, order=4, fs=False, ncheck=50, mode="revolve"):
+    def __init__(self, shots, lpml, dx, dz, dt, order=4, fs=False, ncheck=50, mode="revolve",
+                 batched=True):
         self.shots = list(shots)
+        self.batched = bool(batched)
         self.lpml, self.dx, self.dz, self.dt = int(lpml), float(dx), float(dz), float(dt)
         self.order, self.fs, self.ncheck = int(order), bool(fs), int(ncheck)
         self.mode = mode
@@ -174,6 +176,72 @@ def shot_gradient(vp, rho, survey: Survey, shot: Shot):
     return fwi.misfit64, vg.imagedata, rg.imagedata
 
 
+def batch_gradient(vp, rho, survey: Survey, mode=None):
+    """Misfit and (Vp, rho) gradients summed over all shots of the survey,
+    computed as one batch (rsg_fwi_a2d_gradient_batch): every kernel of a time
+    step runs all shots in one launch.  Shots must share nt and their source
+    and receiver counts.  Returns (misfit, vpgrad, rhograd, dmod)."""
+    dev = _dev()
+    nz, nx = vp.shape
+    shots = survey.shots
+    ns = len(shots)
+    nsrc, nrec, nt = shots[0].scoords.shape[0], shots[0].gcoords.shape[0], shots[0].wavelet.shape[1]
+    for sh in shots:
+        if sh.scoords.shape[0] != nsrc or sh.gcoords.shape[0] != nrec or sh.wavelet.shape[1] != nt:
+            raise R.RsgError("batch_gradient: shots must share nt and trace counts")
+    m = ModelAcoustic2D(nx, nz, survey.lpml, survey.dx, survey.dz, 0.0, 0.0, survey.fs)
+    m.Vp[:] = vp
+    m.R[:] = rho
+    smaps, sshifts, gmaps, gshifts = [], [], [], []
+    for sh in shots:
+        d = Data2D(nsrc, nt, survey.dt, 0.0, "P")
+        d.scoords[:] = sh.scoords
+        d.makeMap(m, SMAP)
+        smaps.append(d.smap)
+        sshifts.append(d.sshift)
+        d = Data2D(nrec, nt, survey.dt, 0.0, "P")
+        d.gcoords[:] = sh.gcoords
+        d.makeMap(m, GMAP)
+        gmaps.append(d.gmap)
+        gshifts.append(d.gshift)
+    m.staggerModels()
+    pml = Pml(m.lpml, survey.dt)
+    pml.setSmax(SMAX)
+    pml.computeABC()
+
+    def dev_t(a, dtype=torch.float32):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev, dtype)
+
+    keep = dict(smap=dev_t(np.stack(smaps), torch.int32), sshift=dev_t(np.stack(sshifts)),
+                gmap=dev_t(np.stack(gmaps), torch.int32), gshift=dev_t(np.stack(gshifts)),
+                wav=dev_t(np.stack([sh.wavelet for sh in shots])),
+                dobs=dev_t(np.stack([sh.dobs for sh in shots])),
+                vp=dev_t(m.Vp), rho=dev_t(m.R))
+    dmod = torch.zeros((ns, nrec, nt), dtype=torch.float32, device=dev)
+    dres = torch.zeros_like(dmod)
+    vg = torch.zeros((nz, nx), dtype=torch.float32, device=dev)
+    rg = torch.zeros_like(vg)
+    g = R.Grid(nx=m.getNx_pml(), ny=1, nz=m.getNz_pml(), lpml=m.lpml, order=survey.order,
+               fs=int(m.fs), dx=m.dx, dy=1.0, dz=m.dz, dt=survey.dt)
+    f = R.Fwi()
+    f.L, f.Rx, f.Rz, f.tab = m._dL.data_ptr(), m._dRx.data_ptr(), m._dRz.data_ptr(), pml.tab.data_ptr()
+    f.Vp, f.Rho = keep["vp"].data_ptr(), keep["rho"].data_ptr()
+    f.nx, f.nz = nx, nz
+    f.src = R.Traces(field=R.PRESSURE, ntrace=nsrc, nt=nt, map=keep["smap"].data_ptr(),
+                     shift=keep["sshift"].data_ptr(), data=keep["wav"].data_ptr(), factor=None)
+    f.obs = R.Traces(field=R.PRESSURE, ntrace=nrec, nt=nt, map=keep["gmap"].data_ptr(),
+                     shift=keep["gshift"].data_ptr(), data=keep["dobs"].data_ptr(), factor=None)
+    f.dmod, f.dres, f.wavgrad = dmod.data_ptr(), dres.data_ptr(), None
+    f.vpgrad, f.rhograd = vg.data_ptr(), rg.data_ptr()
+    mis = C.c_double(0.0)
+    f.misfit = C.pointer(mis)
+    mode = mode or survey.mode
+    f.ncheck, f.mode = survey.ncheck, (R.FWI_STORE if mode == "store" else R.FWI_REVOLVE)
+    R.check(R.lib().rsg_fwi_a2d_gradient_batch(C.byref(g), C.byref(f), ns, _stream()),
+            "rsg_fwi_a2d_gradient_batch")
+    return mis.value, vg, rg, dmod
+
+
 class AcousticFwiMisfit(torch.autograd.Function):
     """(Vp, rho) -> sum over shots of the DIFFERENCE misfit.  The backward
     pass returns the reference's FwiAcoustic2D gradients (crossCorr of the
@@ -183,16 +251,21 @@ class AcousticFwiMisfit(torch.autograd.Function):
     def forward(ctx, vp: torch.Tensor, rho: torch.Tensor, survey: Survey):
         v = vp.detach().to("cpu", torch.float32).numpy()
         r = rho.detach().to("cpu", torch.float32).numpy()
-        total = 0.0
-        gv = np.zeros_like(v)
-        gr = np.zeros_like(r)
-        for shot in survey.shots:
-            mis, a, b = shot_gradient(v, r, survey, shot)
-            total += mis
-            gv += a
-            gr += b
-        ctx.save_for_backward(torch.from_numpy(gv).to(vp.device, vp.dtype),
-                              torch.from_numpy(gr).to(rho.device, rho.dtype))
+        if getattr(survey, "batched", True):
+            total, gv, gr, _ = batch_gradient(v, r, survey)
+            gv, gr = gv.to(vp.dtype), gr.to(rho.dtype)
+        else:
+            total = 0.0
+            gvn = np.zeros_like(v)
+            grn = np.zeros_like(r)
+            for shot in survey.shots:
+                mis, a, b = shot_gradient(v, r, survey, shot)
+                total += mis
+                gvn += a
+                grn += b
+            gv = torch.from_numpy(gvn).to(vp.device, vp.dtype)
+            gr = torch.from_numpy(grn).to(rho.device, rho.dtype)
+        ctx.save_for_backward(gv, gr)
         return torch.tensor(total, dtype=torch.float64, device=vp.device)
 
     @staticmethod
