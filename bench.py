@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=80, help="time steps of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fwi", action="store_true", help="skip the FWI shots/s leg")
+    ap.add_argument("--fwi-plain-shots", type=int, default=8,
+                    help="shots timed on the plain per-shot path (shots/s is per shot)")
     ap.add_argument("--pmc-dispatch", action="store_true",
                     help="no timing: 100 modelling steps, 10 launches of each half-step kernel and 5 "
                          "calibration copies (torch.neg of 256 MiB), for rocprofv3 --pmc passes")
@@ -216,11 +219,71 @@ def main():
         "hbm_frac_whole_step": round(step_gbs / HBM_PEAK_GBS, 4),
         "roofline": roof,
     }
+    if not args.no_fwi:
+        del mod
+        torch.cuda.empty_cache()
+        out["fwi"] = fwi_leg(args.fwi_plain_shots)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, inputs, args.cpu_steps)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+
+
+def fwi_leg(n_plain):
+    """FWI gradient shots/s: acoustic 2D, 801x401 at 10 m (lpml 30), order 8,
+    32 shots every 250 m, 801 receivers, 10 Hz Ricker, nt 3000, dt 1 ms;
+    observed data from the seeded true model, gradient on the model smoothed
+    with a 250 m Gaussian.  Batched = all 32 shots per launch, forward
+    pressure of every step kept in HBM; plain = one shot at a time through
+    run_optimal with 50 Revolve checkpoints (per-step launches)."""
+    import numpy as np
+    import torch
+    from rockseis_amd import fwi, synth
+    nx, nz, dx, lpml, nt, dt, f0, order, nshot = 801, 401, 10.0, 30, 3000, 1e-3, 10.0, 8, 32
+    vtrue = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1500.0, 4000.0, nanom=20, seed=7)
+    rtrue = synth.gardner(vtrue)
+    vp0 = synth.gaussian_smooth(vtrue, 250.0, dx, dx)
+    rho0 = synth.gardner(vp0)
+    wav = synth.ricker(nt, dt, f0)[None]
+    gc = synth.surface_line(nx, dx, 10.0)
+    zero = np.zeros((nx, nt), np.float32)
+    shots = [fwi.Shot(np.float32([[125.0 + 250.0 * k, 10.0]]), wav, gc, zero) for k in range(nshot)]
+    sv = fwi.Survey(shots, lpml, dx, dx, dt, order=order, ncheck=50, mode="store")
+    _, _, _, dobs = fwi.batch_gradient(vtrue, rtrue, sv)         # observed data
+    dobs = dobs.cpu().numpy()
+    for k, sh in enumerate(shots):
+        sh.dobs = np.ascontiguousarray(dobs[k])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    misfit_b, vg, rg, _ = fwi.batch_gradient(vp0, rho0, sv)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    mode_b = sv.mode_used
+    plain = fwi.Survey(shots, lpml, dx, dx, dt, order=order, ncheck=50, mode="revolve")
+    n_plain = max(1, min(n_plain, nshot))
+    t0 = time.perf_counter()
+    mis_p = 0.0
+    gv_p = np.zeros((nz, nx), np.float32)
+    for k in range(n_plain):
+        m1, a, _ = fwi.shot_gradient(vp0, rho0, plain, shots[k])
+        mis_p += m1
+        gv_p += a
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    # the batched gradient of the first n_plain shots must equal the plain one
+    check = fwi.Survey(shots[:n_plain], lpml, dx, dx, dt, order=order, ncheck=50, mode="store")
+    m_chk, vg_chk, _, _ = fwi.batch_gradient(vp0, rho0, check)
+    err = float(np.linalg.norm(vg_chk.cpu().numpy() - gv_p) / np.linalg.norm(gv_p))
+    return {"workload": "acoustic 2D FWI gradient, 801x401 at 10 m (861x461 padded), order 8, 32 shots, "
+                        "801 receivers, 10 Hz Ricker, nt 3000",
+            "batched_shots_per_s": round(nshot / tb, 3), "batched_s": round(tb, 3),
+            "batched_history": mode_b,
+            "plain_shots_per_s": round(n_plain / tp, 3), "plain_s": round(tp, 3),
+            "plain_shots_timed": n_plain, "plain_checkpointing": "Revolve, 50 snapshots in HBM",
+            "speedup_batched_vs_plain": round((nshot / tb) / (n_plain / tp), 2),
+            "misfit_32_shots": misfit_b,
+            "batched_vs_plain_gradient_rel_l2": err}
 
 
 def _traffic(name):

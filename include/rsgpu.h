@@ -205,6 +205,8 @@ typedef struct {
     double *misfit;                 /* out (host): DIFFERENCE misfit 0.5*sum r^2    */
     int ncheck;                     /* Revolve snapshots (RSG_FWI_REVOLVE)          */
     int mode;                       /* RSG_FWI_REVOLVE or RSG_FWI_STORE             */
+    int *mode_used;                 /* out (host, optional): STORE falls back to
+                                       REVOLVE when the history does not fit        */
 } rsg_fwi_a2d;
 
 int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, void *stream);

@@ -50,7 +50,8 @@ class Fwi(C.Structure):
     _fields_ = ([(n, C.c_void_p) for n in ("L", "Rx", "Rz", "tab", "Vp", "Rho")]
                 + [("nx", C.c_int), ("nz", C.c_int), ("src", Traces), ("obs", Traces)]
                 + [(n, C.c_void_p) for n in ("dmod", "dres", "wavgrad", "vpgrad", "rhograd")]
-                + [("misfit", C.POINTER(C.c_double)), ("ncheck", C.c_int), ("mode", C.c_int)])
+                + [("misfit", C.POINTER(C.c_double)), ("ncheck", C.c_int), ("mode", C.c_int),
+                   ("mode_used", C.POINTER(C.c_int))])
 
 
 FWI_REVOLVE, FWI_STORE = 0, 1

@@ -11,7 +11,7 @@
 // rho gradients.
 //
 // Mode RSG_FWI_STORE keeps the forward pressure of every step in HBM instead
-// (nt * nx_pml * nz_pml floats: 4.8 GB for an 861x461 grid and 3000 steps),
+// (its unpadded nx*nz part: 3.9 GB per shot for 801x401 and 3000 steps),
 // so the forward sweep runs once; the gradient is identical because the
 // reference correlates, at reverse step capo, the forward pressure after
 // capo-1 forward steps (after nt-1 steps at the first turn).
@@ -150,37 +150,55 @@ Revolve::Action Revolve::next() {
 // by lpml.  The rho-gradient velocity terms keep the reference's double
 // temporaries.  A batch of nshot shots (fields fstride apart) is summed into
 // the one image, shot 0 first.
+// Pf is either the padded forward pressure (pf_pad = lpml) or its unpadded
+// copy kept by the store-every-step mode (pf_pad = 0); pf_stride separates
+// its shots.
 __global__ void crosscorr_a2d_kernel(const float *__restrict__ Pf, const float *__restrict__ Vxa,
                                      const float *__restrict__ Vza, const float *__restrict__ Rx,
                                      const float *__restrict__ Rz, const float *__restrict__ Vp,
                                      const float *__restrict__ Rho, int nx, int nz, int lpml,
                                      float dx, float dz, float *__restrict__ vpgrad,
-                                     float *__restrict__ rhograd, int nshot, size_t fstride) {
+                                     float *__restrict__ rhograd, int nshot, size_t fstride,
+                                     int pf_pad, size_t pf_stride) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int iz = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (ix >= nx - 1 || iz >= nz - 1) return;
     const int nxp = nx + 2 * lpml;
     const size_t m = (size_t)iz * nx + ix;
     const size_t q = (size_t)(iz + lpml) * nxp + ix + lpml;
+    const int pld = nx + 2 * pf_pad;
+    const size_t qp = (size_t)(iz + pf_pad) * pld + ix + pf_pad;
     const float vpscale = (float)(-2.0 / (double)Vp[m]);
     const float rhoscale = (float)(-1.0 / (double)Rho[m]);
     const double rx0 = Rx[q], rx1 = Rx[q - 1], rz0 = Rz[q], rz1 = Rz[q - nxp];
     float vg = vpgrad[m], rg = rhograd[m];
     for (int k = 0; k < nshot; k++) {
-        const float *P = Pf + k * fstride, *Vx = Vxa + k * fstride, *Vz = Vza + k * fstride;
+        const float *P = Pf + k * pf_stride, *Vx = Vxa + k * fstride, *Vz = Vza + k * fstride;
         const float mrxx = (Vx[q] - Vx[q - 1]) / dx;
         const float mrzz = (Vz[q] - Vz[q - nxp]) / dz;
-        const float ws = P[q];
+        const float ws = P[qp];
         vg -= vpscale * ws * (mrxx + mrzz);
         rg -= rhoscale * ws * (mrxx + mrzz);
-        float ux = (float)(0.5 * (double)Vx[q] * rx0 * (double)(P[q + 1] - ws) / (double)dx);
-        ux = (float)((double)ux + 0.5 * (double)Vx[q - 1] * rx1 * (double)(ws - P[q - 1]) / (double)dx);
-        float uz = (float)(0.5 * (double)Vz[q] * rz0 * (double)(P[q + nxp] - ws) / (double)dz);
-        uz = (float)((double)uz + 0.5 * (double)Vz[q - nxp] * rz1 * (double)(ws - P[q - nxp]) / (double)dz);
+        float ux = (float)(0.5 * (double)Vx[q] * rx0 * (double)(P[qp + 1] - ws) / (double)dx);
+        ux = (float)((double)ux + 0.5 * (double)Vx[q - 1] * rx1 * (double)(ws - P[qp - 1]) / (double)dx);
+        float uz = (float)(0.5 * (double)Vz[q] * rz0 * (double)(P[qp + pld] - ws) / (double)dz);
+        uz = (float)((double)uz + 0.5 * (double)Vz[q - nxp] * rz1 * (double)(ws - P[qp - pld]) / (double)dz);
         rg += ux + uz;
     }
     vpgrad[m] = vg;
     rhograd[m] = rg;
+}
+
+// Unpadded nx x nz window of every shot's padded pressure -> dst
+// [nshot][nz][nx] (what crossCorr reads of the forward wavefield).
+__global__ void pack_interior_kernel(const float *__restrict__ src, float *__restrict__ dst, int nx,
+                                     int nz, int lpml, size_t sstride) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iz = blockIdx.y;
+    const int k = blockIdx.z;
+    if (ix >= nx) return;
+    const int nxp = nx + 2 * lpml;
+    dst[((size_t)k * nz + iz) * nx + ix] = src[k * sstride + (size_t)(iz + lpml) * nxp + ix + lpml];
 }
 
 // DIFFERENCE misfit (fwi.cpp:574-608) and residuals (fwi.cpp:772-800).  The
@@ -310,38 +328,48 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
         RSG_LAUNCH_CHECK("misfit_residual_kernel");
         return RSG_OK;
     };
-    auto reverse_tail = [&](int capo, const float *Pf) -> int {
+    auto reverse_tail = [&](int capo, const float *Pf, int pf_pad, size_t pf_stride) -> int {
         if (int rc = insert_source(RSG_KIND_A2D, g, &bw.st, &res, nullptr, capo, st, ns, fs)) return rc;
         crosscorr_a2d_kernel<<<xg, xb, 0, st>>>(Pf, bw.st.Vx, bw.st.Vz, f->Rx, f->Rz, f->Vp, f->Rho, f->nx,
-                                                f->nz, g->lpml, g->dx, g->dz, f->vpgrad, f->rhograd, ns, fs);
+                                                f->nz, g->lpml, g->dx, g->dz, f->vpgrad, f->rhograd, ns, fs,
+                                                pf_pad, pf_stride);
         RSG_LAUNCH_CHECK("crosscorr_a2d_kernel");
         if (f->wavgrad)
             if (int rc = record_data(RSG_KIND_A2D, g, &bw.st, &wg, nullptr, capo, st, ns, fs)) return rc;
         return RSG_OK;
     };
 
-    if (f->mode == RSG_FWI_STORE) {
-        // forward pressure of every shot after every step: slot k = after step k
-        const size_t slot = (size_t)ns * fw.n;
+    // The store mode keeps the unpadded forward pressure of every shot after
+    // every step (slot k = after step k); it falls back to Revolve when that
+    // does not fit in free device memory (with 2 GiB to spare).
+    const size_t nint = (size_t)f->nx * f->nz, slot = (size_t)ns * nint;
+    int mode = f->mode;
+    if (mode == RSG_FWI_STORE) {
+        size_t free_b = 0, total_b = 0;
+        RSG_HIP(hipMemGetInfo(&free_b, &total_b));
+        if (slot * (size_t)nt * sizeof(float) + ((size_t)2 << 30) > free_b) mode = RSG_FWI_REVOLVE;
+    }
+    if (f->mode_used) *f->mode_used = mode;
+    if (mode == RSG_FWI_STORE) {
         float *hist = nullptr;
         if (int rc = db.alloc(&hist, slot * (size_t)nt)) return rc;
+        const dim3 pb(256), pg((f->nx + 255) / 256, f->nz, ns);
         for (int it = 0; it < nt; it++) {
             if (int rc = fstep(it, true)) return rc;
-            RSG_HIP(hipMemcpyAsync(hist + slot * it, fw.st.P, slot * sizeof(float), hipMemcpyDeviceToDevice, st));
+            pack_interior_kernel<<<pg, pb, 0, st>>>(fw.st.P, hist + slot * it, f->nx, f->nz, g->lpml, fw.n);
+            RSG_LAUNCH_CHECK("pack_interior_kernel");
         }
         if (int rc = first_turn()) return rc;
-        if (int rc = reverse_tail(nt - 1, hist + slot * (nt - 1))) return rc;
+        if (int rc = reverse_tail(nt - 1, hist + slot * (nt - 1), 0, nint)) return rc;
         for (int capo = nt - 2; capo >= 0; capo--) {
             if (int rc = bstep()) return rc;
             // state after capo-1 steps; before the first step it is zero
-            const float *Pf;
             if (capo >= 1) {
-                Pf = hist + slot * (capo - 1);
+                if (int rc = reverse_tail(capo, hist + slot * (capo - 1), 0, nint)) return rc;
             } else {
-                if (int rc = launch_zero(fw.st.P, slot * sizeof(float), st)) return rc;
-                Pf = fw.st.P;
+                if (int rc = launch_zero(fw.st.P, (size_t)ns * fw.n * sizeof(float), st)) return rc;
+                if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n)) return rc;
             }
-            if (int rc = reverse_tail(capo, Pf)) return rc;
         }
     } else {
         Revolve rev(nt, f->ncheck);
@@ -358,11 +386,11 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
             } else if (a == Revolve::FIRSTURN) {
                 if (int rc = fstep(capo, true)) return rc;
                 if (int rc = first_turn()) return rc;
-                if (int rc = reverse_tail(capo, fw.st.P)) return rc;
+                if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n)) return rc;
                 reverse = true;
             } else if (a == Revolve::YOUTURN) {
                 if (int rc = bstep()) return rc;
-                if (int rc = reverse_tail(capo, fw.st.P)) return rc;
+                if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n)) return rc;
             } else if (a == Revolve::TAKESHOT) {
                 RSG_HIP(hipMemcpyAsync(ck + fw.total * rev.check, fw.base, fw.total * sizeof(float),
                                        hipMemcpyDeviceToDevice, st));

@@ -234,11 +234,14 @@ def batch_gradient(vp, rho, survey: Survey, mode=None):
     f.dmod, f.dres, f.wavgrad = dmod.data_ptr(), dres.data_ptr(), None
     f.vpgrad, f.rhograd = vg.data_ptr(), rg.data_ptr()
     mis = C.c_double(0.0)
+    used = C.c_int(-1)
     f.misfit = C.pointer(mis)
+    f.mode_used = C.pointer(used)
     mode = mode or survey.mode
     f.ncheck, f.mode = survey.ncheck, (R.FWI_STORE if mode == "store" else R.FWI_REVOLVE)
     R.check(R.lib().rsg_fwi_a2d_gradient_batch(C.byref(g), C.byref(f), ns, _stream()),
             "rsg_fwi_a2d_gradient_batch")
+    survey.mode_used = "store" if used.value == R.FWI_STORE else "revolve"
     return mis.value, vg, rg, dmod
 
 
