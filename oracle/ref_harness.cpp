@@ -280,10 +280,14 @@ void ref_a3d_run(const float *vp, const float *rho, int nx, int ny, int nz, int 
 // Revolve checkpoints and the DIFFERENCE misfit.  Outputs: vpgrad, rhograd
 // [nz*nx], modelled and residual data [nrec*nt], wavelet gradient
 // [nsrc*nt]; returns the misfit.
-float ref_fwi_a2d(const float *vp, const float *rho, int nx, int nz, int lpml, float dx, float dz,
-                  float dt, int nt, int order, int fs, int ncheck, int nsrc, const float *scoords,
-                  const float *wav, int nrec, const float *gcoords, const float *dobs, float *vpgrad,
-                  float *rhograd, float *dmod, float *dres, float *wgrad) {
+// FwiAcoustic2D gradient of one shot: run_optimal with ncheck in-core
+// Revolve snapshots when ncheck > 0, else run() with snapshots every snapinc
+// steps in the file `snapfile` (removed by run()).
+static float fwi_a2d(const float *vp, const float *rho, int nx, int nz, int lpml, float dx, float dz,
+                     float dt, int nt, int order, int fs, int ncheck, int snapinc, const char *snapfile,
+                     int nsrc, const float *scoords, const float *wav, int nrec, const float *gcoords,
+                     const float *dobs, float *vpgrad, float *rhograd, float *dmod, float *dres,
+                     float *wgrad) {
     auto model = std::make_shared<ModelAcoustic2D<float>>(nx, nz, lpml, dx, dz, 0.f, 0.f, fs != 0);
     model->createModel();
     std::memcpy(model->getVp(), vp, sizeof(float) * nx * nz);
@@ -315,16 +319,41 @@ float ref_fwi_a2d(const float *vp, const float *rho, int nx, int nz, int lpml, f
     fill2<float>(wg, scoords, true);
     wg->makeMap(model->getGeom(), SMAP);
     fwi->setWavgrad(wg);
-    fwi->setNcheck(ncheck);
-    fwi->setIncore(true);
+    if (ncheck > 0) {
+        fwi->setNcheck(ncheck);
+        fwi->setIncore(true);
+    } else {  // FwiAcoustic2D::run: every snapinc-th pressure to a snapshot file
+        fwi->setSnapinc(snapinc);
+        fwi->setSnapfile(snapfile);
+    }
     fwi->setMisfit_type(DIFFERENCE);
-    fwi->run_optimal();
+    if (ncheck > 0)
+        fwi->run_optimal();
+    else
+        fwi->run();
     std::memcpy(vpgrad, vg->getImagedata(), sizeof(float) * nx * nz);
     std::memcpy(rhograd, rg->getImagedata(), sizeof(float) * nx * nz);
     std::memcpy(dmod, mod->getData(), sizeof(float) * nrec * nt);
     std::memcpy(dres, res->getData(), sizeof(float) * nrec * nt);
     std::memcpy(wgrad, wg->getData(), sizeof(float) * nsrc * nt);
     return fwi->getMisfit();
+}
+
+float ref_fwi_a2d(const float *vp, const float *rho, int nx, int nz, int lpml, float dx, float dz,
+                  float dt, int nt, int order, int fs, int ncheck, int nsrc, const float *scoords,
+                  const float *wav, int nrec, const float *gcoords, const float *dobs, float *vpgrad,
+                  float *rhograd, float *dmod, float *dres, float *wgrad) {
+    return fwi_a2d(vp, rho, nx, nz, lpml, dx, dz, dt, nt, order, fs, ncheck, 1, "", nsrc, scoords, wav,
+                   nrec, gcoords, dobs, vpgrad, rhograd, dmod, dres, wgrad);
+}
+
+float ref_fwi_a2d_full(const float *vp, const float *rho, int nx, int nz, int lpml, float dx, float dz,
+                       float dt, int nt, int order, int fs, int snapinc, const char *snapfile, int nsrc,
+                       const float *scoords, const float *wav, int nrec, const float *gcoords,
+                       const float *dobs, float *vpgrad, float *rhograd, float *dmod, float *dres,
+                       float *wgrad) {
+    return fwi_a2d(vp, rho, nx, nz, lpml, dx, dz, dt, nt, order, fs, 0, snapinc, snapfile, nsrc, scoords,
+                   wav, nrec, gcoords, dobs, vpgrad, rhograd, dmod, dres, wgrad);
 }
 
 // Revolve action sequence: actions[k] (advance=0 takeshot=1 restore=2

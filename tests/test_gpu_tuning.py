@@ -1,0 +1,109 @@
+"""Launch-shape knobs and the autotuner: every knob value gives bit-identical
+fields (each cell is computed by the same arithmetic), and the tuner picks
+one of its candidates, sets it and caches it."""
+import numpy as np
+import pytest
+
+from tests.gpu_util import data2d, data3d
+
+pytestmark = pytest.mark.gpu
+
+
+def _a2d(core, order):
+    from rockseis_amd import synth
+    nx, nz, lpml, dx, dt, nt = 150, 90, 12, 10.0, 1e-3, 120
+    vp = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1500, 3000, nanom=5, seed=3)
+    m = core.ModelAcoustic2D(nx, nz, lpml, dx, dx, 0.0, 0.0, False)
+    m.Vp[:] = vp
+    m.R[:] = synth.gardner(vp)
+    src = data2d(np.float32([[700.0, 20.0]]), nt, dt, "P", synth.ricker(nt, dt, 15.0)[None])
+    src.makeMap(m, core.SMAP)
+    rec = data2d(synth.surface_line(nx, dx, 10.0), nt, dt, "P")
+    rec.makeMap(m, core.GMAP)
+    m.staggerModels()
+    mod = core.ModellingAcoustic2D(m, src, order, 1)
+    mod.setRecP(rec)
+    mod.run()
+    return rec.data.copy(), mod.waves.P.cpu().numpy()
+
+
+def _a3d(core):
+    from rockseis_amd import synth
+    nx, ny, nz, lpml, dx, dt, nt = 40, 36, 44, 6, 10.0, 1e-3, 60
+    vp = synth.layered_anomaly_vp3d(nx, ny, nz, dx, dx, dx, seed=4)
+    m = core.ModelAcoustic3D(nx, ny, nz, lpml, dx, dx, dx, 0.0, 0.0, 0.0, False)
+    m.Vp[:] = vp
+    m.R[:] = synth.gardner(vp)
+    src = data3d(np.float32([[200.0, 180.0, 30.0]]), nt, dt, "P", synth.ricker(nt, dt, 20.0)[None])
+    src.makeMap(m, core.SMAP)
+    gx, gy = np.meshgrid(np.arange(nx) * dx, np.arange(ny) * dx)
+    gc = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, 10.0)], 1).astype(np.float32)
+    rec = data3d(gc, nt, dt, "P")
+    rec.makeMap(m, core.GMAP)
+    m.staggerModels()
+    mod = core.ModellingAcoustic3D(m, src, 4, 1)
+    mod.setRecP(rec)
+    mod.run()
+    return rec.data.copy(), mod.waves.P.cpu().numpy()
+
+
+def test_rz2d_values_give_identical_fields():
+    from rockseis_amd import _lib as R, core
+    old = R.get_tuning("rz2d")
+    try:
+        out = {}
+        for v in (2, 4, 8):
+            R.set_tuning("rz2d", v)
+            out[v] = _a2d(core, 8)
+        for v in (4, 8):
+            assert np.array_equal(out[v][0], out[2][0]), v
+            assert np.array_equal(out[v][1], out[2][1]), v
+        assert np.abs(out[2][0]).max() > 0
+    finally:
+        R.set_tuning("rz2d", old)
+
+
+def test_zc3d_values_give_identical_fields():
+    from rockseis_amd import _lib as R, core
+    old = R.get_tuning("zc3d")
+    try:
+        out = {}
+        for v in (0, 3, 16, 64):
+            R.set_tuning("zc3d", v)
+            out[v] = _a3d(core)
+        for v in (3, 16, 64):
+            assert np.array_equal(out[v][0], out[0][0]), v
+            assert np.array_equal(out[v][1], out[0][1]), v
+        assert np.abs(out[0][0]).max() > 0
+    finally:
+        R.set_tuning("zc3d", old)
+
+
+@pytest.mark.parametrize("kind", ["a2d", "e2d", "a3d", "a2d_batch"])
+def test_autotune_picks_and_sets_a_candidate(kind):
+    from rockseis_amd import _lib as R, autotune, core
+    nshot = 1
+    if kind == "a3d":
+        m = core.ModelAcoustic3D(60, 50, 40, 8, 10.0, 10.0, 10.0)
+        w = core.WavesAcoustic3D(m, 10, 1e-3)
+        knob = "zc3d"
+    elif kind == "e2d":
+        m = core.ModelElastic2D(300, 200, 20, 5.0, 5.0)
+        w = core.WavesElastic2D(m, 10, 5e-4)
+        knob = "rz2d"
+    else:
+        m = core.ModelAcoustic2D(300, 200, 20, 10.0, 10.0)
+        w = core.WavesAcoustic2D(m, 10, 1e-3)
+        knob = "rz2d"
+        nshot = 4 if kind == "a2d_batch" else 1
+    old = R.get_tuning(knob)
+    try:
+        res = autotune.tune_waves(w, 8 if kind != "a3d" else 4, nshot=nshot, reps=5, force=True)
+        assert res[knob] in autotune.CANDIDATES[knob]
+        assert R.get_tuning(knob) == res[knob]
+        assert len(res["ms"]) == len(autotune.CANDIDATES[knob])
+        assert all(t > 0 for t in res["ms"].values())
+        # cached: the second call returns the same choice without timing
+        assert autotune.tune_waves(w, 8 if kind != "a3d" else 4, nshot=nshot) is res
+    finally:
+        R.set_tuning(knob, old)
