@@ -189,9 +189,13 @@ int rsg_modelling_run(int kind, const rsg_grid *g, const rsg_state *s, const rsg
                       int nrec, const rsg_traces *recs, int it0, int nsteps, int graph_chunk,
                       void *stream);
 
-/* ---- FWI gradient of one shot (FwiAcoustic2D::run_optimal, fwi.cpp:1159-1332) */
-#define RSG_FWI_REVOLVE 0 /* Revolve checkpoints in HBM (ncheck snapshots)       */
-#define RSG_FWI_STORE 1   /* forward pressure of every step kept in HBM          */
+/* ---- FWI gradient of one shot (FwiAcoustic2D::run_optimal, fwi.cpp:1159-1332,
+ * and FwiAcoustic2D::run, fwi.cpp:1030-1156) */
+#define RSG_FWI_REVOLVE 0 /* run_optimal: Revolve checkpoints in HBM (ncheck)        */
+#define RSG_FWI_STORE 1   /* run_optimal's gradient, forward pressure of every step
+                             kept in HBM (forward sweep once)                       */
+#define RSG_FWI_FULL 2    /* run: pressure every snapinc-th step kept in HBM, in
+                             place of the snapshot file                             */
 
 typedef struct {
     const float *L, *Rx, *Rz, *tab; /* staggered model and PML tables (device)    */
@@ -207,6 +211,9 @@ typedef struct {
     int mode;                       /* RSG_FWI_REVOLVE or RSG_FWI_STORE             */
     int *mode_used;                 /* out (host, optional): STORE falls back to
                                        REVOLVE when the history does not fit        */
+    int snapinc;                    /* RSG_FWI_FULL: snapshot increment (Fwi snapinc) */
+    int fp16;                       /* store snapshots / checkpoints as FP16 with a
+                                       power-of-two scale per stored array          */
 } rsg_fwi_a2d;
 
 int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, void *stream);
@@ -222,6 +229,9 @@ int rsg_revolve_schedule(int nt, int snaps, int *actions, int *capos, int *check
                          int *n);
 /* Revolve::numforw (revolve.cpp:31-62) */
 int rsg_revolve_numforw(int nt, int snaps);
+/* Snapshot2D/3D cadence (snap.cpp:330-335): snapshots are taken at it % snapinc
+ * == 0 and read back at reverse times t with (t - enddiff) % snapinc == 0. */
+int rsg_snap_cadence(int nt, float dt, int snapinc, int *snapnt, int *enddiff);
 
 #ifdef __cplusplus
 }

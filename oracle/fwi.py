@@ -1,9 +1,11 @@
-"""Oracle restatement of FwiAcoustic2D::run_optimal (fwi.cpp:1159-1332): the
-reference's per-shot gradient with Revolve checkpointing, DIFFERENCE misfit
-(fwi.cpp:574-608), residuals (fwi.cpp:772-800) and the zero-lag
-cross-correlation imaging condition crossCorr (fwi.cpp:511-571).  The
-adjoint wavefield is advanced with the forward operator, as in the
-reference.  Also the same run on the compiled reference.
+"""Oracle restatement of FwiAcoustic2D::run_optimal (fwi.cpp:1159-1332) and
+FwiAcoustic2D::run (fwi.cpp:1030-1156): the reference's per-shot gradient
+with Revolve checkpointing or with snapshots every snapinc steps (Snapshot2D
+cadence, snap.cpp:330-335, 638-689), DIFFERENCE misfit (fwi.cpp:574-608),
+residuals (fwi.cpp:772-800) and the zero-lag cross-correlation imaging
+condition crossCorr (fwi.cpp:511-571).  The adjoint wavefield is advanced
+with the forward operator, as in the reference.  Also the same runs on the
+compiled reference.
 
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 """
@@ -13,19 +15,36 @@ import numpy as np
 
 from . import fn, ref
 from . import revolve as orv
-from .runs import A2DState, inject2d, makemap2d, record2d, stagger_a2d
+from .runs import (A2DState, E2DState, e2d_record_factor, inject2d, makemap2d, record2d, stagger_a2d,
+                   stagger_e2d)
+
+
+def _fields(s):
+    if isinstance(s, E2DState):
+        return [s.Vx, s.Vz, s.Sxx, s.Szz, s.Sxz] + list(s.strips.values())
+    return [s.P, s.Vx, s.Vz] + list(s.strips.values())
 
 
 def _snapshot(s):
-    return [s.P.copy(), s.Vx.copy(), s.Vz.copy()] + [a.copy() for a in s.strips.values()]
+    """Revolve payload: fields then PML strips (revolve.cpp:648-1271)."""
+    return [a.copy() for a in _fields(s)]
 
 
 def _restore(s, snap):
-    s.P[...] = snap[0]
-    s.Vx[...] = snap[1]
-    s.Vz[...] = snap[2]
-    for a, b in zip(s.strips.values(), snap[3:]):
+    for a, b in zip(_fields(s), snap):
         a[...] = b
+
+
+def snap_cadence(nt, dt, snapinc):
+    """(snapnt, enddiff) of Snapshot2D (snap.cpp:330-335), in the
+    reference's float32 arithmetic."""
+    f = np.float32
+    snapnt = (nt - 1) // snapinc + 1
+    dt = f(dt)
+    snapdt = f(dt * f(snapinc))
+    a = f(f(nt - 1) * dt)
+    b = f(f(snapnt - 1) * snapdt)
+    return snapnt, int(np.rint(f(f(a - b) / dt)))
 
 
 def fwi_a2d(vp, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, scoords, wav, gcoords, dobs,
@@ -95,6 +114,56 @@ def fwi_a2d(vp, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, scoords, wav, gcoo
     return dict(misfit=misfit, vpgrad=vpgrad, rhograd=rhograd, dmod=dmod, dres=dres, wavgrad=wgrad)
 
 
+def fwi_a2d_full(vp, rho, lpml, dx, dz, dt, nt, order, fs, snapinc, scoords, wav, gcoords, dobs,
+                 real=np.float32):
+    """FwiAcoustic2D::run (fwi.cpp:1030-1156): forward loop writing the
+    unpadded pressure when it % snapinc == 0 (snap.cpp:645-665), misfit and
+    residuals, then a fresh wavefield stepped nt times; at reverse time t:
+    step, inject the residual at t, crossCorr with snapshot
+    (t - enddiff) / snapinc when (t - enddiff) % snapinc == 0 (the read order
+    of snap.cpp:672-689), record the wavelet gradient at t."""
+    nz, nx = vp.shape
+    smap, sshift = makemap2d(scoords, nx, nz, dx, dz, "P")
+    gmap, gshift = makemap2d(gcoords, nx, nz, dx, dz, "P")
+    vp = np.ascontiguousarray(vp, real)
+    rho = np.ascontiguousarray(rho, real)
+    L, Rx, Rz = stagger_a2d(vp, rho, lpml, fs, real)
+    fw = A2DState(L, Rx, Rz, lpml, order, dx, dz, dt, real)
+    bw = A2DState(L, Rx, Rz, lpml, order, dx, dz, dt, real)
+    wav = np.ascontiguousarray(wav, real).reshape(smap.shape[0], nt)
+    dobs = np.ascontiguousarray(dobs, real).reshape(gmap.shape[0], nt)
+    dmod = np.zeros_like(dobs)
+    dres = np.zeros_like(dobs)
+    wgrad = np.zeros_like(wav)
+    vpgrad = np.zeros((nz, nx), real)
+    rhograd = np.zeros((nz, nx), real)
+    valid = ((gmap[:, 0] >= 0) & (gmap[:, 1] >= 0)).astype(np.int32)
+    snapnt, enddiff = snap_cadence(nt, dt, snapinc)
+    snaps = []
+    for it in range(nt):
+        fw.velocity()
+        fw.stress()
+        inject2d(fw.P, fw.L, lpml, dt, smap, sshift, wav, it, real)
+        record2d(fw.P, lpml, gmap, gshift, dmod, it, real)
+        if it % snapinc == 0:
+            snaps.append(np.ascontiguousarray(fw.P[lpml:lpml + nz, lpml:lpml + nx]))
+    assert len(snaps) == snapnt
+    misfit = fn("misfit_diff", real)(dmod.reshape(-1), dobs.reshape(-1), valid, dobs.shape[0], nt)
+    fn("residual_diff", real)(dmod.reshape(-1), dobs.reshape(-1), dres.reshape(-1), dobs.shape[0], nt, 1)
+    xc = fn("a2d_crosscorr", real)
+    for t in range(nt - 1, -1, -1):
+        bw.velocity()
+        bw.stress()
+        inject2d(bw.P, bw.L, lpml, dt, gmap, gshift, dres, t, real)
+        if (t - enddiff) % snapinc == 0:
+            sn = snaps[(t - enddiff) // snapinc]
+            xc(sn.reshape(-1), 0, bw.Vx.reshape(-1), bw.Vz.reshape(-1), lpml, vp.reshape(-1),
+               rho.reshape(-1), fw.Rx.reshape(-1), fw.Rz.reshape(-1), nx, nz, real(dx), real(dz),
+               vpgrad.reshape(-1), rhograd.reshape(-1))
+        record2d(bw.P, lpml, smap, sshift, wgrad, t, real)
+    return dict(misfit=misfit, vpgrad=vpgrad, rhograd=rhograd, dmod=dmod, dres=dres, wavgrad=wgrad)
+
+
 def ref_fwi_a2d(vp, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, scoords, wav, gcoords, dobs):
     nz, nx = vp.shape
     sc = np.ascontiguousarray(scoords, np.float32).reshape(-1, 2)
@@ -112,3 +181,160 @@ def ref_fwi_a2d(vp, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, scoords, wav, 
                           np.ascontiguousarray(dobs, np.float32).ravel(), vg.reshape(-1),
                           rg.reshape(-1), dmod.reshape(-1), dres.reshape(-1), wg.reshape(-1))
     return dict(misfit=np.float32(m), vpgrad=vg, rhograd=rg, dmod=dmod, dres=dres, wavgrad=wg)
+
+
+def ref_fwi_a2d_full(vp, rho, lpml, dx, dz, dt, nt, order, fs, snapinc, scoords, wav, gcoords, dobs):
+    """The reference's FwiAcoustic2D::run; its snapshot file goes to a
+    temporary directory and is removed by run()."""
+    import os
+    import tempfile
+    nz, nx = vp.shape
+    sc = np.ascontiguousarray(scoords, np.float32).reshape(-1, 2)
+    gc = np.ascontiguousarray(gcoords, np.float32).reshape(-1, 2)
+    nsrc, nrec = sc.shape[0], gc.shape[0]
+    vg = np.zeros((nz, nx), np.float32)
+    rg = np.zeros((nz, nx), np.float32)
+    dmod = np.zeros((nrec, nt), np.float32)
+    dres = np.zeros((nrec, nt), np.float32)
+    wg = np.zeros((nsrc, nt), np.float32)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "psnap.rss").encode()
+        m = ref().ref_fwi_a2d_full(np.ascontiguousarray(vp, np.float32).ravel(),
+                                   np.ascontiguousarray(rho, np.float32).ravel(), nx, nz, lpml, dx, dz, dt,
+                                   nt, order, int(fs), snapinc, path, nsrc, sc.ravel(),
+                                   np.ascontiguousarray(wav, np.float32).ravel(), nrec, gc.ravel(),
+                                   np.ascontiguousarray(dobs, np.float32).ravel(), vg.reshape(-1),
+                                   rg.reshape(-1), dmod.reshape(-1), dres.reshape(-1), wg.reshape(-1))
+    return dict(misfit=np.float32(m), vpgrad=vg, rhograd=rg, dmod=dmod, dres=dres, wavgrad=wg)
+
+
+E2D_FIELDS = ("P", "VX", "VZ")
+
+
+def _e2d_inject(s, field, lpml, dt, m, sh, data, it, real):
+    """WavesElastic2D::insertSource (waves.cpp:2085-2171): PRESSURE is an
+    explosive source into Sxx and Szz (no modulus), VX / VZ scale by Rx / Rz."""
+    if field == "P":
+        inject2d(s.Sxx, None, lpml, dt, m, sh, data, it, real)
+        inject2d(s.Szz, None, lpml, dt, m, sh, data, it, real)
+    elif field == "VX":
+        inject2d(s.Vx, s.Rx, lpml, dt, m, sh, data, it, real)
+    else:
+        inject2d(s.Vz, s.Rz, lpml, dt, m, sh, data, it, real)
+
+
+def _e2d_record(s, field, lpml, m, sh, factor, data, it, real):
+    """WavesElastic2D::recordData (waves.cpp:2174-2288)."""
+    if field == "P":
+        fn("record2d_pressure_el", real)(s.Sxx.reshape(-1), s.Szz.reshape(-1), s.nx, lpml, m.reshape(-1),
+                                         np.ascontiguousarray(sh, real).reshape(-1), factor, data.reshape(-1),
+                                         m.shape[0], data.shape[-1], it)
+    else:
+        record2d(s.Vx if field == "VX" else s.Vz, lpml, m, sh, data, it, real)
+
+
+def fwi_e2d(vp, vs, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, srcfield, scoords, wav, gcoords, dobs,
+            real=np.float32):
+    """FwiElastic2D::run_optimal (fwi.cpp:3937-4143) with P, Ux and Uz data
+    (dobs[3, nrec, nt]): forward elastic steps, DIFFERENCE misfit and
+    residuals of the three records (fwi.cpp:2852-2930, 3227-3300), adjoint
+    with backwardstepVelocity/Stress (waves.cpp:1738-2080) driven by the three
+    residuals, crossCorr (fwi.cpp:2591-2775) at every reverse step, then
+    scaleGrad (fwi.cpp:2778-2849).  Returns the lambda/mu/rho kernels before
+    scaleGrad as well as the Vp/Vs/rho gradients after it."""
+    nz, nx = vp.shape
+    smap, sshift = makemap2d(scoords, nx, nz, dx, dz, srcfield)
+    maps = {f: makemap2d(gcoords, nx, nz, dx, dz, f) for f in E2D_FIELDS}
+    vp, vs, rho = (np.ascontiguousarray(a, real) for a in (vp, vs, rho))
+    L, L2M, M, Rx, Rz = stagger_e2d(vp, vs, rho, lpml, fs, real)
+    fw = E2DState(L, L2M, M, Rx, Rz, lpml, order, dx, dz, dt, fs, real)
+    bw = E2DState(L, L2M, M, Rx, Rz, lpml, order, dx, dz, dt, fs, real)
+    nsrc, nrec = smap.shape[0], maps["P"][0].shape[0]
+    wav = np.ascontiguousarray(wav, real).reshape(nsrc, nt)
+    dobs = np.ascontiguousarray(dobs, real).reshape(3, nrec, nt)
+    dmod = np.zeros_like(dobs)
+    dres = np.zeros_like(dobs)
+    wgrad = np.zeros_like(wav)
+    lam = np.zeros((nz, nx), real)
+    mu = np.zeros((nz, nx), real)
+    rhok = np.zeros((nz, nx), real)
+    factor = e2d_record_factor(vp, vs, rho, maps["P"][0], False, real)
+    gx = maps["VX"][0]
+    valid = ((gx[:, 0] >= 0) & (gx[:, 1] >= 0)).astype(np.int32)
+    wf = {"P": 0, "VX": 1, "VZ": 3}[srcfield]
+    xc = fn("e2d_crosscorr", real)
+
+    def corr(capo):
+        xc(fw.Vx.reshape(-1), fw.Vz.reshape(-1), fw.Sxx.reshape(-1), fw.Szz.reshape(-1), fw.Sxz.reshape(-1),
+           bw.Vx.reshape(-1), bw.Vz.reshape(-1), bw.Sxx.reshape(-1), bw.Szz.reshape(-1), bw.Sxz.reshape(-1),
+           fw.Rx.reshape(-1), fw.Rz.reshape(-1), lpml, nx, nz, real(dx), real(dz), lam.reshape(-1),
+           mu.reshape(-1), rhok.reshape(-1), wgrad.ctypes.data, wf, smap.ctypes.data,
+           np.ascontiguousarray(sshift, real).ctypes.data, nsrc, nt, capo)
+
+    def fstep(it, rec):
+        fw.velocity()
+        fw.stress()
+        _e2d_inject(fw, srcfield, lpml, dt, smap, sshift, wav, it, real)
+        if rec:
+            for k, f in enumerate(E2D_FIELDS):
+                _e2d_record(fw, f, lpml, maps[f][0], maps[f][1], factor, dmod[k], it, real)
+
+    def inject_res(capo):
+        for k, f in enumerate(E2D_FIELDS):
+            _e2d_inject(bw, f, lpml, dt, maps[f][0], maps[f][1], dres[k], capo, real)
+
+    misfit = None
+    checks = {}
+    rev = orv.Revolve(nt, ncheck)
+    reverse = False
+    while True:
+        oldcapo = rev.capo
+        a = rev.step()
+        capo = rev.capo
+        if a == orv.ADVANCE:
+            for it in range(oldcapo, capo):
+                fstep(it, not reverse)
+        elif a == orv.FIRSTURN:
+            fstep(capo, True)
+            misfit = fn("misfit_diff3", real)(dmod[0].reshape(-1), dobs[0].reshape(-1), dmod[1].reshape(-1),
+                                              dobs[1].reshape(-1), dmod[2].reshape(-1), dobs[2].reshape(-1),
+                                              valid, nrec, nt)
+            for k in range(3):
+                fn("residual_diff", real)(dmod[k].reshape(-1), dobs[k].reshape(-1), dres[k].reshape(-1),
+                                          nrec, nt, 1)
+            inject_res(capo)
+            corr(capo)
+            reverse = True
+        elif a == orv.YOUTURN:
+            bw.velocity_bw()
+            bw.stress_bw()
+            inject_res(capo)
+            corr(capo)
+        elif a == orv.TAKESHOT:
+            checks[rev.check] = _snapshot(fw)
+        elif a == orv.RESTORE:
+            _restore(fw, checks[rev.check])
+        if a in (orv.TERMINATE, orv.ERROR):
+            break
+    vpg, vsg, rhg = lam.copy(), mu.copy(), rhok.copy()
+    fn("e2d_scalegrad", real)(vp.reshape(-1), vs.reshape(-1), rho.reshape(-1), nx, nz, vpg.reshape(-1),
+                              vsg.reshape(-1), rhg.reshape(-1))
+    return dict(misfit=misfit, lam=lam, mu=mu, rhok=rhok, vpgrad=vpg, vsgrad=vsg, rhograd=rhg, dmod=dmod,
+                dres=dres, wavgrad=wgrad)
+
+
+def ref_fwi_e2d(vp, vs, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, srcfield, scoords, wav, gcoords, dobs):
+    nz, nx = vp.shape
+    sc = np.ascontiguousarray(scoords, np.float32).reshape(-1, 2)
+    gc = np.ascontiguousarray(gcoords, np.float32).reshape(-1, 2)
+    nsrc, nrec = sc.shape[0], gc.shape[0]
+    out = {k: np.zeros((nz, nx), np.float32) for k in ("vpgrad", "vsgrad", "rhograd")}
+    dmod = np.zeros((3, nrec, nt), np.float32)
+    dres = np.zeros((3, nrec, nt), np.float32)
+    wg = np.zeros((nsrc, nt), np.float32)
+    f32 = lambda a: np.ascontiguousarray(a, np.float32).ravel()  # noqa: E731
+    m = ref().ref_fwi_e2d(f32(vp), f32(vs), f32(rho), nx, nz, lpml, dx, dz, dt, nt, order, int(fs), ncheck,
+                          {"P": 0, "VX": 1, "VZ": 3}[srcfield], nsrc, sc.ravel(), f32(wav), nrec, gc.ravel(),
+                          f32(dobs), out["vpgrad"].reshape(-1), out["vsgrad"].reshape(-1),
+                          out["rhograd"].reshape(-1), dmod.reshape(-1), dres.reshape(-1), wg.reshape(-1))
+    return dict(misfit=np.float32(m), dmod=dmod, dres=dres, wavgrad=wg, **out)

@@ -356,6 +356,76 @@ float ref_fwi_a2d_full(const float *vp, const float *rho, int nx, int nz, int lp
                    wav, nrec, gcoords, dobs, vpgrad, rhograd, dmod, dres, wgrad);
 }
 
+// FwiElastic2D::run_optimal with in-core Revolve (ncheck snapshots), P, Ux
+// and Uz data (dobs = 3 records of nrec*nt: P, Ux, Uz), the source field
+// srcfield (0 P, 1 VX, 3 VZ).  Outputs the scaled Vp/Vs/rho gradients, the
+// modelled data and residuals (3 records each), the wavelet gradient; returns
+// the misfit.
+float ref_fwi_e2d(const float *vp, const float *vs, const float *rho, int nx, int nz, int lpml, float dx,
+                  float dz, float dt, int nt, int order, int fs, int ncheck, int srcfield, int nsrc,
+                  const float *scoords, const float *wav, int nrec, const float *gcoords,
+                  const float *dobs, float *vpgrad, float *vsgrad, float *rhograd, float *dmod,
+                  float *dres, float *wgrad) {
+    auto model = std::make_shared<ModelElastic2D<float>>(nx, nz, lpml, dx, dz, 0.f, 0.f, fs != 0);
+    model->createModel();
+    std::memcpy(model->getVp(), vp, sizeof(float) * nx * nz);
+    std::memcpy(model->getVs(), vs, sizeof(float) * nx * nz);
+    std::memcpy(model->getR(), rho, sizeof(float) * nx * nz);
+    auto src = std::make_shared<Data2D<float>>(nsrc, nt, dt, 0.f);
+    src->setField(field_of(srcfield));
+    fill2<float>(src, scoords, true);
+    std::memcpy(src->getData(), wav, sizeof(float) * nsrc * nt);
+    src->makeMap(model->getGeom(), SMAP);
+    const size_t nd = (size_t)nrec * nt;
+    auto mk = [&](rs_field f, const float *d) {
+        auto x = std::make_shared<Data2D<float>>(nrec, nt, dt, 0.f);
+        x->setField(f);
+        fill2<float>(x, gcoords, false);
+        if (d) std::memcpy(x->getData(), d, sizeof(float) * nd);
+        x->makeMap(model->getGeom(), GMAP);
+        return x;
+    };
+    const rs_field fields[3] = {PRESSURE, VX, VZ};
+    std::shared_ptr<Data2D<float>> obs[3], mod[3], res[3];
+    for (int k = 0; k < 3; k++) {
+        obs[k] = mk(fields[k], dobs + nd * k);
+        mod[k] = mk(fields[k], nullptr);
+        res[k] = mk(fields[k], nullptr);
+    }
+    model->staggerModels();
+    auto fwi = std::make_shared<FwiElastic2D<float>>(model, src, obs[0], obs[1], obs[2], order, 1);
+    fwi->setDatamodP(mod[0]);
+    fwi->setDatamodUx(mod[1]);
+    fwi->setDatamodUz(mod[2]);
+    fwi->setDataresP(res[0]);
+    fwi->setDataresUx(res[1]);
+    fwi->setDataresUz(res[2]);
+    auto vg = std::make_shared<Image2D<float>>("vpgrad_unused", model, 1, 1);
+    auto sg = std::make_shared<Image2D<float>>("vsgrad_unused", model, 1, 1);
+    auto rg = std::make_shared<Image2D<float>>("rhograd_unused", model, 1, 1);
+    fwi->setVpgrad(vg);
+    fwi->setVsgrad(sg);
+    fwi->setRhograd(rg);
+    auto wg = std::make_shared<Data2D<float>>(nsrc, nt, dt, 0.f);
+    wg->setField(field_of(srcfield));
+    fill2<float>(wg, scoords, true);
+    wg->makeMap(model->getGeom(), SMAP);
+    fwi->setWavgrad(wg);
+    fwi->setNcheck(ncheck);
+    fwi->setIncore(true);
+    fwi->setMisfit_type(DIFFERENCE);
+    fwi->run_optimal();
+    std::memcpy(vpgrad, vg->getImagedata(), sizeof(float) * nx * nz);
+    std::memcpy(vsgrad, sg->getImagedata(), sizeof(float) * nx * nz);
+    std::memcpy(rhograd, rg->getImagedata(), sizeof(float) * nx * nz);
+    for (int k = 0; k < 3; k++) {
+        std::memcpy(dmod + nd * k, mod[k]->getData(), sizeof(float) * nd);
+        std::memcpy(dres + nd * k, res[k]->getData(), sizeof(float) * nd);
+    }
+    std::memcpy(wgrad, wg->getData(), sizeof(float) * nsrc * nt);
+    return fwi->getMisfit();
+}
+
 // Revolve action sequence: actions[k] (advance=0 takeshot=1 restore=2
 // firsturn=3 youturn=4 terminate=5 error=6) and capo after the call.
 // Returns the number of actions written, or -1 if maxn was too small.

@@ -730,6 +730,126 @@ void FN(orc_a2d_crosscorr)(const REAL *wsp, int pads, const REAL *wrx, const REA
 #undef KM
 }
 
+/*
+ * FwiElastic2D::crossCorr(waves_fw, waves_bw, model, it) (fwi.cpp:2591-2775),
+ * non-domdec form: interior ix,iz in [1,n-1) of the unpadded nx*nz images.
+ * Forward fields ws* and adjoint fields wr*, rs* on padded grids (both pad
+ * lpml), Rx/Rz padded.  lam/mu/rho accumulate the kernels before scaleGrad
+ * (the reference's vpgrad/vsgrad/rhograd images at that point).  When
+ * wavgrad != NULL, the adjoint field of `wfield` (0 PRESSURE: Sxx+Szz,
+ * 1 VX: Vx, 3 VZ: Vz) is gathered bilinearly at the source traces whose map
+ * falls on this interior cell, into wavgrad[it + nt*i].
+ */
+void FN(orc_e2d_crosscorr)(const REAL *wsx, const REAL *wsz, const REAL *wsxx, const REAL *wszz,
+                           const REAL *wsxz, const REAL *wrx, const REAL *wrz, const REAL *rsxx,
+                           const REAL *rszz, const REAL *rsxz, const REAL *Rx, const REAL *Rz,
+                           int lpml, int nx, int nz, REAL dx, REAL dz, REAL *lam, REAL *mu,
+                           REAL *rho, REAL *wavgrad, int wfield, const int *smap, const REAL *sshift,
+                           int ntrace, int nt, int it)
+{
+    const int pads = lpml, padr = lpml;
+    const int nxs = nx + 2 * pads, nxr = nx + 2 * padr;
+#define KS(i, j) ((long)(j) * nxs + (i))
+#define KR(i, j) ((long)(j) * nxr + (i))
+#define KI(i, j) ((long)(j) * nx + (i))
+    for (int ix = 1; ix < nx - 1; ix++)
+        for (int iz = 1; iz < nz - 1; iz++) {
+            const int xs = ix + pads, zs = iz + pads, xr = ix + padr, zr = iz + padr;
+            REAL msxx = (wsx[KS(xs, zs)] - wsx[KS(xs - 1, zs)]) / dx;
+            REAL mszz = (wsz[KS(xs, zs)] - wsz[KS(xs, zs - 1)]) / dz;
+            lam[KI(ix, iz)] -= (msxx + mszz) * (rsxx[KR(xr, zr)] + rszz[KR(xr, zr)]);
+            REAL msxz1 = (wsx[KS(xs - 1, zs)] - wsx[KS(xs - 1, zs - 1)]) / dz +
+                         (wsz[KS(xs, zs - 1)] - wsz[KS(xs - 1, zs - 1)]) / dx;
+            REAL msxz2 = (wsx[KS(xs, zs)] - wsx[KS(xs, zs - 1)]) / dz +
+                         (wsz[KS(xs + 1, zs - 1)] - wsz[KS(xs, zs - 1)]) / dx;
+            REAL msxz3 = (wsx[KS(xs - 1, zs + 1)] - wsx[KS(xs - 1, zs)]) / dz +
+                         (wsz[KS(xs, zs)] - wsz[KS(xs - 1, zs)]) / dx;
+            REAL msxz4 = (wsx[KS(xs, zs + 1)] - wsx[KS(xs, zs)]) / dz +
+                         (wsz[KS(xs + 1, zs)] - wsz[KS(xs, zs)]) / dx;
+            REAL mrxz1 = rsxz[KR(xr - 1, zr - 1)];
+            REAL mrxz2 = rsxz[KR(xr, zr - 1)];
+            REAL mrxz3 = rsxz[KR(xr - 1, zr)];
+            REAL mrxz4 = rsxz[KR(xr, zr)];
+            REAL f1 = msxx * rszz[KR(xr, zr)] + mszz * rsxx[KR(xr, zr)];
+            REAL f2 = msxz1 * mrxz1 + msxz2 * mrxz2 + msxz3 * mrxz3 + msxz4 * mrxz4;
+            mu[KI(ix, iz)] = (REAL)((double)mu[KI(ix, iz)] + (2.0 * (double)f1 - 0.25 * (double)f2));
+            if (wavgrad) {
+                for (int i = 0; i < ntrace; i++) {
+                    if (smap[2 * i] != ix || smap[2 * i + 1] != iz) continue;
+                    REAL xtri[2], ytri[2];
+                    xtri[0] = 1 - sshift[2 * i];
+                    xtri[1] = sshift[2 * i];
+                    ytri[0] = 1 - sshift[2 * i + 1];
+                    ytri[1] = sshift[2 * i + 1];
+                    for (int i1 = 0; i1 < 2; i1++)
+                        for (int i2 = 0; i2 < 2; i2++) {
+                            long q = KR(xr + i2, zr + i1);
+                            REAL v = wfield == 0 ? rsxx[q] + rszz[q] : (wfield == 1 ? wrx[q] : wrz[q]);
+                            wavgrad[(long)i * nt + it] += xtri[i2] * ytri[i1] * v;
+                        }
+                }
+            }
+            /* rho kernel: double temporaries stored to T after each line */
+            REAL uderx, uderz;
+            uderx = (REAL)(0.5 * (double)wrx[KS(xs, zs)] * (double)Rx[KR(xr, zr)] *
+                           (double)(wsxx[KR(xr + 1, zr)] - wsxx[KR(xr, zr)]) / (double)dx);
+            uderx = (REAL)((double)uderx + 0.5 * (double)wrx[KS(xs - 1, zs)] * (double)Rx[KR(xr - 1, zr)] *
+                           (double)(wsxx[KR(xr, zr)] - wsxx[KR(xr - 1, zr)]) / (double)dx);
+            uderx = (REAL)((double)uderx + 0.5 * (double)wrx[KS(xs, zs)] * (double)Rx[KR(xr, zr)] *
+                           (double)(wsxz[KR(xr, zr)] - wsxz[KR(xr, zr - 1)]) / (double)dz);
+            uderx = (REAL)((double)uderx + 0.5 * (double)wrx[KS(xs - 1, zs)] * (double)Rx[KR(xr - 1, zr)] *
+                           (double)(wsxz[KR(xr - 1, zr)] - wsxz[KR(xr - 1, zr - 1)]) / (double)dz);
+            uderz = (REAL)(0.5 * (double)wrz[KS(xs, zs)] * (double)Rz[KR(xr, zr)] *
+                           (double)(wsxz[KR(xr, zr)] - wsxz[KR(xr - 1, zr)]) / (double)dx);
+            uderz = (REAL)((double)uderz + 0.5 * (double)wrz[KS(xs, zs - 1)] * (double)Rz[KR(xr, zr - 1)] *
+                           (double)(wsxz[KR(xr, zr - 1)] - wsxz[KR(xr - 1, zr - 1)]) / (double)dx);
+            uderz = (REAL)((double)uderz + 0.5 * (double)wrz[KS(xs, zs)] * (double)Rz[KR(xr, zr)] *
+                           (double)(wszz[KR(xr, zr + 1)] - wszz[KR(xr, zr)]) / (double)dz);
+            uderz = (REAL)((double)uderz + 0.5 * (double)wrz[KS(xs, zs - 1)] * (double)Rz[KR(xr, zr - 1)] *
+                           (double)(wszz[KR(xr, zr)] - wszz[KR(xr, zr - 1)]) / (double)dz);
+            rho[KI(ix, iz)] -= (uderx + uderz);
+        }
+#undef KS
+#undef KR
+#undef KI
+}
+
+/* FwiElastic2D::scaleGrad (fwi.cpp:2778-2849): lambda/mu/rho kernels ->
+ * Vp/Vs/rho gradients in place, over the whole unpadded model. */
+void FN(orc_e2d_scalegrad)(const REAL *Vp, const REAL *Vs, const REAL *Rho, int nx, int nz,
+                           REAL *vpgrad, REAL *vsgrad, REAL *rhograd)
+{
+    for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
+            long k = (long)iz * nx + ix;
+            REAL vpscale = (REAL)(2.0 * (double)Rho[k] * (double)Vp[k]);
+            REAL vsscale = (REAL)(2.0 * (double)Rho[k] * (double)Vs[k]);
+            REAL rhoscale1 = (REAL)((double)(Vp[k] * Vp[k]) - 2.0 * (double)Vs[k] * (double)Vs[k]);
+            REAL rhoscale2 = Vs[k] * Vs[k];
+            REAL lambda = vpgrad[k], mu = vsgrad[k], rho = rhograd[k];
+            vpgrad[k] = vpscale * lambda;
+            vsgrad[k] = (REAL)((double)(vsscale * mu) - 2.0 * (double)vsscale * (double)lambda);
+            rhograd[k] = rhoscale1 * lambda + rhoscale2 * mu + rho;
+        }
+}
+
+/* Elastic DIFFERENCE misfit over the P, Ux, Uz records (fwi.cpp:2852-2930):
+ * misfit += 0.5*(rp^2 + rx^2 + rz^2) in T, traces whose Ux map is valid. */
+REAL FN(orc_misfit_diff3)(const REAL *modp, const REAL *recp, const REAL *modx, const REAL *recx,
+                          const REAL *modz, const REAL *recz, const int *valid, int ntr, int nt)
+{
+    REAL misfit = 0.0;
+    for (int i = 0; i < ntr; i++) {
+        if (!valid[i]) continue;
+        for (int it = 0; it < nt; it++) {
+            long q = (long)i * nt + it;
+            REAL rp = modp[q] - recp[q], rx = modx[q] - recx[q], rz = modz[q] - recz[q];
+            misfit = (REAL)((double)misfit + 0.5 * (double)(rp * rp + rx * rx + rz * rz));
+        }
+    }
+    return misfit;
+}
+
 /* DIFFERENCE misfit 0.5*sum(r^2) over traces with valid maps, accumulated
  * in T (fwi.cpp:574-608); `valid[i]` nonzero when map x,z >= 0. */
 REAL FN(orc_misfit_diff)(const REAL *mod, const REAL *rec, const int *valid, int ntr, int nt)

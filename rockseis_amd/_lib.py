@@ -51,10 +51,10 @@ class Fwi(C.Structure):
                 + [("nx", C.c_int), ("nz", C.c_int), ("src", Traces), ("obs", Traces)]
                 + [(n, C.c_void_p) for n in ("dmod", "dres", "wavgrad", "vpgrad", "rhograd")]
                 + [("misfit", C.POINTER(C.c_double)), ("ncheck", C.c_int), ("mode", C.c_int),
-                   ("mode_used", C.POINTER(C.c_int))])
+                   ("mode_used", C.POINTER(C.c_int)), ("snapinc", C.c_int), ("fp16", C.c_int)])
 
 
-FWI_REVOLVE, FWI_STORE = 0, 1
+FWI_REVOLVE, FWI_STORE, FWI_FULL = 0, 1, 2
 
 
 class RsgError(RuntimeError):
@@ -109,6 +109,7 @@ def _declare(L):
         "rsg_e2d_stress_batch": ([GP, SP, C.POINTER(Batch), ip, vp], ip),
         "rsg_revolve_schedule": ([ip, ip, vp, vp, vp, ip, C.POINTER(C.c_int)], ip),
         "rsg_revolve_numforw": ([ip, ip], ip),
+        "rsg_snap_cadence": ([ip, C.c_float, ip, C.POINTER(C.c_int), C.POINTER(C.c_int)], ip),
         "rsg_get_tuning": ([C.c_char_p, C.POINTER(C.c_int)], ip),
     }
     for name in ("a2d_forward_velocity", "a2d_forward_stress", "e2d_forward_velocity",
@@ -131,7 +132,7 @@ EXPORTS = (
     "rsg_e2d_backward_stress", "rsg_a3d_forward_velocity", "rsg_a3d_forward_stress",
     "rsg_insert_source", "rsg_record_data", "rsg_modelling_run", "rsg_repeat_step",
     "rsg_set_tuning", "rsg_get_tuning", "rsg_fwi_a2d_gradient", "rsg_revolve_schedule",
-    "rsg_revolve_numforw", "rsg_fwi_a2d_gradient_batch", "rsg_a2d_forward_velocity_batch",
+    "rsg_revolve_numforw", "rsg_snap_cadence", "rsg_fwi_a2d_gradient_batch", "rsg_a2d_forward_velocity_batch",
     "rsg_a2d_forward_stress_batch", "rsg_e2d_velocity_batch", "rsg_e2d_stress_batch",
 )
 

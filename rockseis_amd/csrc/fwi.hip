@@ -1,26 +1,37 @@
-// Acoustic 2D FWI gradient of one shot (FwiAcoustic2D::run_optimal,
-// fwi.cpp:1159-1332) on the device.
+// Acoustic 2D FWI gradient (FwiAcoustic2D::run_optimal, fwi.cpp:1159-1332,
+// and FwiAcoustic2D::run, fwi.cpp:1030-1156) on the device, for one shot or a
+// batch of shots on one grid.
 //
-// The forward wavefield is stepped with the fused acoustic kernels; the
-// Revolve schedule (revolve.cpp:150-305) runs on the host and its checkpoints
-// (pressure, velocities and the 8 PML strips, the payload of
-// revolve.cpp:648-700) live in HBM.  As in the reference, the adjoint
-// wavefield is advanced with the forward operator and driven by the DIFFERENCE
-// residuals injected at the receivers; every reverse step accumulates the
-// zero-lag cross-correlation of crossCorr (fwi.cpp:511-571) into the Vp and
-// rho gradients.
+// The forward wavefield is stepped with the fused acoustic kernels.  As in
+// the reference, the adjoint wavefield is advanced with the forward operator
+// and driven by the DIFFERENCE residuals injected at the receivers; reverse
+// steps accumulate the zero-lag cross-correlation of crossCorr
+// (fwi.cpp:511-571) into the Vp and rho gradients.  Three ways to get the
+// forward pressure back in reverse time:
 //
-// Mode RSG_FWI_STORE keeps the forward pressure of every step in HBM instead
-// (its unpadded nx*nz part: 3.9 GB per shot for 801x401 and 3000 steps),
-// so the forward sweep runs once; the gradient is identical because the
-// reference correlates, at reverse step capo, the forward pressure after
-// capo-1 forward steps (after nt-1 steps at the first turn).
+//  RSG_FWI_REVOLVE  run_optimal: the Revolve schedule (revolve.cpp:150-305)
+//                   runs on the host; its checkpoints (pressure, velocities,
+//                   the 8 PML strips: the payload of revolve.cpp:648-700)
+//                   live in HBM.
+//  RSG_FWI_STORE    same gradient as run_optimal, forward sweep once: the
+//                   unpadded pressure after every step is kept in HBM (the
+//                   reference correlates, at reverse step capo, the forward
+//                   pressure after capo-1 steps; after nt-1 at the first
+//                   turn).  Falls back to REVOLVE when it does not fit.
+//  RSG_FWI_FULL     run: the unpadded pressure every snapinc-th step (the
+//                   Snapshot2D cadence, snap.cpp:289-346, 638-689) kept in
+//                   HBM instead of a snapshot file; the adjoint correlates at
+//                   reverse times t with (t - enddiff) % snapinc == 0.
+//
+// With fp16 set, snapshots and checkpoints are stored as FP16 with one
+// power-of-two scale per stored array (rsg_store16.h).
 #include <cstring>
 #include <vector>
 
 #include "rsg_common.h"
 #include "rsg_fwi.h"
 #include "rsg_srcrec.h"
+#include "rsg_store16.h"
 
 namespace rsg {
 
@@ -146,20 +157,21 @@ Revolve::Action Revolve::next() {
 // ---------------------------------------------------------------------------
 
 // crossCorr for one reverse step, interior ix, iz in [1, n-1) of the
-// unpadded nx x nz image (fwi.cpp:550-569); P_f, Vx_a, Vz_a, Rx, Rz padded
-// by lpml.  The rho-gradient velocity terms keep the reference's double
+// unpadded nx x nz image (fwi.cpp:550-569); Vx_a, Vz_a, Rx, Rz padded by
+// lpml.  The rho-gradient velocity terms keep the reference's double
 // temporaries.  A batch of nshot shots (fields fstride apart) is summed into
 // the one image, shot 0 first.
-// Pf is either the padded forward pressure (pf_pad = lpml) or its unpadded
-// copy kept by the store-every-step mode (pf_pad = 0); pf_stride separates
-// its shots.
-__global__ void crosscorr_a2d_kernel(const float *__restrict__ Pf, const float *__restrict__ Vxa,
-                                     const float *__restrict__ Vza, const float *__restrict__ Rx,
-                                     const float *__restrict__ Rz, const float *__restrict__ Vp,
-                                     const float *__restrict__ Rho, int nx, int nz, int lpml,
-                                     float dx, float dz, float *__restrict__ vpgrad,
-                                     float *__restrict__ rhograd, int nshot, size_t fstride,
-                                     int pf_pad, size_t pf_stride) {
+// Pf is either the padded forward pressure (pf_pad = lpml) or a stored
+// unpadded snapshot (pf_pad = 0), FP32 or (F16) FP16 scaled by amax[shot];
+// pf_stride separates its shots.
+template <bool F16>
+__global__ void crosscorr_a2d_kernel(const void *__restrict__ Pf_, const unsigned *__restrict__ amax,
+                                     const float *__restrict__ Vxa, const float *__restrict__ Vza,
+                                     const float *__restrict__ Rx, const float *__restrict__ Rz,
+                                     const float *__restrict__ Vp, const float *__restrict__ Rho, int nx,
+                                     int nz, int lpml, float dx, float dz, float *__restrict__ vpgrad,
+                                     float *__restrict__ rhograd, int nshot, size_t fstride, int pf_pad,
+                                     size_t pf_stride) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int iz = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (ix >= nx - 1 || iz >= nz - 1) return;
@@ -173,32 +185,36 @@ __global__ void crosscorr_a2d_kernel(const float *__restrict__ Pf, const float *
     const double rx0 = Rx[q], rx1 = Rx[q - 1], rz0 = Rz[q], rz1 = Rz[q - nxp];
     float vg = vpgrad[m], rg = rhograd[m];
     for (int k = 0; k < nshot; k++) {
-        const float *P = Pf + k * pf_stride, *Vx = Vxa + k * fstride, *Vz = Vza + k * fstride;
+        float pc, pe, pw, ps, pn;
+        if constexpr (F16) {
+            const half_t *P = (const half_t *)Pf_ + k * pf_stride;
+            const float us = store16_unscale(amax[k]);
+            pc = (float)P[qp] * us;
+            pe = (float)P[qp + 1] * us;
+            pw = (float)P[qp - 1] * us;
+            ps = (float)P[qp + pld] * us;
+            pn = (float)P[qp - pld] * us;
+        } else {
+            const float *P = (const float *)Pf_ + k * pf_stride;
+            pc = P[qp];
+            pe = P[qp + 1];
+            pw = P[qp - 1];
+            ps = P[qp + pld];
+            pn = P[qp - pld];
+        }
+        const float *Vx = Vxa + k * fstride, *Vz = Vza + k * fstride;
         const float mrxx = (Vx[q] - Vx[q - 1]) / dx;
         const float mrzz = (Vz[q] - Vz[q - nxp]) / dz;
-        const float ws = P[qp];
-        vg -= vpscale * ws * (mrxx + mrzz);
-        rg -= rhoscale * ws * (mrxx + mrzz);
-        float ux = (float)(0.5 * (double)Vx[q] * rx0 * (double)(P[qp + 1] - ws) / (double)dx);
-        ux = (float)((double)ux + 0.5 * (double)Vx[q - 1] * rx1 * (double)(ws - P[qp - 1]) / (double)dx);
-        float uz = (float)(0.5 * (double)Vz[q] * rz0 * (double)(P[qp + pld] - ws) / (double)dz);
-        uz = (float)((double)uz + 0.5 * (double)Vz[q - nxp] * rz1 * (double)(ws - P[qp - pld]) / (double)dz);
+        vg -= vpscale * pc * (mrxx + mrzz);
+        rg -= rhoscale * pc * (mrxx + mrzz);
+        float ux = (float)(0.5 * (double)Vx[q] * rx0 * (double)(pe - pc) / (double)dx);
+        ux = (float)((double)ux + 0.5 * (double)Vx[q - 1] * rx1 * (double)(pc - pw) / (double)dx);
+        float uz = (float)(0.5 * (double)Vz[q] * rz0 * (double)(ps - pc) / (double)dz);
+        uz = (float)((double)uz + 0.5 * (double)Vz[q - nxp] * rz1 * (double)(pc - pn) / (double)dz);
         rg += ux + uz;
     }
     vpgrad[m] = vg;
     rhograd[m] = rg;
-}
-
-// Unpadded nx x nz window of every shot's padded pressure -> dst
-// [nshot][nz][nx] (what crossCorr reads of the forward wavefield).
-__global__ void pack_interior_kernel(const float *__restrict__ src, float *__restrict__ dst, int nx,
-                                     int nz, int lpml, size_t sstride) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iz = blockIdx.y;
-    const int k = blockIdx.z;
-    if (ix >= nx) return;
-    const int nxp = nx + 2 * lpml;
-    dst[((size_t)k * nz + iz) * nx + ix] = src[k * sstride + (size_t)(iz + lpml) * nxp + ix + lpml];
 }
 
 // DIFFERENCE misfit (fwi.cpp:574-608) and residuals (fwi.cpp:772-800).  The
@@ -291,6 +307,9 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
         return fail(RSG_EINVAL, "fwi: source and data must be PRESSURE traces with equal nt");
     if (f->nx + 2 * g->lpml != g->nx || f->nz + 2 * g->lpml != g->nz)
         return fail(RSG_EINVAL, "fwi: image size must be the unpadded grid");
+    if (f->mode == RSG_FWI_FULL && f->snapinc < 1) return fail(RSG_EINVAL, "fwi: snapinc must be >= 1");
+    const bool f16 = f->fp16 != 0;
+    if (f16 && 4 * ns > 65535) return fail(RSG_EINVAL, "fwi: FP16 storage takes at most 16383 shots");
     DevBuf db;
     A2DWaves fw, bw;
     if (int rc = fw.init(db, g, f, ns, st)) return rc;
@@ -328,53 +347,117 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
         RSG_LAUNCH_CHECK("misfit_residual_kernel");
         return RSG_OK;
     };
-    auto reverse_tail = [&](int capo, const float *Pf, int pf_pad, size_t pf_stride) -> int {
-        if (int rc = insert_source(RSG_KIND_A2D, g, &bw.st, &res, nullptr, capo, st, ns, fs)) return rc;
-        crosscorr_a2d_kernel<<<xg, xb, 0, st>>>(Pf, bw.st.Vx, bw.st.Vz, f->Rx, f->Rz, f->Vp, f->Rho, f->nx,
-                                                f->nz, g->lpml, g->dx, g->dz, f->vpgrad, f->rhograd, ns, fs,
-                                                pf_pad, pf_stride);
-        RSG_LAUNCH_CHECK("crosscorr_a2d_kernel");
+    // Residual injection at time t, crossCorr with Pf (skipped when null;
+    // FP16 when amax is set) and the wavelet gradient at t.
+    auto reverse_tail = [&](int t, const void *Pf, int pf_pad, size_t pf_stride, const unsigned *amax) -> int {
+        if (int rc = insert_source(RSG_KIND_A2D, g, &bw.st, &res, nullptr, t, st, ns, fs)) return rc;
+        if (Pf) {
+            if (amax)
+                crosscorr_a2d_kernel<true><<<xg, xb, 0, st>>>(Pf, amax, bw.st.Vx, bw.st.Vz, f->Rx, f->Rz, f->Vp,
+                                                              f->Rho, f->nx, f->nz, g->lpml, g->dx, g->dz,
+                                                              f->vpgrad, f->rhograd, ns, fs, pf_pad, pf_stride);
+            else
+                crosscorr_a2d_kernel<false><<<xg, xb, 0, st>>>(Pf, nullptr, bw.st.Vx, bw.st.Vz, f->Rx, f->Rz,
+                                                               f->Vp, f->Rho, f->nx, f->nz, g->lpml, g->dx,
+                                                               g->dz, f->vpgrad, f->rhograd, ns, fs, pf_pad,
+                                                               pf_stride);
+            RSG_LAUNCH_CHECK("crosscorr_a2d_kernel");
+        }
         if (f->wavgrad)
-            if (int rc = record_data(RSG_KIND_A2D, g, &bw.st, &wg, nullptr, capo, st, ns, fs)) return rc;
+            if (int rc = record_data(RSG_KIND_A2D, g, &bw.st, &wg, nullptr, t, st, ns, fs)) return rc;
         return RSG_OK;
     };
 
-    // The store mode keeps the unpadded forward pressure of every shot after
-    // every step (slot k = after step k); it falls back to Revolve when that
-    // does not fit in free device memory (with 2 GiB to spare).
+    // Snapshot history of the STORE and FULL modes: nslot slots of the
+    // unpadded pressure of every shot, FP32 or FP16 (+ one amax per shot).
     const size_t nint = (size_t)f->nx * f->nz, slot = (size_t)ns * nint;
-    int mode = f->mode;
-    if (mode == RSG_FWI_STORE) {
+    const size_t esz = f16 ? sizeof(half_t) : sizeof(float);
+    int mode = f->mode, snapnt = nt, enddiff = 0;
+    if (mode == RSG_FWI_FULL)
+        if (int rc = rsg_snap_cadence(nt, g->dt, f->snapinc, &snapnt, &enddiff)) return rc;
+    if (mode != RSG_FWI_REVOLVE) {
         size_t free_b = 0, total_b = 0;
         RSG_HIP(hipMemGetInfo(&free_b, &total_b));
-        if (slot * (size_t)nt * sizeof(float) + ((size_t)2 << 30) > free_b) mode = RSG_FWI_REVOLVE;
+        const bool fits = slot * (size_t)snapnt * esz + ((size_t)2 << 30) <= free_b;
+        if (!fits && mode == RSG_FWI_FULL)
+            return fail(RSG_ENOMEM, "fwi: the snapshots of RSG_FWI_FULL do not fit in device memory");
+        if (!fits) mode = RSG_FWI_REVOLVE;
     }
     if (f->mode_used) *f->mode_used = mode;
+    char *hist = nullptr;
+    unsigned *hamax = nullptr;
+    if (mode != RSG_FWI_REVOLVE) {
+        if (int rc = db.alloc(&hist, slot * (size_t)snapnt * esz)) return rc;
+        if (f16)
+            if (int rc = db.alloc(&hamax, (size_t)ns * snapnt)) return rc;
+    }
+    auto snap_at = [&](int k) -> void * { return hist + slot * (size_t)k * esz; };
+    auto amax_at = [&](int k) -> unsigned * { return f16 ? hamax + (size_t)ns * k : nullptr; };
+    auto take_snap = [&](int k) -> int {
+        return pack_interior(fw.st.P, snap_at(k), amax_at(k), f->nx, f->nz, g->lpml, fw.n, ns, f16, st);
+    };
+
     if (mode == RSG_FWI_STORE) {
-        float *hist = nullptr;
-        if (int rc = db.alloc(&hist, slot * (size_t)nt)) return rc;
-        const dim3 pb(256), pg((f->nx + 255) / 256, f->nz, ns);
         for (int it = 0; it < nt; it++) {
             if (int rc = fstep(it, true)) return rc;
-            pack_interior_kernel<<<pg, pb, 0, st>>>(fw.st.P, hist + slot * it, f->nx, f->nz, g->lpml, fw.n);
-            RSG_LAUNCH_CHECK("pack_interior_kernel");
+            if (int rc = take_snap(it)) return rc;
         }
         if (int rc = first_turn()) return rc;
-        if (int rc = reverse_tail(nt - 1, hist + slot * (nt - 1), 0, nint)) return rc;
+        if (int rc = reverse_tail(nt - 1, snap_at(nt - 1), 0, nint, amax_at(nt - 1))) return rc;
         for (int capo = nt - 2; capo >= 0; capo--) {
             if (int rc = bstep()) return rc;
             // state after capo-1 steps; before the first step it is zero
             if (capo >= 1) {
-                if (int rc = reverse_tail(capo, hist + slot * (capo - 1), 0, nint)) return rc;
+                if (int rc = reverse_tail(capo, snap_at(capo - 1), 0, nint, amax_at(capo - 1))) return rc;
             } else {
                 if (int rc = launch_zero(fw.st.P, (size_t)ns * fw.n * sizeof(float), st)) return rc;
-                if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n)) return rc;
+                if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n, nullptr)) return rc;
             }
+        }
+    } else if (mode == RSG_FWI_FULL) {
+        for (int it = 0; it < nt; it++) {
+            if (int rc = fstep(it, true)) return rc;
+            if (it % f->snapinc == 0)
+                if (int rc = take_snap(it / f->snapinc)) return rc;
+        }
+        if (int rc = first_turn()) return rc;
+        for (int t = nt - 1; t >= 0; t--) {
+            if (int rc = bstep()) return rc;
+            const bool corr = (t - enddiff) % f->snapinc == 0;
+            const int k = (t - enddiff) / f->snapinc;
+            if (int rc = reverse_tail(t, corr ? snap_at(k) : nullptr, 0, nint, corr ? amax_at(k) : nullptr))
+                return rc;
         }
     } else {
         Revolve rev(nt, f->ncheck);
+        // checkpoint = P, Vx, Vz (3*ns arrays of n), 4*ns x strips, 4*ns z strips
         float *ck = nullptr;
-        if (int rc = db.alloc(&ck, fw.total * (size_t)rev.snaps)) return rc;
+        half_t *ck16 = nullptr;
+        unsigned *ckamax = nullptr;
+        const int nseg = 11 * ns;
+        if (f16) {
+            if (int rc = db.alloc(&ck16, fw.total * (size_t)rev.snaps)) return rc;
+            if (int rc = db.alloc(&ckamax, (size_t)nseg * rev.snaps)) return rc;
+        } else {
+            if (int rc = db.alloc(&ck, fw.total * (size_t)rev.snaps)) return rc;
+        }
+        const size_t xoff = 3 * fw.n * ns, zoff = xoff + 4 * fw.nstrip_x * ns;
+        auto ck_pack = [&](int c) -> int {
+            half_t *d = ck16 + fw.total * c;
+            unsigned *am = ckamax + (size_t)nseg * c;
+            if (int rc = pack16_segments(fw.base, d, am, fw.n, 3 * ns, st)) return rc;
+            if (int rc = pack16_segments(fw.base + xoff, d + xoff, am + 3 * ns, fw.nstrip_x, 4 * ns, st))
+                return rc;
+            return pack16_segments(fw.base + zoff, d + zoff, am + 7 * ns, fw.nstrip_z, 4 * ns, st);
+        };
+        auto ck_unpack = [&](int c) -> int {
+            const half_t *d = ck16 + fw.total * c;
+            const unsigned *am = ckamax + (size_t)nseg * c;
+            if (int rc = unpack16_segments(d, am, fw.base, fw.n, 3 * ns, st)) return rc;
+            if (int rc = unpack16_segments(d + xoff, am + 3 * ns, fw.base + xoff, fw.nstrip_x, 4 * ns, st))
+                return rc;
+            return unpack16_segments(d + zoff, am + 7 * ns, fw.base + zoff, fw.nstrip_z, 4 * ns, st);
+        };
         bool reverse = false;
         for (;;) {
             const int oldcapo = rev.capo;
@@ -386,17 +469,25 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
             } else if (a == Revolve::FIRSTURN) {
                 if (int rc = fstep(capo, true)) return rc;
                 if (int rc = first_turn()) return rc;
-                if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n)) return rc;
+                if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n, nullptr)) return rc;
                 reverse = true;
             } else if (a == Revolve::YOUTURN) {
                 if (int rc = bstep()) return rc;
-                if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n)) return rc;
+                if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n, nullptr)) return rc;
             } else if (a == Revolve::TAKESHOT) {
-                RSG_HIP(hipMemcpyAsync(ck + fw.total * rev.check, fw.base, fw.total * sizeof(float),
-                                       hipMemcpyDeviceToDevice, st));
+                if (f16) {
+                    if (int rc = ck_pack(rev.check)) return rc;
+                } else {
+                    RSG_HIP(hipMemcpyAsync(ck + fw.total * rev.check, fw.base, fw.total * sizeof(float),
+                                           hipMemcpyDeviceToDevice, st));
+                }
             } else if (a == Revolve::RESTORE) {
-                RSG_HIP(hipMemcpyAsync(fw.base, ck + fw.total * rev.check, fw.total * sizeof(float),
-                                       hipMemcpyDeviceToDevice, st));
+                if (f16) {
+                    if (int rc = ck_unpack(rev.check)) return rc;
+                } else {
+                    RSG_HIP(hipMemcpyAsync(fw.base, ck + fw.total * rev.check, fw.total * sizeof(float),
+                                           hipMemcpyDeviceToDevice, st));
+                }
             } else if (a == Revolve::TERMINATE) {
                 break;
             } else {
@@ -417,7 +508,7 @@ extern "C" int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, voi
     if (!g || !f || !f->L || !f->Rx || !f->Rz || !f->tab || !f->Vp || !f->Rho || !f->vpgrad ||
         !f->rhograd || !f->dmod || !f->dres || !f->obs.data)
         return rsg::fail(RSG_EINVAL, "rsg_fwi_a2d_gradient: null argument");
-    if (f->mode != RSG_FWI_REVOLVE && f->mode != RSG_FWI_STORE)
+    if (f->mode != RSG_FWI_REVOLVE && f->mode != RSG_FWI_STORE && f->mode != RSG_FWI_FULL)
         return rsg::fail(RSG_EINVAL, "rsg_fwi_a2d_gradient: unknown mode");
     return rsg::fwi_a2d_impl(g, f, 1, (hipStream_t)stream);
 }
@@ -428,7 +519,7 @@ extern "C" int rsg_fwi_a2d_gradient_batch(const rsg_grid *g, const rsg_fwi_a2d *
         !f->rhograd || !f->dmod || !f->dres || !f->obs.data)
         return rsg::fail(RSG_EINVAL, "rsg_fwi_a2d_gradient_batch: null argument");
     if (nshot < 1 || nshot > 65535) return rsg::fail(RSG_EINVAL, "rsg_fwi_a2d_gradient_batch: nshot 1..65535");
-    if (f->mode != RSG_FWI_REVOLVE && f->mode != RSG_FWI_STORE)
+    if (f->mode != RSG_FWI_REVOLVE && f->mode != RSG_FWI_STORE && f->mode != RSG_FWI_FULL)
         return rsg::fail(RSG_EINVAL, "rsg_fwi_a2d_gradient_batch: unknown mode");
     return rsg::fwi_a2d_impl(g, f, nshot, (hipStream_t)stream);
 }

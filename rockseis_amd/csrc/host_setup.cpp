@@ -242,4 +242,22 @@ int rsg_makemap3d(const float *coords, int ntrace, int nx, int ny, int nz, float
     return RSG_OK;
 }
 
+// Snapshot2D/3D constructor (snap.cpp:330-335): snapnt = (nt-1)/snapinc + 1
+// snapshots; enddiff = rint(((nt-1)dt - (snapnt-1) snapinc dt) / dt), in
+// float as the reference computes it.
+int rsg_snap_cadence(int nt, float dt, int snapinc, int *snapnt, int *enddiff) {
+    if (!snapnt || !enddiff || nt < 1 || snapinc < 1 || !(dt > 0.f))
+        return rsg::fail(RSG_EINVAL, "rsg_snap_cadence: bad arguments");
+    const int n = (nt - 1) / snapinc + 1;
+    const float snapdt = dt * (float)snapinc;
+    // volatile: both products are rounded to float before the subtraction
+    // (no fused multiply-add), as in the reference build
+    const volatile float a = (float)(nt - 1) * dt;
+    const volatile float b = (float)(n - 1) * snapdt;
+    const float d = (a - b) / dt;
+    *snapnt = n;
+    *enddiff = (int)rintf(d);
+    return RSG_OK;
+}
+
 }  // extern "C"

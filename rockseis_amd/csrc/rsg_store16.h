@@ -1,0 +1,50 @@
+// FP16 storage of wavefield snapshots and checkpoints with a per-snapshot
+// power-of-two scale.
+//
+// Each stored array (one shot's snapshot, or one field / PML strip of one
+// shot's checkpoint) gets its own scale 2^(15-E), where amax = m * 2^E,
+// m in [0.5, 1), is the array's largest magnitude: the stored values then lie
+// in (-32768, 32768), inside FP16's finite range, and the scaling itself is
+// exact, so the only rounding is FP32 -> FP16 (round to nearest even,
+// relative error <= 2^-11 for values above amax * 2^-29).  An all-zero array
+// keeps scale 1.  amax is kept as the bit pattern of a non-negative float,
+// which orders like an unsigned integer, so it is reduced with atomicMax.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace rsg {
+
+using half_t = _Float16;
+
+__device__ inline float store16_scale(unsigned amax_bits) {
+    const float a = __uint_as_float(amax_bits);
+    if (!(a > 0.f) || !isfinite(a)) return 1.f;
+    int e;
+    (void)frexpf(a, &e);
+    return ldexpf(1.f, 15 - e);
+}
+
+__device__ inline float store16_unscale(unsigned amax_bits) {
+    const float a = __uint_as_float(amax_bits);
+    if (!(a > 0.f) || !isfinite(a)) return 1.f;
+    int e;
+    (void)frexpf(a, &e);
+    return ldexpf(1.f, e - 15);
+}
+
+// Contiguous segments: segment s is src[s*seglen .. (s+1)*seglen).  amax
+// must hold nseg zeroed entries (pack16_segments zeroes them first).
+int pack16_segments(const float *src, half_t *dst, unsigned *amax, size_t seglen, int nseg,
+                    hipStream_t st);
+int unpack16_segments(const half_t *src, const unsigned *amax, float *dst, size_t seglen, int nseg,
+                      hipStream_t st);
+
+// Unpadded nx x nz window of every shot's padded (lpml) field, shots sstride
+// apart -> dst [nshot][nz][nx] in FP32 (amax == nullptr) or FP16 with one
+// scale per shot in amax[nshot].
+int pack_interior(const float *src, void *dst, unsigned *amax, int nx, int nz, int lpml, size_t sstride,
+                  int nshot, bool fp16, hipStream_t st);
+
+}  // namespace rsg

@@ -3,9 +3,13 @@
 
 FwiAcoustic2D(model, source, dataP, order, snapinc) with setDatamodP /
 setDataresP / setVpgrad / setRhograd / setWavgrad / setNcheck / setIncore,
-then run_optimal() (Revolve checkpoints kept in HBM) or run() (every forward
-pressure kept in HBM, no recomputation; same gradient), getMisfit().  The
-per-shot work is one call of rsg_fwi_a2d_gradient.
+then run_optimal() (Revolve checkpoints kept in HBM), run() (the reference's
+full-snapshot gradient: the pressure every snapinc-th step kept in HBM in
+place of the snapshot file) or run_stored() (run_optimal's gradient with
+every forward pressure kept in HBM: one forward sweep), getMisfit().
+setFp16Storage(True) stores those snapshots / checkpoints as FP16 with a
+power-of-two scale per stored array.  The per-shot work is one call of
+rsg_fwi_a2d_gradient.
 
 AcousticFwiMisfit is the torch.autograd.Function (Vp, rho) -> misfit summed
 over shots, whose backward returns the reference's Vp and rho gradients.
@@ -21,6 +25,7 @@ from . import _lib as R
 from .core import GMAP, SMAP, Data2D, ModelAcoustic2D, Pml, SMAX, _dev, _stream
 
 DIFFERENCE = 0  # rs_fwimisfit (utils.h:27)
+_MODES = {"revolve": R.FWI_REVOLVE, "store": R.FWI_STORE, "full": R.FWI_FULL}
 
 
 class Image2D:
@@ -43,6 +48,7 @@ class FwiAcoustic2D:
         self.datamodP = self.dataresP = self.vpgrad = self.rhograd = self.wavgrad = None
         self.misfit = 0.0
         self.misfit_type = DIFFERENCE
+        self.fp16 = False
 
     def setDatamodP(self, d):
         self.datamodP = d
@@ -70,6 +76,9 @@ class FwiAcoustic2D:
             raise R.RsgError("only the DIFFERENCE misfit is implemented on this path")
         self.misfit_type = DIFFERENCE
 
+    def setFp16Storage(self, v):
+        self.fp16 = bool(v)
+
     def getMisfit(self):
         return self.misfit
 
@@ -77,6 +86,9 @@ class FwiAcoustic2D:
         return self._run(R.FWI_REVOLVE)
 
     def run(self):
+        return self._run(R.FWI_FULL)
+
+    def run_stored(self):
         return self._run(R.FWI_STORE)
 
     def _run(self, mode):
@@ -113,6 +125,9 @@ class FwiAcoustic2D:
         f.vpgrad, f.rhograd = vg.data_ptr(), rg.data_ptr()
         f.misfit = C.pointer(misfit)
         f.ncheck, f.mode = self.ncheck, mode
+        f.snapinc, f.fp16 = self.snapinc, int(self.fp16)
+        used = C.c_int(-1)
+        f.mode_used = C.pointer(used)
         R.check(R.lib().rsg_fwi_a2d_gradient(C.byref(g), C.byref(f), _stream()), "rsg_fwi_a2d_gradient")
         self.misfit = float(np.float32(misfit.value))
         self.vpgrad.imagedata[...] = vg.cpu().numpy()
@@ -124,6 +139,7 @@ class FwiAcoustic2D:
         if self.wavgrad is not None:
             self.wavgrad.data[...] = wg.cpu().numpy()
         self.misfit64 = misfit.value
+        self.mode_used = {R.FWI_REVOLVE: "revolve", R.FWI_STORE: "store", R.FWI_FULL: "full"}[used.value]
         return 1
 
 
@@ -139,9 +155,15 @@ class Shot:
 
 
 class Survey:
+    """Shots on one model grid plus the gradient settings: mode "revolve"
+    (run_optimal), "store" (run_optimal's gradient, every forward pressure in
+    HBM) or "full" (run with snapshots every snapinc steps); fp16 stores the
+    snapshots / checkpoints as scaled FP16."""
+
     def __init__(self, shots, lpml, dx, dz, dt, order=4, fs=False, ncheck=50, mode="revolve",
-                 batched=True):
+                 batched=True, snapinc=1, fp16=False):
         self.shots = list(shots)
+        self.snapinc, self.fp16 = int(snapinc), bool(fp16)
         self.batched = bool(batched)
         self.lpml, self.dx, self.dz, self.dt = int(lpml), float(dx), float(dz), float(dt)
         self.order, self.fs, self.ncheck = int(order), bool(fs), int(ncheck)
@@ -164,12 +186,15 @@ def shot_gradient(vp, rho, survey: Survey, shot: Shot):
     obs.data[:] = shot.dobs
     obs.makeMap(m, GMAP)
     m.staggerModels()
-    fwi = FwiAcoustic2D(m, src, obs, survey.order, 1)
+    fwi = FwiAcoustic2D(m, src, obs, survey.order, survey.snapinc)
+    fwi.setFp16Storage(survey.fp16)
     vg, rg = Image2D(m), Image2D(m)
     fwi.setVpgrad(vg)
     fwi.setRhograd(rg)
     fwi.setNcheck(survey.ncheck)
     if survey.mode == "store":
+        fwi.run_stored()
+    elif survey.mode == "full":
         fwi.run()
     else:
         fwi.run_optimal()
@@ -238,10 +263,11 @@ def batch_gradient(vp, rho, survey: Survey, mode=None):
     f.misfit = C.pointer(mis)
     f.mode_used = C.pointer(used)
     mode = mode or survey.mode
-    f.ncheck, f.mode = survey.ncheck, (R.FWI_STORE if mode == "store" else R.FWI_REVOLVE)
+    f.ncheck, f.mode = survey.ncheck, _MODES[mode]
+    f.snapinc, f.fp16 = survey.snapinc, int(survey.fp16)
     R.check(R.lib().rsg_fwi_a2d_gradient_batch(C.byref(g), C.byref(f), ns, _stream()),
             "rsg_fwi_a2d_gradient_batch")
-    survey.mode_used = "store" if used.value == R.FWI_STORE else "revolve"
+    survey.mode_used = {v: k for k, v in _MODES.items()}[used.value]
     return mis.value, vg, rg, dmod
 
 

@@ -167,6 +167,64 @@ def fwi_fixtures(R):
              wavgrad=r["wavgrad"])
 
 
+FWI_FULL_CASES = [  # order, fs, snapinc (FwiAcoustic2D::run)
+    (4, 0, 1), (8, 1, 3), (4, 0, 7),
+]
+
+
+def fwi_full_fixtures(R):
+    from oracle import fwi
+    nx, nz, lpml, dx, dt, nt = 60, 50, 10, 10.0, 1e-3, 300
+    vtrue = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1800.0, 3000.0, nanom=5, seed=3)
+    rtrue = synth.gardner(vtrue)
+    vp0 = synth.gaussian_smooth(vtrue, 60.0, dx, dx)
+    rho0 = synth.gardner(vp0)
+    sc = np.float32([[300.0, 10.0]])
+    wav = synth.ricker(nt, dt, 15.0)[None]
+    gc = synth.surface_line(nx, dx, 20.0)
+    for k, (order, fs, snapinc) in enumerate(FWI_FULL_CASES):
+        obs = runs.ref_a2d_run(vtrue, rtrue, lpml, dx, dx, dt, nt, order, fs, "P", sc, wav, "P", gc)["rec"]
+        r = fwi.ref_fwi_a2d_full(vp0, rho0, lpml, dx, dx, dt, nt, order, fs, snapinc, sc, wav, gc, obs)
+        save(f"fwi_full_case{k}", vp=vp0, rho=rho0, scoords=sc, wav=wav, gcoords=gc, dobs=obs,
+             grid=np.float32([nx, nz, lpml, dx, dt, nt]), misfit=np.float32(r["misfit"]),
+             vpgrad=r["vpgrad"], rhograd=r["rhograd"], dmod=r["dmod"], dres=r["dres"],
+             wavgrad=r["wavgrad"])
+
+
+FWI_E2D_CASES = [  # order, fs, ncheck, source field (FwiElastic2D::run_optimal, P/Ux/Uz data)
+    (4, 0, 15, "P"), (8, 1, 6, "VZ"),
+]
+
+
+def fwi_e2d_fixtures(R):
+    """The reference's scaled Vp/Vs/rho gradients, data, residuals, wavelet
+    gradient and misfit; plus the oracle's lambda/mu/rho kernels before
+    scaleGrad (oracle_*), whose scaleGrad reproduces the reference bit for
+    bit (checked here before saving)."""
+    from oracle import fwi
+    nx, nz, lpml, dx, dt, nt = 60, 50, 10, 10.0, 1e-3, 250
+    vtrue = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1800.0, 3000.0, nanom=5, seed=3)
+    vstrue = (vtrue / 1.8).astype(np.float32)
+    rtrue = synth.gardner(vtrue)
+    vp0 = synth.gaussian_smooth(vtrue, 60.0, dx, dx)
+    vs0 = (vp0 / 1.8).astype(np.float32)
+    rho0 = synth.gardner(vp0)
+    sc = np.float32([[300.0, 20.0]])
+    wav = synth.ricker(nt, dt, 15.0)[None]
+    gc = synth.surface_line(nx, dx, 20.0)
+    for k, (order, fs, ncheck, sf) in enumerate(FWI_E2D_CASES):
+        obs = np.stack([runs.ref_e2d_run(vtrue, vstrue, rtrue, lpml, dx, dx, dt, nt, order, fs, sf, sc, wav, f,
+                                         gc)["rec"] for f in ("P", "VX", "VZ")])
+        r = fwi.ref_fwi_e2d(vp0, vs0, rho0, lpml, dx, dx, dt, nt, order, fs, ncheck, sf, sc, wav, gc, obs)
+        o = fwi.fwi_e2d(vp0, vs0, rho0, lpml, dx, dx, dt, nt, order, fs, ncheck, sf, sc, wav, gc, obs)
+        for key in ("vpgrad", "vsgrad", "rhograd", "dmod", "dres", "wavgrad"):
+            assert np.array_equal(r[key], o[key]), key
+        save(f"fwi_e2d_case{k}", vp=vp0, vs=vs0, rho=rho0, scoords=sc, wav=wav, gcoords=gc, dobs=obs,
+             grid=np.float32([nx, nz, lpml, dx, dt, nt]), misfit=np.float32(r["misfit"]),
+             vpgrad=r["vpgrad"], vsgrad=r["vsgrad"], rhograd=r["rhograd"], dmod=r["dmod"], dres=r["dres"],
+             wavgrad=r["wavgrad"], oracle_lam=o["lam"], oracle_mu=o["mu"], oracle_rho=o["rhok"])
+
+
 REVOLVE_CASES = [(10, 3), (100, 5), (7, 2), (50, 60), (1, 2), (257, 4), (3000, 50), (2500, 50), (4000, 50)]
 
 
@@ -184,12 +242,12 @@ def revolve_fixture(R):
     save("revolve", **out)
 
 
+GROUPS = dict(der=der_fixture, pml=pml_fixture, stagger=stagger_fixture, makemap=makemap_fixture,
+              runs=run_fixtures, revolve=revolve_fixture, fwi=fwi_fixtures, fwi_full=fwi_full_fixtures,
+              fwi_e2d=fwi_e2d_fixtures)
+
 if __name__ == "__main__":
+    # python -m tests.golden.make_golden [group ...]   (default: every group)
     R = oracle.ref()
-    der_fixture(R)
-    pml_fixture(R)
-    stagger_fixture(R)
-    makemap_fixture(R)
-    run_fixtures(R)
-    revolve_fixture(R)
-    fwi_fixtures(R)
+    for name in (sys.argv[1:] or list(GROUPS)):
+        GROUPS[name](R)

@@ -136,3 +136,15 @@ def test_revolve_schedule_matches_reference(golden):
         got = np.stack([a[:n.value], c[:n.value], k[:n.value]], 1)
         assert np.array_equal(got, ref), (nt, sn)
         assert L.rsg_revolve_numforw(nt, sn) == int(g[f"n{nt}_s{sn}_numforw"])
+
+
+def test_snap_cadence_matches_oracle():
+    from oracle.fwi import snap_cadence
+    for nt in (1, 300, 2999, 3000, 6000):
+        for dt in (1e-3, 4.5e-4):
+            for snapinc in (1, 3, 7, 50):
+                a, b = C.c_int(), C.c_int()
+                R.check(R.lib().rsg_snap_cadence(nt, dt, snapinc, C.byref(a), C.byref(b)))
+                assert (a.value, b.value) == snap_cadence(nt, dt, snapinc)
+    a, b = C.c_int(), C.c_int()
+    assert R.lib().rsg_snap_cadence(10, 1e-3, 0, C.byref(a), C.byref(b)) == -1  # RSG_EINVAL

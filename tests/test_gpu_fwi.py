@@ -1,20 +1,27 @@
 """FWI gradient on the HIP path (rsg_fwi_a2d_gradient through
 rockseis_amd.fwi) against the compiled reference's FwiAcoustic2D::run_optimal
-(golden fixtures): gradients, modelled data, residuals, wavelet gradient and
-misfit within relative L2 1e-5; Revolve and store-every-step modes agree
-bit for bit; the torch.autograd.Function returns the same gradient."""
+and FwiAcoustic2D::run (golden fixtures): gradients, modelled data,
+residuals, wavelet gradient and misfit within relative L2 1e-5; Revolve and
+store-every-step modes agree bit for bit; the torch.autograd.Function
+returns the same gradient; FP16 snapshot / checkpoint storage stays within
+relative L2 1e-4 (snapshots) of the FP64 oracle's gradient."""
 import numpy as np
 import pytest
 
-from tests.golden.make_golden import FWI_CASES
+from tests.golden.make_golden import FWI_CASES, FWI_FULL_CASES
 from tests.gpu_util import RTOL_L2, data2d, rel_l2
 
 pytestmark = pytest.mark.gpu
 
 
-def _run(g, case, mode):
+def _run(g, case, mode, fp16=False, snapinc=None):
     from rockseis_amd import core, fwi
-    order, fs, ncheck = FWI_CASES[case]
+    if mode == "full":
+        order, fs, si = FWI_FULL_CASES[case]
+        snapinc, ncheck = snapinc or si, 0
+    else:
+        order, fs, ncheck = FWI_CASES[case]
+        snapinc = snapinc or 1
     nx, nz, lpml, dx, dt, nt = g["grid"]
     nx, nz, lpml, nt = int(nx), int(nz), int(lpml), int(nt)
     m = core.ModelAcoustic2D(nx, nz, lpml, dx, dx, 0.0, 0.0, fs)
@@ -27,7 +34,8 @@ def _run(g, case, mode):
     res = data2d(g["gcoords"], nt, dt, "P")
     wgr = data2d(g["scoords"], nt, dt, "P")
     m.staggerModels()
-    f = fwi.FwiAcoustic2D(m, src, obs, order, 1)
+    f = fwi.FwiAcoustic2D(m, src, obs, order, snapinc)
+    f.setFp16Storage(fp16)
     vg, rg = fwi.Image2D(m), fwi.Image2D(m)
     f.setVpgrad(vg)
     f.setRhograd(rg)
@@ -35,7 +43,8 @@ def _run(g, case, mode):
     f.setDataresP(res)
     f.setWavgrad(wgr)
     f.setNcheck(ncheck)
-    (f.run_optimal if mode == "revolve" else f.run)()
+    {"revolve": f.run_optimal, "store": f.run_stored, "full": f.run}[mode]()
+    assert f.mode_used == mode
     return dict(misfit=f.getMisfit(), misfit64=f.misfit64, vpgrad=vg.imagedata, rhograd=rg.imagedata, dmod=mod.data,
                 dres=res.data, wavgrad=wgr.data)
 
@@ -59,6 +68,70 @@ def test_fwi_a2d_vs_reference(golden, case):
     s = _run(g, case, "store")
     for k in ("vpgrad", "rhograd", "dmod", "dres", "wavgrad"):
         assert np.array_equal(s[k], r[k]), k
+
+
+@pytest.mark.parametrize("case", range(len(FWI_FULL_CASES)))
+def test_fwi_a2d_full_vs_reference(golden, case):
+    """FwiAcoustic2D::run: snapshots every snapinc steps held in HBM."""
+    g = golden(f"fwi_full_case{case}")
+    print(f"fwi full case {case}: {FWI_FULL_CASES[case]}", flush=True)
+    r = _run(g, case, "full")
+    for k in ("vpgrad", "rhograd", "dmod", "dres", "wavgrad"):
+        e = rel_l2(r[k], g[k])
+        print(f"  {k}: rel L2 {e:.2e}")
+        assert e <= RTOL_L2, k
+    exact = 0.5 * np.sum((g["dmod"].astype(np.float64) - g["dobs"].astype(np.float64)) ** 2)
+    assert abs(r["misfit64"] - exact) <= RTOL_L2 * exact
+
+
+def test_fp16_storage_error_vs_fp64_oracle(golden):
+    """Gradient error against the FP64 oracle for FP32 and FP16 storage of
+    Revolve checkpoints, the store-every-step history and run()'s
+    snapshots.  FP16 snapshots only perturb the correlated forward pressure
+    (bound 1e-4); FP16 checkpoints perturb the recomputed forward wavefield
+    itself (bound 1e-3, the measured value is printed)."""
+    from oracle import fwi as ofwi
+    g = golden("fwi_case0")
+    order, fs, ncheck = FWI_CASES[0]
+    nx, nz, lpml, dx, dt, nt = g["grid"]
+    args = (g["vp"], g["rho"], int(lpml), dx, dx, dt, int(nt), order, fs)
+    tail = (g["scoords"], g["wav"], g["gcoords"], g["dobs"])
+    o_opt = ofwi.fwi_a2d(*args, ncheck, *tail, real=np.float64)
+    o_full = ofwi.fwi_a2d_full(*args, 3, *tail, real=np.float64)
+    bounds = {("revolve", False): 1e-5, ("store", False): 1e-5, ("full", False): 1e-5,
+              ("revolve", True): 1e-3, ("store", True): 1e-4, ("full", True): 1e-4}
+    for (mode, fp16), bound in bounds.items():
+        if mode == "full":
+            r = _run_full_on_case0(g, fp16)
+            ref = o_full
+        else:
+            r = _run(g, 0, mode, fp16=fp16)
+            ref = o_opt
+        ev = rel_l2(r["vpgrad"], ref["vpgrad"])
+        er = rel_l2(r["rhograd"], ref["rhograd"])
+        print(f"  {mode:8s} fp16={fp16!s:5s} vpgrad {ev:.2e} rhograd {er:.2e} (bound {bound:.0e})", flush=True)
+        assert ev <= bound and er <= bound, (mode, fp16)
+
+
+def _run_full_on_case0(g, fp16):
+    """run() with snapinc 3 on the fwi_case0 inputs (order 4, no free surface)."""
+    from rockseis_amd import core, fwi
+    nx, nz, lpml, dx, dt, nt = g["grid"]
+    nx, nz, lpml, nt = int(nx), int(nz), int(lpml), int(nt)
+    m = core.ModelAcoustic2D(nx, nz, lpml, dx, dx, 0.0, 0.0, False)
+    m.Vp[:], m.R[:] = g["vp"], g["rho"]
+    src = data2d(g["scoords"], nt, dt, "P", g["wav"])
+    src.makeMap(m, core.SMAP)
+    obs = data2d(g["gcoords"], nt, dt, "P", g["dobs"])
+    obs.makeMap(m, core.GMAP)
+    m.staggerModels()
+    f = fwi.FwiAcoustic2D(m, src, obs, FWI_CASES[0][0], 3)
+    f.setFp16Storage(fp16)
+    vg, rg = fwi.Image2D(m), fwi.Image2D(m)
+    f.setVpgrad(vg)
+    f.setRhograd(rg)
+    f.run()
+    return dict(vpgrad=vg.imagedata, rhograd=rg.imagedata)
 
 
 def test_autograd_function_matches_driver(golden):

@@ -118,3 +118,40 @@ def test_fwi_a2d_gradient(golden, case):
     for k in ("vpgrad", "rhograd", "dmod", "dres", "wavgrad"):
         assert np.array_equal(r[k], g[k]), k
     assert np.float32(r["misfit"]) == g["misfit"]
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_fwi_a2d_full_gradient(golden, case):
+    """FwiAcoustic2D::run with snapshots every snapinc steps, restated
+    (oracle/fwi.py), vs the compiled reference: bit-exact, including the
+    enddiff offset of the reverse-time snapshot reads (snapinc 3 and 7 do
+    not divide nt-1 = 299)."""
+    from oracle import fwi
+    from tests.golden.make_golden import FWI_FULL_CASES
+    g = golden(f"fwi_full_case{case}")
+    order, fs, snapinc = FWI_FULL_CASES[case]
+    nx, nz, lpml, dx, dt, nt = g["grid"]
+    r = fwi.fwi_a2d_full(g["vp"], g["rho"], int(lpml), dx, dx, dt, int(nt), order, fs, snapinc,
+                         g["scoords"], g["wav"], g["gcoords"], g["dobs"])
+    for k in ("vpgrad", "rhograd", "dmod", "dres", "wavgrad"):
+        assert np.array_equal(r[k], g[k]), k
+    assert np.float32(r["misfit"]) == g["misfit"]
+
+
+@pytest.mark.parametrize("case", range(2))
+def test_fwi_e2d_gradient(golden, case):
+    """FwiElastic2D::run_optimal restated (oracle/fwi.py: forward and
+    adjoint-state elastic steps, 3-component misfit/residuals, crossCorr,
+    scaleGrad) vs the compiled reference, bit for bit."""
+    from oracle import fwi
+    from tests.golden.make_golden import FWI_E2D_CASES
+    g = golden(f"fwi_e2d_case{case}")
+    order, fs, ncheck, sf = FWI_E2D_CASES[case]
+    nx, nz, lpml, dx, dt, nt = g["grid"]
+    r = fwi.fwi_e2d(g["vp"], g["vs"], g["rho"], int(lpml), dx, dx, dt, int(nt), order, fs, ncheck, sf,
+                    g["scoords"], g["wav"], g["gcoords"], g["dobs"])
+    for k in ("vpgrad", "vsgrad", "rhograd", "dmod", "dres", "wavgrad"):
+        assert np.array_equal(r[k], g[k]), k
+    assert np.float32(r["misfit"]) == g["misfit"]
+    for k, o in (("lam", "oracle_lam"), ("mu", "oracle_mu"), ("rhok", "oracle_rho")):
+        assert np.array_equal(r[k], g[o]), k

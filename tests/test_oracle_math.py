@@ -136,3 +136,14 @@ def test_fp32_vs_fp64_run_and_linearity():
     assert rel < 1e-5
     r2 = runs.a2d_run(vp, rho, lpml, dx, dx, dt, nt, 4, 0, "P", sc, 2 * w, "P", gc, real=np.float64)
     assert np.allclose(r2["rec"], 2 * r64["rec"], rtol=1e-12, atol=1e-12 * np.abs(r64["rec"]).max())
+
+
+def test_snap_cadence_closed_form():
+    """Snapshot2D's float arithmetic for (snapnt, enddiff) equals the integer
+    closed form ((nt-1)//snapinc + 1, (nt-1) % snapinc) for the step sizes
+    and lengths of the modelling and FWI workloads."""
+    from oracle.fwi import snap_cadence
+    for nt in (1, 2, 3, 7, 100, 299, 300, 2000, 2500, 3000, 4000, 6000):
+        for dt in (1e-3, 5e-4, 4.5e-4, 2e-3):
+            for snapinc in (1, 2, 3, 5, 7, 10, 13, 50):
+                assert snap_cadence(nt, dt, snapinc) == ((nt - 1) // snapinc + 1, (nt - 1) % snapinc)
