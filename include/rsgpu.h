@@ -172,6 +172,21 @@ int rsg_e2d_velocity_batch(const rsg_grid *g, const rsg_state *s, const rsg_batc
 int rsg_e2d_stress_batch(const rsg_grid *g, const rsg_state *s, const rsg_batch *b, int adjoint,
                          void *stream);
 
+/* Ragged batch: nshot shots, each on its own grid (an aperture window of a
+ * larger model: its own nx, nz, model arrays, fields and strips), one launch
+ * per half step for all of them.  grids[k] / states[k] describe shot k as the
+ * single-shot calls take them; every shot uses one stencil order.  The plan
+ * copies the pointers: recreate it when they change.  No counterpart in the
+ * reference, which runs one window per MPI worker (inversion.cpp:332-600);
+ * results per shot equal the single-shot calls. */
+typedef struct rsg_ragged rsg_ragged;
+int rsg_ragged2d_create(int kind, int nshot, const rsg_grid *grids, const rsg_state *states,
+                        rsg_ragged **plan);
+/* which: 0 forward velocity, 1 forward stress, 2 adjoint-state velocity,
+ * 3 adjoint-state stress (the last two elastic only) */
+int rsg_ragged2d_step(const rsg_ragged *plan, int which, void *stream);
+int rsg_ragged2d_destroy(rsg_ragged *plan);
+
 /* insertSource (waves.cpp:407-497, 998-1137, 2085-2171): kind is
  * 1 acoustic 2D, 2 elastic 2D, 3 acoustic 3D; the field and its modulus are
  * chosen from src->field as the reference does. */
@@ -223,6 +238,40 @@ int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, void *stream);
  * src/obs (map, shift, data), dmod/dres/wavgrad are [nshot][ntrace][nt],
  * vpgrad/rhograd and the misfit are summed over the shots. */
 int rsg_fwi_a2d_gradient_batch(const rsg_grid *g, const rsg_fwi_a2d *f, int nshot, void *stream);
+/* ---- Elastic 2D FWI gradient (FwiElastic2D::run_optimal, fwi.cpp:3937-4143)
+ * for a ragged batch of shots, each on its own aperture window.  Per shot:
+ * forward elastic steps with the source and P / Ux / Uz gathers, DIFFERENCE
+ * misfit and residuals of the three records, adjoint-state steps driven by
+ * the residuals, crossCorr (fwi.cpp:2591-2775) at every reverse step with
+ * Revolve checkpoints of all shots in HBM.  The windows' kernels are mapped
+ * by scaleGrad (fwi.cpp:2778-2849) when scale = 1, then added into the
+ * stacked images at x offset ox (the reference stacks shot images in its
+ * inversion driver, image.cpp:533-600). */
+typedef struct {
+    rsg_grid g;                          /* window grid, padded (nx, nz incl. 2*lpml)  */
+    const float *L, *L2M, *M, *Rx, *Rz;  /* staggered window model (device, padded)    */
+    const float *Vp, *Vs, *Rho;          /* unpadded window model nz*nx (device)       */
+    int ox;                              /* first column of the window in the images   */
+    rsg_traces src;                      /* source: PRESSURE (explosive), VX or VZ     */
+    rsg_traces obs[3];                   /* observed P, Ux (VX), Uz (VZ) records of one
+                                            receiver set (maps and shifts per field)  */
+    float *dmod[3], *dres[3];            /* out: modelled records and residuals        */
+    float *wavgrad;                      /* out (optional): wavelet gradient           */
+} rsg_fwi_e2d_shot;
+
+typedef struct {
+    const float *tab;  /* PML tables of the shared lpml and dt (device)              */
+    int ncheck;        /* Revolve snapshots                                            */
+    int fp16;          /* checkpoints as FP16 with one power-of-two scale per array   */
+    int scale;         /* 1: Vp / Vs / rho gradients; 0: lambda / mu / rho kernels     */
+    int nx, nz;        /* stacked image size                                           */
+    float *grad[3];    /* in/out (device): gradients added here, nz*nx each            */
+    double *misfit;    /* out (host): misfit summed over the shots                     */
+} rsg_fwi_e2d;
+
+int rsg_fwi_e2d_gradient_batch(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *shots, int nshot,
+                               void *stream);
+
 /* Revolve::revolve() action sequence (revolve.cpp:150-305): advance=0,
  * takeshot=1, restore=2, firsturn=3, youturn=4, terminate=5, error=6. */
 int rsg_revolve_schedule(int nt, int snaps, int *actions, int *capos, int *checks, int maxn,

@@ -54,6 +54,17 @@ class Fwi(C.Structure):
                    ("mode_used", C.POINTER(C.c_int)), ("snapinc", C.c_int), ("fp16", C.c_int)])
 
 
+class FwiE2DShot(C.Structure):
+    _fields_ = ([("g", Grid)] + [(n, C.c_void_p) for n in ("L", "L2M", "M", "Rx", "Rz", "Vp", "Vs", "Rho")]
+                + [("ox", C.c_int), ("src", Traces), ("obs", Traces * 3), ("dmod", C.c_void_p * 3),
+                   ("dres", C.c_void_p * 3), ("wavgrad", C.c_void_p)])
+
+
+class FwiE2D(C.Structure):
+    _fields_ = [("tab", C.c_void_p), ("ncheck", C.c_int), ("fp16", C.c_int), ("scale", C.c_int),
+                ("nx", C.c_int), ("nz", C.c_int), ("grad", C.c_void_p * 3), ("misfit", C.POINTER(C.c_double))]
+
+
 FWI_REVOLVE, FWI_STORE, FWI_FULL = 0, 1, 2
 
 
@@ -111,6 +122,10 @@ def _declare(L):
         "rsg_revolve_numforw": ([ip, ip], ip),
         "rsg_snap_cadence": ([ip, C.c_float, ip, C.POINTER(C.c_int), C.POINTER(C.c_int)], ip),
         "rsg_get_tuning": ([C.c_char_p, C.POINTER(C.c_int)], ip),
+        "rsg_ragged2d_create": ([ip, ip, GP, SP, C.POINTER(C.c_void_p)], ip),
+        "rsg_ragged2d_step": ([vp, ip, vp], ip),
+        "rsg_ragged2d_destroy": ([vp], ip),
+        "rsg_fwi_e2d_gradient_batch": ([C.POINTER(FwiE2D), C.POINTER(FwiE2DShot), ip, vp], ip),
     }
     for name in ("a2d_forward_velocity", "a2d_forward_stress", "e2d_forward_velocity",
                  "e2d_forward_stress", "e2d_backward_velocity", "e2d_backward_stress",
@@ -134,6 +149,7 @@ EXPORTS = (
     "rsg_set_tuning", "rsg_get_tuning", "rsg_fwi_a2d_gradient", "rsg_revolve_schedule",
     "rsg_revolve_numforw", "rsg_snap_cadence", "rsg_fwi_a2d_gradient_batch", "rsg_a2d_forward_velocity_batch",
     "rsg_a2d_forward_stress_batch", "rsg_e2d_velocity_batch", "rsg_e2d_stress_batch",
+    "rsg_ragged2d_create", "rsg_ragged2d_step", "rsg_ragged2d_destroy", "rsg_fwi_e2d_gradient_batch",
 )
 
 

@@ -489,6 +489,41 @@ class WavesAcoustic3D(_Waves):
         return self.nz_pml
 
 
+class RaggedBatch2D:
+    """Shots on 2D grids of different sizes (aperture windows) advanced by
+    one launch per half step (rsg_ragged2d_*).  `waves` and `models` are the
+    per-shot WavesAcoustic2D / WavesElastic2D objects and their models, all
+    of one kind and one stencil order; step(which) with which = 0 forward
+    velocity, 1 forward stress, 2 / 3 the elastic adjoint-state steps."""
+
+    def __init__(self, waves, models, order):
+        kinds = {w.kind for w in waves}
+        if len(kinds) != 1:
+            raise R.RsgError("RaggedBatch2D: every shot must be of one propagator kind")
+        n = len(waves)
+        der = Der(1, 1, 1, 1, 1, 1, order)
+        self._grids = (R.Grid * n)(*[w._grid(der) for w in waves])
+        self._states = (R.State * n)(*[w._state(m) for w, m in zip(waves, models)])
+        self._keep = (list(waves), list(models))
+        self._plan = C.c_void_p()
+        R.check(R.lib().rsg_ragged2d_create(kinds.pop(), n, self._grids, self._states, C.byref(self._plan)),
+                "rsg_ragged2d_create")
+
+    def step(self, which):
+        R.check(R.lib().rsg_ragged2d_step(self._plan, int(which), _stream()), "rsg_ragged2d_step")
+
+    def close(self):
+        if self._plan:
+            R.lib().rsg_ragged2d_destroy(self._plan)
+            self._plan = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # --------------------------------------------------------------------------
 # Modelling drivers
 # --------------------------------------------------------------------------

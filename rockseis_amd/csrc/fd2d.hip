@@ -1,603 +1,7 @@
-// 2D staggered-grid velocity-stress kernels for gfx950: acoustic
-// (WavesAcoustic2D, waves.cpp:217-404) and elastic forward and adjoint-state
-// steps (WavesElastic2D, waves.cpp:1416-2080).
-//
-// One fused kernel per half step.  The reference makes, per derivative, a
-// full-grid Der pass into a scratch array, a full-grid update pass and a PML
-// strip pass (der.cpp, waves.cpp); here each block stages the stencil inputs
-// of a 64 x TZ tile (plus halo) in LDS once, evaluates every derivative of
-// the half step from LDS, and applies the field update and the CFS-PML
-// memory update of every cell in registers, so each array crosses HBM once
-// per half step.  The per-cell operation sequence is the reference's: += dt*M*d
-// for the first derivative, its PML correction, then the next derivative.
-//
-// Block: 64 x 4 threads (four wave64 rows); each thread owns RZ consecutive
-// cells of one x column.  The thread's own field and model values are loaded
-// into registers before the tile is staged so both sets of loads are in
-// flight together; x-derivative taps are conflict-free LDS row reads.  Cells
-// whose stencil reaches the array edge use the reference's reduced-order
-// coefficient sets.  Tiles are numbered so that consecutive tiles, which
-// share halos, run on the same XCD (blocks are dealt round-robin to the 8
-// XCDs, each with its own L2).
-#include "rsg_common.h"
-
-namespace rsg {
-
-constexpr int BX = 64;
-constexpr int BZ = 4;
-
-struct K2D {
-    int nx, nz, lpml, fs;
-    int ntx, ntiles;  // tiles along x, total tiles
-    long sf, sxs, szs;  // per-shot strides of fields, x-side and z-side strips (blockIdx.y = shot)
-    float dt, invdx, invdz;
-    const float *tab;
-    float *P, *Vx, *Vz, *Sxx, *Szz, *Sxz;
-    const float *L, *L2M, *M, *Rx, *Rz;
-    float *psi[RSG_MAX_STRIPS];
-    Coefs cx, cz;
-};
-
-// Tile handled by this block: XCD-contiguous numbering of a 1D grid.
-__device__ __forceinline__ bool tile_of(const K2D &a, int TZ, int &x0, int &z0) {
-    const int nb = gridDim.x;
-    const int per = (nb + 7) / 8;
-    const int t = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (t >= a.ntiles) return false;
-    x0 = (t % a.ntx) * BX;
-    z0 = (t / a.ntx) * TZ;
-    return true;
-}
-
-// A tile is "interior" when it holds no PML cell and no free-surface row and
-// its whole halo lies inside the grid: such tiles (most of the grid) take a
-// straight-line path with no bounds, edge-stencil or strip tests.
-__device__ __forceinline__ bool interior_tile(const K2D &a, int x0, int z0, int TZ, int H) {
-    const int lo = a.lpml > H ? a.lpml : H;
-    return x0 >= lo && x0 + BX + H <= a.nx - a.lpml && z0 >= lo && z0 + TZ + H <= a.nz - a.lpml &&
-           !(a.fs && z0 <= a.lpml);
-}
-
-// Staging of a (TZ+2HZ) x (64+2HX) window into LDS, in two phases so that
-// every global load of a block is in flight before the first LDS write: fetch
-// issues all loads into registers (thread row ty takes window rows ty, ty+4,
-// ...; columns tx and, for halo columns, tx+64), store writes them out.
-// Boundary tiles clamp the address into the grid and select 0 outside it, so
-// the loads stay unconditional; zeros outside the grid are never read by the
-// reduced edge stencils.
-template <int TZ, int HX, int HZ>
-struct Tile {
-    static constexpr int LD = BX + 2 * HX, ROWS = TZ + 2 * HZ, KR = (ROWS + BZ - 1) / BZ;
-    float a[KR], b[KR];
-};
-
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-template <int TZ, int HX, int HZ, bool FAST, class F>
-__device__ __forceinline__ void tile_fetch(Tile<TZ, HX, HZ> &t, F &&ld, int nx, int nz, int x0, int z0) {
-    using T = Tile<TZ, HX, HZ>;
-    const int tx = threadIdx.x, ty = threadIdx.y;
-#pragma unroll
-    for (int k = 0; k < T::KR; k++) {
-        const int r = ty + k * BZ;
-        const int z = z0 - HZ + r;
-        const int x = x0 - HX + tx;
-        if (FAST) {
-            t.a[k] = ld((unsigned)(z * nx + x));
-            if (2 * HX > 0) t.b[k] = tx < 2 * HX ? ld((unsigned)(z * nx + x + BX)) : 0.f;
-        } else {
-            const int zc = clampi(z, 0, nz - 1);
-            const bool zok = z >= 0 && z < nz;
-            const float va = ld((unsigned)(zc * nx + clampi(x, 0, nx - 1)));
-            t.a[k] = (zok && x >= 0 && x < nx) ? va : 0.f;
-            if (2 * HX > 0) {
-                const float vb = ld((unsigned)(zc * nx + clampi(x + BX, 0, nx - 1)));
-                t.b[k] = (zok && x + BX < nx) ? vb : 0.f;
-            }
-        }
-    }
-}
-
-template <int TZ, int HX, int HZ>
-__device__ __forceinline__ void tile_store(const Tile<TZ, HX, HZ> &t, float *lds) {
-    using T = Tile<TZ, HX, HZ>;
-    const int tx = threadIdx.x, ty = threadIdx.y;
-#pragma unroll
-    for (int k = 0; k < T::KR; k++) {
-        const int r = ty + k * BZ;
-        if (T::ROWS % BZ == 0 || r < T::ROWS) {
-            lds[r * T::LD + tx] = t.a[k];
-            if (2 * HX > 0 && tx < 2 * HX) lds[r * T::LD + tx + BX] = t.b[k];
-        }
-    }
-}
-
-// Loaders: a plain array, or the products a*b + c*d / a*b that
-// WavesElastic2D::backwardstepVelocity differentiates (waves.cpp:1771-1775).
-struct Ld1 {
-    const float *__restrict__ g;
-    __device__ __forceinline__ float operator()(unsigned i) const { return g[i]; }
-};
-struct Ld2 {
-    const float *__restrict__ a;
-    const float *__restrict__ b;
-    __device__ __forceinline__ float operator()(unsigned i) const { return a[i] * b[i]; }
-};
-struct Ld4 {
-    const float *__restrict__ a;
-    const float *__restrict__ b;
-    const float *__restrict__ c;
-    const float *__restrict__ d;
-    __device__ __forceinline__ float operator()(unsigned i) const { return a[i] * b[i] + c[i] * d[i]; }
-};
-
-// Derivative of an LDS tile at a cell: compile-time order in the interior,
-// reduced order where the reference's Der shortens the stencil.
-template <int ORDER, bool FAST>
-__device__ __forceinline__ float ddfw(const float *s, int st, const float *c, int k, float invd) {
-    float d = dfw<ORDER>(s, st, c + cind(ORDER));
-    if (!FAST && k != ORDER) d = dfw_k(s, st, k, invd);
-    return d;
-}
-template <int ORDER, bool FAST>
-__device__ __forceinline__ float ddbw(const float *s, int st, const float *c, int k, float invd) {
-    float d = dbw<ORDER>(s, st, c + cind(ORDER));
-    if (!FAST && k != ORDER) d = dbw_k(s, st, k, invd);
-    return d;
-}
-
-// PML table accessors: stag selects the staggered set, right the rbb side.
-struct Tab {
-    const float *t;
-    int L;
-    __device__ __forceinline__ float A(int right, int stag, int i) const {
-        return t[(right * T_RBB + stag * T_STAG + T_A) * L + i];
-    }
-    __device__ __forceinline__ float B(int right, int stag, int i) const {
-        return t[(right * T_RBB + stag * T_STAG + T_B) * L + i];
-    }
-    __device__ __forceinline__ float C(int right, int stag, int i) const {
-        return t[(right * T_RBB + stag * T_STAG + T_C) * L + i];
-    }
-};
-
-// Position of a cell relative to the strips of one axis.
-struct PmlPos {
-    int side;  // -1 none, 0 low (left/top), 1 high (right/bottom)
-    int i;     // index into the strip profile (table index)
-};
-__device__ __forceinline__ PmlPos pml_pos(int x, int n, int L) {
-    PmlPos p{-1, 0};
-    if (x < L) {
-        p.side = 0;
-        p.i = x;
-    } else if (x >= n - L) {
-        p.side = 1;
-        p.i = x - (n - L);
-    }
-    return p;
-}
-
-// Apply dt*M*d then the strip correction of one derivative term.
-template <bool FAST>
-__device__ __forceinline__ void term(float &F, float dtM, float d, const PmlPos &pp, const Tab &tb,
-                                     int stag, float *lo, float *hi, size_t sidx) {
-    F += dtM * d;
-    if (!FAST && pp.side >= 0) {
-        float *psi = (pp.side ? hi : lo) + sidx;
-        pml_corr(F, dtM, psi, tb.A(pp.side, stag, pp.i), tb.B(pp.side, stag, pp.i),
-                 tb.C(pp.side, stag, pp.i), d);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Acoustic 2D
-// ---------------------------------------------------------------------------
-
-// Kernel bodies take the LDS buffers from the kernel so the interior (FAST)
-// and boundary instantiations share one allocation.
-template <int ORDER, int RZ, bool FAST>
-__device__ __forceinline__ void a2d_velocity_body(const K2D &a, int x0, int z0, float *sP) {
-    constexpr int H = ORDER, TZ = BZ * RZ, LD = BX + 2 * H;
-    const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    const bool xin = FAST || x < nx;
-    const size_t fo = blockIdx.y * (size_t)a.sf, xo = blockIdx.y * (size_t)a.sxs,
-                 zo = blockIdx.y * (size_t)a.szs;
-    float *__restrict__ Vx = a.Vx + fo;
-    float *__restrict__ Vz = a.Vz + fo;
-    const float *__restrict__ Rx = a.Rx;
-    const float *__restrict__ Rz = a.Rz;
-    float vx[RZ], vz[RZ], rx[RZ], rz[RZ];
-#pragma unroll
-    for (int r = 0; r < RZ; r++) {
-        const int z = z0 + ty * RZ + r;
-        const size_t p = (size_t)z * nx + x;
-        const bool in = FAST || (xin && z < nz);
-        vx[r] = in ? Vx[p] : 0.f;
-        vz[r] = in ? Vz[p] : 0.f;
-        rx[r] = in ? Rx[p] : 0.f;
-        rz[r] = in ? Rz[p] : 0.f;
-    }
-    Tile<TZ, H, H> tp;
-    tile_fetch<TZ, H, H, FAST>(tp, Ld1{a.P + fo}, nx, nz, x0, z0);
-    tile_store(tp, sP);
-    __syncthreads();
-    if (!xin) return;
-    const Tab tb{a.tab, a.lpml};
-    const PmlPos px = FAST ? PmlPos{-1, 0} : pml_pos(x, nx, a.lpml);
-    const int kx = FAST ? ORDER : kfw(x, nx, ORDER);
-#pragma unroll
-    for (int r = 0; r < RZ; r++) {
-        const int zl = ty * RZ + r, z = z0 + zl;
-        if (!FAST && z >= nz) break;
-        const float *s = sP + (zl + H) * LD + tx + H;
-        const float dPx = ddfw<ORDER, FAST>(s, 1, a.cx.c, kx, a.invdx);
-        const float dPz = ddfw<ORDER, FAST>(s, LD, a.cz.c, FAST ? ORDER : kfw(z, nz, ORDER), a.invdz);
-        const size_t p = (size_t)z * nx + x;
-        const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
-        term<FAST>(vx[r], a.dt * rx[r], dPx, px, tb, 1, (a.psi[RSG_A2D_P_LEFT] + xo),
-                   (a.psi[RSG_A2D_P_RIGHT] + xo), (size_t)z * a.lpml + px.i);
-        term<FAST>(vz[r], a.dt * rz[r], dPz, pz, tb, 1, (a.psi[RSG_A2D_P_TOP] + zo),
-                   (a.psi[RSG_A2D_P_BOTTOM] + zo), (size_t)pz.i * nx + x);
-        Vx[p] = vx[r];
-        Vz[p] = vz[r];
-    }
-}
-
-template <int ORDER, int RZ>
-__global__ __launch_bounds__(BX *BZ) void a2d_velocity_kernel(const K2D a) {
-    constexpr int H = ORDER, TZ = BZ * RZ, LD = BX + 2 * H;
-    __shared__ float sP[(TZ + 2 * H) * LD];
-    int x0, z0;
-    if (!tile_of(a, TZ, x0, z0)) return;
-    if (interior_tile(a, x0, z0, TZ, H))
-        a2d_velocity_body<ORDER, RZ, true>(a, x0, z0, sP);
-    else
-        a2d_velocity_body<ORDER, RZ, false>(a, x0, z0, sP);
-}
-
-template <int ORDER, int RZ, bool FAST>
-__device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, float *sVx, float *sVz) {
-    constexpr int H = ORDER, LDX = BX + 2 * H, LDZ = BX, TZ = BZ * RZ;
-    const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    const bool xin = FAST || x < nx;
-    const size_t fo = blockIdx.y * (size_t)a.sf, xo = blockIdx.y * (size_t)a.sxs,
-                 zo = blockIdx.y * (size_t)a.szs;
-    float *__restrict__ P = a.P + fo;
-    const float *__restrict__ L = a.L;
-    float pv[RZ], lv[RZ];
-#pragma unroll
-    for (int r = 0; r < RZ; r++) {
-        const int z = z0 + ty * RZ + r;
-        const size_t p = (size_t)z * nx + x;
-        const bool in = FAST || (xin && z < nz);
-        pv[r] = in ? P[p] : 0.f;
-        lv[r] = in ? L[p] : 0.f;
-    }
-    Tile<TZ, H, 0> tvx;
-    Tile<TZ, 0, H> tvz;
-    tile_fetch<TZ, H, 0, FAST>(tvx, Ld1{a.Vx + fo}, nx, nz, x0, z0);
-    tile_fetch<TZ, 0, H, FAST>(tvz, Ld1{a.Vz + fo}, nx, nz, x0, z0);
-    tile_store(tvx, sVx);
-    tile_store(tvz, sVz);
-    __syncthreads();
-    if (!xin) return;
-    const Tab tb{a.tab, a.lpml};
-    const PmlPos px = FAST ? PmlPos{-1, 0} : pml_pos(x, nx, a.lpml);
-    const int kx = FAST ? ORDER : kbw(x, nx, ORDER);
-#pragma unroll
-    for (int r = 0; r < RZ; r++) {
-        const int zl = ty * RZ + r, z = z0 + zl;
-        if (!FAST && z >= nz) break;
-        const float dVx = ddbw<ORDER, FAST>(sVx + zl * LDX + tx + H, 1, a.cx.c, kx, a.invdx);
-        const float dVz = ddbw<ORDER, FAST>(sVz + (zl + H) * LDZ + tx, LDZ, a.cz.c,
-                                            FAST ? ORDER : kbw(z, nz, ORDER), a.invdz);
-        const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
-        const float dtL = a.dt * lv[r];
-        term<FAST>(pv[r], dtL, dVx, px, tb, 0, (a.psi[RSG_A2D_VXX_LEFT] + xo), (a.psi[RSG_A2D_VXX_RIGHT] + xo),
-                   (size_t)z * a.lpml + px.i);
-        term<FAST>(pv[r], dtL, dVz, pz, tb, 0, (a.psi[RSG_A2D_VZZ_TOP] + zo), (a.psi[RSG_A2D_VZZ_BOTTOM] + zo),
-                   (size_t)pz.i * nx + x);
-        P[(size_t)z * nx + x] = pv[r];
-    }
-}
-
-template <int ORDER, int RZ>
-__global__ __launch_bounds__(BX *BZ) void a2d_stress_kernel(const K2D a) {
-    constexpr int H = ORDER, TZ = BZ * RZ, LDX = BX + 2 * H, LDZ = BX;
-    __shared__ float sVx[TZ * LDX];
-    __shared__ float sVz[(TZ + 2 * H) * LDZ];
-    int x0, z0;
-    if (!tile_of(a, TZ, x0, z0)) return;
-    if (interior_tile(a, x0, z0, TZ, H))
-        a2d_stress_body<ORDER, RZ, true>(a, x0, z0, sVx, sVz);
-    else
-        a2d_stress_body<ORDER, RZ, false>(a, x0, z0, sVx, sVz);
-}
-
-// ---------------------------------------------------------------------------
-// Elastic 2D
-// ---------------------------------------------------------------------------
-
-// Velocity update shared by the forward (BW=false, plain stresses) and
-// adjoint-state (BW=true, moduli-weighted stresses) steps:
-//   Vx += dt*Rx*(ddx_fw(Wxx) + ddz_bw(Wxz)),  Vz += dt*Rz*(ddx_bw(Wxz) + ddz_fw(Wzz))
-// with Wxx=Sxx, Wxz=Sxz, Wzz=Szz (forward) or Wxx=L2M*Sxx+L*Szz, Wxz=M*Sxz,
-// Wzz=L*Sxx+L2M*Szz (adjoint state, waves.cpp:1771-1878).
-template <int ORDER, int RZ, bool BW, bool FAST>
-__device__ __forceinline__ void e2d_velocity_body(const K2D &a, int x0, int z0, float *sXX,
-                                                  float *sZZ, float *sXZ) {
-    constexpr int H = ORDER, TZ = BZ * RZ;
-    constexpr int LDXX = BX + 2 * H, LDZZ = BX, LDXZ = BX + 2 * H;
-    const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    const bool xin = FAST || x < nx;
-    const size_t fo = blockIdx.y * (size_t)a.sf, xo = blockIdx.y * (size_t)a.sxs,
-                 zo = blockIdx.y * (size_t)a.szs;
-    float *__restrict__ Vx = a.Vx + fo;
-    float *__restrict__ Vz = a.Vz + fo;
-    const float *__restrict__ Rx = a.Rx;
-    const float *__restrict__ Rz = a.Rz;
-    float vx[RZ], vz[RZ], rx[RZ], rz[RZ];
-#pragma unroll
-    for (int r = 0; r < RZ; r++) {
-        const int z = z0 + ty * RZ + r;
-        const size_t p = (size_t)z * nx + x;
-        const bool in = FAST || (xin && z < nz);
-        vx[r] = in ? Vx[p] : 0.f;
-        vz[r] = in ? Vz[p] : 0.f;
-        rx[r] = in ? Rx[p] : 0.f;
-        rz[r] = in ? Rz[p] : 0.f;
-    }
-    Tile<TZ, H, 0> txx;
-    Tile<TZ, 0, H> tzz;
-    Tile<TZ, H, H> txz;
-    if (!BW) {
-        tile_fetch<TZ, H, 0, FAST>(txx, Ld1{a.Sxx + fo}, nx, nz, x0, z0);
-        tile_fetch<TZ, 0, H, FAST>(tzz, Ld1{a.Szz + fo}, nx, nz, x0, z0);
-        tile_fetch<TZ, H, H, FAST>(txz, Ld1{a.Sxz + fo}, nx, nz, x0, z0);
-    } else {
-        tile_fetch<TZ, H, 0, FAST>(txx, Ld4{a.L2M, a.Sxx + fo, a.L, a.Szz + fo}, nx, nz, x0, z0);
-        tile_fetch<TZ, 0, H, FAST>(tzz, Ld4{a.L, a.Sxx + fo, a.L2M, a.Szz + fo}, nx, nz, x0, z0);
-        tile_fetch<TZ, H, H, FAST>(txz, Ld2{a.M, a.Sxz + fo}, nx, nz, x0, z0);
-    }
-    tile_store(txx, sXX);
-    tile_store(tzz, sZZ);
-    tile_store(txz, sXZ);
-    __syncthreads();
-    if (!xin) return;
-    const Tab tb{a.tab, a.lpml};
-    const PmlPos px = FAST ? PmlPos{-1, 0} : pml_pos(x, nx, a.lpml);
-    const int kxf = FAST ? ORDER : kfw(x, nx, ORDER), kxb = FAST ? ORDER : kbw(x, nx, ORDER);
-#pragma unroll
-    for (int r = 0; r < RZ; r++) {
-        const int zl = ty * RZ + r, z = z0 + zl;
-        if (!FAST && z >= nz) break;
-        const float dSxx = ddfw<ORDER, FAST>(sXX + zl * LDXX + tx + H, 1, a.cx.c, kxf, a.invdx);
-        const float *sxz = sXZ + (zl + H) * LDXZ + tx + H;
-        const float dSxz_z = ddbw<ORDER, FAST>(sxz, LDXZ, a.cz.c, FAST ? ORDER : kbw(z, nz, ORDER),
-                                               a.invdz);
-        const float dSxz_x = ddbw<ORDER, FAST>(sxz, 1, a.cx.c, kxb, a.invdx);
-        const float dSzz = ddfw<ORDER, FAST>(sZZ + (zl + H) * LDZZ + tx, LDZZ, a.cz.c,
-                                             FAST ? ORDER : kfw(z, nz, ORDER), a.invdz);
-        const size_t p = (size_t)z * nx + x;
-        const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
-        const size_t ix = (size_t)z * a.lpml + px.i, iz = (size_t)pz.i * nx + x;
-        const float dtRx = a.dt * rx[r];
-        term<FAST>(vx[r], dtRx, dSxx, px, tb, 1, (a.psi[RSG_E2D_SXX_LEFT] + xo), (a.psi[RSG_E2D_SXX_RIGHT] + xo), ix);
-        term<FAST>(vx[r], dtRx, dSxz_z, pz, tb, 0, (a.psi[RSG_E2D_SXZZ_TOP] + zo),
-                   (a.psi[RSG_E2D_SXZZ_BOTTOM] + zo), iz);
-        Vx[p] = vx[r];
-        const float dtRz = a.dt * rz[r];
-        term<FAST>(vz[r], dtRz, dSxz_x, px, tb, 0, (a.psi[RSG_E2D_SXZX_LEFT] + xo),
-                   (a.psi[RSG_E2D_SXZX_RIGHT] + xo), ix);
-        term<FAST>(vz[r], dtRz, dSzz, pz, tb, 1, (a.psi[RSG_E2D_SZZ_TOP] + zo), (a.psi[RSG_E2D_SZZ_BOTTOM] + zo), iz);
-        Vz[p] = vz[r];
-    }
-}
-
-template <int ORDER, int RZ, bool BW>
-__global__ __launch_bounds__(BX *BZ) void e2d_velocity_kernel(const K2D a) {
-    constexpr int H = ORDER, TZ = BZ * RZ;
-    __shared__ float sXX[TZ * (BX + 2 * H)];
-    __shared__ float sZZ[(TZ + 2 * H) * BX];
-    __shared__ float sXZ[(TZ + 2 * H) * (BX + 2 * H)];
-    int x0, z0;
-    if (!tile_of(a, TZ, x0, z0)) return;
-    if (interior_tile(a, x0, z0, TZ, H))
-        e2d_velocity_body<ORDER, RZ, BW, true>(a, x0, z0, sXX, sZZ, sXZ);
-    else
-        e2d_velocity_body<ORDER, RZ, BW, false>(a, x0, z0, sXX, sZZ, sXZ);
-}
-
-// Stress update.  Forward (BW=false, waves.cpp:1571-1735):
-//   Sxx += dt*(L2M ddx_bw(Vx) + L ddz_bw(Vz)), Szz += dt*(L ddx_bw(Vx) + L2M ddz_bw(Vz)),
-//   free surface Szz(z=lpml)=0, Sxz += dt*M*(ddx_fw(Vz) + ddz_fw(Vx)).
-// Adjoint state (BW=true, waves.cpp:1907-2080): the same derivatives with
-// unit moduli, Sxx only from ddx_bw(Vx) and Szz only from ddz_bw(Vz).
-template <int ORDER, int RZ, bool BW, bool FAST>
-__device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, float *sVx, float *sVz) {
-    constexpr int H = ORDER, TZ = BZ * RZ, LD = BX + 2 * H;
-    const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    const bool xin = FAST || x < nx;
-    const size_t fo = blockIdx.y * (size_t)a.sf, xo = blockIdx.y * (size_t)a.sxs,
-                 zo = blockIdx.y * (size_t)a.szs;
-    float *__restrict__ Sxx = a.Sxx + fo;
-    float *__restrict__ Szz = a.Szz + fo;
-    float *__restrict__ Sxz = a.Sxz + fo;
-    float sxx[RZ], szz[RZ], sxz[RZ], ml[RZ], ml2m[RZ], mm[RZ];
-#pragma unroll
-    for (int r = 0; r < RZ; r++) {
-        const int z = z0 + ty * RZ + r;
-        const size_t p = (size_t)z * nx + x;
-        const bool in = FAST || (xin && z < nz);
-        sxx[r] = in ? Sxx[p] : 0.f;
-        szz[r] = in ? Szz[p] : 0.f;
-        sxz[r] = in ? Sxz[p] : 0.f;
-        if (!BW) {
-            ml[r] = in ? a.L[p] : 0.f;
-            ml2m[r] = in ? a.L2M[p] : 0.f;
-            mm[r] = in ? a.M[p] : 0.f;
-        }
-    }
-    Tile<TZ, H, H> tvx, tvz;
-    tile_fetch<TZ, H, H, FAST>(tvx, Ld1{a.Vx + fo}, nx, nz, x0, z0);
-    tile_fetch<TZ, H, H, FAST>(tvz, Ld1{a.Vz + fo}, nx, nz, x0, z0);
-    tile_store(tvx, sVx);
-    tile_store(tvz, sVz);
-    __syncthreads();
-    if (!xin) return;
-    const Tab tb{a.tab, a.lpml};
-    const PmlPos px = FAST ? PmlPos{-1, 0} : pml_pos(x, nx, a.lpml);
-    const int kxf = FAST ? ORDER : kfw(x, nx, ORDER), kxb = FAST ? ORDER : kbw(x, nx, ORDER);
-    const float dt = a.dt;
-#pragma unroll
-    for (int r = 0; r < RZ; r++) {
-        const int zl = ty * RZ + r, z = z0 + zl;
-        if (!FAST && z >= nz) break;
-        const float *svx = sVx + (zl + H) * LD + tx + H;
-        const float *svz = sVz + (zl + H) * LD + tx + H;
-        const int kzf = FAST ? ORDER : kfw(z, nz, ORDER), kzb = FAST ? ORDER : kbw(z, nz, ORDER);
-        const float dVx_x = ddbw<ORDER, FAST>(svx, 1, a.cx.c, kxb, a.invdx);
-        const float dVz_z = ddbw<ORDER, FAST>(svz, LD, a.cz.c, kzb, a.invdz);
-        const float dVz_x = ddfw<ORDER, FAST>(svz, 1, a.cx.c, kxf, a.invdx);
-        const float dVx_z = ddfw<ORDER, FAST>(svx, LD, a.cz.c, kzf, a.invdz);
-        const size_t p = (size_t)z * nx + x;
-        const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
-        const size_t ix = (size_t)z * a.lpml + px.i, iz = (size_t)pz.i * nx + x;
-        if (!BW) {
-            const float dtL2M = dt * ml2m[r], dtL = dt * ml[r];
-            // d/dx Vx feeds Sxx (L2M) and Szz (L) through one memory variable
-            sxx[r] += dtL2M * dVx_x;
-            szz[r] += dtL * dVx_x;
-            if (!FAST && px.side >= 0) {
-                float *psi = a.psi[px.side ? RSG_E2D_VXX_RIGHT : RSG_E2D_VXX_LEFT] + xo + ix;
-                const float A = tb.A(px.side, 0, px.i), B = tb.B(px.side, 0, px.i),
-                            C = tb.C(px.side, 0, px.i);
-                const float q = B * (*psi) + A * dVx_x;
-                *psi = q;
-                sxx[r] -= dtL2M * (q + C * dVx_x);
-                szz[r] -= dtL * (q + C * dVx_x);
-            }
-            sxx[r] += dtL * dVz_z;
-            szz[r] += dtL2M * dVz_z;
-            if (!FAST && pz.side >= 0) {
-                float *psi = a.psi[pz.side ? RSG_E2D_VZZ_BOTTOM : RSG_E2D_VZZ_TOP] + zo + iz;
-                const float A = tb.A(pz.side, 0, pz.i), B = tb.B(pz.side, 0, pz.i),
-                            C = tb.C(pz.side, 0, pz.i);
-                const float q = B * (*psi) + A * dVz_z;
-                *psi = q;
-                sxx[r] -= dtL * (q + C * dVz_z);
-                szz[r] -= dtL2M * (q + C * dVz_z);
-            }
-            if (!FAST && a.fs && z == a.lpml) szz[r] = 0.f;
-            const float dtM = dt * mm[r];
-            term<FAST>(sxz[r], dtM, dVz_x, px, tb, 1, (a.psi[RSG_E2D_VZX_LEFT] + xo), (a.psi[RSG_E2D_VZX_RIGHT] + xo), ix);
-            term<FAST>(sxz[r], dtM, dVx_z, pz, tb, 1, (a.psi[RSG_E2D_VXZ_TOP] + zo), (a.psi[RSG_E2D_VXZ_BOTTOM] + zo), iz);
-        } else {
-            term<FAST>(sxx[r], dt, dVx_x, px, tb, 0, (a.psi[RSG_E2D_VXX_LEFT] + xo), (a.psi[RSG_E2D_VXX_RIGHT] + xo), ix);
-            term<FAST>(szz[r], dt, dVz_z, pz, tb, 0, (a.psi[RSG_E2D_VZZ_TOP] + zo), (a.psi[RSG_E2D_VZZ_BOTTOM] + zo), iz);
-            if (!FAST && a.fs && z == a.lpml) szz[r] = 0.f;
-            term<FAST>(sxz[r], dt, dVz_x, px, tb, 1, (a.psi[RSG_E2D_VZX_LEFT] + xo), (a.psi[RSG_E2D_VZX_RIGHT] + xo), ix);
-            term<FAST>(sxz[r], dt, dVx_z, pz, tb, 1, (a.psi[RSG_E2D_VXZ_TOP] + zo), (a.psi[RSG_E2D_VXZ_BOTTOM] + zo), iz);
-        }
-        Sxx[p] = sxx[r];
-        Szz[p] = szz[r];
-        Sxz[p] = sxz[r];
-    }
-}
-
-template <int ORDER, int RZ, bool BW>
-__global__ __launch_bounds__(BX *BZ) void e2d_stress_kernel(const K2D a) {
-    constexpr int H = ORDER, TZ = BZ * RZ, LD = BX + 2 * H;
-    __shared__ float sVx[(TZ + 2 * H) * LD];
-    __shared__ float sVz[(TZ + 2 * H) * LD];
-    int x0, z0;
-    if (!tile_of(a, TZ, x0, z0)) return;
-    if (interior_tile(a, x0, z0, TZ, H))
-        e2d_stress_body<ORDER, RZ, BW, true>(a, x0, z0, sVx, sVz);
-    else
-        e2d_stress_body<ORDER, RZ, BW, false>(a, x0, z0, sVx, sVz);
-}
-
-// ---------------------------------------------------------------------------
-// Host launchers
-// ---------------------------------------------------------------------------
-
-static int check_grid2d(const rsg_grid *g, const rsg_state *s) {
-    if (!g || !s) return fail(RSG_EINVAL, "null grid/state");
-    if (g->order < 1 || g->order > 8) return fail(RSG_EINVAL, "order must be 1..8");
-    if (g->ny != 1) return fail(RSG_EINVAL, "2D kernels need ny == 1");
-    if (g->lpml < 1 || g->nx <= 2 * g->lpml || g->nz <= 2 * g->lpml)
-        return fail(RSG_EINVAL, "grid must exceed 2*lpml in x and z");
-    if (g->nx < 2 * g->order + 2 || g->nz < 2 * g->order + 2)
-        return fail(RSG_EINVAL, "grid too small for the stencil order");
-    if (!s->tab) return fail(RSG_EINVAL, "PML tables not set");
-    return RSG_OK;
-}
-
-static K2D make_k2d(const rsg_grid *g, const rsg_state *s, int tz) {
-    K2D a;
-    a.nx = g->nx;
-    a.nz = g->nz;
-    a.lpml = g->lpml;
-    a.fs = g->fs;
-    a.ntx = (g->nx + BX - 1) / BX;
-    a.ntiles = a.ntx * ((g->nz + tz - 1) / tz);
-    a.sf = a.sxs = a.szs = 0;
-    a.dt = g->dt;
-    a.invdx = 1.0f / g->dx;
-    a.invdz = 1.0f / g->dz;
-    a.tab = s->tab;
-    a.P = s->P;
-    a.Vx = s->Vx;
-    a.Vz = s->Vz;
-    a.Sxx = s->Sxx;
-    a.Szz = s->Szz;
-    a.Sxz = s->Sxz;
-    a.L = s->L;
-    a.L2M = s->L2M;
-    a.M = s->M;
-    a.Rx = s->Rx;
-    a.Rz = s->Rz;
-    for (int i = 0; i < RSG_MAX_STRIPS; i++) a.psi[i] = s->psi[i];
-    a.cx = scaled_coefs(g->dx);
-    a.cz = scaled_coefs(g->dz);
-    return a;
-}
-
-// Rows per thread from the tuning table (2, 4 or 8).
-static int rz2d() {
-    const int v = tuning().rz2d;
-    return v <= 2 ? 2 : (v <= 4 ? 4 : 8);
-}
-
-static unsigned nblocks(const K2D &a) { return 8u * (unsigned)((a.ntiles + 7) / 8); }
-
-#define RSG_ORDER_CASES(LAUNCH, RZ) \
-    switch (g->order) {              \
-    case 1: LAUNCH(1, RZ); break;    \
-    case 2: LAUNCH(2, RZ); break;    \
-    case 3: LAUNCH(3, RZ); break;    \
-    case 4: LAUNCH(4, RZ); break;    \
-    case 5: LAUNCH(5, RZ); break;    \
-    case 6: LAUNCH(6, RZ); break;    \
-    case 7: LAUNCH(7, RZ); break;    \
-    default: LAUNCH(8, RZ); break;   \
-    }
-
-#define RSG_DISPATCH(LAUNCH)                      \
-    do {                                          \
-        const int rz_ = rz2d();                   \
-        if (rz_ == 2) {                           \
-            RSG_ORDER_CASES(LAUNCH, 2)            \
-        } else if (rz_ == 4) {                    \
-            RSG_ORDER_CASES(LAUNCH, 4)            \
-        } else {                                  \
-            RSG_ORDER_CASES(LAUNCH, 8)            \
-        }                                         \
-    } while (0)
-
-}  // namespace rsg
+// Uniform-batch launchers of the 2D staggered-grid kernels (one grid, one
+// or many shots; blockIdx.y = shot) and their C-ABI entry points.  The
+// kernels themselves are in rsg_fd2d.h.
+#include "rsg_fd2d.h"
 
 using namespace rsg;
 

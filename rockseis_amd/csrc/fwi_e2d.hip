@@ -1,0 +1,500 @@
+// Elastic 2D FWI gradient (FwiElastic2D::run_optimal, fwi.cpp:3937-4143) on
+// the device, for a ragged batch of shots: every shot has its own aperture
+// window (grid size, staggered model, fields, strips) and all shots advance
+// together, one launch per half step (rsg_ragged2d plans).
+//
+// Per shot, as in the reference: forward elastic steps with the source and
+// the P / Ux / Uz gathers; at the first turn the DIFFERENCE misfit of the
+// three records (fwi.cpp:2852-2930) and their residuals (fwi.cpp:3227-3300);
+// the adjoint wavefield advanced with backwardstepVelocity/Stress
+// (waves.cpp:1738-2080) and driven by the three residuals; at every reverse
+// step crossCorr (fwi.cpp:2591-2775) into the lambda / mu / rho kernels of
+// the window, plus the wavelet gradient.  Revolve checkpoints (fields and
+// strips of every shot) live in HBM, FP32 or FP16 with one scale per array.
+// At the end each window's kernels are mapped to Vp / Vs / rho gradients
+// (scaleGrad, fwi.cpp:2778-2849) or kept as lambda / mu / rho, and all
+// windows are stacked into the caller's images at their x offsets, in
+// double, shot by shot.
+#include <algorithm>
+#include <vector>
+
+#include "rsg_common.h"
+#include "rsg_fwi.h"
+#include "rsg_ragged.h"
+#include "rsg_srcrec_dev.h"
+#include "rsg_store16.h"
+
+namespace rsg {
+
+namespace {
+
+// Everything a kernel needs of one shot.
+struct ShotDev {
+    int nxp, nzp, nx, nz, lpml, ox;
+    float *fVx, *fVz, *fSxx, *fSzz, *fSxz;
+    float *bVx, *bVz, *bSxx, *bSzz, *bSxz;
+    const float *Rx, *Rz, *Vp, *Vs, *Rho;
+    int sfield, nsrc, nrec;
+    const int *smap;
+    const float *sshift, *wav;
+    float *wavgrad;
+    const int *gmap[3];
+    const float *gshift[3], *dobs[3];
+    float *dmod[3], *dres[3];
+    float *lam, *mu, *rho;
+};
+
+constexpr int kIoThreads = 64;
+
+// insertSource of the forward wavefield (waves.cpp:2085-2171): PRESSURE is
+// explosive (Sxx and Szz -= dt*wav*w), VX / VZ scale by Rx / Rz.
+__global__ void e2d_source_kernel(const ShotDev *__restrict__ sd, int it, int nt, float dt) {
+    const ShotDev &s = sd[blockIdx.y];
+    const int i = blockIdx.x * kIoThreads + threadIdx.x;
+    if (i >= s.nsrc) return;
+    if (s.sfield == RSG_PRESSURE) {
+        inject2d_trace(i, s.fSxx, nullptr, -dt, s.nxp, s.lpml, s.smap, s.sshift, s.wav, nt, it);
+        inject2d_trace(i, s.fSzz, nullptr, -dt, s.nxp, s.lpml, s.smap, s.sshift, s.wav, nt, it);
+    } else if (s.sfield == RSG_VX) {
+        inject2d_trace(i, s.fVx, s.Rx, dt, s.nxp, s.lpml, s.smap, s.sshift, s.wav, nt, it);
+    } else {
+        inject2d_trace(i, s.fVz, s.Rz, dt, s.nxp, s.lpml, s.smap, s.sshift, s.wav, nt, it);
+    }
+}
+
+// recordData of P (0.5*(Sxx+Szz), no reciprocity scaling), Ux and Uz
+// (waves.cpp:2174-2288).
+__global__ void e2d_record_kernel(const ShotDev *__restrict__ sd, int it, int nt) {
+    const ShotDev &s = sd[blockIdx.y];
+    const int i = blockIdx.x * kIoThreads + threadIdx.x;
+    if (i >= s.nrec) return;
+    record2d_trace(i, s.fSxx, s.fSzz, nullptr, s.nxp, s.lpml, s.gmap[0], s.gshift[0], s.dmod[0], nt, it);
+    record2d_trace(i, s.fVx, nullptr, nullptr, s.nxp, s.lpml, s.gmap[1], s.gshift[1], s.dmod[1], nt, it);
+    record2d_trace(i, s.fVz, nullptr, nullptr, s.nxp, s.lpml, s.gmap[2], s.gshift[2], s.dmod[2], nt, it);
+}
+
+// The three residuals into the adjoint wavefield (insertSource of dataresP,
+// dataresUx, dataresUz, fwi.cpp:4069-4071).
+__global__ void e2d_residual_kernel(const ShotDev *__restrict__ sd, int it, int nt, float dt) {
+    const ShotDev &s = sd[blockIdx.y];
+    const int i = blockIdx.x * kIoThreads + threadIdx.x;
+    if (i >= s.nrec) return;
+    inject2d_trace(i, s.bSxx, nullptr, -dt, s.nxp, s.lpml, s.gmap[0], s.gshift[0], s.dres[0], nt, it);
+    inject2d_trace(i, s.bSzz, nullptr, -dt, s.nxp, s.lpml, s.gmap[0], s.gshift[0], s.dres[0], nt, it);
+    inject2d_trace(i, s.bVx, s.Rx, dt, s.nxp, s.lpml, s.gmap[1], s.gshift[1], s.dres[1], nt, it);
+    inject2d_trace(i, s.bVz, s.Rz, dt, s.nxp, s.lpml, s.gmap[2], s.gshift[2], s.dres[2], nt, it);
+}
+
+// Wavelet gradient of crossCorr: the adjoint field of the source type
+// gathered at source traces whose map falls on an interior cell
+// ix, iz in [1, n-1) (fwi.cpp:2696-2758).
+__global__ void e2d_wavgrad_kernel(const ShotDev *__restrict__ sd, int it, int nt) {
+    const ShotDev &s = sd[blockIdx.y];
+    const int i = blockIdx.x * kIoThreads + threadIdx.x;
+    if (!s.wavgrad || i >= s.nsrc) return;
+    const int mx = s.smap[2 * i], mz = s.smap[2 * i + 1];
+    if (mx < 1 || mx >= s.nx - 1 || mz < 1 || mz >= s.nz - 1) return;
+    const float sx = s.sshift[2 * i], sz = s.sshift[2 * i + 1];
+    const float xtri[2] = {1 - sx, sx}, ytri[2] = {1 - sz, sz};
+    float acc = s.wavgrad[(size_t)i * nt + it];
+    for (int i1 = 0; i1 < 2; i1++)
+        for (int i2 = 0; i2 < 2; i2++) {
+            const size_t q = (size_t)(mz + s.lpml + i1) * s.nxp + (mx + s.lpml + i2);
+            const float v = s.sfield == RSG_PRESSURE ? s.bSxx[q] + s.bSzz[q]
+                                                     : (s.sfield == RSG_VX ? s.bVx[q] : s.bVz[q]);
+            acc += xtri[i2] * ytri[i1] * v;
+        }
+    s.wavgrad[(size_t)i * nt + it] = acc;
+}
+
+// crossCorr of one reverse step (fwi.cpp:2676-2772), interior ix, iz in
+// [1, n-1) of every window (blockIdx.z = shot), with the reference's
+// float / double promotions.
+__global__ __launch_bounds__(256) void e2d_crosscorr_kernel(const ShotDev *__restrict__ sd, float dx,
+                                                             float dz) {
+    const ShotDev &s = sd[blockIdx.z];
+    const int ix = blockIdx.x * 64 + threadIdx.x + 1;
+    const int iz = blockIdx.y * 4 + threadIdx.y + 1;
+    if (ix >= s.nx - 1 || iz >= s.nz - 1) return;
+    const int n = s.nxp;
+    const size_t q = (size_t)(iz + s.lpml) * n + ix + s.lpml;
+    const float *wsx = s.fVx, *wsz = s.fVz, *wsxx = s.fSxx, *wszz = s.fSzz, *wsxz = s.fSxz;
+    const float *wrx = s.bVx, *wrz = s.bVz, *rsxx = s.bSxx, *rszz = s.bSzz, *rsxz = s.bSxz;
+    const float msxx = (wsx[q] - wsx[q - 1]) / dx;
+    const float mszz = (wsz[q] - wsz[q - n]) / dz;
+    const size_t m = (size_t)iz * s.nx + ix;
+    s.lam[m] -= (msxx + mszz) * (rsxx[q] + rszz[q]);
+    const float msxz1 = (wsx[q - 1] - wsx[q - 1 - n]) / dz + (wsz[q - n] - wsz[q - 1 - n]) / dx;
+    const float msxz2 = (wsx[q] - wsx[q - n]) / dz + (wsz[q + 1 - n] - wsz[q - n]) / dx;
+    const float msxz3 = (wsx[q - 1 + n] - wsx[q - 1]) / dz + (wsz[q] - wsz[q - 1]) / dx;
+    const float msxz4 = (wsx[q + n] - wsx[q]) / dz + (wsz[q + 1] - wsz[q]) / dx;
+    const float f1 = msxx * rszz[q] + mszz * rsxx[q];
+    const float f2 = msxz1 * rsxz[q - 1 - n] + msxz2 * rsxz[q - n] + msxz3 * rsxz[q - 1] + msxz4 * rsxz[q];
+    s.mu[m] = (float)((double)s.mu[m] + (2.0 * (double)f1 - 0.25 * (double)f2));
+    const double rx0 = s.Rx[q], rx1 = s.Rx[q - 1], rz0 = s.Rz[q], rz1 = s.Rz[q - n];
+    float ux = (float)(0.5 * (double)wrx[q] * rx0 * (double)(wsxx[q + 1] - wsxx[q]) / (double)dx);
+    ux = (float)((double)ux + 0.5 * (double)wrx[q - 1] * rx1 * (double)(wsxx[q] - wsxx[q - 1]) / (double)dx);
+    ux = (float)((double)ux + 0.5 * (double)wrx[q] * rx0 * (double)(wsxz[q] - wsxz[q - n]) / (double)dz);
+    ux = (float)((double)ux + 0.5 * (double)wrx[q - 1] * rx1 * (double)(wsxz[q - 1] - wsxz[q - 1 - n]) / (double)dz);
+    float uz = (float)(0.5 * (double)wrz[q] * rz0 * (double)(wsxz[q] - wsxz[q - 1]) / (double)dx);
+    uz = (float)((double)uz + 0.5 * (double)wrz[q - n] * rz1 * (double)(wsxz[q - n] - wsxz[q - 1 - n]) / (double)dx);
+    uz = (float)((double)uz + 0.5 * (double)wrz[q] * rz0 * (double)(wszz[q + n] - wszz[q]) / (double)dz);
+    uz = (float)((double)uz + 0.5 * (double)wrz[q - n] * rz1 * (double)(wszz[q] - wszz[q - n]) / (double)dz);
+    s.rho[m] -= (ux + uz);
+}
+
+// Elastic DIFFERENCE misfit and residuals: res = mod - rec for P, Ux, Uz;
+// misfit += 0.5*(rp^2 + rx^2 + rz^2) over traces whose Ux map is valid,
+// reduced in double (blockIdx.y = shot).
+__global__ __launch_bounds__(256) void e2d_misfit_kernel(const ShotDev *__restrict__ sd, int nt, double *misfit) {
+    __shared__ double part[256];
+    const ShotDev &s = sd[blockIdx.y];
+    const size_t n = (size_t)s.nrec * nt;
+    double acc = 0.0;
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+        const float rp = s.dmod[0][p] - s.dobs[0][p];
+        const float rx = s.dmod[1][p] - s.dobs[1][p];
+        const float rz = s.dmod[2][p] - s.dobs[2][p];
+        s.dres[0][p] = rp;
+        s.dres[1][p] = rx;
+        s.dres[2][p] = rz;
+        const size_t tr = p / nt;
+        if (s.gmap[1][2 * tr] >= 0 && s.gmap[1][2 * tr + 1] >= 0) acc += 0.5 * (double)(rp * rp + rx * rx + rz * rz);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(misfit, part[0]);
+}
+
+// Stack every window's kernels into the caller's images (x offset ox, z
+// from 0), shots in order, in double; with `scale` each window value is
+// first mapped by scaleGrad with that window's Vp, Vs, rho.
+__global__ __launch_bounds__(256) void e2d_stack_kernel(const ShotDev *__restrict__ sd, int nshot, int nx,
+                                                         int nz, float *g0, float *g1, float *g2, int scale) {
+    const int x = blockIdx.x * 256 + threadIdx.x, z = blockIdx.y;
+    if (x >= nx || z >= nz) return;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int k = 0; k < nshot; k++) {
+        const ShotDev &s = sd[k];
+        const int wx = x - s.ox;
+        if (wx < 0 || wx >= s.nx || z >= s.nz) continue;
+        const size_t m = (size_t)z * s.nx + wx;
+        float lam = s.lam[m], mu = s.mu[m], rho = s.rho[m];
+        if (scale) {
+            const float vp = s.Vp[m], vs = s.Vs[m], r = s.Rho[m];
+            const float vpscale = (float)(2.0 * (double)r * (double)vp);
+            const float vsscale = (float)(2.0 * (double)r * (double)vs);
+            const float rhoscale1 = (float)((double)(vp * vp) - 2.0 * (double)vs * (double)vs);
+            const float rhoscale2 = vs * vs;
+            const float l = lam, u = mu;
+            lam = vpscale * l;
+            mu = (float)((double)(vsscale * u) - 2.0 * (double)vsscale * (double)l);
+            rho = rhoscale1 * l + rhoscale2 * u + rho;
+        }
+        a0 += lam;
+        a1 += mu;
+        a2 += rho;
+    }
+    const size_t p = (size_t)z * nx + x;
+    g0[p] = (float)((double)g0[p] + a0);
+    g1[p] = (float)((double)g1[p] + a1);
+    g2[p] = (float)((double)g2[p] + a2);
+}
+
+struct DevMem {
+    std::vector<void *> ptrs;
+    ~DevMem() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    int alloc(T **out, size_t n) {
+        void *p = nullptr;
+        if (n == 0) n = 1;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return fail(RSG_ENOMEM, "fwi_e2d: device allocation failed");
+        ptrs.push_back(p);
+        *out = (T *)p;
+        return RSG_OK;
+    }
+};
+
+struct PlanGuard {
+    Ragged2D p;
+    ~PlanGuard() { ragged2d_destroy(&p); }
+};
+
+// Is strip slot i of the elastic state an x-side strip (lpml x nz)?
+inline bool e2d_xstrip(int i) { return (i / 4) % 2 == 0; }
+
+}  // namespace
+
+static int fwi_e2d_impl(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *sh, int ns, hipStream_t st) {
+    const rsg_grid &g0 = sh[0].g;
+    const int nt = sh[0].src.nt, lpml = g0.lpml, order = g0.order;
+    const float dt = g0.dt;
+    // ---- validation
+    size_t maxsrc = 0, maxrec = 0;
+    int maxnx = 0, maxnz = 0;
+    for (int k = 0; k < ns; k++) {
+        const rsg_fwi_e2d_shot &s = sh[k];
+        const int nx = s.g.nx - 2 * lpml, nz = s.g.nz - 2 * lpml;
+        if (s.g.lpml != lpml || s.g.order != order || s.g.dt != dt || s.g.dx != g0.dx || s.g.dz != g0.dz ||
+            s.g.ny != 1)
+            return fail(RSG_EINVAL, "fwi_e2d: shots must share lpml, order, dx, dz, dt (2D grids)");
+        if (!s.L || !s.L2M || !s.M || !s.Rx || !s.Rz || !s.Vp || !s.Vs || !s.Rho)
+            return fail(RSG_EINVAL, "fwi_e2d: null model array");
+        if (s.src.nt != nt || s.src.ntrace < 0 || (s.src.ntrace > 0 && (!s.src.map || !s.src.shift || !s.src.data)))
+            return fail(RSG_EINVAL, "fwi_e2d: every source needs nt samples and its map/shift/data");
+        if (s.src.field != RSG_PRESSURE && s.src.field != RSG_VX && s.src.field != RSG_VZ)
+            return fail(RSG_EINVAL, "fwi_e2d: source field must be P, VX or VZ");
+        const int want[3] = {RSG_PRESSURE, RSG_VX, RSG_VZ};
+        for (int c = 0; c < 3; c++) {
+            const rsg_traces &o = s.obs[c];
+            if (o.field != want[c] || o.nt != nt || o.ntrace != s.obs[0].ntrace || !o.map || !o.shift || !o.data ||
+                !s.dmod[c] || !s.dres[c])
+                return fail(RSG_EINVAL, "fwi_e2d: obs[0..2] must be P, VX, VZ records of one receiver set");
+        }
+        if (nx < 3 || nz < 3 || s.ox < 0 || s.ox + nx > f->nx || nz > f->nz)
+            return fail(RSG_EINVAL, "fwi_e2d: window outside the stacked image");
+        maxsrc = std::max(maxsrc, (size_t)s.src.ntrace);
+        maxrec = std::max(maxrec, (size_t)s.obs[0].ntrace);
+        maxnx = std::max(maxnx, nx);
+        maxnz = std::max(maxnz, nz);
+    }
+    // ---- state layout: per shot [Vx Vz Sxx Szz Sxz | 16 strips], shots back to back
+    std::vector<size_t> off(ns + 1, 0), segs;
+    segs.reserve((size_t)ns * 21 + 1);
+    size_t maxseg = 0;
+    for (int k = 0; k < ns; k++) {
+        const size_t n = (size_t)sh[k].g.nx * sh[k].g.nz, sx = (size_t)lpml * sh[k].g.nz,
+                     sz = (size_t)sh[k].g.nx * lpml;
+        size_t o = off[k];
+        for (int j = 0; j < 5; j++, o += n) segs.push_back(o);
+        for (int i = 0; i < RSG_E2D_NSTRIP; i++) {
+            segs.push_back(o);
+            o += e2d_xstrip(i) ? sx : sz;
+        }
+        off[k + 1] = o;
+        maxseg = std::max({maxseg, n, sx, sz});
+    }
+    segs.push_back(off[ns]);
+    const size_t total = off[ns];
+    const int nseg = (int)segs.size() - 1;
+    Revolve rev(nt, f->ncheck);
+    const bool f16 = f->fp16 != 0;
+    if (f16 && nseg > 65535) return fail(RSG_EINVAL, "fwi_e2d: FP16 checkpoints take at most 3120 shots");
+    {
+        size_t free_b = 0, total_b = 0;
+        RSG_HIP(hipMemGetInfo(&free_b, &total_b));
+        size_t img = 0;
+        for (int k = 0; k < ns; k++)
+            img += 3 * (size_t)(sh[k].g.nx - 2 * lpml) * (sh[k].g.nz - 2 * lpml);
+        const size_t need = total * 4 * 2 + img * 4 + total * (f16 ? 2 : 4) * (size_t)rev.snaps;
+        if (need + ((size_t)1 << 30) > free_b)
+            return fail(RSG_ENOMEM, "fwi_e2d: states and checkpoints do not fit in device memory");
+    }
+    DevMem dm;
+    float *fwb = nullptr, *bwb = nullptr, *imgs = nullptr;
+    if (int rc = dm.alloc(&fwb, total)) return rc;
+    if (int rc = dm.alloc(&bwb, total)) return rc;
+    if (int rc = launch_zero(fwb, total * sizeof(float), st)) return rc;
+    if (int rc = launch_zero(bwb, total * sizeof(float), st)) return rc;
+    std::vector<size_t> img_off(ns + 1, 0);
+    for (int k = 0; k < ns; k++)
+        img_off[k + 1] = img_off[k] + 3 * (size_t)(sh[k].g.nx - 2 * lpml) * (sh[k].g.nz - 2 * lpml);
+    if (int rc = dm.alloc(&imgs, img_off[ns])) return rc;
+    if (int rc = launch_zero(imgs, img_off[ns] * sizeof(float), st)) return rc;
+    // ---- per-shot states, ragged plans, kernel descriptors
+    std::vector<rsg_grid> grids(ns);
+    std::vector<rsg_state> sf(ns), sb(ns);
+    std::vector<ShotDev> hd(ns);
+    for (int k = 0; k < ns; k++) {
+        const rsg_fwi_e2d_shot &s = sh[k];
+        grids[k] = s.g;
+        const size_t n = (size_t)s.g.nx * s.g.nz, sx = (size_t)lpml * s.g.nz, sz = (size_t)s.g.nx * lpml;
+        for (int w = 0; w < 2; w++) {
+            float *b = (w ? bwb : fwb) + off[k];
+            rsg_state &r = w ? sb[k] : sf[k];
+            r = rsg_state{};
+            r.Vx = b;
+            r.Vz = b + n;
+            r.Sxx = b + 2 * n;
+            r.Szz = b + 3 * n;
+            r.Sxz = b + 4 * n;
+            float *p = b + 5 * n;
+            for (int i = 0; i < RSG_E2D_NSTRIP; i++) {
+                r.psi[i] = p;
+                p += e2d_xstrip(i) ? sx : sz;
+            }
+            r.L = s.L;
+            r.L2M = s.L2M;
+            r.M = s.M;
+            r.Rx = s.Rx;
+            r.Rz = s.Rz;
+            r.tab = f->tab;
+        }
+        ShotDev &d = hd[k];
+        d.nxp = s.g.nx;
+        d.nzp = s.g.nz;
+        d.nx = s.g.nx - 2 * lpml;
+        d.nz = s.g.nz - 2 * lpml;
+        d.lpml = lpml;
+        d.ox = s.ox;
+        d.fVx = sf[k].Vx;
+        d.fVz = sf[k].Vz;
+        d.fSxx = sf[k].Sxx;
+        d.fSzz = sf[k].Szz;
+        d.fSxz = sf[k].Sxz;
+        d.bVx = sb[k].Vx;
+        d.bVz = sb[k].Vz;
+        d.bSxx = sb[k].Sxx;
+        d.bSzz = sb[k].Szz;
+        d.bSxz = sb[k].Sxz;
+        d.Rx = s.Rx;
+        d.Rz = s.Rz;
+        d.Vp = s.Vp;
+        d.Vs = s.Vs;
+        d.Rho = s.Rho;
+        d.sfield = s.src.field;
+        d.nsrc = s.src.ntrace;
+        d.nrec = s.obs[0].ntrace;
+        d.smap = s.src.map;
+        d.sshift = s.src.shift;
+        d.wav = s.src.data;
+        d.wavgrad = s.wavgrad;
+        for (int c = 0; c < 3; c++) {
+            d.gmap[c] = s.obs[c].map;
+            d.gshift[c] = s.obs[c].shift;
+            d.dobs[c] = s.obs[c].data;
+            d.dmod[c] = s.dmod[c];
+            d.dres[c] = s.dres[c];
+        }
+        const size_t ni = (size_t)d.nx * d.nz;
+        d.lam = imgs + img_off[k];
+        d.mu = d.lam + ni;
+        d.rho = d.mu + ni;
+        for (int c = 0; c < 3; c++)
+            if (int rc = launch_zero(s.dmod[c], sizeof(float) * d.nrec * nt, st)) return rc;
+        if (s.wavgrad)
+            if (int rc = launch_zero(s.wavgrad, sizeof(float) * d.nsrc * nt, st)) return rc;
+    }
+    PlanGuard pf, pb;
+    if (int rc = ragged2d_create(RSG_KIND_E2D, ns, grids.data(), sf.data(), &pf.p)) return rc;
+    if (int rc = ragged2d_create(RSG_KIND_E2D, ns, grids.data(), sb.data(), &pb.p)) return rc;
+    ShotDev *dsd = nullptr;
+    size_t *dsegs = nullptr;
+    double *dmis = nullptr;
+    if (int rc = dm.alloc(&dsd, ns)) return rc;
+    if (int rc = dm.alloc(&dmis, 1)) return rc;
+    if (int rc = launch_zero(dmis, sizeof(double), st)) return rc;
+    RSG_HIP(hipMemcpyAsync(dsd, hd.data(), sizeof(ShotDev) * ns, hipMemcpyHostToDevice, st));
+    if (f16) {
+        if (int rc = dm.alloc(&dsegs, segs.size())) return rc;
+        RSG_HIP(hipMemcpyAsync(dsegs, segs.data(), sizeof(size_t) * segs.size(), hipMemcpyHostToDevice, st));
+    }
+    float *ck = nullptr;
+    half_t *ck16 = nullptr;
+    unsigned *ckamax = nullptr;
+    if (f16) {
+        if (int rc = dm.alloc(&ck16, total * (size_t)rev.snaps)) return rc;
+        if (int rc = dm.alloc(&ckamax, (size_t)nseg * rev.snaps)) return rc;
+    } else {
+        if (int rc = dm.alloc(&ck, total * (size_t)rev.snaps)) return rc;
+    }
+    // ---- step helpers
+    const dim3 iob_s((unsigned)((maxsrc + kIoThreads - 1) / kIoThreads), ns);
+    const dim3 iob_r((unsigned)((maxrec + kIoThreads - 1) / kIoThreads), ns);
+    const dim3 xcg((maxnx + 63) / 64, (maxnz + 3) / 4, ns);
+    auto fstep = [&](int it, bool rec) -> int {
+        if (int rc = ragged2d_step(&pf.p, 0, st)) return rc;
+        if (int rc = ragged2d_step(&pf.p, 1, st)) return rc;
+        if (maxsrc) {
+            e2d_source_kernel<<<iob_s, kIoThreads, 0, st>>>(dsd, it, nt, dt);
+            RSG_LAUNCH_CHECK("e2d_source_kernel");
+        }
+        if (rec && maxrec) {
+            e2d_record_kernel<<<iob_r, kIoThreads, 0, st>>>(dsd, it, nt);
+            RSG_LAUNCH_CHECK("e2d_record_kernel");
+        }
+        return RSG_OK;
+    };
+    auto reverse_tail = [&](int it) -> int {
+        if (maxrec) {
+            e2d_residual_kernel<<<iob_r, kIoThreads, 0, st>>>(dsd, it, nt, dt);
+            RSG_LAUNCH_CHECK("e2d_residual_kernel");
+        }
+        e2d_crosscorr_kernel<<<xcg, dim3(64, 4), 0, st>>>(dsd, g0.dx, g0.dz);
+        RSG_LAUNCH_CHECK("e2d_crosscorr_kernel");
+        if (maxsrc) {
+            e2d_wavgrad_kernel<<<iob_s, kIoThreads, 0, st>>>(dsd, it, nt);
+            RSG_LAUNCH_CHECK("e2d_wavgrad_kernel");
+        }
+        return RSG_OK;
+    };
+    // ---- Revolve loop (fwi.cpp:3986-4124)
+    bool reverse = false;
+    for (;;) {
+        const int oldcapo = rev.capo;
+        const Revolve::Action a = rev.next();
+        const int capo = rev.capo;
+        if (a == Revolve::ADVANCE) {
+            for (int it = oldcapo; it < capo; it++)
+                if (int rc = fstep(it, !reverse)) return rc;
+        } else if (a == Revolve::FIRSTURN) {
+            if (int rc = fstep(capo, true)) return rc;
+            if (maxrec) {
+                e2d_misfit_kernel<<<dim3(64, ns), 256, 0, st>>>(dsd, nt, dmis);
+                RSG_LAUNCH_CHECK("e2d_misfit_kernel");
+            }
+            if (int rc = reverse_tail(capo)) return rc;
+            reverse = true;
+        } else if (a == Revolve::YOUTURN) {
+            if (int rc = ragged2d_step(&pb.p, 2, st)) return rc;
+            if (int rc = ragged2d_step(&pb.p, 3, st)) return rc;
+            if (int rc = reverse_tail(capo)) return rc;
+        } else if (a == Revolve::TAKESHOT) {
+            if (f16) {
+                if (int rc = pack16_table(fwb, ck16 + total * rev.check, ckamax + (size_t)nseg * rev.check, dsegs,
+                                          nseg, maxseg, st))
+                    return rc;
+            } else {
+                RSG_HIP(hipMemcpyAsync(ck + total * rev.check, fwb, total * sizeof(float), hipMemcpyDeviceToDevice,
+                                       st));
+            }
+        } else if (a == Revolve::RESTORE) {
+            if (f16) {
+                if (int rc = unpack16_table(ck16 + total * rev.check, ckamax + (size_t)nseg * rev.check, fwb, dsegs,
+                                            nseg, maxseg, st))
+                    return rc;
+            } else {
+                RSG_HIP(hipMemcpyAsync(fwb, ck + total * rev.check, total * sizeof(float), hipMemcpyDeviceToDevice,
+                                       st));
+            }
+        } else if (a == Revolve::TERMINATE) {
+            break;
+        } else {
+            return fail(RSG_EINVAL, "fwi_e2d: Revolve returned an error");
+        }
+    }
+    e2d_stack_kernel<<<dim3((f->nx + 255) / 256, f->nz), 256, 0, st>>>(dsd, ns, f->nx, f->nz, f->grad[0], f->grad[1],
+                                                                         f->grad[2], f->scale);
+    RSG_LAUNCH_CHECK("e2d_stack_kernel");
+    double m = 0.0;
+    RSG_HIP(hipMemcpyAsync(&m, dmis, sizeof(double), hipMemcpyDeviceToHost, st));
+    RSG_HIP(hipStreamSynchronize(st));
+    if (f->misfit) *f->misfit = m;
+    return RSG_OK;
+}
+
+}  // namespace rsg
+
+extern "C" int rsg_fwi_e2d_gradient_batch(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *shots, int nshot,
+                                          void *stream) {
+    if (!f || !shots || nshot < 1 || nshot > 65535) return rsg::fail(RSG_EINVAL, "rsg_fwi_e2d_gradient_batch: bad arguments");
+    if (!f->tab || !f->grad[0] || !f->grad[1] || !f->grad[2] || f->nx < 1 || f->nz < 1)
+        return rsg::fail(RSG_EINVAL, "rsg_fwi_e2d_gradient_batch: null tables or images");
+    return rsg::fwi_e2d_impl(f, shots, nshot, (hipStream_t)stream);
+}

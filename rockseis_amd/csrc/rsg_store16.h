@@ -7,8 +7,10 @@
 // in (-32768, 32768), inside FP16's finite range, and the scaling itself is
 // exact, so the only rounding is FP32 -> FP16 (round to nearest even,
 // relative error <= 2^-11 for values above amax * 2^-29).  An all-zero array
-// keeps scale 1.  amax is kept as the bit pattern of a non-negative float,
-// which orders like an unsigned integer, so it is reduced with atomicMax.
+// keeps scale 1; the exponent is capped at 126 so that arrays of tiny values
+// (early PML memory variables, denormals) never get an infinite scale.  amax
+// is kept as the bit pattern of a non-negative float, which orders like an
+// unsigned integer, so it is reduced with atomicMax.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -18,21 +20,20 @@ namespace rsg {
 
 using half_t = _Float16;
 
-__device__ inline float store16_scale(unsigned amax_bits) {
+// Exponent s of the scale 2^s: 15 - E, capped at 126 so that both 2^s and
+// 2^-s are finite normal floats (amax below 2^-111 keeps s = 126).
+__device__ inline int store16_exp(unsigned amax_bits) {
     const float a = __uint_as_float(amax_bits);
-    if (!(a > 0.f) || !isfinite(a)) return 1.f;
+    if (!(a > 0.f) || !isfinite(a)) return 0;
     int e;
     (void)frexpf(a, &e);
-    return ldexpf(1.f, 15 - e);
+    const int s = 15 - e;
+    return s > 126 ? 126 : s;
 }
 
-__device__ inline float store16_unscale(unsigned amax_bits) {
-    const float a = __uint_as_float(amax_bits);
-    if (!(a > 0.f) || !isfinite(a)) return 1.f;
-    int e;
-    (void)frexpf(a, &e);
-    return ldexpf(1.f, e - 15);
-}
+__device__ inline float store16_scale(unsigned amax_bits) { return ldexpf(1.f, store16_exp(amax_bits)); }
+
+__device__ inline float store16_unscale(unsigned amax_bits) { return ldexpf(1.f, -store16_exp(amax_bits)); }
 
 // Contiguous segments: segment s is src[s*seglen .. (s+1)*seglen).  amax
 // must hold nseg zeroed entries (pack16_segments zeroes them first).
@@ -40,6 +41,13 @@ int pack16_segments(const float *src, half_t *dst, unsigned *amax, size_t seglen
                     hipStream_t st);
 int unpack16_segments(const half_t *src, const unsigned *amax, float *dst, size_t seglen, int nseg,
                       hipStream_t st);
+
+// Segments of different lengths: segment s is src[off[s] .. off[s+1]) (off:
+// nseg+1 device offsets, maxlen the longest segment).
+int pack16_table(const float *src, half_t *dst, unsigned *amax, const size_t *off, int nseg, size_t maxlen,
+                 hipStream_t st);
+int unpack16_table(const half_t *src, const unsigned *amax, float *dst, const size_t *off, int nseg,
+                   size_t maxlen, hipStream_t st);
 
 // Unpadded nx x nz window of every shot's padded (lpml) field, shots sstride
 // apart -> dst [nshot][nz][nx] in FP32 (amax == nullptr) or FP16 with one

@@ -58,6 +58,36 @@ __global__ __launch_bounds__(kThreads) void unpack16_segments_kernel(const half_
         dst[o + i] = (float)src[o + i] * us;
 }
 
+// Segments of different lengths: segment s = [off[s], off[s+1]).
+__global__ __launch_bounds__(kThreads) void amax_table_kernel(const float *__restrict__ src, unsigned *amax,
+                                                               const size_t *__restrict__ off) {
+    const size_t o = off[blockIdx.y], n = off[blockIdx.y + 1] - o;
+    float m = 0.f;
+    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
+        m = fmaxf(m, fabsf(src[o + i]));
+    block_amax(m, amax + blockIdx.y);
+}
+
+__global__ __launch_bounds__(kThreads) void pack16_table_kernel(const float *__restrict__ src,
+                                                                 half_t *__restrict__ dst,
+                                                                 const unsigned *__restrict__ amax,
+                                                                 const size_t *__restrict__ off) {
+    const size_t o = off[blockIdx.y], n = off[blockIdx.y + 1] - o;
+    const float sc = store16_scale(amax[blockIdx.y]);
+    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
+        dst[o + i] = (half_t)(src[o + i] * sc);
+}
+
+__global__ __launch_bounds__(kThreads) void unpack16_table_kernel(const half_t *__restrict__ src,
+                                                                   const unsigned *__restrict__ amax,
+                                                                   float *__restrict__ dst,
+                                                                   const size_t *__restrict__ off) {
+    const size_t o = off[blockIdx.y], n = off[blockIdx.y + 1] - o;
+    const float us = store16_unscale(amax[blockIdx.y]);
+    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
+        dst[o + i] = (float)src[o + i] * us;
+}
+
 // Interior window: blockIdx.x covers 256 columns, blockIdx.y 8 rows,
 // blockIdx.z the shot.
 constexpr int kRows = 8;
@@ -128,6 +158,26 @@ int unpack16_segments(const half_t *src, const unsigned *amax, float *dst, size_
     if (nseg < 1 || nseg > 65535) return fail(RSG_EINVAL, "unpack16: 1..65535 segments");
     unpack16_segments_kernel<<<seg_grid(seglen, nseg), kThreads, 0, st>>>(src, amax, dst, seglen);
     RSG_LAUNCH_CHECK("unpack16_segments_kernel");
+    return RSG_OK;
+}
+
+int pack16_table(const float *src, half_t *dst, unsigned *amax, const size_t *off, int nseg, size_t maxlen,
+                 hipStream_t st) {
+    if (nseg < 1 || nseg > 65535) return fail(RSG_EINVAL, "pack16: 1..65535 segments");
+    if (int rc = zero_u32(amax, nseg, st)) return rc;
+    const dim3 g = seg_grid(maxlen, nseg);
+    amax_table_kernel<<<g, kThreads, 0, st>>>(src, amax, off);
+    RSG_LAUNCH_CHECK("amax_table_kernel");
+    pack16_table_kernel<<<g, kThreads, 0, st>>>(src, dst, amax, off);
+    RSG_LAUNCH_CHECK("pack16_table_kernel");
+    return RSG_OK;
+}
+
+int unpack16_table(const half_t *src, const unsigned *amax, float *dst, const size_t *off, int nseg,
+                   size_t maxlen, hipStream_t st) {
+    if (nseg < 1 || nseg > 65535) return fail(RSG_EINVAL, "unpack16: 1..65535 segments");
+    unpack16_table_kernel<<<seg_grid(maxlen, nseg), kThreads, 0, st>>>(src, amax, dst, off);
+    RSG_LAUNCH_CHECK("unpack16_table_kernel");
     return RSG_OK;
 }
 

@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib as R
-from .core import GMAP, SMAP, Data2D, ModelAcoustic2D, Pml, SMAX, _dev, _stream
+from .core import GMAP, SMAP, Data2D, ModelAcoustic2D, ModelElastic2D, Pml, SMAX, _dev, _stream
 
 DIFFERENCE = 0  # rs_fwimisfit (utils.h:27)
 _MODES = {"revolve": R.FWI_REVOLVE, "store": R.FWI_STORE, "full": R.FWI_FULL}
@@ -306,3 +306,225 @@ class AcousticFwiMisfit(torch.autograd.Function):
 
 def misfit(vp, rho, survey):
     return AcousticFwiMisfit.apply(vp, rho, survey)
+
+
+# --------------------------------------------------------------------------
+# Elastic 2D (FwiElastic2D, include/fwi.h:243-300, lib/fwi.cpp:2203-4143)
+# --------------------------------------------------------------------------
+
+E2D_COMPONENTS = ("P", "VX", "VZ")
+
+
+class ElasticShot:
+    """One shot of an elastic survey: source coordinates (nsrc, 2) and
+    wavelet (nsrc, nt) of field `srcfield` (P explosive, VX or VZ), receiver
+    coordinates (nrec, 2) and observed records dobs[3, nrec, nt] (P, Ux, Uz),
+    all in the coordinates of the full model; the shot runs on the window of
+    columns [x0, x0 + nxw) (the whole model when nxw is None)."""
+
+    def __init__(self, scoords, wavelet, gcoords, dobs, x0=0, nxw=None, srcfield="P"):
+        self.scoords = np.ascontiguousarray(scoords, np.float32).reshape(-1, 2)
+        self.wavelet = np.ascontiguousarray(wavelet, np.float32).reshape(self.scoords.shape[0], -1)
+        self.gcoords = np.ascontiguousarray(gcoords, np.float32).reshape(-1, 2)
+        self.dobs = np.ascontiguousarray(dobs, np.float32).reshape(3, self.gcoords.shape[0], -1)
+        self.x0, self.nxw, self.srcfield = int(x0), nxw, srcfield
+
+
+class ElasticSurvey:
+    """Shots of one elastic model plus the gradient settings (order, PML,
+    Revolve snapshots, FP16 checkpoints)."""
+
+    def __init__(self, shots, lpml, dx, dz, dt, order=8, fs=False, ncheck=50, fp16=False):
+        self.shots = list(shots)
+        self.lpml, self.dx, self.dz, self.dt = int(lpml), float(dx), float(dz), float(dt)
+        self.order, self.fs, self.ncheck, self.fp16 = int(order), bool(fs), int(ncheck), bool(fp16)
+
+
+def elastic_batch_gradient(vp, vs, rho, survey: ElasticSurvey, scale=True, wavgrad=False):
+    """Misfit and gradients of all shots of `survey` on the model (vp, vs,
+    rho [nz, nx]), every shot on its own window, as one ragged batch
+    (rsg_fwi_e2d_gradient_batch).  scale=True returns the Vp / Vs / rho
+    gradients of FwiElastic2D::run_optimal, scale=False the lambda / mu / rho
+    kernels before scaleGrad.  Returns (misfit, g0, g1, g2, per-shot dict of
+    dmod / dres / wavgrad device tensors)."""
+    dev = _dev()
+    vp, vs, rho = (np.ascontiguousarray(a, np.float32) for a in (vp, vs, rho))
+    nz, nx = vp.shape
+    ns = len(survey.shots)
+    keep = []
+    shots = (R.FwiE2DShot * ns)()
+    outs = []
+
+    def dev_t(a, dtype=torch.float32):
+        t = torch.from_numpy(np.ascontiguousarray(a)).to(dev, dtype)
+        keep.append(t)
+        return t
+
+    for k, sh in enumerate(survey.shots):
+        nxw = nx - sh.x0 if sh.nxw is None else int(sh.nxw)
+        if sh.x0 < 0 or sh.x0 + nxw > nx:
+            raise R.RsgError(f"elastic_batch_gradient: window of shot {k} outside the model")
+        m = ModelElastic2D(nxw, nz, survey.lpml, survey.dx, survey.dz, sh.x0 * survey.dx, 0.0, survey.fs)
+        m.Vp[:], m.Vs[:], m.R[:] = vp[:, sh.x0:sh.x0 + nxw], vs[:, sh.x0:sh.x0 + nxw], rho[:, sh.x0:sh.x0 + nxw]
+        m.staggerModels()
+        keep.append(m)
+        nt = sh.wavelet.shape[1]
+        src = Data2D(sh.scoords.shape[0], nt, survey.dt, 0.0, sh.srcfield)
+        src.scoords[:] = sh.scoords
+        src.data[:] = sh.wavelet
+        src.makeMap(m, SMAP)
+        t_src, k1 = src._device(SMAP)
+        keep.append(k1)
+        s = shots[k]
+        s.g = R.Grid(nx=m.getNx_pml(), ny=1, nz=m.getNz_pml(), lpml=m.lpml, order=survey.order, fs=int(m.fs),
+                     dx=m.dx, dy=1.0, dz=m.dz, dt=survey.dt)
+        s.L, s.L2M, s.M, s.Rx, s.Rz = (getattr(m, "_d" + n).data_ptr() for n in ("L", "L2M", "M", "Rx", "Rz"))
+        s.Vp, s.Vs, s.Rho = (dev_t(a).data_ptr() for a in (m.Vp, m.Vs, m.R))
+        s.ox = sh.x0
+        s.src = t_src
+        o = {"dmod": [], "dres": []}
+        for c, f in enumerate(E2D_COMPONENTS):
+            d = Data2D(sh.gcoords.shape[0], nt, survey.dt, 0.0, f)
+            d.gcoords[:] = sh.gcoords
+            d.data[:] = sh.dobs[c]
+            d.makeMap(m, GMAP)
+            t, kk = d._device(GMAP)
+            keep.append(kk)
+            s.obs[c] = t
+            dm = torch.zeros((d.ntrace, nt), dtype=torch.float32, device=dev)
+            dr = torch.zeros_like(dm)
+            o["dmod"].append(dm)
+            o["dres"].append(dr)
+            s.dmod[c], s.dres[c] = dm.data_ptr(), dr.data_ptr()
+        if wavgrad:
+            wg = torch.zeros((src.ntrace, nt), dtype=torch.float32, device=dev)
+            o["wavgrad"] = wg
+            s.wavgrad = wg.data_ptr()
+        outs.append(o)
+    pml = Pml(survey.lpml, survey.dt)
+    pml.setSmax(SMAX)
+    pml.computeABC()
+    grads = [torch.zeros((nz, nx), dtype=torch.float32, device=dev) for _ in range(3)]
+    mis = C.c_double(0.0)
+    f = R.FwiE2D()
+    f.tab, f.ncheck, f.fp16, f.scale, f.nx, f.nz = pml.tab.data_ptr(), survey.ncheck, int(survey.fp16), \
+        int(bool(scale)), nx, nz
+    for c in range(3):
+        f.grad[c] = grads[c].data_ptr()
+    f.misfit = C.pointer(mis)
+    R.check(R.lib().rsg_fwi_e2d_gradient_batch(C.byref(f), shots, ns, _stream()), "rsg_fwi_e2d_gradient_batch")
+    return mis.value, grads[0], grads[1], grads[2], outs
+
+
+class FwiElastic2D:
+    """FwiElastic2D(model, source, dataP, dataUx, dataUz, order, snapinc)
+    with the reference's setters and run_optimal() (Revolve checkpoints in
+    HBM); the misfit is DIFFERENCE over the P, Ux and Uz records."""
+
+    def __init__(self, model: ModelElastic2D, source: Data2D, dataP: Data2D, dataUx: Data2D, dataUz: Data2D,
+                 order=4, snapinc=1):
+        self.model, self.source = model, source
+        self.data = [dataP, dataUx, dataUz]
+        self.order = int(order) if 1 < int(order) < 9 else 4      # fwi.cpp:28-33
+        self.snapinc = max(int(snapinc), 1)
+        self.ncheck, self.incore, self.fp16 = 0, True, False
+        self.datamod = [None, None, None]
+        self.datares = [None, None, None]
+        self.vpgrad = self.vsgrad = self.rhograd = self.wavgrad = None
+        self.misfit = 0.0
+
+    def setDatamodP(self, d):
+        self.datamod[0] = d
+
+    def setDatamodUx(self, d):
+        self.datamod[1] = d
+
+    def setDatamodUz(self, d):
+        self.datamod[2] = d
+
+    def setDataresP(self, d):
+        self.datares[0] = d
+
+    def setDataresUx(self, d):
+        self.datares[1] = d
+
+    def setDataresUz(self, d):
+        self.datares[2] = d
+
+    def setVpgrad(self, img):
+        self.vpgrad = img
+
+    def setVsgrad(self, img):
+        self.vsgrad = img
+
+    def setRhograd(self, img):
+        self.rhograd = img
+
+    def setWavgrad(self, d):
+        self.wavgrad = d
+
+    def setNcheck(self, n):
+        self.ncheck = int(n)
+
+    def setIncore(self, v):
+        self.incore = bool(v)
+
+    def setFp16Storage(self, v):
+        self.fp16 = bool(v)
+
+    def setMisfit_type(self, t):
+        if int(t) != DIFFERENCE:
+            raise R.RsgError("only the DIFFERENCE misfit is implemented on this path")
+
+    def getMisfit(self):
+        return self.misfit
+
+    def run_optimal(self):
+        m = self.model
+        if any(d is None for d in self.data):
+            raise R.RsgError("FwiElastic2D: P, Ux and Uz data must be set")
+        shot = ElasticShot(self.source.scoords, self.source.data, self.data[0].gcoords,
+                           np.stack([d.data for d in self.data]), 0, m.nx, _field_name(self.source.field))
+        sv = ElasticSurvey([shot], m.lpml, m.dx, m.dz, self.source.dt, self.order, m.fs, self.ncheck, self.fp16)
+        mis, g0, g1, g2, outs = elastic_batch_gradient(m.Vp, m.Vs, m.R, sv, scale=True,
+                                                        wavgrad=self.wavgrad is not None)
+        self.misfit64 = mis
+        self.misfit = float(np.float32(mis))
+        for img, g in ((self.vpgrad, g0), (self.vsgrad, g1), (self.rhograd, g2)):
+            if img is not None:
+                img.imagedata[...] = g.cpu().numpy()
+        for c in range(3):
+            if self.datamod[c] is not None:
+                self.datamod[c].data[...] = outs[0]["dmod"][c].cpu().numpy()
+            if self.datares[c] is not None:
+                self.datares[c].data[...] = outs[0]["dres"][c].cpu().numpy()
+        if self.wavgrad is not None:
+            self.wavgrad.data[...] = outs[0]["wavgrad"].cpu().numpy()
+        return 1
+
+
+def _field_name(f):
+    return {R.PRESSURE: "P", R.VX: "VX", R.VZ: "VZ"}[f]
+
+
+class ElasticFwiMisfit(torch.autograd.Function):
+    """(Vp, Vs, rho) -> elastic DIFFERENCE misfit summed over the shots of an
+    ElasticSurvey (each on its window, one ragged batch); the backward pass
+    returns the reference's FwiElastic2D Vp / Vs / rho gradients."""
+
+    @staticmethod
+    def forward(ctx, vp: torch.Tensor, vs: torch.Tensor, rho: torch.Tensor, survey: ElasticSurvey):
+        a = [t.detach().to("cpu", torch.float32).numpy() for t in (vp, vs, rho)]
+        total, g0, g1, g2, _ = elastic_batch_gradient(*a, survey, scale=True)
+        ctx.save_for_backward(g0.to(vp.dtype), g1.to(vs.dtype), g2.to(rho.dtype))
+        return torch.tensor(total, dtype=torch.float64, device=vp.device)
+
+    @staticmethod
+    def backward(ctx, gout):
+        g0, g1, g2 = ctx.saved_tensors
+        s = gout.to(g0.dtype)
+        return s * g0, s * g1, s * g2, None
+
+
+def elastic_misfit(vp, vs, rho, survey):
+    return ElasticFwiMisfit.apply(vp, vs, rho, survey)

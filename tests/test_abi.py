@@ -148,3 +148,30 @@ def test_snap_cadence_matches_oracle():
                 assert (a.value, b.value) == snap_cadence(nt, dt, snapinc)
     a, b = C.c_int(), C.c_int()
     assert R.lib().rsg_snap_cadence(10, 1e-3, 0, C.byref(a), C.byref(b)) == -1  # RSG_EINVAL
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every struct of include/rsgpu.h has the size and member offsets that
+    the ctypes mirrors in rockseis_amd._lib assume (compiled with gcc)."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    structs = {"rsg_grid": R.Grid, "rsg_state": R.State, "rsg_traces": R.Traces, "rsg_batch": R.Batch,
+               "rsg_fwi_a2d": R.Fwi, "rsg_fwi_e2d_shot": R.FwiE2DShot, "rsg_fwi_e2d": R.FwiE2D}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rsgpu.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'    printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'    printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("    return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                              text=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(out[cname]) == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(out[f"{cname}.{fname}"]) == getattr(py, fname).offset, (cname, fname)
