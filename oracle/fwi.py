@@ -15,11 +15,13 @@ import numpy as np
 
 from . import fn, ref
 from . import revolve as orv
-from .runs import (A2DState, E2DState, e2d_record_factor, inject2d, makemap2d, record2d, stagger_a2d,
-                   stagger_e2d)
+from .runs import (A2DState, A3DState, E2DState, e2d_record_factor, inject2d, makemap2d, makemap3d,
+                   record2d, stagger_a2d, stagger_a3d, stagger_e2d)
 
 
 def _fields(s):
+    if isinstance(s, A3DState):
+        return [s.P, s.Vx, s.Vy, s.Vz] + list(s.strips.values())
     if isinstance(s, E2DState):
         return [s.Vx, s.Vz, s.Sxx, s.Szz, s.Sxz] + list(s.strips.values())
     return [s.P, s.Vx, s.Vz] + list(s.strips.values())
@@ -338,3 +340,101 @@ def ref_fwi_e2d(vp, vs, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, srcfield, 
                           f32(dobs), out["vpgrad"].reshape(-1), out["vsgrad"].reshape(-1),
                           out["rhograd"].reshape(-1), dmod.reshape(-1), dres.reshape(-1), wg.reshape(-1))
     return dict(misfit=np.float32(m), dmod=dmod, dres=dres, wavgrad=wg, **out)
+
+
+def fwi_a3d(vp, rho, lpml, d3, dt, nt, order, fs, ncheck, scoords, wav, gcoords, dobs, real=np.float32):
+    """FwiAcoustic3D::run_optimal (fwi.cpp:1991-2171): PRESSURE source and
+    data, DIFFERENCE misfit over traces with all three map indices valid
+    (fwi.cpp:1498-1510), residuals (fwi.cpp:1685-1695), adjoint with the
+    forward operator, crossCorr (fwi.cpp:1403-1466, quirks kept) and the
+    wavelet gradient recorded at the source from the adjoint pressure."""
+    nz, ny, nx = vp.shape
+    smap, sshift = makemap3d(scoords, (nx, ny, nz), d3, "P")
+    gmap, gshift = makemap3d(gcoords, (nx, ny, nz), d3, "P")
+    vp = np.ascontiguousarray(vp, real)
+    rho = np.ascontiguousarray(rho, real)
+    L, Rx, Ry, Rz = stagger_a3d(vp, rho, lpml, fs, real)
+    fw = A3DState(L, Rx, Ry, Rz, lpml, order, *d3, dt, real)
+    bw = A3DState(L, Rx, Ry, Rz, lpml, order, *d3, dt, real)
+    wav = np.ascontiguousarray(wav, real).reshape(smap.shape[0], nt)
+    dobs = np.ascontiguousarray(dobs, real).reshape(gmap.shape[0], nt)
+    dmod = np.zeros_like(dobs)
+    dres = np.zeros_like(dobs)
+    wgrad = np.zeros_like(wav)
+    vpgrad = np.zeros((nz, ny, nx), real)
+    rhograd = np.zeros((nz, ny, nx), real)
+    valid = ((gmap[:, 0] >= 0) & (gmap[:, 1] >= 0) & (gmap[:, 2] >= 0)).astype(np.int32)
+    sh32 = np.ascontiguousarray(sshift, real).reshape(-1)
+    gh32 = np.ascontiguousarray(gshift, real).reshape(-1)
+
+    def inject(s, m, sh, data, it):
+        fn("inject3d", real)(s.P.reshape(-1), s.L.reshape(-1), s.nx, s.ny, lpml, real(dt), m.reshape(-1), sh,
+                             data.reshape(-1), m.shape[0], nt, it)
+
+    def record(s, m, sh, data, it):
+        fn("record3d", real)(s.P.reshape(-1), s.nx, s.ny, lpml, m.reshape(-1), sh, data.reshape(-1), m.shape[0],
+                             nt, it)
+
+    xc = fn("a3d_crosscorr", real)
+
+    def corr():
+        xc(fw.P.reshape(-1), lpml, bw.Vx.reshape(-1), bw.Vy.reshape(-1), bw.Vz.reshape(-1), lpml,
+           vp.reshape(-1), rho.reshape(-1), fw.Rx.reshape(-1), fw.Ry.reshape(-1), fw.Rz.reshape(-1), nx, ny, nz,
+           real(d3[0]), real(d3[1]), real(d3[2]), vpgrad.reshape(-1), rhograd.reshape(-1))
+
+    def fstep(it, rec):
+        fw.velocity()
+        fw.stress()
+        inject(fw, smap, sh32, wav, it)
+        if rec:
+            record(fw, gmap, gh32, dmod, it)
+
+    misfit = None
+    checks = {}
+    rev = orv.Revolve(nt, ncheck)
+    reverse = False
+    while True:
+        oldcapo = rev.capo
+        a = rev.step()
+        capo = rev.capo
+        if a == orv.ADVANCE:
+            for it in range(oldcapo, capo):
+                fstep(it, not reverse)
+        elif a == orv.FIRSTURN:
+            fstep(capo, True)
+            misfit = fn("misfit_diff", real)(dmod.reshape(-1), dobs.reshape(-1), valid, dobs.shape[0], nt)
+            fn("residual_diff", real)(dmod.reshape(-1), dobs.reshape(-1), dres.reshape(-1), dobs.shape[0], nt, 1)
+            inject(bw, gmap, gh32, dres, capo)
+            corr()
+            record(bw, smap, sh32, wgrad, capo)
+            reverse = True
+        elif a == orv.YOUTURN:
+            bw.velocity()
+            bw.stress()
+            inject(bw, gmap, gh32, dres, capo)
+            corr()
+            record(bw, smap, sh32, wgrad, capo)
+        elif a == orv.TAKESHOT:
+            checks[rev.check] = _snapshot(fw)
+        elif a == orv.RESTORE:
+            _restore(fw, checks[rev.check])
+        if a in (orv.TERMINATE, orv.ERROR):
+            break
+    return dict(misfit=misfit, vpgrad=vpgrad, rhograd=rhograd, dmod=dmod, dres=dres, wavgrad=wgrad)
+
+
+def ref_fwi_a3d(vp, rho, lpml, d3, dt, nt, order, fs, ncheck, scoords, wav, gcoords, dobs):
+    nz, ny, nx = vp.shape
+    sc = np.ascontiguousarray(scoords, np.float32).reshape(-1, 3)
+    gc = np.ascontiguousarray(gcoords, np.float32).reshape(-1, 3)
+    nrec = gc.shape[0]
+    vg = np.zeros((nz, ny, nx), np.float32)
+    rg = np.zeros((nz, ny, nx), np.float32)
+    dmod = np.zeros((nrec, nt), np.float32)
+    dres = np.zeros((nrec, nt), np.float32)
+    wg = np.zeros((1, nt), np.float32)
+    f32 = lambda a: np.ascontiguousarray(a, np.float32).ravel()  # noqa: E731
+    m = ref().ref_fwi_a3d(f32(vp), f32(rho), nx, ny, nz, lpml, *d3, dt, nt, order, int(fs), ncheck, sc.ravel(),
+                          f32(wav), nrec, gc.ravel(), f32(dobs), vg.reshape(-1), rg.reshape(-1), dmod.reshape(-1),
+                          dres.reshape(-1), wg.reshape(-1))
+    return dict(misfit=np.float32(m), vpgrad=vg, rhograd=rg, dmod=dmod, dres=dres, wavgrad=wg)

@@ -426,6 +426,58 @@ float ref_fwi_e2d(const float *vp, const float *vs, const float *rho, int nx, in
     return fwi->getMisfit();
 }
 
+// FwiAcoustic3D::run_optimal with in-core Revolve (ncheck snapshots), a
+// PRESSURE source and PRESSURE data; coords are (x,y,z) triples.  Returns the
+// misfit; outputs the Vp and rho gradients, modelled data, residuals and the
+// wavelet gradient.
+float ref_fwi_a3d(const float *vp, const float *rho, int nx, int ny, int nz, int lpml, float dx, float dy,
+                  float dz, float dt, int nt, int order, int fs, int ncheck, const float *scoords,
+                  const float *wav, int nrec, const float *gcoords, const float *dobs, float *vpgrad,
+                  float *rhograd, float *dmod, float *dres, float *wgrad) {
+    auto model = std::make_shared<ModelAcoustic3D<float>>(nx, ny, nz, lpml, dx, dy, dz, 0.f, 0.f, 0.f,
+                                                          fs != 0);
+    model->createModel();
+    const size_t nm = (size_t)nx * ny * nz, nd = (size_t)nrec * nt;
+    std::memcpy(model->getVp(), vp, sizeof(float) * nm);
+    std::memcpy(model->getR(), rho, sizeof(float) * nm);
+    auto mk = [&](int ntr, const float *coords, bool src, const float *d) {
+        auto x = std::make_shared<Data3D<float>>(ntr, nt, dt, 0.f);
+        x->setField(PRESSURE);
+        Point3D<float> *c = src ? x->getGeom()->getScoords() : x->getGeom()->getGcoords();
+        for (int i = 0; i < ntr; i++) {
+            c[i].x = coords[3 * i];
+            c[i].y = coords[3 * i + 1];
+            c[i].z = coords[3 * i + 2];
+        }
+        if (d) std::memcpy(x->getData(), d, sizeof(float) * ntr * nt);
+        x->makeMap(model->getGeom(), src ? SMAP : GMAP);
+        return x;
+    };
+    auto src = mk(1, scoords, true, wav);
+    auto obs = mk(nrec, gcoords, false, dobs), mod = mk(nrec, gcoords, false, nullptr),
+         res = mk(nrec, gcoords, false, nullptr);
+    auto wg = mk(1, scoords, true, nullptr);
+    model->staggerModels();
+    auto fwi = std::make_shared<FwiAcoustic3D<float>>(model, src, obs, order, 1);
+    fwi->setDatamodP(mod);
+    fwi->setDataresP(res);
+    auto vg = std::make_shared<Image3D<float>>("vpgrad_unused", model, 1, 1, 1);
+    auto rg = std::make_shared<Image3D<float>>("rhograd_unused", model, 1, 1, 1);
+    fwi->setVpgrad(vg);
+    fwi->setRhograd(rg);
+    fwi->setWavgrad(wg);
+    fwi->setNcheck(ncheck);
+    fwi->setIncore(true);
+    fwi->setMisfit_type(DIFFERENCE);
+    fwi->run_optimal();
+    std::memcpy(vpgrad, vg->getImagedata(), sizeof(float) * nm);
+    std::memcpy(rhograd, rg->getImagedata(), sizeof(float) * nm);
+    std::memcpy(dmod, mod->getData(), sizeof(float) * nd);
+    std::memcpy(dres, res->getData(), sizeof(float) * nd);
+    std::memcpy(wgrad, wg->getData(), sizeof(float) * nt);
+    return fwi->getMisfit();
+}
+
 // Revolve action sequence: actions[k] (advance=0 takeshot=1 restore=2
 // firsturn=3 youturn=4 terminate=5 error=6) and capo after the call.
 // Returns the number of actions written, or -1 if maxn was too small.

@@ -731,6 +731,56 @@ void FN(orc_a2d_crosscorr)(const REAL *wsp, int pads, const REAL *wrx, const REA
 }
 
 /*
+ * FwiAcoustic3D::crossCorr (fwi.cpp:1403-1466), non-domdec form, with the
+ * reference's loop bounds ix, iy, iz in [1, n) (the upper edge included,
+ * unlike 2D) and its index of the second z term: wrz at (ix, iy-1, iz)
+ * against Rz at (ix, iy, iz-1) (fwi.cpp:1459).  wsp forward pressure (pad
+ * pads), wrx/wry/wrz adjoint velocities and Rx/Ry/Rz (pad padr); Vp/Rho and
+ * the images unpadded nx*ny*nz.
+ */
+void FN(orc_a3d_crosscorr)(const REAL *wsp, int pads, const REAL *wrx, const REAL *wry, const REAL *wrz,
+                           int padr, const REAL *Vp, const REAL *Rho, const REAL *Rx, const REAL *Ry,
+                           const REAL *Rz, int nx, int ny, int nz, REAL dx, REAL dy, REAL dz,
+                           REAL *vpgrad, REAL *rhograd)
+{
+    const long nxs = nx + 2 * pads, nys = ny + 2 * pads, nxr = nx + 2 * padr, nyr = ny + 2 * padr;
+#define KS(i, j, k) ((long)(k) * nxs * nys + (long)(j) * nxs + (i))
+#define KR(i, j, k) ((long)(k) * nxr * nyr + (long)(j) * nxr + (i))
+#define KM(i, j, k) ((long)(k) * nx * ny + (long)(j) * nx + (i))
+    for (int ix = 1; ix < nx; ix++)
+        for (int iy = 1; iy < ny; iy++)
+            for (int iz = 1; iz < nz; iz++) {
+                const int xs = ix + pads, ys = iy + pads, zs = iz + pads;
+                const int xr = ix + padr, yr = iy + padr, zr = iz + padr;
+                REAL vpscale = (REAL)(-2.0 / (double)Vp[KM(ix, iy, iz)]);
+                REAL rhoscale1 = (REAL)(-1.0 / (double)Rho[KM(ix, iy, iz)]);
+                REAL mrxx = (wrx[KR(xr, yr, zr)] - wrx[KR(xr - 1, yr, zr)]) / dx;
+                REAL mryy = (wry[KR(xr, yr, zr)] - wry[KR(xr, yr - 1, zr)]) / dy;
+                REAL mrzz = (wrz[KR(xr, yr, zr)] - wrz[KR(xr, yr, zr - 1)]) / dz;
+                REAL ws = wsp[KS(xs, ys, zs)];
+                vpgrad[KM(ix, iy, iz)] -= vpscale * ws * (mrxx + mryy + mrzz);
+                rhograd[KM(ix, iy, iz)] -= rhoscale1 * ws * (mrxx + mryy + mrzz);
+                REAL uderx, udery, uderz;
+                uderx = (REAL)(0.5 * (double)wrx[KR(xr, yr, zr)] * (double)Rx[KR(xr, yr, zr)] *
+                               (double)(wsp[KS(xs + 1, ys, zs)] - ws) / (double)dx);
+                uderx = (REAL)((double)uderx + 0.5 * (double)wrx[KR(xr - 1, yr, zr)] * (double)Rx[KR(xr - 1, yr, zr)] *
+                               (double)(ws - wsp[KS(xs - 1, ys, zs)]) / (double)dx);
+                udery = (REAL)(0.5 * (double)wry[KR(xr, yr, zr)] * (double)Ry[KR(xr, yr, zr)] *
+                               (double)(wsp[KS(xs, ys + 1, zs)] - ws) / (double)dy);
+                udery = (REAL)((double)udery + 0.5 * (double)wry[KR(xr, yr - 1, zr)] * (double)Ry[KR(xr, yr - 1, zr)] *
+                               (double)(ws - wsp[KS(xs, ys - 1, zs)]) / (double)dy);
+                uderz = (REAL)(0.5 * (double)wrz[KR(xr, yr, zr)] * (double)Rz[KR(xr, yr, zr)] *
+                               (double)(wsp[KS(xs, ys, zs + 1)] - ws) / (double)dz);
+                uderz = (REAL)((double)uderz + 0.5 * (double)wrz[KR(xr, yr - 1, zr)] * (double)Rz[KR(xr, yr, zr - 1)] *
+                               (double)(ws - wsp[KS(xs, ys, zs - 1)]) / (double)dz);
+                rhograd[KM(ix, iy, iz)] += (uderx + udery + uderz);
+            }
+#undef KS
+#undef KR
+#undef KM
+}
+
+/*
  * FwiElastic2D::crossCorr(waves_fw, waves_bw, model, it) (fwi.cpp:2591-2775),
  * non-domdec form: interior ix,iz in [1,n-1) of the unpadded nx*nz images.
  * Forward fields ws* and adjoint fields wr*, rs* on padded grids (both pad

@@ -225,6 +225,31 @@ def fwi_e2d_fixtures(R):
              wavgrad=r["wavgrad"], oracle_lam=o["lam"], oracle_mu=o["mu"], oracle_rho=o["rhok"])
 
 
+FWI_A3D_CASES = [  # order, fs, ncheck (FwiAcoustic3D::run_optimal)
+    (4, 0, 10), (8, 1, 5),
+]
+
+
+def fwi_a3d_fixtures(R):
+    from scipy.ndimage import gaussian_filter
+    from oracle import fwi
+    nx, ny, nz, lpml, dx, dt, nt = 26, 24, 22, 6, 10.0, 1e-3, 120
+    vt = synth.layered_anomaly_vp3d(nx, ny, nz, dx, dx, dx, vels=(1800., 2200., 2600., 3000.), nanom=4, seed=2)
+    rt = synth.gardner(vt)
+    v0 = gaussian_filter(vt.astype(np.float64), 2.0, mode="nearest").astype(np.float32)
+    r0 = synth.gardner(v0)
+    sc = np.float32([[130., 120., 20.]])
+    wav = synth.ricker(nt, dt, 25.0)[None]
+    gx, gy = np.meshgrid(np.arange(2, nx - 2, 3) * dx, np.arange(2, ny - 2, 3) * dx)
+    gc = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, 20.0)], 1).astype(np.float32)
+    for k, (order, fs, ncheck) in enumerate(FWI_A3D_CASES):
+        obs = runs.ref_a3d_run(vt, rt, lpml, (dx, dx, dx), dt, nt, order, fs, "P", sc, wav, "P", gc)["rec"]
+        r = fwi.ref_fwi_a3d(v0, r0, lpml, (dx, dx, dx), dt, nt, order, fs, ncheck, sc, wav, gc, obs)
+        save(f"fwi_a3d_case{k}", vp=v0, rho=r0, scoords=sc, wav=wav, gcoords=gc, dobs=obs,
+             grid=np.float32([nx, ny, nz, lpml, dx, dt, nt]), misfit=np.float32(r["misfit"]),
+             vpgrad=r["vpgrad"], rhograd=r["rhograd"], dmod=r["dmod"], dres=r["dres"], wavgrad=r["wavgrad"])
+
+
 REVOLVE_CASES = [(10, 3), (100, 5), (7, 2), (50, 60), (1, 2), (257, 4), (3000, 50), (2500, 50), (4000, 50)]
 
 
@@ -244,7 +269,7 @@ def revolve_fixture(R):
 
 GROUPS = dict(der=der_fixture, pml=pml_fixture, stagger=stagger_fixture, makemap=makemap_fixture,
               runs=run_fixtures, revolve=revolve_fixture, fwi=fwi_fixtures, fwi_full=fwi_full_fixtures,
-              fwi_e2d=fwi_e2d_fixtures)
+              fwi_e2d=fwi_e2d_fixtures, fwi_a3d=fwi_a3d_fixtures)
 
 if __name__ == "__main__":
     # python -m tests.golden.make_golden [group ...]   (default: every group)

@@ -155,3 +155,19 @@ def test_fwi_e2d_gradient(golden, case):
     assert np.float32(r["misfit"]) == g["misfit"]
     for k, o in (("lam", "oracle_lam"), ("mu", "oracle_mu"), ("rhok", "oracle_rho")):
         assert np.array_equal(r[k], g[o]), k
+
+
+@pytest.mark.parametrize("case", range(2))
+def test_fwi_a3d_gradient(golden, case):
+    """FwiAcoustic3D::run_optimal restated (oracle/fwi.py, crossCorr with the
+    reference's 3D loop bounds and wrz index) vs the compiled reference."""
+    from oracle import fwi
+    from tests.golden.make_golden import FWI_A3D_CASES
+    g = golden(f"fwi_a3d_case{case}")
+    order, fs, ncheck = FWI_A3D_CASES[case]
+    nx, ny, nz, lpml, dx, dt, nt = g["grid"]
+    r = fwi.fwi_a3d(g["vp"], g["rho"], int(lpml), (dx, dx, dx), dt, int(nt), order, fs, ncheck, g["scoords"],
+                    g["wav"], g["gcoords"], g["dobs"])
+    for k in ("vpgrad", "rhograd", "dmod", "dres", "wavgrad"):
+        assert np.array_equal(r[k], g[k]), k
+    assert np.float32(r["misfit"]) == g["misfit"]
