@@ -238,6 +238,26 @@ int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, void *stream);
  * src/obs (map, shift, data), dmod/dres/wavgrad are [nshot][ntrace][nt],
  * vpgrad/rhograd and the misfit are summed over the shots. */
 int rsg_fwi_a2d_gradient_batch(const rsg_grid *g, const rsg_fwi_a2d *f, int nshot, void *stream);
+/* ---- Acoustic 3D FWI gradient of one shot (FwiAcoustic3D::run_optimal,
+ * fwi.cpp:1991-2171): Revolve checkpoints in HBM (FP32, or FP16 with one
+ * power-of-two scale per array), adjoint with the forward operator,
+ * crossCorr (fwi.cpp:1403-1466) with the reference's 3D loop bounds. */
+typedef struct {
+    const float *L, *Rx, *Ry, *Rz, *tab; /* staggered model and PML tables (device)  */
+    const float *Vp, *Rho;               /* unpadded model nz*ny*nx (device)         */
+    int nx, ny, nz;                      /* unpadded sizes                           */
+    rsg_traces src;                      /* PRESSURE source, wavelet in data         */
+    rsg_traces obs;                      /* observed PRESSURE data                   */
+    float *dmod, *dres;                  /* out: modelled data, residuals            */
+    float *wavgrad;                      /* out (optional): wavelet gradient         */
+    float *vpgrad, *rhograd;             /* in/out: accumulated gradients            */
+    double *misfit;                      /* out (host)                               */
+    int ncheck;                          /* Revolve snapshots                        */
+    int fp16;                            /* FP16 checkpoints                         */
+} rsg_fwi_a3d;
+
+int rsg_fwi_a3d_gradient(const rsg_grid *g, const rsg_fwi_a3d *f, void *stream);
+
 /* ---- Elastic 2D FWI gradient (FwiElastic2D::run_optimal, fwi.cpp:3937-4143)
  * for a ragged batch of shots, each on its own aperture window.  Per shot:
  * forward elastic steps with the source and P / Ux / Uz gathers, DIFFERENCE

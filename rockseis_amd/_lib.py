@@ -54,6 +54,13 @@ class Fwi(C.Structure):
                    ("mode_used", C.POINTER(C.c_int)), ("snapinc", C.c_int), ("fp16", C.c_int)])
 
 
+class FwiA3D(C.Structure):
+    _fields_ = ([(n, C.c_void_p) for n in ("L", "Rx", "Ry", "Rz", "tab", "Vp", "Rho")]
+                + [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("src", Traces), ("obs", Traces)]
+                + [(n, C.c_void_p) for n in ("dmod", "dres", "wavgrad", "vpgrad", "rhograd")]
+                + [("misfit", C.POINTER(C.c_double)), ("ncheck", C.c_int), ("fp16", C.c_int)])
+
+
 class FwiE2DShot(C.Structure):
     _fields_ = ([("g", Grid)] + [(n, C.c_void_p) for n in ("L", "L2M", "M", "Rx", "Rz", "Vp", "Vs", "Rho")]
                 + [("ox", C.c_int), ("src", Traces), ("obs", Traces * 3), ("dmod", C.c_void_p * 3),
@@ -126,6 +133,7 @@ def _declare(L):
         "rsg_ragged2d_step": ([vp, ip, vp], ip),
         "rsg_ragged2d_destroy": ([vp], ip),
         "rsg_fwi_e2d_gradient_batch": ([C.POINTER(FwiE2D), C.POINTER(FwiE2DShot), ip, vp], ip),
+        "rsg_fwi_a3d_gradient": ([GP, C.POINTER(FwiA3D), vp], ip),
     }
     for name in ("a2d_forward_velocity", "a2d_forward_stress", "e2d_forward_velocity",
                  "e2d_forward_stress", "e2d_backward_velocity", "e2d_backward_stress",
@@ -150,6 +158,7 @@ EXPORTS = (
     "rsg_revolve_numforw", "rsg_snap_cadence", "rsg_fwi_a2d_gradient_batch", "rsg_a2d_forward_velocity_batch",
     "rsg_a2d_forward_stress_batch", "rsg_e2d_velocity_batch", "rsg_e2d_stress_batch",
     "rsg_ragged2d_create", "rsg_ragged2d_step", "rsg_ragged2d_destroy", "rsg_fwi_e2d_gradient_batch",
+    "rsg_fwi_a3d_gradient",
 )
 
 

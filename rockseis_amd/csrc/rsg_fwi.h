@@ -1,5 +1,7 @@
 // Internal declarations of the FWI driver.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <vector>
 
 #include "../../include/rsgpu.h"
@@ -19,5 +21,11 @@ struct Revolve {
     static int adjust(int steps);
     static int numforw(int steps, int snaps);
 };
+
+// DIFFERENCE misfit (double, into *misfit) and residuals of ntr traces;
+// map holds dim indices per trace (fwi.hip).
+__global__ void misfit_residual_kernel(const float *__restrict__ mod, const float *__restrict__ rec,
+                                       float *__restrict__ res, const int *__restrict__ map, int dim,
+                                       int ntr, int nt, int scaling, double *misfit);
 
 }  // namespace rsg

@@ -22,7 +22,8 @@ import numpy as np
 import torch
 
 from . import _lib as R
-from .core import GMAP, SMAP, Data2D, ModelAcoustic2D, ModelElastic2D, Pml, SMAX, _dev, _stream
+from .core import (GMAP, SMAP, Data2D, Data3D, ModelAcoustic2D, ModelAcoustic3D, ModelElastic2D, Pml, SMAX, _dev,
+                   _stream)
 
 DIFFERENCE = 0  # rs_fwimisfit (utils.h:27)
 _MODES = {"revolve": R.FWI_REVOLVE, "store": R.FWI_STORE, "full": R.FWI_FULL}
@@ -34,6 +35,17 @@ class Image2D:
     def __init__(self, model):
         self.nx, self.nz = model.nx, model.nz
         self.imagedata = np.zeros((self.nz, self.nx), np.float32)
+
+    def getImagedata(self):
+        return self.imagedata
+
+
+class Image3D:
+    """Gradient image on the unpadded 3D model grid ([nz, ny, nx] float32)."""
+
+    def __init__(self, model):
+        self.nx, self.ny, self.nz = model.nx, model.ny, model.nz
+        self.imagedata = np.zeros((self.nz, self.ny, self.nx), np.float32)
 
     def getImagedata(self):
         return self.imagedata
@@ -528,3 +540,99 @@ class ElasticFwiMisfit(torch.autograd.Function):
 
 def elastic_misfit(vp, vs, rho, survey):
     return ElasticFwiMisfit.apply(vp, vs, rho, survey)
+
+
+# --------------------------------------------------------------------------
+# Acoustic 3D (FwiAcoustic3D, include/fwi.h:186-240, lib/fwi.cpp:1355-2171)
+# --------------------------------------------------------------------------
+
+class FwiAcoustic3D:
+    """FwiAcoustic3D(model, source, dataP, order, snapinc) with the
+    reference's setters and run_optimal() (Revolve checkpoints in HBM, FP32
+    or, with setFp16Storage(True), FP16 with one scale per array)."""
+
+    def __init__(self, model: ModelAcoustic3D, source: Data3D, dataP: Data3D, order=4, snapinc=1):
+        self.model, self.source, self.dataP = model, source, dataP
+        self.order = int(order) if 1 < int(order) < 9 else 4      # fwi.cpp:28-33
+        self.snapinc = max(int(snapinc), 1)
+        self.ncheck, self.incore, self.fp16 = 0, True, False
+        self.datamodP = self.dataresP = self.vpgrad = self.rhograd = self.wavgrad = None
+        self.misfit = 0.0
+
+    def setDatamodP(self, d):
+        self.datamodP = d
+
+    def setDataresP(self, d):
+        self.dataresP = d
+
+    def setVpgrad(self, img):
+        self.vpgrad = img
+
+    def setRhograd(self, img):
+        self.rhograd = img
+
+    def setWavgrad(self, d):
+        self.wavgrad = d
+
+    def setNcheck(self, n):
+        self.ncheck = int(n)
+
+    def setIncore(self, v):
+        self.incore = bool(v)
+
+    def setFp16Storage(self, v):
+        self.fp16 = bool(v)
+
+    def setMisfit_type(self, t):
+        if int(t) != DIFFERENCE:
+            raise R.RsgError("only the DIFFERENCE misfit is implemented on this path")
+
+    def getMisfit(self):
+        return self.misfit
+
+    def run_optimal(self):
+        dev = _dev()
+        m, src, obs = self.model, self.source, self.dataP
+        if obs.field != R.PRESSURE or src.field != R.PRESSURE:
+            raise R.RsgError("FwiAcoustic3D on this path takes PRESSURE sources and data")
+        if self.vpgrad is None or self.rhograd is None:
+            raise R.RsgError("FwiAcoustic3D:crossCorr: No gradient set in fwi class")
+        if not hasattr(m, "_dL"):
+            m.staggerModels()
+        pml = Pml(m.lpml, src.dt)
+        pml.setSmax(SMAX)
+        pml.computeABC()
+        g = R.Grid(nx=m.getNx_pml(), ny=m.getNy_pml(), nz=m.getNz_pml(), lpml=m.lpml, order=self.order,
+                   fs=int(m.fs), dx=m.dx, dy=m.dy, dz=m.dz, dt=src.dt)
+        vp = torch.from_numpy(np.ascontiguousarray(m.Vp, np.float32)).to(dev)
+        rho = torch.from_numpy(np.ascontiguousarray(m.R, np.float32)).to(dev)
+        t_src, k1 = src._device(SMAP)
+        t_obs, k2 = obs._device(GMAP)
+        ntr, nt = obs.ntrace, obs.nt
+        dmod = torch.zeros((ntr, nt), dtype=torch.float32, device=dev)
+        dres = torch.zeros_like(dmod)
+        wg = torch.zeros((src.ntrace, src.nt), dtype=torch.float32, device=dev)
+        vg = torch.from_numpy(self.vpgrad.imagedata).to(dev)
+        rg = torch.from_numpy(self.rhograd.imagedata).to(dev)
+        misfit = C.c_double(0.0)
+        f = R.FwiA3D()
+        f.L, f.Rx, f.Ry, f.Rz = (getattr(m, "_d" + n).data_ptr() for n in ("L", "Rx", "Ry", "Rz"))
+        f.tab, f.Vp, f.Rho = pml.tab.data_ptr(), vp.data_ptr(), rho.data_ptr()
+        f.nx, f.ny, f.nz = m.nx, m.ny, m.nz
+        f.src, f.obs = t_src, t_obs
+        f.dmod, f.dres, f.wavgrad = dmod.data_ptr(), dres.data_ptr(), wg.data_ptr()
+        f.vpgrad, f.rhograd = vg.data_ptr(), rg.data_ptr()
+        f.misfit = C.pointer(misfit)
+        f.ncheck, f.fp16 = self.ncheck, int(self.fp16)
+        R.check(R.lib().rsg_fwi_a3d_gradient(C.byref(g), C.byref(f), _stream()), "rsg_fwi_a3d_gradient")
+        self.misfit64 = misfit.value
+        self.misfit = float(np.float32(misfit.value))
+        self.vpgrad.imagedata[...] = vg.cpu().numpy()
+        self.rhograd.imagedata[...] = rg.cpu().numpy()
+        if self.datamodP is not None:
+            self.datamodP.data[...] = dmod.cpu().numpy()
+        if self.dataresP is not None:
+            self.dataresP.data[...] = dres.cpu().numpy()
+        if self.wavgrad is not None:
+            self.wavgrad.data[...] = wg.cpu().numpy()
+        return 1

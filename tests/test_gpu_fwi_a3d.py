@@ -1,0 +1,68 @@
+"""Acoustic 3D FWI gradient on the HIP path (rsg_fwi_a3d_gradient through
+rockseis_amd.fwi.FwiAcoustic3D) against the compiled reference's
+FwiAcoustic3D::run_optimal (golden fixtures) within relative L2 1e-5, and
+FP16 checkpoints against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from tests.golden.make_golden import FWI_A3D_CASES
+from tests.gpu_util import RTOL_L2, data3d, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _driver(g, case, fp16=False):
+    from rockseis_amd import core, fwi
+    order, fs, ncheck = FWI_A3D_CASES[case]
+    nx, ny, nz, lpml, dx, dt, nt = g["grid"]
+    nx, ny, nz, lpml, nt = int(nx), int(ny), int(nz), int(lpml), int(nt)
+    m = core.ModelAcoustic3D(nx, ny, nz, lpml, dx, dx, dx, 0.0, 0.0, 0.0, fs)
+    m.Vp[:], m.R[:] = g["vp"], g["rho"]
+    src = data3d(g["scoords"], nt, dt, "P", g["wav"])
+    src.makeMap(m, core.SMAP)
+    obs = data3d(g["gcoords"], nt, dt, "P", g["dobs"])
+    obs.makeMap(m, core.GMAP)
+    mod, res = data3d(g["gcoords"], nt, dt, "P"), data3d(g["gcoords"], nt, dt, "P")
+    wgr = data3d(g["scoords"], nt, dt, "P")
+    m.staggerModels()
+    f = fwi.FwiAcoustic3D(m, src, obs, order, 1)
+    vg, rg = fwi.Image3D(m), fwi.Image3D(m)
+    f.setVpgrad(vg)
+    f.setRhograd(rg)
+    f.setDatamodP(mod)
+    f.setDataresP(res)
+    f.setWavgrad(wgr)
+    f.setNcheck(ncheck)
+    f.setFp16Storage(fp16)
+    f.run_optimal()
+    return dict(misfit64=f.misfit64, vpgrad=vg.imagedata, rhograd=rg.imagedata, dmod=mod.data, dres=res.data,
+                wavgrad=wgr.data)
+
+
+@pytest.mark.parametrize("case", range(len(FWI_A3D_CASES)))
+def test_fwi_a3d_vs_reference(golden, case):
+    g = golden(f"fwi_a3d_case{case}")
+    print(f"fwi a3d case {case}: {FWI_A3D_CASES[case]}", flush=True)
+    r = _driver(g, case)
+    for k in ("vpgrad", "rhograd", "dmod", "dres", "wavgrad"):
+        e = rel_l2(r[k], g[k])
+        print(f"  {k}: rel L2 {e:.2e}")
+        assert e <= RTOL_L2, k
+    exact = 0.5 * np.sum((g["dmod"].astype(np.float64) - g["dobs"].astype(np.float64)) ** 2)
+    assert abs(r["misfit64"] - exact) <= RTOL_L2 * exact
+
+
+def test_fp16_checkpoints_error_vs_fp64_oracle(golden):
+    """Gradient error against the FP64 oracle with FP32 and FP16 Revolve
+    checkpoints (order 4, 26x24x22, nt 120, 10 snapshots)."""
+    from oracle import fwi as ofwi
+    g = golden("fwi_a3d_case0")
+    order, fs, ncheck = FWI_A3D_CASES[0]
+    nx, ny, nz, lpml, dx, dt, nt = g["grid"]
+    o64 = ofwi.fwi_a3d(g["vp"], g["rho"], int(lpml), (dx, dx, dx), dt, int(nt), order, fs, ncheck, g["scoords"],
+                       g["wav"], g["gcoords"], g["dobs"], real=np.float64)
+    for fp16, bound in ((False, 1e-5), (True, 1e-3)):
+        r = _driver(g, 0, fp16=fp16)
+        ev, er = rel_l2(r["vpgrad"], o64["vpgrad"]), rel_l2(r["rhograd"], o64["rhograd"])
+        print(f"  fp16={fp16}: vpgrad {ev:.2e} rhograd {er:.2e} (bound {bound:.0e})", flush=True)
+        assert max(ev, er) <= bound, fp16
