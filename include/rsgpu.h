@@ -196,6 +196,12 @@ int rsg_insert_source(int kind, const rsg_grid *g, const rsg_state *s, const rsg
 int rsg_record_data(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *rec,
                     int it, void *stream);
 
+/* Interp<T>::interp (interp.cpp:110-186) on device records: from[ntr][n1]
+ * sampled at o1 + i*d1 -> to[ntr][n2] at o2 + j*d2.  mode numbered like
+ * rs_interpmode (utils.h:25): 0 LINEAR, 1 BSPLINE, 2 SINC. */
+int rsg_interp(int mode, const float *from, int ntr, int n1, float d1, float o1, float *to, int n2, float d2,
+               float o2, void *stream);
+
 /* ---- whole modelling run (Modelling*::run time loop, modelling.cpp:242-286)
  * Steps it0..it0+nsteps-1: velocity, stress, insertSource(it), recordData(it)
  * for each receiver set.  graph_chunk > 0 captures that many steps in one

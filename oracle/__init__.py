@@ -139,6 +139,8 @@ def _declare(L):
         f.restype = C.c_int
         f = getattr(L, "orc_a2d_crosscorr" + suf)
         f.argtypes = [rp, C.c_int, rp, rp, C.c_int, rp, rp, rp, rp, C.c_int, C.c_int, ct, ct, rp, rp]
+        f = getattr(L, "orc_interp" + suf)
+        f.argtypes = [C.c_int, rp, C.c_int, C.c_int, ct, ct, rp, C.c_int, ct, ct]
         f = getattr(L, "orc_a3d_crosscorr" + suf)
         f.argtypes = [rp, C.c_int, rp, rp, rp, C.c_int] + [rp] * 5 + [C.c_int] * 3 + [ct] * 3 + [rp, rp]
         f = getattr(L, "orc_e2d_crosscorr" + suf)
@@ -194,6 +196,8 @@ def _declare_ref(R):
     R.ref_fwi_a3d.argtypes = ([_f32p, _f32p] + [C.c_int] * 4 + [C.c_float] * 4 + [C.c_int] * 4
                               + [_f32p, _f32p, C.c_int, _f32p, _f32p] + [_f32p] * 5)
     R.ref_fwi_a3d.restype = C.c_float
+    R.ref_interp.argtypes = [C.c_int, _f32p, C.c_int, C.c_int, C.c_float, C.c_float, _f32p, C.c_int, C.c_float,
+                             C.c_float]
     R.ref_revolve.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, C.c_int]
     R.ref_revolve.restype = C.c_int
     R.ref_revolve_numforw.argtypes = [C.c_int, C.c_int]

@@ -20,6 +20,7 @@
 #include "revolve.h"
 #include "fwi.h"
 #include "image.h"
+#include "interp.h"
 
 using namespace rockseis;
 
@@ -476,6 +477,18 @@ float ref_fwi_a3d(const float *vp, const float *rho, int nx, int ny, int nz, int
     std::memcpy(dres, res->getData(), sizeof(float) * nd);
     std::memcpy(wgrad, wg->getData(), sizeof(float) * nt);
     return fwi->getMisfit();
+}
+
+// Interp<float>(mode)::interp between two Data2D of ntr traces (mode 0
+// LINEAR, 1 BSPLINE, 2 SINC).
+void ref_interp(int mode, const float *from, int ntr, int n1, float d1, float o1, float *to, int n2, float d2,
+                float o2) {
+    auto a = std::make_shared<Data2D<float>>(ntr, n1, d1, o1);
+    auto b = std::make_shared<Data2D<float>>(ntr, n2, d2, o2);
+    std::memcpy(a->getData(), from, sizeof(float) * ntr * n1);
+    Interp<float> ip(mode == 0 ? LINEAR : (mode == 1 ? BSPLINE : SINC));
+    ip.interp(a, b);
+    std::memcpy(to, b->getData(), sizeof(float) * ntr * n2);
 }
 
 // Revolve action sequence: actions[k] (advance=0 takeshot=1 restore=2

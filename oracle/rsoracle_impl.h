@@ -900,6 +900,57 @@ REAL FN(orc_misfit_diff3)(const REAL *modp, const REAL *recp, const REAL *modx, 
     return misfit;
 }
 
+/*
+ * Interp::interp (interp.cpp:110-186) for trace-major data: from[ntr][n1]
+ * sampled at o1 + i*d1 -> to[ntr][n2] at o2 + j*d2; mode 0 LINEAR (2 taps,
+ * interp.cpp:69-76), 1 BSPLINE (4 taps, :79-97), 2 SINC (6 taps, Lanczos-3
+ * window, :99-125).  Output samples whose taps leave [0, n1) are zero.  The
+ * reference's float / double promotions are kept (T = REAL).
+ */
+static REAL FN(orc_sinc)(REAL x)
+{
+    if (x == 0.0) return (REAL)1.0;
+    return (REAL)(sin(3.14159265358979323846 * (double)x) / (3.14159265358979323846 * (double)x));
+}
+
+void FN(orc_interp)(int mode, const REAL *from, int ntr, int n1, REAL d1, REAL o1, REAL *to, int n2, REAL d2,
+                    REAL o2)
+{
+    const int pad = mode == 0 ? 1 : (mode == 1 ? 2 : 3);
+    for (int i = 0; i < ntr; i++)
+        for (long j = 0; j < n2; j++) {
+            REAL val = o2 + (REAL)j * d2;
+            long i0 = (long)((val - o1) / d1) - (pad - 1);
+            REAL *out = to + (long)i * n2 + j;
+            if (i0 < 0 || (i0 + (2 * pad) - 1) > (n1 - 1)) {
+                *out = (REAL)0.0;
+                continue;
+            }
+            REAL x0 = (o1 + (REAL)(i0 + (pad - 1)) * d1);
+            REAL d = (val - x0) / d1;
+            const REAL *m = from + (long)i * n1 + i0;
+            if (mode == 0) {
+                *out = (REAL)((1.0 - (double)d) * (double)m[0] + (double)(d * m[1]));
+            } else if (mode == 1) {
+                REAL xp1 = (REAL)((double)d + 1.0);
+                REAL omx = (REAL)(1.0 - (double)d);
+                REAL a0 = (REAL)(((-(1.0 / 6.0) * (double)xp1 + 1.0) * (double)xp1 - 2.0) * (double)xp1 + (4.0 / 3.0));
+                REAL a1 = (REAL)((0.5 * (double)d - 1.0) * (double)d * (double)d + (2.0 / 3.0));
+                REAL a2 = (REAL)((0.5 * (double)omx - 1.0) * (double)omx * (double)omx + (2.0 / 3.0));
+                REAL a3 = (REAL)(1.0 - (double)a0 - (double)a1 - (double)a2);
+                *out = a0 * m[0] + a1 * m[1] + a2 * m[2] + a3 * m[3];
+            } else {
+                REAL acc = (REAL)0.0;
+                for (int k = 0; k < 6; k++) {
+                    REAL s1 = FN(orc_sinc)((REAL)((2.0 - k) + (double)d));
+                    REAL s2 = FN(orc_sinc)((REAL)(((2.0 - k) + (double)d) / 3.0));
+                    acc = acc + m[k] * s1 * s2;
+                }
+                *out = acc;
+            }
+        }
+}
+
 /* DIFFERENCE misfit 0.5*sum(r^2) over traces with valid maps, accumulated
  * in T (fwi.cpp:574-608); `valid[i]` nonzero when map x,z >= 0. */
 REAL FN(orc_misfit_diff)(const REAL *mod, const REAL *rec, const int *valid, int ntr, int nt)

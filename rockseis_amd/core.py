@@ -489,6 +489,36 @@ class WavesAcoustic3D(_Waves):
         return self.nz_pml
 
 
+LINEAR, BSPLINE, SINC = 0, 1, 2  # rs_interpmode (utils.h:25)
+
+
+class Interp:
+    """Interp(mode).interp(from, to): resample the traces of a Data2D/Data3D
+    onto the time axis of another (interp.cpp:110-186) with rsg_interp on
+    the device; coordinates are copied as the reference does."""
+
+    def __init__(self, mode=LINEAR):
+        self.mode = int(mode)
+
+    def interp(self, frm, to):
+        if frm.ntrace != to.ntrace:
+            raise R.RsgError("Interp::interp: Number of traces in the two datasets mismatch.")
+        to.scoords[:] = frm.scoords
+        to.gcoords[:] = frm.gcoords
+        out = self.interp_device(torch.from_numpy(np.ascontiguousarray(frm.data)).to(_dev()), frm.dt, frm.ot,
+                                 to.nt, to.dt, to.ot)
+        to.data[...] = out.cpu().numpy()
+
+    def interp_device(self, data, d1, o1, n2, d2, o2):
+        """Resample device traces data[ntr, n1] (float32); returns [ntr, n2]."""
+        data = data.contiguous()
+        ntr, n1 = data.shape
+        out = torch.empty((ntr, int(n2)), dtype=torch.float32, device=data.device)
+        R.check(R.lib().rsg_interp(self.mode, data.data_ptr(), ntr, n1, float(d1), float(o1), out.data_ptr(),
+                                   int(n2), float(d2), float(o2), _stream()), "Interp::interp")
+        return out
+
+
 class RaggedBatch2D:
     """Shots on 2D grids of different sizes (aperture windows) advanced by
     one launch per half step (rsg_ragged2d_*).  `waves` and `models` are the

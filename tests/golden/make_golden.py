@@ -250,6 +250,25 @@ def fwi_a3d_fixtures(R):
              vpgrad=r["vpgrad"], rhograd=r["rhograd"], dmod=r["dmod"], dres=r["dres"], wavgrad=r["wavgrad"])
 
 
+INTERP_CASES = [  # ntr, n1, d1, o1, n2, d2, o2 (Interp::interp for every mode)
+    (5, 300, 1e-3, 0.0, 700, 4e-4, 0.0), (3, 1000, 4.5e-4, 0.0, 150, 3e-3, 0.0),
+    (4, 257, 2e-3, 0.01, 300, 1.7e-3, -0.003),
+]
+
+
+def interp_fixture(R):
+    rng = np.random.default_rng(17)
+    out = {}
+    for k, (ntr, n1, d1, o1, n2, d2, o2) in enumerate(INTERP_CASES):
+        a = rng.standard_normal((ntr, n1)).astype(np.float32)
+        out[f"in{k}"] = a
+        for mode in (0, 1, 2):
+            b = np.zeros((ntr, n2), np.float32)
+            R.ref_interp(mode, a.ravel(), ntr, n1, d1, o1, b.reshape(-1), n2, d2, o2)
+            out[f"out{k}_m{mode}"] = b
+    save("interp", **out)
+
+
 REVOLVE_CASES = [(10, 3), (100, 5), (7, 2), (50, 60), (1, 2), (257, 4), (3000, 50), (2500, 50), (4000, 50)]
 
 
@@ -269,7 +288,8 @@ def revolve_fixture(R):
 
 GROUPS = dict(der=der_fixture, pml=pml_fixture, stagger=stagger_fixture, makemap=makemap_fixture,
               runs=run_fixtures, revolve=revolve_fixture, fwi=fwi_fixtures, fwi_full=fwi_full_fixtures,
-              fwi_e2d=fwi_e2d_fixtures, fwi_a3d=fwi_a3d_fixtures)
+              fwi_e2d=fwi_e2d_fixtures, fwi_a3d=fwi_a3d_fixtures,
+              interp=interp_fixture)
 
 if __name__ == "__main__":
     # python -m tests.golden.make_golden [group ...]   (default: every group)

@@ -171,3 +171,16 @@ def test_fwi_a3d_gradient(golden, case):
     for k in ("vpgrad", "rhograd", "dmod", "dres", "wavgrad"):
         assert np.array_equal(r[k], g[k]), k
     assert np.float32(r["misfit"]) == g["misfit"]
+
+
+def test_interp_matches_reference(golden):
+    """Interp::interp restated (LINEAR, BSPLINE, SINC) vs the reference."""
+    from oracle import fn
+    from tests.golden.make_golden import INTERP_CASES
+    g = golden("interp")
+    for k, (ntr, n1, d1, o1, n2, d2, o2) in enumerate(INTERP_CASES):
+        for mode in (0, 1, 2):
+            out = np.zeros((ntr, n2), np.float32)
+            fn("interp")(mode, g[f"in{k}"].ravel(), ntr, n1, np.float32(d1), np.float32(o1), out.reshape(-1), n2,
+                         np.float32(d2), np.float32(o2))
+            assert np.array_equal(out, g[f"out{k}_m{mode}"]), (k, mode)

@@ -147,3 +147,46 @@ def test_snap_cadence_closed_form():
         for dt in (1e-3, 5e-4, 4.5e-4, 2e-3):
             for snapinc in (1, 2, 3, 5, 7, 10, 13, 50):
                 assert snap_cadence(nt, dt, snapinc) == ((nt - 1) // snapinc + 1, (nt - 1) % snapinc)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_interp_closed_forms(mode):
+    """Interp pinned by what the kernels fix: on the input grid (d2 == d1)
+    LINEAR and SINC return the input (SINC's taps are sinc at integers, 1 or
+    ~1e-17) and BSPLINE the (1, 4, 1)/6 smoothing; LINEAR reproduces a linear ramp exactly at any rate; BSPLINE's
+    weights sum to one, so a constant trace stays constant; samples whose
+    taps leave the trace are zero (the first pad-1 and last pad ones).  The
+    reference computes the tap position in float, (j*d2 - x0)/d1, which
+    cancels to a relative error of about j*2^-24 in the fractional offset:
+    hence an absolute tolerance of 2e-4 on unit-variance traces (a wrong tap
+    or weight is an O(1) error)."""
+    from oracle import fn
+    pad = (1, 2, 3)[mode]
+    n1, d1 = 200, 1e-3
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((2, n1)).astype(np.float32)
+    out = np.zeros((2, n1), np.float32)
+    fn("interp")(mode, a.ravel(), 2, n1, np.float32(d1), np.float32(0), out.reshape(-1), n1, np.float32(d1),
+                 np.float32(0))
+    lo, hi = pad - 1, n1 - pad
+    if mode == 1:
+        # cubic B-spline smoothing: (1/6, 4/6, 1/6) on the grid
+        want = (a[:, lo - 1:hi - 1] + 4 * a[:, lo:hi] + a[:, lo + 1:hi + 1]) / 6
+        np.testing.assert_allclose(out[:, lo:hi], want, atol=2e-4)
+    else:
+        np.testing.assert_allclose(out[:, lo:hi], a[:, lo:hi], atol=2e-4)
+    assert not out[:, :lo].any() and not out[:, hi:].any()
+    n2, d2 = 333, 5.9e-4
+    ramp = (0.25 + 3.0 * np.arange(n1) * d1).astype(np.float32)[None]
+    o = np.zeros((1, n2), np.float32)
+    fn("interp")(0, ramp.ravel(), 1, n1, np.float32(d1), np.float32(0), o.reshape(-1), n2, np.float32(d2),
+                 np.float32(0))
+    t = np.arange(n2) * np.float32(d2)
+    ok = t < (n1 - 1) * d1 - d1
+    np.testing.assert_allclose(o[0, ok], 0.25 + 3.0 * t[ok], rtol=1e-5)
+    ones = np.ones((1, n1), np.float32)
+    o = np.zeros((1, n2), np.float32)
+    fn("interp")(1, ones.ravel(), 1, n1, np.float32(d1), np.float32(0), o.reshape(-1), n2, np.float32(d2),
+                 np.float32(0))
+    nz = o[0] != 0
+    np.testing.assert_allclose(o[0, nz], 1.0, atol=1e-6)
