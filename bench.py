@@ -46,6 +46,11 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=80, help="time steps of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fwi", action="store_true", help="skip the FWI shots/s leg")
+    ap.add_argument("--legs", default="",
+                    help="extra legs, comma separated: a3d (acoustic 3D modelling + FWI gradient, "
+                         "440^3 padded, nt 2500) and e2dfwi (128 ragged elastic FWI windows, nt 4000); "
+                         "each takes about a minute")
+    ap.add_argument("--e2dfwi-shots", type=int, default=128)
     ap.add_argument("--fwi-plain-shots", type=int, default=8,
                     help="shots timed on the plain per-shot path (shots/s is per shot)")
     ap.add_argument("--pmc-dispatch", action="store_true",
@@ -223,6 +228,13 @@ def main():
         del mod
         torch.cuda.empty_cache()
         out["fwi"] = fwi_leg(args.fwi_plain_shots)
+    legs = [x for x in args.legs.split(",") if x]
+    if "a3d" in legs:
+        torch.cuda.empty_cache()
+        out["a3d"] = a3d_leg()
+    if "e2dfwi" in legs:
+        torch.cuda.empty_cache()
+        out["e2dfwi"] = e2dfwi_leg(args.e2dfwi_shots)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, inputs, args.cpu_steps)
     else:
@@ -284,6 +296,147 @@ def fwi_leg(n_plain):
             "speedup_batched_vs_plain": round((nshot / tb) / (n_plain / tp), 2),
             "misfit_32_shots": misfit_b,
             "batched_vs_plain_gradient_rel_l2": err}
+
+
+def a3d_leg():
+    """Acoustic 3D modelling and single-shot FWI gradient: 400^3 at 10 m
+    (440^3 padded, lpml 20), order 4, layered Vp (1500-3300 m/s, 10 seeded
+    anomalies of +-8%), Gardner rho, 12 Hz Ricker at the surface centre,
+    200 x 200 receivers every 20 m at z = 10 m, dt 1 ms, nt 2500.  The
+    modelled records of the true model are the observed data of the gradient,
+    which runs on the layered background (the true model without its
+    anomalies), with 50 Revolve checkpoints in FP16."""
+    import numpy as np
+    import torch
+    from rockseis_amd import _lib as R, core, fwi, synth
+    n, lpml, dx, nt, dt, order = 400, 20, 10.0, 2500, 1e-3, 4
+    vp = synth.layered_anomaly_vp3d(n, n, n, dx, dx, dx, nanom=10, seed=44)
+    rho = synth.gardner(vp)
+    m = core.ModelAcoustic3D(n, n, n, lpml, dx, dx, dx)
+    m.Vp[:], m.R[:] = vp, rho
+    src = core.Data3D(1, nt, dt, 0.0, "P")
+    src.scoords[:] = np.float32([[n // 2 * dx, n // 2 * dx, 10.0]])
+    src.data[:] = synth.ricker(nt, dt, 12.0)
+    src.makeMap(m, core.SMAP)
+    gx, gy = np.meshgrid(np.arange(200) * 20.0, np.arange(200) * 20.0)
+    gc = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, 10.0)], 1).astype(np.float32)
+    rec = core.Data3D(gc.shape[0], nt, dt, 0.0, "P")
+    rec.gcoords[:] = gc
+    rec.makeMap(m, core.GMAP)
+    m.staggerModels()
+    mod = core.ModellingAcoustic3D(m, src, order, 1)
+    mod.setRecP(rec)
+    mod.setGraphChunk(50)
+    mod.prepare()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mod.launch(0, nt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    mod.finish()
+    N = m.getNx_pml() * m.getNy_pml() * m.getNz_pml()
+    g, st = mod._args[0], mod._args[1]
+    strips = lpml * (m.getNy_pml() * m.getNz_pml() + m.getNx_pml() * m.getNz_pml() + m.getNx_pml() * m.getNy_pml())
+    kt = {}
+    for which, name, b in ((0, "velocity", 40 * N + 16 * strips), (1, "stress", 24 * N + 16 * strips)):
+        s_ = torch.cuda.current_stream()
+        R.check(R.lib().rsg_repeat_step(R.KIND_A3D, which, C.byref(g), C.byref(st), 3, C.c_void_p(s_.cuda_stream)))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s_)
+        R.check(R.lib().rsg_repeat_step(R.KIND_A3D, which, C.byref(g), C.byref(st), 20, C.c_void_p(s_.cuda_stream)))
+        e1.record(s_)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / 20
+        kt[name] = {"avg_launch_us": round(ms * 1e3, 1), "bytes_per_launch": b,
+                    "achieved_GBs": round(b / (ms * 1e-3) / 1e9, 1),
+                    "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    dobs = rec.data.copy()
+    del mod
+    torch.cuda.empty_cache()
+    v0 = synth.layered_anomaly_vp3d(n, n, n, dx, dx, dx, nanom=0, seed=44)
+    m0 = core.ModelAcoustic3D(n, n, n, lpml, dx, dx, dx)
+    m0.Vp[:], m0.R[:] = v0, synth.gardner(v0)
+    src.makeMap(m0, core.SMAP)
+    obs = core.Data3D(gc.shape[0], nt, dt, 0.0, "P")
+    obs.gcoords[:] = gc
+    obs.data[:] = dobs
+    obs.makeMap(m0, core.GMAP)
+    m0.staggerModels()
+    f = fwi.FwiAcoustic3D(m0, src, obs, order, 1)
+    vg, rg = fwi.Image3D(m0), fwi.Image3D(m0)
+    f.setVpgrad(vg)
+    f.setRhograd(rg)
+    f.setNcheck(50)
+    f.setFp16Storage(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f.run_optimal()
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t0
+    return {"workload": "acoustic 3D, 400^3 at 10 m (440^3 padded), order 4, nt 2500, 200x200 receivers",
+            "modelling_gcell_per_s": round(N * nt / wall / 1e9, 3), "modelling_s": round(wall, 3),
+            "kernels": kt, "fwi_gradient_s": round(tg, 2),
+            "fwi": "FwiAcoustic3D::run_optimal, 50 Revolve checkpoints in FP16, layered background model",
+            "misfit": f.misfit64}
+
+
+def e2dfwi_leg(nshot, n_plain=4):
+    """Batched elastic FWI gradients over ragged windows: a 6000 x 600 model
+    at 5 m (Vp 1500-4500 m/s, Vs = Vp/1.8, Gardner rho), nshot windows of
+    seeded widths 600..2000 x 600 cells, order 8, lpml 30, 15 Hz Ricker
+    explosive source at the window centre (z = 10 m), receivers at every cell
+    of the window at z = 10 m (P, Ux, Uz), dt 0.45 ms, nt 4000, 50 Revolve
+    checkpoints in FP16; lambda / mu / rho kernels.  The observed records are
+    zero (the timed work does not depend on their values).  Plain = the first
+    n_plain windows one at a time."""
+    import numpy as np
+    import torch
+    from rockseis_amd import fwi, synth
+    nx, nz, dx, lpml, nt, dt, order = 6000, 600, 5.0, 30, 4000, 4.5e-4, 8
+    rng = np.random.default_rng(5)
+    vp = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1500.0, 4500.0, nanom=60, seed=5)
+    vp = np.clip(vp, 1500.0, 4500.0).astype(np.float32)
+    vs = (vp / 1.8).astype(np.float32)
+    rho = synth.gardner(vp)
+    wav = synth.ricker(nt, dt, 15.0)[None]
+    shots = []
+    cells = 0
+    for k in range(nshot):
+        w = int(rng.integers(600, 2001))
+        x0 = int(rng.integers(0, nx - w + 1))
+        gc = synth.surface_line(w, dx, 10.0, x0 * dx)
+        sc = np.float32([[(x0 + w // 2) * dx, 10.0]])
+        shots.append(fwi.ElasticShot(sc, wav, gc, np.zeros((3, w, nt), np.float32), x0, w, "P"))
+        cells += (w + 2 * lpml) * (nz + 2 * lpml)
+    sv = fwi.ElasticSurvey(shots, lpml, dx, dx, dt, order, False, 50, fp16=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mis, l, u, r, _ = fwi.elastic_batch_gradient(vp, vs, rho, sv, scale=False)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    del l, u, r
+    torch.cuda.empty_cache()
+    n_plain = max(1, min(n_plain, nshot))
+    t0 = time.perf_counter()
+    for sh in shots[:n_plain]:
+        one = fwi.ElasticSurvey([sh], lpml, dx, dx, dt, order, False, 50, fp16=True)
+        fwi.elastic_batch_gradient(vp, vs, rho, one, scale=False)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    steps = nt + _numforw(nt, 50)  # forward steps incl. recomputation + adjoint steps
+    return {"workload": f"{nshot} elastic windows of 600-2000 x 600 cells (+lpml 30) from a 6000x600 model at "
+                        "5 m, order 8, nt 4000, FP16 Revolve (50)",
+            "padded_cells_per_step": cells, "batched_s": round(tb, 2),
+            "batched_shots_per_s": round(nshot / tb, 3),
+            "cell_updates_per_s_G": round(cells * steps / tb / 1e9, 2),
+            "plain_shots_timed": n_plain, "plain_s": round(tp, 2),
+            "plain_shots_per_s": round(n_plain / tp, 3),
+            "speedup_batched_vs_plain": round((nshot / tb) / (n_plain / tp), 2)}
+
+
+def _numforw(nt, snaps):
+    from rockseis_amd import _lib as R
+    return R.lib().rsg_revolve_numforw(nt, snaps)
 
 
 def _traffic(name):

@@ -235,6 +235,9 @@ typedef struct {
     int snapinc;                    /* RSG_FWI_FULL: snapshot increment (Fwi snapinc) */
     int fp16;                       /* store snapshots / checkpoints as FP16 with a
                                        power-of-two scale per stored array          */
+    int misfit_type;                /* rs_fwimisfit (utils.h:26): 0 DIFFERENCE,
+                                       1 CORRELATION                                 */
+    const float *weight;            /* data weights like obs (setDataweight) or NULL */
 } rsg_fwi_a2d;
 
 int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, void *stream);
@@ -260,6 +263,8 @@ typedef struct {
     double *misfit;                      /* out (host)                               */
     int ncheck;                          /* Revolve snapshots                        */
     int fp16;                            /* FP16 checkpoints                         */
+    int misfit_type;                     /* 0 DIFFERENCE, 1 CORRELATION              */
+    const float *weight;                 /* data weights like obs, or NULL           */
 } rsg_fwi_a3d;
 
 int rsg_fwi_a3d_gradient(const rsg_grid *g, const rsg_fwi_a3d *f, void *stream);

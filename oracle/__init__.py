@@ -139,6 +139,11 @@ def _declare(L):
         f.restype = C.c_int
         f = getattr(L, "orc_a2d_crosscorr" + suf)
         f.argtypes = [rp, C.c_int, rp, rp, C.c_int, rp, rp, rp, rp, C.c_int, C.c_int, ct, ct, rp, rp]
+        f = getattr(L, "orc_misfit_a" + suf)
+        f.argtypes = [C.c_int, rp, rp, C.c_void_p, _i32p, C.c_int, C.c_int]
+        f.restype = ct
+        f = getattr(L, "orc_residual_a" + suf)
+        f.argtypes = [C.c_int, rp, rp, C.c_void_p, rp, C.c_int, C.c_int, C.c_int]
         f = getattr(L, "orc_interp" + suf)
         f.argtypes = [C.c_int, rp, C.c_int, C.c_int, ct, ct, rp, C.c_int, ct, ct]
         f = getattr(L, "orc_a3d_crosscorr" + suf)
@@ -186,6 +191,10 @@ def _declare_ref(R):
     R.ref_fwi_a2d.argtypes = ([_f32p, _f32p] + [C.c_int] * 3 + [C.c_float] * 3 + [C.c_int] * 5
                               + [_f32p, _f32p, C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p])
     R.ref_fwi_a2d.restype = C.c_float
+    R.ref_fwi_a2d_misfit.argtypes = ([_f32p, _f32p] + [C.c_int] * 3 + [C.c_float] * 3 + [C.c_int] * 5
+                                     + [_f32p, _f32p, C.c_int, _f32p, _f32p, C.c_int, C.c_void_p]
+                                     + [_f32p] * 5)
+    R.ref_fwi_a2d_misfit.restype = C.c_float
     R.ref_fwi_a2d_full.argtypes = ([_f32p, _f32p] + [C.c_int] * 3 + [C.c_float] * 3 + [C.c_int] * 4
                                    + [C.c_char_p, C.c_int, _f32p, _f32p, C.c_int, _f32p, _f32p, _f32p,
                                       _f32p, _f32p, _f32p, _f32p])

@@ -50,8 +50,10 @@ def snap_cadence(nt, dt, snapinc):
 
 
 def fwi_a2d(vp, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, scoords, wav, gcoords, dobs,
-            real=np.float32):
-    """Returns dict(misfit, vpgrad, rhograd, dmod, dres, wavgrad)."""
+            real=np.float32, misfit_type=0, weight=None):
+    """Returns dict(misfit, vpgrad, rhograd, dmod, dres, wavgrad).
+    misfit_type 0 DIFFERENCE, 1 CORRELATION; weight: optional data weights
+    (nrec, nt) as setDataweight."""
     nz, nx = vp.shape
     smap, sshift = makemap2d(scoords, nx, nz, dx, dz, "P")
     gmap, gshift = makemap2d(gcoords, nx, nz, dx, dz, "P")
@@ -68,6 +70,7 @@ def fwi_a2d(vp, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, scoords, wav, gcoo
     vpgrad = np.zeros((nz, nx), real)
     rhograd = np.zeros((nz, nx), real)
     valid = ((gmap[:, 0] >= 0) & (gmap[:, 1] >= 0)).astype(np.int32)
+    wei = None if weight is None else np.ascontiguousarray(weight, real).reshape(gmap.shape[0], nt)
     misfit = None
     checks = {}
     rev = orv.Revolve(nt, ncheck)
@@ -95,8 +98,11 @@ def fwi_a2d(vp, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, scoords, wav, gcoo
                 fstep(it, not reverse)
         elif a == orv.FIRSTURN:
             fstep(capo, True)
-            misfit = fn("misfit_diff", real)(dmod.reshape(-1), dobs.reshape(-1), valid, dobs.shape[0], nt)
-            fn("residual_diff", real)(dmod.reshape(-1), dobs.reshape(-1), dres.reshape(-1), dobs.shape[0], nt, 1)
+            wp = None if wei is None else wei.ctypes.data
+            misfit = fn("misfit_a", real)(misfit_type, dmod.reshape(-1), dobs.reshape(-1), wp, valid,
+                                          dobs.shape[0], nt)
+            fn("residual_a", real)(misfit_type, dmod.reshape(-1), dobs.reshape(-1), wp, dres.reshape(-1),
+                                   dobs.shape[0], nt, 1)
             inject2d(bw.P, bw.L, lpml, dt, gmap, gshift, dres, capo, real)
             corr()
             record2d(bw.P, lpml, smap, sshift, wgrad, capo, real)
@@ -114,6 +120,29 @@ def fwi_a2d(vp, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, scoords, wav, gcoo
         if a in (orv.TERMINATE, orv.ERROR):
             break
     return dict(misfit=misfit, vpgrad=vpgrad, rhograd=rhograd, dmod=dmod, dres=dres, wavgrad=wgrad)
+
+
+def ref_fwi_a2d_misfit(vp, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, scoords, wav, gcoords, dobs,
+                       misfit_type=0, weight=None):
+    """The reference's run_optimal with a misfit type and data weights."""
+    nz, nx = vp.shape
+    sc = np.ascontiguousarray(scoords, np.float32).reshape(-1, 2)
+    gc = np.ascontiguousarray(gcoords, np.float32).reshape(-1, 2)
+    nsrc, nrec = sc.shape[0], gc.shape[0]
+    vg = np.zeros((nz, nx), np.float32)
+    rg = np.zeros((nz, nx), np.float32)
+    dmod = np.zeros((nrec, nt), np.float32)
+    dres = np.zeros((nrec, nt), np.float32)
+    wg = np.zeros((nsrc, nt), np.float32)
+    w = None if weight is None else np.ascontiguousarray(weight, np.float32)
+    m = ref().ref_fwi_a2d_misfit(np.ascontiguousarray(vp, np.float32).ravel(),
+                                 np.ascontiguousarray(rho, np.float32).ravel(), nx, nz, lpml, dx, dz, dt, nt,
+                                 order, int(fs), ncheck, nsrc, sc.ravel(),
+                                 np.ascontiguousarray(wav, np.float32).ravel(), nrec, gc.ravel(),
+                                 np.ascontiguousarray(dobs, np.float32).ravel(), int(misfit_type),
+                                 None if w is None else w.ctypes.data, vg.reshape(-1), rg.reshape(-1),
+                                 dmod.reshape(-1), dres.reshape(-1), wg.reshape(-1))
+    return dict(misfit=np.float32(m), vpgrad=vg, rhograd=rg, dmod=dmod, dres=dres, wavgrad=wg)
 
 
 def fwi_a2d_full(vp, rho, lpml, dx, dz, dt, nt, order, fs, snapinc, scoords, wav, gcoords, dobs,

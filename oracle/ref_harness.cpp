@@ -288,7 +288,7 @@ static float fwi_a2d(const float *vp, const float *rho, int nx, int nz, int lpml
                      float dt, int nt, int order, int fs, int ncheck, int snapinc, const char *snapfile,
                      int nsrc, const float *scoords, const float *wav, int nrec, const float *gcoords,
                      const float *dobs, float *vpgrad, float *rhograd, float *dmod, float *dres,
-                     float *wgrad) {
+                     float *wgrad, int misfit_type = 0, const float *weight = nullptr) {
     auto model = std::make_shared<ModelAcoustic2D<float>>(nx, nz, lpml, dx, dz, 0.f, 0.f, fs != 0);
     model->createModel();
     std::memcpy(model->getVp(), vp, sizeof(float) * nx * nz);
@@ -327,7 +327,8 @@ static float fwi_a2d(const float *vp, const float *rho, int nx, int nz, int lpml
         fwi->setSnapinc(snapinc);
         fwi->setSnapfile(snapfile);
     }
-    fwi->setMisfit_type(DIFFERENCE);
+    fwi->setMisfit_type(misfit_type == 1 ? CORRELATION : DIFFERENCE);
+    if (weight) fwi->setDataweight(mk(weight));
     if (ncheck > 0)
         fwi->run_optimal();
     else
@@ -346,6 +347,17 @@ float ref_fwi_a2d(const float *vp, const float *rho, int nx, int nz, int lpml, f
                   float *rhograd, float *dmod, float *dres, float *wgrad) {
     return fwi_a2d(vp, rho, nx, nz, lpml, dx, dz, dt, nt, order, fs, ncheck, 1, "", nsrc, scoords, wav,
                    nrec, gcoords, dobs, vpgrad, rhograd, dmod, dres, wgrad);
+}
+
+// run_optimal with a misfit type (0 DIFFERENCE, 1 CORRELATION) and optional
+// data weights (nrec*nt).
+float ref_fwi_a2d_misfit(const float *vp, const float *rho, int nx, int nz, int lpml, float dx, float dz,
+                         float dt, int nt, int order, int fs, int ncheck, int nsrc, const float *scoords,
+                         const float *wav, int nrec, const float *gcoords, const float *dobs, int misfit_type,
+                         const float *weight, float *vpgrad, float *rhograd, float *dmod, float *dres,
+                         float *wgrad) {
+    return fwi_a2d(vp, rho, nx, nz, lpml, dx, dz, dt, nt, order, fs, ncheck, 1, "", nsrc, scoords, wav, nrec,
+                   gcoords, dobs, vpgrad, rhograd, dmod, dres, wgrad, misfit_type, weight);
 }
 
 float ref_fwi_a2d_full(const float *vp, const float *rho, int nx, int nz, int lpml, float dx, float dz,

@@ -951,6 +951,102 @@ void FN(orc_interp)(int mode, const REAL *from, int ntr, int n1, REAL d1, REAL o
         }
 }
 
+/*
+ * FwiAcoustic2D::computeMisfit (fwi.cpp:574-660) for type 0 DIFFERENCE
+ * (optional data weights: res *= wei) and 1 CORRELATION (per trace
+ * -sum(mod*rec)/(|mod||rec|), norms accumulated sample by sample in T, sqrt
+ * rounded to T, zero norms replaced by 1); traces with valid maps only;
+ * misfit accumulated in T.  wei may be NULL.
+ */
+REAL FN(orc_misfit_a)(int type, const REAL *mod, const REAL *rec, const REAL *wei, const int *valid, int ntr,
+                      int nt)
+{
+    REAL misfit = 0.0;
+    for (int i = 0; i < ntr; i++) {
+        if (!valid[i]) continue;
+        const REAL *m = mod + (long)i * nt, *r = rec + (long)i * nt, *w = wei ? wei + (long)i * nt : NULL;
+        if (type == 1) {
+            REAL n1 = 0.0, n2 = 0.0;
+            for (int it = 0; it < nt; it++) {
+                n1 += m[it] * m[it];
+                n2 += r[it] * r[it];
+            }
+            n1 = (REAL)sqrt((double)n1);
+            n2 = (REAL)sqrt((double)n2);
+            if (n1 == 0) n1 = 1.0;
+            if (n2 == 0) n2 = 1.0;
+            for (int it = 0; it < nt; it++) {
+                REAL res = (REAL)((-1.0) * (double)(m[it] * r[it] / (n1 * n2)));
+                if (w) res *= w[it];
+                misfit += res;
+            }
+        } else {
+            for (int it = 0; it < nt; it++) {
+                REAL res = m[it] - r[it];
+                if (w) res *= w[it];
+                misfit = (REAL)((double)misfit + 0.5 * (double)res * (double)res);
+            }
+        }
+    }
+    return misfit;
+}
+
+static int FN(orc_cmp)(const void *a, const void *b)
+{
+    REAL x = *(const REAL *)a, y = *(const REAL *)b;
+    return (x > y) - (x < y);
+}
+
+/*
+ * FwiAcoustic2D::computeResiduals (fwi.cpp:803-880): type 0 DIFFERENCE
+ * res = scaling*(mod-rec) (*= wei^2); type 1 CORRELATION
+ * res = -scaling*(rec/(n1 n2) - mod/(n1 n1) * (sum(mod*rec)/(n1 n2)))
+ * (*= wei^2) on every trace, then every |res| at or above the PCLIP = 97th
+ * percentile (element (int)(97*nt*ntr/100) of the sorted magnitudes) is set
+ * to zero.
+ */
+void FN(orc_residual_a)(int type, const REAL *mod, const REAL *rec, const REAL *wei, REAL *res, int ntr, int nt,
+                        int scaling)
+{
+    const long n = (long)ntr * nt;
+    if (type != 1) {
+        for (long p = 0; p < n; p++) {
+            res[p] = scaling * (mod[p] - rec[p]);
+            if (wei) res[p] *= wei[p] * wei[p];
+        }
+        return;
+    }
+    for (int i = 0; i < ntr; i++) {
+        const REAL *m = mod + (long)i * nt, *r = rec + (long)i * nt;
+        REAL n1 = 0.0, n2 = 0.0, n3 = 0.0;
+        for (int it = 0; it < nt; it++) {
+            n1 += m[it] * m[it];
+            n2 += r[it] * r[it];
+            n3 += m[it] * r[it];
+        }
+        n1 = (REAL)sqrt((double)n1);
+        n2 = (REAL)sqrt((double)n2);
+        if (n1 == 0) n1 = 1.0;
+        if (n2 == 0) n2 = 1.0;
+        n3 /= (n1 * n2);
+        for (int it = 0; it < nt; it++) {
+            long p = (long)i * nt + it;
+            res[p] = (REAL)((-1.0) * scaling * (double)((r[it] / (n1 * n2)) - (m[it] / (n1 * n1)) * n3));
+            if (wei) res[p] *= wei[p] * wei[p];
+        }
+    }
+    REAL *f = (REAL *)malloc(sizeof(REAL) * (n > 0 ? n : 1));
+    for (long p = 0; p < n; p++) f[p] = res[p] < 0 ? -res[p] : res[p];
+    qsort(f, n, sizeof(REAL), FN(orc_cmp));
+    long pos = (long)(int)((97L * nt * ntr) / 100);
+    REAL pclip = f[pos];
+    free(f);
+    for (long p = 0; p < n; p++) {
+        REAL a = res[p] < 0 ? -res[p] : res[p];
+        if (a >= pclip) res[p] = 0.0;
+    }
+}
+
 /* DIFFERENCE misfit 0.5*sum(r^2) over traces with valid maps, accumulated
  * in T (fwi.cpp:574-608); `valid[i]` nonzero when map x,z >= 0. */
 REAL FN(orc_misfit_diff)(const REAL *mod, const REAL *rec, const int *valid, int ntr, int nt)

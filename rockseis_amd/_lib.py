@@ -51,14 +51,16 @@ class Fwi(C.Structure):
                 + [("nx", C.c_int), ("nz", C.c_int), ("src", Traces), ("obs", Traces)]
                 + [(n, C.c_void_p) for n in ("dmod", "dres", "wavgrad", "vpgrad", "rhograd")]
                 + [("misfit", C.POINTER(C.c_double)), ("ncheck", C.c_int), ("mode", C.c_int),
-                   ("mode_used", C.POINTER(C.c_int)), ("snapinc", C.c_int), ("fp16", C.c_int)])
+                   ("mode_used", C.POINTER(C.c_int)), ("snapinc", C.c_int), ("fp16", C.c_int),
+                   ("misfit_type", C.c_int), ("weight", C.c_void_p)])
 
 
 class FwiA3D(C.Structure):
     _fields_ = ([(n, C.c_void_p) for n in ("L", "Rx", "Ry", "Rz", "tab", "Vp", "Rho")]
                 + [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("src", Traces), ("obs", Traces)]
                 + [(n, C.c_void_p) for n in ("dmod", "dres", "wavgrad", "vpgrad", "rhograd")]
-                + [("misfit", C.POINTER(C.c_double)), ("ncheck", C.c_int), ("fp16", C.c_int)])
+                + [("misfit", C.POINTER(C.c_double)), ("ncheck", C.c_int), ("fp16", C.c_int),
+                   ("misfit_type", C.c_int), ("weight", C.c_void_p)])
 
 
 class FwiE2DShot(C.Structure):

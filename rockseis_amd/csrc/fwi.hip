@@ -217,31 +217,6 @@ __global__ void crosscorr_a2d_kernel(const void *__restrict__ Pf_, const unsigne
     rhograd[m] = rg;
 }
 
-// DIFFERENCE misfit (fwi.cpp:574-608, 3D: 1498-1510) and residuals
-// (fwi.cpp:772-800, 3D: 1685-1695).  The misfit is reduced in double over
-// traces whose dim map indices are all valid; res = scaling*(mod - rec).
-__global__ void misfit_residual_kernel(const float *__restrict__ mod, const float *__restrict__ rec,
-                                       float *__restrict__ res, const int *__restrict__ map, int dim,
-                                       int ntr, int nt, int scaling, double *misfit) {
-    __shared__ double part[256];
-    double acc = 0.0;
-    const size_t n = (size_t)ntr * nt;
-    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-        const float r = mod[p] - rec[p];
-        res[p] = scaling * r;
-        const size_t tr = p / nt;  // trace of the batch; maps of shot k follow shot k-1
-        const int *mt = map + dim * tr;
-        if (mt[0] >= 0 && mt[1] >= 0 && (dim == 2 || mt[2] >= 0)) acc += 0.5 * (double)r * (double)r;
-    }
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(misfit, part[0]);
-}
-
 // ---------------------------------------------------------------------------
 // Driver
 // ---------------------------------------------------------------------------
@@ -344,10 +319,8 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
         return rsg_a2d_forward_stress_batch(g, &bw.st, &bw.b, st);
     };
     auto first_turn = [&]() -> int {
-        misfit_residual_kernel<<<1024, 256, 0, st>>>(f->dmod, f->obs.data, f->dres, f->obs.map, 2,
-                                                     ns * f->obs.ntrace, nt, 1, misfit_d);
-        RSG_LAUNCH_CHECK("misfit_residual_kernel");
-        return RSG_OK;
+        return acoustic_misfit_residuals(f->misfit_type, f->dmod, f->obs.data, f->weight, f->dres, f->obs.map, 2,
+                                         f->obs.ntrace, nt, ns, 1, misfit_d, st);
     };
     // Residual injection at time t, crossCorr with Pf (skipped when null;
     // FP16 when amax is set) and the wavelet gradient at t.

@@ -184,9 +184,9 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
                 if (int rc = fstep(it, !reverse)) return rc;
         } else if (a == Revolve::FIRSTURN) {
             if (int rc = fstep(capo, true)) return rc;
-            misfit_residual_kernel<<<1024, 256, 0, st>>>(f->dmod, f->obs.data, f->dres, f->obs.map, 3,
-                                                         f->obs.ntrace, nt, 1, dmis);
-            RSG_LAUNCH_CHECK("misfit_residual_kernel");
+            if (int rc = acoustic_misfit_residuals(f->misfit_type, f->dmod, f->obs.data, f->weight, f->dres,
+                                                   f->obs.map, 3, f->obs.ntrace, nt, 1, 1, dmis, st))
+                return rc;
             if (int rc = reverse_tail(capo)) return rc;
             reverse = true;
         } else if (a == Revolve::YOUTURN) {

@@ -22,10 +22,12 @@ struct Revolve {
     static int numforw(int steps, int snaps);
 };
 
-// DIFFERENCE misfit (double, into *misfit) and residuals of ntr traces;
-// map holds dim indices per trace (fwi.hip).
-__global__ void misfit_residual_kernel(const float *__restrict__ mod, const float *__restrict__ rec,
-                                       float *__restrict__ res, const int *__restrict__ map, int dim,
-                                       int ntr, int nt, int scaling, double *misfit);
+// Misfit (added to *misfit in double) and residuals of nshot shots of ntr
+// traces (shot k's traces follow shot k-1's) for type 0 DIFFERENCE or
+// 1 CORRELATION, optional data weights; map holds dim indices per trace
+// (misfit.hip).
+int acoustic_misfit_residuals(int type, const float *mod, const float *rec, const float *wei, float *res,
+                              const int *map, int dim, int ntr, int nt, int nshot, int scaling, double *misfit,
+                              hipStream_t st);
 
 }  // namespace rsg

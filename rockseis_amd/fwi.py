@@ -25,7 +25,7 @@ from . import _lib as R
 from .core import (GMAP, SMAP, Data2D, Data3D, ModelAcoustic2D, ModelAcoustic3D, ModelElastic2D, Pml, SMAX, _dev,
                    _stream)
 
-DIFFERENCE = 0  # rs_fwimisfit (utils.h:27)
+DIFFERENCE, CORRELATION = 0, 1  # rs_fwimisfit (utils.h:26)
 _MODES = {"revolve": R.FWI_REVOLVE, "store": R.FWI_STORE, "full": R.FWI_FULL}
 
 
@@ -60,6 +60,7 @@ class FwiAcoustic2D:
         self.datamodP = self.dataresP = self.vpgrad = self.rhograd = self.wavgrad = None
         self.misfit = 0.0
         self.misfit_type = DIFFERENCE
+        self.dataweight = None
         self.fp16 = False
 
     def setDatamodP(self, d):
@@ -84,9 +85,12 @@ class FwiAcoustic2D:
         self.incore = bool(v)
 
     def setMisfit_type(self, t):
-        if int(t) != DIFFERENCE:
-            raise R.RsgError("only the DIFFERENCE misfit is implemented on this path")
-        self.misfit_type = DIFFERENCE
+        if int(t) not in (DIFFERENCE, CORRELATION):
+            raise R.RsgError("the acoustic misfit on this path is DIFFERENCE or CORRELATION")
+        self.misfit_type = int(t)
+
+    def setDataweight(self, d):
+        self.dataweight = d
 
     def setFp16Storage(self, v):
         self.fp16 = bool(v)
@@ -138,6 +142,10 @@ class FwiAcoustic2D:
         f.misfit = C.pointer(misfit)
         f.ncheck, f.mode = self.ncheck, mode
         f.snapinc, f.fp16 = self.snapinc, int(self.fp16)
+        f.misfit_type = self.misfit_type
+        if self.dataweight is not None:
+            wt = torch.from_numpy(np.ascontiguousarray(self.dataweight.data, np.float32)).to(dev)
+            f.weight = wt.data_ptr()
         used = C.c_int(-1)
         f.mode_used = C.pointer(used)
         R.check(R.lib().rsg_fwi_a2d_gradient(C.byref(g), C.byref(f), _stream()), "rsg_fwi_a2d_gradient")
@@ -558,6 +566,7 @@ class FwiAcoustic3D:
         self.ncheck, self.incore, self.fp16 = 0, True, False
         self.datamodP = self.dataresP = self.vpgrad = self.rhograd = self.wavgrad = None
         self.misfit = 0.0
+        self.misfit_type, self.dataweight = DIFFERENCE, None
 
     def setDatamodP(self, d):
         self.datamodP = d
@@ -584,8 +593,12 @@ class FwiAcoustic3D:
         self.fp16 = bool(v)
 
     def setMisfit_type(self, t):
-        if int(t) != DIFFERENCE:
-            raise R.RsgError("only the DIFFERENCE misfit is implemented on this path")
+        if int(t) not in (DIFFERENCE, CORRELATION):
+            raise R.RsgError("the acoustic misfit on this path is DIFFERENCE or CORRELATION")
+        self.misfit_type = int(t)
+
+    def setDataweight(self, d):
+        self.dataweight = d
 
     def getMisfit(self):
         return self.misfit
@@ -624,6 +637,10 @@ class FwiAcoustic3D:
         f.vpgrad, f.rhograd = vg.data_ptr(), rg.data_ptr()
         f.misfit = C.pointer(misfit)
         f.ncheck, f.fp16 = self.ncheck, int(self.fp16)
+        f.misfit_type = self.misfit_type
+        if self.dataweight is not None:
+            wt = torch.from_numpy(np.ascontiguousarray(self.dataweight.data, np.float32)).to(dev)
+            f.weight = wt.data_ptr()
         R.check(R.lib().rsg_fwi_a3d_gradient(C.byref(g), C.byref(f), _stream()), "rsg_fwi_a3d_gradient")
         self.misfit64 = misfit.value
         self.misfit = float(np.float32(misfit.value))

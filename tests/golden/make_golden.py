@@ -167,6 +167,33 @@ def fwi_fixtures(R):
              wavgrad=r["wavgrad"])
 
 
+FWI_MISFIT_CASES = [  # order, fs, ncheck, misfit type (1 CORRELATION), data weights
+    (4, 0, 20, 1, False), (8, 1, 7, 1, True), (4, 0, 20, 0, True),
+]
+
+
+def fwi_misfit_fixtures(R):
+    """run_optimal with CORRELATION misfit / data weights (setDataweight)."""
+    from oracle import fwi
+    nx, nz, lpml, dx, dt, nt = 60, 50, 10, 10.0, 1e-3, 300
+    vtrue = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1800.0, 3000.0, nanom=5, seed=3)
+    rtrue = synth.gardner(vtrue)
+    vp0 = synth.gaussian_smooth(vtrue, 60.0, dx, dx)
+    rho0 = synth.gardner(vp0)
+    sc = np.float32([[300.0, 10.0]])
+    wav = synth.ricker(nt, dt, 15.0)[None]
+    gc = synth.surface_line(nx, dx, 20.0)
+    rng = np.random.default_rng(1)
+    for k, (order, fs, ncheck, mt, usew) in enumerate(FWI_MISFIT_CASES):
+        obs = runs.ref_a2d_run(vtrue, rtrue, lpml, dx, dx, dt, nt, order, fs, "P", sc, wav, "P", gc)["rec"]
+        w = (0.5 + rng.random((nx, nt))).astype(np.float32) if usew else np.ones((nx, nt), np.float32)
+        r = fwi.ref_fwi_a2d_misfit(vp0, rho0, lpml, dx, dx, dt, nt, order, fs, ncheck, sc, wav, gc, obs, mt,
+                                   w if usew else None)
+        save(f"fwi_misfit_case{k}", vp=vp0, rho=rho0, scoords=sc, wav=wav, gcoords=gc, dobs=obs, weight=w,
+             grid=np.float32([nx, nz, lpml, dx, dt, nt]), misfit=np.float32(r["misfit"]),
+             vpgrad=r["vpgrad"], rhograd=r["rhograd"], dmod=r["dmod"], dres=r["dres"], wavgrad=r["wavgrad"])
+
+
 FWI_FULL_CASES = [  # order, fs, snapinc (FwiAcoustic2D::run)
     (4, 0, 1), (8, 1, 3), (4, 0, 7),
 ]
@@ -289,7 +316,7 @@ def revolve_fixture(R):
 GROUPS = dict(der=der_fixture, pml=pml_fixture, stagger=stagger_fixture, makemap=makemap_fixture,
               runs=run_fixtures, revolve=revolve_fixture, fwi=fwi_fixtures, fwi_full=fwi_full_fixtures,
               fwi_e2d=fwi_e2d_fixtures, fwi_a3d=fwi_a3d_fixtures,
-              interp=interp_fixture)
+              interp=interp_fixture, fwi_misfit=fwi_misfit_fixtures)
 
 if __name__ == "__main__":
     # python -m tests.golden.make_golden [group ...]   (default: every group)

@@ -181,3 +181,38 @@ def test_batched_gradient_equals_sum_of_shots(golden):
         assert abs(mis - tot) <= 1e-6 * tot
         assert rel_l2(vg.cpu().numpy(), gv) <= 1e-6
         assert rel_l2(rg.cpu().numpy(), gr) <= 1e-6
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_misfit_types_vs_reference(golden, case):
+    """CORRELATION misfit (97th-percentile residual clip) and data weights
+    against the reference's run_optimal."""
+    from rockseis_amd import core, fwi
+    from tests.golden.make_golden import FWI_MISFIT_CASES
+    g = golden(f"fwi_misfit_case{case}")
+    order, fs, ncheck, mt, usew = FWI_MISFIT_CASES[case]
+    nx, nz, lpml, dx, dt, nt = g["grid"]
+    nx, nz, lpml, nt = int(nx), int(nz), int(lpml), int(nt)
+    m = core.ModelAcoustic2D(nx, nz, lpml, dx, dx, 0.0, 0.0, fs)
+    m.Vp[:], m.R[:] = g["vp"], g["rho"]
+    src = data2d(g["scoords"], nt, dt, "P", g["wav"])
+    src.makeMap(m, core.SMAP)
+    obs = data2d(g["gcoords"], nt, dt, "P", g["dobs"])
+    obs.makeMap(m, core.GMAP)
+    res = data2d(g["gcoords"], nt, dt, "P")
+    m.staggerModels()
+    f = fwi.FwiAcoustic2D(m, src, obs, order, 1)
+    vg, rg = fwi.Image2D(m), fwi.Image2D(m)
+    f.setVpgrad(vg)
+    f.setRhograd(rg)
+    f.setDataresP(res)
+    f.setNcheck(ncheck)
+    f.setMisfit_type(mt)
+    if usew:
+        f.setDataweight(data2d(g["gcoords"], nt, dt, "P", g["weight"]))
+    f.run_optimal()
+    for k, got in (("vpgrad", vg.imagedata), ("rhograd", rg.imagedata), ("dres", res.data)):
+        e = rel_l2(got, g[k])
+        print(f"  case {case} {k}: rel L2 {e:.2e}", flush=True)
+        assert e <= RTOL_L2, k
+    assert abs(f.misfit64 - float(g["misfit"])) <= 1e-4 * abs(float(g["misfit"]))

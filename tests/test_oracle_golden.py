@@ -184,3 +184,20 @@ def test_interp_matches_reference(golden):
             fn("interp")(mode, g[f"in{k}"].ravel(), ntr, n1, np.float32(d1), np.float32(o1), out.reshape(-1), n2,
                          np.float32(d2), np.float32(o2))
             assert np.array_equal(out, g[f"out{k}_m{mode}"]), (k, mode)
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_fwi_a2d_misfit_types(golden, case):
+    """run_optimal with the CORRELATION misfit (per-trace normalised
+    correlation, 97th-percentile clip of the residuals) and data weights:
+    the oracle vs the reference, bit for bit."""
+    from oracle import fwi
+    from tests.golden.make_golden import FWI_MISFIT_CASES
+    g = golden(f"fwi_misfit_case{case}")
+    order, fs, ncheck, mt, usew = FWI_MISFIT_CASES[case]
+    nx, nz, lpml, dx, dt, nt = g["grid"]
+    r = fwi.fwi_a2d(g["vp"], g["rho"], int(lpml), dx, dx, dt, int(nt), order, fs, ncheck, g["scoords"], g["wav"],
+                    g["gcoords"], g["dobs"], misfit_type=mt, weight=g["weight"] if usew else None)
+    for k in ("vpgrad", "rhograd", "dmod", "dres", "wavgrad"):
+        assert np.array_equal(r[k], g[k]), k
+    assert np.float32(r["misfit"]) == g["misfit"]
