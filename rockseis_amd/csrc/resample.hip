@@ -14,7 +14,8 @@ constexpr double kPi = 3.14159265358979323846;  // utils.h:13
 
 __device__ inline float sinc_ref(float x) {
     if (x == 0.f) return 1.f;
-    return (float)(sin(kPi * (double)x) / (kPi * (double)x));
+    const double px = __dmul_rn(kPi, (double)x);
+    return (float)__ddiv_rn(sin(px), px);
 }
 
 __global__ __launch_bounds__(256) void interp_kernel(int mode, const float *__restrict__ from, int ntr, int n1,
@@ -23,32 +24,41 @@ __global__ __launch_bounds__(256) void interp_kernel(int mode, const float *__re
     const int j = blockIdx.x * 256 + threadIdx.x, i = blockIdx.y;
     if (j >= n2 || i >= ntr) return;
     const int pad = mode == 0 ? 1 : (mode == 1 ? 2 : 3);
-    const float val = o2 + (float)j * d2;
-    const long i0 = (long)((val - o1) / d1) - (pad - 1);
+    // explicitly rounded fp32 operations: no fused multiply-add, as the
+    // reference's host build
+    const float val = __fadd_rn(o2, __fmul_rn((float)j, d2));
+    const long i0 = (long)__fdiv_rn(__fsub_rn(val, o1), d1) - (pad - 1);
     float *out = to + (size_t)i * n2 + j;
     if (i0 < 0 || (i0 + (2 * pad) - 1) > (n1 - 1)) {
         *out = 0.f;
         return;
     }
-    const float x0 = o1 + (float)(i0 + (pad - 1)) * d1;
-    const float d = (val - x0) / d1;
+    const float x0 = __fadd_rn(o1, __fmul_rn((float)(i0 + (pad - 1)), d1));
+    const float d = __fdiv_rn(__fsub_rn(val, x0), d1);
     const float *m = from + (size_t)i * n1 + i0;
     if (mode == 0) {
-        *out = (float)((1.0 - (double)d) * (double)m[0] + (double)(d * m[1]));
+        *out = (float)(__dadd_rn(__dmul_rn(__dsub_rn(1.0, (double)d), (double)m[0]), (double)__fmul_rn(d, m[1])));
     } else if (mode == 1) {
-        const float xp1 = (float)((double)d + 1.0);
-        const float omx = (float)(1.0 - (double)d);
-        const float a0 = (float)(((-(1.0 / 6.0) * (double)xp1 + 1.0) * (double)xp1 - 2.0) * (double)xp1 + (4.0 / 3.0));
-        const float a1 = (float)((0.5 * (double)d - 1.0) * (double)d * (double)d + (2.0 / 3.0));
-        const float a2 = (float)((0.5 * (double)omx - 1.0) * (double)omx * (double)omx + (2.0 / 3.0));
-        const float a3 = (float)(1.0 - (double)a0 - (double)a1 - (double)a2);
-        *out = a0 * m[0] + a1 * m[1] + a2 * m[2] + a3 * m[3];
+        const double dd = d;
+        const float xp1 = (float)(dd + 1.0);
+        const float omx = (float)(1.0 - dd);
+        const double x = xp1, o = omx;
+        const float a0 = (float)__dadd_rn(__dmul_rn(__dsub_rn(__dmul_rn(__dadd_rn(__dmul_rn(-(1.0 / 6.0), x), 1.0), x), 2.0), x),
+                                          (4.0 / 3.0));
+        const float a1 = (float)__dadd_rn(__dmul_rn(__dmul_rn(__dsub_rn(__dmul_rn(0.5, dd), 1.0), dd), dd), (2.0 / 3.0));
+        const float a2 = (float)__dadd_rn(__dmul_rn(__dmul_rn(__dsub_rn(__dmul_rn(0.5, o), 1.0), o), o), (2.0 / 3.0));
+        const float a3 = (float)__dsub_rn(__dsub_rn(__dsub_rn(1.0, (double)a0), (double)a1), (double)a2);
+        float acc = __fmul_rn(a0, m[0]);
+        acc = __fadd_rn(acc, __fmul_rn(a1, m[1]));
+        acc = __fadd_rn(acc, __fmul_rn(a2, m[2]));
+        *out = __fadd_rn(acc, __fmul_rn(a3, m[3]));
     } else {
         float acc = 0.f;
         for (int k = 0; k < 6; k++) {
-            const float s1 = sinc_ref((float)((2.0 - k) + (double)d));
-            const float s2 = sinc_ref((float)(((2.0 - k) + (double)d) / 3.0));
-            acc = acc + m[k] * s1 * s2;
+            const double u = __dadd_rn(2.0 - k, (double)d);
+            const float s1 = sinc_ref((float)u);
+            const float s2 = sinc_ref((float)__ddiv_rn(u, 3.0));
+            acc = __fadd_rn(acc, __fmul_rn(__fmul_rn(m[k], s1), s2));
         }
         *out = acc;
     }

@@ -354,10 +354,11 @@ class ElasticSurvey:
     """Shots of one elastic model plus the gradient settings (order, PML,
     Revolve snapshots, FP16 checkpoints)."""
 
-    def __init__(self, shots, lpml, dx, dz, dt, order=8, fs=False, ncheck=50, fp16=False):
+    def __init__(self, shots, lpml, dx, dz, dt, order=8, fs=False, ncheck=50, fp16=False, ox=0.0, oz=0.0):
         self.shots = list(shots)
         self.lpml, self.dx, self.dz, self.dt = int(lpml), float(dx), float(dz), float(dt)
         self.order, self.fs, self.ncheck, self.fp16 = int(order), bool(fs), int(ncheck), bool(fp16)
+        self.ox, self.oz = float(ox), float(oz)  # origin of the full model
 
 
 def elastic_batch_gradient(vp, vs, rho, survey: ElasticSurvey, scale=True, wavgrad=False):
@@ -384,7 +385,8 @@ def elastic_batch_gradient(vp, vs, rho, survey: ElasticSurvey, scale=True, wavgr
         nxw = nx - sh.x0 if sh.nxw is None else int(sh.nxw)
         if sh.x0 < 0 or sh.x0 + nxw > nx:
             raise R.RsgError(f"elastic_batch_gradient: window of shot {k} outside the model")
-        m = ModelElastic2D(nxw, nz, survey.lpml, survey.dx, survey.dz, sh.x0 * survey.dx, 0.0, survey.fs)
+        m = ModelElastic2D(nxw, nz, survey.lpml, survey.dx, survey.dz, survey.ox + sh.x0 * survey.dx, survey.oz,
+                           survey.fs)
         m.Vp[:], m.Vs[:], m.R[:] = vp[:, sh.x0:sh.x0 + nxw], vs[:, sh.x0:sh.x0 + nxw], rho[:, sh.x0:sh.x0 + nxw]
         m.staggerModels()
         keep.append(m)
@@ -505,7 +507,8 @@ class FwiElastic2D:
             raise R.RsgError("FwiElastic2D: P, Ux and Uz data must be set")
         shot = ElasticShot(self.source.scoords, self.source.data, self.data[0].gcoords,
                            np.stack([d.data for d in self.data]), 0, m.nx, _field_name(self.source.field))
-        sv = ElasticSurvey([shot], m.lpml, m.dx, m.dz, self.source.dt, self.order, m.fs, self.ncheck, self.fp16)
+        sv = ElasticSurvey([shot], m.lpml, m.dx, m.dz, self.source.dt, self.order, m.fs, self.ncheck, self.fp16,
+                           m.ox, m.oz)
         mis, g0, g1, g2, outs = elastic_batch_gradient(m.Vp, m.Vs, m.R, sv, scale=True,
                                                         wavgrad=self.wavgrad is not None)
         self.misfit64 = mis

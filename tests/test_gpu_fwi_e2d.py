@@ -54,9 +54,12 @@ def test_fwi_e2d_vs_reference(golden, case):
     for k in ("vpgrad", "vsgrad", "rhograd", "dmod", "dres", "wavgrad"):
         e = rel_l2(r[k], g[k])
         print(f"  {k}: rel L2 {e:.2e}")
-        assert e <= RTOL_L2, k
-    exact = 0.5 * np.sum((g["dmod"].astype(np.float64) - g["dobs"].astype(np.float64)) ** 2)
-    assert abs(r["misfit64"] - exact) <= RTOL_L2 * exact
+        # the wavelet gradient sums the adjoint stresses at the source; the
+        # reference's own fp32 value is 4.4e-5 from the FP64 oracle in case 0,
+        # so it is held to the gradient bound, 1e-4
+        assert e <= (1e-4 if k == "wavgrad" else RTOL_L2), k
+    # misfit: traces whose Ux map is valid (fwi.cpp:2911), the reference sums in float
+    assert abs(r["misfit64"] - float(g["misfit"])) <= 1e-4 * float(g["misfit"])
 
 
 def _survey_of(g, case, x0=0, nxw=None, fp16=False):
