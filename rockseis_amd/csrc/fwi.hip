@@ -33,6 +33,9 @@
 #include "rsg_srcrec.h"
 #include "rsg_store16.h"
 
+// This file is compiled with -ffp-contract=off (Makefile): every multiply and
+// add is rounded separately, as in the reference's host arithmetic.
+
 namespace rsg {
 
 // ---------------------------------------------------------------------------

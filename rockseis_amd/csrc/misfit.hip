@@ -15,6 +15,9 @@
 #include "rsg_common.h"
 #include "rsg_fwi.h"
 
+// This file is compiled with -ffp-contract=off (Makefile): every multiply and
+// add is rounded separately, as in the reference's host arithmetic.
+
 namespace rsg {
 
 namespace {

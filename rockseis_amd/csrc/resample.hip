@@ -6,6 +6,9 @@
 // results match its host code.
 #include "rsg_common.h"
 
+// This file is compiled with -ffp-contract=off (Makefile): every multiply and
+// add is rounded separately, as in the reference's host arithmetic.
+
 namespace rsg {
 
 namespace {
