@@ -29,6 +29,38 @@
 
 namespace rsg {
 
+// Raw buffer access to one grid array: 32-bit byte offsets in a VGPR plus an
+// optional uniform offset in an SGPR, no 64-bit address arithmetic per load;
+// loads whose VGPR offset lies outside the array return 0 and never fault
+// (the out-of-range check covers the VGPR offset, so boundary tiles put the
+// whole offset there; interior tiles, always in range, use the SGPR part).
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ float ld(unsigned off, int soff) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, soff, 0));
+    }
+    __device__ __forceinline__ void st(float v, unsigned off, int soff) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, soff, 0);
+    }
+};
+// Descriptor of `bytes` bytes at p (dword 3 as gfx9 raw buffers take it).
+__device__ __forceinline__ Buf mkbuf(const float *p, unsigned bytes) {
+    return Buf{__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, (int)bytes, 0x00020000)};
+}
+
+// Row r of a thread's RZ cells, whose first row sits at byte offset cb.
+template <bool FAST>
+__device__ __forceinline__ float cell_ld(const Buf &b, unsigned cb, int r, int rowb) {
+    return FAST ? b.ld(cb, r * rowb) : b.ld(cb + (unsigned)(r * rowb), 0);
+}
+template <bool FAST>
+__device__ __forceinline__ void cell_st(const Buf &b, float v, unsigned cb, int r, int rowb) {
+    if (FAST)
+        b.st(v, cb, r * rowb);
+    else
+        b.st(v, cb + (unsigned)(r * rowb), 0);
+}
+
 constexpr int BX = 64;
 constexpr int BZ = 4;
 
@@ -83,22 +115,29 @@ template <int TZ, int HX, int HZ, bool FAST, class F>
 __device__ __forceinline__ void tile_fetch(Tile<TZ, HX, HZ> &t, F &&ld, int nx, int nz, int x0, int z0) {
     using T = Tile<TZ, HX, HZ>;
     const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x = x0 - HX + tx;
+    if (FAST) {
+        // one VGPR offset; rows k*BZ apart as uniform SGPR offsets
+        const unsigned v0 = (unsigned)(((z0 - HZ + ty) * nx + x) * 4);
 #pragma unroll
-    for (int k = 0; k < T::KR; k++) {
-        const int r = ty + k * BZ;
-        const int z = z0 - HZ + r;
-        const int x = x0 - HX + tx;
-        if (FAST) {
-            t.a[k] = ld((unsigned)(z * nx + x));
-            if (2 * HX > 0) t.b[k] = tx < 2 * HX ? ld((unsigned)(z * nx + x + BX)) : 0.f;
-        } else {
-            const int zc = clampi(z, 0, nz - 1);
-            const bool zok = z >= 0 && z < nz;
-            const float va = ld((unsigned)(zc * nx + clampi(x, 0, nx - 1)));
-            t.a[k] = (zok && x >= 0 && x < nx) ? va : 0.f;
+        for (int k = 0; k < T::KR; k++) {
+            const int soff = k * BZ * nx * 4;
+            t.a[k] = ld(v0, soff);
+            if (2 * HX > 0) t.b[k] = tx < 2 * HX ? ld(v0 + BX * 4, soff) : 0.f;
+        }
+    } else {
+        // rows outside the grid fall outside the buffer (loads return 0);
+        // columns outside it would wrap to a neighbouring row: select 0
+        const bool xa = x >= 0 && x < nx, xb = x + BX < nx;
+#pragma unroll
+        for (int k = 0; k < T::KR; k++) {
+            const int z = z0 - HZ + ty + k * BZ;
+            const unsigned v = (unsigned)((z * nx + x) * 4);
+            const float va = ld(v, 0);
+            t.a[k] = xa ? va : 0.f;
             if (2 * HX > 0) {
-                const float vb = ld((unsigned)(zc * nx + clampi(x + BX, 0, nx - 1)));
-                t.b[k] = (zok && x + BX < nx) ? vb : 0.f;
+                const float vb = tx < 2 * HX ? ld(v + BX * 4, 0) : 0.f;
+                t.b[k] = xb ? vb : 0.f;
             }
         }
     }
@@ -121,20 +160,18 @@ __device__ __forceinline__ void tile_store(const Tile<TZ, HX, HZ> &t, float *lds
 // Loaders: a plain array, or the products a*b + c*d / a*b that
 // WavesElastic2D::backwardstepVelocity differentiates (waves.cpp:1771-1775).
 struct Ld1 {
-    const float *__restrict__ g;
-    __device__ __forceinline__ float operator()(unsigned i) const { return g[i]; }
+    Buf g;
+    __device__ __forceinline__ float operator()(unsigned o, int so) const { return g.ld(o, so); }
 };
 struct Ld2 {
-    const float *__restrict__ a;
-    const float *__restrict__ b;
-    __device__ __forceinline__ float operator()(unsigned i) const { return a[i] * b[i]; }
+    Buf a, b;
+    __device__ __forceinline__ float operator()(unsigned o, int so) const { return a.ld(o, so) * b.ld(o, so); }
 };
 struct Ld4 {
-    const float *__restrict__ a;
-    const float *__restrict__ b;
-    const float *__restrict__ c;
-    const float *__restrict__ d;
-    __device__ __forceinline__ float operator()(unsigned i) const { return a[i] * b[i] + c[i] * d[i]; }
+    Buf a, b, c, d;
+    __device__ __forceinline__ float operator()(unsigned o, int so) const {
+        return a.ld(o, so) * b.ld(o, so) + c.ld(o, so) * d.ld(o, so);
+    }
 };
 
 // Derivative of an LDS tile at a cell: compile-time order in the interior,
@@ -209,23 +246,21 @@ __device__ __forceinline__ void a2d_velocity_body(const K2D &a, int x0, int z0, 
     const bool xin = FAST || x < nx;
     const size_t fo = blockIdx.y * (size_t)a.sf, xo = blockIdx.y * (size_t)a.sxs,
                  zo = blockIdx.y * (size_t)a.szs;
-    float *__restrict__ Vx = a.Vx + fo;
-    float *__restrict__ Vz = a.Vz + fo;
-    const float *__restrict__ Rx = a.Rx;
-    const float *__restrict__ Rz = a.Rz;
+    const unsigned nb = (unsigned)(nx * nz * 4);
+    const int rowb = nx * 4;
+    const unsigned cb = (unsigned)(((z0 + ty * RZ) * nx + x) * 4);
+    const Buf bVx = mkbuf(a.Vx + fo, nb), bVz = mkbuf(a.Vz + fo, nb), bRx = mkbuf(a.Rx, nb),
+              bRz = mkbuf(a.Rz, nb);
     float vx[RZ], vz[RZ], rx[RZ], rz[RZ];
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
-        const int z = z0 + ty * RZ + r;
-        const size_t p = (size_t)z * nx + x;
-        const bool in = FAST || (xin && z < nz);
-        vx[r] = in ? Vx[p] : 0.f;
-        vz[r] = in ? Vz[p] : 0.f;
-        rx[r] = in ? Rx[p] : 0.f;
-        rz[r] = in ? Rz[p] : 0.f;
+        vx[r] = cell_ld<FAST>(bVx, cb, r, rowb);
+        vz[r] = cell_ld<FAST>(bVz, cb, r, rowb);
+        rx[r] = cell_ld<FAST>(bRx, cb, r, rowb);
+        rz[r] = cell_ld<FAST>(bRz, cb, r, rowb);
     }
     Tile<TZ, H, H> tp;
-    tile_fetch<TZ, H, H, FAST>(tp, Ld1{a.P + fo}, nx, nz, x0, z0);
+    tile_fetch<TZ, H, H, FAST>(tp, Ld1{mkbuf(a.P + fo, nb)}, nx, nz, x0, z0);
     tile_store(tp, sP);
     __syncthreads();
     if (!xin) return;
@@ -239,14 +274,13 @@ __device__ __forceinline__ void a2d_velocity_body(const K2D &a, int x0, int z0, 
         const float *s = sP + (zl + H) * LD + tx + H;
         const float dPx = ddfw<ORDER, FAST>(s, 1, a.cx.c, kx, a.invdx);
         const float dPz = ddfw<ORDER, FAST>(s, LD, a.cz.c, FAST ? ORDER : kfw(z, nz, ORDER), a.invdz);
-        const size_t p = (size_t)z * nx + x;
         const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
         term<FAST>(vx[r], a.dt * rx[r], dPx, px, tb, 1, (a.psi[RSG_A2D_P_LEFT] + xo),
                    (a.psi[RSG_A2D_P_RIGHT] + xo), (size_t)z * a.lpml + px.i);
         term<FAST>(vz[r], a.dt * rz[r], dPz, pz, tb, 1, (a.psi[RSG_A2D_P_TOP] + zo),
                    (a.psi[RSG_A2D_P_BOTTOM] + zo), (size_t)pz.i * nx + x);
-        Vx[p] = vx[r];
-        Vz[p] = vz[r];
+        cell_st<FAST>(bVx, vx[r], cb, r, rowb);
+        cell_st<FAST>(bVz, vz[r], cb, r, rowb);
     }
 }
 
@@ -269,21 +303,20 @@ __device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, fl
     const bool xin = FAST || x < nx;
     const size_t fo = blockIdx.y * (size_t)a.sf, xo = blockIdx.y * (size_t)a.sxs,
                  zo = blockIdx.y * (size_t)a.szs;
-    float *__restrict__ P = a.P + fo;
-    const float *__restrict__ L = a.L;
+    const unsigned nb = (unsigned)(nx * nz * 4);
+    const int rowb = nx * 4;
+    const unsigned cb = (unsigned)(((z0 + ty * RZ) * nx + x) * 4);
+    const Buf bP = mkbuf(a.P + fo, nb), bL = mkbuf(a.L, nb);
     float pv[RZ], lv[RZ];
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
-        const int z = z0 + ty * RZ + r;
-        const size_t p = (size_t)z * nx + x;
-        const bool in = FAST || (xin && z < nz);
-        pv[r] = in ? P[p] : 0.f;
-        lv[r] = in ? L[p] : 0.f;
+        pv[r] = cell_ld<FAST>(bP, cb, r, rowb);
+        lv[r] = cell_ld<FAST>(bL, cb, r, rowb);
     }
     Tile<TZ, H, 0> tvx;
     Tile<TZ, 0, H> tvz;
-    tile_fetch<TZ, H, 0, FAST>(tvx, Ld1{a.Vx + fo}, nx, nz, x0, z0);
-    tile_fetch<TZ, 0, H, FAST>(tvz, Ld1{a.Vz + fo}, nx, nz, x0, z0);
+    tile_fetch<TZ, H, 0, FAST>(tvx, Ld1{mkbuf(a.Vx + fo, nb)}, nx, nz, x0, z0);
+    tile_fetch<TZ, 0, H, FAST>(tvz, Ld1{mkbuf(a.Vz + fo, nb)}, nx, nz, x0, z0);
     tile_store(tvx, sVx);
     tile_store(tvz, sVz);
     __syncthreads();
@@ -304,7 +337,7 @@ __device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, fl
                    (size_t)z * a.lpml + px.i);
         term<FAST>(pv[r], dtL, dVz, pz, tb, 0, (a.psi[RSG_A2D_VZZ_TOP] + zo), (a.psi[RSG_A2D_VZZ_BOTTOM] + zo),
                    (size_t)pz.i * nx + x);
-        P[(size_t)z * nx + x] = pv[r];
+        cell_st<FAST>(bP, pv[r], cb, r, rowb);
     }
 }
 
@@ -339,32 +372,32 @@ __device__ __forceinline__ void e2d_velocity_body(const K2D &a, int x0, int z0, 
     const bool xin = FAST || x < nx;
     const size_t fo = blockIdx.y * (size_t)a.sf, xo = blockIdx.y * (size_t)a.sxs,
                  zo = blockIdx.y * (size_t)a.szs;
-    float *__restrict__ Vx = a.Vx + fo;
-    float *__restrict__ Vz = a.Vz + fo;
-    const float *__restrict__ Rx = a.Rx;
-    const float *__restrict__ Rz = a.Rz;
+    const unsigned nb = (unsigned)(nx * nz * 4);
+    const int rowb = nx * 4;
+    const unsigned cb = (unsigned)(((z0 + ty * RZ) * nx + x) * 4);
+    const Buf bVx = mkbuf(a.Vx + fo, nb), bVz = mkbuf(a.Vz + fo, nb), bRx = mkbuf(a.Rx, nb),
+              bRz = mkbuf(a.Rz, nb);
     float vx[RZ], vz[RZ], rx[RZ], rz[RZ];
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
-        const int z = z0 + ty * RZ + r;
-        const size_t p = (size_t)z * nx + x;
-        const bool in = FAST || (xin && z < nz);
-        vx[r] = in ? Vx[p] : 0.f;
-        vz[r] = in ? Vz[p] : 0.f;
-        rx[r] = in ? Rx[p] : 0.f;
-        rz[r] = in ? Rz[p] : 0.f;
+        vx[r] = cell_ld<FAST>(bVx, cb, r, rowb);
+        vz[r] = cell_ld<FAST>(bVz, cb, r, rowb);
+        rx[r] = cell_ld<FAST>(bRx, cb, r, rowb);
+        rz[r] = cell_ld<FAST>(bRz, cb, r, rowb);
     }
     Tile<TZ, H, 0> txx;
     Tile<TZ, 0, H> tzz;
     Tile<TZ, H, H> txz;
+    const Buf bSxx = mkbuf(a.Sxx + fo, nb), bSzz = mkbuf(a.Szz + fo, nb), bSxz = mkbuf(a.Sxz + fo, nb);
     if (!BW) {
-        tile_fetch<TZ, H, 0, FAST>(txx, Ld1{a.Sxx + fo}, nx, nz, x0, z0);
-        tile_fetch<TZ, 0, H, FAST>(tzz, Ld1{a.Szz + fo}, nx, nz, x0, z0);
-        tile_fetch<TZ, H, H, FAST>(txz, Ld1{a.Sxz + fo}, nx, nz, x0, z0);
+        tile_fetch<TZ, H, 0, FAST>(txx, Ld1{bSxx}, nx, nz, x0, z0);
+        tile_fetch<TZ, 0, H, FAST>(tzz, Ld1{bSzz}, nx, nz, x0, z0);
+        tile_fetch<TZ, H, H, FAST>(txz, Ld1{bSxz}, nx, nz, x0, z0);
     } else {
-        tile_fetch<TZ, H, 0, FAST>(txx, Ld4{a.L2M, a.Sxx + fo, a.L, a.Szz + fo}, nx, nz, x0, z0);
-        tile_fetch<TZ, 0, H, FAST>(tzz, Ld4{a.L, a.Sxx + fo, a.L2M, a.Szz + fo}, nx, nz, x0, z0);
-        tile_fetch<TZ, H, H, FAST>(txz, Ld2{a.M, a.Sxz + fo}, nx, nz, x0, z0);
+        const Buf bL = mkbuf(a.L, nb), bL2M = mkbuf(a.L2M, nb), bM = mkbuf(a.M, nb);
+        tile_fetch<TZ, H, 0, FAST>(txx, Ld4{bL2M, bSxx, bL, bSzz}, nx, nz, x0, z0);
+        tile_fetch<TZ, 0, H, FAST>(tzz, Ld4{bL, bSxx, bL2M, bSzz}, nx, nz, x0, z0);
+        tile_fetch<TZ, H, H, FAST>(txz, Ld2{bM, bSxz}, nx, nz, x0, z0);
     }
     tile_store(txx, sXX);
     tile_store(tzz, sZZ);
@@ -385,19 +418,18 @@ __device__ __forceinline__ void e2d_velocity_body(const K2D &a, int x0, int z0, 
         const float dSxz_x = ddbw<ORDER, FAST>(sxz, 1, a.cx.c, kxb, a.invdx);
         const float dSzz = ddfw<ORDER, FAST>(sZZ + (zl + H) * LDZZ + tx, LDZZ, a.cz.c,
                                              FAST ? ORDER : kfw(z, nz, ORDER), a.invdz);
-        const size_t p = (size_t)z * nx + x;
         const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
         const size_t ix = (size_t)z * a.lpml + px.i, iz = (size_t)pz.i * nx + x;
         const float dtRx = a.dt * rx[r];
         term<FAST>(vx[r], dtRx, dSxx, px, tb, 1, (a.psi[RSG_E2D_SXX_LEFT] + xo), (a.psi[RSG_E2D_SXX_RIGHT] + xo), ix);
         term<FAST>(vx[r], dtRx, dSxz_z, pz, tb, 0, (a.psi[RSG_E2D_SXZZ_TOP] + zo),
                    (a.psi[RSG_E2D_SXZZ_BOTTOM] + zo), iz);
-        Vx[p] = vx[r];
+        cell_st<FAST>(bVx, vx[r], cb, r, rowb);
         const float dtRz = a.dt * rz[r];
         term<FAST>(vz[r], dtRz, dSxz_x, px, tb, 0, (a.psi[RSG_E2D_SXZX_LEFT] + xo),
                    (a.psi[RSG_E2D_SXZX_RIGHT] + xo), ix);
         term<FAST>(vz[r], dtRz, dSzz, pz, tb, 1, (a.psi[RSG_E2D_SZZ_TOP] + zo), (a.psi[RSG_E2D_SZZ_BOTTOM] + zo), iz);
-        Vz[p] = vz[r];
+        cell_st<FAST>(bVz, vz[r], cb, r, rowb);
     }
 }
 
@@ -427,27 +459,25 @@ __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, fl
     const bool xin = FAST || x < nx;
     const size_t fo = blockIdx.y * (size_t)a.sf, xo = blockIdx.y * (size_t)a.sxs,
                  zo = blockIdx.y * (size_t)a.szs;
-    float *__restrict__ Sxx = a.Sxx + fo;
-    float *__restrict__ Szz = a.Szz + fo;
-    float *__restrict__ Sxz = a.Sxz + fo;
+    const unsigned nb = (unsigned)(nx * nz * 4);
+    const int rowb = nx * 4;
+    const unsigned cb = (unsigned)(((z0 + ty * RZ) * nx + x) * 4);
+    const Buf bSxx = mkbuf(a.Sxx + fo, nb), bSzz = mkbuf(a.Szz + fo, nb), bSxz = mkbuf(a.Sxz + fo, nb);
     float sxx[RZ], szz[RZ], sxz[RZ], ml[RZ], ml2m[RZ], mm[RZ];
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
-        const int z = z0 + ty * RZ + r;
-        const size_t p = (size_t)z * nx + x;
-        const bool in = FAST || (xin && z < nz);
-        sxx[r] = in ? Sxx[p] : 0.f;
-        szz[r] = in ? Szz[p] : 0.f;
-        sxz[r] = in ? Sxz[p] : 0.f;
+        sxx[r] = cell_ld<FAST>(bSxx, cb, r, rowb);
+        szz[r] = cell_ld<FAST>(bSzz, cb, r, rowb);
+        sxz[r] = cell_ld<FAST>(bSxz, cb, r, rowb);
         if (!BW) {
-            ml[r] = in ? a.L[p] : 0.f;
-            ml2m[r] = in ? a.L2M[p] : 0.f;
-            mm[r] = in ? a.M[p] : 0.f;
+            ml[r] = cell_ld<FAST>(mkbuf(a.L, nb), cb, r, rowb);
+            ml2m[r] = cell_ld<FAST>(mkbuf(a.L2M, nb), cb, r, rowb);
+            mm[r] = cell_ld<FAST>(mkbuf(a.M, nb), cb, r, rowb);
         }
     }
     Tile<TZ, H, H> tvx, tvz;
-    tile_fetch<TZ, H, H, FAST>(tvx, Ld1{a.Vx + fo}, nx, nz, x0, z0);
-    tile_fetch<TZ, H, H, FAST>(tvz, Ld1{a.Vz + fo}, nx, nz, x0, z0);
+    tile_fetch<TZ, H, H, FAST>(tvx, Ld1{mkbuf(a.Vx + fo, nb)}, nx, nz, x0, z0);
+    tile_fetch<TZ, H, H, FAST>(tvz, Ld1{mkbuf(a.Vz + fo, nb)}, nx, nz, x0, z0);
     tile_store(tvx, sVx);
     tile_store(tvz, sVz);
     __syncthreads();
@@ -467,7 +497,6 @@ __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, fl
         const float dVz_z = ddbw<ORDER, FAST>(svz, LD, a.cz.c, kzb, a.invdz);
         const float dVz_x = ddfw<ORDER, FAST>(svz, 1, a.cx.c, kxf, a.invdx);
         const float dVx_z = ddfw<ORDER, FAST>(svx, LD, a.cz.c, kzf, a.invdz);
-        const size_t p = (size_t)z * nx + x;
         const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
         const size_t ix = (size_t)z * a.lpml + px.i, iz = (size_t)pz.i * nx + x;
         if (!BW) {
@@ -506,9 +535,9 @@ __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, fl
             term<FAST>(sxz[r], dt, dVz_x, px, tb, 1, (a.psi[RSG_E2D_VZX_LEFT] + xo), (a.psi[RSG_E2D_VZX_RIGHT] + xo), ix);
             term<FAST>(sxz[r], dt, dVx_z, pz, tb, 1, (a.psi[RSG_E2D_VXZ_TOP] + zo), (a.psi[RSG_E2D_VXZ_BOTTOM] + zo), iz);
         }
-        Sxx[p] = sxx[r];
-        Szz[p] = szz[r];
-        Sxz[p] = sxz[r];
+        cell_st<FAST>(bSxx, sxx[r], cb, r, rowb);
+        cell_st<FAST>(bSzz, szz[r], cb, r, rowb);
+        cell_st<FAST>(bSxz, sxz[r], cb, r, rowb);
     }
 }
 
@@ -627,6 +656,8 @@ static int check_grid2d(const rsg_grid *g, const rsg_state *s) {
         return fail(RSG_EINVAL, "grid must exceed 2*lpml in x and z");
     if (g->nx < 2 * g->order + 2 || g->nz < 2 * g->order + 2)
         return fail(RSG_EINVAL, "grid too small for the stencil order");
+    if ((long long)g->nx * g->nz * 4 >= (1LL << 31))
+        return fail(RSG_EINVAL, "2D grid larger than 2^29 cells (32-bit buffer offsets)");
     if (!s->tab) return fail(RSG_EINVAL, "PML tables not set");
     return RSG_OK;
 }
