@@ -218,6 +218,18 @@ int rsg_modelling_run(int kind, const rsg_grid *g, const rsg_state *s, const rsg
 #define RSG_FWI_FULL 2    /* run: pressure every snapinc-th step kept in HBM, in
                              place of the snapshot file                             */
 
+/* Adjoint of the acoustic 2D gradient.  REFERENCE: the reference's
+ * run_optimal / run (adjoint advanced with the forward operator, crossCorr
+ * imaging).  EXACT (no reference counterpart): the transposed discrete step
+ * (transposed staggered stencils and edge stencils, time-reversed PML,
+ * residuals injected at the receivers with weight 1) and the transposed model
+ * staggering, so vpgrad/rhograd are the gradient of the DIFFERENCE misfit of
+ * the discrete forward map and wavgrad its gradient in the source wavelet.
+ * EXACT takes RSG_FWI_REVOLVE and DIFFERENCE only; the image gradients are
+ * summed over time in FP32 per shot and mapped to (Vp, rho) in FP64. */
+#define RSG_ADJOINT_REFERENCE 0
+#define RSG_ADJOINT_EXACT 1
+
 typedef struct {
     const float *L, *Rx, *Rz, *tab; /* staggered model and PML tables (device)    */
     const float *Vp, *Rho;          /* unpadded model nz*nx (device), for crossCorr */
@@ -238,6 +250,7 @@ typedef struct {
     int misfit_type;                /* rs_fwimisfit (utils.h:26): 0 DIFFERENCE,
                                        1 CORRELATION                                 */
     const float *weight;            /* data weights like obs (setDataweight) or NULL */
+    int adjoint;                    /* RSG_ADJOINT_REFERENCE or RSG_ADJOINT_EXACT    */
 } rsg_fwi_a2d;
 
 int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, void *stream);

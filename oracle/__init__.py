@@ -137,6 +137,12 @@ def _declare(L):
         f = getattr(L, "orc_stagger_a3d" + suf)
         f.argtypes = [rp, rp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, rp, rp, rp, rp]
         f.restype = C.c_int
+        f = getattr(L, "orc_deriv_t" + suf)
+        f.argtypes = [rp, rp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ct]
+        f = getattr(L, "orc_a2d_adj_stress" + suf)
+        f.argtypes = [C.POINTER(A2D), C.POINTER(A2D), rp]
+        f = getattr(L, "orc_a2d_adj_velocity" + suf)
+        f.argtypes = [C.POINTER(A2D), C.POINTER(A2D), rp, rp]
         f = getattr(L, "orc_a2d_crosscorr" + suf)
         f.argtypes = [rp, C.c_int, rp, rp, C.c_int, rp, rp, rp, rp, C.c_int, C.c_int, ct, ct, rp, rp]
         f = getattr(L, "orc_misfit_a" + suf)

@@ -251,6 +251,131 @@ void FN(orc_a2d_stress)(FN(orc_a2d_t) *s)
     FN(upd_z2d_)(s->nx, s->nz, s->lpml, s->dt, s->tab, 0, s->df, s->P, s->L, 0, 0, s->Vzz_top, s->Vzz_bottom);
 }
 
+/* ---------------------------------------------------------------------- */
+/* Exact discrete adjoint of the acoustic 2D step (no reference            */
+/* counterpart: the reference's FwiAcoustic2D back-propagates with the     */
+/* forward operator).  Written from the forward restatement above: every   */
+/* linear map of the step is transposed as it stands, edge stencils and    */
+/* PML strips included, so <F x, y> = <x, F^T y> holds to rounding.        */
+/* ---------------------------------------------------------------------- */
+
+/* out[j] += sum_i D[i,j] u[i] for the derivative matrix D of orc_deriv
+ * (same axis, direction, order and step): row i of D holds +c/d at the
+ * cells orc_deriv adds and -c/d at those it subtracts. */
+void FN(orc_deriv_t)(const REAL *u, REAL *out, int nx, int ny, int nz,
+                     int axis, int fw, int order, REAL d)
+{
+    const REAL *coef = FN(coefs)();
+    long sxy = (long)nx * ny;
+    long s = axis == 0 ? 1 : (axis == 1 ? nx : sxy);
+    int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    if (order < 1) order = 1;
+    if (order > 8) order = 8;
+    for (int iz = 0; iz < nz; iz++)
+        for (int iy = 0; iy < ny; iy++)
+            for (int ix = 0; ix < nx; ix++) {
+                long p = iz * sxy + (long)iy * nx + ix;
+                int i = axis == 0 ? ix : (axis == 1 ? iy : iz);
+                int k = fw ? orc_kfw(i, n, order) : orc_kbw(i, n, order);
+                if (k == 0) continue;
+                const REAL *c = coef + ((k - 1) * k) / 2;
+                REAL ui = u[p] / d;
+                for (int io = 0; io < k; io++) {
+                    if (fw) {
+                        out[p + (io + 1) * s] += c[io] * ui;
+                        out[p - io * s] -= c[io] * ui;
+                    } else {
+                        out[p + io * s] += c[io] * ui;
+                        out[p - (io + 1) * s] -= c[io] * ui;
+                    }
+                }
+            }
+}
+
+/*
+ * Reverse of one derivative term of upd_x2d_ (axis 0) / upd_z2d_ (axis 2).
+ * Forward, with d the derivative, M the modulus and, in a strip cell,
+ * psi' = B*psi + A*d:
+ *     F' = F + dt*M*d                          (outside the strips)
+ *     F' = F + dt*M*(1-A-C)*d - dt*M*B*psi     (inside)
+ * Given lF = adjoint of F' and lpsi = adjoint of psi', this writes
+ * w = adjoint of d, replaces lpsi by the adjoint of the old psi and adds the
+ * gradient of M, lF * dF'/dM, to g.  psi holds the forward strip before
+ * (psi_old=1) or after (psi_old=0) the update.
+ */
+static void FN(adj_upd_2d_)(int axis, int nx, int nz, int L, REAL dt, const REAL *tab, int stag,
+                            const REAL *df, const REAL *lF, const REAL *M, const REAL *psiLo,
+                            const REAL *psiHi, int psi_old, REAL *lpsiLo, REAL *lpsiHi, REAL *w,
+                            REAL *g)
+{
+    long n = (long)nx * nz;
+    for (long p = 0; p < n; p++) {
+        w[p] = dt * M[p] * lF[p];
+        g[p] += lF[p] * dt * df[p];
+    }
+    int n1 = axis == 0 ? nz : L, n2 = axis == 0 ? L : nx;
+    for (int side = 0; side < 2; side++) {
+        const REAL *At = side ? TAB_A_R(tab, L, stag) : TAB_A_L(tab, L, stag);
+        const REAL *Bt = side ? TAB_B_R(tab, L, stag) : TAB_B_L(tab, L, stag);
+        const REAL *Ct = side ? TAB_C_R(tab, L, stag) : TAB_C_L(tab, L, stag);
+        const REAL *psi = side ? psiHi : psiLo;
+        REAL *lpsi = side ? lpsiHi : lpsiLo;
+        for (int i1 = 0; i1 < n1; i1++)
+            for (int i2 = 0; i2 < n2; i2++) {
+                long q = (long)i1 * n2 + i2, p;
+                int t;
+                if (axis == 0) {
+                    t = i2;
+                    p = (long)i1 * nx + i2 + (side ? nx - L : 0);
+                } else {
+                    t = i1;
+                    p = (long)(i1 + (side ? nz - L : 0)) * nx + i2;
+                }
+                REAL A = At[t], B = Bt[t], C = Ct[t];
+                REAL pn = psi_old ? B * psi[q] + A * df[p] : psi[q];
+                REAL lf = lF[p], lq = lpsi[q], dtM = dt * M[p];
+                w[p] = dtM * (1 - A - C) * lf + A * lq;
+                lpsi[q] = B * lq - dtM * B * lf;
+                g[p] -= lf * dt * (pn + C * df[p]);
+            }
+    }
+}
+
+/* Reverse of the stress half step.  s: forward state holding V of this
+ * step and the Vxx/Vzz strips before it (the forward velocity half step
+ * applied, the stress half step not); a: adjoint state (P = adjoint of the
+ * pressure after the step, unchanged; Vx, Vz and the Vxx/Vzz strips are
+ * carried back through the step; a->df is scratch).  gL += dJ/dL. */
+void FN(orc_a2d_adj_stress)(FN(orc_a2d_t) *s, FN(orc_a2d_t) *a, REAL *gL)
+{
+    int nx = s->nx, nz = s->nz;
+    FN(orc_deriv)(s->Vx, s->df, nx, 1, nz, 0, 0, s->order, s->dx);
+    FN(adj_upd_2d_)(0, nx, nz, s->lpml, s->dt, s->tab, 0, s->df, a->P, s->L, s->Vxx_left, s->Vxx_right, 1,
+                    a->Vxx_left, a->Vxx_right, a->df, gL);
+    FN(orc_deriv_t)(a->df, a->Vx, nx, 1, nz, 0, 0, s->order, s->dx);
+    FN(orc_deriv)(s->Vz, s->df, nx, 1, nz, 2, 0, s->order, s->dz);
+    FN(adj_upd_2d_)(2, nx, nz, s->lpml, s->dt, s->tab, 0, s->df, a->P, s->L, s->Vzz_top, s->Vzz_bottom, 1,
+                    a->Vzz_top, a->Vzz_bottom, a->df, gL);
+    FN(orc_deriv_t)(a->df, a->Vz, nx, 1, nz, 2, 0, s->order, s->dz);
+}
+
+/* Reverse of the velocity half step.  s: forward state with the pressure
+ * before the step and the P strips after the velocity half step; a: adjoint
+ * state (Vx, Vz = adjoints of the velocities after the half step, unchanged;
+ * P and the P strips carried back).  gRx, gRz += dJ/dRx, dJ/dRz. */
+void FN(orc_a2d_adj_velocity)(FN(orc_a2d_t) *s, FN(orc_a2d_t) *a, REAL *gRx, REAL *gRz)
+{
+    int nx = s->nx, nz = s->nz;
+    FN(orc_deriv)(s->P, s->df, nx, 1, nz, 0, 1, s->order, s->dx);
+    FN(adj_upd_2d_)(0, nx, nz, s->lpml, s->dt, s->tab, 1, s->df, a->Vx, s->Rx, s->P_left, s->P_right, 0,
+                    a->P_left, a->P_right, a->df, gRx);
+    FN(orc_deriv_t)(a->df, a->P, nx, 1, nz, 0, 1, s->order, s->dx);
+    FN(orc_deriv)(s->P, s->df, nx, 1, nz, 2, 1, s->order, s->dz);
+    FN(adj_upd_2d_)(2, nx, nz, s->lpml, s->dt, s->tab, 1, s->df, a->Vz, s->Rz, s->P_top, s->P_bottom, 0,
+                    a->P_top, a->P_bottom, a->df, gRz);
+    FN(orc_deriv_t)(a->df, a->P, nx, 1, nz, 2, 1, s->order, s->dz);
+}
+
 /*
  * Bilinear source injection (waves.cpp:407-497, 2085-2171).
  * F += dt*Mod*wav[it+nt*i]*xtri*ytri over the 2x2 corner cells of each

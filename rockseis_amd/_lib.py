@@ -52,7 +52,7 @@ class Fwi(C.Structure):
                 + [(n, C.c_void_p) for n in ("dmod", "dres", "wavgrad", "vpgrad", "rhograd")]
                 + [("misfit", C.POINTER(C.c_double)), ("ncheck", C.c_int), ("mode", C.c_int),
                    ("mode_used", C.POINTER(C.c_int)), ("snapinc", C.c_int), ("fp16", C.c_int),
-                   ("misfit_type", C.c_int), ("weight", C.c_void_p)])
+                   ("misfit_type", C.c_int), ("weight", C.c_void_p), ("adjoint", C.c_int)])
 
 
 class FwiA3D(C.Structure):
@@ -75,6 +75,7 @@ class FwiE2D(C.Structure):
 
 
 FWI_REVOLVE, FWI_STORE, FWI_FULL = 0, 1, 2
+ADJOINT_REFERENCE, ADJOINT_EXACT = 0, 1
 
 
 class RsgError(RuntimeError):

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "rsg_common.h"
+#include "rsg_adjoint.h"
 #include "rsg_fwi.h"
 #include "rsg_srcrec.h"
 #include "rsg_store16.h"
@@ -288,6 +289,11 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
     if (f->nx + 2 * g->lpml != g->nx || f->nz + 2 * g->lpml != g->nz)
         return fail(RSG_EINVAL, "fwi: image size must be the unpadded grid");
     if (f->mode == RSG_FWI_FULL && f->snapinc < 1) return fail(RSG_EINVAL, "fwi: snapinc must be >= 1");
+    if (f->adjoint != RSG_ADJOINT_REFERENCE && f->adjoint != RSG_ADJOINT_EXACT)
+        return fail(RSG_EINVAL, "fwi: adjoint must be RSG_ADJOINT_REFERENCE or RSG_ADJOINT_EXACT");
+    const bool exact = f->adjoint == RSG_ADJOINT_EXACT;
+    if (exact && (f->mode != RSG_FWI_REVOLVE || f->misfit_type != 0))
+        return fail(RSG_EINVAL, "fwi: the exact adjoint takes RSG_FWI_REVOLVE and the DIFFERENCE misfit");
     const bool f16 = f->fp16 != 0;
     if (f16 && 4 * ns > 65535) return fail(RSG_EINVAL, "fwi: FP16 storage takes at most 16383 shots");
     DevBuf db;
@@ -407,6 +413,27 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
                 return rc;
         }
     } else {
+        // exact adjoint: work arrays and a copy of the state before the last
+        // forward step (firsturn runs that step for the records first)
+        A2DExactBufs ex{};
+        float *turn_state = nullptr;
+        if (exact) {
+            const size_t nn = (size_t)ns * fw.n;
+            float *w = nullptr;
+            if (int rc = db.alloc(&w, 5 * nn)) return rc;
+            if (int rc = launch_zero(w, 5 * nn * sizeof(float), st)) return rc;
+            ex.W1 = w;
+            ex.W2 = w + nn;
+            ex.gL = w + 2 * nn;
+            ex.gRx = w + 3 * nn;
+            ex.gRz = w + 4 * nn;
+            if (int rc = db.alloc(&ex.gvp, fw.n)) return rc;
+            if (int rc = db.alloc(&ex.grho, fw.n)) return rc;
+            if (int rc = db.alloc(&turn_state, fw.total)) return rc;
+        }
+        auto exact_step = [&](int t) -> int {
+            return a2d_exact_step(g, &fw.st, &bw.st, &fw.b, ex, &res, &f->src, f->wavgrad, t, st);
+        };
         Revolve rev(nt, f->ncheck);
         // checkpoint = P, Vx, Vz (3*ns arrays of n), 4*ns x strips, 4*ns z strips
         float *ck = nullptr;
@@ -445,11 +472,25 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
                 for (int it = oldcapo; it < capo; it++)
                     if (int rc = fstep(it, !reverse)) return rc;
             } else if (a == Revolve::FIRSTURN) {
+                if (exact)
+                    RSG_HIP(hipMemcpyAsync(turn_state, fw.base, fw.total * sizeof(float), hipMemcpyDeviceToDevice,
+                                           st));
                 if (int rc = fstep(capo, true)) return rc;
                 if (int rc = first_turn()) return rc;
-                if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n, nullptr)) return rc;
+                if (exact) {
+                    RSG_HIP(hipMemcpyAsync(fw.base, turn_state, fw.total * sizeof(float), hipMemcpyDeviceToDevice,
+                                           st));
+                    if (int rc = exact_step(capo)) return rc;
+                } else if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n, nullptr)) {
+                    return rc;
+                }
                 reverse = true;
             } else if (a == Revolve::YOUTURN) {
+                // the state is discarded after a turn: Revolve restores next
+                if (exact) {
+                    if (int rc = exact_step(capo)) return rc;
+                    continue;
+                }
                 if (int rc = bstep()) return rc;
                 if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n, nullptr)) return rc;
             } else if (a == Revolve::TAKESHOT) {
@@ -472,6 +513,10 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
                 return fail(RSG_EINVAL, "fwi: Revolve returned an error");
             }
         }
+        if (exact)
+            if (int rc = a2d_exact_model_grad(g, f->nx, f->nz, f->Vp, f->Rho, ex, ns, fw.n, f->vpgrad, f->rhograd,
+                                              st))
+                return rc;
     }
     double m = 0.0;
     RSG_HIP(hipMemcpyAsync(&m, misfit_d, sizeof(double), hipMemcpyDeviceToHost, st));

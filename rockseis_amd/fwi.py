@@ -11,8 +11,14 @@ setFp16Storage(True) stores those snapshots / checkpoints as FP16 with a
 power-of-two scale per stored array.  The per-shot work is one call of
 rsg_fwi_a2d_gradient.
 
+setAdjoint("exact") replaces the reference's adjoint (the forward operator
+plus crossCorr) by the exact discrete adjoint of the step and of the model
+staggering: the gradients are then those of the DIFFERENCE misfit of the
+discrete forward map (Revolve only), and the wavelet gradient dJ/d(wavelet).
+
 AcousticFwiMisfit is the torch.autograd.Function (Vp, rho) -> misfit summed
-over shots, whose backward returns the reference's Vp and rho gradients.
+over shots, whose backward returns the survey's gradients (the reference's by
+default, the exact ones with Survey(adjoint="exact")).
 """
 from __future__ import annotations
 
@@ -27,6 +33,13 @@ from .core import (GMAP, SMAP, Data2D, Data3D, ModelAcoustic2D, ModelAcoustic3D,
 
 DIFFERENCE, CORRELATION = 0, 1  # rs_fwimisfit (utils.h:26)
 _MODES = {"revolve": R.FWI_REVOLVE, "store": R.FWI_STORE, "full": R.FWI_FULL}
+_ADJOINTS = {"reference": R.ADJOINT_REFERENCE, "exact": R.ADJOINT_EXACT}
+
+
+def _adjoint_kind(kind):
+    if kind not in _ADJOINTS:
+        raise R.RsgError('adjoint must be "reference" or "exact"')
+    return _ADJOINTS[kind]
 
 
 class Image2D:
@@ -62,6 +75,7 @@ class FwiAcoustic2D:
         self.misfit_type = DIFFERENCE
         self.dataweight = None
         self.fp16 = False
+        self.adjoint = "reference"
 
     def setDatamodP(self, d):
         self.datamodP = d
@@ -94,6 +108,12 @@ class FwiAcoustic2D:
 
     def setFp16Storage(self, v):
         self.fp16 = bool(v)
+
+    def setAdjoint(self, kind):
+        """"reference" (the reference's gradient) or "exact" (the exact
+        discrete adjoint; run_optimal and DIFFERENCE only)."""
+        _adjoint_kind(kind)
+        self.adjoint = kind
 
     def getMisfit(self):
         return self.misfit
@@ -143,6 +163,7 @@ class FwiAcoustic2D:
         f.ncheck, f.mode = self.ncheck, mode
         f.snapinc, f.fp16 = self.snapinc, int(self.fp16)
         f.misfit_type = self.misfit_type
+        f.adjoint = _adjoint_kind(self.adjoint)
         if self.dataweight is not None:
             wt = torch.from_numpy(np.ascontiguousarray(self.dataweight.data, np.float32)).to(dev)
             f.weight = wt.data_ptr()
@@ -178,11 +199,14 @@ class Survey:
     """Shots on one model grid plus the gradient settings: mode "revolve"
     (run_optimal), "store" (run_optimal's gradient, every forward pressure in
     HBM) or "full" (run with snapshots every snapinc steps); fp16 stores the
-    snapshots / checkpoints as scaled FP16."""
+    snapshots / checkpoints as scaled FP16; adjoint "reference" or "exact"
+    (exact discrete adjoint, mode "revolve")."""
 
     def __init__(self, shots, lpml, dx, dz, dt, order=4, fs=False, ncheck=50, mode="revolve",
-                 batched=True, snapinc=1, fp16=False):
+                 batched=True, snapinc=1, fp16=False, adjoint="reference"):
         self.shots = list(shots)
+        _adjoint_kind(adjoint)
+        self.adjoint = adjoint
         self.snapinc, self.fp16 = int(snapinc), bool(fp16)
         self.batched = bool(batched)
         self.lpml, self.dx, self.dz, self.dt = int(lpml), float(dx), float(dz), float(dt)
@@ -208,6 +232,7 @@ def shot_gradient(vp, rho, survey: Survey, shot: Shot):
     m.staggerModels()
     fwi = FwiAcoustic2D(m, src, obs, survey.order, survey.snapinc)
     fwi.setFp16Storage(survey.fp16)
+    fwi.setAdjoint(getattr(survey, "adjoint", "reference"))
     vg, rg = Image2D(m), Image2D(m)
     fwi.setVpgrad(vg)
     fwi.setRhograd(rg)
@@ -285,6 +310,7 @@ def batch_gradient(vp, rho, survey: Survey, mode=None):
     mode = mode or survey.mode
     f.ncheck, f.mode = survey.ncheck, _MODES[mode]
     f.snapinc, f.fp16 = survey.snapinc, int(survey.fp16)
+    f.adjoint = _adjoint_kind(getattr(survey, "adjoint", "reference"))
     R.check(R.lib().rsg_fwi_a2d_gradient_batch(C.byref(g), C.byref(f), ns, _stream()),
             "rsg_fwi_a2d_gradient_batch")
     survey.mode_used = {v: k for k, v in _MODES.items()}[used.value]
@@ -294,7 +320,8 @@ def batch_gradient(vp, rho, survey: Survey, mode=None):
 class AcousticFwiMisfit(torch.autograd.Function):
     """(Vp, rho) -> sum over shots of the DIFFERENCE misfit.  The backward
     pass returns the reference's FwiAcoustic2D gradients (crossCorr of the
-    forward pressure with the residual-driven wavefield)."""
+    forward pressure with the residual-driven wavefield) or, for a survey with
+    adjoint="exact", the exact gradient of the misfit."""
 
     @staticmethod
     def forward(ctx, vp: torch.Tensor, rho: torch.Tensor, survey: Survey):
