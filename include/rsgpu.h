@@ -106,7 +106,7 @@ int rsg_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 /* Zero with a fill kernel (safe inside hipGraph capture). */
 int rsg_zero(void *ptr, size_t bytes, void *stream);
 
-/* Launch-shape knobs: "rz2d" (rows per thread of the 2D tiles: 2, 4, 8),
+/* Launch-shape knobs: "rz2d" (rows per thread of the 2D tiles: 1, 2, 4, 8),
  * "zc3d" (z planes per 3D block, 0 = automatic).  Results do not depend on
  * them; rockseis_amd.autotune picks them per workload. */
 int rsg_set_tuning(const char *key, int value);

@@ -1,7 +1,7 @@
 """Launch-shape autotuner for the FD step kernels.
 
 The fused 2D kernels give each thread ``rz2d`` rows of a 64-wide tile
-(2, 4 or 8); the 3D kernels march ``zc3d`` z planes per block (0 = chosen
+(1, 2, 4 or 8); the 3D kernels march ``zc3d`` z planes per block (0 = chosen
 from the grid so that a launch has >= 2048 blocks).  Results do not depend on
 either knob (every cell is computed by the same arithmetic), only the time
 does, so the tuner times a velocity + stress half-step pair for every
@@ -20,7 +20,7 @@ import torch
 
 from . import _lib as R
 
-CANDIDATES = {"rz2d": (2, 4, 8), "zc3d": (0, 2, 4, 8, 16, 32)}
+CANDIDATES = {"rz2d": (1, 2, 4, 8), "zc3d": (0, 2, 4, 8, 16, 32)}
 _cache: dict = {}
 
 

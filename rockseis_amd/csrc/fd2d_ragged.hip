@@ -15,7 +15,7 @@ int ragged2d_create(int kind, int nshot, const rsg_grid *grids, const rsg_state 
     if (kind != RSG_KIND_A2D && kind != RSG_KIND_E2D) return fail(RSG_EINVAL, "ragged2d: kind must be A2D or E2D");
     if (nshot < 1) return fail(RSG_EINVAL, "ragged2d: nshot must be >= 1");
     std::vector<K2D> desc(nshot);
-    std::vector<int> tile0(3 * (size_t)(nshot + 1), 0);
+    std::vector<int> tile0(4 * (size_t)(nshot + 1), 0);
     for (int k = 0; k < nshot; k++) {
         const rsg_grid *g = grids + k;
         const rsg_state *s = states + k;
@@ -27,8 +27,8 @@ int ragged2d_create(int kind, int nshot, const rsg_grid *grids, const rsg_state 
                                      !s->M || !s->Rx || !s->Rz))
             return fail(RSG_EINVAL, "ragged2d: e2d shot with a null array");
         desc[k] = make_k2d(g, s, BZ * 2);
-        for (int r = 0; r < 3; r++) {
-            const int tz = BZ * (2 << r);
+        for (int r = 0; r < 4; r++) {
+            const int tz = BZ * (1 << r);
             const long nt = (long)desc[k].ntx * ((g->nz + tz - 1) / tz);
             const long next = tile0[r * (nshot + 1) + k] + nt;
             if (next > (1L << 30)) return fail(RSG_EINVAL, "ragged2d: too many tiles");
@@ -39,7 +39,7 @@ int ragged2d_create(int kind, int nshot, const rsg_grid *grids, const rsg_state 
     p.kind = kind;
     p.nshot = nshot;
     p.order = grids[0].order;
-    for (int r = 0; r < 3; r++) p.total[r] = tile0[r * (nshot + 1) + nshot];
+    for (int r = 0; r < 4; r++) p.total[r] = tile0[r * (nshot + 1) + nshot];
     if (hipMalloc((void **)&p.d_desc, sizeof(K2D) * nshot) != hipSuccess)
         return fail(RSG_ENOMEM, "ragged2d: device allocation failed");
     if (hipMalloc((void **)&p.d_tile0, sizeof(int) * tile0.size()) != hipSuccess) {
@@ -70,7 +70,7 @@ int ragged2d_step(const Ragged2D *p, int which, hipStream_t stream) {
     if (which < 0 || which > 3 || (p->kind == RSG_KIND_A2D && which > 1))
         return fail(RSG_EINVAL, "ragged2d: which must be 0..1 (A2D) or 0..3 (E2D)");
     const int rz = rz2d();
-    const int r = rz == 2 ? 0 : (rz == 4 ? 1 : 2);
+    const int r = rz == 1 ? 0 : (rz == 2 ? 1 : (rz == 4 ? 2 : 3));
     const int *t0 = p->d_tile0 + r * (p->nshot + 1);
     const unsigned nb = 8u * (unsigned)((p->total[r] + 7) / 8);
     const K2D *d = p->d_desc;

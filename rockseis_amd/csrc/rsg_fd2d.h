@@ -692,10 +692,10 @@ static K2D make_k2d(const rsg_grid *g, const rsg_state *s, int tz) {
     return a;
 }
 
-// Rows per thread from the tuning table (2, 4 or 8).
+// Rows per thread from the tuning table (1, 2, 4 or 8).
 static int rz2d() {
     const int v = tuning().rz2d;
-    return v <= 2 ? 2 : (v <= 4 ? 4 : 8);
+    return v <= 1 ? 1 : (v <= 2 ? 2 : (v <= 4 ? 4 : 8));
 }
 
 static unsigned nblocks(const K2D &a) { return 8u * (unsigned)((a.ntiles + 7) / 8); }
@@ -715,7 +715,9 @@ static unsigned nblocks(const K2D &a) { return 8u * (unsigned)((a.ntiles + 7) / 
 #define RSG_DISPATCH(LAUNCH)                      \
     do {                                          \
         const int rz_ = rz2d();                   \
-        if (rz_ == 2) {                           \
+        if (rz_ == 1) {                           \
+            RSG_ORDER_CASES(LAUNCH, 1)            \
+        } else if (rz_ == 2) {                    \
             RSG_ORDER_CASES(LAUNCH, 2)            \
         } else if (rz_ == 4) {                    \
             RSG_ORDER_CASES(LAUNCH, 4)            \

@@ -9,12 +9,12 @@ namespace rsg {
 struct K2D;
 
 // Device copy of every shot's kernel arguments and of the tile prefix sums
-// for the three tile heights (rows per thread 2, 4, 8).
+// for the four tile heights (rows per thread 1, 2, 4, 8).
 struct Ragged2D {
     int kind = 0, nshot = 0, order = 0;
     K2D *d_desc = nullptr;
-    int *d_tile0 = nullptr;  // 3 x (nshot + 1)
-    int total[3] = {0, 0, 0};
+    int *d_tile0 = nullptr;  // 4 x (nshot + 1)
+    int total[4] = {0, 0, 0, 0};
 };
 
 int ragged2d_create(int kind, int nshot, const rsg_grid *grids, const rsg_state *states, Ragged2D *out);

@@ -132,7 +132,8 @@ int rsg_set_tuning(const char *key, int value) {
     if (!key) return fail(RSG_EINVAL, "null key");
     std::string k(key);
     if (k == "rz2d") {
-        if (value != 2 && value != 4 && value != 8) return fail(RSG_EINVAL, "rz2d must be 2, 4 or 8");
+        if (value != 1 && value != 2 && value != 4 && value != 8)
+            return fail(RSG_EINVAL, "rz2d must be 1, 2, 4 or 8");
         tuning().rz2d = value;
     } else if (k == "zc3d") {
         if (value < 0 || value > 4096) return fail(RSG_EINVAL, "zc3d must be 0..4096");

@@ -44,7 +44,7 @@ def test_ragged_equals_single_shot(kind, order):
     from rockseis_amd import _lib as R, core
     old = R.get_tuning("rz2d")
     try:
-        for rz in (2, 4, 8):
+        for rz in (1, 2, 4, 8):
             R.set_tuning("rz2d", rz)
             wa, ma, fields = _setup(core, kind, 11)
             wb, mb, _ = _setup(core, kind, 11)

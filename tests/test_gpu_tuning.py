@@ -52,7 +52,7 @@ def test_rz2d_values_give_identical_fields():
     old = R.get_tuning("rz2d")
     try:
         out = {}
-        for v in (2, 4, 8):
+        for v in (1, 2, 4, 8):
             R.set_tuning("rz2d", v)
             out[v] = _a2d(core, 8)
         for v in (4, 8):
