@@ -79,6 +79,14 @@ def pmc_dispatch():
     print("pmc dispatch done", flush=True)
 
 
+_T0 = time.perf_counter()
+
+
+def _progress(msg):
+    """One progress line per finished stage, on stderr (stdout holds the JSON)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def build_workload():
     import numpy as np
     from rockseis_amd import core, synth
@@ -187,6 +195,7 @@ def main():
     # launch shape picked by the autotuner (timed on a scratch state of this grid)
     from rockseis_amd import autotune
     tuned = autotune.tune_waves(mod.waves, cfg["order"], nshot=1)
+    _progress(f"modelling workload built, tile rows per thread {tuned['rz2d']}")
     if W:
         mod.launch(0, W)
     torch.cuda.synchronize()
@@ -196,6 +205,7 @@ def main():
     mod.launch(0, K)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    _progress(f"timed {K} modelling steps: {wall:.2f} s")
 
     cells = cfg["nx_pml"] * cfg["nz_pml"]
     value = cells * K / wall / 1e9
@@ -267,6 +277,7 @@ def fwi_leg(n_plain):
     shots = [fwi.Shot(np.float32([[125.0 + 250.0 * k, 10.0]]), wav, gc, zero) for k in range(nshot)]
     sv = fwi.Survey(shots, lpml, dx, dx, dt, order=order, ncheck=50, mode="store")
     _, _, _, dobs = fwi.batch_gradient(vtrue, rtrue, sv)         # observed data
+    _progress("fwi: observed data modelled")
     dobs = dobs.cpu().numpy()
     for k, sh in enumerate(shots):
         sh.dobs = np.ascontiguousarray(dobs[k])
@@ -275,6 +286,7 @@ def fwi_leg(n_plain):
     misfit_b, vg, rg, _ = fwi.batch_gradient(vp0, rho0, sv)
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
+    _progress(f"fwi: batched gradient of {nshot} shots {tb:.2f} s")
     mode_b = sv.mode_used
     plain = fwi.Survey(shots, lpml, dx, dx, dt, order=order, ncheck=50, mode="revolve")
     n_plain = max(1, min(n_plain, nshot))
@@ -287,6 +299,7 @@ def fwi_leg(n_plain):
         gv_p += a
     torch.cuda.synchronize()
     tp = time.perf_counter() - t0
+    _progress(f"fwi: plain gradient of {n_plain} shots {tp:.2f} s")
     # the batched gradient of the first n_plain shots must equal the plain one
     check = fwi.Survey(shots[:n_plain], lpml, dx, dx, dt, order=order, ncheck=50, mode="store")
     m_chk, vg_chk, _, _ = fwi.batch_gradient(vp0, rho0, check)
@@ -337,6 +350,7 @@ def a3d_leg():
     mod.launch(0, nt)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    _progress(f"a3d: {nt} modelling steps {wall:.2f} s")
     mod.finish()
     N = m.getNx_pml() * m.getNy_pml() * m.getNz_pml()
     g, st = mod._args[0], mod._args[1]
@@ -377,6 +391,7 @@ def a3d_leg():
     f.run_optimal()
     torch.cuda.synchronize()
     tg = time.perf_counter() - t0
+    _progress(f"a3d: FWI gradient {tg:.2f} s")
     return {"workload": "acoustic 3D, 400^3 at 10 m (440^3 padded), order 4, nt 2500, 200x200 receivers",
             "modelling_gcell_per_s": round(N * nt / wall / 1e9, 3), "modelling_s": round(wall, 3),
             "kernels": kt, "fwi_gradient_s": round(tg, 2),
@@ -413,11 +428,13 @@ def e2dfwi_leg(nshot, n_plain=4):
         shots.append(fwi.ElasticShot(sc, wav, gc, np.zeros((3, w, nt), np.float32), x0, w, "P"))
         cells += (w + 2 * lpml) * (nz + 2 * lpml)
     sv = fwi.ElasticSurvey(shots, lpml, dx, dx, dt, order, False, 50, fp16=True)
+    _progress(f"e2dfwi: {nshot} windows set up ({cells} padded cells)")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     mis, l, u, r, _ = fwi.elastic_batch_gradient(vp, vs, rho, sv, scale=False)
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
+    _progress(f"e2dfwi: batched gradients {tb:.2f} s")
     del l, u, r
     torch.cuda.empty_cache()
     n_plain = max(1, min(n_plain, nshot))
@@ -427,6 +444,7 @@ def e2dfwi_leg(nshot, n_plain=4):
         fwi.elastic_batch_gradient(vp, vs, rho, one, scale=False)
     torch.cuda.synchronize()
     tp = time.perf_counter() - t0
+    _progress(f"e2dfwi: plain gradients of {n_plain} windows {tp:.2f} s")
     steps = nt + _numforw(nt, 50)  # forward steps incl. recomputation + adjoint steps
     return {"workload": f"{nshot} elastic windows of 600-2000 x 600 cells (+lpml 30) from a 6000x600 model at "
                         "5 m, order 8, nt 4000, FP16 Revolve (50)",
