@@ -6,7 +6,11 @@
 // of z planes.  The z-derivative taps of every column live in a register
 // queue of 2*ORDER values that shifts by one plane per step (one new global
 // load per cell), the x- and y-derivative taps come from the current plane
-// staged in LDS with its halo.
+// staged in LDS with its halo.  The march is software-pipelined: the next
+// plane's staging loads, queue entry and own cells are issued before the
+// current plane is computed, so memory latency is paid once per chunk rather
+// than once per plane.  Raw buffer loads with per-plane descriptors and fixed
+// 32-bit in-plane offsets keep the address arithmetic out of the loop.
 #include "rsg_common.h"
 
 namespace rsg {
@@ -53,22 +57,6 @@ __device__ __forceinline__ void term3(float &F, float dtM, float d, int side, in
     }
 }
 
-// Stage plane z rows [y0-HY, y0+T3Y+HY) x cols [x0-HX, x0+B3X+HX) of g.
-template <int HX, int HY>
-__device__ __forceinline__ void load_plane(float *lds, const float *__restrict__ g, int nx, int ny,
-                                           int nz, int x0, int y0, int z) {
-    constexpr int LD = B3X + 2 * HX, N = LD * (T3Y + 2 * HY);
-    const bool zin = z >= 0 && z < nz;
-    const size_t base = (size_t)z * nx * ny;
-    for (int i = threadIdx.x + threadIdx.y * B3X; i < N; i += B3X * B3Y) {
-        const int r = i / LD, c = i - r * LD;
-        const int y = y0 - HY + r, x = x0 - HX + c;
-        float v = 0.f;
-        if (zin && x >= 0 && x < nx && y >= 0 && y < ny) v = g[base + (size_t)y * nx + x];
-        lds[i] = v;
-    }
-}
-
 // z-derivative from a queue q[j] = f(z + off + j): both the forward
 // (off = 1-H) and the backward (off = -H) staggered forms reduce to
 // sum_io c_io (q[H+io] - q[H-1-io]).
@@ -90,10 +78,75 @@ __device__ __forceinline__ float dq_k(const float (&q)[2 * H], int k, float invd
     return acc * invd;
 }
 
-__device__ __forceinline__ float ld_or0(const float *g, int x, int y, int z, int nx, int ny, int nz) {
-    return (z >= 0 && z < nz) ? g[((size_t)z * ny + y) * nx + x] : 0.f;
+// Byte offset past every plane: raw buffer loads there return 0 and stores
+// are dropped, so out-of-grid lanes and halo cells need no branches.
+constexpr unsigned kOut = 0xFFFFFF00u;
+
+// One plane of a field as a raw buffer (32-bit in-plane byte offsets).
+__device__ __forceinline__ Buf plane_buf(const float *f, int z, size_t nxy) {
+    return mkbuf(f + (size_t)z * nxy, (unsigned)(nxy * 4));
 }
 
+// Staging of the (T3Y+2HY) x (64+2HX) window of one plane.  Each thread's
+// element list (in-plane offsets, kOut outside the grid) is fixed for the
+// whole z march; fetch issues the loads into registers, store writes them to
+// LDS, so the next plane's loads are in flight while this plane computes.
+template <int HX, int HY>
+struct Plane {
+    static constexpr int LD = B3X + 2 * HX, N = LD * (T3Y + 2 * HY), K = (N + B3X * B3Y - 1) / (B3X * B3Y);
+    unsigned off[K];
+    float v[K];
+    __device__ __forceinline__ void init(int nx, int ny, int x0, int y0) {
+        const int tid = threadIdx.x + threadIdx.y * B3X;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = tid + k * B3X * B3Y;
+            const int r = i / LD, c = i - r * LD;
+            const int y = y0 - HY + r, x = x0 - HX + c;
+            off[k] = (i < N && x >= 0 && x < nx && y >= 0 && y < ny) ? (unsigned)((y * nx + x) * 4) : kOut;
+        }
+    }
+    __device__ __forceinline__ void fetch(const Buf &b) {
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = b.ld(off[k], 0);
+    }
+    __device__ __forceinline__ void store(float *lds) const {
+        const int tid = threadIdx.x + threadIdx.y * B3X;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (N % (B3X * B3Y) == 0 || tid + k * B3X * B3Y < N) lds[tid + k * B3X * B3Y] = v[k];
+    }
+};
+
+// Per-thread column set-up shared by both half steps: in-plane offsets of the
+// thread's R3Y cells (kOut outside the grid) and their PML positions and
+// edge stencil orders in x and y, fixed for the whole z march.
+struct Cols {
+    unsigned oc[R3Y];
+    bool ok[R3Y];
+    int sy[R3Y], iy[R3Y], ky[R3Y];
+    int sx, ix, kx;
+};
+template <bool FW>
+__device__ __forceinline__ Cols make_cols(const K3D &a, int x, int y0, int order) {
+    Cols c;
+    side_of(x, a.nx, a.lpml, c.sx, c.ix);
+    c.kx = FW ? kfw(x, a.nx, order) : kbw(x, a.nx, order);
+#pragma unroll
+    for (int r = 0; r < R3Y; r++) {
+        const int y = y0 + threadIdx.y * R3Y + r;
+        c.ok[r] = x < a.nx && y < a.ny;
+        c.oc[r] = c.ok[r] ? (unsigned)((y * a.nx + x) * 4) : kOut;
+        side_of(y, a.ny, a.lpml, c.sy[r], c.iy[r]);
+        c.ky[r] = FW ? kfw(y, a.ny, order) : kbw(y, a.ny, order);
+    }
+    return c;
+}
+
+// Velocity half step.  Per plane z of the block's z chunk: the staged plane
+// of P (x and y taps) and the queue of P along z for the thread's own cells
+// (z taps); the next plane of P, the next queue entry and the next own cells
+// (Vx, Vy, Vz, Rx, Ry, Rz) are loaded while this plane computes.
 template <int ORDER>
 __global__ __launch_bounds__(B3X *B3Y) void a3d_velocity_kernel(const K3D a, int zc) {
     constexpr int H = ORDER, LD = B3X + 2 * H;
@@ -101,61 +154,105 @@ __global__ __launch_bounds__(B3X *B3Y) void a3d_velocity_kernel(const K3D a, int
     const int x0 = blockIdx.x * B3X, y0 = blockIdx.y * T3Y;
     const int zb = blockIdx.z * zc, ze = min(zb + zc, a.nz);
     const int tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    const bool xin = x < a.nx;
+    const size_t nxy = (size_t)a.nx * a.ny;
     const Tab3 tb{a.tab, a.lpml};
-    int sx, ix;
-    side_of(x, a.nx, a.lpml, sx, ix);
-    const int kx = kfw(x, a.nx, ORDER);
+    const Cols c = make_cols<true>(a, x, y0, ORDER);
+    Plane<H, H> pl;
+    pl.init(a.nx, a.ny, x0, y0);
     float q[R3Y][2 * H];
 #pragma unroll
-    for (int r = 0; r < R3Y; r++) {
-        const int y = y0 + ty * R3Y + r;
+    for (int j = 0; j < 2 * H; j++) {
+        const int zz = zb + 1 - H + j;
+        const bool in = zz >= 0 && zz < a.nz;
+        const Buf b = plane_buf(a.P, in ? zz : 0, nxy);
 #pragma unroll
-        for (int j = 0; j < 2 * H; j++)
-            q[r][j] = (xin && y < a.ny) ? ld_or0(a.P, x, y, zb + 1 - H + j, a.nx, a.ny, a.nz) : 0.f;
+        for (int r = 0; r < R3Y; r++) q[r][j] = in ? b.ld(c.oc[r], 0) : 0.f;
+    }
+    float vx[R3Y], vy[R3Y], vz[R3Y], rx[R3Y], ry[R3Y], rz[R3Y];
+    auto own = [&](int z) {
+        const Buf bVx = plane_buf(a.Vx, z, nxy), bVy = plane_buf(a.Vy, z, nxy), bVz = plane_buf(a.Vz, z, nxy),
+                  bRx = plane_buf(a.Rx, z, nxy), bRy = plane_buf(a.Ry, z, nxy), bRz = plane_buf(a.Rz, z, nxy);
+#pragma unroll
+        for (int r = 0; r < R3Y; r++) {
+            vx[r] = bVx.ld(c.oc[r], 0);
+            vy[r] = bVy.ld(c.oc[r], 0);
+            vz[r] = bVz.ld(c.oc[r], 0);
+            rx[r] = bRx.ld(c.oc[r], 0);
+            ry[r] = bRy.ld(c.oc[r], 0);
+            rz[r] = bRz.ld(c.oc[r], 0);
+        }
+    };
+    if (zb < ze) {
+        pl.fetch(plane_buf(a.P, zb, nxy));
+        own(zb);
     }
     for (int z = zb; z < ze; z++) {
+        // zero, opaque to the compiler: keeps the plane's LDS addresses inside
+        // the loop instead of hoisted, each holding a VGPR for the whole march
+        int o = 0;
+        asm volatile("" : "+v"(o));
+        float *const sp = sP + o;
+        pl.store(sp);
+        float cvx[R3Y], cvy[R3Y], cvz[R3Y], crx[R3Y], cry[R3Y], crz[R3Y], qn[R3Y];
+#pragma unroll
+        for (int r = 0; r < R3Y; r++) {
+            cvx[r] = vx[r];
+            cvy[r] = vy[r];
+            cvz[r] = vz[r];
+            crx[r] = rx[r];
+            cry[r] = ry[r];
+            crz[r] = rz[r];
+        }
         __syncthreads();
-        load_plane<H, H>(sP, a.P, a.nx, a.ny, a.nz, x0, y0, z);
-        __syncthreads();
+        // loads of the next plane, in flight during this plane's compute
+        if (z + 1 < ze) {
+            pl.fetch(plane_buf(a.P, z + 1, nxy));
+            own(z + 1);
+        }
+        {
+            const int zq = z + 1 + H;
+            const bool in = zq < a.nz;
+            const Buf b = plane_buf(a.P, in ? zq : 0, nxy);
+#pragma unroll
+            for (int r = 0; r < R3Y; r++) qn[r] = in ? b.ld(c.oc[r], 0) : 0.f;
+        }
         int sz, iz;
         side_of(z, a.nz, a.lpml, sz, iz);
         const int kz = kfw(z, a.nz, ORDER);
+        const Buf bVx = plane_buf(a.Vx, z, nxy), bVy = plane_buf(a.Vy, z, nxy), bVz = plane_buf(a.Vz, z, nxy);
 #pragma unroll
         for (int r = 0; r < R3Y; r++) {
             const int yl = ty * R3Y + r, y = y0 + yl;
-            if (xin && y < a.ny) {
-                const float *s = sP + (yl + H) * LD + tx + H;
-                float dPx = dfw<ORDER>(s, 1, a.cx.c + cind(ORDER));
-                if (kx != ORDER) dPx = dfw_k(s, 1, kx, a.invdx);
-                const int ky = kfw(y, a.ny, ORDER);
-                float dPy = dfw<ORDER>(s, LD, a.cy.c + cind(ORDER));
-                if (ky != ORDER) dPy = dfw_k(s, LD, ky, a.invdy);
-                float dPz = dq<H>(q[r], a.cz.c + cind(ORDER));
-                if (kz != ORDER) dPz = dq_k<H>(q[r], kz, a.invdz);
-                int sy, iy;
-                side_of(y, a.ny, a.lpml, sy, iy);
-                const size_t p = ((size_t)z * a.ny + y) * a.nx + x;
-                float vx = a.Vx[p];
-                term3(vx, a.dt * a.Rx[p], dPx, sx, ix, tb, 1, a.psi[RSG_A3D_P_LEFT],
-                      a.psi[RSG_A3D_P_RIGHT], ((size_t)z * a.ny + y) * a.lpml + ix);
-                a.Vx[p] = vx;
-                float vy = a.Vy[p];
-                term3(vy, a.dt * a.Ry[p], dPy, sy, iy, tb, 1, a.psi[RSG_A3D_P_FRONT],
-                      a.psi[RSG_A3D_P_BACK], ((size_t)z * a.lpml + iy) * a.nx + x);
-                a.Vy[p] = vy;
-                float vz = a.Vz[p];
-                term3(vz, a.dt * a.Rz[p], dPz, sz, iz, tb, 1, a.psi[RSG_A3D_P_TOP],
-                      a.psi[RSG_A3D_P_BOTTOM], ((size_t)iz * a.ny + y) * a.nx + x);
-                a.Vz[p] = vz;
+            const float *s = sp + (yl + H) * LD + tx + H;
+            float dPx = dfw<ORDER>(s, 1, a.cx.c + cind(ORDER));
+            if (c.kx != ORDER) dPx = dfw_k(s, 1, c.kx, a.invdx);
+            float dPy = dfw<ORDER>(s, LD, a.cy.c + cind(ORDER));
+            if (c.ky[r] != ORDER) dPy = dfw_k(s, LD, c.ky[r], a.invdy);
+            float dPz = dq<H>(q[r], a.cz.c + cind(ORDER));
+            if (kz != ORDER) dPz = dq_k<H>(q[r], kz, a.invdz);
+            if (c.ok[r]) {
+                const int sx = c.sx, ix = c.ix, sy = c.sy[r], iy = c.iy[r];
+                term3(cvx[r], a.dt * crx[r], dPx, sx, ix, tb, 1, a.psi[RSG_A3D_P_LEFT], a.psi[RSG_A3D_P_RIGHT],
+                      ((size_t)z * a.ny + y) * a.lpml + ix);
+                term3(cvy[r], a.dt * cry[r], dPy, sy, iy, tb, 1, a.psi[RSG_A3D_P_FRONT], a.psi[RSG_A3D_P_BACK],
+                      ((size_t)z * a.lpml + iy) * a.nx + x);
+                term3(cvz[r], a.dt * crz[r], dPz, sz, iz, tb, 1, a.psi[RSG_A3D_P_TOP], a.psi[RSG_A3D_P_BOTTOM],
+                      ((size_t)iz * a.ny + y) * a.nx + x);
             }
+            bVx.st(cvx[r], c.oc[r], 0);
+            bVy.st(cvy[r], c.oc[r], 0);
+            bVz.st(cvz[r], c.oc[r], 0);
 #pragma unroll
             for (int j = 0; j < 2 * H - 1; j++) q[r][j] = q[r][j + 1];
-            q[r][2 * H - 1] = (xin && y < a.ny) ? ld_or0(a.P, x, y, z + 1 + H, a.nx, a.ny, a.nz) : 0.f;
+            q[r][2 * H - 1] = qn[r];
         }
+        __syncthreads();  // the next plane's staging overwrites the LDS
     }
 }
 
+// Stress half step: staged planes of Vx (x taps) and Vy (y taps), the queue
+// of Vz along z, own cells P and L; the next planes, queue entry and own
+// cells are loaded while this plane computes.
 template <int ORDER>
 __global__ __launch_bounds__(B3X *B3Y) void a3d_stress_kernel(const K3D a, int zc) {
     constexpr int H = ORDER, LDX = B3X + 2 * H, LDY = B3X;
@@ -164,57 +261,92 @@ __global__ __launch_bounds__(B3X *B3Y) void a3d_stress_kernel(const K3D a, int z
     const int x0 = blockIdx.x * B3X, y0 = blockIdx.y * T3Y;
     const int zb = blockIdx.z * zc, ze = min(zb + zc, a.nz);
     const int tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
-    const bool xin = x < a.nx;
+    const size_t nxy = (size_t)a.nx * a.ny;
     const Tab3 tb{a.tab, a.lpml};
-    int sx, ix;
-    side_of(x, a.nx, a.lpml, sx, ix);
-    const int kx = kbw(x, a.nx, ORDER);
+    const Cols c = make_cols<false>(a, x, y0, ORDER);
+    Plane<H, 0> px;
+    Plane<0, H> py;
+    px.init(a.nx, a.ny, x0, y0);
+    py.init(a.nx, a.ny, x0, y0);
     float q[R3Y][2 * H];
 #pragma unroll
-    for (int r = 0; r < R3Y; r++) {
-        const int y = y0 + ty * R3Y + r;
+    for (int j = 0; j < 2 * H; j++) {
+        const int zz = zb - H + j;
+        const bool in = zz >= 0 && zz < a.nz;
+        const Buf b = plane_buf(a.Vz, in ? zz : 0, nxy);
 #pragma unroll
-        for (int j = 0; j < 2 * H; j++)
-            q[r][j] = (xin && y < a.ny) ? ld_or0(a.Vz, x, y, zb - H + j, a.nx, a.ny, a.nz) : 0.f;
+        for (int r = 0; r < R3Y; r++) q[r][j] = in ? b.ld(c.oc[r], 0) : 0.f;
+    }
+    float pv[R3Y], lv[R3Y];
+    auto own = [&](int z) {
+        const Buf bP = plane_buf(a.P, z, nxy), bL = plane_buf(a.L, z, nxy);
+#pragma unroll
+        for (int r = 0; r < R3Y; r++) {
+            pv[r] = bP.ld(c.oc[r], 0);
+            lv[r] = bL.ld(c.oc[r], 0);
+        }
+    };
+    if (zb < ze) {
+        px.fetch(plane_buf(a.Vx, zb, nxy));
+        py.fetch(plane_buf(a.Vy, zb, nxy));
+        own(zb);
     }
     for (int z = zb; z < ze; z++) {
+        int o = 0;  // as in the velocity kernel
+        asm volatile("" : "+v"(o));
+        float *const svx = sVx + o, *const svy = sVy + o;
+        px.store(svx);
+        py.store(svy);
+        float cp[R3Y], cl[R3Y], qn[R3Y];
+#pragma unroll
+        for (int r = 0; r < R3Y; r++) {
+            cp[r] = pv[r];
+            cl[r] = lv[r];
+        }
         __syncthreads();
-        load_plane<H, 0>(sVx, a.Vx, a.nx, a.ny, a.nz, x0, y0, z);
-        load_plane<0, H>(sVy, a.Vy, a.nx, a.ny, a.nz, x0, y0, z);
-        __syncthreads();
+        if (z + 1 < ze) {
+            px.fetch(plane_buf(a.Vx, z + 1, nxy));
+            py.fetch(plane_buf(a.Vy, z + 1, nxy));
+            own(z + 1);
+        }
+        {
+            const int zq = z + H;
+            const bool in = zq < a.nz;
+            const Buf b = plane_buf(a.Vz, in ? zq : 0, nxy);
+#pragma unroll
+            for (int r = 0; r < R3Y; r++) qn[r] = in ? b.ld(c.oc[r], 0) : 0.f;
+        }
         int sz, iz;
         side_of(z, a.nz, a.lpml, sz, iz);
         const int kz = kbw(z, a.nz, ORDER);
+        const Buf bP = plane_buf(a.P, z, nxy);
 #pragma unroll
         for (int r = 0; r < R3Y; r++) {
             const int yl = ty * R3Y + r, y = y0 + yl;
-            if (xin && y < a.ny) {
-                const float *sx_ = sVx + yl * LDX + tx + H;
-                float dVx = dbw<ORDER>(sx_, 1, a.cx.c + cind(ORDER));
-                if (kx != ORDER) dVx = dbw_k(sx_, 1, kx, a.invdx);
-                const float *sy_ = sVy + (yl + H) * LDY + tx;
-                const int ky = kbw(y, a.ny, ORDER);
-                float dVy = dbw<ORDER>(sy_, LDY, a.cy.c + cind(ORDER));
-                if (ky != ORDER) dVy = dbw_k(sy_, LDY, ky, a.invdy);
-                float dVz = dq<H>(q[r], a.cz.c + cind(ORDER));
-                if (kz != ORDER) dVz = dq_k<H>(q[r], kz, a.invdz);
-                int sy, iy;
-                side_of(y, a.ny, a.lpml, sy, iy);
-                const size_t p = ((size_t)z * a.ny + y) * a.nx + x;
-                const float dtL = a.dt * a.L[p];
-                float P = a.P[p];
-                term3(P, dtL, dVx, sx, ix, tb, 0, a.psi[RSG_A3D_VXX_LEFT], a.psi[RSG_A3D_VXX_RIGHT],
+            const float *sx_ = svx + yl * LDX + tx + H;
+            float dVx = dbw<ORDER>(sx_, 1, a.cx.c + cind(ORDER));
+            if (c.kx != ORDER) dVx = dbw_k(sx_, 1, c.kx, a.invdx);
+            const float *sy_ = svy + (yl + H) * LDY + tx;
+            float dVy = dbw<ORDER>(sy_, LDY, a.cy.c + cind(ORDER));
+            if (c.ky[r] != ORDER) dVy = dbw_k(sy_, LDY, c.ky[r], a.invdy);
+            float dVz = dq<H>(q[r], a.cz.c + cind(ORDER));
+            if (kz != ORDER) dVz = dq_k<H>(q[r], kz, a.invdz);
+            if (c.ok[r]) {
+                const int sx = c.sx, ix = c.ix, sy = c.sy[r], iy = c.iy[r];
+                const float dtL = a.dt * cl[r];
+                term3(cp[r], dtL, dVx, sx, ix, tb, 0, a.psi[RSG_A3D_VXX_LEFT], a.psi[RSG_A3D_VXX_RIGHT],
                       ((size_t)z * a.ny + y) * a.lpml + ix);
-                term3(P, dtL, dVy, sy, iy, tb, 0, a.psi[RSG_A3D_VYY_FRONT], a.psi[RSG_A3D_VYY_BACK],
+                term3(cp[r], dtL, dVy, sy, iy, tb, 0, a.psi[RSG_A3D_VYY_FRONT], a.psi[RSG_A3D_VYY_BACK],
                       ((size_t)z * a.lpml + iy) * a.nx + x);
-                term3(P, dtL, dVz, sz, iz, tb, 0, a.psi[RSG_A3D_VZZ_TOP], a.psi[RSG_A3D_VZZ_BOTTOM],
+                term3(cp[r], dtL, dVz, sz, iz, tb, 0, a.psi[RSG_A3D_VZZ_TOP], a.psi[RSG_A3D_VZZ_BOTTOM],
                       ((size_t)iz * a.ny + y) * a.nx + x);
-                a.P[p] = P;
             }
+            bP.st(cp[r], c.oc[r], 0);
 #pragma unroll
             for (int j = 0; j < 2 * H - 1; j++) q[r][j] = q[r][j + 1];
-            q[r][2 * H - 1] = (xin && y < a.ny) ? ld_or0(a.Vz, x, y, z + H, a.nx, a.ny, a.nz) : 0.f;
+            q[r][2 * H - 1] = qn[r];
         }
+        __syncthreads();  // the next planes' staging overwrites the LDS
     }
 }
 
@@ -225,6 +357,8 @@ static int check_grid3d(const rsg_grid *g, const rsg_state *s) {
         return fail(RSG_EINVAL, "grid must exceed 2*lpml on every axis");
     if (g->nx < 2 * g->order + 2 || g->ny < 2 * g->order + 2 || g->nz < 2 * g->order + 2)
         return fail(RSG_EINVAL, "grid too small for the stencil order");
+    if ((long long)g->nx * g->ny * 4 >= (1LL << 31))
+        return fail(RSG_EINVAL, "3D plane larger than 2^29 cells (32-bit in-plane offsets)");
     if (!s->tab || !s->P || !s->Vx || !s->Vy || !s->Vz) return fail(RSG_EINVAL, "a3d: null array");
     return RSG_OK;
 }
@@ -255,13 +389,15 @@ static K3D make_k3d(const rsg_grid *g, const rsg_state *s) {
     return a;
 }
 
-// z-chunk length: enough blocks to fill 256 CUs several times over while
-// keeping the queue warm-up (2*ORDER planes) small against the chunk.
+// z-chunk length: short chunks give many blocks (the pipelined march keeps
+// 2-3 blocks per CU resident, so the grid must fill the CUs many times over
+// to balance), long ones amortize the queue warm-up (2*ORDER planes).
+// Measured on MI355X at 440^3: 8 and 16 planes are best, 64 costs 25%.
 static int zchunk(const rsg_grid *g) {
     if (tuning().zc3d > 0) return tuning().zc3d;
     const long cols = (long)((g->nx + B3X - 1) / B3X) * ((g->ny + T3Y - 1) / T3Y);
-    int zc = 64;
-    while (zc > 16 && cols * ((g->nz + zc - 1) / zc) < 2048) zc /= 2;
+    int zc = 16;
+    while (zc > 8 && cols * ((g->nz + zc - 1) / zc) < 16384) zc /= 2;
     return zc;
 }
 

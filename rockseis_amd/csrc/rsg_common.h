@@ -28,6 +28,25 @@ int check_hip(hipError_t e, const char *what);
         if (e_ != hipSuccess) return ::rsg::check_hip(e_, what);                \
     } while (0)
 
+// Raw buffer access to one grid array: 32-bit byte offsets in a VGPR plus an
+// optional uniform offset in an SGPR, no 64-bit address arithmetic per load;
+// loads whose VGPR offset lies outside the array return 0 and never fault
+// (the out-of-range check covers the VGPR offset, so boundary tiles put the
+// whole offset there; interior tiles, always in range, use the SGPR part).
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ float ld(unsigned off, int soff) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, soff, 0));
+    }
+    __device__ __forceinline__ void st(float v, unsigned off, int soff) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, soff, 0);
+    }
+};
+// Descriptor of `bytes` bytes at p (dword 3 as gfx9 raw buffers take it).
+__device__ __forceinline__ Buf mkbuf(const float *p, unsigned bytes) {
+    return Buf{__builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, (int)bytes, 0x00020000)};
+}
+
 // ---- optimised staggered coefficients (der.cpp:3-53) -----------------------
 // Set for stencil order k starts at k(k-1)/2.  The reference stores double
 // literals into T, except w11 which is the float literal 1.0029f.
