@@ -105,3 +105,44 @@ extern "C" int rsg_e2d_backward_velocity(const rsg_grid *g, const rsg_state *s, 
 extern "C" int rsg_e2d_backward_stress(const rsg_grid *g, const rsg_state *s, void *stream) {
     return rsg_e2d_stress_batch(g, s, &kOneShot, 1, stream);
 }
+
+namespace rsg {
+
+// Stress half step of a single-shot modelling step with the step's source
+// injection and velocity gathers fused in (io), at `rz` rows per thread (the
+// tile height the receiver tables were built for).
+int stress_fused(int kind, const rsg_grid *g, const rsg_state *s, const FusedIO &io, int rz, hipStream_t st) {
+    if (int rc = check_grid2d(g, s)) return rc;
+    if (kind != RSG_KIND_A2D && kind != RSG_KIND_E2D) return fail(RSG_EINVAL, "fused stress: 2D kinds only");
+    if (kind == RSG_KIND_A2D && io.nrec) return fail(RSG_EINVAL, "fused stress: gathers are elastic only");
+#define FG(KERNEL, RZ)                                                                                 \
+    do {                                                                                               \
+        K2D a = make_k2d(g, s, BZ * RZ);                                                               \
+        a.io = io;                                                                                     \
+        KERNEL<<<dim3(nblocks(a), 1), dim3(BX, BZ), 0, st>>>(a);                                       \
+    } while (0)
+#define A_(O, RZ) FG((a2d_stress_kernel<O, RZ>), RZ)
+#define E_(O, RZ) FG((e2d_stress_kernel<O, RZ, false>), RZ)
+#define BY_RZ(L)                    \
+    switch (rz) {                   \
+    case 1: RSG_ORDER_CASES(L, 1) break; \
+    case 2: RSG_ORDER_CASES(L, 2) break; \
+    case 4: RSG_ORDER_CASES(L, 4) break; \
+    default: RSG_ORDER_CASES(L, 8) break; \
+    }
+    if (kind == RSG_KIND_A2D) {
+        BY_RZ(A_)
+    } else {
+        BY_RZ(E_)
+    }
+#undef BY_RZ
+#undef E_
+#undef A_
+#undef FG
+    RSG_LAUNCH_CHECK("fused stress kernel");
+    return RSG_OK;
+}
+
+int tile_rows() { return rz2d(); }
+
+}  // namespace rsg

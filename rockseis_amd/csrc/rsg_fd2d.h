@@ -45,6 +45,34 @@ __device__ __forceinline__ void cell_st(const Buf &b, float v, unsigned cb, int 
 constexpr int BX = 64;
 constexpr int BZ = 4;
 
+// Source injection and velocity gathers done by the stress kernel itself
+// (insertSource / recordData of the same time step, waves.cpp:407-595,
+// 2085-2288), so a modelling step is two launches: each stress-type source
+// cell is updated by the thread that owns it, right after its stress update,
+// traces in order (the reference's serial order); each velocity receiver is
+// gathered by one thread of the block whose tile holds its base cell, from
+// that block's LDS tiles of Vx and Vz (the velocities of the step are final
+// and the tiles carry a halo on both axes).
+constexpr int kFusedSrc = 4, kFusedRec = 2;
+struct FusedIO {
+    int nsrc;                           // traces injected (0: none)
+    int sx[kFusedSrc], sz[kFusedSrc];   // padded base cell; sx < 0: trace skipped
+    float wx[kFusedSrc][2], wz[kFusedSrc][2];
+    const float *wav;                   // wavelets, src_nt samples per trace
+    int src_nt;
+    float scale;                        // dt (acoustic: times L) or -dt (elastic explosive)
+    int nrec;                           // velocity receiver sets gathered (elastic only)
+    int rvz[kFusedRec];                 // 0: Vx, 1: Vz
+    const int *rmap[kFusedRec];
+    const float *rshift[kFusedRec];
+    float *rdata[kFusedRec];
+    int rnt[kFusedRec];
+    const int *rtile[kFusedRec];        // receivers of tile t: rlist[rtile[t] .. rtile[t+1])
+    const int *rlist[kFusedRec];
+    const int *it_dev;                  // time index *it_dev + it, or it when null
+    int it;
+};
+
 struct K2D {
     int nx, nz, lpml, fs;
     int ntx, ntiles;  // tiles along x, total tiles
@@ -55,7 +83,57 @@ struct K2D {
     const float *L, *L2M, *M, *Rx, *Rz;
     float *psi[RSG_MAX_STRIPS];
     Coefs cx, cz;
+    FusedIO io;  // stress kernels of single-shot modelling runs only
 };
+
+// Injection of the fused sources, run by each thread after it has stored
+// its updated stress cells (column x, rows zt .. zt+RZ-1 at byte offset cb,
+// rows rowb apart): F += ((scale * modulus) * w) * xtri * ytri at every
+// source corner among them, traces in order (the cell was written by this
+// same thread, so reading it back is ordered).  Mod null: modulus 1; F2 set:
+// the same term into F2 (elastic explosive source, Sxx and Szz).
+template <int RZ, bool FAST>
+__device__ __forceinline__ void fused_inject(const FusedIO &io, const Buf &bF, const Buf *bF2, const Buf *bMod,
+                                          int x, int zt, int nz, unsigned cb, int rowb, int it) {
+    for (int i = 0; i < io.nsrc; i++) {
+        const int i2 = x - io.sx[i];
+        if (io.sx[i] < 0 || i2 < 0 || i2 > 1) continue;
+        const float w = io.wav[(size_t)i * io.src_nt + it];
+        for (int i1 = 0; i1 < 2; i1++) {
+            const int r = io.sz[i] + i1 - zt;
+            if (r < 0 || r >= RZ || zt + r >= nz) continue;
+            const float m = bMod ? io.scale * cell_ld<FAST>(*bMod, cb, r, rowb) : io.scale;
+            const float v = m * w * io.wx[i][i2] * io.wz[i][i1];
+            cell_st<FAST>(bF, cell_ld<FAST>(bF, cb, r, rowb) + v, cb, r, rowb);
+            if (bF2) cell_st<FAST>(*bF2, cell_ld<FAST>(*bF2, cb, r, rowb) + v, cb, r, rowb);
+        }
+    }
+}
+
+// Velocity gathers of the tile (tile index t, origin x0, z0) from the LDS
+// tiles sVx, sVz (origin x0 - H, z0 - H, row length LD): acc = data, then
+// acc += xtri * ytri * V over the 2x2 cells in the reference's corner order.
+__device__ __forceinline__ void fused_record(const FusedIO &io, int t, int x0, int z0, int H, int LD, int lpml,
+                                             const float *sVx, const float *sVz, int it) {
+    const int tid = threadIdx.x + threadIdx.y * BX;
+    for (int r = 0; r < io.nrec; r++) {
+        const int b = io.rtile[r][t], e = io.rtile[r][t + 1];
+        const float *V = io.rvz[r] ? sVz : sVx;
+        for (int k = b + tid; k < e; k += BX * BZ) {
+            const int i = io.rlist[r][k];
+            const int px = lpml + io.rmap[r][2 * i] - x0 + H, pz = lpml + io.rmap[r][2 * i + 1] - z0 + H;
+            const float sx = io.rshift[r][2 * i], sz = io.rshift[r][2 * i + 1];
+            const float xtri[2] = {1 - sx, sx}, ytri[2] = {1 - sz, sz};
+            float *d = io.rdata[r] + (size_t)i * io.rnt[r] + it;
+            float acc = *d;
+#pragma unroll
+            for (int i1 = 0; i1 < 2; i1++)
+#pragma unroll
+                for (int i2 = 0; i2 < 2; i2++) acc += xtri[i2] * ytri[i1] * V[(pz + i1) * LD + px + i2];
+            *d = acc;
+        }
+    }
+}
 
 // Tile handled by this block: XCD-contiguous numbering of a 1D grid.
 __device__ __forceinline__ bool tile_of(const K2D &a, int TZ, int &x0, int &z0) {
@@ -302,6 +380,7 @@ __device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, fl
     tile_store(tvz, sVz);
     __syncthreads();
     if (!xin) return;
+    const int it = a.io.nsrc ? (a.io.it_dev ? *a.io.it_dev + a.io.it : a.io.it) : 0;
     const Tab tb{a.tab, a.lpml};
     const PmlPos px = FAST ? PmlPos{-1, 0} : pml_pos(x, nx, a.lpml);
     const int kx = FAST ? ORDER : kbw(x, nx, ORDER);
@@ -320,6 +399,7 @@ __device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, fl
                    (size_t)pz.i * nx + x);
         cell_st<FAST>(bP, pv[r], cb, r, rowb);
     }
+    if (a.io.nsrc) fused_inject<RZ, FAST>(a.io, bP, nullptr, &bL, x, z0 + ty * RZ, nz, cb, rowb, it);
 }
 
 template <int ORDER, int RZ>
@@ -462,6 +542,9 @@ __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, fl
     tile_store(tvx, sVx);
     tile_store(tvz, sVz);
     __syncthreads();
+    const int it = (a.io.nsrc || a.io.nrec) ? (a.io.it_dev ? *a.io.it_dev + a.io.it : a.io.it) : 0;
+    if (!BW && a.io.nrec)
+        fused_record(a.io, (z0 / TZ) * a.ntx + x0 / BX, x0, z0, H, LD, a.lpml, sVx, sVz, it);
     if (!xin) return;
     const Tab tb{a.tab, a.lpml};
     const PmlPos px = FAST ? PmlPos{-1, 0} : pml_pos(x, nx, a.lpml);
@@ -520,6 +603,8 @@ __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, fl
         cell_st<FAST>(bSzz, szz[r], cb, r, rowb);
         cell_st<FAST>(bSxz, sxz[r], cb, r, rowb);
     }
+    // explosive source: Sxx, Szz -= dt * w * xtri * ytri
+    if (!BW && a.io.nsrc) fused_inject<RZ, FAST>(a.io, bSxx, &bSzz, nullptr, x, z0 + ty * RZ, nz, cb, rowb, it);
 }
 
 template <int ORDER, int RZ, bool BW>
@@ -670,6 +755,7 @@ static K2D make_k2d(const rsg_grid *g, const rsg_state *s, int tz) {
     for (int i = 0; i < RSG_MAX_STRIPS; i++) a.psi[i] = s->psi[i];
     a.cx = scaled_coefs(g->dx);
     a.cz = scaled_coefs(g->dz);
+    a.io = FusedIO{};
     return a;
 }
 

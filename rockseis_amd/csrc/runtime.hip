@@ -138,6 +138,9 @@ int rsg_set_tuning(const char *key, int value) {
     } else if (k == "zc3d") {
         if (value < 0 || value > 4096) return fail(RSG_EINVAL, "zc3d must be 0..4096");
         tuning().zc3d = value;
+    } else if (k == "fuse_io") {
+        if (value != 0 && value != 1) return fail(RSG_EINVAL, "fuse_io must be 0 or 1");
+        tuning().fuse_io = value;
     } else {
         return fail(RSG_EINVAL, "unknown tuning key " + k);
     }
@@ -151,6 +154,8 @@ int rsg_get_tuning(const char *key, int *value) {
         *value = tuning().rz2d;
     else if (k == "zc3d")
         *value = tuning().zc3d;
+    else if (k == "fuse_io")
+        *value = tuning().fuse_io;
     else
         return fail(RSG_EINVAL, "unknown tuning key " + k);
     return RSG_OK;
