@@ -53,6 +53,10 @@ def parse():
     ap.add_argument("--e2dfwi-shots", type=int, default=128)
     ap.add_argument("--fwi-plain-shots", type=int, default=8,
                     help="shots timed on the plain per-shot path (shots/s is per shot)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the roofline kernel timing (20 modelling steps for a non-zero state, then "
+                         "--kernel-reps back-to-back launches of each half-step kernel): the launches a "
+                         "rocprofv3 --kernel-trace --stats run of this command summarizes")
     ap.add_argument("--pmc-dispatch", action="store_true",
                     help="no timing: 100 modelling steps, 10 launches of each half-step kernel and 5 "
                          "calibration copies (torch.neg of 256 MiB), for rocprofv3 --pmc passes")
@@ -62,8 +66,10 @@ def parse():
 def pmc_dispatch():
     import torch
     from rockseis_amd import _lib as R
+    from rockseis_amd import autotune
     mod, cfg, _ = build_workload()
     mod.prepare()
+    autotune.tune_waves(mod.waves, cfg["order"], nshot=1)  # the bench's tile shape
     mod.launch(0, 100)
     torch.cuda.synchronize()
     g, s = mod._args[0], mod._args[1]
@@ -77,6 +83,19 @@ def pmc_dispatch():
         torch.neg(x, out=y)
     torch.cuda.synchronize()
     print("pmc dispatch done", flush=True)
+
+
+def roofline_only(reps):
+    import torch
+    from rockseis_amd import autotune
+    mod, cfg, _ = build_workload()
+    mod.prepare()
+    tuned = autotune.tune_waves(mod.waves, cfg["order"], nshot=1)
+    mod.launch(0, 20)  # a non-zero state; runs this short keep the separate I/O kernels
+    torch.cuda.synchronize()
+    t_vel, t_str = kernel_time_ms(mod, 0, reps), kernel_time_ms(mod, 1, reps)
+    return {"rows_per_thread": tuned["rz2d"], "kernel_reps": reps,
+            "e2d_velocity_kernel_us": round(t_vel * 1e3, 2), "e2d_stress_kernel_us": round(t_str * 1e3, 2)}
 
 
 _T0 = time.perf_counter()
@@ -186,6 +205,9 @@ def main():
     torch.cuda.set_device(0)
     if args.pmc_dispatch:
         pmc_dispatch()
+        return
+    if args.roofline_only:
+        print(json.dumps(roofline_only(args.kernel_reps)), flush=True)
         return
     mod, cfg, inputs = build_workload()
     K = min(args.steps, cfg["nt"])
