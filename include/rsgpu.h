@@ -248,12 +248,21 @@ typedef struct {
     int fp16;                       /* store snapshots / checkpoints as FP16 with a
                                        power-of-two scale per stored array          */
     int misfit_type;                /* rs_fwimisfit (utils.h:26): 0 DIFFERENCE,
-                                       1 CORRELATION                                 */
+                                       1 CORRELATION, 2 ADAPTIVE_GAUSS,
+                                       3 ADAPTIVE_LINEAR                             */
     const float *weight;            /* data weights like obs (setDataweight) or NULL */
     int adjoint;                    /* RSG_ADJOINT_REFERENCE or RSG_ADJOINT_EXACT    */
 } rsg_fwi_a2d;
 
 int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, void *stream);
+/* ---- misfit and residuals alone (FwiAcoustic2D/3D::computeMisfit and
+ * computeResiduals, fwi.cpp:574-1027, 1468-1900): type as misfit_type above,
+ * mod/rec/wei/res device arrays [nshot][ntr][nt], map [nshot][ntr][dim]
+ * receiver map (traces with a negative entry count in no misfit), scaling
+ * 1 (P) or -1 (VX/VZ data); *misfit (host, optional) = the misfit summed in
+ * double over the shots. */
+int rsg_misfit_residuals(int type, const float *mod, const float *rec, const float *wei, float *res, const int *map,
+                         int dim, int ntr, int nt, int nshot, int scaling, double *misfit, void *stream);
 /* nshot shots on one grid in one set of launches per step (the reference
  * runs one shot per MPI worker, inversion.cpp:332-600, and stacks the shot
  * gradients, image.cpp:533-600): traces of shot k follow shot k-1 in
@@ -276,7 +285,8 @@ typedef struct {
     double *misfit;                      /* out (host)                               */
     int ncheck;                          /* Revolve snapshots                        */
     int fp16;                            /* FP16 checkpoints                         */
-    int misfit_type;                     /* 0 DIFFERENCE, 1 CORRELATION              */
+    int misfit_type;                     /* 0 DIFFERENCE, 1 CORRELATION, 2 ADAPTIVE_GAUSS,
+                                            3 ADAPTIVE_LINEAR                         */
     const float *weight;                 /* data weights like obs, or NULL           */
 } rsg_fwi_a3d;
 

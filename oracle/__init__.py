@@ -137,6 +137,10 @@ def _declare(L):
         f = getattr(L, "orc_stagger_a3d" + suf)
         f.argtypes = [rp, rp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, rp, rp, rp, rp]
         f.restype = C.c_int
+        f = getattr(L, "orc_conv" + suf)
+        f.argtypes = [C.c_int, C.c_int, rp, C.c_int, C.c_int, rp, C.c_int, C.c_int, rp]
+        f = getattr(L, "orc_stoep" + suf)
+        f.argtypes = [C.c_int, rp, rp, rp, rp]
         f = getattr(L, "orc_deriv_t" + suf)
         f.argtypes = [rp, rp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ct]
         f = getattr(L, "orc_a2d_adj_stress" + suf)

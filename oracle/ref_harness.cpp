@@ -327,7 +327,7 @@ static float fwi_a2d(const float *vp, const float *rho, int nx, int nz, int lpml
         fwi->setSnapinc(snapinc);
         fwi->setSnapfile(snapfile);
     }
-    fwi->setMisfit_type(misfit_type == 1 ? CORRELATION : DIFFERENCE);
+    fwi->setMisfit_type((rs_fwimisfit)misfit_type);  // DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS, ADAPTIVE_LINEAR
     if (weight) fwi->setDataweight(mk(weight));
     if (ncheck > 0)
         fwi->run_optimal();

@@ -1083,9 +1083,263 @@ void FN(orc_interp)(int mode, const REAL *from, int ntr, int n1, REAL d1, REAL o
  * rounded to T, zero norms replaced by 1); traces with valid maps only;
  * misfit accumulated in T.  wei may be NULL.
  */
+/* ---------------------------------------------------------------------- */
+/* ADAPTIVE misfits (fwi.cpp:152-186 stoep, 188-378 convolve, 385-395 xcor, */
+/* 397-420 gauss / linear; misfit 647-736, residuals 851-1014)              */
+/* ---------------------------------------------------------------------- */
+
+/* z = x convolved with y on the index ranges [ifx, ifx+lx), [ify, ify+ly),
+ * output [ifz, ifz+lz): Hale's pairwise scheme as in Fwi::convolve -- zero
+ * off the left, the rolling-on outputs two at a time walking j down, the
+ * middle outputs two at a time walking j up (a leftover term when the
+ * overlap is odd), the rolling-off outputs two at a time, zero off the right;
+ * single outputs where a phase has an odd count.  Every sum accumulates its
+ * terms in that order, so float results equal the reference's.  Reads the
+ * scheme makes past an array end and never uses are not made here. */
+static void FN(conv_)(int lx, int ifx, const REAL *x, int ly, int ify, const REAL *y, int lz, int ifz, REAL *z)
+{
+    int ilx = ifx + lx - 1, ily = ify + ly - 1, ilz = ifz + lz - 1;
+    int i, j, ilow, ihigh, jlow, jhigh, t;
+    REAL sa, sb, xa, xb, ya, yb;
+    if (lx > ly) {
+        const REAL *tp;
+        t = ifx; ifx = ify; ify = t;
+        t = ilx; ilx = ily; ily = t;
+        t = lx; lx = ly; ly = t;
+        tp = x; x = y; y = tp;
+    }
+    x -= ifx;
+    y -= ify;
+    z -= ifz;
+    /* off left */
+    ilow = ifz;
+    ihigh = ify + ifx - 1;
+    if (ihigh > ilz) ihigh = ilz;
+    for (i = ilow; i <= ihigh; ++i) z[i] = 0.0;
+    /* rolling on: one single output first if the overlap count is even */
+    if (i < ify + ilx && i <= ilz) {
+        jlow = ifx;
+        jhigh = i - ify;
+        if ((jhigh - jlow) % 2) {
+            sa = 0.0;
+            for (j = jlow; j <= jhigh; ++j) sa += x[j] * y[i - j];
+            z[i++] = sa;
+        }
+    }
+    ilow = i;
+    ihigh = ilx + ify - 1;
+    if (ihigh > ilz) ihigh = ilz;
+    jlow = ifx;
+    jhigh = ilow - ify;
+    for (i = ilow; i < ihigh; i += 2, jhigh += 2) {
+        sa = sb = 0.0;
+        xb = x[jhigh + 1];
+        yb = 0.0;
+        for (j = jhigh; j >= jlow; j -= 2) {
+            sa += xb * yb;
+            ya = y[i - j];
+            sb += xb * ya;
+            xa = x[j];
+            sa += xa * ya;
+            yb = y[i + 1 - j];
+            sb += xa * yb;
+            if (j - 1 >= jlow) xb = x[j - 1];
+        }
+        z[i] = sa;
+        z[i + 1] = sb;
+    }
+    if (i == ihigh) {
+        jlow = ifx;
+        jhigh = i - ify;
+        sa = 0.0;
+        for (j = jlow; j <= jhigh; ++j) sa += x[j] * y[i - j];
+        z[i++] = sa;
+    }
+    /* middle: full overlap */
+    ilow = i;
+    ihigh = ily + ifx;
+    if (ihigh > ilz) ihigh = ilz;
+    jlow = ifx;
+    jhigh = ilx;
+    for (i = ilow; i < ihigh; i += 2) {
+        sa = sb = 0.0;
+        yb = y[i + 1 - jlow];
+        xa = x[jlow];
+        for (j = jlow; j < jhigh; j += 2) {
+            sb += xa * yb;
+            ya = y[i - j];
+            sa += xa * ya;
+            xb = x[j + 1];
+            sb += xb * ya;
+            yb = y[i - 1 - j];
+            sa += xb * yb;
+            if (j + 2 <= jhigh) xa = x[j + 2];
+        }
+        if ((jhigh - jlow) % 2) {
+            z[i] = sa;
+            z[i + 1] = sb;
+        } else {
+            z[i] = sa + x[jhigh] * y[i - jhigh];
+            z[i + 1] = sb + x[jhigh] * y[i + 1 - jhigh];
+        }
+    }
+    if (i == ihigh) {
+        sa = 0.0;
+        for (j = jlow; j <= jhigh; ++j) sa += x[j] * y[i - j];
+        z[i++] = sa;
+    }
+    /* rolling off: one single output first if the overlap count is odd */
+    if (i <= ily + ilx && i <= ilz) {
+        jlow = i - ily;
+        jhigh = ilx;
+        if (!((jhigh - jlow) % 2)) {
+            sa = 0.0;
+            for (j = jlow; j <= jhigh; ++j) sa += x[j] * y[i - j];
+            z[i++] = sa;
+        }
+    }
+    ilow = i;
+    ihigh = ily + ilx;
+    if (ihigh > ilz) ihigh = ilz;
+    jlow = ilow - ily;
+    jhigh = ilx - 2;
+    for (i = ilow; i < ihigh; i += 2, jlow += 2) {
+        sa = sb = 0.0;
+        xa = x[jlow];
+        yb = 0.0;
+        for (j = jlow; j < jhigh; j += 2) {
+            sb += xa * yb;
+            ya = y[i - j];
+            sa += xa * ya;
+            xb = x[j + 1];
+            sb += xb * ya;
+            yb = y[i - 1 - j];
+            sa += xb * yb;
+            xa = x[j + 2];
+        }
+        sb += xa * yb;
+        ya = y[i - j];
+        sa += xa * ya;
+        xb = x[j + 1];
+        sb += xb * ya;
+        yb = y[i - 1 - j];
+        sa += xb * yb;
+        z[i] = sa;
+        z[i + 1] = sb;
+    }
+    if (i == ihigh) {
+        jlow = i - ily;
+        jhigh = ilx;
+        sa = 0.0;
+        for (j = jlow; j <= jhigh; ++j) sa += x[j] * y[i - j];
+        z[i++] = sa;
+    }
+    /* off right */
+    for (; i <= ilz; ++i) z[i] = 0.0;
+}
+
+/* z = x cross-correlated with y: x reversed, then convolved (xr: lx scratch). */
+static void FN(xcor_)(int lx, int ifx, const REAL *x, int ly, int ify, const REAL *y, int lz, int ifz, REAL *z,
+                      REAL *xr)
+{
+    for (int i = 0, j = lx - 1; i < lx; ++i, --j) xr[i] = x[j];
+    FN(conv_)(lx, 1 - ifx - lx, xr, ly, ify, y, lz, ifz, z);
+}
+
+/* Levinson solve of the symmetric Toeplitz system R f = g (R's first row r),
+ * a: scratch (Claerbout's recursion as in Fwi::stoep). */
+static void FN(stoep_)(int n, const REAL *r, const REAL *g, REAL *f, REAL *a)
+{
+    REAL v, e, c, w, bot;
+    if (r[0] == 0.0) return;
+    a[0] = 1.0;
+    v = r[0];
+    f[0] = g[0] / r[0];
+    for (int j = 1; j < n; j++) {
+        int i;
+        a[j] = 0.0;
+        f[j] = 0.0;
+        for (i = 0, e = 0.0; i < j; i++) e += a[i] * r[j - i];
+        c = e / v;
+        v -= c * e;
+        for (i = 0; i <= j / 2; i++) {
+            bot = a[j - i] - c * a[i];
+            a[i] -= c * a[j - i];
+            a[j - i] = bot;
+        }
+        for (i = 0, w = 0.0; i < j; i++) w += f[i] * r[j - i];
+        c = (w - g[j]) / v;
+        for (i = 0; i <= j; i++) f[i] -= c * a[j - i];
+    }
+}
+
+/* Exported for the closed-form checks of the tests. */
+void FN(orc_conv)(int lx, int ifx, const REAL *x, int ly, int ify, const REAL *y, int lz, int ifz, REAL *z)
+{
+    FN(conv_)(lx, ifx, x, ly, ify, y, lz, ifz, z);
+}
+void FN(orc_stoep)(int n, const REAL *r, const REAL *g, REAL *f, REAL *a) { FN(stoep_)(n, r, g, f, a); }
+
+/* Time weight of the ADAPTIVE misfits: type 2 Gaussian exp(-it^2 / (2 s^2)),
+ * s = (T)(0.01 nt); type 3 linear min(it, 500) (fwi.cpp:397-420, fwi.h:28-31). */
+static REAL FN(adaptive_h_)(int type, int it, int nt)
+{
+    if (type == 2) {
+        REAL sd = (REAL)(0.01 * nt);
+        if (sd == 0.0) return 0;
+        return (REAL)exp(-1.0 * (it * it) / (2.0 * (sd * sd)));
+    }
+    REAL sd = (REAL)500;
+    if (it < sd) return (REAL)it;
+    return sd;
+}
+
+/* Shaping filter of one trace (first half of both ADAPTIVE cases):
+ * autocorr of the weighted record, crosscorr record x model, whitening by
+ * 1 + 0.01 of the zero lag, shaper = R^-1 crosscorr; returns its squared
+ * norm (1 when zero).  w: 6*nt scratch (modwei, recwei, autocorr,
+ * crosscorr, spiker, reversed copy), shaper: nt. */
+static REAL FN(adaptive_shaper_)(const REAL *m, const REAL *r, const REAL *wt, int nt, REAL *w, REAL *shaper)
+{
+    REAL *modwei = w, *recwei = w + nt, *autoc = w + 2 * nt, *cross = w + 3 * nt, *spiker = w + 4 * nt,
+         *xr = w + 5 * nt;
+    for (int it = 0; it < nt; it++) {
+        modwei[it] = wt ? m[it] * wt[it] : m[it];
+        recwei[it] = wt ? r[it] * wt[it] : r[it];
+    }
+    FN(xcor_)(nt, 0, recwei, nt, 0, recwei, nt, 0, autoc, xr);
+    FN(xcor_)(nt, 0, recwei, nt, 0, modwei, nt, 0, cross, xr);
+    if (autoc[0] == 0.0) autoc[0] = 1.0;
+    autoc[0] = (REAL)((double)autoc[0] * (1.0 + 0.01));
+    for (int it = 0; it < nt; it++) shaper[it] = 0.0;
+    FN(stoep_)(nt, autoc, cross, shaper, spiker);
+    REAL norm = 0.0;
+    for (int it = 0; it < nt; it++) norm += shaper[it] * shaper[it];
+    if (norm == 0.0) norm = 1.0;
+    return norm;
+}
+
 REAL FN(orc_misfit_a)(int type, const REAL *mod, const REAL *rec, const REAL *wei, const int *valid, int ntr,
                       int nt)
 {
+    if (type == 2 || type == 3) {
+        REAL misfit = 0.0;
+        REAL *w = (REAL *)malloc(sizeof(REAL) * 7 * (nt > 0 ? nt : 1)), *shaper = w + 6 * nt;
+        for (int i = 0; i < ntr; i++) {
+            if (!valid[i]) continue;
+            const REAL *m = mod + (long)i * nt, *r = rec + (long)i * nt, *wt = wei ? wei + (long)i * nt : NULL;
+            REAL norm = FN(adaptive_shaper_)(m, r, wt, nt, w, shaper);
+            for (int it = 0; it < nt; it++) {
+                REAL res = FN(adaptive_h_)(type, it, nt) * shaper[it];
+                if (type == 2)
+                    misfit = (REAL)((double)misfit - 0.5 * (double)res * (double)res / (double)norm);
+                else
+                    misfit = (REAL)((double)misfit + 0.5 * (double)res * (double)res / (double)norm);
+            }
+        }
+        free(w);
+        return misfit;
+    }
     REAL misfit = 0.0;
     for (int i = 0; i < ntr; i++) {
         if (!valid[i]) continue;
@@ -1134,13 +1388,39 @@ void FN(orc_residual_a)(int type, const REAL *mod, const REAL *rec, const REAL *
                         int scaling)
 {
     const long n = (long)ntr * nt;
-    if (type != 1) {
+    if (type == 0) {
         for (long p = 0; p < n; p++) {
             res[p] = scaling * (mod[p] - rec[p]);
             if (wei) res[p] *= wei[p] * wei[p];
         }
         return;
     }
+    if (type == 2 || type == 3) {
+        REAL *w = (REAL *)malloc(sizeof(REAL) * 7 * (nt > 0 ? nt : 1)), *shaper = w + 6 * nt;
+        for (int i = 0; i < ntr; i++) {
+            const REAL *m = mod + (long)i * nt, *r = rec + (long)i * nt, *wt = wei ? wei + (long)i * nt : NULL;
+            REAL *rs = res + (long)i * nt;
+            REAL norm = FN(adaptive_shaper_)(m, r, wt, nt, w, shaper);
+            REAL misfit = 0.0;
+            for (int it = 0; it < nt; it++) {
+                REAL h = FN(adaptive_h_)(type, it, nt);
+                misfit = (REAL)((double)misfit + 0.5 * (double)(h * shaper[it]) * (double)(h * shaper[it]) /
+                                                     (double)norm);
+            }
+            for (int it = 0; it < nt; it++) {
+                REAL h = FN(adaptive_h_)(type, it, nt);
+                double sgn = type == 2 ? -1.0 * scaling : (double)scaling;
+                rs[it] = (REAL)(sgn * ((double)(h * h) - 2.0 * (double)misfit) * (double)shaper[it] / (double)norm);
+            }
+            /* solve again with the residual as right-hand side, then convolve
+             * with the weighted record */
+            FN(stoep_)(nt, w + 2 * nt, rs, shaper, w + 4 * nt);
+            FN(conv_)(nt, 0, shaper, nt, 0, w + nt, nt, 0, rs);
+            if (wt)
+                for (int it = 0; it < nt; it++) rs[it] *= wt[it];
+        }
+        free(w);
+    } else {
     for (int i = 0; i < ntr; i++) {
         const REAL *m = mod + (long)i * nt, *r = rec + (long)i * nt;
         REAL n1 = 0.0, n2 = 0.0, n3 = 0.0;
@@ -1159,6 +1439,7 @@ void FN(orc_residual_a)(int type, const REAL *mod, const REAL *rec, const REAL *
             res[p] = (REAL)((-1.0) * scaling * (double)((r[it] / (n1 * n2)) - (m[it] / (n1 * n1)) * n3));
             if (wei) res[p] *= wei[p] * wei[p];
         }
+    }
     }
     REAL *f = (REAL *)malloc(sizeof(REAL) * (n > 0 ? n : 1));
     for (long p = 0; p < n; p++) f[p] = res[p] < 0 ? -res[p] : res[p];

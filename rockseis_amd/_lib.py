@@ -138,6 +138,7 @@ def _declare(L):
         "rsg_fwi_e2d_gradient_batch": ([C.POINTER(FwiE2D), C.POINTER(FwiE2DShot), ip, vp], ip),
         "rsg_fwi_a3d_gradient": ([GP, C.POINTER(FwiA3D), vp], ip),
         "rsg_interp": ([ip, vp, ip, ip, fp, fp, vp, ip, fp, fp, vp], ip),
+        "rsg_misfit_residuals": ([ip, vp, vp, vp, vp, vp, ip, ip, ip, ip, ip, C.POINTER(C.c_double), vp], ip),
     }
     for name in ("a2d_forward_velocity", "a2d_forward_stress", "e2d_forward_velocity",
                  "e2d_forward_stress", "e2d_backward_velocity", "e2d_backward_stress",
@@ -162,7 +163,7 @@ EXPORTS = (
     "rsg_revolve_numforw", "rsg_snap_cadence", "rsg_fwi_a2d_gradient_batch", "rsg_a2d_forward_velocity_batch",
     "rsg_a2d_forward_stress_batch", "rsg_e2d_velocity_batch", "rsg_e2d_stress_batch",
     "rsg_ragged2d_create", "rsg_ragged2d_step", "rsg_ragged2d_destroy", "rsg_fwi_e2d_gradient_batch",
-    "rsg_fwi_a3d_gradient", "rsg_interp",
+    "rsg_fwi_a3d_gradient", "rsg_interp", "rsg_misfit_residuals",
 )
 
 

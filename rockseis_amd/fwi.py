@@ -31,7 +31,8 @@ from . import _lib as R
 from .core import (GMAP, SMAP, Data2D, Data3D, ModelAcoustic2D, ModelAcoustic3D, ModelElastic2D, Pml, SMAX, _dev,
                    _stream)
 
-DIFFERENCE, CORRELATION = 0, 1  # rs_fwimisfit (utils.h:26)
+DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS, ADAPTIVE_LINEAR = 0, 1, 2, 3  # rs_fwimisfit (utils.h:26)
+_ACOUSTIC_MISFITS = (DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS, ADAPTIVE_LINEAR)
 _MODES = {"revolve": R.FWI_REVOLVE, "store": R.FWI_STORE, "full": R.FWI_FULL}
 _ADJOINTS = {"reference": R.ADJOINT_REFERENCE, "exact": R.ADJOINT_EXACT}
 
@@ -99,8 +100,8 @@ class FwiAcoustic2D:
         self.incore = bool(v)
 
     def setMisfit_type(self, t):
-        if int(t) not in (DIFFERENCE, CORRELATION):
-            raise R.RsgError("the acoustic misfit on this path is DIFFERENCE or CORRELATION")
+        if int(t) not in _ACOUSTIC_MISFITS:
+            raise R.RsgError("the acoustic misfit is DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS or ADAPTIVE_LINEAR")
         self.misfit_type = int(t)
 
     def setDataweight(self, d):
@@ -182,6 +183,30 @@ class FwiAcoustic2D:
         self.misfit64 = misfit.value
         self.mode_used = {R.FWI_REVOLVE: "revolve", R.FWI_STORE: "store", R.FWI_FULL: "full"}[used.value]
         return 1
+
+
+def misfit_residuals(misfit_type, dmod, dobs, gmap, weight=None, scaling=1):
+    """FwiAcoustic2D/3D::computeMisfit and computeResiduals on the device for
+    modelled records dmod and observed dobs (ntrace, nt) or (nshot, ntrace,
+    nt), receiver maps gmap (..., ntrace, 2 or 3; traces with a negative entry
+    count in no misfit), optional data weights.  Returns (misfit, residuals)."""
+    dev = _dev()
+    mod = torch.as_tensor(np.ascontiguousarray(dmod, np.float32)).to(dev)
+    rec = torch.as_tensor(np.ascontiguousarray(dobs, np.float32)).to(dev)
+    mp = torch.as_tensor(np.ascontiguousarray(gmap, np.int32)).to(dev)
+    if int(misfit_type) not in _ACOUSTIC_MISFITS:
+        raise R.RsgError("the acoustic misfit is DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS or ADAPTIVE_LINEAR")
+    nt, ntr = mod.shape[-1], mod.shape[-2]
+    nshot = mod.numel() // (nt * ntr) if nt * ntr else 1
+    dim = mp.shape[-1]
+    wt = None if weight is None else torch.as_tensor(np.ascontiguousarray(weight, np.float32)).to(dev)
+    res = torch.zeros_like(mod)
+    mis = C.c_double(0.0)
+    R.check(R.lib().rsg_misfit_residuals(int(misfit_type), mod.data_ptr(), rec.data_ptr(),
+                                         None if wt is None else wt.data_ptr(), res.data_ptr(), mp.data_ptr(), dim,
+                                         ntr, nt, nshot, int(scaling), C.byref(mis), _stream()),
+            "rsg_misfit_residuals")
+    return mis.value, res.cpu().numpy()
 
 
 class Shot:
@@ -623,8 +648,8 @@ class FwiAcoustic3D:
         self.fp16 = bool(v)
 
     def setMisfit_type(self, t):
-        if int(t) not in (DIFFERENCE, CORRELATION):
-            raise R.RsgError("the acoustic misfit on this path is DIFFERENCE or CORRELATION")
+        if int(t) not in _ACOUSTIC_MISFITS:
+            raise R.RsgError("the acoustic misfit is DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS or ADAPTIVE_LINEAR")
         self.misfit_type = int(t)
 
     def setDataweight(self, d):

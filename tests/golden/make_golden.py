@@ -167,13 +167,13 @@ def fwi_fixtures(R):
              wavgrad=r["wavgrad"])
 
 
-FWI_MISFIT_CASES = [  # order, fs, ncheck, misfit type (1 CORRELATION), data weights
-    (4, 0, 20, 1, False), (8, 1, 7, 1, True), (4, 0, 20, 0, True),
+FWI_MISFIT_CASES = [  # order, fs, ncheck, misfit type (1 CORRELATION, 2/3 ADAPTIVE_GAUSS/LINEAR), data weights
+    (4, 0, 20, 1, False), (8, 1, 7, 1, True), (4, 0, 20, 0, True), (4, 0, 20, 2, False), (8, 1, 7, 3, True),
 ]
 
 
 def fwi_misfit_fixtures(R):
-    """run_optimal with CORRELATION misfit / data weights (setDataweight)."""
+    """run_optimal with the CORRELATION and ADAPTIVE misfits / data weights (setDataweight)."""
     from oracle import fwi
     nx, nz, lpml, dx, dt, nt = 60, 50, 10, 10.0, 1e-3, 300
     vtrue = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1800.0, 3000.0, nanom=5, seed=3)

@@ -183,10 +183,10 @@ def test_batched_gradient_equals_sum_of_shots(golden):
         assert rel_l2(rg.cpu().numpy(), gr) <= 1e-6
 
 
-@pytest.mark.parametrize("case", range(3))
+@pytest.mark.parametrize("case", range(5))
 def test_misfit_types_vs_reference(golden, case):
-    """CORRELATION misfit (97th-percentile residual clip) and data weights
-    against the reference's run_optimal."""
+    """CORRELATION and ADAPTIVE misfits (97th-percentile residual clip) and
+    data weights against the reference's run_optimal."""
     from rockseis_amd import core, fwi
     from tests.golden.make_golden import FWI_MISFIT_CASES
     g = golden(f"fwi_misfit_case{case}")
@@ -215,7 +215,32 @@ def test_misfit_types_vs_reference(golden, case):
         e = rel_l2(got, g[k])
         print(f"  case {case} {k}: rel L2 {e:.2e}", flush=True)
         # CORRELATION residuals difference two nearly equal normalised terms:
-        # the reference's own fp32 residuals are 1.2e-5 from FP64 in case 1
-        bound = 3e-5 if (k == "dres" and mt == 1) else RTOL_L2
+        # the reference's own fp32 residuals are 1.2e-5 from FP64 in case 1.
+        # ADAPTIVE residuals pass the records through two Levinson solves and
+        # the 97th-percentile clip: a 1-ulp random perturbation of the
+        # reference's own records moves its residuals by 5.2e-5 to 5.9e-5
+        # (clip flips), so records agreeing to ~1e-6 bound them at 1e-4.
+        bound = 3e-5 if (k == "dres" and mt == 1) else (1e-4 if mt >= 2 else RTOL_L2)
         assert e <= bound, k
     assert abs(f.misfit64 - float(g["misfit"])) <= 1e-4 * abs(float(g["misfit"]))
+
+
+@pytest.mark.parametrize("case", range(5))
+def test_misfit_residuals_bit_exact(golden, case):
+    """The device misfit / residual kernels alone (rsg_misfit_residuals) on the
+    reference's own modelled records: residuals equal the reference's bit for
+    bit for DIFFERENCE, CORRELATION (per-trace sums in the reference's order,
+    exact 97th-percentile clip) and ADAPTIVE (the reference's sequential
+    convolutions and Levinson recursions, one lane per trace); the misfit
+    (reduced in double, the reference sums in float) to 1e-4."""
+    from rockseis_amd import core, fwi
+    from tests.golden.make_golden import FWI_MISFIT_CASES
+    g = golden(f"fwi_misfit_case{case}")
+    order, fs, ncheck, mt, usew = FWI_MISFIT_CASES[case]
+    nx, nz, lpml, dx, dt, nt = g["grid"]
+    m = core.ModelAcoustic2D(int(nx), int(nz), int(lpml), dx, dx, 0.0, 0.0, fs)
+    obs = data2d(g["gcoords"], int(nt), dt, "P", g["dobs"])
+    obs.makeMap(m, core.GMAP)
+    mis, res = fwi.misfit_residuals(mt, g["dmod"], g["dobs"], obs.gmap, g["weight"] if usew else None)
+    assert np.array_equal(res, g["dres"])
+    assert abs(mis - float(g["misfit"])) <= 1e-4 * abs(float(g["misfit"]))

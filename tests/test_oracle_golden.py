@@ -186,11 +186,13 @@ def test_interp_matches_reference(golden):
             assert np.array_equal(out, g[f"out{k}_m{mode}"]), (k, mode)
 
 
-@pytest.mark.parametrize("case", range(3))
+@pytest.mark.parametrize("case", range(5))
 def test_fwi_a2d_misfit_types(golden, case):
     """run_optimal with the CORRELATION misfit (per-trace normalised
-    correlation, 97th-percentile clip of the residuals) and data weights:
-    the oracle vs the reference, bit for bit."""
+    correlation, 97th-percentile clip of the residuals), the ADAPTIVE_GAUSS and
+    ADAPTIVE_LINEAR misfits (per-trace Wiener shaping filter by Levinson
+    recursion, time-weighted) and data weights: the oracle vs the reference,
+    bit for bit."""
     from oracle import fwi
     from tests.golden.make_golden import FWI_MISFIT_CASES
     g = golden(f"fwi_misfit_case{case}")

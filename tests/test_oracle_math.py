@@ -190,3 +190,40 @@ def test_interp_closed_forms(mode):
                  np.float32(0))
     nz = o[0] != 0
     np.testing.assert_allclose(o[0, nz], 1.0, atol=1e-6)
+
+
+def test_adaptive_convolution_and_levinson_closed_forms():
+    """The ADAPTIVE misfits' building blocks (oracle conv_ / stoep_, Hale's
+    pairwise convolution and Claerbout's Levinson recursion) against the
+    textbook: every output window of x * y (all first-sample offsets, odd and
+    even lengths, x longer or shorter than y) against numpy's full
+    convolution; the Toeplitz solve against a dense solve of R f = g for a
+    whitened autocorrelation."""
+    import numpy as np
+    import oracle
+    conv = oracle.fn("conv", np.float64)
+    rng = np.random.default_rng(8)
+    for lx, ly in ((7, 12), (12, 7), (10, 10), (9, 9), (1, 5), (16, 3)):
+        x, y = rng.standard_normal(lx), rng.standard_normal(ly)
+        full = np.convolve(x, y)  # index k <-> ifx + ify + k
+        for ifx in (-lx, 1 - lx, 0, 2):
+            for ify in (-3, 0, 1):
+                for lz, ifz in ((lx + ly - 1, ifx + ify), (5, ifx + ify - 2), (6, ifx + ify + lx), (lx, 0)):
+                    z = np.zeros(lz)
+                    conv(lx, ifx, x, ly, ify, y, lz, ifz, z)
+                    ref = np.zeros(lz)
+                    for k in range(lz):
+                        q = ifz + k - ifx - ify
+                        if 0 <= q < full.size:
+                            ref[k] = full[q]
+                    assert np.allclose(z, ref, rtol=0, atol=1e-12), (lx, ly, ifx, ify, lz, ifz)
+    stoep = oracle.fn("stoep", np.float64)
+    for n in (1, 2, 5, 33):
+        s = rng.standard_normal(3 * n)
+        r = np.correlate(s, s, "full")[3 * n - 1:3 * n - 1 + n].copy()
+        r[0] *= 1.01
+        g = rng.standard_normal(n)
+        f, a = np.zeros(n), np.zeros(n)
+        stoep(n, r, g, f, a)
+        R = np.array([[r[abs(i - j)] for j in range(n)] for i in range(n)])
+        assert np.allclose(R @ f, g, rtol=0, atol=1e-9 * np.abs(g).max()), n
