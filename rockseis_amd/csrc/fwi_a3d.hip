@@ -145,15 +145,23 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
     half_t *ck16 = nullptr;
     unsigned *ckamax = nullptr;
     size_t *dsegs = nullptr;
+    Chunk16 *dchunks = nullptr;
+    unsigned *ckpart = nullptr;  // per-chunk maxima of the FP16 packing
+    // FP16 checkpoint slots start at multiples of 4 values (8-byte aligned)
+    const size_t cstride = (total + 3) & ~(size_t)3;
+    const std::vector<Chunk16> chunks = f16 ? store16_chunks(segs) : std::vector<Chunk16>();
+    const int nchunk = (int)chunks.size();
     if (f16) {
-        if (int rc = dm.alloc(&ck16, total * (size_t)rev.snaps)) return rc;
+        if (int rc = dm.alloc(&ck16, cstride * (size_t)rev.snaps)) return rc;
         if (int rc = dm.alloc(&ckamax, (size_t)nseg * rev.snaps)) return rc;
         if (int rc = dm.alloc(&dsegs, segs.size())) return rc;
+        if (int rc = dm.alloc(&dchunks, chunks.size())) return rc;
+        if (int rc = dm.alloc(&ckpart, chunks.size())) return rc;
         RSG_HIP(hipMemcpyAsync(dsegs, segs.data(), sizeof(size_t) * segs.size(), hipMemcpyHostToDevice, st));
+        RSG_HIP(hipMemcpyAsync(dchunks, chunks.data(), sizeof(Chunk16) * chunks.size(), hipMemcpyHostToDevice, st));
     } else {
         if (int rc = dm.alloc(&ck, total * (size_t)rev.snaps)) return rc;
     }
-    const size_t maxseg = std::max({n, face[0], face[1], face[2]});
     rsg_traces mod = f->obs, res = f->obs, wg = f->src;
     mod.data = f->dmod;
     res.data = f->dres;
@@ -198,16 +206,16 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
             if (int rc = reverse_tail(capo)) return rc;
         } else if (a == Revolve::TAKESHOT) {
             if (f16) {
-                if (int rc = pack16_table(fwb, ck16 + total * rev.check, ckamax + (size_t)nseg * rev.check, dsegs, nseg,
-                                          maxseg, st))
+                if (int rc = pack16_table(fwb, ck16 + cstride * rev.check, ckamax + (size_t)nseg * rev.check, ckpart,
+                                          dsegs, dchunks, nchunk, nseg, total, st))
                     return rc;
             } else {
                 RSG_HIP(hipMemcpyAsync(ck + total * rev.check, fwb, total * sizeof(float), hipMemcpyDeviceToDevice, st));
             }
         } else if (a == Revolve::RESTORE) {
             if (f16) {
-                if (int rc = unpack16_table(ck16 + total * rev.check, ckamax + (size_t)nseg * rev.check, fwb, dsegs,
-                                            nseg, maxseg, st))
+                if (int rc = unpack16_table(ck16 + cstride * rev.check, ckamax + (size_t)nseg * rev.check, fwb, dsegs,
+                                            dchunks, nchunk, total, st))
                     return rc;
             } else {
                 RSG_HIP(hipMemcpyAsync(fwb, ck + total * rev.check, total * sizeof(float), hipMemcpyDeviceToDevice, st));

@@ -270,7 +270,6 @@ static int fwi_e2d_impl(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *sh, int ns
     // ---- state layout: per shot [Vx Vz Sxx Szz Sxz | 16 strips], shots back to back
     std::vector<size_t> off(ns + 1, 0), segs;
     segs.reserve((size_t)ns * 21 + 1);
-    size_t maxseg = 0;
     for (int k = 0; k < ns; k++) {
         const size_t n = (size_t)sh[k].g.nx * sh[k].g.nz, sx = (size_t)lpml * sh[k].g.nz,
                      sz = (size_t)sh[k].g.nx * lpml;
@@ -281,7 +280,6 @@ static int fwi_e2d_impl(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *sh, int ns
             o += e2d_xstrip(i) ? sx : sz;
         }
         off[k + 1] = o;
-        maxseg = std::max({maxseg, n, sx, sz});
     }
     segs.push_back(off[ns]);
     const size_t total = off[ns];
@@ -394,15 +392,24 @@ static int fwi_e2d_impl(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *sh, int ns
     if (int rc = dm.alloc(&dmis, 1)) return rc;
     if (int rc = launch_zero(dmis, sizeof(double), st)) return rc;
     RSG_HIP(hipMemcpyAsync(dsd, hd.data(), sizeof(ShotDev) * ns, hipMemcpyHostToDevice, st));
+    Chunk16 *dchunks = nullptr;
+    unsigned *ckpart = nullptr;  // per-chunk maxima of the FP16 packing
+    // FP16 checkpoint slots start at multiples of 4 values (8-byte aligned)
+    const size_t cstride = (total + 3) & ~(size_t)3;
+    const std::vector<Chunk16> chunks = f16 ? store16_chunks(segs) : std::vector<Chunk16>();
+    const int nchunk = (int)chunks.size();
     if (f16) {
         if (int rc = dm.alloc(&dsegs, segs.size())) return rc;
+        if (int rc = dm.alloc(&dchunks, chunks.size())) return rc;
+        if (int rc = dm.alloc(&ckpart, chunks.size())) return rc;
         RSG_HIP(hipMemcpyAsync(dsegs, segs.data(), sizeof(size_t) * segs.size(), hipMemcpyHostToDevice, st));
+        RSG_HIP(hipMemcpyAsync(dchunks, chunks.data(), sizeof(Chunk16) * chunks.size(), hipMemcpyHostToDevice, st));
     }
     float *ck = nullptr;
     half_t *ck16 = nullptr;
     unsigned *ckamax = nullptr;
     if (f16) {
-        if (int rc = dm.alloc(&ck16, total * (size_t)rev.snaps)) return rc;
+        if (int rc = dm.alloc(&ck16, cstride * (size_t)rev.snaps)) return rc;
         if (int rc = dm.alloc(&ckamax, (size_t)nseg * rev.snaps)) return rc;
     } else {
         if (int rc = dm.alloc(&ck, total * (size_t)rev.snaps)) return rc;
@@ -460,8 +467,8 @@ static int fwi_e2d_impl(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *sh, int ns
             if (int rc = reverse_tail(capo)) return rc;
         } else if (a == Revolve::TAKESHOT) {
             if (f16) {
-                if (int rc = pack16_table(fwb, ck16 + total * rev.check, ckamax + (size_t)nseg * rev.check, dsegs,
-                                          nseg, maxseg, st))
+                if (int rc = pack16_table(fwb, ck16 + cstride * rev.check, ckamax + (size_t)nseg * rev.check, ckpart,
+                                          dsegs, dchunks, nchunk, nseg, total, st))
                     return rc;
             } else {
                 RSG_HIP(hipMemcpyAsync(ck + total * rev.check, fwb, total * sizeof(float), hipMemcpyDeviceToDevice,
@@ -469,8 +476,8 @@ static int fwi_e2d_impl(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *sh, int ns
             }
         } else if (a == Revolve::RESTORE) {
             if (f16) {
-                if (int rc = unpack16_table(ck16 + total * rev.check, ckamax + (size_t)nseg * rev.check, fwb, dsegs,
-                                            nseg, maxseg, st))
+                if (int rc = unpack16_table(ck16 + cstride * rev.check, ckamax + (size_t)nseg * rev.check, fwb, dsegs,
+                                            dchunks, nchunk, total, st))
                     return rc;
             } else {
                 RSG_HIP(hipMemcpyAsync(fwb, ck + total * rev.check, total * sizeof(float), hipMemcpyDeviceToDevice,

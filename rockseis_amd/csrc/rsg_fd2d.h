@@ -355,7 +355,7 @@ __global__ __launch_bounds__(BX *BZ) void a2d_velocity_kernel(const K2D a) {
         a2d_velocity_body<ORDER, RZ, false>(a, x0, z0, sP);
 }
 
-template <int ORDER, int RZ, bool FAST>
+template <int ORDER, int RZ, bool FAST, bool IO = true>
 __device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, float *sVx, float *sVz) {
     constexpr int H = ORDER, LDX = BX + 2 * H, LDZ = BX, TZ = BZ * RZ;
     const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
@@ -380,7 +380,7 @@ __device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, fl
     tile_store(tvz, sVz);
     __syncthreads();
     if (!xin) return;
-    const int it = a.io.nsrc ? (a.io.it_dev ? *a.io.it_dev + a.io.it : a.io.it) : 0;
+    const int it = (IO && a.io.nsrc) ? (a.io.it_dev ? *a.io.it_dev + a.io.it : a.io.it) : 0;
     const Tab tb{a.tab, a.lpml};
     const PmlPos px = FAST ? PmlPos{-1, 0} : pml_pos(x, nx, a.lpml);
     const int kx = FAST ? ORDER : kbw(x, nx, ORDER);
@@ -399,7 +399,7 @@ __device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, fl
                    (size_t)pz.i * nx + x);
         cell_st<FAST>(bP, pv[r], cb, r, rowb);
     }
-    if (a.io.nsrc) fused_inject<RZ, FAST>(a.io, bP, nullptr, &bL, x, z0 + ty * RZ, nz, cb, rowb, it);
+    if (IO && a.io.nsrc) fused_inject<RZ, FAST>(a.io, bP, nullptr, &bL, x, z0 + ty * RZ, nz, cb, rowb, it);
 }
 
 template <int ORDER, int RZ>
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(BX *BZ) void e2d_velocity_kernel(const K2D a) {
 //   free surface Szz(z=lpml)=0, Sxz += dt*M*(ddx_fw(Vz) + ddz_fw(Vx)).
 // Adjoint state (BW=true, waves.cpp:1907-2080): the same derivatives with
 // unit moduli, Sxx only from ddx_bw(Vx) and Szz only from ddz_bw(Vz).
-template <int ORDER, int RZ, bool BW, bool FAST>
+template <int ORDER, int RZ, bool BW, bool FAST, bool IO = true>
 __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, float *sVx, float *sVz) {
     constexpr int H = ORDER, TZ = BZ * RZ, LD = BX + 2 * H;
     const int nx = a.nx, nz = a.nz, tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
@@ -542,8 +542,8 @@ __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, fl
     tile_store(tvx, sVx);
     tile_store(tvz, sVz);
     __syncthreads();
-    const int it = (a.io.nsrc || a.io.nrec) ? (a.io.it_dev ? *a.io.it_dev + a.io.it : a.io.it) : 0;
-    if (!BW && a.io.nrec)
+    const int it = (IO && (a.io.nsrc || a.io.nrec)) ? (a.io.it_dev ? *a.io.it_dev + a.io.it : a.io.it) : 0;
+    if (IO && !BW && a.io.nrec)
         fused_record(a.io, (z0 / TZ) * a.ntx + x0 / BX, x0, z0, H, LD, a.lpml, sVx, sVz, it);
     if (!xin) return;
     const Tab tb{a.tab, a.lpml};
@@ -604,7 +604,7 @@ __device__ __forceinline__ void e2d_stress_body(const K2D &a, int x0, int z0, fl
         cell_st<FAST>(bSxz, sxz[r], cb, r, rowb);
     }
     // explosive source: Sxx, Szz -= dt * w * xtri * ytri
-    if (!BW && a.io.nsrc) fused_inject<RZ, FAST>(a.io, bSxx, &bSzz, nullptr, x, z0 + ty * RZ, nz, cb, rowb, it);
+    if (IO && !BW && a.io.nsrc) fused_inject<RZ, FAST>(a.io, bSxx, &bSzz, nullptr, x, z0 + ty * RZ, nz, cb, rowb, it);
 }
 
 template <int ORDER, int RZ, bool BW>
@@ -628,7 +628,10 @@ __global__ __launch_bounds__(BX *BZ) void e2d_stress_kernel(const K2D a) {
 // first tile of shot k for the launch's tile height, tile0[nshot] = total);
 // the XCD-contiguous block order of tile_of is kept over that numbering.  A
 // block finds its shot by binary search and copies that shot's kernel
-// arguments from desc[] (uniform across the block: scalar loads).
+// arguments from desc[] (uniform across the block: scalar loads).  The fused
+// source/receiver pass is compiled out of these kernels (IO = false): its
+// dynamically indexed arrays would put the copied descriptor in private
+// (scratch) memory.
 __device__ __forceinline__ bool ragged_tile(const K2D *__restrict__ desc, const int *__restrict__ tile0,
                                             int nshot, int TZ, K2D &a, int &x0, int &z0) {
     const int nb = gridDim.x;
@@ -674,9 +677,9 @@ __global__ __launch_bounds__(BX *BZ) void a2d_stress_ragged_kernel(const K2D *__
     int x0, z0;
     if (!ragged_tile(desc, tile0, nshot, TZ, a, x0, z0)) return;
     if (interior_tile(a, x0, z0, TZ, H))
-        a2d_stress_body<ORDER, RZ, true>(a, x0, z0, sVx, sVz);
+        a2d_stress_body<ORDER, RZ, true, false>(a, x0, z0, sVx, sVz);
     else
-        a2d_stress_body<ORDER, RZ, false>(a, x0, z0, sVx, sVz);
+        a2d_stress_body<ORDER, RZ, false, false>(a, x0, z0, sVx, sVz);
 }
 
 template <int ORDER, int RZ, bool BW>
@@ -705,9 +708,9 @@ __global__ __launch_bounds__(BX *BZ) void e2d_stress_ragged_kernel(const K2D *__
     int x0, z0;
     if (!ragged_tile(desc, tile0, nshot, TZ, a, x0, z0)) return;
     if (interior_tile(a, x0, z0, TZ, H))
-        e2d_stress_body<ORDER, RZ, BW, true>(a, x0, z0, sVx, sVz);
+        e2d_stress_body<ORDER, RZ, BW, true, false>(a, x0, z0, sVx, sVz);
     else
-        e2d_stress_body<ORDER, RZ, BW, false>(a, x0, z0, sVx, sVz);
+        e2d_stress_body<ORDER, RZ, BW, false, false>(a, x0, z0, sVx, sVz);
 }
 
 // ---------------------------------------------------------------------------

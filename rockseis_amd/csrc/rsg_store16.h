@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <vector>
 
 namespace rsg {
 
@@ -42,12 +43,21 @@ int pack16_segments(const float *src, half_t *dst, unsigned *amax, size_t seglen
 int unpack16_segments(const half_t *src, const unsigned *amax, float *dst, size_t seglen, int nseg,
                       hipStream_t st);
 
-// Segments of different lengths: segment s is src[off[s] .. off[s+1]) (off:
-// nseg+1 device offsets, maxlen the longest segment).
-int pack16_table(const float *src, half_t *dst, unsigned *amax, const size_t *off, int nseg, size_t maxlen,
-                 hipStream_t st);
-int unpack16_table(const half_t *src, const unsigned *amax, float *dst, const size_t *off, int nseg,
-                   size_t maxlen, hipStream_t st);
+// Segments of different lengths: segment s is src[off[s] .. off[s+1]) of one
+// flat buffer of total = off[nseg] values (off: nseg+1 device offsets,
+// off[0] = 0), swept in chunks of kChunk16 values; cs (device) is
+// store16_chunks(off) for the same offsets, nchunk its length; part is
+// device scratch of nchunk values.  src/dst must be 16-byte (float) and
+// 8-byte (half) aligned.
+constexpr int kChunk16 = 4096;
+struct Chunk16 {
+    int first, last;  // segments touched by the chunk
+};
+std::vector<Chunk16> store16_chunks(const std::vector<size_t> &off);
+int pack16_table(const float *src, half_t *dst, unsigned *amax, unsigned *part, const size_t *off,
+                 const Chunk16 *cs, int nchunk, int nseg, size_t total, hipStream_t st);
+int unpack16_table(const half_t *src, const unsigned *amax, float *dst, const size_t *off, const Chunk16 *cs,
+                   int nchunk, size_t total, hipStream_t st);
 
 // Unpadded nx x nz window of every shot's padded (lpml) field, shots sstride
 // apart -> dst [nshot][nz][nx] in FP32 (amax == nullptr) or FP16 with one

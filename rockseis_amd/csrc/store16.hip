@@ -1,6 +1,9 @@
 // FP16 snapshot / checkpoint storage with per-snapshot power-of-two scales
 // (declarations and the scaling rule in rsg_store16.h), and the FP32 copy of
 // the unpadded window that crossCorr reads.
+#include <algorithm>
+#include <cstdint>
+
 #include "rsg_common.h"
 #include "rsg_store16.h"
 
@@ -58,34 +61,132 @@ __global__ __launch_bounds__(kThreads) void unpack16_segments_kernel(const half_
         dst[o + i] = (float)src[o + i] * us;
 }
 
-// Segments of different lengths: segment s = [off[s], off[s+1]).
-__global__ __launch_bounds__(kThreads) void amax_table_kernel(const float *__restrict__ src, unsigned *amax,
-                                                               const size_t *__restrict__ off) {
-    const size_t o = off[blockIdx.y], n = off[blockIdx.y + 1] - o;
+// Segments of different lengths, segment s = [off[s], off[s+1]) of one flat
+// buffer of `total` values, swept in chunks of kChunk16 values: block c
+// covers [c*kChunk16, min((c+1)*kChunk16, total)), which touches segments
+// cs[c].first .. cs[c].last.  A chunk inside one segment (nearly all of
+// them) is read with float4 loads by every thread and reduced to one
+// atomicMax; a chunk across segment boundaries walks the boundaries value by
+// value.  The grid therefore scales with the buffer, not with the number of
+// segments times the longest segment.
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+constexpr int kVec = kChunk16 / (4 * kThreads);  // float4 loads per thread
+
+// Largest magnitude of a block's values (no atomics): the result is in
+// thread 0.
+__device__ inline float block_max(float v) {
+    __shared__ float part[kThreads / 64];
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) part[w] = v;
+    __syncthreads();
+    float m = part[0];
+    for (int i = 1; i < kThreads / 64; i++) m = fmaxf(m, part[i]);
+    return m;
+}
+
+// First pass: a chunk inside one segment stores its maximum in part[chunk]
+// (reduced per segment by amax_finish_kernel: no contended atomics, a
+// segment may span tens of thousands of chunks); a chunk across boundaries
+// folds each segment's share into amax[] with atomicMax.
+__global__ __launch_bounds__(kThreads) void amax_chunks_kernel(const float *__restrict__ src, unsigned *amax,
+                                                                unsigned *__restrict__ part,
+                                                                const size_t *__restrict__ off,
+                                                                const Chunk16 *__restrict__ cs, size_t total) {
+    const size_t c0 = (size_t)blockIdx.x * kChunk16, c1 = min(c0 + (size_t)kChunk16, total);
+    const Chunk16 q = cs[blockIdx.x];
+    if (q.first == q.last) {
+        float m = 0.f;
+        if (c1 - c0 == (size_t)kChunk16) {
+            float4 v[kVec];
+#pragma unroll
+            for (int j = 0; j < kVec; j++) v[j] = *(const float4 *)(src + c0 + 4 * (threadIdx.x + j * kThreads));
+#pragma unroll
+            for (int j = 0; j < kVec; j++)
+                m = fmaxf(m, fmaxf(fmaxf(fabsf(v[j].x), fabsf(v[j].y)), fmaxf(fabsf(v[j].z), fabsf(v[j].w))));
+        } else {
+            for (size_t i = c0 + threadIdx.x; i < c1; i += kThreads) m = fmaxf(m, fabsf(src[i]));
+        }
+        m = block_max(m);
+        if (threadIdx.x == 0) part[blockIdx.x] = __float_as_uint(m);
+        return;
+    }
+    for (int k = q.first; k <= q.last; k++) {
+        const size_t b = max(off[k], c0), e = min(off[k + 1], c1);
+        float m = 0.f;
+        for (size_t i = b + threadIdx.x; i < e; i += kThreads) m = fmaxf(m, fabsf(src[i]));
+        block_amax(m, amax + k);
+        __syncthreads();  // part[] of block_amax is reused by the next segment
+    }
+}
+
+// Second pass, one block per segment: the chunks lying wholly inside segment
+// s are [ceil(off[s]/kChunk16), floor(off[s+1]/kChunk16)), or up to the last
+// (partial) chunk when the segment ends the buffer.
+__global__ __launch_bounds__(kThreads) void amax_finish_kernel(unsigned *amax, const unsigned *__restrict__ part,
+                                                                const size_t *__restrict__ off, size_t total) {
+    const int s = blockIdx.x;
+    const size_t nchunk = (total + kChunk16 - 1) / kChunk16;
+    const size_t cf = (off[s] + kChunk16 - 1) / kChunk16;
+    const size_t cl = off[s + 1] == total ? nchunk : off[s + 1] / kChunk16;
     float m = 0.f;
-    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
-        m = fmaxf(m, fabsf(src[o + i]));
-    block_amax(m, amax + blockIdx.y);
+    for (size_t c = cf + threadIdx.x; c < cl; c += kThreads) m = fmaxf(m, __uint_as_float(part[c]));
+    block_amax(m, amax + s);
 }
 
-__global__ __launch_bounds__(kThreads) void pack16_table_kernel(const float *__restrict__ src,
-                                                                 half_t *__restrict__ dst,
-                                                                 const unsigned *__restrict__ amax,
-                                                                 const size_t *__restrict__ off) {
-    const size_t o = off[blockIdx.y], n = off[blockIdx.y + 1] - o;
-    const float sc = store16_scale(amax[blockIdx.y]);
-    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
-        dst[o + i] = (half_t)(src[o + i] * sc);
+__global__ __launch_bounds__(kThreads) void pack16_chunks_kernel(const float *__restrict__ src,
+                                                                  half_t *__restrict__ dst,
+                                                                  const unsigned *__restrict__ amax,
+                                                                  const size_t *__restrict__ off,
+                                                                  const Chunk16 *__restrict__ cs, size_t total) {
+    const size_t c0 = (size_t)blockIdx.x * kChunk16, c1 = min(c0 + (size_t)kChunk16, total);
+    const Chunk16 q = cs[blockIdx.x];
+    if (q.first == q.last && c1 - c0 == (size_t)kChunk16) {
+        const float sc = store16_scale(amax[q.first]);
+        float4 v[kVec];
+#pragma unroll
+        for (int j = 0; j < kVec; j++) v[j] = *(const float4 *)(src + c0 + 4 * (threadIdx.x + j * kThreads));
+#pragma unroll
+        for (int j = 0; j < kVec; j++) {
+            half4_t h;
+            h.x = (half_t)(v[j].x * sc);
+            h.y = (half_t)(v[j].y * sc);
+            h.z = (half_t)(v[j].z * sc);
+            h.w = (half_t)(v[j].w * sc);
+            *(half4_t *)(dst + c0 + 4 * (threadIdx.x + j * kThreads)) = h;
+        }
+        return;
+    }
+    int k = q.first;
+    for (size_t i = c0 + threadIdx.x; i < c1; i += kThreads) {
+        while (off[k + 1] <= i) k++;
+        dst[i] = (half_t)(src[i] * store16_scale(amax[k]));
+    }
 }
 
-__global__ __launch_bounds__(kThreads) void unpack16_table_kernel(const half_t *__restrict__ src,
-                                                                   const unsigned *__restrict__ amax,
-                                                                   float *__restrict__ dst,
-                                                                   const size_t *__restrict__ off) {
-    const size_t o = off[blockIdx.y], n = off[blockIdx.y + 1] - o;
-    const float us = store16_unscale(amax[blockIdx.y]);
-    for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
-        dst[o + i] = (float)src[o + i] * us;
+__global__ __launch_bounds__(kThreads) void unpack16_chunks_kernel(const half_t *__restrict__ src,
+                                                                    const unsigned *__restrict__ amax,
+                                                                    float *__restrict__ dst,
+                                                                    const size_t *__restrict__ off,
+                                                                    const Chunk16 *__restrict__ cs, size_t total) {
+    const size_t c0 = (size_t)blockIdx.x * kChunk16, c1 = min(c0 + (size_t)kChunk16, total);
+    const Chunk16 q = cs[blockIdx.x];
+    if (q.first == q.last && c1 - c0 == (size_t)kChunk16) {
+        const float us = store16_unscale(amax[q.first]);
+        half4_t h[kVec];
+#pragma unroll
+        for (int j = 0; j < kVec; j++) h[j] = *(const half4_t *)(src + c0 + 4 * (threadIdx.x + j * kThreads));
+#pragma unroll
+        for (int j = 0; j < kVec; j++)
+            *(float4 *)(dst + c0 + 4 * (threadIdx.x + j * kThreads)) =
+                make_float4((float)h[j].x * us, (float)h[j].y * us, (float)h[j].z * us, (float)h[j].w * us);
+        return;
+    }
+    int k = q.first;
+    for (size_t i = c0 + threadIdx.x; i < c1; i += kThreads) {
+        while (off[k + 1] <= i) k++;
+        dst[i] = (float)src[i] * store16_unscale(amax[k]);
+    }
 }
 
 // Interior window: blockIdx.x covers 256 columns, blockIdx.y 8 rows,
@@ -161,23 +262,43 @@ int unpack16_segments(const half_t *src, const unsigned *amax, float *dst, size_
     return RSG_OK;
 }
 
-int pack16_table(const float *src, half_t *dst, unsigned *amax, const size_t *off, int nseg, size_t maxlen,
-                 hipStream_t st) {
-    if (nseg < 1 || nseg > 65535) return fail(RSG_EINVAL, "pack16: 1..65535 segments");
+std::vector<Chunk16> store16_chunks(const std::vector<size_t> &off) {
+    std::vector<Chunk16> cs;
+    if (off.size() < 2) return cs;
+    const size_t total = off.back();
+    const size_t nchunk = (total + kChunk16 - 1) / kChunk16;
+    cs.resize(nchunk);
+    int k = 0;
+    for (size_t c = 0; c < nchunk; c++) {
+        const size_t c0 = c * kChunk16, c1 = std::min(c0 + (size_t)kChunk16, total);
+        while (off[k + 1] <= c0) k++;  // segment of the chunk's first value
+        int l = k;
+        while (off[l + 1] < c1) l++;  // segment of its last value
+        cs[c] = Chunk16{k, l};
+    }
+    return cs;
+}
+
+int pack16_table(const float *src, half_t *dst, unsigned *amax, unsigned *part, const size_t *off,
+                 const Chunk16 *cs, int nchunk, int nseg, size_t total, hipStream_t st) {
+    if (nseg < 1 || nseg > 65535 || nchunk < 1) return fail(RSG_EINVAL, "pack16: bad segment table");
+    if (((uintptr_t)src & 15) || ((uintptr_t)dst & 7)) return fail(RSG_EINVAL, "pack16: misaligned buffers");
     if (int rc = zero_u32(amax, nseg, st)) return rc;
-    const dim3 g = seg_grid(maxlen, nseg);
-    amax_table_kernel<<<g, kThreads, 0, st>>>(src, amax, off);
-    RSG_LAUNCH_CHECK("amax_table_kernel");
-    pack16_table_kernel<<<g, kThreads, 0, st>>>(src, dst, amax, off);
-    RSG_LAUNCH_CHECK("pack16_table_kernel");
+    amax_chunks_kernel<<<nchunk, kThreads, 0, st>>>(src, amax, part, off, cs, total);
+    RSG_LAUNCH_CHECK("amax_chunks_kernel");
+    amax_finish_kernel<<<nseg, kThreads, 0, st>>>(amax, part, off, total);
+    RSG_LAUNCH_CHECK("amax_finish_kernel");
+    pack16_chunks_kernel<<<nchunk, kThreads, 0, st>>>(src, dst, amax, off, cs, total);
+    RSG_LAUNCH_CHECK("pack16_chunks_kernel");
     return RSG_OK;
 }
 
-int unpack16_table(const half_t *src, const unsigned *amax, float *dst, const size_t *off, int nseg,
-                   size_t maxlen, hipStream_t st) {
-    if (nseg < 1 || nseg > 65535) return fail(RSG_EINVAL, "unpack16: 1..65535 segments");
-    unpack16_table_kernel<<<seg_grid(maxlen, nseg), kThreads, 0, st>>>(src, amax, dst, off);
-    RSG_LAUNCH_CHECK("unpack16_table_kernel");
+int unpack16_table(const half_t *src, const unsigned *amax, float *dst, const size_t *off, const Chunk16 *cs,
+                   int nchunk, size_t total, hipStream_t st) {
+    if (nchunk < 1) return fail(RSG_EINVAL, "unpack16: empty segment table");
+    if (((uintptr_t)src & 7) || ((uintptr_t)dst & 15)) return fail(RSG_EINVAL, "unpack16: misaligned buffers");
+    unpack16_chunks_kernel<<<nchunk, kThreads, 0, st>>>(src, amax, dst, off, cs, total);
+    RSG_LAUNCH_CHECK("unpack16_chunks_kernel");
     return RSG_OK;
 }
 

@@ -176,3 +176,56 @@ def test_struct_layouts_match_header(tmp_path):
         assert int(out[cname]) == C.sizeof(py), cname
         for fname, _ in py._fields_:
             assert int(out[f"{cname}.{fname}"]) == getattr(py, fname).offset, (cname, fname)
+
+
+def _kernel_private_sizes():
+    """{kernel: private segment bytes per lane} for every gfx950 kernel of
+    librsgpu.so, read from the metadata notes of its code objects (the
+    .hip_fatbin section holds one offload bundle per source file)."""
+    import struct
+    import subprocess
+    import tempfile
+    llvm = "/opt/rocm/llvm/bin"
+    if not os.path.exists(os.path.join(llvm, "llvm-readelf")):
+        pytest.skip("llvm-readelf not installed")
+    magic = b"__CLANG_OFFLOAD_BUNDLE__"
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        fb = os.path.join(tmp, "fatbin")
+        subprocess.run([os.path.join(llvm, "llvm-objcopy"), "--dump-section", f".hip_fatbin={fb}", R.LIBPATH,
+                        os.path.join(tmp, "lib.so")], check=True, capture_output=True)
+        data = open(fb, "rb").read()
+        start = data.find(magic)
+        assert start >= 0
+        while start >= 0:
+            n, = struct.unpack_from("<Q", data, start + 24)
+            pos = start + 32
+            for _ in range(n):
+                off, size, tl = struct.unpack_from("<QQQ", data, pos)
+                triple = data[pos + 24:pos + 24 + tl].decode()
+                pos += 24 + tl
+                if "gfx950" in triple and size:
+                    co = os.path.join(tmp, "co.o")
+                    open(co, "wb").write(data[start + off:start + off + size])
+                    notes = subprocess.run([os.path.join(llvm, "llvm-readelf"), "--notes", co], check=True,
+                                           capture_output=True, text=True).stdout
+                    name = None
+                    for line in notes.splitlines():
+                        line = line.strip()
+                        if line.startswith(".name:"):
+                            name = line.split(":", 1)[1].strip()
+                        elif line.startswith(".private_segment_fixed_size:") and name:
+                            out[name] = int(line.split(":", 1)[1])
+            start = data.find(magic, start + 1)
+    return out
+
+
+def test_no_kernel_uses_scratch():
+    """Register spills or a private copy of a kernel's argument struct put
+    per-lane data in scratch memory, which costs an order of magnitude in
+    the stencil kernels: every kernel of the library must use none."""
+    sizes = _kernel_private_sizes()
+    assert len(sizes) > 100
+    assert any("e2d_stress_ragged_kernel" in k for k in sizes)
+    bad = {k: v for k, v in sizes.items() if v}
+    assert not bad, bad
