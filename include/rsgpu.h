@@ -249,7 +249,9 @@ typedef struct {
                                        power-of-two scale per stored array          */
     int misfit_type;                /* rs_fwimisfit (utils.h:26): 0 DIFFERENCE,
                                        1 CORRELATION, 2 ADAPTIVE_GAUSS,
-                                       3 ADAPTIVE_LINEAR                             */
+                                       3 ADAPTIVE_LINEAR, 4 ADAPTIVE_SINGLE (the
+                                       acoustic classes' default branch:
+                                       DIFFERENCE)                                   */
     const float *weight;            /* data weights like obs (setDataweight) or NULL */
     int adjoint;                    /* RSG_ADJOINT_REFERENCE or RSG_ADJOINT_EXACT    */
 } rsg_fwi_a2d;
@@ -286,7 +288,7 @@ typedef struct {
     int ncheck;                          /* Revolve snapshots                        */
     int fp16;                            /* FP16 checkpoints                         */
     int misfit_type;                     /* 0 DIFFERENCE, 1 CORRELATION, 2 ADAPTIVE_GAUSS,
-                                            3 ADAPTIVE_LINEAR                         */
+                                            3 ADAPTIVE_LINEAR, 4 (= DIFFERENCE)       */
     const float *weight;                 /* data weights like obs, or NULL           */
 } rsg_fwi_a3d;
 
@@ -294,9 +296,12 @@ int rsg_fwi_a3d_gradient(const rsg_grid *g, const rsg_fwi_a3d *f, void *stream);
 
 /* ---- Elastic 2D FWI gradient (FwiElastic2D::run_optimal, fwi.cpp:3937-4143)
  * for a ragged batch of shots, each on its own aperture window.  Per shot:
- * forward elastic steps with the source and P / Ux / Uz gathers, DIFFERENCE
- * misfit and residuals of the three records, adjoint-state steps driven by
- * the residuals, crossCorr (fwi.cpp:2591-2775) at every reverse step with
+ * forward elastic steps with the source and P / Ux / Uz gathers, the misfit
+ * and residuals of the three records (FwiElastic2D::computeMisfit /
+ * computeResiduals, fwi.cpp:2852-3773: DIFFERENCE, CORRELATION,
+ * ADAPTIVE_GAUSS, ADAPTIVE_LINEAR, ADAPTIVE_SINGLE, optional data weights),
+ * adjoint-state steps driven by the residuals, crossCorr
+ * (fwi.cpp:2591-2775) at every reverse step with
  * Revolve checkpoints of all shots in HBM.  The windows' kernels are mapped
  * by scaleGrad (fwi.cpp:2778-2849) when scale = 1, then added into the
  * stacked images at x offset ox (the reference stacks shot images in its
@@ -311,6 +316,8 @@ typedef struct {
                                             receiver set (maps and shifts per field)  */
     float *dmod[3], *dres[3];            /* out: modelled records and residuals        */
     float *wavgrad;                      /* out (optional): wavelet gradient           */
+    const float *weight[3];              /* data weights of P, Ux, Uz like obs, or NULL
+                                            (setDataweightp/x/z)                       */
 } rsg_fwi_e2d_shot;
 
 typedef struct {
@@ -321,10 +328,31 @@ typedef struct {
     int nx, nz;        /* stacked image size                                           */
     float *grad[3];    /* in/out (device): gradients added here, nz*nx each            */
     double *misfit;    /* out (host): misfit summed over the shots                     */
+    int misfit_type;   /* 0 DIFFERENCE, 1 CORRELATION, 2 ADAPTIVE_GAUSS, 3 ADAPTIVE_LINEAR,
+                          4 ADAPTIVE_SINGLE (rs_fwimisfit, utils.h:26); the 97th-percentile
+                          clips of types 1-3 are per shot, as the reference runs one
+                          shot at a time; types 2-4 leave the P residual at zero      */
 } rsg_fwi_e2d;
 
 int rsg_fwi_e2d_gradient_batch(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *shots, int nshot,
                                void *stream);
+
+/* ---- elastic misfit and residuals alone (FwiElastic2D::computeMisfit and
+ * computeResiduals, fwi.cpp:2852-3773), shot by shot as the reference runs
+ * them: type as rsg_fwi_e2d.misfit_type; per shot the P, Ux, Uz records
+ * [nrec][nt] (device), optional weights, the Ux receiver map [nrec][2]
+ * (traces with a negative entry count in no misfit); *misfit (host,
+ * optional) = the misfit summed over the shots in double. */
+typedef struct {
+    int nrec;
+    const int *map;                      /* Ux receiver map (device)                   */
+    const float *mod[3], *obs[3];        /* modelled and observed P, Ux, Uz (device)    */
+    const float *weight[3];              /* data weights or NULL                        */
+    float *res[3];                       /* out: residuals (device)                     */
+} rsg_e2d_records;
+
+int rsg_misfit_residuals_e2d(int type, const rsg_e2d_records *shots, int nshot, int nt, double *misfit,
+                             void *stream);
 
 /* Revolve::revolve() action sequence (revolve.cpp:150-305): advance=0,
  * takeshot=1, restore=2, firsturn=3, youturn=4, terminate=5, error=6. */

@@ -154,6 +154,11 @@ def _declare(L):
         f.restype = ct
         f = getattr(L, "orc_residual_a" + suf)
         f.argtypes = [C.c_int, rp, rp, C.c_void_p, rp, C.c_int, C.c_int, C.c_int]
+        f = getattr(L, "orc_misfit_e" + suf)
+        f.argtypes = [C.c_int] + [rp] * 6 + [C.c_void_p] * 3 + [_i32p, C.c_int, C.c_int]
+        f.restype = ct
+        f = getattr(L, "orc_residual_e" + suf)
+        f.argtypes = [C.c_int] + [rp] * 6 + [C.c_void_p] * 3 + [rp] * 3 + [C.c_int, C.c_int]
         f = getattr(L, "orc_interp" + suf)
         f.argtypes = [C.c_int, rp, C.c_int, C.c_int, ct, ct, rp, C.c_int, ct, ct]
         f = getattr(L, "orc_a3d_crosscorr" + suf)
@@ -210,7 +215,7 @@ def _declare_ref(R):
                                       _f32p, _f32p, _f32p, _f32p])
     R.ref_fwi_a2d_full.restype = C.c_float
     R.ref_fwi_e2d.argtypes = ([_f32p, _f32p, _f32p] + [C.c_int] * 3 + [C.c_float] * 3 + [C.c_int] * 6
-                              + [_f32p, _f32p, C.c_int] + [_f32p] * 8)
+                              + [_f32p, _f32p, C.c_int] + [_f32p] * 8 + [C.c_int] + [C.c_void_p] * 3)
     R.ref_fwi_e2d.restype = C.c_float
     R.ref_fwi_a3d.argtypes = ([_f32p, _f32p] + [C.c_int] * 4 + [C.c_float] * 4 + [C.c_int] * 4
                               + [_f32p, _f32p, C.c_int, _f32p, _f32p] + [_f32p] * 5)

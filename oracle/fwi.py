@@ -264,15 +264,26 @@ def _e2d_record(s, field, lpml, m, sh, factor, data, it, real):
         record2d(s.Vx if field == "VX" else s.Vz, lpml, m, sh, data, it, real)
 
 
+def _weights3(weights, nrec, nt, real):
+    """Three optional [nrec, nt] weight arrays (P, Ux, Uz) or None."""
+    if weights is None:
+        return [None] * 3
+    return [None if w is None else np.ascontiguousarray(w, real).reshape(nrec, nt) for w in weights]
+
+
 def fwi_e2d(vp, vs, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, srcfield, scoords, wav, gcoords, dobs,
-            real=np.float32):
+            real=np.float32, misfit_type=0, weights=None, residuals=None):
     """FwiElastic2D::run_optimal (fwi.cpp:3937-4143) with P, Ux and Uz data
-    (dobs[3, nrec, nt]): forward elastic steps, DIFFERENCE misfit and
-    residuals of the three records (fwi.cpp:2852-2930, 3227-3300), adjoint
-    with backwardstepVelocity/Stress (waves.cpp:1738-2080) driven by the three
+    (dobs[3, nrec, nt]): forward elastic steps, the misfit and residuals of
+    the three records (fwi.cpp:2852-3773; misfit_type 0 DIFFERENCE,
+    1 CORRELATION, 2 ADAPTIVE_GAUSS, 3 ADAPTIVE_LINEAR, 4 ADAPTIVE_SINGLE;
+    weights: None or three optional [nrec, nt] data weights), adjoint with
+    backwardstepVelocity/Stress (waves.cpp:1738-2080) driven by the three
     residuals, crossCorr (fwi.cpp:2591-2775) at every reverse step, then
     scaleGrad (fwi.cpp:2778-2849).  Returns the lambda/mu/rho kernels before
-    scaleGrad as well as the Vp/Vs/rho gradients after it."""
+    scaleGrad as well as the Vp/Vs/rho gradients after it.  residuals: None,
+    or [3, nrec, nt] residuals that drive the adjoint in place of the
+    computed ones (to check an adjoint pass on given residuals)."""
     nz, nx = vp.shape
     smap, sshift = makemap2d(scoords, nx, nz, dx, dz, srcfield)
     maps = {f: makemap2d(gcoords, nx, nz, dx, dz, f) for f in E2D_FIELDS}
@@ -327,12 +338,15 @@ def fwi_e2d(vp, vs, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, srcfield, scoo
                 fstep(it, not reverse)
         elif a == orv.FIRSTURN:
             fstep(capo, True)
-            misfit = fn("misfit_diff3", real)(dmod[0].reshape(-1), dobs[0].reshape(-1), dmod[1].reshape(-1),
-                                              dobs[1].reshape(-1), dmod[2].reshape(-1), dobs[2].reshape(-1),
-                                              valid, nrec, nt)
-            for k in range(3):
-                fn("residual_diff", real)(dmod[k].reshape(-1), dobs[k].reshape(-1), dres[k].reshape(-1),
-                                          nrec, nt, 1)
+            wts = _weights3(weights, nrec, nt, real)
+            wp = [None if w is None else w.ctypes.data for w in wts]
+            md = [dmod[k].reshape(-1) for k in range(3)] + [dobs[k].reshape(-1) for k in range(3)]
+            md = [md[0], md[3], md[1], md[4], md[2], md[5]]
+            misfit = fn("misfit_e", real)(misfit_type, *md, *wp, valid, nrec, nt)
+            fn("residual_e", real)(misfit_type, *md, *wp, dres[0].reshape(-1), dres[1].reshape(-1),
+                                   dres[2].reshape(-1), nrec, nt)
+            if residuals is not None:
+                dres[...] = np.asarray(residuals, real).reshape(dres.shape)
             inject_res(capo)
             corr(capo)
             reverse = True
@@ -354,7 +368,8 @@ def fwi_e2d(vp, vs, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, srcfield, scoo
                 dres=dres, wavgrad=wgrad)
 
 
-def ref_fwi_e2d(vp, vs, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, srcfield, scoords, wav, gcoords, dobs):
+def ref_fwi_e2d(vp, vs, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, srcfield, scoords, wav, gcoords, dobs,
+                misfit_type=0, weights=None):
     nz, nx = vp.shape
     sc = np.ascontiguousarray(scoords, np.float32).reshape(-1, 2)
     gc = np.ascontiguousarray(gcoords, np.float32).reshape(-1, 2)
@@ -364,10 +379,12 @@ def ref_fwi_e2d(vp, vs, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, srcfield, 
     dres = np.zeros((3, nrec, nt), np.float32)
     wg = np.zeros((nsrc, nt), np.float32)
     f32 = lambda a: np.ascontiguousarray(a, np.float32).ravel()  # noqa: E731
+    wts = _weights3(weights, nrec, nt, np.float32)  # kept alive across the call
     m = ref().ref_fwi_e2d(f32(vp), f32(vs), f32(rho), nx, nz, lpml, dx, dz, dt, nt, order, int(fs), ncheck,
                           {"P": 0, "VX": 1, "VZ": 3}[srcfield], nsrc, sc.ravel(), f32(wav), nrec, gc.ravel(),
                           f32(dobs), out["vpgrad"].reshape(-1), out["vsgrad"].reshape(-1),
-                          out["rhograd"].reshape(-1), dmod.reshape(-1), dres.reshape(-1), wg.reshape(-1))
+                          out["rhograd"].reshape(-1), dmod.reshape(-1), dres.reshape(-1), wg.reshape(-1),
+                          int(misfit_type), *[None if w is None else w.ctypes.data for w in wts])
     return dict(misfit=np.float32(m), dmod=dmod, dres=dres, wavgrad=wg, **out)
 
 

@@ -373,12 +373,14 @@ float ref_fwi_a2d_full(const float *vp, const float *rho, int nx, int nz, int lp
 // and Uz data (dobs = 3 records of nrec*nt: P, Ux, Uz), the source field
 // srcfield (0 P, 1 VX, 3 VZ).  Outputs the scaled Vp/Vs/rho gradients, the
 // modelled data and residuals (3 records each), the wavelet gradient; returns
-// the misfit.
+// the misfit.  misfit_type as rs_fwimisfit; weip/weix/weiz: optional data
+// weights of the P, Ux, Uz records (nullptr = not set).
 float ref_fwi_e2d(const float *vp, const float *vs, const float *rho, int nx, int nz, int lpml, float dx,
                   float dz, float dt, int nt, int order, int fs, int ncheck, int srcfield, int nsrc,
                   const float *scoords, const float *wav, int nrec, const float *gcoords,
                   const float *dobs, float *vpgrad, float *vsgrad, float *rhograd, float *dmod,
-                  float *dres, float *wgrad) {
+                  float *dres, float *wgrad, int misfit_type, const float *weip, const float *weix,
+                  const float *weiz) {
     auto model = std::make_shared<ModelElastic2D<float>>(nx, nz, lpml, dx, dz, 0.f, 0.f, fs != 0);
     model->createModel();
     std::memcpy(model->getVp(), vp, sizeof(float) * nx * nz);
@@ -426,7 +428,17 @@ float ref_fwi_e2d(const float *vp, const float *vs, const float *rho, int nx, in
     fwi->setWavgrad(wg);
     fwi->setNcheck(ncheck);
     fwi->setIncore(true);
-    fwi->setMisfit_type(DIFFERENCE);
+    // DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS, ADAPTIVE_LINEAR, ADAPTIVE_SINGLE
+    fwi->setMisfit_type((rs_fwimisfit)misfit_type);
+    const float *wei[3] = {weip, weix, weiz};
+    std::shared_ptr<Data2D<float>> wd[3];
+    for (int k = 0; k < 3; k++) {
+        if (!wei[k]) continue;
+        wd[k] = mk(fields[k], wei[k]);
+        if (k == 0) fwi->setDataweightp(wd[k]);
+        if (k == 1) fwi->setDataweightx(wd[k]);
+        if (k == 2) fwi->setDataweightz(wd[k]);
+    }
     fwi->run_optimal();
     std::memcpy(vpgrad, vg->getImagedata(), sizeof(float) * nx * nz);
     std::memcpy(vsgrad, sg->getImagedata(), sizeof(float) * nx * nz);

@@ -1319,6 +1319,32 @@ static REAL FN(adaptive_shaper_)(const REAL *m, const REAL *r, const REAL *wt, i
     return norm;
 }
 
+/* Residual of one trace for ADAPTIVE_GAUSS (type 2) / ADAPTIVE_LINEAR (3)
+ * (fwi.cpp:851-1014, per component fwi.cpp:3440-3595): the shaper's own
+ * misfit, rs = sgn (H^2 - 2 misfit) shaper / norm, solved again with R and
+ * convolved with the weighted record, then weighted.  w: 6*nt, shaper: nt. */
+static void FN(adaptive_res_)(int type, const REAL *m, const REAL *r, const REAL *wt, REAL *rs, int nt, REAL *w,
+                              REAL *shaper, int scaling)
+{
+    REAL norm = FN(adaptive_shaper_)(m, r, wt, nt, w, shaper);
+    REAL misfit = 0.0;
+    for (int it = 0; it < nt; it++) {
+        REAL h = FN(adaptive_h_)(type, it, nt);
+        misfit = (REAL)((double)misfit + 0.5 * (double)(h * shaper[it]) * (double)(h * shaper[it]) / (double)norm);
+    }
+    for (int it = 0; it < nt; it++) {
+        REAL h = FN(adaptive_h_)(type, it, nt);
+        double sgn = type == 2 ? -1.0 * scaling : (double)scaling;
+        rs[it] = (REAL)(sgn * ((double)(h * h) - 2.0 * (double)misfit) * (double)shaper[it] / (double)norm);
+    }
+    /* solve again with the residual as right-hand side, then convolve with
+     * the weighted record */
+    FN(stoep_)(nt, w + 2 * nt, rs, shaper, w + 4 * nt);
+    FN(conv_)(nt, 0, shaper, nt, 0, w + nt, nt, 0, rs);
+    if (wt)
+        for (int it = 0; it < nt; it++) rs[it] *= wt[it];
+}
+
 REAL FN(orc_misfit_a)(int type, const REAL *mod, const REAL *rec, const REAL *wei, const int *valid, int ntr,
                       int nt)
 {
@@ -1397,28 +1423,9 @@ void FN(orc_residual_a)(int type, const REAL *mod, const REAL *rec, const REAL *
     }
     if (type == 2 || type == 3) {
         REAL *w = (REAL *)malloc(sizeof(REAL) * 7 * (nt > 0 ? nt : 1)), *shaper = w + 6 * nt;
-        for (int i = 0; i < ntr; i++) {
-            const REAL *m = mod + (long)i * nt, *r = rec + (long)i * nt, *wt = wei ? wei + (long)i * nt : NULL;
-            REAL *rs = res + (long)i * nt;
-            REAL norm = FN(adaptive_shaper_)(m, r, wt, nt, w, shaper);
-            REAL misfit = 0.0;
-            for (int it = 0; it < nt; it++) {
-                REAL h = FN(adaptive_h_)(type, it, nt);
-                misfit = (REAL)((double)misfit + 0.5 * (double)(h * shaper[it]) * (double)(h * shaper[it]) /
-                                                     (double)norm);
-            }
-            for (int it = 0; it < nt; it++) {
-                REAL h = FN(adaptive_h_)(type, it, nt);
-                double sgn = type == 2 ? -1.0 * scaling : (double)scaling;
-                rs[it] = (REAL)(sgn * ((double)(h * h) - 2.0 * (double)misfit) * (double)shaper[it] / (double)norm);
-            }
-            /* solve again with the residual as right-hand side, then convolve
-             * with the weighted record */
-            FN(stoep_)(nt, w + 2 * nt, rs, shaper, w + 4 * nt);
-            FN(conv_)(nt, 0, shaper, nt, 0, w + nt, nt, 0, rs);
-            if (wt)
-                for (int it = 0; it < nt; it++) rs[it] *= wt[it];
-        }
+        for (int i = 0; i < ntr; i++)
+            FN(adaptive_res_)(type, mod + (long)i * nt, rec + (long)i * nt, wei ? wei + (long)i * nt : NULL,
+                              res + (long)i * nt, nt, w, shaper, scaling);
         free(w);
     } else {
     for (int i = 0; i < ntr; i++) {
@@ -1451,6 +1458,243 @@ void FN(orc_residual_a)(int type, const REAL *mod, const REAL *rec, const REAL *
         REAL a = res[p] < 0 ? -res[p] : res[p];
         if (a >= pclip) res[p] = 0.0;
     }
+}
+
+/* ---- Elastic 2D misfits and residuals (FwiElastic2D::computeMisfit /
+ * computeResiduals, fwi.cpp:2852-3773) over the P, Ux and Uz records of one
+ * shot, [ntr][nt] each; data weights optional per component (NULL = not
+ * set); `valid[i]` nonzero when trace i's Ux map is inside the model. */
+
+/* Joint shaping filter of ADAPTIVE_SINGLE: the Ux and Uz traces (weighted)
+ * back to back as one 2 nt sequence, auto- and crosscorrelation over nt lags,
+ * whitening, Levinson.  w: 2nt modwei, 2nt recwei, 2nt reversed copy, nt
+ * autocorr, nt crosscorr, nt spiker; returns the squared norm (1 when 0). */
+static REAL FN(single_shaper_)(const REAL *mx, const REAL *rx, const REAL *wx, const REAL *mz, const REAL *rz,
+                               const REAL *wz, int nt, REAL *w, REAL *shaper)
+{
+    REAL *modwei = w, *recwei = w + 2 * nt, *xr = w + 4 * nt, *autoc = w + 6 * nt, *cross = w + 7 * nt,
+         *spiker = w + 8 * nt;
+    for (int it = 0; it < nt; it++) {
+        modwei[it] = wx ? mx[it] * wx[it] : mx[it];
+        recwei[it] = wx ? rx[it] * wx[it] : rx[it];
+        modwei[nt + it] = wz ? mz[it] * wz[it] : mz[it];
+        recwei[nt + it] = wz ? rz[it] * wz[it] : rz[it];
+    }
+    FN(xcor_)(2 * nt, 0, recwei, 2 * nt, 0, recwei, nt, 0, autoc, xr);
+    FN(xcor_)(2 * nt, 0, recwei, 2 * nt, 0, modwei, nt, 0, cross, xr);
+    if (autoc[0] == 0.0) autoc[0] = 1.0;
+    autoc[0] = (REAL)((double)autoc[0] * (1.0 + 0.01));
+    for (int it = 0; it < nt; it++) shaper[it] = 0.0;
+    FN(stoep_)(nt, autoc, cross, shaper, spiker);
+    REAL norm = 0.0;
+    for (int it = 0; it < nt; it++) norm += shaper[it] * shaper[it];
+    if (norm == 0.0) norm = 1.0;
+    return norm;
+}
+
+static int FN(finite_)(REAL v) { return !isnan(v) && !isinf(v); }
+
+/* Misfit: type 0 DIFFERENCE 0.5*(rp^2+rx^2+rz^2) (weighted differences),
+ * 1 CORRELATION sum of -mod*rec/(n1 n2) (weighted, non-finite terms 0) over
+ * the three records, 2/3 ADAPTIVE_GAUSS/LINEAR -/+ 0.5*(Hx sx)^2/|sx|^2 +
+ * (Hz sz)^2/|sz|^2 over Ux and Uz, 4 ADAPTIVE_SINGLE with one joint shaper;
+ * accumulated in T with the reference's promotions. */
+REAL FN(orc_misfit_e)(int type, const REAL *modp, const REAL *recp, const REAL *modx, const REAL *recx,
+                      const REAL *modz, const REAL *recz, const REAL *weip, const REAL *weix, const REAL *weiz,
+                      const int *valid, int ntr, int nt)
+{
+    REAL misfit = 0.0;
+    REAL *w = (REAL *)malloc(sizeof(REAL) * 12 * (nt > 0 ? nt : 1));
+    REAL *shx = w + 10 * nt, *shz = w + 11 * nt;
+    for (int i = 0; i < ntr; i++) {
+        if (!valid[i]) continue;
+        const long o = (long)i * nt;
+        const REAL *mp = modp + o, *rp = recp + o, *mx = modx + o, *rx = recx + o, *mz = modz + o, *rz = recz + o;
+        const REAL *wp = weip ? weip + o : NULL, *wx = weix ? weix + o : NULL, *wz = weiz ? weiz + o : NULL;
+        if (type == 0) {
+            for (int it = 0; it < nt; it++) {
+                REAL a = mp[it] - rp[it], b = mx[it] - rx[it], c = mz[it] - rz[it];
+                if (wp) a *= wp[it];
+                if (wx) b *= wx[it];
+                if (wz) c *= wz[it];
+                misfit = (REAL)((double)misfit + 0.5 * (double)(a * a + b * b + c * c));
+            }
+        } else if (type == 1) {
+            REAL p1 = 0.0, p2 = 0.0, x1 = 0.0, x2 = 0.0, z1 = 0.0, z2 = 0.0;
+            for (int it = 0; it < nt; it++) {
+                p1 += mp[it] * mp[it];
+                p2 += rp[it] * rp[it];
+                x1 += mx[it] * mx[it];
+                x2 += rx[it] * rx[it];
+                z1 += mz[it] * mz[it];
+                z2 += rz[it] * rz[it];
+            }
+            p1 = (REAL)sqrt((double)p1);
+            p2 = (REAL)sqrt((double)p2);
+            if (p1 == 0) p1 = 1.0;
+            if (p2 == 0) p2 = 1.0;
+            x1 = (REAL)sqrt((double)x1);
+            x2 = (REAL)sqrt((double)x2);
+            if (x1 == 0) x1 = 1.0;
+            if (x2 == 0) x2 = 1.0;
+            z1 = (REAL)sqrt((double)z1);
+            z2 = (REAL)sqrt((double)z2);
+            if (z1 == 0) z1 = 1.0;
+            if (z2 == 0) z2 = 1.0;
+            for (int it = 0; it < nt; it++) {
+                REAL a = (REAL)((-1.0) * (double)(mp[it] * rp[it] / (p1 * p2)));
+                REAL b = (REAL)((-1.0) * (double)(mx[it] * rx[it] / (x1 * x2)));
+                REAL c = (REAL)((-1.0) * (double)(mz[it] * rz[it] / (z1 * z2)));
+                if (wp) a *= wp[it];
+                if (wx) b *= wx[it];
+                if (wz) c *= wz[it];
+                if (!FN(finite_)(a)) a = 0.0;
+                if (!FN(finite_)(b)) b = 0.0;
+                if (!FN(finite_)(c)) c = 0.0;
+                misfit += (a + b + c);
+            }
+        } else if (type == 2 || type == 3) {
+            REAL nx = FN(adaptive_shaper_)(mx, rx, wx, nt, w, shx);
+            REAL nz = FN(adaptive_shaper_)(mz, rz, wz, nt, w, shz);
+            for (int it = 0; it < nt; it++) {
+                REAL h = FN(adaptive_h_)(type, it, nt);
+                REAL a = h * shx[it], b = h * shz[it];
+                REAL v = a * a / nx + b * b / nz;
+                misfit = (REAL)((double)misfit + (type == 2 ? -0.5 : 0.5) * (double)v);
+            }
+        } else {
+            REAL nx = FN(single_shaper_)(mx, rx, wx, mz, rz, wz, nt, w, shx);
+            for (int it = 0; it < nt; it++) {
+                REAL a = FN(adaptive_h_)(3, it, nt) * shx[it];
+                misfit = (REAL)((double)misfit + 0.5 * (double)(a * a / nx));
+            }
+        }
+    }
+    free(w);
+    return misfit;
+}
+
+/* Value at the 97th percentile of |a| (element (int)(97*nt*ntr/100) of the
+ * sorted magnitudes, fwi.cpp:3380-3390). */
+static REAL FN(pclip_)(const REAL *a, int ntr, int nt)
+{
+    const long n = (long)ntr * nt;
+    REAL *f = (REAL *)malloc(sizeof(REAL) * (n > 0 ? n : 1));
+    for (long p = 0; p < n; p++) f[p] = a[p] < 0 ? -a[p] : a[p];
+    qsort(f, n, sizeof(REAL), FN(orc_cmp));
+    REAL v = f[(long)(int)((97L * nt * ntr) / 100)];
+    free(f);
+    return v;
+}
+
+/* Combined clip of Ux and Uz: the larger of the two percentiles; both
+ * components zeroed where |resx| reaches it (|resz| is not tested), then
+ * with `nanfix` non-finite values zeroed (fwi.cpp:3372-3406). */
+static void FN(clip_xz_)(REAL *rx, REAL *rz, int ntr, int nt, int nanfix)
+{
+    const long n = (long)ntr * nt;
+    if (n == 0) return;
+    REAL pc = FN(pclip_)(rx, ntr, nt), pz = FN(pclip_)(rz, ntr, nt);
+    if (pz > pc) pc = pz;
+    for (long p = 0; p < n; p++) {
+        REAL a = rx[p] < 0 ? -rx[p] : rx[p];
+        if (a >= pc) {
+            rx[p] = 0.0;
+            rz[p] = 0.0;
+        }
+        if (nanfix) {
+            if (!FN(finite_)(rx[p])) rx[p] = 0.0;
+            if (!FN(finite_)(rz[p])) rz[p] = 0.0;
+        }
+    }
+}
+
+/* Residuals of every trace: type 0 (mod-rec)*w^2; 1 CORRELATION
+ * -(rec/(n1 n2) - mod/(n1 n1) * sum(mod rec)/(n1 n2)) * w^2, combined Ux/Uz
+ * clip with non-finite values zeroed, P clipped on its own; 2/3 the
+ * ADAPTIVE residual of Ux and Uz separately, combined clip; 4 the joint
+ * ADAPTIVE_SINGLE residual, no clip.  Types 2-4 leave P at zero. */
+void FN(orc_residual_e)(int type, const REAL *modp, const REAL *recp, const REAL *modx, const REAL *recx,
+                        const REAL *modz, const REAL *recz, const REAL *weip, const REAL *weix, const REAL *weiz,
+                        REAL *resp, REAL *resx, REAL *resz, int ntr, int nt)
+{
+    const long n = (long)ntr * nt;
+    const REAL *mod[3] = {modp, modx, modz}, *rec[3] = {recp, recx, recz}, *wei[3] = {weip, weix, weiz};
+    REAL *res[3] = {resp, resx, resz};
+    if (type == 0) {
+        for (int c = 0; c < 3; c++)
+            for (long p = 0; p < n; p++) {
+                res[c][p] = mod[c][p] - rec[c][p];
+                if (wei[c]) res[c][p] *= wei[c][p] * wei[c][p];
+            }
+        return;
+    }
+    if (type == 1) {
+        for (int c = 0; c < 3; c++)
+            for (int i = 0; i < ntr; i++) {
+                const REAL *m = mod[c] + (long)i * nt, *r = rec[c] + (long)i * nt;
+                REAL n1 = 0.0, n2 = 0.0, n3 = 0.0;
+                for (int it = 0; it < nt; it++) {
+                    n1 += m[it] * m[it];
+                    n2 += r[it] * r[it];
+                    n3 += m[it] * r[it];
+                }
+                n1 = (REAL)sqrt((double)n1);
+                n2 = (REAL)sqrt((double)n2);
+                if (n1 == 0) n1 = 1.0;
+                if (n2 == 0) n2 = 1.0;
+                n3 /= (n1 * n2);
+                for (int it = 0; it < nt; it++) {
+                    long p = (long)i * nt + it;
+                    res[c][p] = (REAL)((-1.0) * (double)((r[it] / (n1 * n2)) - (m[it] / (n1 * n1)) * n3));
+                    if (wei[c]) res[c][p] *= wei[c][p] * wei[c][p];
+                }
+            }
+        FN(clip_xz_)(resx, resz, ntr, nt, 1);
+        if (n > 0) {
+            REAL pc = FN(pclip_)(resp, ntr, nt);
+            for (long p = 0; p < n; p++) {
+                REAL a = resp[p] < 0 ? -resp[p] : resp[p];
+                if (a >= pc) resp[p] = 0.0;
+                if (!FN(finite_)(resp[p])) resp[p] = 0.0;
+            }
+        }
+        return;
+    }
+    for (long p = 0; p < n; p++) resp[p] = 0.0;
+    REAL *w = (REAL *)malloc(sizeof(REAL) * 11 * (nt > 0 ? nt : 1)), *shaper = w + 10 * nt;
+    if (type == 2 || type == 3) {
+        for (int c = 1; c < 3; c++)
+            for (int i = 0; i < ntr; i++)
+                FN(adaptive_res_)(type, mod[c] + (long)i * nt, rec[c] + (long)i * nt,
+                                  wei[c] ? wei[c] + (long)i * nt : NULL, res[c] + (long)i * nt, nt, w, shaper, 1);
+        free(w);
+        FN(clip_xz_)(resx, resz, ntr, nt, 0);
+        return;
+    }
+    for (int i = 0; i < ntr; i++) {
+        const long o = (long)i * nt;
+        const REAL *wx = weix ? weix + o : NULL, *wz = weiz ? weiz + o : NULL;
+        REAL norm = FN(single_shaper_)(modx + o, recx + o, wx, modz + o, recz + o, wz, nt, w, shaper);
+        REAL misfit = 0.0;
+        for (int it = 0; it < nt; it++) {
+            REAL h = FN(adaptive_h_)(3, it, nt);
+            misfit = (REAL)((double)misfit + 0.5 * (double)(h * shaper[it]) * (double)(h * shaper[it]) / (double)norm);
+        }
+        REAL *cross = w + 7 * nt;
+        for (int it = 0; it < nt; it++) {
+            REAL h = FN(adaptive_h_)(3, it, nt);
+            cross[it] = (REAL)(((double)(h * h) - 2.0 * (double)misfit) * (double)shaper[it] / (double)norm);
+        }
+        FN(stoep_)(nt, w + 6 * nt, cross, shaper, w + 8 * nt);
+        /* the joint residual (2 nt) overwrites the weighted model traces */
+        FN(conv_)(nt, 0, shaper, 2 * nt, 0, w + 2 * nt, 2 * nt, 0, w);
+        for (int it = 0; it < nt; it++) {
+            resx[o + it] = wx ? w[it] * wx[it] : w[it];
+            resz[o + it] = wz ? w[nt + it] * wz[it] : w[nt + it];
+        }
+    }
+    free(w);
 }
 
 /* DIFFERENCE misfit 0.5*sum(r^2) over traces with valid maps, accumulated

@@ -66,12 +66,18 @@ class FwiA3D(C.Structure):
 class FwiE2DShot(C.Structure):
     _fields_ = ([("g", Grid)] + [(n, C.c_void_p) for n in ("L", "L2M", "M", "Rx", "Rz", "Vp", "Vs", "Rho")]
                 + [("ox", C.c_int), ("src", Traces), ("obs", Traces * 3), ("dmod", C.c_void_p * 3),
-                   ("dres", C.c_void_p * 3), ("wavgrad", C.c_void_p)])
+                   ("dres", C.c_void_p * 3), ("wavgrad", C.c_void_p), ("weight", C.c_void_p * 3)])
+
+
+class E2DRecords(C.Structure):
+    _fields_ = [("nrec", C.c_int), ("map", C.c_void_p), ("mod", C.c_void_p * 3), ("obs", C.c_void_p * 3),
+                ("weight", C.c_void_p * 3), ("res", C.c_void_p * 3)]
 
 
 class FwiE2D(C.Structure):
     _fields_ = [("tab", C.c_void_p), ("ncheck", C.c_int), ("fp16", C.c_int), ("scale", C.c_int),
-                ("nx", C.c_int), ("nz", C.c_int), ("grad", C.c_void_p * 3), ("misfit", C.POINTER(C.c_double))]
+                ("nx", C.c_int), ("nz", C.c_int), ("grad", C.c_void_p * 3), ("misfit", C.POINTER(C.c_double)),
+                ("misfit_type", C.c_int)]
 
 
 FWI_REVOLVE, FWI_STORE, FWI_FULL = 0, 1, 2
@@ -139,6 +145,7 @@ def _declare(L):
         "rsg_fwi_a3d_gradient": ([GP, C.POINTER(FwiA3D), vp], ip),
         "rsg_interp": ([ip, vp, ip, ip, fp, fp, vp, ip, fp, fp, vp], ip),
         "rsg_misfit_residuals": ([ip, vp, vp, vp, vp, vp, ip, ip, ip, ip, ip, C.POINTER(C.c_double), vp], ip),
+        "rsg_misfit_residuals_e2d": ([ip, C.POINTER(E2DRecords), ip, ip, C.POINTER(C.c_double), vp], ip),
     }
     for name in ("a2d_forward_velocity", "a2d_forward_stress", "e2d_forward_velocity",
                  "e2d_forward_stress", "e2d_backward_velocity", "e2d_backward_stress",
@@ -163,7 +170,7 @@ EXPORTS = (
     "rsg_revolve_numforw", "rsg_snap_cadence", "rsg_fwi_a2d_gradient_batch", "rsg_a2d_forward_velocity_batch",
     "rsg_a2d_forward_stress_batch", "rsg_e2d_velocity_batch", "rsg_e2d_stress_batch",
     "rsg_ragged2d_create", "rsg_ragged2d_step", "rsg_ragged2d_destroy", "rsg_fwi_e2d_gradient_batch",
-    "rsg_fwi_a3d_gradient", "rsg_interp", "rsg_misfit_residuals",
+    "rsg_fwi_a3d_gradient", "rsg_interp", "rsg_misfit_residuals", "rsg_misfit_residuals_e2d",
 )
 
 

@@ -292,7 +292,7 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
     if (f->adjoint != RSG_ADJOINT_REFERENCE && f->adjoint != RSG_ADJOINT_EXACT)
         return fail(RSG_EINVAL, "fwi: adjoint must be RSG_ADJOINT_REFERENCE or RSG_ADJOINT_EXACT");
     const bool exact = f->adjoint == RSG_ADJOINT_EXACT;
-    if (exact && (f->mode != RSG_FWI_REVOLVE || f->misfit_type != 0))
+    if (exact && (f->mode != RSG_FWI_REVOLVE || (f->misfit_type != 0 && f->misfit_type != 4)))
         return fail(RSG_EINVAL, "fwi: the exact adjoint takes RSG_FWI_REVOLVE and the DIFFERENCE misfit");
     const bool f16 = f->fp16 != 0;
     if (f16 && 4 * ns > 65535) return fail(RSG_EINVAL, "fwi: FP16 storage takes at most 16383 shots");

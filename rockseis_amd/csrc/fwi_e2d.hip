@@ -4,8 +4,9 @@
 // together, one launch per half step (rsg_ragged2d plans).
 //
 // Per shot, as in the reference: forward elastic steps with the source and
-// the P / Ux / Uz gathers; at the first turn the DIFFERENCE misfit of the
-// three records (fwi.cpp:2852-2930) and their residuals (fwi.cpp:3227-3300);
+// the P / Ux / Uz gathers; at the first turn the misfit of the three records
+// and their residuals (any of the five misfit types, data weights; the
+// percentile clips per shot, fwi.cpp:2852-3773, misfit.hip);
 // the adjoint wavefield advanced with backwardstepVelocity/Stress
 // (waves.cpp:1738-2080) and driven by the three residuals; at every reverse
 // step crossCorr (fwi.cpp:2591-2775) into the lambda / mu / rho kernels of
@@ -144,33 +145,6 @@ __global__ __launch_bounds__(256) void e2d_crosscorr_kernel(const ShotDev *__res
     uz = (float)((double)uz + 0.5 * (double)wrz[q] * rz0 * (double)(wszz[q + n] - wszz[q]) / (double)dz);
     uz = (float)((double)uz + 0.5 * (double)wrz[q - n] * rz1 * (double)(wszz[q] - wszz[q - n]) / (double)dz);
     s.rho[m] -= (ux + uz);
-}
-
-// Elastic DIFFERENCE misfit and residuals: res = mod - rec for P, Ux, Uz;
-// misfit += 0.5*(rp^2 + rx^2 + rz^2) over traces whose Ux map is valid,
-// reduced in double (blockIdx.y = shot).
-__global__ __launch_bounds__(256) void e2d_misfit_kernel(const ShotDev *__restrict__ sd, int nt, double *misfit) {
-    __shared__ double part[256];
-    const ShotDev &s = sd[blockIdx.y];
-    const size_t n = (size_t)s.nrec * nt;
-    double acc = 0.0;
-    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-        const float rp = s.dmod[0][p] - s.dobs[0][p];
-        const float rx = s.dmod[1][p] - s.dobs[1][p];
-        const float rz = s.dmod[2][p] - s.dobs[2][p];
-        s.dres[0][p] = rp;
-        s.dres[1][p] = rx;
-        s.dres[2][p] = rz;
-        const size_t tr = p / nt;
-        if (s.gmap[1][2 * tr] >= 0 && s.gmap[1][2 * tr + 1] >= 0) acc += 0.5 * (double)(rp * rp + rx * rx + rz * rz);
-    }
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(misfit, part[0]);
 }
 
 // Stack every window's kernels into the caller's images (x offset ox, z
@@ -382,6 +356,20 @@ static int fwi_e2d_impl(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *sh, int ns
         if (s.wavgrad)
             if (int rc = launch_zero(s.wavgrad, sizeof(float) * d.nsrc * nt, st)) return rc;
     }
+    // the three records of every shot for the misfit and residuals
+    std::vector<E2DRecs> recs(ns);
+    for (int k = 0; k < ns; k++) {
+        E2DRecs &r = recs[k];
+        r = E2DRecs{};
+        r.nrec = sh[k].obs[0].ntrace;
+        r.map = sh[k].obs[1].map;  // traces count in the misfit by their Ux map (fwi.cpp:2911)
+        for (int c = 0; c < 3; c++) {
+            r.mod[c] = sh[k].dmod[c];
+            r.obs[c] = sh[k].obs[c].data;
+            r.wei[c] = sh[k].weight[c];
+            r.res[c] = sh[k].dres[c];
+        }
+    }
     PlanGuard pf, pb;
     if (int rc = ragged2d_create(RSG_KIND_E2D, ns, grids.data(), sf.data(), &pf.p)) return rc;
     if (int rc = ragged2d_create(RSG_KIND_E2D, ns, grids.data(), sb.data(), &pb.p)) return rc;
@@ -455,10 +443,8 @@ static int fwi_e2d_impl(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *sh, int ns
                 if (int rc = fstep(it, !reverse)) return rc;
         } else if (a == Revolve::FIRSTURN) {
             if (int rc = fstep(capo, true)) return rc;
-            if (maxrec) {
-                e2d_misfit_kernel<<<dim3(64, ns), 256, 0, st>>>(dsd, nt, dmis);
-                RSG_LAUNCH_CHECK("e2d_misfit_kernel");
-            }
+            if (maxrec)
+                if (int rc = elastic_misfit_residuals(f->misfit_type, recs, nt, dmis, st)) return rc;
             if (int rc = reverse_tail(capo)) return rc;
             reverse = true;
         } else if (a == Revolve::YOUTURN) {
@@ -506,5 +492,9 @@ extern "C" int rsg_fwi_e2d_gradient_batch(const rsg_fwi_e2d *f, const rsg_fwi_e2
     if (!f || !shots || nshot < 1 || nshot > 65535) return rsg::fail(RSG_EINVAL, "rsg_fwi_e2d_gradient_batch: bad arguments");
     if (!f->tab || !f->grad[0] || !f->grad[1] || !f->grad[2] || f->nx < 1 || f->nz < 1)
         return rsg::fail(RSG_EINVAL, "rsg_fwi_e2d_gradient_batch: null tables or images");
+    if (f->misfit_type < 0 || f->misfit_type > 4)
+        return rsg::fail(RSG_EINVAL,
+                         "rsg_fwi_e2d_gradient_batch: misfit type must be DIFFERENCE (0), CORRELATION (1), "
+                         "ADAPTIVE_GAUSS (2), ADAPTIVE_LINEAR (3) or ADAPTIVE_SINGLE (4)");
     return rsg::fwi_e2d_impl(f, shots, nshot, (hipStream_t)stream);
 }

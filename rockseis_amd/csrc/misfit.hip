@@ -10,7 +10,9 @@
 // radix select over the bit patterns of |res| (non-negative floats order as
 // unsigned integers), four 8-bit passes per shot.  Misfits are reduced in
 // double (the reference sums in T).
+#include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "rsg_common.h"
@@ -102,24 +104,6 @@ __global__ __launch_bounds__(kThreads) void misfit_res_kernel(int type, const fl
         __syncthreads();
     }
     if (threadIdx.x == 0) atomicAdd(misfit, part[0]);
-}
-
-// Histogram of the 8-bit digit at `shift` of |res| bit patterns whose higher
-// bits equal prefix[shot] (under mask), blockIdx.y = shot.
-__global__ __launch_bounds__(kThreads) void radix_hist_kernel(const float *__restrict__ res, size_t nper,
-                                                               const unsigned *__restrict__ prefix, unsigned mask,
-                                                               int shift, unsigned *hist) {
-    __shared__ unsigned h[256];
-    h[threadIdx.x] = 0u;
-    __syncthreads();
-    const float *r = res + blockIdx.y * nper;
-    const unsigned pf = prefix[blockIdx.y];
-    for (size_t p = blockIdx.x * (size_t)kThreads + threadIdx.x; p < nper; p += (size_t)gridDim.x * kThreads) {
-        const unsigned key = __float_as_uint(r[p]) & 0x7fffffffu;
-        if ((key & mask) == pf) atomicAdd(&h[(key >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    if (h[threadIdx.x]) atomicAdd(&hist[blockIdx.y * 256 + threadIdx.x], h[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(kThreads) void clip_kernel(float *__restrict__ res, size_t nper,
@@ -306,26 +290,19 @@ __device__ void stoep_dev(int n, const float *r, const float *g, float *f, float
     }
 }
 
-// One lane per trace (shots' traces back to back).  scratch: 7 nt floats per
-// trace (weighted model, weighted record, autocorrelation, crosscorrelation,
-// Levinson scratch, reversed copy, shaper).
-__global__ __launch_bounds__(64) void adaptive_kernel(int type, const float *__restrict__ mod,
-                                                      const float *__restrict__ rec, const float *__restrict__ wei,
-                                                      float *__restrict__ res, const int *__restrict__ map, int dim,
-                                                      int ntr_all, int nt, int scaling, const float *__restrict__ H,
-                                                      float *__restrict__ scratch, double *misfit) {
-    const int tr = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tr >= ntr_all) return;
-    const size_t o = (size_t)tr * nt;
-    const float *m = mod + o, *r = rec + o, *wt = wei ? wei + o : nullptr;
-    float *w = scratch + (size_t)tr * 7 * nt;
+// Shaping filter of one trace: autocorrelation of the weighted record and
+// its crosscorrelation with the weighted model trace (the record reversed,
+// convolved), 1% white noise on the zero lag, Levinson solve.  w: 6 nt
+// scratch (weighted model, weighted record, autocorrelation,
+// crosscorrelation, Levinson scratch, reversed copy); returns the shaper's
+// squared norm (1 when zero).
+__device__ float shaper_dev(const float *m, const float *r, const float *wt, int nt, float *w, float *shaper) {
     float *modwei = w, *recwei = w + nt, *autoc = w + 2 * nt, *cross = w + 3 * nt, *spiker = w + 4 * nt,
-          *xr = w + 5 * nt, *shaper = w + 6 * nt, *rs = res + o;
+          *xr = w + 5 * nt;
     for (int it = 0; it < nt; it++) {
         modwei[it] = wt ? m[it] * wt[it] : m[it];
         recwei[it] = wt ? r[it] * wt[it] : r[it];
     }
-    // autocorrelation and crosscorrelation: the record reversed, convolved
     for (int i = 0, j = nt - 1; i < nt; ++i, --j) xr[i] = recwei[j];
     conv_dev(nt, 1 - nt, xr, nt, 0, recwei, nt, 0, autoc);
     conv_dev(nt, 1 - nt, xr, nt, 0, modwei, nt, 0, cross);
@@ -336,14 +313,14 @@ __global__ __launch_bounds__(64) void adaptive_kernel(int type, const float *__r
     float norm = 0.f;
     for (int it = 0; it < nt; it++) norm += shaper[it] * shaper[it];
     if (norm == 0.f) norm = 1.f;
-    if (trace_valid(map, dim, tr)) {
-        double acc = 0.0;
-        for (int it = 0; it < nt; it++) {
-            const float v = H[it] * shaper[it];
-            acc += 0.5 * (double)v * (double)v / (double)norm;
-        }
-        atomicAdd(misfit, type == 2 ? -acc : acc);
-    }
+    return norm;
+}
+
+// Residual of one trace after shaper_dev on the same scratch: the shaper's
+// own misfit in float, rs = sgn (H^2 - 2 misfit) shaper / norm, solved again
+// with the autocorrelation, convolved with the weighted record, weighted.
+__device__ void adaptive_res_dev(int type, int nt, const float *wt, const float *H, float *w, float *shaper,
+                                 float norm, int scaling, float *rs) {
     float mis = 0.f;
     for (int it = 0; it < nt; it++) {
         const float v = H[it] * shaper[it];
@@ -352,10 +329,285 @@ __global__ __launch_bounds__(64) void adaptive_kernel(int type, const float *__r
     const double sgn = type == 2 ? -1.0 * scaling : (double)scaling;
     for (int it = 0; it < nt; it++)
         rs[it] = (float)(sgn * ((double)(H[it] * H[it]) - 2.0 * (double)mis) * (double)shaper[it] / (double)norm);
-    stoep_dev(nt, autoc, rs, shaper, spiker);
-    conv_dev(nt, 0, shaper, nt, 0, recwei, nt, 0, rs);
+    stoep_dev(nt, w + 2 * nt, rs, shaper, w + 4 * nt);
+    conv_dev(nt, 0, shaper, nt, 0, w + nt, nt, 0, rs);
     if (wt)
         for (int it = 0; it < nt; it++) rs[it] *= wt[it];
+}
+
+// One lane per trace (shots' traces back to back).  scratch: 7 nt floats per
+// trace (shaper_dev's six and the shaper).
+__global__ __launch_bounds__(64) void adaptive_kernel(int type, const float *__restrict__ mod,
+                                                      const float *__restrict__ rec, const float *__restrict__ wei,
+                                                      float *__restrict__ res, const int *__restrict__ map, int dim,
+                                                      int ntr_all, int nt, int scaling, const float *__restrict__ H,
+                                                      float *__restrict__ scratch, double *misfit) {
+    const int tr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tr >= ntr_all) return;
+    const size_t o = (size_t)tr * nt;
+    const float *wt = wei ? wei + o : nullptr;
+    float *w = scratch + (size_t)tr * 7 * nt, *shaper = w + 6 * nt;
+    const float norm = shaper_dev(mod + o, rec + o, wt, nt, w, shaper);
+    if (trace_valid(map, dim, tr)) {
+        double acc = 0.0;
+        for (int it = 0; it < nt; it++) {
+            const float v = H[it] * shaper[it];
+            acc += 0.5 * (double)v * (double)v / (double)norm;
+        }
+        atomicAdd(misfit, type == 2 ? -acc : acc);
+    }
+    adaptive_res_dev(type, nt, wt, H, w, shaper, norm, scaling, res + o);
+}
+
+// ---- Elastic records (FwiElastic2D::computeMisfit / computeResiduals,
+// fwi.cpp:2852-3773).  blockIdx.y = shot for the sample-parallel kernels;
+// the per-trace kernels find their shot in tr0 (prefix of nrec).
+
+__device__ inline bool finite_f(float v) { return !isnan(v) && !isinf(v); }
+
+__device__ inline bool e2d_valid(const E2DRecs &s, int tr) { return s.map[2 * tr] >= 0 && s.map[2 * tr + 1] >= 0; }
+
+// DIFFERENCE (fwi.cpp:2915-2937, 3288-3308): misfit 0.5*(a^2+b^2+c^2) of the
+// weighted differences, residuals (mod-rec)*w^2.
+__global__ __launch_bounds__(kThreads) void e2d_diff_kernel(const E2DRecs *__restrict__ sd, int nt, double *misfit) {
+    __shared__ double part[kThreads];
+    const E2DRecs &s = sd[blockIdx.y];
+    const size_t n = (size_t)s.nrec * nt;
+    double acc = 0.0;
+    for (size_t p = blockIdx.x * (size_t)kThreads + threadIdx.x; p < n; p += (size_t)gridDim.x * kThreads) {
+        float a[3];
+        for (int c = 0; c < 3; c++) {
+            const float d = s.mod[c][p] - s.obs[c][p];
+            a[c] = d;
+            float q = d;
+            if (s.wei[c]) {
+                const float w = s.wei[c][p];
+                a[c] = d * w;
+                q = d * (w * w);
+            }
+            s.res[c][p] = q;
+        }
+        if (e2d_valid(s, (int)(p / nt))) acc += 0.5 * (double)(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(misfit, part[0]);
+}
+
+// CORRELATION norms, one lane per (trace, component): |mod|, |rec| (zero
+// replaced by 1) and sum(mod*rec)/(|mod||rec|) (fwi.cpp:3310-3358).
+__global__ __launch_bounds__(kThreads) void e2d_corr_norms_kernel(const E2DRecs *__restrict__ sd, int nt) {
+    const E2DRecs &s = sd[blockIdx.y];
+    const int tr = blockIdx.x * kThreads + threadIdx.x, c = blockIdx.z;
+    if (tr >= s.nrec) return;
+    const float *m = s.mod[c] + (size_t)tr * nt, *r = s.obs[c] + (size_t)tr * nt;
+    float a = 0.f, b = 0.f, x = 0.f;
+    for (int it = 0; it < nt; it++) {
+        a += m[it] * m[it];
+        b += r[it] * r[it];
+        x += m[it] * r[it];
+    }
+    a = (float)sqrt((double)a);
+    b = (float)sqrt((double)b);
+    if (a == 0.f) a = 1.f;
+    if (b == 0.f) b = 1.f;
+    x /= (a * b);
+    float *o = s.norms + 3 * ((size_t)c * s.nrec + tr);
+    o[0] = a;
+    o[1] = b;
+    o[2] = x;
+}
+
+// CORRELATION misfit (fwi.cpp:2939-2990: -mod*rec/(n1 n2), weighted,
+// non-finite terms 0, summed over the three records) and residuals
+// (fwi.cpp:3359-3374: -(rec/(n1 n2) - mod/(n1 n1) n3) * w^2) before the clips.
+__global__ __launch_bounds__(kThreads) void e2d_corr_kernel(const E2DRecs *__restrict__ sd, int nt, double *misfit) {
+    __shared__ double part[kThreads];
+    const E2DRecs &s = sd[blockIdx.y];
+    const size_t n = (size_t)s.nrec * nt;
+    double acc = 0.0;
+    for (size_t p = blockIdx.x * (size_t)kThreads + threadIdx.x; p < n; p += (size_t)gridDim.x * kThreads) {
+        const int tr = (int)(p / nt);
+        const bool valid = e2d_valid(s, tr);
+        float v[3];
+        for (int c = 0; c < 3; c++) {
+            const float *nn = s.norms + 3 * ((size_t)c * s.nrec + tr);
+            const float n1 = nn[0], n2 = nn[1], n3 = nn[2];
+            const float m = s.mod[c][p], r = s.obs[c][p];
+            float t = -(m * r / (n1 * n2));
+            if (s.wei[c]) t *= s.wei[c][p];
+            v[c] = finite_f(t) ? t : 0.f;
+            float q = -((r / (n1 * n2)) - (m / (n1 * n1)) * n3);
+            if (s.wei[c]) q *= s.wei[c][p] * s.wei[c][p];
+            s.res[c][p] = q;
+        }
+        if (valid) acc += (double)(v[0] + v[1] + v[2]);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(misfit, part[0]);
+}
+
+__device__ inline int shot_of(const int *tr0, int ns, int g) {
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tr0[mid] <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// ADAPTIVE_GAUSS / ADAPTIVE_LINEAR, one lane per trace (global traces
+// [j0, j0 + nj) of all shots), the Ux and Uz shapers side by side (scratch
+// 14 nt): misfit -/+ 0.5*((H sx)^2/|sx|^2 + (H sz)^2/|sz|^2) summed in float
+// per sample as fwi.cpp:2990-3060, then each component's residual
+// (fwi.cpp:3440-3510).
+__global__ __launch_bounds__(64) void e2d_adaptive_kernel(int type, const E2DRecs *__restrict__ sd,
+                                                          const int *__restrict__ tr0, int ns, int j0, int nj, int nt,
+                                                          const float *__restrict__ H, float *__restrict__ scratch,
+                                                          double *misfit) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nj) return;
+    const int g = j0 + j, k = shot_of(tr0, ns, g), tr = g - tr0[k];
+    const E2DRecs &s = sd[k];
+    const size_t o = (size_t)tr * nt;
+    const float *wx = s.wei[1] ? s.wei[1] + o : nullptr, *wz = s.wei[2] ? s.wei[2] + o : nullptr;
+    float *w = scratch + (size_t)j * 14 * nt, *shx = w + 6 * nt, *wzs = w + 7 * nt, *shz = w + 13 * nt;
+    const float nx = shaper_dev(s.mod[1] + o, s.obs[1] + o, wx, nt, w, shx);
+    const float nz = shaper_dev(s.mod[2] + o, s.obs[2] + o, wz, nt, wzs, shz);
+    if (e2d_valid(s, tr)) {
+        double acc = 0.0;
+        for (int it = 0; it < nt; it++) {
+            const float a = H[it] * shx[it], b = H[it] * shz[it];
+            acc += 0.5 * (double)(a * a / nx + b * b / nz);
+        }
+        atomicAdd(misfit, type == 2 ? -acc : acc);
+    }
+    adaptive_res_dev(type, nt, wx, H, w, shx, nx, 1, s.res[1] + o);
+    adaptive_res_dev(type, nt, wz, H, wzs, shz, nz, 1, s.res[2] + o);
+}
+
+// ADAPTIVE_SINGLE (fwi.cpp:3149-3200, 3675-3752): the weighted Ux and Uz
+// traces back to back as one 2 nt sequence, one shaping filter of nt lags,
+// linear time weight; no clip.  scratch 10 nt per lane: weighted model and
+// record (2 nt each), reversed copy (2 nt), autocorrelation,
+// crosscorrelation, Levinson scratch, shaper.
+__global__ __launch_bounds__(64) void e2d_single_kernel(const E2DRecs *__restrict__ sd, const int *__restrict__ tr0,
+                                                        int ns, int j0, int nj, int nt, const float *__restrict__ H,
+                                                        float *__restrict__ scratch, double *misfit) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nj) return;
+    const int g = j0 + j, k = shot_of(tr0, ns, g), tr = g - tr0[k];
+    const E2DRecs &s = sd[k];
+    const size_t o = (size_t)tr * nt;
+    const float *mx = s.mod[1] + o, *rx = s.obs[1] + o, *mz = s.mod[2] + o, *rz = s.obs[2] + o;
+    const float *wx = s.wei[1] ? s.wei[1] + o : nullptr, *wz = s.wei[2] ? s.wei[2] + o : nullptr;
+    const int n2 = 2 * nt;
+    float *w = scratch + (size_t)j * 10 * nt;
+    float *modwei = w, *recwei = w + n2, *xr = w + 2 * n2, *autoc = w + 6 * nt, *cross = w + 7 * nt,
+          *spiker = w + 8 * nt, *shaper = w + 9 * nt;
+    for (int it = 0; it < nt; it++) {
+        modwei[it] = wx ? mx[it] * wx[it] : mx[it];
+        recwei[it] = wx ? rx[it] * wx[it] : rx[it];
+        modwei[nt + it] = wz ? mz[it] * wz[it] : mz[it];
+        recwei[nt + it] = wz ? rz[it] * wz[it] : rz[it];
+    }
+    for (int i = 0, q = n2 - 1; i < n2; ++i, --q) xr[i] = recwei[q];
+    conv_dev(n2, 1 - n2, xr, n2, 0, recwei, nt, 0, autoc);
+    conv_dev(n2, 1 - n2, xr, n2, 0, modwei, nt, 0, cross);
+    if (autoc[0] == 0.f) autoc[0] = 1.f;
+    autoc[0] = (float)((double)autoc[0] * (1.0 + 0.01));
+    for (int it = 0; it < nt; it++) shaper[it] = 0.f;
+    stoep_dev(nt, autoc, cross, shaper, spiker);
+    float norm = 0.f;
+    for (int it = 0; it < nt; it++) norm += shaper[it] * shaper[it];
+    if (norm == 0.f) norm = 1.f;
+    if (e2d_valid(s, tr)) {
+        double acc = 0.0;
+        for (int it = 0; it < nt; it++) {
+            const float a = H[it] * shaper[it];
+            acc += 0.5 * (double)(a * a / norm);
+        }
+        atomicAdd(misfit, acc);
+    }
+    float mis = 0.f;
+    for (int it = 0; it < nt; it++) {
+        const float v = H[it] * shaper[it];
+        mis = (float)((double)mis + 0.5 * (double)v * (double)v / (double)norm);
+    }
+    for (int it = 0; it < nt; it++)
+        cross[it] = (float)(((double)(H[it] * H[it]) - 2.0 * (double)mis) * (double)shaper[it] / (double)norm);
+    stoep_dev(nt, autoc, cross, shaper, spiker);
+    conv_dev(nt, 0, shaper, n2, 0, recwei, n2, 0, modwei);  // the joint residual overwrites the model traces
+    float *rsx = s.res[1] + o, *rsz = s.res[2] + o;
+    for (int it = 0; it < nt; it++) {
+        rsx[it] = wx ? modwei[it] * wx[it] : modwei[it];
+        rsz[it] = wz ? modwei[nt + it] * wz[it] : modwei[nt + it];
+    }
+}
+
+// Clips of every shot: Ux and Uz zeroed together where |res_x| reaches the
+// larger of their two percentiles pc[3k], pc[3k+1] (|res_z| is not tested,
+// fwi.cpp:3376-3406); with `corr` non-finite values zeroed and P clipped on
+// its own at pc[3k+2] (fwi.cpp:3408-3430).
+__global__ __launch_bounds__(kThreads) void e2d_clip_kernel(const E2DRecs *__restrict__ sd, int nt,
+                                                            const float *__restrict__ pc, int corr) {
+    const int k = blockIdx.y;
+    const E2DRecs &s = sd[k];
+    const size_t n = (size_t)s.nrec * nt;
+    float cx = pc[3 * k];
+    if (pc[3 * k + 1] > cx) cx = pc[3 * k + 1];
+    const float cp = pc[3 * k + 2];
+    for (size_t p = blockIdx.x * (size_t)kThreads + threadIdx.x; p < n; p += (size_t)gridDim.x * kThreads) {
+        float rx = s.res[1][p], rz = s.res[2][p];
+        if (fabsf(rx) >= cx) {
+            rx = 0.f;
+            rz = 0.f;
+        }
+        if (corr) {
+            if (!finite_f(rx)) rx = 0.f;
+            if (!finite_f(rz)) rz = 0.f;
+            float rp = s.res[0][p];
+            if (fabsf(rp) >= cp) rp = 0.f;
+            if (!finite_f(rp)) rp = 0.f;
+            s.res[0][p] = rp;
+        }
+        s.res[1][p] = rx;
+        s.res[2][p] = rz;
+    }
+}
+
+// Histogram of the 8-bit digit at `shift` of |v| bit patterns whose higher
+// bits equal prefix[seg] (under mask), blockIdx.y = segment of segs.
+struct SelSeg {
+    const float *p;
+    size_t n;
+};
+__global__ __launch_bounds__(kThreads) void radix_hist_seg_kernel(const SelSeg *__restrict__ segs,
+                                                                   const unsigned *__restrict__ prefix, unsigned mask,
+                                                                   int shift, unsigned *hist) {
+    __shared__ unsigned h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const SelSeg sg = segs[blockIdx.y];
+    const unsigned pf = prefix[blockIdx.y];
+    for (size_t p = blockIdx.x * (size_t)kThreads + threadIdx.x; p < sg.n; p += (size_t)gridDim.x * kThreads) {
+        const unsigned key = __float_as_uint(sg.p[p]) & 0x7fffffffu;
+        if ((key & mask) == pf) atomicAdd(&h[(key >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&hist[blockIdx.y * 256 + threadIdx.x], h[threadIdx.x]);
 }
 
 __global__ void fill_u32(unsigned *p, int n) {
@@ -363,60 +615,116 @@ __global__ void fill_u32(unsigned *p, int n) {
     if (i < n) p[i] = 0u;
 }
 
+struct FreeList {
+    std::vector<void *> p;
+    ~FreeList() {
+        for (void *q : p) (void)hipFree(q);
+    }
+    template <class T>
+    int alloc(T **out, size_t n, const char *what) {
+        void *q = nullptr;
+        if (hipMalloc(&q, sizeof(T) * (n ? n : 1)) != hipSuccess) return fail(RSG_ENOMEM, what);
+        p.push_back(q);
+        *out = (T *)q;
+        return RSG_OK;
+    }
+};
+
+// k[s]-th smallest |v| (0-based) of every segment, exactly: a radix select
+// over the bit patterns of |v| (non-negative floats order as unsigned
+// integers), four 8-bit passes, all segments per launch (at most 65535 a
+// launch).  out[s] = the bit pattern.
+int select_kth(const std::vector<SelSeg> &segs, const std::vector<unsigned long long> &kth, std::vector<unsigned> &out,
+               hipStream_t st) {
+    const int nseg = (int)segs.size();
+    out.assign(nseg, 0u);
+    if (nseg == 0) return RSG_OK;
+    FreeList fr;
+    SelSeg *dsegs = nullptr;
+    unsigned *hist = nullptr, *prefix = nullptr;
+    const int cap = std::min(nseg, 65535);
+    if (int rc = fr.alloc(&dsegs, nseg, "fwi: misfit scratch allocation failed")) return rc;
+    if (int rc = fr.alloc(&hist, 256 * (size_t)cap, "fwi: misfit scratch allocation failed")) return rc;
+    if (int rc = fr.alloc(&prefix, nseg, "fwi: misfit scratch allocation failed")) return rc;
+    RSG_HIP(hipMemcpyAsync(dsegs, segs.data(), sizeof(SelSeg) * nseg, hipMemcpyHostToDevice, st));
+    size_t maxn = 1;
+    for (const SelSeg &g : segs) maxn = std::max(maxn, g.n);
+    unsigned blocks = (unsigned)std::min<size_t>((maxn + kThreads * 8 - 1) / (kThreads * 8), 1024);
+    if (blocks < 1) blocks = 1;
+    std::vector<unsigned long long> k(kth);
+    std::vector<unsigned> h(256 * (size_t)cap);
+    for (int pass = 0; pass < 4; pass++) {
+        const int shift = 24 - 8 * pass;
+        const unsigned mask = pass == 0 ? 0u : (0xffffffffu << (32 - 8 * pass));
+        RSG_HIP(hipMemcpyAsync(prefix, out.data(), sizeof(unsigned) * nseg, hipMemcpyHostToDevice, st));
+        for (int s0 = 0; s0 < nseg; s0 += cap) {
+            const int ng = std::min(cap, nseg - s0);
+            fill_u32<<<(256 * ng + kThreads - 1) / kThreads, kThreads, 0, st>>>(hist, 256 * ng);
+            RSG_LAUNCH_CHECK("fill_u32");
+            radix_hist_seg_kernel<<<dim3(blocks, ng), kThreads, 0, st>>>(dsegs + s0, prefix + s0, mask, shift, hist);
+            RSG_LAUNCH_CHECK("radix_hist_seg_kernel");
+            RSG_HIP(hipMemcpyAsync(h.data(), hist, sizeof(unsigned) * 256 * ng, hipMemcpyDeviceToHost, st));
+            RSG_HIP(hipStreamSynchronize(st));
+            for (int j = 0; j < ng; j++) {
+                const int sg = s0 + j;
+                unsigned long long cum = 0;
+                int b = 0;
+                for (; b < 256; b++) {
+                    const unsigned long long c = h[(size_t)j * 256 + b];
+                    if (cum + c > k[sg]) break;
+                    cum += c;
+                }
+                if (b == 256) b = 255;  // unreachable while k < n
+                out[sg] |= (unsigned)b << shift;
+                k[sg] -= cum;
+            }
+        }
+    }
+    return RSG_OK;
+}
+
+// Host table of the ADAPTIVE time weight (fwi.cpp:397-420) in the
+// reference's arithmetic: type 2 Gaussian exp(-it^2/(2 s^2)), s = (float)(0.01
+// nt); otherwise the ramp min(it, 500).
+std::vector<float> adaptive_h(int type, int nt) {
+    std::vector<float> h(nt);
+    const float sd = type == 2 ? (float)(0.01 * nt) : 500.f;
+    for (int it = 0; it < nt; it++) {
+        if (type == 2)
+            h[it] = sd == 0.f ? 0.f : (float)std::exp(-1.0 * (it * it) / (2.0 * (sd * sd)));
+        else
+            h[it] = it < sd ? (float)it : sd;
+    }
+    return h;
+}
+
 }  // namespace
 
 int acoustic_misfit_residuals(int type, const float *mod, const float *rec, const float *wei, float *res,
                               const int *map, int dim, int ntr, int nt, int nshot, int scaling, double *misfit,
                               hipStream_t st) {
+    if (type == 4) type = 0;  // ADAPTIVE_SINGLE has no acoustic branch: the default one is DIFFERENCE
     if (type < 0 || type > 3)
         return fail(RSG_EINVAL,
-                    "fwi: misfit type must be DIFFERENCE (0), CORRELATION (1), ADAPTIVE_GAUSS (2) or "
-                    "ADAPTIVE_LINEAR (3)");
+                    "fwi: misfit type must be DIFFERENCE (0), CORRELATION (1), ADAPTIVE_GAUSS (2), "
+                    "ADAPTIVE_LINEAR (3) or ADAPTIVE_SINGLE (4)");
     const size_t nper = (size_t)ntr * nt, n = nper * nshot;
     if (n == 0) return RSG_OK;
     float *norms = nullptr;
-    unsigned *hist = nullptr, *prefix = nullptr;
-    struct Free {
-        std::vector<void *> p;
-        ~Free() {
-            for (void *q : p) (void)hipFree(q);
-        }
-    } fr;
+    FreeList fr;
     if (type >= 2) {
-        // time weights in the reference's arithmetic: Gaussian exp(-it^2/(2 s^2)),
-        // s = (float)(0.01 nt), or the ramp min(it, 500) (fwi.cpp:397-420)
-        std::vector<float> h(nt);
-        const float sd = type == 2 ? (float)(0.01 * nt) : 500.f;
-        for (int it = 0; it < nt; it++) {
-            if (type == 2)
-                h[it] = sd == 0.f ? 0.f : (float)std::exp(-1.0 * (it * it) / (2.0 * (sd * sd)));
-            else
-                h[it] = it < sd ? (float)it : sd;
-        }
+        const std::vector<float> h = adaptive_h(type, nt);
         float *hd = nullptr, *scr = nullptr;
         const size_t ntr_all = (size_t)ntr * nshot;
-        if (hipMalloc((void **)&hd, sizeof(float) * nt) != hipSuccess ||
-            hipMalloc((void **)&scr, sizeof(float) * 7 * nt * ntr_all) != hipSuccess) {
-            if (hd) (void)hipFree(hd);
-            return fail(RSG_ENOMEM, "fwi: ADAPTIVE misfit scratch allocation failed");
-        }
-        fr.p = {hd, scr};
+        if (int rc = fr.alloc(&hd, nt, "fwi: ADAPTIVE misfit scratch allocation failed")) return rc;
+        if (int rc = fr.alloc(&scr, 7 * (size_t)nt * ntr_all, "fwi: ADAPTIVE misfit scratch allocation failed"))
+            return rc;
         RSG_HIP(hipMemcpyAsync(hd, h.data(), sizeof(float) * nt, hipMemcpyHostToDevice, st));
         adaptive_kernel<<<(unsigned)((ntr_all + 63) / 64), 64, 0, st>>>(type, mod, rec, wei, res, map, dim,
                                                                          (int)ntr_all, nt, scaling, hd, scr, misfit);
         RSG_LAUNCH_CHECK("adaptive_kernel");
-        RSG_HIP(hipStreamSynchronize(st));  // the scratch is freed on return
-        if (hipMalloc((void **)&hist, sizeof(unsigned) * 256 * nshot) != hipSuccess ||
-            hipMalloc((void **)&prefix, sizeof(unsigned) * nshot) != hipSuccess)
-            return fail(RSG_ENOMEM, "fwi: misfit scratch allocation failed");
-        fr.p.push_back(hist);
-        fr.p.push_back(prefix);
     } else if (type == 1) {
-        if (hipMalloc((void **)&norms, sizeof(float) * 3 * (size_t)ntr * nshot) != hipSuccess ||
-            hipMalloc((void **)&hist, sizeof(unsigned) * 256 * nshot) != hipSuccess ||
-            hipMalloc((void **)&prefix, sizeof(unsigned) * nshot) != hipSuccess)
-            return fail(RSG_ENOMEM, "fwi: misfit scratch allocation failed");
-        fr.p = {norms, hist, prefix};
+        if (int rc = fr.alloc(&norms, 3 * (size_t)ntr * nshot, "fwi: misfit scratch allocation failed")) return rc;
         const int nt_all = ntr * nshot;
         corr_norms_kernel<<<(nt_all + kThreads - 1) / kThreads, kThreads, 0, st>>>(mod, rec, nt_all, nt, norms);
         RSG_LAUNCH_CHECK("corr_norms_kernel");
@@ -428,38 +736,121 @@ int acoustic_misfit_residuals(int type, const float *mod, const float *rec, cons
     }
     if (type == 0) return RSG_OK;
     // 97th-percentile clip per shot: element (int)(97*nt*ntr/100) of the sorted |res|
-    const long long kth0 = (97LL * nt * ntr) / 100;
-    std::vector<unsigned long long> k(nshot, (unsigned long long)kth0);
-    std::vector<unsigned> pf(nshot, 0u), h(256 * (size_t)nshot);
+    std::vector<SelSeg> segs(nshot);
+    for (int k = 0; k < nshot; k++) segs[k] = SelSeg{res + k * nper, nper};
+    std::vector<unsigned long long> kth(nshot, (unsigned long long)((97LL * nt * ntr) / 100));
+    std::vector<unsigned> pf;
+    if (int rc = select_kth(segs, kth, pf, st)) return rc;
+    unsigned *dpf = nullptr;
+    if (int rc = fr.alloc(&dpf, nshot, "fwi: misfit scratch allocation failed")) return rc;
+    RSG_HIP(hipMemcpyAsync(dpf, pf.data(), sizeof(unsigned) * nshot, hipMemcpyHostToDevice, st));
     unsigned blocks = (unsigned)std::min<size_t>((nper + kThreads * 8 - 1) / (kThreads * 8), 1024);
     if (blocks < 1) blocks = 1;
-    for (int pass = 0; pass < 4; pass++) {
-        const int shift = 24 - 8 * pass;
-        const unsigned mask = pass == 0 ? 0u : (0xffffffffu << (32 - 8 * pass));
-        RSG_HIP(hipMemcpyAsync(prefix, pf.data(), sizeof(unsigned) * nshot, hipMemcpyHostToDevice, st));
-        fill_u32<<<(256 * nshot + kThreads - 1) / kThreads, kThreads, 0, st>>>(hist, 256 * nshot);
-        RSG_LAUNCH_CHECK("fill_u32");
-        radix_hist_kernel<<<dim3(blocks, nshot), kThreads, 0, st>>>(res, nper, prefix, mask, shift, hist);
-        RSG_LAUNCH_CHECK("radix_hist_kernel");
-        RSG_HIP(hipMemcpyAsync(h.data(), hist, sizeof(unsigned) * 256 * nshot, hipMemcpyDeviceToHost, st));
-        RSG_HIP(hipStreamSynchronize(st));
-        for (int s = 0; s < nshot; s++) {
-            unsigned long long cum = 0;
-            int b = 0;
-            for (; b < 256; b++) {
-                const unsigned long long c = h[(size_t)s * 256 + b];
-                if (cum + c > k[s]) break;
-                cum += c;
+    clip_kernel<<<dim3(blocks, nshot), kThreads, 0, st>>>(res, nper, dpf);
+    RSG_LAUNCH_CHECK("clip_kernel");
+    RSG_HIP(hipStreamSynchronize(st));  // the scratch is freed on return
+    return RSG_OK;
+}
+
+int elastic_misfit_residuals(int type, const std::vector<E2DRecs> &shots_in, int nt, double *misfit,
+                             hipStream_t st) {
+    if (type < 0 || type > 4)
+        return fail(RSG_EINVAL,
+                    "fwi_e2d: misfit type must be DIFFERENCE (0), CORRELATION (1), ADAPTIVE_GAUSS (2), "
+                    "ADAPTIVE_LINEAR (3) or ADAPTIVE_SINGLE (4)");
+    const int ns = (int)shots_in.size();
+    if (ns == 0 || nt == 0) return RSG_OK;
+    std::vector<E2DRecs> shots(shots_in);
+    size_t maxrec = 0, ntr_all = 0;
+    std::vector<int> tr0(ns + 1, 0);
+    for (int k = 0; k < ns; k++) {
+        maxrec = std::max(maxrec, (size_t)shots[k].nrec);
+        ntr_all += shots[k].nrec;
+        tr0[k + 1] = (int)ntr_all;
+    }
+    if (ntr_all == 0) return RSG_OK;
+    FreeList fr;
+    E2DRecs *dsh = nullptr;
+    const char *oom = "fwi_e2d: misfit scratch allocation failed";
+    float *norms = nullptr;
+    if (type == 1) {
+        if (int rc = fr.alloc(&norms, 9 * ntr_all, oom)) return rc;
+        for (int k = 0; k < ns; k++) shots[k].norms = norms + 9 * (size_t)tr0[k];
+    }
+    if (int rc = fr.alloc(&dsh, ns, oom)) return rc;
+    RSG_HIP(hipMemcpyAsync(dsh, shots.data(), sizeof(E2DRecs) * ns, hipMemcpyHostToDevice, st));
+    const size_t maxn = maxrec * nt;
+    unsigned blocks = (unsigned)std::min<size_t>((maxn + kThreads * 8 - 1) / (kThreads * 8), 1024);
+    if (blocks < 1) blocks = 1;
+    if (ns > 65535) return fail(RSG_EINVAL, "fwi_e2d: at most 65535 shots");
+    if (type == 0) {
+        e2d_diff_kernel<<<dim3(blocks, ns), kThreads, 0, st>>>(dsh, nt, misfit);
+        RSG_LAUNCH_CHECK("e2d_diff_kernel");
+        RSG_HIP(hipStreamSynchronize(st));  // the shot table is freed on return
+        return RSG_OK;
+    }
+    if (type == 1) {
+        e2d_corr_norms_kernel<<<dim3((unsigned)((maxrec + kThreads - 1) / kThreads), ns, 3), kThreads, 0, st>>>(dsh,
+                                                                                                              nt);
+        RSG_LAUNCH_CHECK("e2d_corr_norms_kernel");
+        e2d_corr_kernel<<<dim3(blocks, ns), kThreads, 0, st>>>(dsh, nt, misfit);
+        RSG_LAUNCH_CHECK("e2d_corr_kernel");
+    } else {
+        // P residuals stay zero (the reference does not write them for these types)
+        for (int k = 0; k < ns; k++)
+            if (int rc = launch_zero(shots[k].res[0], sizeof(float) * shots[k].nrec * (size_t)nt, st)) return rc;
+        const std::vector<float> h = adaptive_h(type == 4 ? 3 : type, nt);
+        float *hd = nullptr, *scr = nullptr;
+        int *dtr0 = nullptr;
+        const size_t per = (type == 4 ? 10 : 14) * (size_t)nt;
+        // lanes per launch: scratch of at most 1 GiB (at least one wave)
+        const size_t lanes = std::max<size_t>(64, std::min<size_t>(ntr_all, ((size_t)1 << 28) / per));
+        if (int rc = fr.alloc(&hd, nt, oom)) return rc;
+        if (int rc = fr.alloc(&dtr0, ns + 1, oom)) return rc;
+        if (int rc = fr.alloc(&scr, per * lanes, oom)) return rc;
+        RSG_HIP(hipMemcpyAsync(hd, h.data(), sizeof(float) * nt, hipMemcpyHostToDevice, st));
+        RSG_HIP(hipMemcpyAsync(dtr0, tr0.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice, st));
+        for (size_t j0 = 0; j0 < ntr_all; j0 += lanes) {
+            const int nj = (int)std::min(lanes, ntr_all - j0);
+            if (type == 4) {
+                e2d_single_kernel<<<(nj + 63) / 64, 64, 0, st>>>(dsh, dtr0, ns, (int)j0, nj, nt, hd, scr, misfit);
+                RSG_LAUNCH_CHECK("e2d_single_kernel");
+            } else {
+                e2d_adaptive_kernel<<<(nj + 63) / 64, 64, 0, st>>>(type, dsh, dtr0, ns, (int)j0, nj, nt, hd, scr,
+                                                                   misfit);
+                RSG_LAUNCH_CHECK("e2d_adaptive_kernel");
             }
-            if (b == 256) b = 255;  // unreachable while k < nper
-            pf[s] |= (unsigned)b << shift;
-            k[s] -= cum;
+        }
+        if (type == 4) {
+            RSG_HIP(hipStreamSynchronize(st));  // the scratch is freed on return
+            return RSG_OK;
         }
     }
-    RSG_HIP(hipMemcpyAsync(prefix, pf.data(), sizeof(unsigned) * nshot, hipMemcpyHostToDevice, st));
-    clip_kernel<<<dim3(blocks, nshot), kThreads, 0, st>>>(res, nper, prefix);
-    RSG_LAUNCH_CHECK("clip_kernel");
-    RSG_HIP(hipStreamSynchronize(st));
+    // percentiles: |res_x|, |res_z| (and |res_p| for CORRELATION) of every shot
+    std::vector<SelSeg> segs(3 * (size_t)ns, SelSeg{nullptr, 0});
+    std::vector<unsigned long long> kth(3 * (size_t)ns, 0);
+    for (int k = 0; k < ns; k++) {
+        const size_t n = (size_t)shots[k].nrec * nt;
+        const unsigned long long q = (unsigned long long)((97LL * nt * shots[k].nrec) / 100);
+        for (int c = 0; c < 3; c++) {
+            const int comp = c == 2 ? 0 : c + 1;  // order x, z, p
+            if (n && (comp != 0 || type == 1)) segs[3 * k + c] = SelSeg{shots[k].res[comp], n};
+            kth[3 * k + c] = q;
+        }
+    }
+    std::vector<unsigned> bits;
+    if (int rc = select_kth(segs, kth, bits, st)) return rc;
+    std::vector<float> pc(3 * (size_t)ns);
+    for (size_t i = 0; i < pc.size(); i++) {
+        unsigned b = bits[i];
+        std::memcpy(&pc[i], &b, sizeof(float));
+    }
+    float *dpc = nullptr;
+    if (int rc = fr.alloc(&dpc, pc.size(), oom)) return rc;
+    RSG_HIP(hipMemcpyAsync(dpc, pc.data(), sizeof(float) * pc.size(), hipMemcpyHostToDevice, st));
+    e2d_clip_kernel<<<dim3(blocks, ns), kThreads, 0, st>>>(dsh, nt, dpc, type == 1);
+    RSG_LAUNCH_CHECK("e2d_clip_kernel");
+    RSG_HIP(hipStreamSynchronize(st));  // the scratch is freed on return
     return RSG_OK;
 }
 
@@ -482,6 +873,48 @@ extern "C" int rsg_misfit_residuals(int type, const float *mod, const float *rec
         const hipError_t e = hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, st);
         const hipError_t e2 = hipStreamSynchronize(st);
         if (e != hipSuccess || e2 != hipSuccess) rc = rsg::check_hip(e != hipSuccess ? e : e2, "rsg_misfit_residuals");
+    } else {
+        (void)hipStreamSynchronize(st);
+    }
+    (void)hipFree(d);
+    if (rc == RSG_OK && misfit) *misfit = h;
+    return rc;
+}
+
+extern "C" int rsg_misfit_residuals_e2d(int type, const rsg_e2d_records *shots, int nshot, int nt, double *misfit,
+                                        void *stream) {
+    if (!shots || nshot < 1 || nshot > 65535 || nt < 0)
+        return rsg::fail(RSG_EINVAL, "rsg_misfit_residuals_e2d: bad arguments");
+    std::vector<rsg::E2DRecs> recs(nshot);
+    for (int k = 0; k < nshot; k++) {
+        const rsg_e2d_records &a = shots[k];
+        if (a.nrec < 0 || (a.nrec > 0 && !a.map))
+            return rsg::fail(RSG_EINVAL, "rsg_misfit_residuals_e2d: bad shot");
+        rsg::E2DRecs &r = recs[k];
+        r = rsg::E2DRecs{};
+        r.nrec = a.nrec;
+        r.map = a.map;
+        for (int c = 0; c < 3; c++) {
+            if (a.nrec > 0 && (!a.mod[c] || !a.obs[c] || !a.res[c]))
+                return rsg::fail(RSG_EINVAL, "rsg_misfit_residuals_e2d: null record");
+            r.mod[c] = a.mod[c];
+            r.obs[c] = a.obs[c];
+            r.wei[c] = a.weight[c];
+            r.res[c] = a.res[c];
+        }
+    }
+    hipStream_t st = (hipStream_t)stream;
+    double *d = nullptr;
+    if (hipMalloc((void **)&d, sizeof(double)) != hipSuccess)
+        return rsg::fail(RSG_ENOMEM, "rsg_misfit_residuals_e2d: allocation failed");
+    int rc = rsg::launch_zero(d, sizeof(double), st);
+    if (rc == RSG_OK) rc = rsg::elastic_misfit_residuals(type, recs, nt, d, st);
+    double h = 0.0;
+    if (rc == RSG_OK) {
+        const hipError_t e = hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, st);
+        const hipError_t e2 = hipStreamSynchronize(st);
+        if (e != hipSuccess || e2 != hipSuccess)
+            rc = rsg::check_hip(e != hipSuccess ? e : e2, "rsg_misfit_residuals_e2d");
     } else {
         (void)hipStreamSynchronize(st);
     }

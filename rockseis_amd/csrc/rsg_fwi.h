@@ -23,11 +23,30 @@ struct Revolve {
 };
 
 // Misfit (added to *misfit in double) and residuals of nshot shots of ntr
-// traces (shot k's traces follow shot k-1's) for type 0 DIFFERENCE or
-// 1 CORRELATION, optional data weights; map holds dim indices per trace
-// (misfit.hip).
+// traces (shot k's traces follow shot k-1's) for type 0 DIFFERENCE,
+// 1 CORRELATION, 2 ADAPTIVE_GAUSS, 3 ADAPTIVE_LINEAR (4 ADAPTIVE_SINGLE is
+// the reference's default branch, DIFFERENCE), optional data weights; map
+// holds dim indices per trace (misfit.hip).
 int acoustic_misfit_residuals(int type, const float *mod, const float *rec, const float *wei, float *res,
                               const int *map, int dim, int ntr, int nt, int nshot, int scaling, double *misfit,
                               hipStream_t st);
+
+// The P (0), Ux (1) and Uz (2) records of one elastic shot, [nrec][nt] each
+// on the device; wei[c] may be null; map: the Ux receiver map (a trace
+// counts in the misfit when both its indices are >= 0).
+struct E2DRecs {
+    int nrec;
+    const int *map;
+    const float *mod[3], *obs[3], *wei[3];
+    float *res[3];
+    float *norms;  // scratch of CORRELATION (9 nrec floats), set internally
+};
+
+// Elastic misfit (added to *misfit in double) and residuals of every shot
+// (FwiElastic2D::computeMisfit / computeResiduals, fwi.cpp:2852-3773), type
+// 0..4 as rs_fwimisfit; the percentile clips are per shot.  Types 1-4
+// allocate scratch and synchronize the stream; DIFFERENCE only enqueues.
+int elastic_misfit_residuals(int type, const std::vector<E2DRecs> &shots, int nt, double *misfit,
+                             hipStream_t st);
 
 }  // namespace rsg

@@ -31,8 +31,10 @@ from . import _lib as R
 from .core import (GMAP, SMAP, Data2D, Data3D, ModelAcoustic2D, ModelAcoustic3D, ModelElastic2D, Pml, SMAX, _dev,
                    _stream)
 
-DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS, ADAPTIVE_LINEAR = 0, 1, 2, 3  # rs_fwimisfit (utils.h:26)
-_ACOUSTIC_MISFITS = (DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS, ADAPTIVE_LINEAR)
+DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS, ADAPTIVE_LINEAR, ADAPTIVE_SINGLE = 0, 1, 2, 3, 4  # rs_fwimisfit (utils.h:26)
+# the acoustic classes have no ADAPTIVE_SINGLE branch: it takes their default one, DIFFERENCE
+_ACOUSTIC_MISFITS = (DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS, ADAPTIVE_LINEAR, ADAPTIVE_SINGLE)
+_ELASTIC_MISFITS = (DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS, ADAPTIVE_LINEAR, ADAPTIVE_SINGLE)
 _MODES = {"revolve": R.FWI_REVOLVE, "store": R.FWI_STORE, "full": R.FWI_FULL}
 _ADJOINTS = {"reference": R.ADJOINT_REFERENCE, "exact": R.ADJOINT_EXACT}
 
@@ -101,7 +103,7 @@ class FwiAcoustic2D:
 
     def setMisfit_type(self, t):
         if int(t) not in _ACOUSTIC_MISFITS:
-            raise R.RsgError("the acoustic misfit is DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS or ADAPTIVE_LINEAR")
+            raise R.RsgError("the acoustic misfit type is one of rs_fwimisfit's 0..4")
         self.misfit_type = int(t)
 
     def setDataweight(self, d):
@@ -195,7 +197,7 @@ def misfit_residuals(misfit_type, dmod, dobs, gmap, weight=None, scaling=1):
     rec = torch.as_tensor(np.ascontiguousarray(dobs, np.float32)).to(dev)
     mp = torch.as_tensor(np.ascontiguousarray(gmap, np.int32)).to(dev)
     if int(misfit_type) not in _ACOUSTIC_MISFITS:
-        raise R.RsgError("the acoustic misfit is DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS or ADAPTIVE_LINEAR")
+        raise R.RsgError("the acoustic misfit type is one of rs_fwimisfit's 0..4")
     nt, ntr = mod.shape[-1], mod.shape[-2]
     nshot = mod.numel() // (nt * ntr) if nt * ntr else 1
     dim = mp.shape[-1]
@@ -206,6 +208,34 @@ def misfit_residuals(misfit_type, dmod, dobs, gmap, weight=None, scaling=1):
                                          None if wt is None else wt.data_ptr(), res.data_ptr(), mp.data_ptr(), dim,
                                          ntr, nt, nshot, int(scaling), C.byref(mis), _stream()),
             "rsg_misfit_residuals")
+    return mis.value, res.cpu().numpy()
+
+
+def elastic_misfit_residuals(misfit_type, dmod, dobs, gmap, weights=None):
+    """FwiElastic2D::computeMisfit and computeResiduals on the device for one
+    shot: modelled and observed records dmod / dobs [3, nrec, nt] (P, Ux,
+    Uz), the Ux receiver map gmap [nrec, 2] (traces with a negative entry
+    count in no misfit), weights None or three optional [nrec, nt].  Returns
+    (misfit, residuals [3, nrec, nt])."""
+    if int(misfit_type) not in _ELASTIC_MISFITS:
+        raise R.RsgError(f"elastic misfit type {misfit_type} is not one of rs_fwimisfit's 0..4")
+    dev = _dev()
+    mod = torch.as_tensor(np.ascontiguousarray(dmod, np.float32)).to(dev)
+    rec = torch.as_tensor(np.ascontiguousarray(dobs, np.float32)).to(dev)
+    if mod.dim() != 3 or mod.shape[0] != 3 or rec.shape != mod.shape:
+        raise R.RsgError("elastic_misfit_residuals: records must be [3, nrec, nt]")
+    mp = torch.as_tensor(np.ascontiguousarray(gmap, np.int32).reshape(-1, 2)).to(dev)
+    res = torch.zeros_like(mod)
+    wts = [None if weights is None or weights[c] is None else
+           torch.as_tensor(np.ascontiguousarray(weights[c], np.float32)).to(dev) for c in range(3)]
+    r = R.E2DRecords()
+    r.nrec, r.map = mod.shape[1], mp.data_ptr()
+    for c in range(3):
+        r.mod[c], r.obs[c], r.res[c] = mod[c].data_ptr(), rec[c].data_ptr(), res[c].data_ptr()
+        r.weight[c] = None if wts[c] is None else wts[c].data_ptr()
+    mis = C.c_double(0.0)
+    R.check(R.lib().rsg_misfit_residuals_e2d(int(misfit_type), C.byref(r), 1, mod.shape[2], C.byref(mis),
+                                             _stream()), "rsg_misfit_residuals_e2d")
     return mis.value, res.cpu().numpy()
 
 
@@ -392,21 +422,33 @@ class ElasticShot:
     wavelet (nsrc, nt) of field `srcfield` (P explosive, VX or VZ), receiver
     coordinates (nrec, 2) and observed records dobs[3, nrec, nt] (P, Ux, Uz),
     all in the coordinates of the full model; the shot runs on the window of
-    columns [x0, x0 + nxw) (the whole model when nxw is None)."""
+    columns [x0, x0 + nxw) (the whole model when nxw is None).  weights: None
+    or three data weights (P, Ux, Uz), each None or [nrec, nt]
+    (setDataweightp/x/z)."""
 
-    def __init__(self, scoords, wavelet, gcoords, dobs, x0=0, nxw=None, srcfield="P"):
+    def __init__(self, scoords, wavelet, gcoords, dobs, x0=0, nxw=None, srcfield="P", weights=None):
         self.scoords = np.ascontiguousarray(scoords, np.float32).reshape(-1, 2)
         self.wavelet = np.ascontiguousarray(wavelet, np.float32).reshape(self.scoords.shape[0], -1)
         self.gcoords = np.ascontiguousarray(gcoords, np.float32).reshape(-1, 2)
         self.dobs = np.ascontiguousarray(dobs, np.float32).reshape(3, self.gcoords.shape[0], -1)
         self.x0, self.nxw, self.srcfield = int(x0), nxw, srcfield
+        if weights is None:
+            weights = (None, None, None)
+        if len(weights) != 3:
+            raise R.RsgError("ElasticShot: weights must be three arrays (P, Ux, Uz) or None each")
+        self.weights = [None if w is None else np.ascontiguousarray(w, np.float32).reshape(self.dobs.shape[1:])
+                        for w in weights]
 
 
 class ElasticSurvey:
     """Shots of one elastic model plus the gradient settings (order, PML,
-    Revolve snapshots, FP16 checkpoints)."""
+    Revolve snapshots, FP16 checkpoints, misfit type 0..4 as rs_fwimisfit)."""
 
-    def __init__(self, shots, lpml, dx, dz, dt, order=8, fs=False, ncheck=50, fp16=False, ox=0.0, oz=0.0):
+    def __init__(self, shots, lpml, dx, dz, dt, order=8, fs=False, ncheck=50, fp16=False, ox=0.0, oz=0.0,
+                 misfit_type=DIFFERENCE):
+        if int(misfit_type) not in _ELASTIC_MISFITS:
+            raise R.RsgError(f"ElasticSurvey: misfit type {misfit_type} is not one of rs_fwimisfit's 0..4")
+        self.misfit_type = int(misfit_type)
         self.shots = list(shots)
         self.lpml, self.dx, self.dz, self.dt = int(lpml), float(dx), float(dz), float(dt)
         self.order, self.fs, self.ncheck, self.fp16 = int(order), bool(fs), int(ncheck), bool(fp16)
@@ -470,6 +512,7 @@ def elastic_batch_gradient(vp, vs, rho, survey: ElasticSurvey, scale=True, wavgr
             o["dmod"].append(dm)
             o["dres"].append(dr)
             s.dmod[c], s.dres[c] = dm.data_ptr(), dr.data_ptr()
+            s.weight[c] = None if sh.weights[c] is None else dev_t(sh.weights[c]).data_ptr()
         if wavgrad:
             wg = torch.zeros((src.ntrace, nt), dtype=torch.float32, device=dev)
             o["wavgrad"] = wg
@@ -486,6 +529,7 @@ def elastic_batch_gradient(vp, vs, rho, survey: ElasticSurvey, scale=True, wavgr
     for c in range(3):
         f.grad[c] = grads[c].data_ptr()
     f.misfit = C.pointer(mis)
+    f.misfit_type = survey.misfit_type
     R.check(R.lib().rsg_fwi_e2d_gradient_batch(C.byref(f), shots, ns, _stream()), "rsg_fwi_e2d_gradient_batch")
     return mis.value, grads[0], grads[1], grads[2], outs
 
@@ -493,7 +537,9 @@ def elastic_batch_gradient(vp, vs, rho, survey: ElasticSurvey, scale=True, wavgr
 class FwiElastic2D:
     """FwiElastic2D(model, source, dataP, dataUx, dataUz, order, snapinc)
     with the reference's setters and run_optimal() (Revolve checkpoints in
-    HBM); the misfit is DIFFERENCE over the P, Ux and Uz records."""
+    HBM); misfit types DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS,
+    ADAPTIVE_LINEAR and ADAPTIVE_SINGLE over the P, Ux and Uz records, with
+    optional data weights."""
 
     def __init__(self, model: ModelElastic2D, source: Data2D, dataP: Data2D, dataUx: Data2D, dataUz: Data2D,
                  order=4, snapinc=1):
@@ -506,6 +552,17 @@ class FwiElastic2D:
         self.datares = [None, None, None]
         self.vpgrad = self.vsgrad = self.rhograd = self.wavgrad = None
         self.misfit = 0.0
+        self.misfit_type = DIFFERENCE
+        self.dataweight = [None, None, None]
+
+    def setDataweightp(self, d):
+        self.dataweight[0] = d
+
+    def setDataweightx(self, d):
+        self.dataweight[1] = d
+
+    def setDataweightz(self, d):
+        self.dataweight[2] = d
 
     def setDatamodP(self, d):
         self.datamod[0] = d
@@ -547,8 +604,12 @@ class FwiElastic2D:
         self.fp16 = bool(v)
 
     def setMisfit_type(self, t):
-        if int(t) != DIFFERENCE:
-            raise R.RsgError("only the DIFFERENCE misfit is implemented on this path")
+        if int(t) not in _ELASTIC_MISFITS:
+            raise R.RsgError(f"FwiElastic2D: misfit type {t} is not one of rs_fwimisfit's 0..4")
+        self.misfit_type = int(t)
+
+    def getMisfit_type(self):
+        return self.misfit_type
 
     def getMisfit(self):
         return self.misfit
@@ -557,10 +618,11 @@ class FwiElastic2D:
         m = self.model
         if any(d is None for d in self.data):
             raise R.RsgError("FwiElastic2D: P, Ux and Uz data must be set")
+        wts = [None if w is None else w.data for w in self.dataweight]
         shot = ElasticShot(self.source.scoords, self.source.data, self.data[0].gcoords,
-                           np.stack([d.data for d in self.data]), 0, m.nx, _field_name(self.source.field))
+                           np.stack([d.data for d in self.data]), 0, m.nx, _field_name(self.source.field), wts)
         sv = ElasticSurvey([shot], m.lpml, m.dx, m.dz, self.source.dt, self.order, m.fs, self.ncheck, self.fp16,
-                           m.ox, m.oz)
+                           m.ox, m.oz, self.misfit_type)
         mis, g0, g1, g2, outs = elastic_batch_gradient(m.Vp, m.Vs, m.R, sv, scale=True,
                                                         wavgrad=self.wavgrad is not None)
         self.misfit64 = mis
@@ -583,7 +645,7 @@ def _field_name(f):
 
 
 class ElasticFwiMisfit(torch.autograd.Function):
-    """(Vp, Vs, rho) -> elastic DIFFERENCE misfit summed over the shots of an
+    """(Vp, Vs, rho) -> elastic misfit (the survey's misfit type) summed over the shots of an
     ElasticSurvey (each on its window, one ragged batch); the backward pass
     returns the reference's FwiElastic2D Vp / Vs / rho gradients."""
 
@@ -649,7 +711,7 @@ class FwiAcoustic3D:
 
     def setMisfit_type(self, t):
         if int(t) not in _ACOUSTIC_MISFITS:
-            raise R.RsgError("the acoustic misfit is DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS or ADAPTIVE_LINEAR")
+            raise R.RsgError("the acoustic misfit type is one of rs_fwimisfit's 0..4")
         self.misfit_type = int(t)
 
     def setDataweight(self, d):

@@ -252,6 +252,44 @@ def fwi_e2d_fixtures(R):
              wavgrad=r["wavgrad"], oracle_lam=o["lam"], oracle_mu=o["mu"], oracle_rho=o["rhok"])
 
 
+FWI_E2D_MISFIT_CASES = [  # order, fs, ncheck, source field, misfit type, data weights set (P, Ux, Uz)
+    (4, 0, 15, "P", 1, ""), (8, 1, 6, "VZ", 1, "pxz"), (8, 1, 6, "VZ", 2, "xz"), (4, 0, 15, "P", 3, ""),
+    (4, 0, 15, "VZ", 4, "xz"), (4, 0, 15, "P", 0, "pz"),
+]
+
+
+def fwi_e2d_misfit_fixtures(R):
+    """FwiElastic2D::run_optimal with the CORRELATION / ADAPTIVE_GAUSS /
+    ADAPTIVE_LINEAR / ADAPTIVE_SINGLE misfits and data weights
+    (setDataweightp/x/z); the oracle's run is checked bit for bit first."""
+    from oracle import fwi
+    nx, nz, lpml, dx, dt, nt = 60, 50, 10, 10.0, 1e-3, 250
+    vtrue = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1800.0, 3000.0, nanom=5, seed=3)
+    vstrue = (vtrue / 1.8).astype(np.float32)
+    rtrue = synth.gardner(vtrue)
+    vp0 = synth.gaussian_smooth(vtrue, 60.0, dx, dx)
+    vs0 = (vp0 / 1.8).astype(np.float32)
+    rho0 = synth.gardner(vp0)
+    sc = np.float32([[300.0, 20.0]])
+    wav = synth.ricker(nt, dt, 15.0)[None]
+    gc = synth.surface_line(nx, dx, 20.0)
+    rng = np.random.default_rng(4)
+    for k, (order, fs, ncheck, sf, mt, wset) in enumerate(FWI_E2D_MISFIT_CASES):
+        obs = np.stack([runs.ref_e2d_run(vtrue, vstrue, rtrue, lpml, dx, dx, dt, nt, order, fs, sf, sc, wav, f,
+                                         gc)["rec"] for f in ("P", "VX", "VZ")])
+        w = [(0.5 + rng.random((nx, nt))).astype(np.float32) if c in wset else None for c in "pxz"]
+        r = fwi.ref_fwi_e2d(vp0, vs0, rho0, lpml, dx, dx, dt, nt, order, fs, ncheck, sf, sc, wav, gc, obs, mt, w)
+        o = fwi.fwi_e2d(vp0, vs0, rho0, lpml, dx, dx, dt, nt, order, fs, ncheck, sf, sc, wav, gc, obs,
+                        misfit_type=mt, weights=w)
+        for key in ("vpgrad", "vsgrad", "rhograd", "dmod", "dres", "wavgrad"):
+            assert np.array_equal(r[key], o[key]), key
+        assert np.float32(r["misfit"]) == np.float32(o["misfit"])
+        wts = {f"wei_{c}": w[i] for i, c in enumerate("pxz") if w[i] is not None}
+        save(f"fwi_e2d_misfit_case{k}", vp=vp0, vs=vs0, rho=rho0, scoords=sc, wav=wav, gcoords=gc, dobs=obs,
+             grid=np.float32([nx, nz, lpml, dx, dt, nt]), misfit=np.float32(r["misfit"]), vpgrad=r["vpgrad"],
+             vsgrad=r["vsgrad"], rhograd=r["rhograd"], dmod=r["dmod"], dres=r["dres"], wavgrad=r["wavgrad"], **wts)
+
+
 FWI_A3D_CASES = [  # order, fs, ncheck (FwiAcoustic3D::run_optimal)
     (4, 0, 10), (8, 1, 5),
 ]
@@ -316,7 +354,8 @@ def revolve_fixture(R):
 GROUPS = dict(der=der_fixture, pml=pml_fixture, stagger=stagger_fixture, makemap=makemap_fixture,
               runs=run_fixtures, revolve=revolve_fixture, fwi=fwi_fixtures, fwi_full=fwi_full_fixtures,
               fwi_e2d=fwi_e2d_fixtures, fwi_a3d=fwi_a3d_fixtures,
-              interp=interp_fixture, fwi_misfit=fwi_misfit_fixtures)
+              interp=interp_fixture, fwi_misfit=fwi_misfit_fixtures,
+              fwi_e2d_misfit=fwi_e2d_misfit_fixtures)
 
 if __name__ == "__main__":
     # python -m tests.golden.make_golden [group ...]   (default: every group)

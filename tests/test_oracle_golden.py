@@ -203,3 +203,48 @@ def test_fwi_a2d_misfit_types(golden, case):
     for k in ("vpgrad", "rhograd", "dmod", "dres", "wavgrad"):
         assert np.array_equal(r[k], g[k]), k
     assert np.float32(r["misfit"]) == g["misfit"]
+
+
+def _e2d_misfit_inputs(g, case):
+    from tests.golden.make_golden import FWI_E2D_MISFIT_CASES
+    order, fs, ncheck, sf, mt, wset = FWI_E2D_MISFIT_CASES[case]
+    nx, nz, lpml, dx, dt, nt = g["grid"]
+    wts = [g[f"wei_{c}"] if f"wei_{c}" in g.files else None for c in "pxz"]
+    return (order, fs, ncheck, sf, mt), (int(nx), int(nz), int(lpml), dx, dt, int(nt)), wts
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_e2d_misfit_residuals_vs_reference(golden, case):
+    """FwiElastic2D::computeMisfit / computeResiduals restated (orc_misfit_e,
+    orc_residual_e) on the reference's own modelled P / Ux / Uz records: the
+    reference's residuals bit for bit (CORRELATION with the combined Ux/Uz
+    clip and the separate P clip, ADAPTIVE_GAUSS / LINEAR per component,
+    ADAPTIVE_SINGLE with one joint shaper, data weights) and its misfit."""
+    g = golden(f"fwi_e2d_misfit_case{case}")
+    (order, fs, ncheck, sf, mt), (nx, nz, lpml, dx, dt, nt), wts = _e2d_misfit_inputs(g, case)
+    gx, _ = runs.makemap2d(g["gcoords"], nx, nz, dx, dx, "VX")
+    valid = ((gx[:, 0] >= 0) & (gx[:, 1] >= 0)).astype(np.int32)
+    nrec = g["dmod"].shape[1]
+    dmod, dobs = np.ascontiguousarray(g["dmod"]), np.ascontiguousarray(g["dobs"])
+    md = [dmod[0], dobs[0], dmod[1], dobs[1], dmod[2], dobs[2]]
+    md = [np.ascontiguousarray(a).reshape(-1) for a in md]
+    wp = [None if w is None else np.ascontiguousarray(w).ctypes.data for w in wts]
+    res = np.zeros_like(dmod)
+    oracle.fn("residual_e")(mt, *md, *wp, res[0].reshape(-1), res[1].reshape(-1), res[2].reshape(-1), nrec, nt)
+    assert np.array_equal(res, g["dres"])
+    mis = oracle.fn("misfit_e")(mt, *md, *wp, valid, nrec, nt)
+    assert np.float32(mis) == g["misfit"]
+
+
+def test_fwi_e2d_adaptive_single_run(golden):
+    """The whole elastic run_optimal with ADAPTIVE_SINGLE and Ux / Uz data
+    weights (the oracle's fwi_e2d): gradients, records, residuals and wavelet
+    gradient equal the reference's bit for bit."""
+    from oracle import fwi
+    g = golden("fwi_e2d_misfit_case4")
+    (order, fs, ncheck, sf, mt), (nx, nz, lpml, dx, dt, nt), wts = _e2d_misfit_inputs(g, 4)
+    r = fwi.fwi_e2d(g["vp"], g["vs"], g["rho"], lpml, dx, dx, dt, nt, order, fs, ncheck, sf, g["scoords"],
+                    g["wav"], g["gcoords"], g["dobs"], misfit_type=mt, weights=wts)
+    for k in ("vpgrad", "vsgrad", "rhograd", "dmod", "dres", "wavgrad"):
+        assert np.array_equal(r[k], g[k]), k
+    assert np.float32(r["misfit"]) == g["misfit"]
