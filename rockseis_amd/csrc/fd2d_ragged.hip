@@ -69,7 +69,7 @@ int ragged2d_step(const Ragged2D *p, int which, hipStream_t stream) {
     if (!p || !p->d_desc) return fail(RSG_EINVAL, "ragged2d: no plan");
     if (which < 0 || which > 3 || (p->kind == RSG_KIND_A2D && which > 1))
         return fail(RSG_EINVAL, "ragged2d: which must be 0..1 (A2D) or 0..3 (E2D)");
-    const int rz = rz2d();
+    const int rz = rz2d_ragged();
     const int r = rz == 1 ? 0 : (rz == 2 ? 1 : (rz == 4 ? 2 : 3));
     const int *t0 = p->d_tile0 + r * (p->nshot + 1);
     const unsigned nb = 8u * (unsigned)((p->total[r] + 7) / 8);

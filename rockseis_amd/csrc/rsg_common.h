@@ -133,6 +133,7 @@ enum { T_A = 0, T_B = 1, T_C = 2, T_STAG = 3, T_RBB = 6 };
 // Launch-shape knobs (set through rsg_set_tuning; picked by the autotuner).
 struct Tuning {
     int rz2d = 2;   // rows per thread of the 2D tiles: 1, 2, 4 or 8
+    int rz2d_ragged = 2;  // the same for the ragged batches (elastic FWI windows)
     int zc3d = 0;   // z planes per 3D block (0: automatic)
     int fuse_io = 1;  // modelling runs: source injection / velocity gathers in the stress kernel
 };

@@ -763,10 +763,14 @@ static K2D make_k2d(const rsg_grid *g, const rsg_state *s, int tz) {
 }
 
 // Rows per thread from the tuning table (1, 2, 4 or 8).
-static int rz2d() {
-    const int v = tuning().rz2d;
-    return v <= 1 ? 1 : (v <= 2 ? 2 : (v <= 4 ? 4 : 8));
-}
+static int rz_of(int v) { return v <= 1 ? 1 : (v <= 2 ? 2 : (v <= 4 ? 4 : 8)); }
+static int rz2d() { return rz_of(tuning().rz2d); }
+// Ragged batches have their own knob: measured on MI355X over 16 elastic FWI
+// windows of 600-2000 x 600 cells, 2 rows per thread is fastest for all four
+// half steps (the adjoint-state velocity step, which forms the moduli-weighted
+// stresses in its tile loader, 240 us at 2 vs 294 us at 1), while the
+// single-shot modelling kernel prefers 1.
+static int rz2d_ragged() { return rz_of(tuning().rz2d_ragged); }
 
 static unsigned nblocks(const K2D &a) { return 8u * (unsigned)((a.ntiles + 7) / 8); }
 

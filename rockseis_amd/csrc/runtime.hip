@@ -135,6 +135,10 @@ int rsg_set_tuning(const char *key, int value) {
         if (value != 1 && value != 2 && value != 4 && value != 8)
             return fail(RSG_EINVAL, "rz2d must be 1, 2, 4 or 8");
         tuning().rz2d = value;
+    } else if (k == "rz2d_ragged") {
+        if (value != 1 && value != 2 && value != 4 && value != 8)
+            return fail(RSG_EINVAL, "rz2d_ragged must be 1, 2, 4 or 8");
+        tuning().rz2d_ragged = value;
     } else if (k == "zc3d") {
         if (value < 0 || value > 4096) return fail(RSG_EINVAL, "zc3d must be 0..4096");
         tuning().zc3d = value;
@@ -152,6 +156,8 @@ int rsg_get_tuning(const char *key, int *value) {
     std::string k(key);
     if (k == "rz2d")
         *value = tuning().rz2d;
+    else if (k == "rz2d_ragged")
+        *value = tuning().rz2d_ragged;
     else if (k == "zc3d")
         *value = tuning().zc3d;
     else if (k == "fuse_io")

@@ -42,10 +42,10 @@ def _setup(core, kind, seed):
 @pytest.mark.parametrize("kind,order", [("e2d", 8), ("e2d", 3), ("a2d", 4)])
 def test_ragged_equals_single_shot(kind, order):
     from rockseis_amd import _lib as R, core
-    old = R.get_tuning("rz2d")
+    old = R.get_tuning("rz2d_ragged")
     try:
         for rz in (1, 2, 4, 8):
-            R.set_tuning("rz2d", rz)
+            R.set_tuning("rz2d_ragged", rz)
             wa, ma, fields = _setup(core, kind, 11)
             wb, mb, _ = _setup(core, kind, 11)
             rb = core.RaggedBatch2D(wb, mb, order)
@@ -69,4 +69,4 @@ def test_ragged_equals_single_shot(kind, order):
                 for n in x.psi:
                     assert np.array_equal(x.psi[n].cpu().numpy(), y.psi[n].cpu().numpy()), (rz, k, n)
     finally:
-        R.set_tuning("rz2d", old)
+        R.set_tuning("rz2d_ragged", old)
