@@ -94,7 +94,7 @@ def roofline_only(reps):
     mod.launch(0, 20)  # a non-zero state; runs this short keep the separate I/O kernels
     torch.cuda.synchronize()
     t_vel, t_str = kernel_time_ms(mod, 0, reps), kernel_time_ms(mod, 1, reps)
-    return {"rows_per_thread": tuned["rz2d"], "kernel_reps": reps,
+    return {"rows_per_thread": tuned["rz2d"], "pml_tile_cost": tuned.get("pml_cost"), "kernel_reps": reps,
             "e2d_velocity_kernel_us": round(t_vel * 1e3, 2), "e2d_stress_kernel_us": round(t_str * 1e3, 2)}
 
 
@@ -217,7 +217,7 @@ def main():
     # launch shape picked by the autotuner (timed on a scratch state of this grid)
     from rockseis_amd import autotune
     tuned = autotune.tune_waves(mod.waves, cfg["order"], nshot=1)
-    _progress(f"modelling workload built, tile rows per thread {tuned['rz2d']}")
+    _progress(f"modelling workload built, tile rows per thread {tuned['rz2d']}, PML tile cost {tuned.get('pml_cost')}")
     if W:
         mod.launch(0, W)
     torch.cuda.synchronize()
@@ -256,6 +256,7 @@ def main():
                    "dx_m": cfg["dx"], "order": cfg["order"], "dt_s": cfg["dt"], "nt": cfg["nt"],
                    "source": f"explosive {cfg['f0']:g} Hz Ricker", "receivers": "2001 x (vx, vz)",
                    "graph_chunk": args.graph_chunk, "rows_per_thread": tuned["rz2d"],
+                   "pml_tile_cost": tuned.get("pml_cost"),
                    "parallelism": "single GPU"},
         "hbm_frac_whole_step": round(step_gbs / HBM_PEAK_GBS, 4),
         "roofline": roof,

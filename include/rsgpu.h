@@ -107,10 +107,13 @@ int rsg_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 int rsg_zero(void *ptr, size_t bytes, void *stream);
 
 /* Launch-shape knobs: "rz2d" (rows per thread of the 2D tiles: 1, 2, 4, 8),
- * "rz2d_ragged" (the same for ragged batches, default 2), "zc3d" (z planes
- * per 3D block, 0 = automatic), "fuse_io" (modelling runs: sources and
- * velocity gathers in the stress kernel, 1 = on).  Results do not depend on
- * them; rockseis_amd.autotune picks rz2d / zc3d per workload. */
+ * "rz2d_ragged" (the same for ragged batches, default 2), "pml_cost" (2D:
+ * weight in percent of a PML / edge tile against an interior one when the
+ * tiles are split into eight XCD ranges of equal cost, default 200), "zc3d"
+ * (z planes per 3D block, 0 = automatic), "fuse_io" (modelling runs: sources
+ * and velocity gathers in the stress kernel, 1 = on).  Results do not depend
+ * on them; rockseis_amd.autotune picks rz2d, pml_cost and zc3d per
+ * workload. */
 int rsg_set_tuning(const char *key, int value);
 int rsg_get_tuning(const char *key, int *value);
 

@@ -1,12 +1,15 @@
 """Launch-shape autotuner for the FD step kernels.
 
 The fused 2D kernels give each thread ``rz2d`` rows of a 64-wide tile
-(1, 2, 4 or 8); the 3D kernels march ``zc3d`` z planes per block (0 = chosen
+(1, 2, 4 or 8) and split the tiles into eight contiguous XCD ranges of equal
+estimated cost, a PML / edge tile counting ``pml_cost`` percent of an
+interior one; the 3D kernels march ``zc3d`` z planes per block (0 = chosen
 from the grid so that a launch has >= 2048 blocks).  Results do not depend on
-either knob (every cell is computed by the same arithmetic), only the time
+any knob (every cell is computed by the same arithmetic), only the time
 does, so the tuner times a velocity + stress half-step pair for every
 candidate on a zeroed scratch state of the caller's shape, keeps the fastest
-and sets it with ``rsg_set_tuning``.  Choices are cached per
+and sets it with ``rsg_set_tuning`` (2D: rz2d first, then pml_cost at that
+rz2d).  Choices are cached per
 (kind, padded grid, order, shots) in-process and, if ``RSG_TUNE_CACHE`` names
 a JSON file, across processes.
 """
@@ -20,7 +23,7 @@ import torch
 
 from . import _lib as R
 
-CANDIDATES = {"rz2d": (1, 2, 4, 8), "zc3d": (0, 2, 4, 8, 16, 32)}
+CANDIDATES = {"rz2d": (1, 2, 4, 8), "zc3d": (0, 2, 4, 8, 16, 32), "pml_cost": (100, 150, 200, 250, 300)}
 _cache: dict = {}
 
 
@@ -130,23 +133,33 @@ def tune(kind, g, tab_ptr, nshot=1, reps=20, force=False):
         _load_file()
     key = _key(kind, g, nshot)
     if key in _cache and not force:
-        R.set_tuning(knob, _cache[key][knob])
+        for k in (knob, "pml_cost"):
+            if k in _cache[key]:
+                R.set_tuning(k, _cache[key][k])
         return _cache[key]
     sc = _Scratch(kind, g, tab_ptr, nshot)
-    times = {}
-    old = R.get_tuning(knob)
-    try:
-        for v in CANDIDATES[knob]:
-            R.set_tuning(knob, v)
-            times[v] = time_pair_ms(kind, g, sc, nshot, reps)
-    except Exception:
-        R.set_tuning(knob, old)
-        raise
-    best = min(times, key=times.get)
-    R.set_tuning(knob, best)
-    _cache[key] = {knob: best, "ms": {str(k): round(t, 5) for k, t in times.items()}}
+
+    def sweep(name):
+        times = {}
+        old = R.get_tuning(name)
+        try:
+            for v in CANDIDATES[name]:
+                R.set_tuning(name, v)
+                times[v] = time_pair_ms(kind, g, sc, nshot, reps)
+        except Exception:
+            R.set_tuning(name, old)
+            raise
+        best = min(times, key=times.get)
+        R.set_tuning(name, best)
+        return best, {str(k): round(t, 5) for k, t in times.items()}
+
+    best, ms = sweep(knob)
+    res = {knob: best, "ms": ms}
+    if kind != R.KIND_A3D:
+        res["pml_cost"], res["ms_pml_cost"] = sweep("pml_cost")
+    _cache[key] = res
     _save_file()
-    return _cache[key]
+    return res
 
 
 def tune_waves(waves, order, nshot=1, reps=20, force=False):

@@ -82,28 +82,28 @@ int ragged2d_step(const Ragged2D *p, int which, hipStream_t stream) {
     if (p->kind == RSG_KIND_A2D) {
         if (which == 0) {
 #define L_(O, RZ) RG((a2d_velocity_ragged_kernel<O, RZ>))
-            RSG_DISPATCH(L_);
+            RSG_DISPATCH_RZ(L_, rz);
 #undef L_
         } else {
 #define L_(O, RZ) RG((a2d_stress_ragged_kernel<O, RZ>))
-            RSG_DISPATCH(L_);
+            RSG_DISPATCH_RZ(L_, rz);
 #undef L_
         }
     } else if (which == 0) {
 #define L_(O, RZ) RG((e2d_velocity_ragged_kernel<O, RZ, false>))
-        RSG_DISPATCH(L_);
+        RSG_DISPATCH_RZ(L_, rz);
 #undef L_
     } else if (which == 1) {
 #define L_(O, RZ) RG((e2d_stress_ragged_kernel<O, RZ, false>))
-        RSG_DISPATCH(L_);
+        RSG_DISPATCH_RZ(L_, rz);
 #undef L_
     } else if (which == 2) {
 #define L_(O, RZ) RG((e2d_velocity_ragged_kernel<O, RZ, true>))
-        RSG_DISPATCH(L_);
+        RSG_DISPATCH_RZ(L_, rz);
 #undef L_
     } else {
 #define L_(O, RZ) RG((e2d_stress_ragged_kernel<O, RZ, true>))
-        RSG_DISPATCH(L_);
+        RSG_DISPATCH_RZ(L_, rz);
 #undef L_
     }
 #undef RG

@@ -136,6 +136,7 @@ struct Tuning {
     int rz2d_ragged = 2;  // the same for the ragged batches (elastic FWI windows)
     int zc3d = 0;   // z planes per 3D block (0: automatic)
     int fuse_io = 1;  // modelling runs: source injection / velocity gathers in the stress kernel
+    int pml_cost = 200;  // weight (percent of an interior tile) of a PML / edge tile in the XCD split
 };
 Tuning &tuning();
 

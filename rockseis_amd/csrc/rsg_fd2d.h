@@ -3,6 +3,9 @@
 // fd2d_ragged.hip): kernel arguments, LDS tile staging, per-cell update
 // bodies, the kernels, and the host helpers that build their arguments.
 #pragma once
+#include <algorithm>
+#include <vector>
+
 #include "rsg_common.h"
 
 // 2D staggered-grid velocity-stress kernels for gfx950: acoustic
@@ -76,6 +79,7 @@ struct FusedIO {
 struct K2D {
     int nx, nz, lpml, fs;
     int ntx, ntiles;  // tiles along x, total tiles
+    int xs[9], xper;  // tiles of XCD k: [xs[k], xs[k+1]); the most any XCD holds
     long sf, sxs, szs;  // per-shot strides of fields, x-side and z-side strips (blockIdx.y = shot)
     float dt, invdx, invdz;
     const float *tab;
@@ -135,12 +139,14 @@ __device__ __forceinline__ void fused_record(const FusedIO &io, int t, int x0, i
     }
 }
 
-// Tile handled by this block: XCD-contiguous numbering of a 1D grid.
+// Tile handled by this block: blocks are dealt round-robin to the 8 XCDs,
+// so XCD k runs blocks k, k+8, ...; it gets the contiguous tile range
+// [xs[k], xs[k+1]) (tiles that share halos stay in one L2), ranges of equal
+// estimated cost (make_k2d), and blocks past its range exit.
 __device__ __forceinline__ bool tile_of(const K2D &a, int TZ, int &x0, int &z0) {
-    const int nb = gridDim.x;
-    const int per = (nb + 7) / 8;
-    const int t = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (t >= a.ntiles) return false;
+    const int k = blockIdx.x % 8;
+    const int t = a.xs[k] + (int)(blockIdx.x / 8);
+    if (t >= a.xs[k + 1]) return false;
     x0 = (t % a.ntx) * BX;
     z0 = (t / a.ntx) * TZ;
     return true;
@@ -149,7 +155,7 @@ __device__ __forceinline__ bool tile_of(const K2D &a, int TZ, int &x0, int &z0) 
 // A tile is "interior" when it holds no PML cell and no free-surface row and
 // its whole halo lies inside the grid: such tiles (most of the grid) take a
 // straight-line path with no bounds, edge-stencil or strip tests.
-__device__ __forceinline__ bool interior_tile(const K2D &a, int x0, int z0, int TZ, int H) {
+__host__ __device__ inline bool interior_tile(const K2D &a, int x0, int z0, int TZ, int H) {
     const int lo = a.lpml > H ? a.lpml : H;
     return x0 >= lo && x0 + BX + H <= a.nx - a.lpml && z0 >= lo && z0 + TZ + H <= a.nz - a.lpml &&
            !(a.fs && z0 <= a.lpml);
@@ -759,6 +765,28 @@ static K2D make_k2d(const rsg_grid *g, const rsg_state *s, int tz) {
     a.cx = scaled_coefs(g->dx);
     a.cz = scaled_coefs(g->dz);
     a.io = FusedIO{};
+    // XCD ranges of equal cost: a tile off the straight-line path (PML, free
+    // surface, grid edge) counts pml_cost percent of an interior tile.  The
+    // PML rows of the top and bottom all fall to the first and last XCD, so
+    // equal tile counts leave those two XCDs the longest.
+    {
+        const int pc = tuning().pml_cost;
+        std::vector<long long> pre(a.ntiles + 1, 0);
+        for (int t = 0; t < a.ntiles; t++) {
+            const int x0 = (t % a.ntx) * BX, z0 = (t / a.ntx) * tz;
+            pre[t + 1] = pre[t] + (interior_tile(a, x0, z0, tz, g->order) ? 100 : pc);
+        }
+        a.xs[0] = 0;
+        a.xper = 0;
+        int t = 0;
+        for (int k = 1; k <= 8; k++) {
+            const long long target = pre[a.ntiles] * k / 8;
+            while (t < a.ntiles && pre[t + 1] <= target) t++;
+            if (k == 8) t = a.ntiles;
+            a.xs[k] = t;
+            a.xper = std::max(a.xper, a.xs[k] - a.xs[k - 1]);
+        }
+    }
     return a;
 }
 
@@ -772,7 +800,7 @@ static int rz2d() { return rz_of(tuning().rz2d); }
 // single-shot modelling kernel prefers 1.
 static int rz2d_ragged() { return rz_of(tuning().rz2d_ragged); }
 
-static unsigned nblocks(const K2D &a) { return 8u * (unsigned)((a.ntiles + 7) / 8); }
+static unsigned nblocks(const K2D &a) { return 8u * (unsigned)std::max(a.xper, 1); }
 
 #define RSG_ORDER_CASES(LAUNCH, RZ) \
     switch (g->order) {              \
@@ -786,9 +814,11 @@ static unsigned nblocks(const K2D &a) { return 8u * (unsigned)((a.ntiles + 7) / 
     default: LAUNCH(8, RZ); break;   \
     }
 
-#define RSG_DISPATCH(LAUNCH)                      \
+// Launch with the RZ template of the value rzv (1, 2, 4 or 8): the launcher
+// must size its grid and tile table for the same value.
+#define RSG_DISPATCH_RZ(LAUNCH, rzv)              \
     do {                                          \
-        const int rz_ = rz2d();                   \
+        const int rz_ = (rzv);                    \
         if (rz_ == 1) {                           \
             RSG_ORDER_CASES(LAUNCH, 1)            \
         } else if (rz_ == 2) {                    \
@@ -799,5 +829,6 @@ static unsigned nblocks(const K2D &a) { return 8u * (unsigned)((a.ntiles + 7) / 
             RSG_ORDER_CASES(LAUNCH, 8)            \
         }                                         \
     } while (0)
+#define RSG_DISPATCH(LAUNCH) RSG_DISPATCH_RZ(LAUNCH, rz2d())
 
 }  // namespace rsg

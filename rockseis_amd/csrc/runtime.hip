@@ -145,6 +145,9 @@ int rsg_set_tuning(const char *key, int value) {
     } else if (k == "fuse_io") {
         if (value != 0 && value != 1) return fail(RSG_EINVAL, "fuse_io must be 0 or 1");
         tuning().fuse_io = value;
+    } else if (k == "pml_cost") {
+        if (value < 1 || value > 10000) return fail(RSG_EINVAL, "pml_cost must be 1..10000");
+        tuning().pml_cost = value;
     } else {
         return fail(RSG_EINVAL, "unknown tuning key " + k);
     }
@@ -162,6 +165,8 @@ int rsg_get_tuning(const char *key, int *value) {
         *value = tuning().zc3d;
     else if (k == "fuse_io")
         *value = tuning().fuse_io;
+    else if (k == "pml_cost")
+        *value = tuning().pml_cost;
     else
         return fail(RSG_EINVAL, "unknown tuning key " + k);
     return RSG_OK;

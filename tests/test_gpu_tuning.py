@@ -63,6 +63,29 @@ def test_rz2d_values_give_identical_fields():
         R.set_tuning("rz2d", old)
 
 
+def test_pml_cost_values_give_identical_fields():
+    """Every split of the 2D tiles into XCD ranges (pml_cost from 1 to 10000
+    percent, the extremes leaving ranges of a few tiles or none) covers each
+    tile exactly once: records and fields bit-identical, acoustic and elastic
+    (free surface), at two tile heights."""
+    from rockseis_amd import _lib as R, core
+    old = R.get_tuning("pml_cost"), R.get_tuning("rz2d")
+    try:
+        for rz in (1, 2):
+            R.set_tuning("rz2d", rz)
+            out = {}
+            for v in (100, 1, 200, 777, 10000):
+                R.set_tuning("pml_cost", v)
+                out[v] = list(_a2d(core, 8)) + _e2d_run(core, 0, fs=True)
+            for v in (1, 200, 777, 10000):
+                for a, b in zip(out[v], out[100]):
+                    assert np.array_equal(a, b), (rz, v)
+            assert np.abs(out[100][0]).max() > 0
+    finally:
+        R.set_tuning("pml_cost", old[0])
+        R.set_tuning("rz2d", old[1])
+
+
 def test_zc3d_values_give_identical_fields():
     from rockseis_amd import _lib as R, core
     old = R.get_tuning("zc3d")
@@ -96,17 +119,22 @@ def test_autotune_picks_and_sets_a_candidate(kind):
         w = core.WavesAcoustic2D(m, 10, 1e-3)
         knob = "rz2d"
         nshot = 4 if kind == "a2d_batch" else 1
-    old = R.get_tuning(knob)
+    old = R.get_tuning(knob), R.get_tuning("pml_cost")
     try:
         res = autotune.tune_waves(w, 8 if kind != "a3d" else 4, nshot=nshot, reps=5, force=True)
         assert res[knob] in autotune.CANDIDATES[knob]
         assert R.get_tuning(knob) == res[knob]
         assert len(res["ms"]) == len(autotune.CANDIDATES[knob])
         assert all(t > 0 for t in res["ms"].values())
+        if kind != "a3d":  # 2D: then the XCD split's PML weight at that tile height
+            assert res["pml_cost"] in autotune.CANDIDATES["pml_cost"]
+            assert R.get_tuning("pml_cost") == res["pml_cost"]
+            assert len(res["ms_pml_cost"]) == len(autotune.CANDIDATES["pml_cost"])
         # cached: the second call returns the same choice without timing
         assert autotune.tune_waves(w, 8 if kind != "a3d" else 4, nshot=nshot) is res
     finally:
-        R.set_tuning(knob, old)
+        R.set_tuning(knob, old[0])
+        R.set_tuning("pml_cost", old[1])
 
 
 def _e2d_run(core, graph, srcfield="P", order=8, fs=False):
