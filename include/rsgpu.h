@@ -342,6 +342,26 @@ typedef struct {
 int rsg_fwi_e2d_gradient_batch(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *shots, int nshot,
                                void *stream);
 
+/* ---- the same gradient as a persistent plan (no reference counterpart: the
+ * reference runs one shot per MPI worker, inversion.cpp:332-600).  create
+ * validates the batch, allocates every state and checkpoint buffer and
+ * captures the Revolve sweep as two hipGraphs on a stream of its own: up to
+ * the forward step of the first turn, and the reverse sweep with the stacking.
+ * run replays them, with the misfit and residuals computed between the two
+ * (eagerly: the percentile clips read back to the host), and returns the
+ * misfit; it waits for work already on `stream` before it starts and returns
+ * with the plan's work finished.  The graphs hold the caller's pointers
+ * (models, records, residual and gradient arrays): between runs the caller
+ * may change their contents (a new model restaggered in place, new
+ * observed data), not the arrays themselves.  Gradients are added into
+ * f->grad on every run, so zero them in between for a fresh gradient.
+ * f->misfit is ignored (run returns it). */
+typedef struct rsg_fwi_e2d_plan rsg_fwi_e2d_plan;
+int rsg_fwi_e2d_plan_create(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *shots, int nshot,
+                            rsg_fwi_e2d_plan **plan);
+int rsg_fwi_e2d_plan_run(rsg_fwi_e2d_plan *plan, double *misfit, void *stream);
+int rsg_fwi_e2d_plan_destroy(rsg_fwi_e2d_plan *plan);
+
 /* ---- elastic misfit and residuals alone (FwiElastic2D::computeMisfit and
  * computeResiduals, fwi.cpp:2852-3773), shot by shot as the reference runs
  * them: type as rsg_fwi_e2d.misfit_type; per shot the P, Ux, Uz records

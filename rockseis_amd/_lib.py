@@ -142,6 +142,9 @@ def _declare(L):
         "rsg_ragged2d_step": ([vp, ip, vp], ip),
         "rsg_ragged2d_destroy": ([vp], ip),
         "rsg_fwi_e2d_gradient_batch": ([C.POINTER(FwiE2D), C.POINTER(FwiE2DShot), ip, vp], ip),
+        "rsg_fwi_e2d_plan_create": ([C.POINTER(FwiE2D), C.POINTER(FwiE2DShot), ip, C.POINTER(C.c_void_p)], ip),
+        "rsg_fwi_e2d_plan_run": ([vp, C.POINTER(C.c_double), vp], ip),
+        "rsg_fwi_e2d_plan_destroy": ([vp], ip),
         "rsg_fwi_a3d_gradient": ([GP, C.POINTER(FwiA3D), vp], ip),
         "rsg_interp": ([ip, vp, ip, ip, fp, fp, vp, ip, fp, fp, vp], ip),
         "rsg_misfit_residuals": ([ip, vp, vp, vp, vp, vp, ip, ip, ip, ip, ip, C.POINTER(C.c_double), vp], ip),
@@ -171,6 +174,7 @@ EXPORTS = (
     "rsg_a2d_forward_stress_batch", "rsg_e2d_velocity_batch", "rsg_e2d_stress_batch",
     "rsg_ragged2d_create", "rsg_ragged2d_step", "rsg_ragged2d_destroy", "rsg_fwi_e2d_gradient_batch",
     "rsg_fwi_a3d_gradient", "rsg_interp", "rsg_misfit_residuals", "rsg_misfit_residuals_e2d",
+    "rsg_fwi_e2d_plan_create", "rsg_fwi_e2d_plan_run", "rsg_fwi_e2d_plan_destroy",
 )
 
 

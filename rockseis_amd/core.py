@@ -113,9 +113,16 @@ class _Model:
         return self.fs
 
     def _upload(self, names):
+        # in place when the device array exists with the same shape, so that
+        # captured launches (FWI plans) keep pointing at the model
         dev = _dev()
         for n in names:
-            setattr(self, "_d" + n, torch.from_numpy(np.ascontiguousarray(getattr(self, n))).to(dev))
+            h = torch.from_numpy(np.ascontiguousarray(getattr(self, n)))
+            d = getattr(self, "_d" + n, None)
+            if d is not None and d.shape == h.shape and d.dtype == h.dtype:
+                d.copy_(h)
+            else:
+                setattr(self, "_d" + n, h.to(dev))
 
 
 class ModelAcoustic2D(_Model):

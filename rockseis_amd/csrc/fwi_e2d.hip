@@ -17,6 +17,7 @@
 // windows are stacked into the caller's images at their x offsets, in
 // double, shot by shot.
 #include <algorithm>
+#include <memory>
 #include <vector>
 
 #include "rsg_common.h"
@@ -208,205 +209,222 @@ inline bool e2d_xstrip(int i) { return (i / 4) % 2 == 0; }
 
 }  // namespace
 
-static int fwi_e2d_impl(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *sh, int ns, hipStream_t st) {
-    const rsg_grid &g0 = sh[0].g;
-    const int nt = sh[0].src.nt, lpml = g0.lpml, order = g0.order;
-    const float dt = g0.dt;
-    // ---- validation
+// One batched elastic FWI gradient: the validated inputs, every device buffer
+// and the two halves of the Revolve sweep.  Phase A runs from the zeroed
+// state through the forward step of the first turn; the misfit and residuals
+// follow (eager: the percentile clips need host round trips); phase B runs
+// the reverse sweep with its recomputations and stacks the windows' kernels.
+// Both phases only enqueue kernels and device-to-device copies on the
+// stream, so a plan captures each as a hipGraph once and replays it.
+struct E2DFwi {
+    const rsg_fwi_e2d *f = nullptr;
+    const rsg_fwi_e2d_shot *sh = nullptr;
+    int ns = 0, nt = 0, lpml = 0, order = 0;
+    float dt = 0.f;
     size_t maxsrc = 0, maxrec = 0;
     int maxnx = 0, maxnz = 0;
-    for (int k = 0; k < ns; k++) {
-        const rsg_fwi_e2d_shot &s = sh[k];
-        const int nx = s.g.nx - 2 * lpml, nz = s.g.nz - 2 * lpml;
-        if (s.g.lpml != lpml || s.g.order != order || s.g.dt != dt || s.g.dx != g0.dx || s.g.dz != g0.dz ||
-            s.g.ny != 1)
-            return fail(RSG_EINVAL, "fwi_e2d: shots must share lpml, order, dx, dz, dt (2D grids)");
-        if (!s.L || !s.L2M || !s.M || !s.Rx || !s.Rz || !s.Vp || !s.Vs || !s.Rho)
-            return fail(RSG_EINVAL, "fwi_e2d: null model array");
-        if (s.src.nt != nt || s.src.ntrace < 0 || (s.src.ntrace > 0 && (!s.src.map || !s.src.shift || !s.src.data)))
-            return fail(RSG_EINVAL, "fwi_e2d: every source needs nt samples and its map/shift/data");
-        if (s.src.field != RSG_PRESSURE && s.src.field != RSG_VX && s.src.field != RSG_VZ)
-            return fail(RSG_EINVAL, "fwi_e2d: source field must be P, VX or VZ");
-        const int want[3] = {RSG_PRESSURE, RSG_VX, RSG_VZ};
-        for (int c = 0; c < 3; c++) {
-            const rsg_traces &o = s.obs[c];
-            if (o.field != want[c] || o.nt != nt || o.ntrace != s.obs[0].ntrace || !o.map || !o.shift || !o.data ||
-                !s.dmod[c] || !s.dres[c])
-                return fail(RSG_EINVAL, "fwi_e2d: obs[0..2] must be P, VX, VZ records of one receiver set");
-        }
-        if (nx < 3 || nz < 3 || s.ox < 0 || s.ox + nx > f->nx || nz > f->nz)
-            return fail(RSG_EINVAL, "fwi_e2d: window outside the stacked image");
-        maxsrc = std::max(maxsrc, (size_t)s.src.ntrace);
-        maxrec = std::max(maxrec, (size_t)s.obs[0].ntrace);
-        maxnx = std::max(maxnx, nx);
-        maxnz = std::max(maxnz, nz);
-    }
-    // ---- state layout: per shot [Vx Vz Sxx Szz Sxz | 16 strips], shots back to back
-    std::vector<size_t> off(ns + 1, 0), segs;
-    segs.reserve((size_t)ns * 21 + 1);
-    for (int k = 0; k < ns; k++) {
-        const size_t n = (size_t)sh[k].g.nx * sh[k].g.nz, sx = (size_t)lpml * sh[k].g.nz,
-                     sz = (size_t)sh[k].g.nx * lpml;
-        size_t o = off[k];
-        for (int j = 0; j < 5; j++, o += n) segs.push_back(o);
-        for (int i = 0; i < RSG_E2D_NSTRIP; i++) {
-            segs.push_back(o);
-            o += e2d_xstrip(i) ? sx : sz;
-        }
-        off[k + 1] = o;
-    }
-    segs.push_back(off[ns]);
-    const size_t total = off[ns];
-    const int nseg = (int)segs.size() - 1;
-    Revolve rev(nt, f->ncheck);
-    const bool f16 = f->fp16 != 0;
-    if (f16 && nseg > 65535) return fail(RSG_EINVAL, "fwi_e2d: FP16 checkpoints take at most 3120 shots");
-    {
-        size_t free_b = 0, total_b = 0;
-        RSG_HIP(hipMemGetInfo(&free_b, &total_b));
-        size_t img = 0;
-        for (int k = 0; k < ns; k++)
-            img += 3 * (size_t)(sh[k].g.nx - 2 * lpml) * (sh[k].g.nz - 2 * lpml);
-        const size_t need = total * 4 * 2 + img * 4 + total * (f16 ? 2 : 4) * (size_t)rev.snaps;
-        if (need + ((size_t)1 << 30) > free_b)
-            return fail(RSG_ENOMEM, "fwi_e2d: states and checkpoints do not fit in device memory");
-    }
+    std::vector<size_t> off, segs, img_off;
+    size_t total = 0, cstride = 0;
+    int nseg = 0, nchunk = 0;
+    bool f16 = false;
+    std::unique_ptr<Revolve> rev;
+    bool reverse = false;
     DevMem dm;
-    float *fwb = nullptr, *bwb = nullptr, *imgs = nullptr;
-    if (int rc = dm.alloc(&fwb, total)) return rc;
-    if (int rc = dm.alloc(&bwb, total)) return rc;
-    if (int rc = launch_zero(fwb, total * sizeof(float), st)) return rc;
-    if (int rc = launch_zero(bwb, total * sizeof(float), st)) return rc;
-    std::vector<size_t> img_off(ns + 1, 0);
-    for (int k = 0; k < ns; k++)
-        img_off[k + 1] = img_off[k] + 3 * (size_t)(sh[k].g.nx - 2 * lpml) * (sh[k].g.nz - 2 * lpml);
-    if (int rc = dm.alloc(&imgs, img_off[ns])) return rc;
-    if (int rc = launch_zero(imgs, img_off[ns] * sizeof(float), st)) return rc;
-    // ---- per-shot states, ragged plans, kernel descriptors
-    std::vector<rsg_grid> grids(ns);
-    std::vector<rsg_state> sf(ns), sb(ns);
-    std::vector<ShotDev> hd(ns);
-    for (int k = 0; k < ns; k++) {
-        const rsg_fwi_e2d_shot &s = sh[k];
-        grids[k] = s.g;
-        const size_t n = (size_t)s.g.nx * s.g.nz, sx = (size_t)lpml * s.g.nz, sz = (size_t)s.g.nx * lpml;
-        for (int w = 0; w < 2; w++) {
-            float *b = (w ? bwb : fwb) + off[k];
-            rsg_state &r = w ? sb[k] : sf[k];
-            r = rsg_state{};
-            r.Vx = b;
-            r.Vz = b + n;
-            r.Sxx = b + 2 * n;
-            r.Szz = b + 3 * n;
-            r.Sxz = b + 4 * n;
-            float *p = b + 5 * n;
-            for (int i = 0; i < RSG_E2D_NSTRIP; i++) {
-                r.psi[i] = p;
-                p += e2d_xstrip(i) ? sx : sz;
-            }
-            r.L = s.L;
-            r.L2M = s.L2M;
-            r.M = s.M;
-            r.Rx = s.Rx;
-            r.Rz = s.Rz;
-            r.tab = f->tab;
-        }
-        ShotDev &d = hd[k];
-        d.nxp = s.g.nx;
-        d.nzp = s.g.nz;
-        d.nx = s.g.nx - 2 * lpml;
-        d.nz = s.g.nz - 2 * lpml;
-        d.lpml = lpml;
-        d.ox = s.ox;
-        d.fVx = sf[k].Vx;
-        d.fVz = sf[k].Vz;
-        d.fSxx = sf[k].Sxx;
-        d.fSzz = sf[k].Szz;
-        d.fSxz = sf[k].Sxz;
-        d.bVx = sb[k].Vx;
-        d.bVz = sb[k].Vz;
-        d.bSxx = sb[k].Sxx;
-        d.bSzz = sb[k].Szz;
-        d.bSxz = sb[k].Sxz;
-        d.Rx = s.Rx;
-        d.Rz = s.Rz;
-        d.Vp = s.Vp;
-        d.Vs = s.Vs;
-        d.Rho = s.Rho;
-        d.sfield = s.src.field;
-        d.nsrc = s.src.ntrace;
-        d.nrec = s.obs[0].ntrace;
-        d.smap = s.src.map;
-        d.sshift = s.src.shift;
-        d.wav = s.src.data;
-        d.wavgrad = s.wavgrad;
-        for (int c = 0; c < 3; c++) {
-            d.gmap[c] = s.obs[c].map;
-            d.gshift[c] = s.obs[c].shift;
-            d.dobs[c] = s.obs[c].data;
-            d.dmod[c] = s.dmod[c];
-            d.dres[c] = s.dres[c];
-        }
-        const size_t ni = (size_t)d.nx * d.nz;
-        d.lam = imgs + img_off[k];
-        d.mu = d.lam + ni;
-        d.rho = d.mu + ni;
-        for (int c = 0; c < 3; c++)
-            if (int rc = launch_zero(s.dmod[c], sizeof(float) * d.nrec * nt, st)) return rc;
-        if (s.wavgrad)
-            if (int rc = launch_zero(s.wavgrad, sizeof(float) * d.nsrc * nt, st)) return rc;
-    }
-    // the three records of every shot for the misfit and residuals
-    std::vector<E2DRecs> recs(ns);
-    for (int k = 0; k < ns; k++) {
-        E2DRecs &r = recs[k];
-        r = E2DRecs{};
-        r.nrec = sh[k].obs[0].ntrace;
-        r.map = sh[k].obs[1].map;  // traces count in the misfit by their Ux map (fwi.cpp:2911)
-        for (int c = 0; c < 3; c++) {
-            r.mod[c] = sh[k].dmod[c];
-            r.obs[c] = sh[k].obs[c].data;
-            r.wei[c] = sh[k].weight[c];
-            r.res[c] = sh[k].dres[c];
-        }
-    }
-    PlanGuard pf, pb;
-    if (int rc = ragged2d_create(RSG_KIND_E2D, ns, grids.data(), sf.data(), &pf.p)) return rc;
-    if (int rc = ragged2d_create(RSG_KIND_E2D, ns, grids.data(), sb.data(), &pb.p)) return rc;
-    ShotDev *dsd = nullptr;
-    size_t *dsegs = nullptr;
-    double *dmis = nullptr;
-    if (int rc = dm.alloc(&dsd, ns)) return rc;
-    if (int rc = dm.alloc(&dmis, 1)) return rc;
-    if (int rc = launch_zero(dmis, sizeof(double), st)) return rc;
-    RSG_HIP(hipMemcpyAsync(dsd, hd.data(), sizeof(ShotDev) * ns, hipMemcpyHostToDevice, st));
-    Chunk16 *dchunks = nullptr;
-    unsigned *ckpart = nullptr;  // per-chunk maxima of the FP16 packing
-    // FP16 checkpoint slots start at multiples of 4 values (8-byte aligned)
-    const size_t cstride = (total + 3) & ~(size_t)3;
-    const std::vector<Chunk16> chunks = f16 ? store16_chunks(segs) : std::vector<Chunk16>();
-    const int nchunk = (int)chunks.size();
-    if (f16) {
-        if (int rc = dm.alloc(&dsegs, segs.size())) return rc;
-        if (int rc = dm.alloc(&dchunks, chunks.size())) return rc;
-        if (int rc = dm.alloc(&ckpart, chunks.size())) return rc;
-        RSG_HIP(hipMemcpyAsync(dsegs, segs.data(), sizeof(size_t) * segs.size(), hipMemcpyHostToDevice, st));
-        RSG_HIP(hipMemcpyAsync(dchunks, chunks.data(), sizeof(Chunk16) * chunks.size(), hipMemcpyHostToDevice, st));
-    }
-    float *ck = nullptr;
+    float *fwb = nullptr, *bwb = nullptr, *imgs = nullptr, *ck = nullptr;
     half_t *ck16 = nullptr;
-    unsigned *ckamax = nullptr;
-    if (f16) {
-        if (int rc = dm.alloc(&ck16, cstride * (size_t)rev.snaps)) return rc;
-        if (int rc = dm.alloc(&ckamax, (size_t)nseg * rev.snaps)) return rc;
-    } else {
-        if (int rc = dm.alloc(&ck, total * (size_t)rev.snaps)) return rc;
+    unsigned *ckamax = nullptr, *ckpart = nullptr;
+    size_t *dsegs = nullptr;
+    Chunk16 *dchunks = nullptr;
+    ShotDev *dsd = nullptr;
+    double *dmis = nullptr;
+    std::vector<E2DRecs> recs;
+    PlanGuard pf, pb;
+    dim3 iob_s, iob_r, xcg;
+
+    int init(const rsg_fwi_e2d *f_, const rsg_fwi_e2d_shot *sh_, int ns_, hipStream_t st) {
+        f = f_;
+        sh = sh_;
+        ns = ns_;
+        const rsg_grid &g0 = sh[0].g;
+        nt = sh[0].src.nt;
+        lpml = g0.lpml;
+        order = g0.order;
+        dt = g0.dt;
+        // ---- validation
+        for (int k = 0; k < ns; k++) {
+            const rsg_fwi_e2d_shot &s = sh[k];
+            const int nx = s.g.nx - 2 * lpml, nz = s.g.nz - 2 * lpml;
+            if (s.g.lpml != lpml || s.g.order != order || s.g.dt != dt || s.g.dx != g0.dx || s.g.dz != g0.dz ||
+                s.g.ny != 1)
+                return fail(RSG_EINVAL, "fwi_e2d: shots must share lpml, order, dx, dz, dt (2D grids)");
+            if (!s.L || !s.L2M || !s.M || !s.Rx || !s.Rz || !s.Vp || !s.Vs || !s.Rho)
+                return fail(RSG_EINVAL, "fwi_e2d: null model array");
+            if (s.src.nt != nt || s.src.ntrace < 0 ||
+                (s.src.ntrace > 0 && (!s.src.map || !s.src.shift || !s.src.data)))
+                return fail(RSG_EINVAL, "fwi_e2d: every source needs nt samples and its map/shift/data");
+            if (s.src.field != RSG_PRESSURE && s.src.field != RSG_VX && s.src.field != RSG_VZ)
+                return fail(RSG_EINVAL, "fwi_e2d: source field must be P, VX or VZ");
+            const int want[3] = {RSG_PRESSURE, RSG_VX, RSG_VZ};
+            for (int c = 0; c < 3; c++) {
+                const rsg_traces &o = s.obs[c];
+                if (o.field != want[c] || o.nt != nt || o.ntrace != s.obs[0].ntrace || !o.map || !o.shift ||
+                    !o.data || !s.dmod[c] || !s.dres[c])
+                    return fail(RSG_EINVAL, "fwi_e2d: obs[0..2] must be P, VX, VZ records of one receiver set");
+            }
+            if (nx < 3 || nz < 3 || s.ox < 0 || s.ox + nx > f->nx || nz > f->nz)
+                return fail(RSG_EINVAL, "fwi_e2d: window outside the stacked image");
+            maxsrc = std::max(maxsrc, (size_t)s.src.ntrace);
+            maxrec = std::max(maxrec, (size_t)s.obs[0].ntrace);
+            maxnx = std::max(maxnx, nx);
+            maxnz = std::max(maxnz, nz);
+        }
+        // ---- state layout: per shot [Vx Vz Sxx Szz Sxz | 16 strips], shots back to back
+        off.assign(ns + 1, 0);
+        segs.reserve((size_t)ns * 21 + 1);
+        for (int k = 0; k < ns; k++) {
+            const size_t n = (size_t)sh[k].g.nx * sh[k].g.nz, sx = (size_t)lpml * sh[k].g.nz,
+                         sz = (size_t)sh[k].g.nx * lpml;
+            size_t o = off[k];
+            for (int j = 0; j < 5; j++, o += n) segs.push_back(o);
+            for (int i = 0; i < RSG_E2D_NSTRIP; i++) {
+                segs.push_back(o);
+                o += e2d_xstrip(i) ? sx : sz;
+            }
+            off[k + 1] = o;
+        }
+        segs.push_back(off[ns]);
+        total = off[ns];
+        nseg = (int)segs.size() - 1;
+        rev.reset(new Revolve(nt, f->ncheck));
+        f16 = f->fp16 != 0;
+        if (f16 && nseg > 65535) return fail(RSG_EINVAL, "fwi_e2d: FP16 checkpoints take at most 3120 shots");
+        {
+            size_t free_b = 0, total_b = 0;
+            RSG_HIP(hipMemGetInfo(&free_b, &total_b));
+            size_t img = 0;
+            for (int k = 0; k < ns; k++) img += 3 * (size_t)(sh[k].g.nx - 2 * lpml) * (sh[k].g.nz - 2 * lpml);
+            const size_t need = total * 4 * 2 + img * 4 + total * (f16 ? 2 : 4) * (size_t)rev->snaps;
+            if (need + ((size_t)1 << 30) > free_b)
+                return fail(RSG_ENOMEM, "fwi_e2d: states and checkpoints do not fit in device memory");
+        }
+        if (int rc = dm.alloc(&fwb, total)) return rc;
+        if (int rc = dm.alloc(&bwb, total)) return rc;
+        img_off.assign(ns + 1, 0);
+        for (int k = 0; k < ns; k++)
+            img_off[k + 1] = img_off[k] + 3 * (size_t)(sh[k].g.nx - 2 * lpml) * (sh[k].g.nz - 2 * lpml);
+        if (int rc = dm.alloc(&imgs, img_off[ns])) return rc;
+        // ---- per-shot states, ragged plans, kernel descriptors
+        std::vector<rsg_grid> grids(ns);
+        std::vector<rsg_state> sf(ns), sb(ns);
+        std::vector<ShotDev> hd(ns);
+        for (int k = 0; k < ns; k++) {
+            const rsg_fwi_e2d_shot &s = sh[k];
+            grids[k] = s.g;
+            const size_t n = (size_t)s.g.nx * s.g.nz, sx = (size_t)lpml * s.g.nz, sz = (size_t)s.g.nx * lpml;
+            for (int w = 0; w < 2; w++) {
+                float *b = (w ? bwb : fwb) + off[k];
+                rsg_state &r = w ? sb[k] : sf[k];
+                r = rsg_state{};
+                r.Vx = b;
+                r.Vz = b + n;
+                r.Sxx = b + 2 * n;
+                r.Szz = b + 3 * n;
+                r.Sxz = b + 4 * n;
+                float *q = b + 5 * n;
+                for (int i = 0; i < RSG_E2D_NSTRIP; i++) {
+                    r.psi[i] = q;
+                    q += e2d_xstrip(i) ? sx : sz;
+                }
+                r.L = s.L;
+                r.L2M = s.L2M;
+                r.M = s.M;
+                r.Rx = s.Rx;
+                r.Rz = s.Rz;
+                r.tab = f->tab;
+            }
+            ShotDev &d = hd[k];
+            d.nxp = s.g.nx;
+            d.nzp = s.g.nz;
+            d.nx = s.g.nx - 2 * lpml;
+            d.nz = s.g.nz - 2 * lpml;
+            d.lpml = lpml;
+            d.ox = s.ox;
+            d.fVx = sf[k].Vx;
+            d.fVz = sf[k].Vz;
+            d.fSxx = sf[k].Sxx;
+            d.fSzz = sf[k].Szz;
+            d.fSxz = sf[k].Sxz;
+            d.bVx = sb[k].Vx;
+            d.bVz = sb[k].Vz;
+            d.bSxx = sb[k].Sxx;
+            d.bSzz = sb[k].Szz;
+            d.bSxz = sb[k].Sxz;
+            d.Rx = s.Rx;
+            d.Rz = s.Rz;
+            d.Vp = s.Vp;
+            d.Vs = s.Vs;
+            d.Rho = s.Rho;
+            d.sfield = s.src.field;
+            d.nsrc = s.src.ntrace;
+            d.nrec = s.obs[0].ntrace;
+            d.smap = s.src.map;
+            d.sshift = s.src.shift;
+            d.wav = s.src.data;
+            d.wavgrad = s.wavgrad;
+            for (int c = 0; c < 3; c++) {
+                d.gmap[c] = s.obs[c].map;
+                d.gshift[c] = s.obs[c].shift;
+                d.dobs[c] = s.obs[c].data;
+                d.dmod[c] = s.dmod[c];
+                d.dres[c] = s.dres[c];
+            }
+            const size_t ni = (size_t)d.nx * d.nz;
+            d.lam = imgs + img_off[k];
+            d.mu = d.lam + ni;
+            d.rho = d.mu + ni;
+        }
+        // the three records of every shot for the misfit and residuals
+        recs.assign(ns, E2DRecs{});
+        for (int k = 0; k < ns; k++) {
+            E2DRecs &r = recs[k];
+            r.nrec = sh[k].obs[0].ntrace;
+            r.map = sh[k].obs[1].map;  // traces count in the misfit by their Ux map (fwi.cpp:2911)
+            for (int c = 0; c < 3; c++) {
+                r.mod[c] = sh[k].dmod[c];
+                r.obs[c] = sh[k].obs[c].data;
+                r.wei[c] = sh[k].weight[c];
+                r.res[c] = sh[k].dres[c];
+            }
+        }
+        if (int rc = ragged2d_create(RSG_KIND_E2D, ns, grids.data(), sf.data(), &pf.p)) return rc;
+        if (int rc = ragged2d_create(RSG_KIND_E2D, ns, grids.data(), sb.data(), &pb.p)) return rc;
+        if (int rc = dm.alloc(&dsd, ns)) return rc;
+        if (int rc = dm.alloc(&dmis, 1)) return rc;
+        RSG_HIP(hipMemcpyAsync(dsd, hd.data(), sizeof(ShotDev) * ns, hipMemcpyHostToDevice, st));
+        // FP16 checkpoint slots start at multiples of 4 values (8-byte aligned)
+        cstride = (total + 3) & ~(size_t)3;
+        const std::vector<Chunk16> chunks = f16 ? store16_chunks(segs) : std::vector<Chunk16>();
+        nchunk = (int)chunks.size();
+        if (f16) {
+            if (int rc = dm.alloc(&dsegs, segs.size())) return rc;
+            if (int rc = dm.alloc(&dchunks, chunks.size())) return rc;
+            if (int rc = dm.alloc(&ckpart, chunks.size())) return rc;
+            if (int rc = dm.alloc(&ck16, cstride * (size_t)rev->snaps)) return rc;
+            if (int rc = dm.alloc(&ckamax, (size_t)nseg * rev->snaps)) return rc;
+            RSG_HIP(hipMemcpyAsync(dsegs, segs.data(), sizeof(size_t) * segs.size(), hipMemcpyHostToDevice, st));
+            RSG_HIP(
+                hipMemcpyAsync(dchunks, chunks.data(), sizeof(Chunk16) * chunks.size(), hipMemcpyHostToDevice, st));
+        } else {
+            if (int rc = dm.alloc(&ck, total * (size_t)rev->snaps)) return rc;
+        }
+        RSG_HIP(hipStreamSynchronize(st));  // the host tables above go out of scope
+        iob_s = dim3((unsigned)((maxsrc + kIoThreads - 1) / kIoThreads), ns);
+        iob_r = dim3((unsigned)((maxrec + kIoThreads - 1) / kIoThreads), ns);
+        xcg = dim3((maxnx + 63) / 64, (maxnz + 3) / 4, ns);
+        return RSG_OK;
     }
-    // ---- step helpers
-    const dim3 iob_s((unsigned)((maxsrc + kIoThreads - 1) / kIoThreads), ns);
-    const dim3 iob_r((unsigned)((maxrec + kIoThreads - 1) / kIoThreads), ns);
-    const dim3 xcg((maxnx + 63) / 64, (maxnz + 3) / 4, ns);
-    auto fstep = [&](int it, bool rec) -> int {
+
+    int fstep(int it, bool rec, hipStream_t st) {
         if (int rc = ragged2d_step(&pf.p, 0, st)) return rc;
         if (int rc = ragged2d_step(&pf.p, 1, st)) return rc;
         if (maxsrc) {
@@ -418,71 +436,144 @@ static int fwi_e2d_impl(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *sh, int ns
             RSG_LAUNCH_CHECK("e2d_record_kernel");
         }
         return RSG_OK;
-    };
-    auto reverse_tail = [&](int it) -> int {
+    }
+
+    int reverse_tail(int it, hipStream_t st) {
         if (maxrec) {
             e2d_residual_kernel<<<iob_r, kIoThreads, 0, st>>>(dsd, it, nt, dt);
             RSG_LAUNCH_CHECK("e2d_residual_kernel");
         }
-        e2d_crosscorr_kernel<<<xcg, dim3(64, 4), 0, st>>>(dsd, g0.dx, g0.dz);
+        e2d_crosscorr_kernel<<<xcg, dim3(64, 4), 0, st>>>(dsd, sh[0].g.dx, sh[0].g.dz);
         RSG_LAUNCH_CHECK("e2d_crosscorr_kernel");
         if (maxsrc) {
             e2d_wavgrad_kernel<<<iob_s, kIoThreads, 0, st>>>(dsd, it, nt);
             RSG_LAUNCH_CHECK("e2d_wavgrad_kernel");
         }
         return RSG_OK;
-    };
-    // ---- Revolve loop (fwi.cpp:3986-4124)
-    bool reverse = false;
-    for (;;) {
-        const int oldcapo = rev.capo;
-        const Revolve::Action a = rev.next();
-        const int capo = rev.capo;
-        if (a == Revolve::ADVANCE) {
-            for (int it = oldcapo; it < capo; it++)
-                if (int rc = fstep(it, !reverse)) return rc;
-        } else if (a == Revolve::FIRSTURN) {
-            if (int rc = fstep(capo, true)) return rc;
-            if (maxrec)
-                if (int rc = elastic_misfit_residuals(f->misfit_type, recs, nt, dmis, st)) return rc;
-            if (int rc = reverse_tail(capo)) return rc;
-            reverse = true;
-        } else if (a == Revolve::YOUTURN) {
-            if (int rc = ragged2d_step(&pb.p, 2, st)) return rc;
-            if (int rc = ragged2d_step(&pb.p, 3, st)) return rc;
-            if (int rc = reverse_tail(capo)) return rc;
-        } else if (a == Revolve::TAKESHOT) {
-            if (f16) {
-                if (int rc = pack16_table(fwb, ck16 + cstride * rev.check, ckamax + (size_t)nseg * rev.check, ckpart,
-                                          dsegs, dchunks, nchunk, nseg, total, st))
-                    return rc;
+    }
+
+    // Phase A: zero every state and output, then the Revolve loop
+    // (fwi.cpp:3986-4124) up to and including the forward step of the
+    // first turn.  Returns that step's index in *turn.
+    int phase_a(hipStream_t st, int *turn) {
+        if (int rc = launch_zero(fwb, total * sizeof(float), st)) return rc;
+        if (int rc = launch_zero(bwb, total * sizeof(float), st)) return rc;
+        if (int rc = launch_zero(imgs, img_off[ns] * sizeof(float), st)) return rc;
+        if (int rc = launch_zero(dmis, sizeof(double), st)) return rc;
+        for (int k = 0; k < ns; k++) {
+            for (int c = 0; c < 3; c++)
+                if (int rc = launch_zero(sh[k].dmod[c], sizeof(float) * sh[k].obs[0].ntrace * nt, st)) return rc;
+            if (sh[k].wavgrad)
+                if (int rc = launch_zero(sh[k].wavgrad, sizeof(float) * sh[k].src.ntrace * nt, st)) return rc;
+        }
+        rev.reset(new Revolve(nt, f->ncheck));
+        reverse = false;
+        for (;;) {
+            const int oldcapo = rev->capo;
+            const Revolve::Action a = rev->next();
+            const int capo = rev->capo;
+            if (a == Revolve::ADVANCE) {
+                for (int it = oldcapo; it < capo; it++)
+                    if (int rc = fstep(it, true, st)) return rc;
+            } else if (a == Revolve::FIRSTURN) {
+                if (int rc = fstep(capo, true, st)) return rc;
+                *turn = capo;
+                return RSG_OK;
+            } else if (a == Revolve::TAKESHOT) {
+                if (int rc = takeshot(st)) return rc;
             } else {
-                RSG_HIP(hipMemcpyAsync(ck + total * rev.check, fwb, total * sizeof(float), hipMemcpyDeviceToDevice,
-                                       st));
+                return fail(RSG_EINVAL, "fwi_e2d: unexpected Revolve action before the first turn");
             }
-        } else if (a == Revolve::RESTORE) {
-            if (f16) {
-                if (int rc = unpack16_table(ck16 + cstride * rev.check, ckamax + (size_t)nseg * rev.check, fwb, dsegs,
-                                            dchunks, nchunk, total, st))
-                    return rc;
-            } else {
-                RSG_HIP(hipMemcpyAsync(fwb, ck + total * rev.check, total * sizeof(float), hipMemcpyDeviceToDevice,
-                                       st));
-            }
-        } else if (a == Revolve::TERMINATE) {
-            break;
-        } else {
-            return fail(RSG_EINVAL, "fwi_e2d: Revolve returned an error");
         }
     }
-    e2d_stack_kernel<<<dim3((f->nx + 255) / 256, f->nz), 256, 0, st>>>(dsd, ns, f->nx, f->nz, f->grad[0], f->grad[1],
-                                                                         f->grad[2], f->scale);
-    RSG_LAUNCH_CHECK("e2d_stack_kernel");
-    double m = 0.0;
-    RSG_HIP(hipMemcpyAsync(&m, dmis, sizeof(double), hipMemcpyDeviceToHost, st));
-    RSG_HIP(hipStreamSynchronize(st));
-    if (f->misfit) *f->misfit = m;
-    return RSG_OK;
+
+    int takeshot(hipStream_t st) {
+        if (f16)
+            return pack16_table(fwb, ck16 + cstride * rev->check, ckamax + (size_t)nseg * rev->check, ckpart, dsegs,
+                                dchunks, nchunk, nseg, total, st);
+        RSG_HIP(hipMemcpyAsync(ck + total * rev->check, fwb, total * sizeof(float), hipMemcpyDeviceToDevice, st));
+        return RSG_OK;
+    }
+
+    int misfit(hipStream_t st) {
+        if (!maxrec) return RSG_OK;
+        return elastic_misfit_residuals(f->misfit_type, recs, nt, dmis, st);
+    }
+
+    // Phase B: the first turn's adjoint terms, the rest of the Revolve loop,
+    // then every window stacked into the caller's images.
+    int phase_b(int turn, hipStream_t st) {
+        if (int rc = reverse_tail(turn, st)) return rc;
+        reverse = true;
+        for (;;) {
+            const int oldcapo = rev->capo;
+            const Revolve::Action a = rev->next();
+            const int capo = rev->capo;
+            if (a == Revolve::ADVANCE) {
+                for (int it = oldcapo; it < capo; it++)
+                    if (int rc = fstep(it, !reverse, st)) return rc;
+            } else if (a == Revolve::YOUTURN) {
+                if (int rc = ragged2d_step(&pb.p, 2, st)) return rc;
+                if (int rc = ragged2d_step(&pb.p, 3, st)) return rc;
+                if (int rc = reverse_tail(capo, st)) return rc;
+            } else if (a == Revolve::TAKESHOT) {
+                if (int rc = takeshot(st)) return rc;
+            } else if (a == Revolve::RESTORE) {
+                if (f16) {
+                    if (int rc = unpack16_table(ck16 + cstride * rev->check, ckamax + (size_t)nseg * rev->check, fwb,
+                                                dsegs, dchunks, nchunk, total, st))
+                        return rc;
+                } else {
+                    RSG_HIP(hipMemcpyAsync(fwb, ck + total * rev->check, total * sizeof(float),
+                                           hipMemcpyDeviceToDevice, st));
+                }
+            } else if (a == Revolve::TERMINATE) {
+                break;
+            } else {
+                return fail(RSG_EINVAL, "fwi_e2d: Revolve returned an error");
+            }
+        }
+        e2d_stack_kernel<<<dim3((f->nx + 255) / 256, f->nz), 256, 0, st>>>(dsd, ns, f->nx, f->nz, f->grad[0],
+                                                                             f->grad[1], f->grad[2], f->scale);
+        RSG_LAUNCH_CHECK("e2d_stack_kernel");
+        return RSG_OK;
+    }
+
+    int read_misfit(double *out, hipStream_t st) {
+        double m = 0.0;
+        RSG_HIP(hipMemcpyAsync(&m, dmis, sizeof(double), hipMemcpyDeviceToHost, st));
+        RSG_HIP(hipStreamSynchronize(st));
+        if (out) *out = m;
+        return RSG_OK;
+    }
+};
+
+static int fwi_e2d_impl(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *sh, int ns, hipStream_t st) {
+    E2DFwi r;
+    int turn = 0;
+    if (int rc = r.init(f, sh, ns, st)) return rc;
+    if (int rc = r.phase_a(st, &turn)) return rc;
+    if (int rc = r.misfit(st)) return rc;
+    if (int rc = r.phase_b(turn, st)) return rc;
+    return r.read_misfit(f->misfit, st);
+}
+
+// Capture of one phase as a graph: the stream must not be the legacy
+// default stream (the plan makes its own).
+template <class F>
+static int capture(hipStream_t st, hipGraphExec_t *exec, F &&body) {
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return fail(RSG_EHIP, "fwi_e2d plan: hipStreamBeginCapture failed");
+    int rc = body();
+    const hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc == RSG_OK && e != hipSuccess) rc = check_hip(e, "fwi_e2d plan: hipStreamEndCapture");
+    if (rc == RSG_OK) {
+        const hipError_t e2 = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+        if (e2 != hipSuccess) rc = check_hip(e2, "fwi_e2d plan: hipGraphInstantiate");
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
 }
 
 }  // namespace rsg
@@ -497,4 +588,65 @@ extern "C" int rsg_fwi_e2d_gradient_batch(const rsg_fwi_e2d *f, const rsg_fwi_e2
                          "rsg_fwi_e2d_gradient_batch: misfit type must be DIFFERENCE (0), CORRELATION (1), "
                          "ADAPTIVE_GAUSS (2), ADAPTIVE_LINEAR (3) or ADAPTIVE_SINGLE (4)");
     return rsg::fwi_e2d_impl(f, shots, nshot, (hipStream_t)stream);
+}
+
+struct rsg_fwi_e2d_plan {
+    rsg_fwi_e2d f;                         // copy of the settings (pointers stay the caller's)
+    std::vector<rsg_fwi_e2d_shot> shots;   // copy of the shot table
+    rsg::E2DFwi run;
+    hipStream_t st = nullptr;
+    hipGraphExec_t a = nullptr, b = nullptr;
+    int turn = 0;
+    ~rsg_fwi_e2d_plan() {
+        if (a) (void)hipGraphExecDestroy(a);
+        if (b) (void)hipGraphExecDestroy(b);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+extern "C" int rsg_fwi_e2d_plan_create(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *shots, int nshot,
+                                       rsg_fwi_e2d_plan **plan) {
+    if (!plan) return rsg::fail(RSG_EINVAL, "rsg_fwi_e2d_plan_create: null plan");
+    *plan = nullptr;
+    if (!f || !shots || nshot < 1 || nshot > 65535) return rsg::fail(RSG_EINVAL, "rsg_fwi_e2d_plan_create: bad arguments");
+    if (!f->tab || !f->grad[0] || !f->grad[1] || !f->grad[2] || f->nx < 1 || f->nz < 1)
+        return rsg::fail(RSG_EINVAL, "rsg_fwi_e2d_plan_create: null tables or images");
+    if (f->misfit_type < 0 || f->misfit_type > 4)
+        return rsg::fail(RSG_EINVAL, "rsg_fwi_e2d_plan_create: misfit type must be 0..4");
+    std::unique_ptr<rsg_fwi_e2d_plan> p(new rsg_fwi_e2d_plan);
+    p->f = *f;
+    p->f.misfit = nullptr;
+    p->shots.assign(shots, shots + nshot);
+    if (hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking) != hipSuccess)
+        return rsg::fail(RSG_EHIP, "rsg_fwi_e2d_plan_create: stream creation failed");
+    if (int rc = p->run.init(&p->f, p->shots.data(), nshot, p->st)) return rc;
+    rsg_fwi_e2d_plan *q = p.get();
+    if (int rc = rsg::capture(q->st, &q->a, [q]() { return q->run.phase_a(q->st, &q->turn); })) return rc;
+    if (int rc = rsg::capture(q->st, &q->b, [q]() { return q->run.phase_b(q->turn, q->st); })) return rc;
+    *plan = p.release();
+    return RSG_OK;
+}
+
+extern "C" int rsg_fwi_e2d_plan_run(rsg_fwi_e2d_plan *plan, double *misfit, void *stream) {
+    if (!plan || !plan->a || !plan->b) return rsg::fail(RSG_EINVAL, "rsg_fwi_e2d_plan_run: no plan");
+    hipStream_t st = plan->st;
+    // the caller's stream work (new models, zeroed images) comes first
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+        return rsg::fail(RSG_EHIP, "rsg_fwi_e2d_plan_run: event creation failed");
+    hipError_t e = hipEventRecord(ev, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, ev, 0);
+    (void)hipEventDestroy(ev);
+    if (e != hipSuccess) return rsg::check_hip(e, "rsg_fwi_e2d_plan_run: stream ordering");
+    RSG_HIP(hipGraphLaunch(plan->a, st));
+    if (int rc = plan->run.misfit(st)) return rc;
+    RSG_HIP(hipGraphLaunch(plan->b, st));
+    return plan->run.read_misfit(misfit, st);  // synchronizes the plan's stream
+}
+
+extern "C" int rsg_fwi_e2d_plan_destroy(rsg_fwi_e2d_plan *plan) {
+    if (!plan) return RSG_OK;
+    (void)hipStreamSynchronize(plan->st);
+    delete plan;
+    return RSG_OK;
 }

@@ -455,6 +455,96 @@ class ElasticSurvey:
         self.ox, self.oz = float(ox), float(oz)  # origin of the full model
 
 
+class _E2DBatch:
+    """Device set-up of an ElasticSurvey on the model (vp, vs, rho [nz, nx]):
+    per shot a staggered window model, device records, residual and
+    optional wavelet-gradient arrays, and the C structures of
+    rsg_fwi_e2d_gradient_batch / rsg_fwi_e2d_plan_create."""
+
+    def __init__(self, vp, vs, rho, survey: ElasticSurvey, scale=True, wavgrad=False):
+        dev = _dev()
+        vp, vs, rho = (np.ascontiguousarray(a, np.float32) for a in (vp, vs, rho))
+        nz, nx = vp.shape
+        self.survey, self.nx, self.nz = survey, nx, nz
+        ns = len(survey.shots)
+        self.keep, self.models, self.windows, self.outs = [], [], [], []
+        self.shots = (R.FwiE2DShot * ns)()
+
+        def dev_t(a, dtype=torch.float32):
+            t = torch.from_numpy(np.ascontiguousarray(a)).to(dev, dtype)
+            self.keep.append(t)
+            return t
+
+        for k, sh in enumerate(survey.shots):
+            nxw = nx - sh.x0 if sh.nxw is None else int(sh.nxw)
+            if sh.x0 < 0 or sh.x0 + nxw > nx:
+                raise R.RsgError(f"elastic_batch_gradient: window of shot {k} outside the model")
+            m = ModelElastic2D(nxw, nz, survey.lpml, survey.dx, survey.dz, survey.ox + sh.x0 * survey.dx,
+                               survey.oz, survey.fs)
+            m.Vp[:], m.Vs[:], m.R[:] = vp[:, sh.x0:sh.x0 + nxw], vs[:, sh.x0:sh.x0 + nxw], rho[:, sh.x0:sh.x0 + nxw]
+            m.staggerModels()
+            self.models.append(m)
+            self.windows.append((sh.x0, nxw))
+            nt = sh.wavelet.shape[1]
+            src = Data2D(sh.scoords.shape[0], nt, survey.dt, 0.0, sh.srcfield)
+            src.scoords[:] = sh.scoords
+            src.data[:] = sh.wavelet
+            src.makeMap(m, SMAP)
+            t_src, k1 = src._device(SMAP)
+            self.keep.append(k1)
+            s = self.shots[k]
+            s.g = R.Grid(nx=m.getNx_pml(), ny=1, nz=m.getNz_pml(), lpml=m.lpml, order=survey.order, fs=int(m.fs),
+                         dx=m.dx, dy=1.0, dz=m.dz, dt=survey.dt)
+            s.L, s.L2M, s.M, s.Rx, s.Rz = (getattr(m, "_d" + n).data_ptr() for n in ("L", "L2M", "M", "Rx", "Rz"))
+            tv = [dev_t(a) for a in (m.Vp, m.Vs, m.R)]
+            m._dwin = tv  # the unpadded window model the gradient scaling reads
+            s.Vp, s.Vs, s.Rho = (t.data_ptr() for t in tv)
+            s.ox = sh.x0
+            s.src = t_src
+            o = {"dmod": [], "dres": []}
+            for c, f in enumerate(E2D_COMPONENTS):
+                d = Data2D(sh.gcoords.shape[0], nt, survey.dt, 0.0, f)
+                d.gcoords[:] = sh.gcoords
+                d.data[:] = sh.dobs[c]
+                d.makeMap(m, GMAP)
+                t, kk = d._device(GMAP)
+                self.keep.append(kk)
+                s.obs[c] = t
+                dm = torch.zeros((d.ntrace, nt), dtype=torch.float32, device=dev)
+                dr = torch.zeros_like(dm)
+                o["dmod"].append(dm)
+                o["dres"].append(dr)
+                s.dmod[c], s.dres[c] = dm.data_ptr(), dr.data_ptr()
+                s.weight[c] = None if sh.weights[c] is None else dev_t(sh.weights[c]).data_ptr()
+            if wavgrad:
+                wg = torch.zeros((src.ntrace, nt), dtype=torch.float32, device=dev)
+                o["wavgrad"] = wg
+                s.wavgrad = wg.data_ptr()
+            self.outs.append(o)
+        self.pml = Pml(survey.lpml, survey.dt)
+        self.pml.setSmax(SMAX)
+        self.pml.computeABC()
+        self.grads = [torch.zeros((nz, nx), dtype=torch.float32, device=dev) for _ in range(3)]
+        f = R.FwiE2D()
+        f.tab, f.ncheck, f.fp16, f.scale, f.nx, f.nz = self.pml.tab.data_ptr(), survey.ncheck, int(survey.fp16), \
+            int(bool(scale)), nx, nz
+        for c in range(3):
+            f.grad[c] = self.grads[c].data_ptr()
+        f.misfit_type = survey.misfit_type
+        self.f = f
+
+    def set_model(self, vp, vs, rho):
+        """A new model into the same device arrays (restaggered in place)."""
+        vp, vs, rho = (np.ascontiguousarray(a, np.float32) for a in (vp, vs, rho))
+        if vp.shape != (self.nz, self.nx):
+            raise R.RsgError("set_model: model shape differs from the plan's")
+        for m, (x0, nxw) in zip(self.models, self.windows):
+            m.Vp[:], m.Vs[:], m.R[:] = vp[:, x0:x0 + nxw], vs[:, x0:x0 + nxw], rho[:, x0:x0 + nxw]
+            m.staggerModels()
+            for t, a in zip(m._dwin, (m.Vp, m.Vs, m.R)):
+                t.copy_(torch.from_numpy(np.ascontiguousarray(a)))
+
+
 def elastic_batch_gradient(vp, vs, rho, survey: ElasticSurvey, scale=True, wavgrad=False):
     """Misfit and gradients of all shots of `survey` on the model (vp, vs,
     rho [nz, nx]), every shot on its own window, as one ragged batch
@@ -462,76 +552,53 @@ def elastic_batch_gradient(vp, vs, rho, survey: ElasticSurvey, scale=True, wavgr
     gradients of FwiElastic2D::run_optimal, scale=False the lambda / mu / rho
     kernels before scaleGrad.  Returns (misfit, g0, g1, g2, per-shot dict of
     dmod / dres / wavgrad device tensors)."""
-    dev = _dev()
-    vp, vs, rho = (np.ascontiguousarray(a, np.float32) for a in (vp, vs, rho))
-    nz, nx = vp.shape
-    ns = len(survey.shots)
-    keep = []
-    shots = (R.FwiE2DShot * ns)()
-    outs = []
-
-    def dev_t(a, dtype=torch.float32):
-        t = torch.from_numpy(np.ascontiguousarray(a)).to(dev, dtype)
-        keep.append(t)
-        return t
-
-    for k, sh in enumerate(survey.shots):
-        nxw = nx - sh.x0 if sh.nxw is None else int(sh.nxw)
-        if sh.x0 < 0 or sh.x0 + nxw > nx:
-            raise R.RsgError(f"elastic_batch_gradient: window of shot {k} outside the model")
-        m = ModelElastic2D(nxw, nz, survey.lpml, survey.dx, survey.dz, survey.ox + sh.x0 * survey.dx, survey.oz,
-                           survey.fs)
-        m.Vp[:], m.Vs[:], m.R[:] = vp[:, sh.x0:sh.x0 + nxw], vs[:, sh.x0:sh.x0 + nxw], rho[:, sh.x0:sh.x0 + nxw]
-        m.staggerModels()
-        keep.append(m)
-        nt = sh.wavelet.shape[1]
-        src = Data2D(sh.scoords.shape[0], nt, survey.dt, 0.0, sh.srcfield)
-        src.scoords[:] = sh.scoords
-        src.data[:] = sh.wavelet
-        src.makeMap(m, SMAP)
-        t_src, k1 = src._device(SMAP)
-        keep.append(k1)
-        s = shots[k]
-        s.g = R.Grid(nx=m.getNx_pml(), ny=1, nz=m.getNz_pml(), lpml=m.lpml, order=survey.order, fs=int(m.fs),
-                     dx=m.dx, dy=1.0, dz=m.dz, dt=survey.dt)
-        s.L, s.L2M, s.M, s.Rx, s.Rz = (getattr(m, "_d" + n).data_ptr() for n in ("L", "L2M", "M", "Rx", "Rz"))
-        s.Vp, s.Vs, s.Rho = (dev_t(a).data_ptr() for a in (m.Vp, m.Vs, m.R))
-        s.ox = sh.x0
-        s.src = t_src
-        o = {"dmod": [], "dres": []}
-        for c, f in enumerate(E2D_COMPONENTS):
-            d = Data2D(sh.gcoords.shape[0], nt, survey.dt, 0.0, f)
-            d.gcoords[:] = sh.gcoords
-            d.data[:] = sh.dobs[c]
-            d.makeMap(m, GMAP)
-            t, kk = d._device(GMAP)
-            keep.append(kk)
-            s.obs[c] = t
-            dm = torch.zeros((d.ntrace, nt), dtype=torch.float32, device=dev)
-            dr = torch.zeros_like(dm)
-            o["dmod"].append(dm)
-            o["dres"].append(dr)
-            s.dmod[c], s.dres[c] = dm.data_ptr(), dr.data_ptr()
-            s.weight[c] = None if sh.weights[c] is None else dev_t(sh.weights[c]).data_ptr()
-        if wavgrad:
-            wg = torch.zeros((src.ntrace, nt), dtype=torch.float32, device=dev)
-            o["wavgrad"] = wg
-            s.wavgrad = wg.data_ptr()
-        outs.append(o)
-    pml = Pml(survey.lpml, survey.dt)
-    pml.setSmax(SMAX)
-    pml.computeABC()
-    grads = [torch.zeros((nz, nx), dtype=torch.float32, device=dev) for _ in range(3)]
+    bt = _E2DBatch(vp, vs, rho, survey, scale, wavgrad)
     mis = C.c_double(0.0)
-    f = R.FwiE2D()
-    f.tab, f.ncheck, f.fp16, f.scale, f.nx, f.nz = pml.tab.data_ptr(), survey.ncheck, int(survey.fp16), \
-        int(bool(scale)), nx, nz
-    for c in range(3):
-        f.grad[c] = grads[c].data_ptr()
-    f.misfit = C.pointer(mis)
-    f.misfit_type = survey.misfit_type
-    R.check(R.lib().rsg_fwi_e2d_gradient_batch(C.byref(f), shots, ns, _stream()), "rsg_fwi_e2d_gradient_batch")
-    return mis.value, grads[0], grads[1], grads[2], outs
+    bt.f.misfit = C.pointer(mis)
+    R.check(R.lib().rsg_fwi_e2d_gradient_batch(C.byref(bt.f), bt.shots, len(survey.shots), _stream()),
+            "rsg_fwi_e2d_gradient_batch")
+    return mis.value, bt.grads[0], bt.grads[1], bt.grads[2], bt.outs
+
+
+class ElasticFwiPlan:
+    """The batched elastic gradient as a persistent plan
+    (rsg_fwi_e2d_plan_*): built once for a survey and a model shape, the
+    Revolve sweep captured as two hipGraphs; run() replays them and returns
+    (misfit, g0, g1, g2, per-shot outputs) like elastic_batch_gradient;
+    set_model() changes the model between runs (an inversion's iterations).
+    The gradient tensors are zeroed by run() and owned by the plan."""
+
+    def __init__(self, vp, vs, rho, survey: ElasticSurvey, scale=True, wavgrad=False):
+        self._b = _E2DBatch(vp, vs, rho, survey, scale, wavgrad)
+        torch.cuda.current_stream().synchronize()  # inputs in place before the capture
+        h = C.c_void_p()
+        R.check(R.lib().rsg_fwi_e2d_plan_create(C.byref(self._b.f), self._b.shots, len(survey.shots), C.byref(h)),
+                "rsg_fwi_e2d_plan_create")
+        self._h = h
+
+    def set_model(self, vp, vs, rho):
+        self._b.set_model(vp, vs, rho)
+
+    def run(self):
+        if self._h is None:
+            raise R.RsgError("ElasticFwiPlan: closed")
+        for g in self._b.grads:
+            g.zero_()
+        mis = C.c_double(0.0)
+        R.check(R.lib().rsg_fwi_e2d_plan_run(self._h, C.byref(mis), _stream()), "rsg_fwi_e2d_plan_run")
+        g = self._b.grads
+        return mis.value, g[0], g[1], g[2], self._b.outs
+
+    def close(self):
+        if self._h is not None:
+            R.check(R.lib().rsg_fwi_e2d_plan_destroy(self._h), "rsg_fwi_e2d_plan_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class FwiElastic2D:

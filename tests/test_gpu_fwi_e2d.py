@@ -259,3 +259,62 @@ def test_e2d_misfit_residuals_bit_exact(golden, case):
     em = abs(mis - float(g["misfit"])) / abs(float(g["misfit"]))
     print(f"  misfit {mis:.9e} vs reference {float(g['misfit']):.9e} (rel {em:.1e})")
     assert em <= 1e-4
+
+
+@pytest.mark.parametrize("fp16,misfit_type", [(True, 1), (False, 0)])
+def test_plan_replays_equal_eager(golden, fp16, misfit_type):
+    """The persistent plan (rsg_fwi_e2d_plan_*: the Revolve sweep captured as
+    two hipGraphs) gives the eager batch's records, residuals and gradients
+    bit for bit on three ragged windows; a second replay gives them again; and
+    after set_model (the model restaggered in place) a replay equals the eager
+    batch on the new model.  Misfits to 1e-12 (double atomics)."""
+    from oracle import runs
+    from rockseis_amd import fwi, synth
+    g = golden("fwi_e2d_case1")
+    order, fs, ncheck, sf = FWI_E2D_CASES[1]
+    _, nz, lpml, dx, dt, nt = g["grid"]
+    nz, lpml, nt = int(nz), int(lpml), 160
+    nx = 150
+    wav = np.ascontiguousarray(g["wav"][:, :nt])
+    vtrue = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1800.0, 3000.0, nanom=8, seed=5)
+    vp0 = synth.gaussian_smooth(vtrue, 60.0, dx, dx)
+    vs0, rho0 = (vp0 / 1.8).astype(np.float32), synth.gardner(vp0)
+    shots = []
+    for x0, w in ((0, 60), (40, 90), (85, 65)):
+        sc = np.float32([[(x0 + w // 2) * dx, 20.0]])
+        gc = synth.surface_line(w, dx, 20.0, x0 * dx)
+        wv, ws, wr = (a[:, x0:x0 + w] for a in (vtrue, (vtrue / 1.8).astype(np.float32), synth.gardner(vtrue)))
+        obs = np.stack([runs.e2d_run(wv, ws, wr, lpml, dx, dx, dt, nt, order, fs, sf,
+                                     sc - np.float32([[x0 * dx, 0.0]]), wav, f,
+                                     gc - np.float32([[x0 * dx, 0.0]]))["rec"] for f in ("P", "VX", "VZ")])
+        shots.append(fwi.ElasticShot(sc, wav, gc, obs, x0, w, sf))
+    sv = fwi.ElasticSurvey(shots, lpml, dx, dx, dt, order, bool(fs), ncheck, fp16, misfit_type=misfit_type)
+
+    def same(a, b):
+        ma, ga, oa = a[0], a[1:4], a[4]
+        mb, gb, ob = b[0], b[1:4], b[4]
+        assert abs(ma - mb) <= 1e-12 * abs(mb)
+        for x, y in zip(ga, gb):
+            assert np.array_equal(x.cpu().numpy(), y.cpu().numpy())
+        for p, q in zip(oa, ob):
+            for key in ("dmod", "dres"):
+                for c in range(3):
+                    assert np.array_equal(p[key][c].cpu().numpy(), q[key][c].cpu().numpy()), key
+
+    def snap(r):
+        return (r[0], *[t.clone() for t in r[1:4]],
+                [{k: [t.clone() for t in v] for k, v in o.items()} for o in r[4]])
+
+    eager = fwi.elastic_batch_gradient(vp0, vs0, rho0, sv, scale=True)
+    plan = fwi.ElasticFwiPlan(vp0, vs0, rho0, sv, scale=True)
+    try:
+        first = snap(plan.run())
+        same(first, eager)
+        same(snap(plan.run()), eager)
+        vp1 = (vp0 * 1.02).astype(np.float32)
+        vs1, rho1 = (vp1 / 1.8).astype(np.float32), synth.gardner(vp1)
+        plan.set_model(vp1, vs1, rho1)
+        same(snap(plan.run()), fwi.elastic_batch_gradient(vp1, vs1, rho1, sv, scale=True))
+        print(f"  plan == eager (fp16 {fp16}, misfit type {misfit_type}), misfit {first[0]:.6e}")
+    finally:
+        plan.close()
