@@ -212,31 +212,52 @@ def misfit_residuals(misfit_type, dmod, dobs, gmap, weight=None, scaling=1):
 
 
 def elastic_misfit_residuals(misfit_type, dmod, dobs, gmap, weights=None):
-    """FwiElastic2D::computeMisfit and computeResiduals on the device for one
-    shot: modelled and observed records dmod / dobs [3, nrec, nt] (P, Ux,
-    Uz), the Ux receiver map gmap [nrec, 2] (traces with a negative entry
-    count in no misfit), weights None or three optional [nrec, nt].  Returns
-    (misfit, residuals [3, nrec, nt])."""
+    """FwiElastic2D::computeMisfit and computeResiduals on the device, shot
+    by shot as the reference runs them (percentile clips per shot): modelled
+    and observed records dmod / dobs [3, nrec, nt] (P, Ux, Uz), the Ux
+    receiver map gmap [nrec, 2] (traces with a negative entry count in no
+    misfit), weights None or three optional [nrec, nt].  Lists of these (one
+    entry per shot, nrec may differ) run as one batch.  Returns (misfit,
+    residuals [3, nrec, nt]) or, for lists, (summed misfit, list of
+    residuals)."""
     if int(misfit_type) not in _ELASTIC_MISFITS:
         raise R.RsgError(f"elastic misfit type {misfit_type} is not one of rs_fwimisfit's 0..4")
+    single = not isinstance(dmod, (list, tuple))
+    if single:
+        dmod, dobs, gmap, weights = [dmod], [dobs], [gmap], [weights]
+    elif weights is None:
+        weights = [None] * len(dmod)
     dev = _dev()
-    mod = torch.as_tensor(np.ascontiguousarray(dmod, np.float32)).to(dev)
-    rec = torch.as_tensor(np.ascontiguousarray(dobs, np.float32)).to(dev)
-    if mod.dim() != 3 or mod.shape[0] != 3 or rec.shape != mod.shape:
-        raise R.RsgError("elastic_misfit_residuals: records must be [3, nrec, nt]")
-    mp = torch.as_tensor(np.ascontiguousarray(gmap, np.int32).reshape(-1, 2)).to(dev)
-    res = torch.zeros_like(mod)
-    wts = [None if weights is None or weights[c] is None else
-           torch.as_tensor(np.ascontiguousarray(weights[c], np.float32)).to(dev) for c in range(3)]
-    r = R.E2DRecords()
-    r.nrec, r.map = mod.shape[1], mp.data_ptr()
-    for c in range(3):
-        r.mod[c], r.obs[c], r.res[c] = mod[c].data_ptr(), rec[c].data_ptr(), res[c].data_ptr()
-        r.weight[c] = None if wts[c] is None else wts[c].data_ptr()
+    ns = len(dmod)
+    recs = (R.E2DRecords * ns)()
+    keep, res_all = [], []
+    nt = None
+    for k in range(ns):
+        mod = torch.as_tensor(np.ascontiguousarray(dmod[k], np.float32)).to(dev)
+        rec = torch.as_tensor(np.ascontiguousarray(dobs[k], np.float32)).to(dev)
+        if mod.dim() != 3 or mod.shape[0] != 3 or rec.shape != mod.shape:
+            raise R.RsgError("elastic_misfit_residuals: records must be [3, nrec, nt]")
+        if nt is None:
+            nt = mod.shape[2]
+        elif mod.shape[2] != nt:
+            raise R.RsgError("elastic_misfit_residuals: every shot needs the same nt")
+        mp = torch.as_tensor(np.ascontiguousarray(gmap[k], np.int32).reshape(-1, 2)).to(dev)
+        res = torch.zeros_like(mod)
+        w = weights[k]
+        wts = [None if w is None or w[c] is None else
+               torch.as_tensor(np.ascontiguousarray(w[c], np.float32)).to(dev) for c in range(3)]
+        keep += [mod, rec, mp, res] + [t for t in wts if t is not None]
+        r = recs[k]
+        r.nrec, r.map = mod.shape[1], mp.data_ptr()
+        for c in range(3):
+            r.mod[c], r.obs[c], r.res[c] = mod[c].data_ptr(), rec[c].data_ptr(), res[c].data_ptr()
+            r.weight[c] = None if wts[c] is None else wts[c].data_ptr()
+        res_all.append(res)
     mis = C.c_double(0.0)
-    R.check(R.lib().rsg_misfit_residuals_e2d(int(misfit_type), C.byref(r), 1, mod.shape[2], C.byref(mis),
-                                             _stream()), "rsg_misfit_residuals_e2d")
-    return mis.value, res.cpu().numpy()
+    R.check(R.lib().rsg_misfit_residuals_e2d(int(misfit_type), recs, ns, nt, C.byref(mis), _stream()),
+            "rsg_misfit_residuals_e2d")
+    out = [r.cpu().numpy() for r in res_all]
+    return mis.value, (out[0] if single else out)
 
 
 class Shot:

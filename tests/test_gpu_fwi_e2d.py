@@ -318,3 +318,40 @@ def test_plan_replays_equal_eager(golden, fp16, misfit_type):
         print(f"  plan == eager (fp16 {fp16}, misfit type {misfit_type}), misfit {first[0]:.6e}")
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("misfit_type", range(5))
+def test_e2d_misfit_residuals_several_shots(golden, misfit_type):
+    """rsg_misfit_residuals_e2d on three shots at once with 60, 37 and 51
+    traces (the reference's records of two fixtures, cut and reordered):
+    every shot's residuals equal, bit for bit, the restated computeResiduals
+    of that shot alone (the percentile clips are per shot), and the misfit is
+    the sum of the shots' misfits."""
+    import oracle
+    from rockseis_amd import fwi
+    ga, gb = golden("fwi_e2d_misfit_case0"), golden("fwi_e2d_misfit_case1")
+    parts = [(ga, slice(0, 60)), (gb, slice(5, 42)), (ga, slice(9, 60))]
+    mods, obss, maps, wts = [], [], [], []
+    for g, sl in parts:
+        mods.append(np.ascontiguousarray(g["dmod"][:, sl]))
+        obss.append(np.ascontiguousarray(g["dobs"][:, sl]))
+        mp = np.zeros((mods[-1].shape[1], 2), np.int32)
+        mp[::7, 0] = -1  # a few traces outside the model: they count in no misfit
+        maps.append(mp)
+        wts.append([g[f"wei_{c}"][sl] if f"wei_{c}" in g.files else None for c in "pxz"])
+    mis, res = fwi.elastic_misfit_residuals(misfit_type, mods, obss, maps, wts)
+    total = 0.0
+    nt = mods[0].shape[2]
+    for k in range(3):
+        m, o = mods[k], obss[k]
+        md = [np.ascontiguousarray(a).reshape(-1) for a in (m[0], o[0], m[1], o[1], m[2], o[2])]
+        wk = [None if w is None else np.ascontiguousarray(w, np.float32) for w in wts[k]]
+        wp = [None if w is None else w.ctypes.data for w in wk]
+        want = np.zeros_like(m)
+        oracle.fn("residual_e")(misfit_type, *md, *wp, want[0].reshape(-1), want[1].reshape(-1),
+                                want[2].reshape(-1), m.shape[1], nt)
+        assert np.array_equal(res[k], want), k
+        valid = (maps[k][:, 0] >= 0).astype(np.int32)
+        total += float(oracle.fn("misfit_e")(misfit_type, *md, *wp, valid, m.shape[1], nt))
+    print(f"  misfit {mis:.9e} vs sum of the shots {total:.9e}")
+    assert abs(mis - total) <= 1e-4 * abs(total)
