@@ -47,6 +47,11 @@ struct ShotDev {
     const float *gshift[3], *dobs[3];
     float *dmod[3], *dres[3];
     float *lam, *mu, *rho;
+    // residual injection per touched cell: cells rcell[c][j], their
+    // contributions rent[c][rstart[c][j] .. rstart[c][j+1]) = trace*4 + corner
+    // (i1*2 + i2), in trace order
+    int rcn[3];
+    const int *rcell[3], *rstart[3], *rent[3];
 };
 
 constexpr int kIoThreads = 64;
@@ -79,15 +84,34 @@ __global__ void e2d_record_kernel(const ShotDev *__restrict__ sd, int it, int nt
 }
 
 // The three residuals into the adjoint wavefield (insertSource of dataresP,
-// dataresUx, dataresUz, fwi.cpp:4069-4071).
+// dataresUx, dataresUz, fwi.cpp:4069-4071): P into Sxx and Szz with -dt, Ux
+// into Vx with dt*Rx, Uz into Vz with dt*Rz, bilinear over each trace's 2x2
+// cells.  Neighbouring receivers share corner cells (the staggered Ux / Uz
+// positions fall between grid nodes), so one thread per touched cell
+// (blockIdx.z = component) adds the cell's contributions in trace order, the
+// reference's order, and the result does not depend on scheduling.
 __global__ void e2d_residual_kernel(const ShotDev *__restrict__ sd, int it, int nt, float dt) {
     const ShotDev &s = sd[blockIdx.y];
-    const int i = blockIdx.x * kIoThreads + threadIdx.x;
-    if (i >= s.nrec) return;
-    inject2d_trace(i, s.bSxx, nullptr, -dt, s.nxp, s.lpml, s.gmap[0], s.gshift[0], s.dres[0], nt, it);
-    inject2d_trace(i, s.bSzz, nullptr, -dt, s.nxp, s.lpml, s.gmap[0], s.gshift[0], s.dres[0], nt, it);
-    inject2d_trace(i, s.bVx, s.Rx, dt, s.nxp, s.lpml, s.gmap[1], s.gshift[1], s.dres[1], nt, it);
-    inject2d_trace(i, s.bVz, s.Rz, dt, s.nxp, s.lpml, s.gmap[2], s.gshift[2], s.dres[2], nt, it);
+    const int c = blockIdx.z;
+    const int j = blockIdx.x * kIoThreads + threadIdx.x;
+    if (j >= s.rcn[c]) return;
+    const size_t p = (size_t)s.rcell[c][j];
+    const float *shift = s.gshift[c], *res = s.dres[c];
+    float *F = c == 0 ? s.bSxx : (c == 1 ? s.bVx : s.bVz);
+    const float *Mod = c == 0 ? nullptr : (c == 1 ? s.Rx : s.Rz);
+    const float scale = c == 0 ? -dt : dt;
+    const float m = Mod ? scale * Mod[p] : scale;
+    float a = F[p], b = c == 0 ? s.bSzz[p] : 0.f;
+    for (int q = s.rstart[c][j]; q < s.rstart[c][j + 1]; q++) {
+        const int e = s.rent[c][q], i = e >> 2, i1 = (e >> 1) & 1, i2 = e & 1;
+        const float sx = shift[2 * i], sz = shift[2 * i + 1];
+        const float xt = i2 ? sx : 1 - sx, yt = i1 ? sz : 1 - sz;
+        const float v = m * res[(size_t)i * nt + it] * xt * yt;
+        a += v;
+        b += v;
+    }
+    F[p] = a;
+    if (c == 0) s.bSzz[p] = b;
 }
 
 // Wavelet gradient of crossCorr: the adjoint field of the source type
@@ -221,7 +245,7 @@ struct E2DFwi {
     const rsg_fwi_e2d_shot *sh = nullptr;
     int ns = 0, nt = 0, lpml = 0, order = 0;
     float dt = 0.f;
-    size_t maxsrc = 0, maxrec = 0;
+    size_t maxsrc = 0, maxrec = 0, maxrcell = 0;
     int maxnx = 0, maxnz = 0;
     std::vector<size_t> off, segs, img_off;
     size_t total = 0, cstride = 0;
@@ -239,7 +263,7 @@ struct E2DFwi {
     double *dmis = nullptr;
     std::vector<E2DRecs> recs;
     PlanGuard pf, pb;
-    dim3 iob_s, iob_r, xcg;
+    dim3 iob_s, iob_r, iob_c, xcg;
 
     int init(const rsg_fwi_e2d *f_, const rsg_fwi_e2d_shot *sh_, int ns_, hipStream_t st) {
         f = f_;
@@ -383,6 +407,63 @@ struct E2DFwi {
             d.mu = d.lam + ni;
             d.rho = d.mu + ni;
         }
+        // residual injection tables: per shot and component the touched cells,
+        // each with its (trace, corner) contributions in trace order
+        {
+            std::vector<int> tab;
+            std::vector<size_t> base(ns * 3 * 3);  // offsets of cells, starts, entries
+            std::vector<int> mp;
+            for (int k = 0; k < ns; k++) {
+                const int nrec = sh[k].obs[0].ntrace, nxp = sh[k].g.nx;
+                for (int c = 0; c < 3; c++) {
+                    mp.resize(2 * (size_t)nrec);
+                    if (nrec)
+                        RSG_HIP(hipMemcpy(mp.data(), sh[k].obs[c].map, sizeof(int) * 2 * nrec,
+                                          hipMemcpyDeviceToHost));
+                    std::vector<std::pair<int, int>> ent;  // (cell, trace*4 + corner)
+                    ent.reserve(4 * (size_t)nrec);
+                    for (int i = 0; i < nrec; i++) {
+                        const int mx = mp[2 * i], mz = mp[2 * i + 1];
+                        if (mx < 0 || mz < 0) continue;
+                        for (int i1 = 0; i1 < 2; i1++)
+                            for (int i2 = 0; i2 < 2; i2++)
+                                ent.emplace_back((lpml + mz + i1) * nxp + (lpml + mx + i2), i * 4 + i1 * 2 + i2);
+                    }
+                    std::stable_sort(ent.begin(), ent.end(),
+                                     [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+                                         return x.first < y.first;
+                                     });
+                    std::vector<int> cells, starts, es;
+                    for (size_t q = 0; q < ent.size(); q++) {
+                        if (q == 0 || ent[q].first != ent[q - 1].first) {
+                            cells.push_back(ent[q].first);
+                            starts.push_back((int)q);
+                        }
+                        es.push_back(ent[q].second);
+                    }
+                    starts.push_back((int)ent.size());
+                    hd[k].rcn[c] = (int)cells.size();
+                    maxrcell = std::max(maxrcell, cells.size());
+                    const size_t b = 9 * (size_t)k + 3 * c;
+                    base[b] = tab.size();
+                    tab.insert(tab.end(), cells.begin(), cells.end());
+                    base[b + 1] = tab.size();
+                    tab.insert(tab.end(), starts.begin(), starts.end());
+                    base[b + 2] = tab.size();
+                    tab.insert(tab.end(), es.begin(), es.end());
+                }
+            }
+            int *dtab = nullptr;
+            if (int rc = dm.alloc(&dtab, tab.size())) return rc;
+            RSG_HIP(hipMemcpy(dtab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+            for (int k = 0; k < ns; k++)
+                for (int c = 0; c < 3; c++) {
+                    const size_t b = 9 * (size_t)k + 3 * c;
+                    hd[k].rcell[c] = dtab + base[b];
+                    hd[k].rstart[c] = dtab + base[b + 1];
+                    hd[k].rent[c] = dtab + base[b + 2];
+                }
+        }
         // the three records of every shot for the misfit and residuals
         recs.assign(ns, E2DRecs{});
         for (int k = 0; k < ns; k++) {
@@ -420,6 +501,7 @@ struct E2DFwi {
         RSG_HIP(hipStreamSynchronize(st));  // the host tables above go out of scope
         iob_s = dim3((unsigned)((maxsrc + kIoThreads - 1) / kIoThreads), ns);
         iob_r = dim3((unsigned)((maxrec + kIoThreads - 1) / kIoThreads), ns);
+        iob_c = dim3((unsigned)std::max<size_t>((maxrcell + kIoThreads - 1) / kIoThreads, 1), ns, 3);
         xcg = dim3((maxnx + 63) / 64, (maxnz + 3) / 4, ns);
         return RSG_OK;
     }
@@ -440,7 +522,7 @@ struct E2DFwi {
 
     int reverse_tail(int it, hipStream_t st) {
         if (maxrec) {
-            e2d_residual_kernel<<<iob_r, kIoThreads, 0, st>>>(dsd, it, nt, dt);
+            e2d_residual_kernel<<<iob_c, kIoThreads, 0, st>>>(dsd, it, nt, dt);
             RSG_LAUNCH_CHECK("e2d_residual_kernel");
         }
         e2d_crosscorr_kernel<<<xcg, dim3(64, 4), 0, st>>>(dsd, sh[0].g.dx, sh[0].g.dz);
