@@ -229,3 +229,26 @@ def test_no_kernel_uses_scratch():
     assert any("e2d_stress_ragged_kernel" in k for k in sizes)
     bad = {k: v for k, v in sizes.items() if v}
     assert not bad, bad
+
+
+def test_fwi_e2d_plan_rejects_bad_arguments():
+    """The plan entry points validate their arguments before any device
+    work: null plan pointer, null settings, no shots, a bad misfit type;
+    destroy takes null."""
+    L = R.lib()
+    h = C.c_void_p()
+    assert L.rsg_fwi_e2d_plan_create(None, None, 1, C.byref(h)) != 0
+    assert L.rsg_fwi_e2d_plan_create(None, None, 1, None) != 0
+    f = R.FwiE2D()
+    shots = (R.FwiE2DShot * 1)()
+    assert L.rsg_fwi_e2d_plan_create(C.byref(f), shots, 0, C.byref(h)) != 0
+    assert L.rsg_fwi_e2d_plan_create(C.byref(f), shots, 1, C.byref(h)) != 0  # null tables / images
+    assert b"null" in L.rsg_last_error()
+    f.tab, f.nx, f.nz, f.misfit_type = 1, 10, 10, 7
+    for c in range(3):
+        f.grad[c] = 1
+    assert L.rsg_fwi_e2d_plan_create(C.byref(f), shots, 1, C.byref(h)) != 0
+    assert b"misfit type" in L.rsg_last_error()
+    assert h.value is None
+    assert L.rsg_fwi_e2d_plan_run(None, None, None) != 0
+    assert L.rsg_fwi_e2d_plan_destroy(None) == 0
