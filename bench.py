@@ -484,7 +484,8 @@ def e2dfwi_leg(nshot, n_plain=4, eager=False):
     plan = fwi.ElasticFwiPlan(vp, vs, rho, sv, scale=False)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
-    _progress(f"e2dfwi: plan built {t_build:.2f} s")
+    rz_ragged = plan.tuned["rz2d_ragged"]
+    _progress(f"e2dfwi: plan built {t_build:.2f} s (autotuned rows per thread {rz_ragged})")
     t0 = time.perf_counter()
     mis_p = plan.run()[0]
     torch.cuda.synchronize()
@@ -506,7 +507,8 @@ def e2dfwi_leg(nshot, n_plain=4, eager=False):
                         "5 m, order 8, nt 4000, FP16 Revolve (50)",
             "padded_cells_per_step": cells, "batched_s": round(tb, 2),
             "batched": "persistent plan: the Revolve sweep as two hipGraphs replayed (rsg_fwi_e2d_plan_run)",
-            "plan_build_s": round(t_build, 2), "batched_eager_s": None if te is None else round(te, 2),
+            "plan_build_s": round(t_build, 2), "rows_per_thread": rz_ragged,
+            "batched_eager_s": None if te is None else round(te, 2),
             "plan_misfit_equals_eager": None if mis is None else bool(abs(mis_p - mis) <= 1e-9 * abs(mis)),
             "batched_shots_per_s": round(nshot / tb, 3),
             "cell_updates_per_s_G": round(cells * steps / tb / 1e9, 2),

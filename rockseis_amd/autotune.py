@@ -174,3 +174,60 @@ def tune_waves(waves, order, nshot=1, reps=20, force=False):
     if waves.Pml.tab is None:
         waves.Pml.computeABC()
     return tune(waves.kind, g, waves.Pml.tab.data_ptr(), nshot, reps, force)
+
+
+def tune_ragged(kind, grids, tab_ptr, reps=5, force=False):
+    """Pick ``rz2d_ragged`` (rows per thread of the ragged-batch launches)
+    for shots on the windows ``grids`` (``_lib.Grid`` each, one stencil
+    order; PML tables at device pointer `tab_ptr`): every candidate times the
+    half steps of one ragged batch (elastic: forward and adjoint-state
+    velocity and stress) on zeroed scratch states of those windows, three
+    interleaved rounds, fastest kept.  Returns ``{"rz2d_ragged": value,
+    "ms": {value: ms}}``."""
+    if kind not in (R.KIND_A2D, R.KIND_E2D):
+        raise R.RsgError("tune_ragged: 2D kinds only")
+    key = f"ragged{kind}:" + ",".join(f"{g.nx}x{g.nz}" for g in grids) + f":lpml{grids[0].lpml}:o{grids[0].order}"
+    if not _cache:
+        _load_file()
+    if key in _cache and not force:
+        R.set_tuning("rz2d_ragged", _cache[key]["rz2d_ragged"])
+        return _cache[key]
+    scratch = [_Scratch(kind, g, tab_ptr, 1) for g in grids]
+    n = len(grids)
+    garr = (R.Grid * n)(*grids)
+    sarr = (R.State * n)(*[sc.state for sc in scratch])
+    plan = C.c_void_p()
+    lib = R.lib()
+    R.check(lib.rsg_ragged2d_create(kind, n, garr, sarr, C.byref(plan)), "rsg_ragged2d_create")
+    st = torch.cuda.current_stream()
+    stream = C.c_void_p(st.cuda_stream)
+    whichs = (0, 1, 2, 3) if kind == R.KIND_E2D else (0, 1)
+
+    def one():
+        for w in whichs:
+            R.check(lib.rsg_ragged2d_step(plan, w, stream), "rsg_ragged2d_step")
+
+    times, old = {}, R.get_tuning("rz2d_ragged")
+    try:
+        for _ in range(3):
+            for v in CANDIDATES["rz2d"]:
+                R.set_tuning("rz2d_ragged", v)
+                one()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(st)
+                for _ in range(reps):
+                    one()
+                e1.record(st)
+                e1.synchronize()
+                t = e0.elapsed_time(e1) / reps
+                times[v] = min(times.get(v, t), t)
+    except Exception:
+        R.set_tuning("rz2d_ragged", old)
+        raise
+    finally:
+        lib.rsg_ragged2d_destroy(plan)
+    best = min(times, key=times.get)
+    R.set_tuning("rz2d_ragged", best)
+    _cache[key] = {"rz2d_ragged": best, "ms": {str(k): round(t, 5) for k, t in times.items()}}
+    _save_file()
+    return _cache[key]

@@ -554,6 +554,12 @@ class _E2DBatch:
         f.misfit_type = survey.misfit_type
         self.f = f
 
+    def tune(self, reps=5):
+        """Pick the ragged launches' rows per thread for these windows
+        (autotune.tune_ragged); call before a plan is built."""
+        from . import autotune
+        return autotune.tune_ragged(R.KIND_E2D, [s.g for s in self.shots], self.f.tab, reps)
+
     def set_model(self, vp, vs, rho):
         """A new model into the same device arrays (restaggered in place)."""
         vp, vs, rho = (np.ascontiguousarray(a, np.float32) for a in (vp, vs, rho))
@@ -587,10 +593,14 @@ class ElasticFwiPlan:
     Revolve sweep captured as two hipGraphs; run() replays them and returns
     (misfit, g0, g1, g2, per-shot outputs) like elastic_batch_gradient;
     set_model() changes the model between runs (an inversion's iterations).
-    The gradient tensors are zeroed by run() and owned by the plan."""
+    The gradient tensors are zeroed by run() and owned by the plan.  With
+    tune (default) the ragged launches' rows per thread are first picked by
+    the autotuner for these windows (autotune.tune_ragged)."""
 
-    def __init__(self, vp, vs, rho, survey: ElasticSurvey, scale=True, wavgrad=False):
+    def __init__(self, vp, vs, rho, survey: ElasticSurvey, scale=True, wavgrad=False, tune=True):
         self._b = _E2DBatch(vp, vs, rho, survey, scale, wavgrad)
+        # the launch shape is fixed in the captured graphs: choose it first
+        self.tuned = self._b.tune() if tune else None
         torch.cuda.current_stream().synchronize()  # inputs in place before the capture
         h = C.c_void_p()
         R.check(R.lib().rsg_fwi_e2d_plan_create(C.byref(self._b.f), self._b.shots, len(survey.shots), C.byref(h)),

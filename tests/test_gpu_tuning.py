@@ -185,3 +185,24 @@ def test_fused_io_matches_separate_kernels(graph, fs):
     finally:
         R.set_tuning("fuse_io", old[0])
         R.set_tuning("rz2d", old[1])
+
+
+def test_tune_ragged_picks_and_sets_a_candidate():
+    """The ragged autotuner times the elastic half steps of three windows
+    of different widths for every rows-per-thread candidate, sets the
+    fastest as rz2d_ragged and caches it."""
+    from rockseis_amd import _lib as R, autotune, core
+    old = R.get_tuning("rz2d_ragged")
+    pml = core.Pml(10, 5e-4)
+    pml.setSmax(core.SMAX)
+    pml.computeABC()
+    grids = [R.Grid(nx=w + 20, ny=1, nz=80, lpml=10, order=8, fs=0, dx=5.0, dy=1.0, dz=5.0, dt=5e-4)
+             for w in (60, 130, 95)]
+    try:
+        res = autotune.tune_ragged(R.KIND_E2D, grids, pml.tab.data_ptr(), reps=2, force=True)
+        assert res["rz2d_ragged"] in autotune.CANDIDATES["rz2d"]
+        assert R.get_tuning("rz2d_ragged") == res["rz2d_ragged"]
+        assert len(res["ms"]) == len(autotune.CANDIDATES["rz2d"]) and all(t > 0 for t in res["ms"].values())
+        assert autotune.tune_ragged(R.KIND_E2D, grids, pml.tab.data_ptr()) is res
+    finally:
+        R.set_tuning("rz2d_ragged", old)
