@@ -355,7 +355,8 @@ int rsg_fwi_e2d_gradient_batch(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *sho
  * may change their contents (a new model restaggered in place, new
  * observed data), not the arrays themselves.  Gradients are added into
  * f->grad on every run, so zero them in between for a fresh gradient.
- * f->misfit is ignored (run returns it). */
+ * f->misfit is ignored (run returns it).  One run at a time per plan (its
+ * states and graphs are shared by all runs); separate plans are independent. */
 typedef struct rsg_fwi_e2d_plan rsg_fwi_e2d_plan;
 int rsg_fwi_e2d_plan_create(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *shots, int nshot,
                             rsg_fwi_e2d_plan **plan);
