@@ -267,7 +267,9 @@ int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, void *stream);
  * mod/rec/wei/res device arrays [nshot][ntr][nt], map [nshot][ntr][dim]
  * receiver map (traces with a negative entry count in no misfit), scaling
  * 1 (P) or -1 (VX/VZ data); *misfit (host, optional) = the misfit summed in
- * double over the shots. */
+ * double over the shots.  dim 2 follows FwiAcoustic2D, whose types 1-3 end
+ * with a 97th-percentile clip per shot; dim 3 follows FwiAcoustic3D, which
+ * has none. */
 int rsg_misfit_residuals(int type, const float *mod, const float *rec, const float *wei, float *res, const int *map,
                          int dim, int ntr, int nt, int nshot, int scaling, double *misfit, void *stream);
 /* nshot shots on one grid in one set of launches per step (the reference

@@ -153,7 +153,7 @@ def _declare(L):
         f.argtypes = [C.c_int, rp, rp, C.c_void_p, _i32p, C.c_int, C.c_int]
         f.restype = ct
         f = getattr(L, "orc_residual_a" + suf)
-        f.argtypes = [C.c_int, rp, rp, C.c_void_p, rp, C.c_int, C.c_int, C.c_int]
+        f.argtypes = [C.c_int, rp, rp, C.c_void_p, rp, C.c_int, C.c_int, C.c_int, C.c_int]
         f = getattr(L, "orc_misfit_e" + suf)
         f.argtypes = [C.c_int] + [rp] * 6 + [C.c_void_p] * 3 + [_i32p, C.c_int, C.c_int]
         f.restype = ct
@@ -218,7 +218,7 @@ def _declare_ref(R):
                               + [_f32p, _f32p, C.c_int] + [_f32p] * 8 + [C.c_int] + [C.c_void_p] * 3)
     R.ref_fwi_e2d.restype = C.c_float
     R.ref_fwi_a3d.argtypes = ([_f32p, _f32p] + [C.c_int] * 4 + [C.c_float] * 4 + [C.c_int] * 4
-                              + [_f32p, _f32p, C.c_int, _f32p, _f32p] + [_f32p] * 5)
+                              + [_f32p, _f32p, C.c_int, _f32p, _f32p] + [_f32p] * 5 + [C.c_int, C.c_void_p])
     R.ref_fwi_a3d.restype = C.c_float
     R.ref_interp.argtypes = [C.c_int, _f32p, C.c_int, C.c_int, C.c_float, C.c_float, _f32p, C.c_int, C.c_float,
                              C.c_float]

@@ -129,7 +129,7 @@ def _misfit_res(dmod, dobs, gmap, weight, real):
     wp = None if wei is None else wei.ctypes.data
     dres = np.zeros_like(dmod)
     mis = fn("misfit_a", real)(0, dmod.reshape(-1), dobs.reshape(-1), wp, valid, dmod.shape[0], nt)
-    fn("residual_a", real)(0, dmod.reshape(-1), dobs.reshape(-1), wp, dres.reshape(-1), dmod.shape[0], nt, 1)
+    fn("residual_a", real)(0, dmod.reshape(-1), dobs.reshape(-1), wp, dres.reshape(-1), dmod.shape[0], nt, 1, 1)
     return float(mis), dres
 
 

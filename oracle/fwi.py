@@ -102,7 +102,7 @@ def fwi_a2d(vp, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, scoords, wav, gcoo
             misfit = fn("misfit_a", real)(misfit_type, dmod.reshape(-1), dobs.reshape(-1), wp, valid,
                                           dobs.shape[0], nt)
             fn("residual_a", real)(misfit_type, dmod.reshape(-1), dobs.reshape(-1), wp, dres.reshape(-1),
-                                   dobs.shape[0], nt, 1)
+                                   dobs.shape[0], nt, 1, 1)
             inject2d(bw.P, bw.L, lpml, dt, gmap, gshift, dres, capo, real)
             corr()
             record2d(bw.P, lpml, smap, sshift, wgrad, capo, real)
@@ -388,12 +388,17 @@ def ref_fwi_e2d(vp, vs, rho, lpml, dx, dz, dt, nt, order, fs, ncheck, srcfield, 
     return dict(misfit=np.float32(m), dmod=dmod, dres=dres, wavgrad=wg, **out)
 
 
-def fwi_a3d(vp, rho, lpml, d3, dt, nt, order, fs, ncheck, scoords, wav, gcoords, dobs, real=np.float32):
+def fwi_a3d(vp, rho, lpml, d3, dt, nt, order, fs, ncheck, scoords, wav, gcoords, dobs, real=np.float32,
+            misfit_type=0, weight=None):
     """FwiAcoustic3D::run_optimal (fwi.cpp:1991-2171): PRESSURE source and
-    data, DIFFERENCE misfit over traces with all three map indices valid
-    (fwi.cpp:1498-1510), residuals (fwi.cpp:1685-1695), adjoint with the
-    forward operator, crossCorr (fwi.cpp:1403-1466, quirks kept) and the
-    wavelet gradient recorded at the source from the adjoint pressure."""
+    data, misfit over traces with all three map indices valid
+    (computeMisfit, fwi.cpp:1468-1649) and residuals (computeResiduals,
+    fwi.cpp:1651-1857) of misfit_type 0 DIFFERENCE, 1 CORRELATION,
+    2 ADAPTIVE_GAUSS, 3 ADAPTIVE_LINEAR with optional data weights (the
+    same arithmetic as the 2D driver's, so the same C functions), adjoint
+    with the forward operator, crossCorr (fwi.cpp:1403-1466, quirks kept)
+    and the wavelet gradient recorded at the source from the adjoint
+    pressure."""
     nz, ny, nx = vp.shape
     smap, sshift = makemap3d(scoords, (nx, ny, nz), d3, "P")
     gmap, gshift = makemap3d(gcoords, (nx, ny, nz), d3, "P")
@@ -410,6 +415,7 @@ def fwi_a3d(vp, rho, lpml, d3, dt, nt, order, fs, ncheck, scoords, wav, gcoords,
     vpgrad = np.zeros((nz, ny, nx), real)
     rhograd = np.zeros((nz, ny, nx), real)
     valid = ((gmap[:, 0] >= 0) & (gmap[:, 1] >= 0) & (gmap[:, 2] >= 0)).astype(np.int32)
+    wei = None if weight is None else np.ascontiguousarray(weight, real).reshape(gmap.shape[0], nt)
     sh32 = np.ascontiguousarray(sshift, real).reshape(-1)
     gh32 = np.ascontiguousarray(gshift, real).reshape(-1)
 
@@ -448,8 +454,11 @@ def fwi_a3d(vp, rho, lpml, d3, dt, nt, order, fs, ncheck, scoords, wav, gcoords,
                 fstep(it, not reverse)
         elif a == orv.FIRSTURN:
             fstep(capo, True)
-            misfit = fn("misfit_diff", real)(dmod.reshape(-1), dobs.reshape(-1), valid, dobs.shape[0], nt)
-            fn("residual_diff", real)(dmod.reshape(-1), dobs.reshape(-1), dres.reshape(-1), dobs.shape[0], nt, 1)
+            wp = None if wei is None else wei.ctypes.data
+            misfit = fn("misfit_a", real)(misfit_type, dmod.reshape(-1), dobs.reshape(-1), wp, valid,
+                                          dobs.shape[0], nt)
+            fn("residual_a", real)(misfit_type, dmod.reshape(-1), dobs.reshape(-1), wp, dres.reshape(-1),
+                                   dobs.shape[0], nt, 1, 0)  # the 3D class has no percentile clip
             inject(bw, gmap, gh32, dres, capo)
             corr()
             record(bw, smap, sh32, wgrad, capo)
@@ -469,7 +478,8 @@ def fwi_a3d(vp, rho, lpml, d3, dt, nt, order, fs, ncheck, scoords, wav, gcoords,
     return dict(misfit=misfit, vpgrad=vpgrad, rhograd=rhograd, dmod=dmod, dres=dres, wavgrad=wgrad)
 
 
-def ref_fwi_a3d(vp, rho, lpml, d3, dt, nt, order, fs, ncheck, scoords, wav, gcoords, dobs):
+def ref_fwi_a3d(vp, rho, lpml, d3, dt, nt, order, fs, ncheck, scoords, wav, gcoords, dobs, misfit_type=0,
+                weight=None):
     nz, ny, nx = vp.shape
     sc = np.ascontiguousarray(scoords, np.float32).reshape(-1, 3)
     gc = np.ascontiguousarray(gcoords, np.float32).reshape(-1, 3)
@@ -480,7 +490,9 @@ def ref_fwi_a3d(vp, rho, lpml, d3, dt, nt, order, fs, ncheck, scoords, wav, gcoo
     dres = np.zeros((nrec, nt), np.float32)
     wg = np.zeros((1, nt), np.float32)
     f32 = lambda a: np.ascontiguousarray(a, np.float32).ravel()  # noqa: E731
+    wts = None if weight is None else f32(weight)  # bound here: the pointer must outlive the call
     m = ref().ref_fwi_a3d(f32(vp), f32(rho), nx, ny, nz, lpml, *d3, dt, nt, order, int(fs), ncheck, sc.ravel(),
                           f32(wav), nrec, gc.ravel(), f32(dobs), vg.reshape(-1), rg.reshape(-1), dmod.reshape(-1),
-                          dres.reshape(-1), wg.reshape(-1))
+                          dres.reshape(-1), wg.reshape(-1), int(misfit_type),
+                          None if wts is None else wts.ctypes.data)
     return dict(misfit=np.float32(m), vpgrad=vg, rhograd=rg, dmod=dmod, dres=dres, wavgrad=wg)

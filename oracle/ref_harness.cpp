@@ -452,13 +452,14 @@ float ref_fwi_e2d(const float *vp, const float *vs, const float *rho, int nx, in
 }
 
 // FwiAcoustic3D::run_optimal with in-core Revolve (ncheck snapshots), a
-// PRESSURE source and PRESSURE data; coords are (x,y,z) triples.  Returns the
+// PRESSURE source and PRESSURE data; coords are (x,y,z) triples; misfit_type
+// as rs_fwimisfit, weight: optional data weights [nrec][nt].  Returns the
 // misfit; outputs the Vp and rho gradients, modelled data, residuals and the
 // wavelet gradient.
 float ref_fwi_a3d(const float *vp, const float *rho, int nx, int ny, int nz, int lpml, float dx, float dy,
                   float dz, float dt, int nt, int order, int fs, int ncheck, const float *scoords,
                   const float *wav, int nrec, const float *gcoords, const float *dobs, float *vpgrad,
-                  float *rhograd, float *dmod, float *dres, float *wgrad) {
+                  float *rhograd, float *dmod, float *dres, float *wgrad, int misfit_type, const float *weight) {
     auto model = std::make_shared<ModelAcoustic3D<float>>(nx, ny, nz, lpml, dx, dy, dz, 0.f, 0.f, 0.f,
                                                           fs != 0);
     model->createModel();
@@ -493,7 +494,8 @@ float ref_fwi_a3d(const float *vp, const float *rho, int nx, int ny, int nz, int
     fwi->setWavgrad(wg);
     fwi->setNcheck(ncheck);
     fwi->setIncore(true);
-    fwi->setMisfit_type(DIFFERENCE);
+    fwi->setMisfit_type((rs_fwimisfit)misfit_type);
+    if (weight) fwi->setDataweight(mk(nrec, gcoords, false, weight));
     fwi->run_optimal();
     std::memcpy(vpgrad, vg->getImagedata(), sizeof(float) * nm);
     std::memcpy(rhograd, rg->getImagedata(), sizeof(float) * nm);

@@ -1403,18 +1403,21 @@ static int FN(orc_cmp)(const void *a, const void *b)
 }
 
 /*
- * FwiAcoustic2D::computeResiduals (fwi.cpp:803-880): type 0 DIFFERENCE
+ * FwiAcoustic2D::computeResiduals (fwi.cpp:803-880) and
+ * FwiAcoustic3D::computeResiduals (fwi.cpp:1651-1853): type 0 DIFFERENCE
  * res = scaling*(mod-rec) (*= wei^2); type 1 CORRELATION
  * res = -scaling*(rec/(n1 n2) - mod/(n1 n1) * (sum(mod*rec)/(n1 n2)))
- * (*= wei^2) on every trace, then every |res| at or above the PCLIP = 97th
- * percentile (element (int)(97*nt*ntr/100) of the sorted magnitudes) is set
- * to zero.
+ * (*= wei^2) on every trace; types 2/3 the ADAPTIVE residuals; any other
+ * type the default branch, DIFFERENCE.  With clip (the 2D class; the 3D
+ * class has no such step) every |res| of types 1-3 at or above the PCLIP =
+ * 97th percentile (element (int)(97*nt*ntr/100) of the sorted magnitudes)
+ * is then set to zero.
  */
 void FN(orc_residual_a)(int type, const REAL *mod, const REAL *rec, const REAL *wei, REAL *res, int ntr, int nt,
-                        int scaling)
+                        int scaling, int clip)
 {
     const long n = (long)ntr * nt;
-    if (type == 0) {
+    if (type < 1 || type > 3) {
         for (long p = 0; p < n; p++) {
             res[p] = scaling * (mod[p] - rec[p]);
             if (wei) res[p] *= wei[p] * wei[p];
@@ -1448,6 +1451,7 @@ void FN(orc_residual_a)(int type, const REAL *mod, const REAL *rec, const REAL *
         }
     }
     }
+    if (!clip) return;
     REAL *f = (REAL *)malloc(sizeof(REAL) * (n > 0 ? n : 1));
     for (long p = 0; p < n; p++) f[p] = res[p] < 0 ? -res[p] : res[p];
     qsort(f, n, sizeof(REAL), FN(orc_cmp));

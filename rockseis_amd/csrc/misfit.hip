@@ -734,8 +734,9 @@ int acoustic_misfit_residuals(int type, const float *mod, const float *rec, cons
                                                      misfit);
         RSG_LAUNCH_CHECK("misfit_res_kernel");
     }
-    if (type == 0) return RSG_OK;
-    // 97th-percentile clip per shot: element (int)(97*nt*ntr/100) of the sorted |res|
+    // FwiAcoustic3D::computeResiduals (fwi.cpp:1651-1853) has no clip step
+    if (type == 0 || dim == 3) return RSG_OK;
+    // 97th-percentile clip per shot (FwiAcoustic2D): element (int)(97*nt*ntr/100) of the sorted |res|
     std::vector<SelSeg> segs(nshot);
     for (int k = 0; k < nshot; k++) segs[k] = SelSeg{res + k * nper, nper};
     std::vector<unsigned long long> kth(nshot, (unsigned long long)((97LL * nt * ntr) / 100));

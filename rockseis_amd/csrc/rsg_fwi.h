@@ -26,7 +26,8 @@ struct Revolve {
 // traces (shot k's traces follow shot k-1's) for type 0 DIFFERENCE,
 // 1 CORRELATION, 2 ADAPTIVE_GAUSS, 3 ADAPTIVE_LINEAR (4 ADAPTIVE_SINGLE is
 // the reference's default branch, DIFFERENCE), optional data weights; map
-// holds dim indices per trace (misfit.hip).
+// holds dim indices per trace; dim 2 clips types 1-3 at the 97th percentile
+// as FwiAcoustic2D does, dim 3 does not, as FwiAcoustic3D (misfit.hip).
 int acoustic_misfit_residuals(int type, const float *mod, const float *rec, const float *wei, float *res,
                               const int *map, int dim, int ntr, int nt, int nshot, int scaling, double *misfit,
                               hipStream_t st);

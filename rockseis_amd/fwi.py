@@ -191,7 +191,9 @@ def misfit_residuals(misfit_type, dmod, dobs, gmap, weight=None, scaling=1):
     """FwiAcoustic2D/3D::computeMisfit and computeResiduals on the device for
     modelled records dmod and observed dobs (ntrace, nt) or (nshot, ntrace,
     nt), receiver maps gmap (..., ntrace, 2 or 3; traces with a negative entry
-    count in no misfit), optional data weights.  Returns (misfit, residuals)."""
+    count in no misfit), optional data weights.  Maps with 2 indices follow
+    FwiAcoustic2D (types 1-3 clipped at the 97th percentile), with 3
+    FwiAcoustic3D (no clip).  Returns (misfit, residuals)."""
     dev = _dev()
     mod = torch.as_tensor(np.ascontiguousarray(dmod, np.float32)).to(dev)
     rec = torch.as_tensor(np.ascontiguousarray(dobs, np.float32)).to(dev)

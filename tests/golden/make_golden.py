@@ -315,6 +315,29 @@ def fwi_a3d_fixtures(R):
              vpgrad=r["vpgrad"], rhograd=r["rhograd"], dmod=r["dmod"], dres=r["dres"], wavgrad=r["wavgrad"])
 
 
+FWI_A3D_MISFIT_CASES = [  # order, fs, ncheck, misfit type (1 CORRELATION, 2/3 ADAPTIVE_GAUSS/LINEAR, 4 default), weights
+    (4, 0, 10, 1, False), (4, 0, 10, 1, True), (4, 0, 10, 2, False), (4, 0, 10, 3, True), (4, 0, 10, 4, True),
+]
+
+
+def fwi_a3d_misfit_fixtures(R):
+    """FwiAcoustic3D::run_optimal with the CORRELATION and ADAPTIVE misfits,
+    the default branch (type 4) and data weights (setDataweight): the 3D class
+    has no percentile clip of the residuals, unlike the 2D one."""
+    from oracle import fwi
+    g = np.load(os.path.join(HERE, "fwi_a3d_case0.npz"), allow_pickle=False)
+    nx, ny, nz, lpml, dx, dt, nt = g["grid"]
+    lpml, nt = int(lpml), int(nt)
+    rng = np.random.default_rng(4)
+    for k, (order, fs, ncheck, mt, usew) in enumerate(FWI_A3D_MISFIT_CASES):
+        w = (0.5 + rng.random(g["dobs"].shape)).astype(np.float32) if usew else None
+        r = fwi.ref_fwi_a3d(g["vp"], g["rho"], lpml, (dx, dx, dx), dt, nt, order, fs, ncheck, g["scoords"], g["wav"],
+                            g["gcoords"], g["dobs"], mt, w)
+        extra = {} if w is None else dict(weight=w)
+        save(f"fwi_a3d_misfit_case{k}", misfit=np.float32(r["misfit"]), vpgrad=r["vpgrad"], rhograd=r["rhograd"],
+             dmod=r["dmod"], dres=r["dres"], wavgrad=r["wavgrad"], **extra)
+
+
 INTERP_CASES = [  # ntr, n1, d1, o1, n2, d2, o2 (Interp::interp for every mode)
     (5, 300, 1e-3, 0.0, 700, 4e-4, 0.0), (3, 1000, 4.5e-4, 0.0, 150, 3e-3, 0.0),
     (4, 257, 2e-3, 0.01, 300, 1.7e-3, -0.003),
@@ -355,7 +378,7 @@ GROUPS = dict(der=der_fixture, pml=pml_fixture, stagger=stagger_fixture, makemap
               runs=run_fixtures, revolve=revolve_fixture, fwi=fwi_fixtures, fwi_full=fwi_full_fixtures,
               fwi_e2d=fwi_e2d_fixtures, fwi_a3d=fwi_a3d_fixtures,
               interp=interp_fixture, fwi_misfit=fwi_misfit_fixtures,
-              fwi_e2d_misfit=fwi_e2d_misfit_fixtures)
+              fwi_e2d_misfit=fwi_e2d_misfit_fixtures, fwi_a3d_misfit=fwi_a3d_misfit_fixtures)
 
 if __name__ == "__main__":
     # python -m tests.golden.make_golden [group ...]   (default: every group)

@@ -11,7 +11,7 @@ from tests.gpu_util import RTOL_L2, data3d, rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _driver(g, case, fp16=False):
+def _driver(g, case, fp16=False, misfit_type=0, weight=None):
     from rockseis_amd import core, fwi
     order, fs, ncheck = FWI_A3D_CASES[case]
     nx, ny, nz, lpml, dx, dt, nt = g["grid"]
@@ -34,6 +34,9 @@ def _driver(g, case, fp16=False):
     f.setWavgrad(wgr)
     f.setNcheck(ncheck)
     f.setFp16Storage(fp16)
+    f.setMisfit_type(misfit_type)
+    if weight is not None:
+        f.setDataweight(data3d(g["gcoords"], nt, dt, "P", weight))
     f.run_optimal()
     return dict(misfit64=f.misfit64, vpgrad=vg.imagedata, rhograd=rg.imagedata, dmod=mod.data, dres=res.data,
                 wavgrad=wgr.data)
@@ -101,3 +104,25 @@ def test_offgrid_receivers_reproducible_and_vs_oracle(golden):
         e = rel_l2(r1[k], want[k])
         print(f"  off-grid receivers: {k} rel L2 {e:.2e} vs the oracle")
         assert e <= (1e-4 if k == "wavgrad" else RTOL_L2), k
+
+
+@pytest.mark.parametrize("case", range(5))
+def test_misfit_types_vs_reference(golden, case):
+    """CORRELATION, ADAPTIVE_GAUSS/LINEAR, the default branch (type 4) and data
+    weights against the reference's FwiAcoustic3D::run_optimal: residuals
+    without the 2D class's percentile clip."""
+    from tests.golden.make_golden import FWI_A3D_MISFIT_CASES
+    g = golden("fwi_a3d_case0")
+    f = golden(f"fwi_a3d_misfit_case{case}")
+    order, fs, ncheck, mt, usew = FWI_A3D_MISFIT_CASES[case]
+    print(f"fwi a3d misfit case {case}: type {mt}, weights {usew}", flush=True)
+    r = _driver(g, 0, misfit_type=mt, weight=f["weight"] if usew else None)
+    for k in ("vpgrad", "rhograd", "dres", "wavgrad"):
+        e = rel_l2(r[k], f[k])
+        print(f"  {k}: rel L2 {e:.2e}", flush=True)
+        # ADAPTIVE residuals pass the records through two Levinson solves: the
+        # FP64 oracle's are 6e-5 to 1.2e-4 from the reference's fp32 ones here,
+        # so they are held to 3e-4; the wavelet gradient to 1e-4 as above
+        bound = 3e-4 if mt in (2, 3) else (1e-4 if k == "wavgrad" else RTOL_L2)
+        assert e <= bound, k
+    assert abs(r["misfit64"] - float(f["misfit"])) <= 1e-4 * abs(float(f["misfit"]))

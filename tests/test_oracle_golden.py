@@ -173,6 +173,26 @@ def test_fwi_a3d_gradient(golden, case):
     assert np.float32(r["misfit"]) == g["misfit"]
 
 
+@pytest.mark.parametrize("case", range(5))
+def test_fwi_a3d_misfit_types(golden, case):
+    """FwiAcoustic3D::run_optimal with CORRELATION, ADAPTIVE_GAUSS/LINEAR, the
+    default branch (type 4: DIFFERENCE) and data weights, restated, vs the
+    compiled reference, bit for bit: the 3D class leaves the residuals
+    unclipped (the 2D class clips types 1-3 at the 97th percentile, so a
+    restatement that clipped would fail here)."""
+    from oracle import fwi
+    from tests.golden.make_golden import FWI_A3D_MISFIT_CASES
+    g = golden("fwi_a3d_case0")
+    f = golden(f"fwi_a3d_misfit_case{case}")
+    order, fs, ncheck, mt, usew = FWI_A3D_MISFIT_CASES[case]
+    nx, ny, nz, lpml, dx, dt, nt = g["grid"]
+    r = fwi.fwi_a3d(g["vp"], g["rho"], int(lpml), (dx, dx, dx), dt, int(nt), order, fs, ncheck, g["scoords"],
+                    g["wav"], g["gcoords"], g["dobs"], misfit_type=mt, weight=f["weight"] if usew else None)
+    for k in ("vpgrad", "rhograd", "dmod", "dres", "wavgrad"):
+        assert np.array_equal(r[k], f[k]), k
+    assert np.float32(r["misfit"]) == f["misfit"]
+
+
 def test_interp_matches_reference(golden):
     """Interp::interp restated (LINEAR, BSPLINE, SINC) vs the reference."""
     from oracle import fn
