@@ -295,7 +295,10 @@ typedef struct {
     int ncheck;                          /* Revolve snapshots                        */
     int fp16;                            /* FP16 checkpoints                         */
     int misfit_type;                     /* 0 DIFFERENCE, 1 CORRELATION, 2 ADAPTIVE_GAUSS,
-                                            3 ADAPTIVE_LINEAR, 4 (= DIFFERENCE)       */
+                                            3 ADAPTIVE_LINEAR, 4 (= DIFFERENCE); the
+                                            residuals are not percentile-clipped
+                                            (FwiAcoustic3D::computeResiduals,
+                                            fwi.cpp:1651-1853)                        */
     const float *weight;                 /* data weights like obs, or NULL           */
 } rsg_fwi_a3d;
 
