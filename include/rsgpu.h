@@ -304,6 +304,26 @@ typedef struct {
 
 int rsg_fwi_a3d_gradient(const rsg_grid *g, const rsg_fwi_a3d *f, void *stream);
 
+/* ---- Acoustic 2D reverse-time migration of one shot
+ * (RtmAcoustic2D::run_optimal, rtm.cpp:423-544): forward steps with the
+ * source on the Revolve schedule (FP32 checkpoints in HBM), the receiver
+ * wavefield advanced with the forward operator from the recorded data, and
+ * the imaging condition with subsurface offsets (RtmAcoustic2D::crossCorr,
+ * rtm.cpp:191-219) at every reverse step.  The image is zeroed first, as
+ * run_optimal's allocateImage does; the call returns when the image is
+ * complete. */
+typedef struct {
+    const float *L, *Rx, *Rz, *tab;  /* staggered model and PML tables (device)      */
+    int nx, nz;                      /* unpadded sizes                               */
+    rsg_traces src;                  /* PRESSURE source, wavelet in data             */
+    rsg_traces data;                 /* recorded PRESSURE data (GMAP)                */
+    float *image;                    /* out: [nhz][nhx][nz][nx] (Image2D ki2D order)  */
+    int nhx, nhz;                    /* subsurface offsets, hx = ihx - (nhx-1)/2     */
+    int ncheck;                      /* Revolve snapshots                            */
+} rsg_rtm_a2d;
+
+int rsg_rtm_a2d_image(const rsg_grid *g, const rsg_rtm_a2d *r, void *stream);
+
 /* ---- Elastic 2D FWI gradient (FwiElastic2D::run_optimal, fwi.cpp:3937-4143)
  * for a ragged batch of shots, each on its own aperture window.  Per shot:
  * forward elastic steps with the source and P / Ux / Uz gathers, the misfit

@@ -46,11 +46,22 @@ def _adjoint_kind(kind):
 
 
 class Image2D:
-    """Gradient image on the unpadded model grid ([nz, nx] float32)."""
+    """Image on the unpadded model grid: [nz, nx] float32, or
+    [nhz, nhx, nz, nx] with subsurface offsets (Image2D's ki2D order)."""
 
-    def __init__(self, model):
+    def __init__(self, model, nhx=1, nhz=1):
         self.nx, self.nz = model.nx, model.nz
-        self.imagedata = np.zeros((self.nz, self.nx), np.float32)
+        self.nhx, self.nhz = int(nhx), int(nhz)
+        if self.nhx < 1 or self.nhz < 1:
+            raise R.RsgError("Image2D: nhx and nhz must be >= 1")
+        shape = (self.nz, self.nx) if self.nhx * self.nhz == 1 else (self.nhz, self.nhx, self.nz, self.nx)
+        self.imagedata = np.zeros(shape, np.float32)
+
+    def getNhx(self):
+        return self.nhx
+
+    def getNhz(self):
+        return self.nhz
 
     def getImagedata(self):
         return self.imagedata

@@ -338,6 +338,30 @@ def fwi_a3d_misfit_fixtures(R):
              dmod=r["dmod"], dres=r["dres"], wavgrad=r["wavgrad"], **extra)
 
 
+RTM_CASES = [  # order, fs, ncheck, nhx, nhz (RtmAcoustic2D::run_optimal)
+    (4, 0, 20, 1, 1), (8, 1, 7, 5, 3), (4, 0, 20, 4, 2),
+]
+
+
+def rtm_fixtures(R):
+    """RtmAcoustic2D::run_optimal on a smoothed model with the true model's
+    records; even and odd subsurface-offset counts and a free surface."""
+    from oracle import rtm
+    nx, nz, lpml, dx, dt, nt = 60, 50, 10, 10.0, 1e-3, 300
+    vtrue = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1800.0, 3000.0, nanom=5, seed=3)
+    rtrue = synth.gardner(vtrue)
+    vp0 = synth.gaussian_smooth(vtrue, 60.0, dx, dx)
+    rho0 = synth.gardner(vp0)
+    sc = np.float32([[300.0, 10.0]])
+    wav = synth.ricker(nt, dt, 15.0)[None]
+    gc = synth.surface_line(nx, dx, 20.0)
+    for k, (order, fs, ncheck, nhx, nhz) in enumerate(RTM_CASES):
+        data = runs.ref_a2d_run(vtrue, rtrue, lpml, dx, dx, dt, nt, order, fs, "P", sc, wav, "P", gc)["rec"]
+        img = rtm.ref_rtm_a2d(vp0, rho0, lpml, dx, dx, dt, nt, order, fs, ncheck, sc, wav, gc, data, nhx, nhz)
+        save(f"rtm_case{k}", vp=vp0, rho=rho0, scoords=sc, wav=wav, gcoords=gc, data=data,
+             grid=np.float32([nx, nz, lpml, dx, dt, nt]), image=img)
+
+
 INTERP_CASES = [  # ntr, n1, d1, o1, n2, d2, o2 (Interp::interp for every mode)
     (5, 300, 1e-3, 0.0, 700, 4e-4, 0.0), (3, 1000, 4.5e-4, 0.0, 150, 3e-3, 0.0),
     (4, 257, 2e-3, 0.01, 300, 1.7e-3, -0.003),
@@ -378,7 +402,7 @@ GROUPS = dict(der=der_fixture, pml=pml_fixture, stagger=stagger_fixture, makemap
               runs=run_fixtures, revolve=revolve_fixture, fwi=fwi_fixtures, fwi_full=fwi_full_fixtures,
               fwi_e2d=fwi_e2d_fixtures, fwi_a3d=fwi_a3d_fixtures,
               interp=interp_fixture, fwi_misfit=fwi_misfit_fixtures,
-              fwi_e2d_misfit=fwi_e2d_misfit_fixtures, fwi_a3d_misfit=fwi_a3d_misfit_fixtures)
+              fwi_e2d_misfit=fwi_e2d_misfit_fixtures, fwi_a3d_misfit=fwi_a3d_misfit_fixtures, rtm=rtm_fixtures)
 
 if __name__ == "__main__":
     # python -m tests.golden.make_golden [group ...]   (default: every group)
