@@ -340,6 +340,13 @@ def fwi_leg(n_plain):
     check = fwi.Survey(shots[:n_plain], lpml, dx, dx, dt, order=order, ncheck=50, mode="store")
     m_chk, vg_chk, _, _ = fwi.batch_gradient(vp0, rho0, check)
     err = float(np.linalg.norm(vg_chk.cpu().numpy() - gv_p) / np.linalg.norm(gv_p))
+    del vg_chk
+    # RtmAcoustic2D::run_optimal of shot 0 on the same model with its observed
+    # records: zero offset, then 21 horizontal subsurface offsets
+    rtm_s = {}
+    for nhx in (1, 21):
+        rtm_s[nhx] = _rtm_shot(vp0, rho0, lpml, dx, dt, nt, order, shots[0], nhx)
+        _progress(f"fwi: RTM of one shot, {nhx} subsurface offsets {rtm_s[nhx]:.2f} s")
     return {"workload": "acoustic 2D FWI gradient, 801x401 at 10 m (861x461 padded), order 8, 32 shots, "
                         "801 receivers, 10 Hz Ricker, nt 3000",
             "batched_shots_per_s": round(nshot / tb, 3), "batched_s": round(tb, 3),
@@ -351,7 +358,35 @@ def fwi_leg(n_plain):
             "speedup_batched_revolve_vs_plain": round((nshot / tr) / (n_plain / tp), 2),
             "batched_revolve_gradient_equals_store": same_r,
             "misfit_32_shots": misfit_b,
-            "batched_vs_plain_gradient_rel_l2": err}
+            "batched_vs_plain_gradient_rel_l2": err,
+            "rtm_shot_s": round(rtm_s[1], 3), "rtm_shot_21_offsets_s": round(rtm_s[21], 3),
+            "rtm": "RtmAcoustic2D::run_optimal of one shot, 50 Revolve checkpoints in HBM, image with nhx 1 / 21"}
+
+
+def _rtm_shot(vp0, rho0, lpml, dx, dt, nt, order, shot, nhx):
+    """Wall time of RtmAcoustic2D::run_optimal for one shot of the FWI leg
+    (the image returned to the host included)."""
+    import torch
+    from rockseis_amd import core, fwi, rtm
+    nz, nx = vp0.shape
+    m = core.ModelAcoustic2D(nx, nz, lpml, dx, dx, 0.0, 0.0, False)
+    m.Vp[:], m.R[:] = vp0, rho0
+    src = core.Data2D(1, nt, dt, 0.0, "P")
+    src.scoords[:] = shot.scoords
+    src.data[:] = shot.wavelet
+    src.makeMap(m, core.SMAP)
+    dat = core.Data2D(shot.gcoords.shape[0], nt, dt, 0.0, "P")
+    dat.gcoords[:] = shot.gcoords
+    dat.data[:] = shot.dobs
+    dat.makeMap(m, core.GMAP)
+    m.staggerModels()
+    img = fwi.Image2D(m, nhx, 1)
+    r = rtm.RtmAcoustic2D(m, img, src, dat, order, 1)
+    r.setNcheck(50)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r.run_optimal()
+    return time.perf_counter() - t0
 
 
 def a3d_leg():
