@@ -324,6 +324,22 @@ typedef struct {
 
 int rsg_rtm_a2d_image(const rsg_grid *g, const rsg_rtm_a2d *r, void *stream);
 
+/* ---- Acoustic 3D reverse-time migration of one shot
+ * (RtmAcoustic3D::run_optimal and crossCorr, rtm.cpp): as above on the 3D
+ * propagator, with nhx x nhy x nhz subsurface offsets; image
+ * [nhz][nhy][nhx][nz][ny][nx] (Image3D ki3D order, rtm.h:32). */
+typedef struct {
+    const float *L, *Rx, *Ry, *Rz, *tab; /* staggered model and PML tables (device)  */
+    int nx, ny, nz;                      /* unpadded sizes (nz <= 65535)             */
+    rsg_traces src;                      /* PRESSURE source, wavelet in data         */
+    rsg_traces data;                     /* recorded PRESSURE data (GMAP)            */
+    float *image;                        /* out, zeroed first                        */
+    int nhx, nhy, nhz;                   /* subsurface offsets                       */
+    int ncheck;                          /* Revolve snapshots                        */
+} rsg_rtm_a3d;
+
+int rsg_rtm_a3d_image(const rsg_grid *g, const rsg_rtm_a3d *r, void *stream);
+
 /* ---- Elastic 2D FWI gradient (FwiElastic2D::run_optimal, fwi.cpp:3937-4143)
  * for a ragged batch of shots, each on its own aperture window.  Per shot:
  * forward elastic steps with the source and P / Ux / Uz gathers, the misfit

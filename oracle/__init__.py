@@ -223,6 +223,9 @@ def _declare_ref(R):
     R.ref_rtm_a2d.argtypes = ([_f32p, _f32p] + [C.c_int] * 3 + [C.c_float] * 3 + [C.c_int] * 4
                               + [_f32p, _f32p, C.c_int, _f32p, _f32p, C.c_int, C.c_int, _f32p])
     R.ref_rtm_a2d.restype = None
+    R.ref_rtm_a3d.argtypes = ([_f32p, _f32p] + [C.c_int] * 4 + [C.c_float] * 4 + [C.c_int] * 4
+                              + [_f32p, _f32p, C.c_int, _f32p, _f32p] + [C.c_int] * 3 + [_f32p])
+    R.ref_rtm_a3d.restype = None
     R.ref_interp.argtypes = [C.c_int, _f32p, C.c_int, C.c_int, C.c_float, C.c_float, _f32p, C.c_int, C.c_float,
                              C.c_float]
     R.ref_revolve.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, C.c_int]

@@ -1,4 +1,4 @@
-// Reference harness, part 2: RtmAcoustic2D (test infrastructure).  rtm.h
+// Reference harness, part 2: RtmAcoustic2D/3D (test infrastructure).  rtm.h
 // and fwi.h declare the same enum, so the RTM classes are driven from this
 // translation unit of their own; extern "C" with plain pointers, arrays
 // x-fastest as in ref_harness.cpp.
@@ -52,6 +52,41 @@ void ref_rtm_a2d(const float *vp, const float *rho, int nx, int nz, int lpml, fl
     rtm->setIncore(true);
     rtm->run_optimal();
     std::memcpy(image, img->getImagedata(), sizeof(float) * nx * nz * nhx * nhz);
+}
+
+// RtmAcoustic3D::run_optimal with in-core Revolve (ncheck snapshots), one
+// PRESSURE source and PRESSURE data, (x,y,z) coordinate triples; image out
+// [nhz][nhy][nhx][nz][ny][nx].
+void ref_rtm_a3d(const float *vp, const float *rho, int nx, int ny, int nz, int lpml, float dx, float dy, float dz,
+                 float dt, int nt, int order, int fs, int ncheck, const float *scoords, const float *wav, int nrec,
+                 const float *gcoords, const float *data, int nhx, int nhy, int nhz, float *image) {
+    auto model = std::make_shared<ModelAcoustic3D<float>>(nx, ny, nz, lpml, dx, dy, dz, 0.f, 0.f, 0.f, fs != 0);
+    model->createModel();
+    const size_t nm = (size_t)nx * ny * nz;
+    std::memcpy(model->getVp(), vp, sizeof(float) * nm);
+    std::memcpy(model->getR(), rho, sizeof(float) * nm);
+    auto mk = [&](int ntr, const float *coords, bool src, const float *d) {
+        auto x = std::make_shared<Data3D<float>>(ntr, nt, dt, 0.f);
+        x->setField(PRESSURE);
+        Point3D<float> *c = src ? x->getGeom()->getScoords() : x->getGeom()->getGcoords();
+        for (int i = 0; i < ntr; i++) {
+            c[i].x = coords[3 * i];
+            c[i].y = coords[3 * i + 1];
+            c[i].z = coords[3 * i + 2];
+        }
+        std::memcpy(x->getData(), d, sizeof(float) * ntr * nt);
+        x->makeMap(model->getGeom(), src ? SMAP : GMAP);
+        return x;
+    };
+    auto src = mk(1, scoords, true, wav);
+    auto dat = mk(nrec, gcoords, false, data);
+    model->staggerModels();
+    auto img = std::make_shared<Image3D<float>>("rtm_image_unused", model, nhx, nhy, nhz);
+    auto rtm = std::make_shared<RtmAcoustic3D<float>>(model, img, src, dat, order, 1);
+    rtm->setNcheck(ncheck);
+    rtm->setIncore(true);
+    rtm->run_optimal();
+    std::memcpy(image, img->getImagedata(), sizeof(float) * nm * nhx * nhy * nhz);
 }
 
 }  // extern "C"

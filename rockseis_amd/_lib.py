@@ -69,6 +69,12 @@ class RtmA2D(C.Structure):
                    ("nhx", C.c_int), ("nhz", C.c_int), ("ncheck", C.c_int)])
 
 
+class RtmA3D(C.Structure):
+    _fields_ = ([(n, C.c_void_p) for n in ("L", "Rx", "Ry", "Rz", "tab")]
+                + [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("src", Traces), ("data", Traces),
+                   ("image", C.c_void_p), ("nhx", C.c_int), ("nhy", C.c_int), ("nhz", C.c_int), ("ncheck", C.c_int)])
+
+
 class FwiE2DShot(C.Structure):
     _fields_ = ([("g", Grid)] + [(n, C.c_void_p) for n in ("L", "L2M", "M", "Rx", "Rz", "Vp", "Vs", "Rho")]
                 + [("ox", C.c_int), ("src", Traces), ("obs", Traces * 3), ("dmod", C.c_void_p * 3),
@@ -153,6 +159,7 @@ def _declare(L):
         "rsg_fwi_e2d_plan_destroy": ([vp], ip),
         "rsg_fwi_a3d_gradient": ([GP, C.POINTER(FwiA3D), vp], ip),
         "rsg_rtm_a2d_image": ([GP, C.POINTER(RtmA2D), vp], ip),
+        "rsg_rtm_a3d_image": ([GP, C.POINTER(RtmA3D), vp], ip),
         "rsg_interp": ([ip, vp, ip, ip, fp, fp, vp, ip, fp, fp, vp], ip),
         "rsg_misfit_residuals": ([ip, vp, vp, vp, vp, vp, ip, ip, ip, ip, ip, C.POINTER(C.c_double), vp], ip),
         "rsg_misfit_residuals_e2d": ([ip, C.POINTER(E2DRecords), ip, ip, C.POINTER(C.c_double), vp], ip),
@@ -180,7 +187,7 @@ EXPORTS = (
     "rsg_revolve_numforw", "rsg_snap_cadence", "rsg_fwi_a2d_gradient_batch", "rsg_a2d_forward_velocity_batch",
     "rsg_a2d_forward_stress_batch", "rsg_e2d_velocity_batch", "rsg_e2d_stress_batch",
     "rsg_ragged2d_create", "rsg_ragged2d_step", "rsg_ragged2d_destroy", "rsg_fwi_e2d_gradient_batch",
-    "rsg_fwi_a3d_gradient", "rsg_rtm_a2d_image", "rsg_interp", "rsg_misfit_residuals", "rsg_misfit_residuals_e2d",
+    "rsg_fwi_a3d_gradient", "rsg_rtm_a2d_image", "rsg_rtm_a3d_image", "rsg_interp", "rsg_misfit_residuals", "rsg_misfit_residuals_e2d",
     "rsg_fwi_e2d_plan_create", "rsg_fwi_e2d_plan_run", "rsg_fwi_e2d_plan_destroy",
 )
 

@@ -68,11 +68,19 @@ class Image2D:
 
 
 class Image3D:
-    """Gradient image on the unpadded 3D model grid ([nz, ny, nx] float32)."""
+    """Image on the unpadded 3D model grid: [nz, ny, nx] float32, or
+    [nhz, nhy, nhx, nz, ny, nx] with subsurface offsets (Image3D's ki3D
+    order)."""
 
-    def __init__(self, model):
+    def __init__(self, model, nhx=1, nhy=1, nhz=1):
         self.nx, self.ny, self.nz = model.nx, model.ny, model.nz
-        self.imagedata = np.zeros((self.nz, self.ny, self.nx), np.float32)
+        self.nhx, self.nhy, self.nhz = int(nhx), int(nhy), int(nhz)
+        if min(self.nhx, self.nhy, self.nhz) < 1:
+            raise R.RsgError("Image3D: nhx, nhy and nhz must be >= 1")
+        shape = (self.nz, self.ny, self.nx)
+        if self.nhx * self.nhy * self.nhz > 1:
+            shape = (self.nhz, self.nhy, self.nhx) + shape
+        self.imagedata = np.zeros(shape, np.float32)
 
     def getImagedata(self):
         return self.imagedata

@@ -362,6 +362,23 @@ def rtm_fixtures(R):
              grid=np.float32([nx, nz, lpml, dx, dt, nt]), image=img)
 
 
+RTM_A3D_CASES = [  # order, fs, ncheck, nhx, nhy, nhz (RtmAcoustic3D::run_optimal)
+    (4, 0, 10, 1, 1, 1), (4, 1, 5, 3, 2, 3),
+]
+
+
+def rtm_a3d_fixtures(R):
+    """RtmAcoustic3D::run_optimal on the 3D FWI fixture's model with its
+    observed records; subsurface offsets in all three directions."""
+    from oracle import rtm
+    g = np.load(os.path.join(HERE, "fwi_a3d_case0.npz"), allow_pickle=False)
+    nx, ny, nz, lpml, dx, dt, nt = g["grid"]
+    for k, (order, fs, ncheck, nhx, nhy, nhz) in enumerate(RTM_A3D_CASES):
+        img = rtm.ref_rtm_a3d(g["vp"], g["rho"], int(lpml), (dx, dx, dx), dt, int(nt), order, fs, ncheck,
+                              g["scoords"], g["wav"], g["gcoords"], g["dobs"], nhx, nhy, nhz)
+        save(f"rtm_a3d_case{k}", image=img)
+
+
 INTERP_CASES = [  # ntr, n1, d1, o1, n2, d2, o2 (Interp::interp for every mode)
     (5, 300, 1e-3, 0.0, 700, 4e-4, 0.0), (3, 1000, 4.5e-4, 0.0, 150, 3e-3, 0.0),
     (4, 257, 2e-3, 0.01, 300, 1.7e-3, -0.003),
@@ -402,7 +419,8 @@ GROUPS = dict(der=der_fixture, pml=pml_fixture, stagger=stagger_fixture, makemap
               runs=run_fixtures, revolve=revolve_fixture, fwi=fwi_fixtures, fwi_full=fwi_full_fixtures,
               fwi_e2d=fwi_e2d_fixtures, fwi_a3d=fwi_a3d_fixtures,
               interp=interp_fixture, fwi_misfit=fwi_misfit_fixtures,
-              fwi_e2d_misfit=fwi_e2d_misfit_fixtures, fwi_a3d_misfit=fwi_a3d_misfit_fixtures, rtm=rtm_fixtures)
+              fwi_e2d_misfit=fwi_e2d_misfit_fixtures, fwi_a3d_misfit=fwi_a3d_misfit_fixtures, rtm=rtm_fixtures,
+              rtm_a3d=rtm_a3d_fixtures)
 
 if __name__ == "__main__":
     # python -m tests.golden.make_golden [group ...]   (default: every group)

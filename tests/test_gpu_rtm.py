@@ -50,3 +50,31 @@ def test_rtm_a2d_rerun_identical(golden):
     g = golden("rtm_case1")
     a, b = _run(g, 1), _run(g, 1)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("case", range(2))
+def test_rtm_a3d_vs_reference(golden, case):
+    from rockseis_amd import core, fwi, rtm
+    from tests.golden.make_golden import RTM_A3D_CASES
+    from tests.gpu_util import data3d
+    g = golden("fwi_a3d_case0")
+    f = golden(f"rtm_a3d_case{case}")
+    order, fs, ncheck, nhx, nhy, nhz = RTM_A3D_CASES[case]
+    print(f"rtm a3d case {case}: {RTM_A3D_CASES[case]}", flush=True)
+    nx, ny, nz, lpml, dx, dt, nt = g["grid"]
+    nx, ny, nz, lpml, nt = int(nx), int(ny), int(nz), int(lpml), int(nt)
+    m = core.ModelAcoustic3D(nx, ny, nz, lpml, dx, dx, dx, 0.0, 0.0, 0.0, fs)
+    m.Vp[:], m.R[:] = g["vp"], g["rho"]
+    src = data3d(g["scoords"], nt, dt, "P", g["wav"])
+    src.makeMap(m, core.SMAP)
+    dat = data3d(g["gcoords"], nt, dt, "P", g["dobs"])
+    dat.makeMap(m, core.GMAP)
+    m.staggerModels()
+    img = fwi.Image3D(m, nhx, nhy, nhz)
+    r = rtm.RtmAcoustic3D(m, img, src, dat, order, 1)
+    r.setNcheck(ncheck)
+    r.run_optimal()
+    got = img.imagedata.reshape(f["image"].shape)
+    e = rel_l2(got, f["image"])
+    print(f"  image: rel L2 {e:.2e}", flush=True)
+    assert e <= RTOL_L2

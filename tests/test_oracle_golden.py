@@ -303,3 +303,18 @@ def test_rtm_image_condition_offsets():
     want = np.zeros_like(img)
     want[2, 4, 4, 4] = 6.0
     assert np.array_equal(img, want)
+
+
+@pytest.mark.parametrize("case", range(2))
+def test_rtm_a3d_image(golden, case):
+    """RtmAcoustic3D::run_optimal restated (offsets along x, y and z, free
+    surface) vs the compiled reference, bit for bit."""
+    from oracle import rtm
+    from tests.golden.make_golden import RTM_A3D_CASES
+    g = golden("fwi_a3d_case0")
+    f = golden(f"rtm_a3d_case{case}")
+    order, fs, ncheck, nhx, nhy, nhz = RTM_A3D_CASES[case]
+    nx, ny, nz, lpml, dx, dt, nt = g["grid"]
+    img = rtm.rtm_a3d(g["vp"], g["rho"], int(lpml), (dx, dx, dx), dt, int(nt), order, fs, ncheck, g["scoords"],
+                      g["wav"], g["gcoords"], g["dobs"], nhx, nhy, nhz)
+    assert np.array_equal(img, f["image"])
