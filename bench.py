@@ -347,6 +347,14 @@ def fwi_leg(n_plain):
     for nhx in (1, 21):
         rtm_s[nhx] = _rtm_shot(vp0, rho0, lpml, dx, dt, nt, order, shots[0], nhx)
         _progress(f"fwi: RTM of one shot, {nhx} subsurface offsets {rtm_s[nhx]:.2f} s")
+    from rockseis_amd import rtm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    img = rtm.batch_image(vp0, rho0, sv_r)
+    torch.cuda.synchronize()
+    rtm_b = time.perf_counter() - t0
+    del img
+    _progress(f"fwi: RTM of {nshot} shots as one batch {rtm_b:.2f} s")
     return {"workload": "acoustic 2D FWI gradient, 801x401 at 10 m (861x461 padded), order 8, 32 shots, "
                         "801 receivers, 10 Hz Ricker, nt 3000",
             "batched_shots_per_s": round(nshot / tb, 3), "batched_s": round(tb, 3),
@@ -360,7 +368,10 @@ def fwi_leg(n_plain):
             "misfit_32_shots": misfit_b,
             "batched_vs_plain_gradient_rel_l2": err,
             "rtm_shot_s": round(rtm_s[1], 3), "rtm_shot_21_offsets_s": round(rtm_s[21], 3),
-            "rtm": "RtmAcoustic2D::run_optimal of one shot, 50 Revolve checkpoints in HBM, image with nhx 1 / 21"}
+            "rtm_batch_s": round(rtm_b, 3), "rtm_batch_shots_per_s": round(nshot / rtm_b, 3),
+            "speedup_rtm_batch_vs_single": round((nshot / rtm_b) / (1.0 / rtm_s[1]), 2),
+            "rtm": "RtmAcoustic2D::run_optimal of one shot, 50 Revolve checkpoints in HBM, image with nhx 1 / 21; "
+                   "batch: the 32 shots stacked in one set of launches per step (rsg_rtm_a2d_batch), same schedule"}
 
 
 def _rtm_shot(vp0, rho0, lpml, dx, dt, nt, order, shot, nhx):

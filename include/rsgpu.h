@@ -323,6 +323,11 @@ typedef struct {
 } rsg_rtm_a2d;
 
 int rsg_rtm_a2d_image(const rsg_grid *g, const rsg_rtm_a2d *r, void *stream);
+/* nshot shots on one grid in one set of launches per step (the reference
+ * migrates one shot per MPI worker and stacks the images, image.cpp): traces
+ * of shot k follow shot k-1 in src/data (map, shift, data); the image is the
+ * stack over the shots, added in shot order at every step. */
+int rsg_rtm_a2d_batch(const rsg_grid *g, const rsg_rtm_a2d *r, int nshot, void *stream);
 
 /* ---- Acoustic 3D reverse-time migration of one shot
  * (RtmAcoustic3D::run_optimal and crossCorr, rtm.cpp): as above on the 3D

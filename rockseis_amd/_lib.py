@@ -159,6 +159,7 @@ def _declare(L):
         "rsg_fwi_e2d_plan_destroy": ([vp], ip),
         "rsg_fwi_a3d_gradient": ([GP, C.POINTER(FwiA3D), vp], ip),
         "rsg_rtm_a2d_image": ([GP, C.POINTER(RtmA2D), vp], ip),
+        "rsg_rtm_a2d_batch": ([GP, C.POINTER(RtmA2D), ip, vp], ip),
         "rsg_rtm_a3d_image": ([GP, C.POINTER(RtmA3D), vp], ip),
         "rsg_interp": ([ip, vp, ip, ip, fp, fp, vp, ip, fp, fp, vp], ip),
         "rsg_misfit_residuals": ([ip, vp, vp, vp, vp, vp, ip, ip, ip, ip, ip, C.POINTER(C.c_double), vp], ip),
@@ -187,7 +188,7 @@ EXPORTS = (
     "rsg_revolve_numforw", "rsg_snap_cadence", "rsg_fwi_a2d_gradient_batch", "rsg_a2d_forward_velocity_batch",
     "rsg_a2d_forward_stress_batch", "rsg_e2d_velocity_batch", "rsg_e2d_stress_batch",
     "rsg_ragged2d_create", "rsg_ragged2d_step", "rsg_ragged2d_destroy", "rsg_fwi_e2d_gradient_batch",
-    "rsg_fwi_a3d_gradient", "rsg_rtm_a2d_image", "rsg_rtm_a3d_image", "rsg_interp", "rsg_misfit_residuals", "rsg_misfit_residuals_e2d",
+    "rsg_fwi_a3d_gradient", "rsg_rtm_a2d_image", "rsg_rtm_a2d_batch", "rsg_rtm_a3d_image", "rsg_interp", "rsg_misfit_residuals", "rsg_misfit_residuals_e2d",
     "rsg_fwi_e2d_plan_create", "rsg_fwi_e2d_plan_run", "rsg_fwi_e2d_plan_destroy",
 )
 

@@ -66,6 +66,26 @@ __global__ __launch_bounds__(256) void rtm_image_a3d_kernel(const float *__restr
     }
 }
 
+// A batch of shots (blockIdx.z = offset): image += ws_k * wr_k for k in shot
+// order, shot k's fields at k*fstride; the stacked image of the batch.
+__global__ __launch_bounds__(256) void rtm_image_a2d_batch_kernel(const float *__restrict__ ws,
+                                                                  const float *__restrict__ wr, float *__restrict__ img,
+                                                                  int nx, int nz, int lpml, int nhx, int nhz, int ns,
+                                                                  size_t fstride) {
+    const int ix = blockIdx.x * 64 + threadIdx.x, iz = blockIdx.y * 4 + threadIdx.y;
+    if (ix >= nx || iz >= nz) return;
+    const int ihx = (int)blockIdx.z % nhx, ihz = (int)blockIdx.z / nhx;
+    const int hx = -(nhx - 1) / 2 + ihx, hz = -(nhz - 1) / 2 + ihz;
+    if (ix - hx < 0 || ix - hx >= nx || ix + hx < 0 || ix + hx >= nx) return;
+    if (iz - hz < 0 || iz - hz >= nz || iz + hz < 0 || iz + hz >= nz) return;
+    const size_t nxp = (size_t)nx + 2 * lpml;
+    const size_t m = (size_t)blockIdx.z * nx * nz + (size_t)iz * nx + ix;
+    const size_t ps = (size_t)(iz - hz + lpml) * nxp + (ix - hx + lpml), pr = (size_t)(iz + hz + lpml) * nxp + (ix + hx + lpml);
+    float a = img[m];
+    for (int k = 0; k < ns; k++) a += ws[k * fstride + ps] * wr[k * fstride + pr];
+    img[m] = a;
+}
+
 namespace {
 
 struct RtmMem {
@@ -140,7 +160,106 @@ int a3d_state(RtmMem &dm, const rsg_grid *g, const rsg_rtm_a3d *r, rsg_state *s,
     return RSG_OK;
 }
 
+// The WavesAcoustic2D states of ns shots in one allocation: each array holds
+// all shots back to back (P, Vx, Vz, then the 8 strips).
+int a2d_batch_state(RtmMem &dm, const rsg_grid *g, const rsg_rtm_a2d *r, int ns, rsg_state *s, rsg_batch *b,
+                    float **base, size_t *total, hipStream_t st) {
+    const size_t n = (size_t)g->nx * g->nz, sx = (size_t)g->lpml * g->nz, sz = (size_t)g->nx * g->lpml;
+    *total = (size_t)ns * (3 * n + 4 * sx + 4 * sz);
+    if (int rc = dm.alloc(base, *total)) return rc;
+    if (int rc = launch_zero(*base, *total * sizeof(float), st)) return rc;
+    *s = rsg_state{};
+    s->P = *base;
+    s->Vx = *base + ns * n;
+    s->Vz = *base + 2 * ns * n;
+    float *p = *base + 3 * ns * n;
+    const int xs[4] = {RSG_A2D_P_LEFT, RSG_A2D_VXX_LEFT, RSG_A2D_P_RIGHT, RSG_A2D_VXX_RIGHT};
+    const int zs[4] = {RSG_A2D_P_TOP, RSG_A2D_VZZ_TOP, RSG_A2D_P_BOTTOM, RSG_A2D_VZZ_BOTTOM};
+    for (int i = 0; i < 4; i++, p += ns * sx) s->psi[xs[i]] = p;
+    for (int i = 0; i < 4; i++, p += ns * sz) s->psi[zs[i]] = p;
+    s->L = r->L;
+    s->Rx = r->Rx;
+    s->Rz = r->Rz;
+    s->tab = r->tab;
+    *b = rsg_batch{};
+    b->nshot = ns;
+    b->field_stride = n;
+    b->xstrip_stride = sx;
+    b->zstrip_stride = sz;
+    return RSG_OK;
+}
+
 }  // namespace
+
+static int rtm_a2d_batch_impl(const rsg_grid *g, const rsg_rtm_a2d *r, int ns, hipStream_t st) {
+    const int nt = r->src.nt;
+    if (ns < 1 || ns > 65535) return fail(RSG_EINVAL, "rtm_a2d_batch: 1 <= nshot <= 65535");
+    if (r->data.nt != nt || r->data.field != RSG_PRESSURE || r->src.field != RSG_PRESSURE)
+        return fail(RSG_EINVAL, "rtm_a2d_batch: source and data must be PRESSURE traces with equal nt");
+    if (r->nx + 2 * g->lpml != g->nx || r->nz + 2 * g->lpml != g->nz || g->ny != 1)
+        return fail(RSG_EINVAL, "rtm_a2d_batch: image size must be the unpadded 2D grid");
+    if (r->nhx < 1 || r->nhz < 1 || (long)r->nhx * r->nhz > 65535 || !r->image)
+        return fail(RSG_EINVAL, "rtm_a2d_batch: need an image and 1 <= nhx*nhz <= 65535 subsurface offsets");
+    if (r->ncheck < 1 || nt < 1) return fail(RSG_EINVAL, "rtm_a2d_batch: need nt >= 1 and ncheck >= 1");
+    Revolve rev(nt, r->ncheck);
+    {
+        const size_t one = (size_t)ns * (3 * (size_t)g->nx * g->nz + 4 * (size_t)g->lpml * (g->nx + g->nz));
+        size_t free_b = 0, total_b = 0;
+        RSG_HIP(hipMemGetInfo(&free_b, &total_b));
+        if (one * sizeof(float) * (2 + (size_t)rev.snaps) + ((size_t)1 << 30) > free_b)
+            return fail(RSG_ENOMEM, "rtm_a2d_batch: wavefields and checkpoints do not fit in device memory");
+    }
+    RtmMem dm;
+    rsg_state sf{}, sb{};
+    rsg_batch bf{}, bb{};
+    float *fwb = nullptr, *bwb = nullptr, *ck = nullptr;
+    size_t total = 0;
+    if (int rc = a2d_batch_state(dm, g, r, ns, &sf, &bf, &fwb, &total, st)) return rc;
+    if (int rc = a2d_batch_state(dm, g, r, ns, &sb, &bb, &bwb, &total, st)) return rc;
+    if (int rc = dm.alloc(&ck, total * (size_t)rev.snaps)) return rc;
+    const size_t nimg = (size_t)r->nhx * r->nhz * r->nx * r->nz, fs = (size_t)g->nx * g->nz;
+    if (int rc = launch_zero(r->image, nimg * sizeof(float), st)) return rc;
+    const dim3 xb(64, 4), xg((r->nx + 63) / 64, (r->nz + 3) / 4, r->nhx * r->nhz);
+
+    auto fstep = [&](int it) -> int {
+        if (int rc = rsg_a2d_forward_velocity_batch(g, &sf, &bf, st)) return rc;
+        if (int rc = rsg_a2d_forward_stress_batch(g, &sf, &bf, st)) return rc;
+        return insert_source(RSG_KIND_A2D, g, &sf, &r->src, nullptr, it, st, ns, fs);
+    };
+    auto image_step = [&](int capo) -> int {
+        if (int rc = insert_source(RSG_KIND_A2D, g, &sb, &r->data, nullptr, capo, st, ns, fs)) return rc;
+        rtm_image_a2d_batch_kernel<<<xg, xb, 0, st>>>(sf.P, sb.P, r->image, r->nx, r->nz, g->lpml, r->nhx, r->nhz,
+                                                      ns, fs);
+        RSG_LAUNCH_CHECK("rtm_image_a2d_batch_kernel");
+        return RSG_OK;
+    };
+    for (;;) {
+        const int oldcapo = rev.capo;
+        const Revolve::Action a = rev.next();
+        const int capo = rev.capo;
+        if (a == Revolve::ADVANCE) {
+            for (int it = oldcapo; it < capo; it++)
+                if (int rc = fstep(it)) return rc;
+        } else if (a == Revolve::FIRSTURN) {
+            if (int rc = fstep(capo)) return rc;
+            if (int rc = image_step(capo)) return rc;
+        } else if (a == Revolve::YOUTURN) {
+            if (int rc = rsg_a2d_forward_velocity_batch(g, &sb, &bb, st)) return rc;
+            if (int rc = rsg_a2d_forward_stress_batch(g, &sb, &bb, st)) return rc;
+            if (int rc = image_step(capo)) return rc;
+        } else if (a == Revolve::TAKESHOT) {
+            RSG_HIP(hipMemcpyAsync(ck + total * rev.check, fwb, total * sizeof(float), hipMemcpyDeviceToDevice, st));
+        } else if (a == Revolve::RESTORE) {
+            RSG_HIP(hipMemcpyAsync(fwb, ck + total * rev.check, total * sizeof(float), hipMemcpyDeviceToDevice, st));
+        } else if (a == Revolve::TERMINATE) {
+            break;
+        } else {
+            return fail(RSG_EINVAL, "rtm_a2d_batch: Revolve returned an error");
+        }
+    }
+    RSG_HIP(hipStreamSynchronize(st));
+    return RSG_OK;
+}
 
 static int rtm_a2d_impl(const rsg_grid *g, const rsg_rtm_a2d *r, hipStream_t st) {
     const int nt = r->src.nt;
@@ -287,6 +406,11 @@ static int rtm_a3d_impl(const rsg_grid *g, const rsg_rtm_a3d *r, hipStream_t st)
 extern "C" int rsg_rtm_a3d_image(const rsg_grid *g, const rsg_rtm_a3d *r, void *stream) {
     if (!g || !r) return rsg::fail(RSG_EINVAL, "rsg_rtm_a3d_image: null argument");
     return rsg::rtm_a3d_impl(g, r, (hipStream_t)stream);
+}
+
+extern "C" int rsg_rtm_a2d_batch(const rsg_grid *g, const rsg_rtm_a2d *r, int nshot, void *stream) {
+    if (!g || !r) return rsg::fail(RSG_EINVAL, "rsg_rtm_a2d_batch: null argument");
+    return rsg::rtm_a2d_batch_impl(g, r, nshot, (hipStream_t)stream);
 }
 
 extern "C" int rsg_rtm_a2d_image(const rsg_grid *g, const rsg_rtm_a2d *r, void *stream) {

@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
 import torch
 
 from . import _lib as R
@@ -111,3 +112,61 @@ class RtmAcoustic3D(RtmAcoustic2D):
         R.check(R.lib().rsg_rtm_a3d_image(C.byref(g), C.byref(r), _stream()), "rsg_rtm_a3d_image")
         img.imagedata[...] = image.cpu().numpy().reshape(img.imagedata.shape)
         return 1
+
+
+def batch_image(vp, rho, survey, nhx=1, nhz=1):
+    """Stacked RTM image of every shot of a survey (fwi.Survey; each shot's
+    dobs is its recorded data) as one batch through rsg_rtm_a2d_batch: every
+    kernel of a time step runs all shots in one launch, on the survey's
+    Revolve schedule (ncheck checkpoints).  Shots must share nt and their
+    trace counts.  Returns the image [nhz, nhx, nz, nx] (a device tensor)."""
+    dev = _dev()
+    nz, nx = vp.shape
+    shots = survey.shots
+    ns = len(shots)
+    nsrc, nrec, nt = shots[0].scoords.shape[0], shots[0].gcoords.shape[0], shots[0].wavelet.shape[1]
+    for sh in shots:
+        if sh.scoords.shape[0] != nsrc or sh.gcoords.shape[0] != nrec or sh.wavelet.shape[1] != nt:
+            raise R.RsgError("batch_image: shots must share nt and trace counts")
+    if survey.ncheck < 1:
+        raise R.RsgError("batch_image: the survey needs ncheck >= 1 Revolve checkpoints")
+    m = ModelAcoustic2D(nx, nz, survey.lpml, survey.dx, survey.dz, 0.0, 0.0, survey.fs)
+    m.Vp[:] = vp
+    m.R[:] = rho
+    smaps, sshifts, gmaps, gshifts = [], [], [], []
+    for sh in shots:
+        d = Data2D(nsrc, nt, survey.dt, 0.0, "P")
+        d.scoords[:] = sh.scoords
+        d.makeMap(m, SMAP)
+        smaps.append(d.smap)
+        sshifts.append(d.sshift)
+        d = Data2D(nrec, nt, survey.dt, 0.0, "P")
+        d.gcoords[:] = sh.gcoords
+        d.makeMap(m, GMAP)
+        gmaps.append(d.gmap)
+        gshifts.append(d.gshift)
+    m.staggerModels()
+    pml = Pml(m.lpml, survey.dt)
+    pml.setSmax(SMAX)
+    pml.computeABC()
+
+    def dev_t(a, dtype=torch.float32):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(dev, dtype)
+
+    keep = dict(smap=dev_t(np.stack(smaps), torch.int32), sshift=dev_t(np.stack(sshifts)),
+                gmap=dev_t(np.stack(gmaps), torch.int32), gshift=dev_t(np.stack(gshifts)),
+                wav=dev_t(np.stack([sh.wavelet for sh in shots])), data=dev_t(np.stack([sh.dobs for sh in shots])))
+    image = torch.zeros((int(nhz), int(nhx), nz, nx), dtype=torch.float32, device=dev)
+    g = R.Grid(nx=m.getNx_pml(), ny=1, nz=m.getNz_pml(), lpml=m.lpml, order=survey.order,
+               fs=int(m.fs), dx=m.dx, dy=1.0, dz=m.dz, dt=survey.dt)
+    r = R.RtmA2D()
+    r.L, r.Rx, r.Rz, r.tab = m._dL.data_ptr(), m._dRx.data_ptr(), m._dRz.data_ptr(), pml.tab.data_ptr()
+    r.nx, r.nz = nx, nz
+    r.src = R.Traces(field=R.PRESSURE, ntrace=nsrc, nt=nt, map=keep["smap"].data_ptr(),
+                     shift=keep["sshift"].data_ptr(), data=keep["wav"].data_ptr(), factor=None)
+    r.data = R.Traces(field=R.PRESSURE, ntrace=nrec, nt=nt, map=keep["gmap"].data_ptr(),
+                      shift=keep["gshift"].data_ptr(), data=keep["data"].data_ptr(), factor=None)
+    r.image = image.data_ptr()
+    r.nhx, r.nhz, r.ncheck = int(nhx), int(nhz), survey.ncheck
+    R.check(R.lib().rsg_rtm_a2d_batch(C.byref(g), C.byref(r), ns, _stream()), "rsg_rtm_a2d_batch")
+    return image

@@ -78,3 +78,50 @@ def test_rtm_a3d_vs_reference(golden, case):
     e = rel_l2(got, f["image"])
     print(f"  image: rel L2 {e:.2e}", flush=True)
     assert e <= RTOL_L2
+
+
+def test_rtm_a2d_batch_equals_stack_of_shots():
+    """Three shots migrated as one batch (rsg_rtm_a2d_batch) against the sum
+    of the FP32 oracle's single-shot images (relative L2 1e-5; the batch adds
+    the shots' products cell by cell at every step, the stack adds whole
+    images), with 3 x 1 subsurface offsets; and a batch of one shot equals
+    the single-shot entry point bit for bit."""
+    from oracle import rtm as ortm
+    from oracle import runs
+    from rockseis_amd import fwi, rtm, synth
+    nx, nz, lpml, dx, dt, nt, order, ncheck = 60, 50, 10, 10.0, 1e-3, 300, 8, 7
+    vtrue = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1800.0, 3000.0, nanom=5, seed=3)
+    rtrue = synth.gardner(vtrue)
+    vp0 = synth.gaussian_smooth(vtrue, 60.0, dx, dx)
+    rho0 = synth.gardner(vp0)
+    wav = synth.ricker(nt, dt, 15.0)[None]
+    gc = synth.surface_line(nx, dx, 20.0)
+    shots, want = [], 0.0
+    for sx in (150.0, 300.0, 450.0):
+        sc = np.float32([[sx, 10.0]])
+        dobs = runs.a2d_run(vtrue, rtrue, lpml, dx, dx, dt, nt, order, False, "P", sc, wav, "P", gc)["rec"]
+        shots.append(fwi.Shot(sc, wav, gc, dobs))
+        want = want + ortm.rtm_a2d(vp0, rho0, lpml, dx, dx, dt, nt, order, False, ncheck, sc, wav, gc, dobs, 3,
+                                   1).astype(np.float64)
+    sv = fwi.Survey(shots, lpml, dx, dx, dt, order=order, ncheck=ncheck, mode="revolve")
+    img = rtm.batch_image(vp0, rho0, sv, nhx=3, nhz=1).cpu().numpy()
+    e = rel_l2(img, want)
+    print(f"  batch of 3 shots vs the oracle's stack: rel L2 {e:.2e}", flush=True)
+    assert e <= RTOL_L2
+    one = fwi.Survey(shots[:1], lpml, dx, dx, dt, order=order, ncheck=ncheck, mode="revolve")
+    b1 = rtm.batch_image(vp0, rho0, one, nhx=3, nhz=1).cpu().numpy()
+    g = dict(grid=np.float32([nx, nz, lpml, dx, dt, nt]), vp=vp0, rho=rho0, scoords=shots[0].scoords, wav=wav,
+             gcoords=gc, data=shots[0].dobs)
+    from rockseis_amd import core
+    m = core.ModelAcoustic2D(nx, nz, lpml, dx, dx, 0.0, 0.0, False)
+    m.Vp[:], m.R[:] = vp0, rho0
+    src = data2d(g["scoords"], nt, dt, "P", wav)
+    src.makeMap(m, core.SMAP)
+    dat = data2d(gc, nt, dt, "P", g["data"])
+    dat.makeMap(m, core.GMAP)
+    m.staggerModels()
+    im = fwi.Image2D(m, 3, 1)
+    r = rtm.RtmAcoustic2D(m, im, src, dat, order, 1)
+    r.setNcheck(ncheck)
+    r.run_optimal()
+    assert np.array_equal(b1, im.imagedata.reshape(b1.shape))
