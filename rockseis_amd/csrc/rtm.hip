@@ -12,6 +12,7 @@
 
 #include "rsg_common.h"
 #include "rsg_fwi.h"
+#include "rsg_inject_ordered.h"
 #include "rsg_srcrec.h"
 
 // This file is compiled with -ffp-contract=off (Makefile): the product and
@@ -93,12 +94,13 @@ struct RtmMem {
     ~RtmMem() {
         for (void *p : ptrs) (void)hipFree(p);
     }
-    int alloc(float **out, size_t n) {
+    template <class T>
+    int alloc(T **out, size_t n) {
         void *p = nullptr;
-        if (hipMalloc(&p, (n ? n : 1) * sizeof(float)) != hipSuccess)
+        if (hipMalloc(&p, (n ? n : 1) * sizeof(T)) != hipSuccess)
             return fail(RSG_ENOMEM, "rtm: device allocation failed");
         ptrs.push_back(p);
-        *out = (float *)p;
+        *out = (T *)p;
         return RSG_OK;
     }
 };
@@ -217,6 +219,9 @@ static int rtm_a2d_batch_impl(const rsg_grid *g, const rsg_rtm_a2d *r, int ns, h
     if (int rc = a2d_batch_state(dm, g, r, ns, &sf, &bf, &fwb, &total, st)) return rc;
     if (int rc = a2d_batch_state(dm, g, r, ns, &sb, &bb, &bwb, &total, st)) return rc;
     if (int rc = dm.alloc(&ck, total * (size_t)rev.snaps)) return rc;
+    OrderedInject2D oi;
+    if (r->data.ntrace > 0)
+        if (int rc = oi.build(dm, r->data.map, r->data.ntrace, ns, g->nx, g->lpml)) return rc;
     const size_t nimg = (size_t)r->nhx * r->nhz * r->nx * r->nz, fs = (size_t)g->nx * g->nz;
     if (int rc = launch_zero(r->image, nimg * sizeof(float), st)) return rc;
     const dim3 xb(64, 4), xg((r->nx + 63) / 64, (r->nz + 3) / 4, r->nhx * r->nhz);
@@ -227,7 +232,12 @@ static int rtm_a2d_batch_impl(const rsg_grid *g, const rsg_rtm_a2d *r, int ns, h
         return insert_source(RSG_KIND_A2D, g, &sf, &r->src, nullptr, it, st, ns, fs);
     };
     auto image_step = [&](int capo) -> int {
-        if (int rc = insert_source(RSG_KIND_A2D, g, &sb, &r->data, nullptr, capo, st, ns, fs)) return rc;
+        if (oi.maxcells > 0) {
+            inject2d_ordered_kernel<<<dim3((oi.maxcells + 63) / 64, ns), 64, 0, st>>>(
+                sb.P, r->L, g->dt, oi.cells, oi.starts, oi.ents, oi.coff, r->data.shift, r->data.data,
+                r->data.ntrace, nt, capo, fs);
+            RSG_LAUNCH_CHECK("inject2d_ordered_kernel");
+        }
         rtm_image_a2d_batch_kernel<<<xg, xb, 0, st>>>(sf.P, sb.P, r->image, r->nx, r->nz, g->lpml, r->nhx, r->nhz,
                                                       ns, fs);
         RSG_LAUNCH_CHECK("rtm_image_a2d_batch_kernel");
@@ -284,6 +294,9 @@ static int rtm_a2d_impl(const rsg_grid *g, const rsg_rtm_a2d *r, hipStream_t st)
             return fail(RSG_ENOMEM, "rtm_a2d: checkpoints do not fit in device memory");
     }
     if (int rc = dm.alloc(&ck, total * (size_t)rev.snaps)) return rc;
+    OrderedInject2D oi;
+    if (r->data.ntrace > 0)
+        if (int rc = oi.build(dm, r->data.map, r->data.ntrace, 1, g->nx, g->lpml)) return rc;
     // RtmAcoustic2D::run_optimal allocates (zeroes) the image before the run
     const size_t nimg = (size_t)r->nhx * r->nhz * r->nx * r->nz;
     if (int rc = launch_zero(r->image, nimg * sizeof(float), st)) return rc;
@@ -295,7 +308,12 @@ static int rtm_a2d_impl(const rsg_grid *g, const rsg_rtm_a2d *r, hipStream_t st)
         return insert_source(RSG_KIND_A2D, g, &sf, &r->src, nullptr, it, st);
     };
     auto image_step = [&](int capo) -> int {
-        if (int rc = insert_source(RSG_KIND_A2D, g, &sb, &r->data, nullptr, capo, st)) return rc;
+        if (oi.maxcells > 0) {
+            inject2d_ordered_kernel<<<dim3((oi.maxcells + 63) / 64, 1), 64, 0, st>>>(
+                sb.P, r->L, g->dt, oi.cells, oi.starts, oi.ents, oi.coff, r->data.shift, r->data.data,
+                r->data.ntrace, nt, capo, 0);
+            RSG_LAUNCH_CHECK("inject2d_ordered_kernel");
+        }
         rtm_image_a2d_kernel<<<xg, xb, 0, st>>>(sf.P, sb.P, r->image, r->nx, r->nz, g->lpml, r->nhx, r->nhz);
         RSG_LAUNCH_CHECK("rtm_image_a2d_kernel");
         return RSG_OK;
