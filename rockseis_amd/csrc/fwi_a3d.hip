@@ -16,6 +16,7 @@
 
 #include "rsg_common.h"
 #include "rsg_fwi.h"
+#include "rsg_inject_ordered.h"
 #include "rsg_srcrec.h"
 #include "rsg_store16.h"
 
@@ -76,87 +77,6 @@ struct Mem3 {
         return RSG_OK;
     }
 };
-
-// Residual injection in trace order (insertSource of dataresP, fwi.cpp:2093
-// and 2132; the trace loop of WavesAcoustic3D::insertSource,
-// waves.cpp:1033-1057): neighbouring off-grid receivers put non-zero weights
-// on shared corner cells, and per-trace atomic adds would leave those sums in
-// scheduling order.  At set-up the host lists the touched cells of the padded
-// grid, each with its (trace, corner) contributions in trace order (entry =
-// trace*8 + z*4 + y*2 + x corner bit); one thread per cell adds them in that
-// order.
-struct OrderedInject3D {
-    const size_t *cells = nullptr;
-    const int *starts = nullptr, *ents = nullptr;
-    int ncells = 0;
-    int build(Mem3 &dm, const int *map_dev, int ntr, int nx, int ny, int nz, int lpml, hipStream_t st) {
-        if (ntr > (INT_MAX >> 3)) return fail(RSG_EINVAL, "fwi_a3d: too many receiver traces");
-        std::vector<int> mp(3 * (size_t)ntr);
-        RSG_HIP(hipMemcpyAsync(mp.data(), map_dev, sizeof(int) * mp.size(), hipMemcpyDeviceToHost, st));
-        RSG_HIP(hipStreamSynchronize(st));
-        const size_t nxy = (size_t)nx * ny;
-        std::vector<std::pair<size_t, int>> e;  // (cell, trace*8 + corner)
-        e.reserve(8 * (size_t)ntr);
-        for (int i = 0; i < ntr; i++) {
-            const int mx = mp[3 * (size_t)i], my = mp[3 * (size_t)i + 1], mz = mp[3 * (size_t)i + 2];
-            if (mx < 0 || my < 0 || mz < 0) continue;
-            if (lpml + mx + 1 >= nx || lpml + my + 1 >= ny || lpml + mz + 1 >= nz)
-                return fail(RSG_EINVAL, "fwi_a3d: receiver map outside the padded grid");
-            for (int i1 = 0; i1 < 2; i1++)
-                for (int i2 = 0; i2 < 2; i2++)
-                    for (int i3 = 0; i3 < 2; i3++)
-                        e.emplace_back((size_t)(lpml + mz + i1) * nxy + (size_t)(lpml + my + i2) * nx + (lpml + mx + i3),
-                                       i * 8 + i1 * 4 + i2 * 2 + i3);
-        }
-        std::stable_sort(e.begin(), e.end(), [](const std::pair<size_t, int> &x, const std::pair<size_t, int> &y) {
-            return x.first < y.first;
-        });
-        std::vector<size_t> hc;
-        std::vector<int> hs, he;
-        for (size_t q = 0; q < e.size(); q++) {
-            if (q == 0 || e[q].first != e[q - 1].first) {
-                hc.push_back(e[q].first);
-                hs.push_back((int)he.size());
-            }
-            he.push_back(e[q].second);
-        }
-        hs.push_back((int)he.size());
-        ncells = (int)hc.size();
-        if (ncells == 0) return RSG_OK;
-        size_t *dc = nullptr;
-        int *di = nullptr;
-        if (int rc = dm.alloc(&dc, hc.size())) return rc;
-        if (int rc = dm.alloc(&di, hs.size() + he.size())) return rc;
-        RSG_HIP(hipMemcpyAsync(dc, hc.data(), sizeof(size_t) * hc.size(), hipMemcpyHostToDevice, st));
-        RSG_HIP(hipMemcpyAsync(di, hs.data(), sizeof(int) * hs.size(), hipMemcpyHostToDevice, st));
-        RSG_HIP(hipMemcpyAsync(di + hs.size(), he.data(), sizeof(int) * he.size(), hipMemcpyHostToDevice, st));
-        // the host vectors go out of scope with this call
-        RSG_HIP(hipStreamSynchronize(st));
-        cells = dc;
-        starts = di;
-        ents = di + hs.size();
-        return RSG_OK;
-    }
-};
-
-// P += dt * L * res * xtri * ytri * ztri, cell by cell in trace order.
-__global__ void inject3d_ordered_kernel(float *P, const float *__restrict__ L, float dt,
-                                        const size_t *__restrict__ cells, const int *__restrict__ starts,
-                                        const int *__restrict__ ents, int ncells, const float *__restrict__ shift,
-                                        const float *__restrict__ wav, int nt, int it) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ncells) return;
-    const size_t p = cells[j];
-    const float m = dt * L[p];
-    float a = P[p];
-    for (int q = starts[j]; q < starts[j + 1]; q++) {
-        const int e = ents[q], i = e >> 3, i1 = (e >> 2) & 1, i2 = (e >> 1) & 1, i3 = e & 1;
-        const float sx = shift[3 * (size_t)i], sy = shift[3 * (size_t)i + 1], sz = shift[3 * (size_t)i + 2];
-        const float xt = i3 ? sx : 1 - sx, yt = i2 ? sy : 1 - sy, zt = i1 ? sz : 1 - sz;
-        a += m * wav[(size_t)i * nt + it] * xt * yt * zt;
-    }
-    P[p] = a;
-}
 
 // Strip slot i of the acoustic 3D state: 0 left/right (nz*ny*lpml),
 // 1 front/back (nz*lpml*nx), 2 top/bottom (lpml*ny*nx).

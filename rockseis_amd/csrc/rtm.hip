@@ -375,6 +375,9 @@ static int rtm_a3d_impl(const rsg_grid *g, const rsg_rtm_a3d *r, hipStream_t st)
     if (int rc = a3d_state(dm, g, r, &sf, &fwb, &total, st)) return rc;
     if (int rc = a3d_state(dm, g, r, &sb, &bwb, &total, st)) return rc;
     if (int rc = dm.alloc(&ck, total * (size_t)rev.snaps)) return rc;
+    OrderedInject3D oi;
+    if (r->data.ntrace > 0)
+        if (int rc = oi.build(dm, r->data.map, r->data.ntrace, g->nx, g->ny, g->nz, g->lpml, st)) return rc;
     const size_t nimg = (size_t)r->nhx * r->nhy * r->nhz * r->nx * r->ny * r->nz;
     if (int rc = launch_zero(r->image, nimg * sizeof(float), st)) return rc;
     const dim3 xb(64, 4), xg((r->nx + 63) / 64, (r->ny + 3) / 4, r->nz);
@@ -385,7 +388,12 @@ static int rtm_a3d_impl(const rsg_grid *g, const rsg_rtm_a3d *r, hipStream_t st)
         return insert_source(RSG_KIND_A3D, g, &sf, &r->src, nullptr, it, st);
     };
     auto image_step = [&](int capo) -> int {
-        if (int rc = insert_source(RSG_KIND_A3D, g, &sb, &r->data, nullptr, capo, st)) return rc;
+        if (oi.ncells > 0) {
+            inject3d_ordered_kernel<<<(oi.ncells + 63) / 64, 64, 0, st>>>(sb.P, r->L, g->dt, oi.cells, oi.starts,
+                                                                          oi.ents, oi.ncells, r->data.shift,
+                                                                          r->data.data, nt, capo);
+            RSG_LAUNCH_CHECK("inject3d_ordered_kernel");
+        }
         rtm_image_a3d_kernel<<<xg, xb, 0, st>>>(sf.P, sb.P, r->image, r->nx, r->ny, r->nz, g->lpml, r->nhx, r->nhy,
                                                 r->nhz);
         RSG_LAUNCH_CHECK("rtm_image_a3d_kernel");
