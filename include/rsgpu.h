@@ -48,6 +48,12 @@ typedef struct {
     int order;  /* stencil half-width 1..8 (Der clamps, der.cpp:94-95)      */
     int fs;     /* free surface: Szz row z=lpml zeroed (waves.cpp:1666-1674) */
     float dx, dy, dz, dt;
+    int domdec; /* domain-decomposition mode of a one-domain model built by
+                   Model*::getDomainmodel (model.cpp:1511-1620): nx, ny, nz
+                   already hold the PML padding, source/receiver maps index
+                   the grid with no lpml offset (waves.cpp:411-418) and
+                   FWI images and snapshots cover the whole grid.  The PML
+                   strips are still applied on every side (lpml wide).    */
 } rsg_grid;
 
 /* PML memory strip slots of rsg_state.psi[] */
@@ -123,6 +129,14 @@ int rsg_get_tuning(const char *key, int *value);
 int rsg_pml_tables(int lpml, float dt, float smax, int sign, float *tab_host);
 /* Der<T> coefficient table, 36 entries, order k at k(k-1)/2 (der.cpp:3-53). */
 int rsg_der_coeffs(float *out36);
+/* Der<T>::dd{x,y,z}_{fw,bw} (der.cpp:155-540) on a device array: df = the
+ * staggered derivative of f along axis 0 (x), 1 (y) or 2 (z) of the
+ * nx x ny x nz grid with step d, forward (backward = 0) or backward, with
+ * the reference's reduced orders at the ends of the axis and its
+ * operation order (equal to the reference's floats).  order is clamped to
+ * 1..8 as Der does; the axis must hold at least 2*order+1 cells. */
+int rsg_der_apply(int axis, int backward, const float *f, float *df, int nx, int ny, int nz, float d,
+                  int order, void *stream);
 /* Model*::staggerModels (model.cpp:1350-1411, 2720-2801, 1965-2040):
  * unpadded [nz][nx] (or [nz][ny][nx]) host inputs -> padded host outputs. */
 int rsg_stagger_a2d(const float *vp, const float *rho, int nx, int nz, int lpml, int fs,
@@ -214,6 +228,21 @@ int rsg_interp(int mode, const float *from, int ntr, int n1, float d1, float o1,
 int rsg_modelling_run(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src,
                       int nrec, const rsg_traces *recs, int it0, int nsteps, int graph_chunk,
                       void *stream);
+/* The same loop with snapshots (Modelling*::setSnapP/Vx/Vz/Sxx/Szz/Sxz,
+ * modelling.h:79-81, 155-160, written at modelling.cpp:272-282 through
+ * Snapshot2D/3D::writeSnap, snap.cpp:638-669): after the gathers of step
+ * it with it % snapinc == 0, out + (it / snapinc) * interior receives
+ * f[0] (+ f[1]) (+ f[2]) over the interior of the grid (the cells at
+ * lpml .. n-lpml-1 on every padded axis; the whole grid with g->domdec),
+ * x fastest.  out holds (nt-1)/snapinc + 1 slots. */
+typedef struct {
+    const float *f[3]; /* device fields summed into the snapshot (NULL: unused) */
+    float *out;        /* device, [snapnt][interior]                             */
+    int snapinc;
+} rsg_snapset;
+int rsg_modelling_run_snap(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src,
+                           int nrec, const rsg_traces *recs, int nsnap, const rsg_snapset *snaps, int it0,
+                           int nsteps, int graph_chunk, void *stream);
 
 /* ---- FWI gradient of one shot (FwiAcoustic2D::run_optimal, fwi.cpp:1159-1332,
  * and FwiAcoustic2D::run, fwi.cpp:1030-1156) */
@@ -237,8 +266,12 @@ int rsg_modelling_run(int kind, const rsg_grid *g, const rsg_state *s, const rsg
 
 typedef struct {
     const float *L, *Rx, *Rz, *tab; /* staggered model and PML tables (device)    */
-    const float *Vp, *Rho;          /* unpadded model nz*nx (device), for crossCorr */
-    int nx, nz;                     /* unpadded sizes; grid is (nx+2lpml)(nz+2lpml) */
+    const float *Vp, *Rho;          /* unpadded model nz*nx (device), for crossCorr;
+                                       with g->domdec the model on the grid       */
+    int nx, nz;                     /* image sizes: the unpadded sizes, grid
+                                       (nx+2lpml)(nz+2lpml); with g->domdec the
+                                       grid's own sizes (crossCorr over the
+                                       domain with no pad, fwi.cpp:535-544)       */
     rsg_traces src;                 /* PRESSURE source, wavelet in data             */
     rsg_traces obs;                 /* observed PRESSURE data at the receivers      */
     float *dmod, *dres;             /* out: modelled data, residuals (ntrace*nt)   */
@@ -259,6 +292,10 @@ typedef struct {
                                        DIFFERENCE)                                   */
     const float *weight;            /* data weights like obs (setDataweight) or NULL */
     int adjoint;                    /* RSG_ADJOINT_REFERENCE or RSG_ADJOINT_EXACT    */
+    float *srcilum;                 /* in/out (optional, like vpgrad): source
+                                       illumination -= (-2/Vp) P_f^2 at every
+                                       imaging step (crossCorr with srcilumset,
+                                       fwi.cpp:567-569); REFERENCE adjoint only */
 } rsg_fwi_a2d;
 
 int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, void *stream);
@@ -278,6 +315,22 @@ int rsg_misfit_residuals(int type, const float *mod, const float *rec, const flo
  * src/obs (map, shift, data), dmod/dres/wavgrad are [nshot][ntrace][nt],
  * vpgrad/rhograd and the misfit are summed over the shots. */
 int rsg_fwi_a2d_gradient_batch(const rsg_grid *g, const rsg_fwi_a2d *f, int nshot, void *stream);
+/* FwiAcoustic2D::crossCorr (fwi.cpp:511-571) on device arrays, one imaging
+ * step: over ix, iz in [1, n-1) of the nx x nz image, vpgrad -= (-2/Vp) wsp
+ * (Dx wrx + Dz wrz), rhograd -= (-1/rho) wsp (...) and += the velocity terms,
+ * srcilum (optional) -= (-2/Vp) wsp^2.  wsp is padded by pads cells on each
+ * side, wrx / wrz / Rx / Rz by padr; domdec = 1 takes pads = padr = 0 and
+ * images on the padded domain, as the reference does. */
+int rsg_crosscorr_a2d(const float *wsp, int pads, const float *wrx, const float *wrz, int padr,
+                      const float *Vp, const float *Rho, const float *Rx, const float *Rz, int nx, int nz,
+                      float dx, float dz, int domdec, float *vpgrad, float *rhograd, float *srcilum,
+                      void *stream);
+/* Snapshot2D/3D::writeSnap (snap.cpp:638-669, 1138-1172): out[k][j][i] =
+ * f0 + f1 + f2 (those given, added in that order) at cell (i+pad, j+pad,
+ * k+pad) of the g->nx x g->ny x g->nz grid, for the (nx-2pad) x (ny-2pad)
+ * x (nz-2pad) interior (2D: ny = 1, no y padding). */
+int rsg_snap_interior(const float *f0, const float *f1, const float *f2, const rsg_grid *g, int pad,
+                      float *out, void *stream);
 /* ---- Acoustic 3D FWI gradient of one shot (FwiAcoustic3D::run_optimal,
  * fwi.cpp:1991-2171): Revolve checkpoints in HBM (FP32, or FP16 with one
  * power-of-two scale per array), adjoint with the forward operator,
@@ -426,12 +479,52 @@ typedef struct {
 int rsg_misfit_residuals_e2d(int type, const rsg_e2d_records *shots, int nshot, int nt, double *misfit,
                              void *stream);
 
+/* Model<T>::getLocalsize2d (model.cpp:271-340): the aperture window of a
+ * gather.  coords [ntrace][4] hold sx, sz, gx, gz per trace; map 0 (SMAP)
+ * centres the window on the sources, 1 (GMAP) on the receivers. */
+int rsg_local_window_2d(const float *coords, int ntrace, float dx, float ox, float aperture, int map,
+                        long long *start, long long *size);
+/* ModelAcoustic2D::getDomainmodel (model.cpp:1511-1620) for one domain:
+ * the window [start, start+size) of the global [nz][nx] model, padded by
+ * lpml cells with edge values, as the arrays of the domain model
+ * ((size+2lpml) x (nz+2lpml)): Vp, R, L = R Vp^2, Rx = 2/(R + R_x+1)
+ * (1/R in the last global column), Rz = 2/(R + R_z+1), free surface
+ * Rx *= 2, L = 0 on row lpml; *ox_out / *oz_out its origin. */
+int rsg_domain_model_a2d(const float *vp, const float *rho, int nx, int nz, int lpml, int fs, float dx,
+                         float dz, float ox, float oz, long long start, long long size, float *Vp, float *R,
+                         float *L, float *Rx, float *Rz, float *ox_out, float *oz_out);
+/* ModelElastic2D::getDomainmodel for one domain: as above, plus Vs, L2M,
+ * M (moduli and buoyancies as model.cpp writes them for the domain model). */
+int rsg_domain_model_e2d(const float *vp, const float *vs, const float *rho, int nx, int nz, int lpml, int fs,
+                         float dx, float dz, float ox, float oz, long long start, long long size, float *Vp,
+                         float *Vs, float *R, float *L, float *L2M, float *M, float *Rx, float *Rz,
+                         float *ox_out, float *oz_out);
+/* Image2D::stackImage_parallel's placement (image.cpp:577-578): the first
+ * column (row) of a local image of origin ol and step dl in a global image
+ * of origin og and step dg, (int)(ol/dl - og/dg) in float arithmetic. */
+int rsg_image_offset(float ol, float dl, float og, float dg);
+
 /* Revolve::revolve() action sequence (revolve.cpp:150-305): advance=0,
  * takeshot=1, restore=2, firsturn=3, youturn=4, terminate=5, error=6. */
 int rsg_revolve_schedule(int nt, int snaps, int *actions, int *capos, int *checks, int maxn,
                          int *n);
 /* Revolve::numforw (revolve.cpp:31-62) */
 int rsg_revolve_numforw(int nt, int snaps);
+/* Revolve::adjust (revolve.cpp:114-148): the snapshot count the
+ * constructor takes when the requested one is invalid. */
+int rsg_revolve_adjust(int nt);
+/* The Revolve state machine one action at a time, for callers that drive
+ * the schedule themselves (the Revolve class API: constructor,
+ * revolve() and getCapo/getCheck/getFine, revolve.h:25-124).  init takes
+ * the constructor's arguments (snaps outside [2, nt] replaced by
+ * rsg_revolve_adjust(nt), *adjusted set to 1); next returns the action
+ * numbered as rsg_revolve_schedule and updates the state. */
+typedef struct {
+    int steps, snaps, capo, fine, check, turn, oldfine;
+    int ch[512];
+} rsg_revolve_state;
+int rsg_revolve_init(rsg_revolve_state *st, int nt, int snaps, int *adjusted);
+int rsg_revolve_next(rsg_revolve_state *st);
 /* Snapshot2D/3D cadence (snap.cpp:330-335): snapshots are taken at it % snapinc
  * == 0 and read back at reverse times t with (t - enddiff) % snapinc == 0. */
 int rsg_snap_cadence(int nt, float dt, int snapinc, int *snapnt, int *enddiff);

@@ -27,7 +27,7 @@ A3D_STRIPS = ("P_left", "P_right", "P_front", "P_back", "P_top", "P_bottom", "Vx
 class Grid(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("lpml", C.c_int),
                 ("order", C.c_int), ("fs", C.c_int), ("dx", C.c_float), ("dy", C.c_float),
-                ("dz", C.c_float), ("dt", C.c_float)]
+                ("dz", C.c_float), ("dt", C.c_float), ("domdec", C.c_int)]
 
 
 class State(C.Structure):
@@ -52,7 +52,8 @@ class Fwi(C.Structure):
                 + [(n, C.c_void_p) for n in ("dmod", "dres", "wavgrad", "vpgrad", "rhograd")]
                 + [("misfit", C.POINTER(C.c_double)), ("ncheck", C.c_int), ("mode", C.c_int),
                    ("mode_used", C.POINTER(C.c_int)), ("snapinc", C.c_int), ("fp16", C.c_int),
-                   ("misfit_type", C.c_int), ("weight", C.c_void_p), ("adjoint", C.c_int)])
+                   ("misfit_type", C.c_int), ("weight", C.c_void_p), ("adjoint", C.c_int),
+                   ("srcilum", C.c_void_p)])
 
 
 class FwiA3D(C.Structure):
@@ -79,6 +80,10 @@ class FwiE2DShot(C.Structure):
     _fields_ = ([("g", Grid)] + [(n, C.c_void_p) for n in ("L", "L2M", "M", "Rx", "Rz", "Vp", "Vs", "Rho")]
                 + [("ox", C.c_int), ("src", Traces), ("obs", Traces * 3), ("dmod", C.c_void_p * 3),
                    ("dres", C.c_void_p * 3), ("wavgrad", C.c_void_p), ("weight", C.c_void_p * 3)])
+
+
+class SnapSet(C.Structure):
+    _fields_ = [("f", C.c_void_p * 3), ("out", C.c_void_p), ("snapinc", C.c_int)]
 
 
 class E2DRecords(C.Structure):
@@ -164,6 +169,19 @@ def _declare(L):
         "rsg_interp": ([ip, vp, ip, ip, fp, fp, vp, ip, fp, fp, vp], ip),
         "rsg_misfit_residuals": ([ip, vp, vp, vp, vp, vp, ip, ip, ip, ip, ip, C.POINTER(C.c_double), vp], ip),
         "rsg_misfit_residuals_e2d": ([ip, C.POINTER(E2DRecords), ip, ip, C.POINTER(C.c_double), vp], ip),
+        "rsg_modelling_run_snap": ([ip, GP, SP, TP, ip, TP, ip, C.POINTER(SnapSet), ip, ip, ip, vp], ip),
+        "rsg_snap_interior": ([vp, vp, vp, GP, ip, vp, vp], ip),
+        "rsg_crosscorr_a2d": ([vp, ip, vp, vp, ip, vp, vp, vp, vp, ip, ip, fp, fp, ip, vp, vp, vp, vp], ip),
+        "rsg_local_window_2d": ([vp, ip, fp, fp, fp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], ip),
+        "rsg_domain_model_a2d": ([vp, vp, ip, ip, ip, ip, fp, fp, fp, fp, C.c_longlong, C.c_longlong,
+                                  vp, vp, vp, vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_float)], ip),
+        "rsg_domain_model_e2d": ([vp, vp, vp, ip, ip, ip, ip, fp, fp, fp, fp, C.c_longlong, C.c_longlong,
+                                  vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_float)], ip),
+        "rsg_image_offset": ([fp, fp, fp, fp], ip),
+        "rsg_der_apply": ([ip, ip, vp, vp, ip, ip, ip, fp, ip, vp], ip),
+        "rsg_revolve_adjust": ([ip], ip),
+        "rsg_revolve_init": ([vp, ip, ip, C.POINTER(C.c_int)], ip),
+        "rsg_revolve_next": ([vp], ip),
     }
     for name in ("a2d_forward_velocity", "a2d_forward_stress", "e2d_forward_velocity",
                  "e2d_forward_stress", "e2d_backward_velocity", "e2d_backward_stress",
@@ -190,6 +208,9 @@ EXPORTS = (
     "rsg_ragged2d_create", "rsg_ragged2d_step", "rsg_ragged2d_destroy", "rsg_fwi_e2d_gradient_batch",
     "rsg_fwi_a3d_gradient", "rsg_rtm_a2d_image", "rsg_rtm_a2d_batch", "rsg_rtm_a3d_image", "rsg_interp", "rsg_misfit_residuals", "rsg_misfit_residuals_e2d",
     "rsg_fwi_e2d_plan_create", "rsg_fwi_e2d_plan_run", "rsg_fwi_e2d_plan_destroy",
+    "rsg_modelling_run_snap", "rsg_snap_interior", "rsg_crosscorr_a2d", "rsg_local_window_2d",
+    "rsg_domain_model_a2d", "rsg_domain_model_e2d", "rsg_image_offset", "rsg_der_apply",
+    "rsg_revolve_adjust", "rsg_revolve_init", "rsg_revolve_next",
 )
 
 

@@ -573,11 +573,40 @@ class _Modelling:
         self.order = int(order) if 1 < int(order) < 9 else 4      # modelling.cpp:17-23
         self.snapinc = max(int(snapinc), 1)
         self.recs = {}
+        self.snaps = {}
+        self.snapfiles = {}
         self.graph_chunk = 0
         self.waves = None
 
     def getOrder(self):
         return self.order
+
+    def getSnapinc(self):
+        return self.snapinc
+
+    # Snapshots (Modelling*::setSnapP/Vx/Vz/Sxx/Szz/Sxz, modelling.h:79-81,
+    # 155-160): every snapinc-th step the interior of the field (P of the
+    # elastic propagator is Sxx + Szz, as modelling.cpp snaps it) is captured
+    # on the device inside the time loop; after run() it is in
+    # self.snaps[name] as a [snapnt][nz][nx] (3D [snapnt][nz][ny][nx]) numpy
+    # array, and written as an RSS SNAPSHOT file when a file name is given.
+    _SNAP_FIELDS = {"P": ("P",), "VX": ("Vx",), "VY": ("Vy",), "VZ": ("Vz",)}
+
+    def _set_snap(self, name, filename):
+        self.snapfiles[name] = filename
+        self.snaps[name] = None
+
+    def setSnapP(self, filename=None):
+        self._set_snap("P", filename)
+
+    def setSnapVx(self, filename=None):
+        self._set_snap("VX", filename)
+
+    def setSnapVy(self, filename=None):
+        self._set_snap("VY", filename)
+
+    def setSnapVz(self, filename=None):
+        self._set_snap("VZ", filename)
 
     def setRecP(self, d):
         self.recs["P"] = d
@@ -624,11 +653,36 @@ class _Modelling:
             self._keep.append(kk)
             self._rec_dev[k] = kk[2]
         self._args = (g, s, t_src, recs, len(self.recs))
+        self._snapsets = self._prepare_snaps(w)
         return self._args
+
+    def _prepare_snaps(self, w):
+        if not self.snaps:
+            return None
+        pad = 0 if getattr(w, "domdec", False) else w.lpml
+        shape = (w.nz_pml - 2 * pad,) + ((w.ny_pml - 2 * pad,) if w.ny_pml > 1 else ()) + (w.nx_pml - 2 * pad,)
+        snapnt = (self.source.nt - 1) // self.snapinc + 1
+        sets = (R.SnapSet * len(self.snaps))()
+        self._snap_dev = {}
+        for i, name in enumerate(self.snaps):
+            fields = [getattr(w, f) for f in self._SNAP_FIELDS[name]]
+            out = torch.zeros((snapnt,) + shape, dtype=torch.float32, device=fields[0].device)
+            for k, f in enumerate(fields):
+                sets[i].f[k] = f.data_ptr()
+            sets[i].out = out.data_ptr()
+            sets[i].snapinc = self.snapinc
+            self._snap_dev[name] = out
+        return sets
 
     def launch(self, it0=0, nsteps=None):
         g, s, t_src, recs, nrec = self._args
         nsteps = self.source.nt - it0 if nsteps is None else nsteps
+        if self._snapsets is not None:
+            R.check(R.lib().rsg_modelling_run_snap(self.waves.kind, C.byref(g), C.byref(s), C.byref(t_src), nrec,
+                                                   recs, len(self._snapsets), self._snapsets, it0, nsteps,
+                                                   self.graph_chunk, _stream()),
+                    "rsg_modelling_run_snap")
+            return
         R.check(R.lib().rsg_modelling_run(self.waves.kind, C.byref(g), C.byref(s), C.byref(t_src), nrec,
                                           recs, it0, nsteps, self.graph_chunk, _stream()),
                 "rsg_modelling_run")
@@ -637,6 +691,20 @@ class _Modelling:
         torch.cuda.current_stream().synchronize()
         for k, d in self.recs.items():
             d.data[...] = self._rec_dev[k].cpu().numpy()
+        if self._snapsets is not None:
+            from . import rss
+            m, w = self.model, self.waves
+            for name, out in self._snap_dev.items():
+                a = out.cpu().numpy()
+                self.snaps[name] = a
+                fn = self.snapfiles.get(name)
+                if fn:
+                    nx, nz = a.shape[-1], a.shape[1]
+                    ny = a.shape[2] if a.ndim == 4 else 1
+                    h = rss.Header(4, 0, rss.SNAPSHOT, 0, 1, (nx, ny, nz, a.shape[0]),
+                                   (m.dx, getattr(m, "dy", 1.0), m.dz, w.dt * self.snapinc),
+                                   (m.ox, getattr(m, "oy", 0.0), m.oz, w.ot))
+                    rss.write(fn, h, a)
 
     def run(self):
         self.prepare()
@@ -658,6 +726,17 @@ class ModellingAcoustic2D(_Modelling):
 
 class ModellingElastic2D(ModellingAcoustic2D):
     waves_cls = WavesElastic2D
+    _SNAP_FIELDS = {"P": ("Sxx", "Szz"), "VX": ("Vx",), "VZ": ("Vz",), "SXX": ("Sxx",), "SZZ": ("Szz",),
+                    "SXZ": ("Sxz",)}
+
+    def setSnapSxx(self, filename=None):
+        self._set_snap("SXX", filename)
+
+    def setSnapSzz(self, filename=None):
+        self._set_snap("SZZ", filename)
+
+    def setSnapSxz(self, filename=None):
+        self._set_snap("SXZ", filename)
 
 
 class ModellingAcoustic3D(_Modelling):

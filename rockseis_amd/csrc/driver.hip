@@ -58,7 +58,9 @@ struct IoPlan {
 static int make_io_plan(int kind, const rsg_grid *g, const rsg_traces *src, int nrec, const rsg_traces *recs,
                         hipStream_t st, IoPlan &p) {
     p.rest.assign(recs, recs + nrec);
-    if ((kind != RSG_KIND_A2D && kind != RSG_KIND_E2D) || !tuning().fuse_io) return RSG_OK;
+    // the fused gathers index tiles from lpml-offset maps: domain models
+    // (maps without the offset) take the separate kernels
+    if ((kind != RSG_KIND_A2D && kind != RSG_KIND_E2D) || !tuning().fuse_io || g->domdec) return RSG_OK;
     FusedIO &io = p.io;
     p.rz = tile_rows();
     const int tz = BZ * p.rz, ntx = (g->nx + BX - 1) / BX, ntiles = ntx * ((g->nz + tz - 1) / tz);
@@ -138,9 +140,46 @@ static int make_io_plan(int kind, const rsg_grid *g, const rsg_traces *src, int 
     return RSG_OK;
 }
 
+// Snapshot2D/3D::writeSnap (snap.cpp:638-669, 1138-1172) on the device:
+// out = f0 (+ f1) (+ f2), added in that order as the reference adds
+// data1 + data2 + data3, over the interior cells (i+pad, j+pad(3D), k+pad).
+// With snapinc > 0 the kernel writes only at steps t % snapinc == 0, into
+// slot t / snapinc of out (t = *it_dev + it0, or it0).
+__global__ void snap_interior_kernel(const float *__restrict__ f0, const float *__restrict__ f1,
+                                     const float *__restrict__ f2, int nx, int ny, int pad, int ypad, long nxi,
+                                     long nyi, long ntot, float *__restrict__ out, const int *it_dev, int it0,
+                                     int snapinc) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ntot) return;
+    long slot = 0;
+    if (snapinc > 0) {
+        const int t = it_dev ? *it_dev + it0 : it0;
+        if (t % snapinc) return;
+        slot = t / snapinc;
+    }
+    const long i = idx % nxi, r = idx / nxi, j = r % nyi, k = r / nyi;
+    const size_t p = ((size_t)(k + pad) * ny + (size_t)(j + ypad)) * nx + (size_t)(i + pad);
+    float v = f0 ? f0[p] : 0.f;
+    if (f1) v += f1[p];
+    if (f2) v += f2[p];
+    out[slot * ntot + idx] = v;
+}
+
+static int launch_snap(const rsg_snapset &sn, const rsg_grid *g, const int *it_dev, int it, hipStream_t st) {
+    const int pad = g->domdec ? 0 : g->lpml;
+    const int ypad = g->ny > 1 ? pad : 0;
+    const long nxi = g->nx - 2 * pad, nyi = g->ny - 2 * ypad, nzi = g->nz - 2 * pad;
+    const long ntot = nxi * nyi * nzi;
+    if (ntot <= 0) return RSG_OK;
+    snap_interior_kernel<<<(unsigned)((ntot + 255) / 256), 256, 0, st>>>(
+        sn.f[0], sn.f[1], sn.f[2], g->nx, g->ny, pad, ypad, nxi, nyi, ntot, sn.out, it_dev, it, sn.snapinc);
+    RSG_LAUNCH_CHECK("snap_interior_kernel");
+    return RSG_OK;
+}
+
 // One modelling step; it_dev null means `it` is the absolute time index.
 static int one_step(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src, const IoPlan &p,
-                    const int *it_dev, int it, hipStream_t st) {
+                    int nsnap, const rsg_snapset *snaps, const int *it_dev, int it, hipStream_t st) {
     if (int rc = step_velocity(kind, g, s, st)) return rc;
     if (p.active) {
         FusedIO io = p.io;
@@ -150,31 +189,70 @@ static int one_step(int kind, const rsg_grid *g, const rsg_state *s, const rsg_t
     } else if (int rc = step_stress(kind, g, s, st)) {
         return rc;
     }
-    return step_io(kind, g, s, p.src_fused ? nullptr : src, (int)p.rest.size(), p.rest.data(), it_dev, it, st);
+    if (int rc = step_io(kind, g, s, p.src_fused ? nullptr : src, (int)p.rest.size(), p.rest.data(), it_dev, it, st))
+        return rc;
+    for (int k = 0; k < nsnap; k++)
+        if (int rc = launch_snap(snaps[k], g, it_dev, it, st)) return rc;
+    return RSG_OK;
 }
+
+static int modelling_run(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src, int nrec,
+                         const rsg_traces *recs, int nsnap, const rsg_snapset *snaps, int it0, int nsteps,
+                         int graph_chunk, hipStream_t st);
 
 }  // namespace rsg
 
 using namespace rsg;
 
+extern "C" int rsg_snap_interior(const float *f0, const float *f1, const float *f2, const rsg_grid *g, int pad,
+                                 float *out, void *stream) {
+    if (!g || !out || g->nx < 1 || g->ny < 1 || g->nz < 1 || pad < 0)
+        return fail(RSG_EINVAL, "rsg_snap_interior: bad arguments");
+    if (g->nx <= 2 * pad || g->nz <= 2 * pad || (g->ny > 1 && g->ny <= 2 * pad))
+        return fail(RSG_EINVAL, "rsg_snap_interior: grid smaller than its padding");
+    rsg_grid gg = *g;
+    gg.lpml = pad;
+    gg.domdec = 0;
+    rsg_snapset sn{{f0, f1, f2}, out, 0};
+    return launch_snap(sn, &gg, nullptr, 0, (hipStream_t)stream);
+}
+
 extern "C" int rsg_modelling_run(int kind, const rsg_grid *g, const rsg_state *s,
                                  const rsg_traces *src, int nrec, const rsg_traces *recs, int it0,
                                  int nsteps, int graph_chunk, void *stream) {
+    return modelling_run(kind, g, s, src, nrec, recs, 0, nullptr, it0, nsteps, graph_chunk, (hipStream_t)stream);
+}
+
+extern "C" int rsg_modelling_run_snap(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src,
+                                      int nrec, const rsg_traces *recs, int nsnap, const rsg_snapset *snaps,
+                                      int it0, int nsteps, int graph_chunk, void *stream) {
+    if (nsnap < 0 || (nsnap > 0 && !snaps)) return fail(RSG_EINVAL, "rsg_modelling_run_snap: bad snapshot sets");
+    for (int k = 0; k < nsnap; k++)
+        if (!snaps[k].out || snaps[k].snapinc < 1 || (!snaps[k].f[0] && !snaps[k].f[1] && !snaps[k].f[2]))
+            return fail(RSG_EINVAL, "rsg_modelling_run_snap: a snapshot set needs a field, out and snapinc >= 1");
+    return modelling_run(kind, g, s, src, nrec, recs, nsnap, snaps, it0, nsteps, graph_chunk, (hipStream_t)stream);
+}
+
+namespace rsg {
+
+static int modelling_run(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src, int nrec,
+                         const rsg_traces *recs, int nsnap, const rsg_snapset *snaps, int it0, int nsteps,
+                         int graph_chunk, hipStream_t st) {
     if (!g || !s || nsteps < 0 || it0 < 0 || nrec < 0 || (nrec > 0 && !recs))
         return fail(RSG_EINVAL, "rsg_modelling_run: bad arguments");
     if (src && it0 + nsteps > src->nt) return fail(RSG_EINVAL, "rsg_modelling_run: past wavelet end");
     for (int r = 0; r < nrec; r++)
         if (it0 + nsteps > recs[r].nt) return fail(RSG_EINVAL, "rsg_modelling_run: past record end");
-    hipStream_t st = (hipStream_t)stream;
     // fusing pays for its set-up (map downloads, receiver tables) over a few
-    // tens of steps; short calls keep the separate I/O kernels
+    // tens of steps; short calls keep the separate I/O kernels unless the
+    // caller asked for a captured graph
     IoPlan plan;
-    if (nsteps >= 32)
+    if (nsteps >= 32 || graph_chunk > 1)
         if (int rc = make_io_plan(kind, g, src, nrec, recs, st, plan)) return rc;
     if (!plan.active) plan.rest.assign(recs, recs + nrec);
     if (graph_chunk <= 1 || nsteps < 2 * graph_chunk) {
         int rc = RSG_OK;
-        for (int it = it0; it < it0 + nsteps && rc == RSG_OK; it++) rc = one_step(kind, g, s, src, plan, nullptr, it, st);
+        for (int it = it0; it < it0 + nsteps && rc == RSG_OK; it++) rc = one_step(kind, g, s, src, plan, nsnap, snaps, nullptr, it, st);
         // the receiver tables are freed with the plan: drain first
         if (plan.active) {
             const hipError_t e = hipStreamSynchronize(st);
@@ -206,7 +284,7 @@ extern "C" int rsg_modelling_run(int kind, const rsg_grid *g, const rsg_state *s
             break;
         }
         for (int k = 0; k < graph_chunk && rc == RSG_OK; k++)
-            rc = one_step(kind, g, s, src, plan, it_dev, k, st);
+            rc = one_step(kind, g, s, src, plan, nsnap, snaps, it_dev, k, st);
         if (rc == RSG_OK) rc = launch_add_int(it_dev, graph_chunk, st);
         hipError_t e = hipStreamEndCapture(st, &graph);
         if (rc) break;
@@ -225,7 +303,7 @@ extern "C" int rsg_modelling_run(int kind, const rsg_grid *g, const rsg_state *s
         }
         if (rc) break;
         for (int it = it0 + nchunk * graph_chunk; it < it0 + nsteps && rc == RSG_OK; it++)
-            rc = one_step(kind, g, s, src, plan, nullptr, it, st);
+            rc = one_step(kind, g, s, src, plan, nsnap, snaps, nullptr, it, st);
         if (rc) break;
         e = hipStreamSynchronize(st);
         if (e != hipSuccess) rc = check_hip(e, "hipStreamSynchronize");
@@ -239,6 +317,8 @@ extern "C" int rsg_modelling_run(int kind, const rsg_grid *g, const rsg_state *s
     if (own) (void)hipStreamDestroy(st);
     return rc;
 }
+
+}  // namespace rsg
 
 extern "C" int rsg_repeat_step(int kind, int which, const rsg_grid *g, const rsg_state *s, int n,
                                void *stream) {

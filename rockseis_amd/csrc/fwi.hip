@@ -164,33 +164,35 @@ Revolve::Action Revolve::next() {
 // ---------------------------------------------------------------------------
 
 // crossCorr for one reverse step, interior ix, iz in [1, n-1) of the
-// unpadded nx x nz image (fwi.cpp:550-569); Vx_a, Vz_a, Rx, Rz padded by
-// lpml.  The rho-gradient velocity terms keep the reference's double
-// temporaries.  A batch of nshot shots (fields fstride apart) is summed into
-// the one image, shot 0 first.
-// Pf is either the padded forward pressure (pf_pad = lpml) or a stored
+// nx x nz image (fwi.cpp:550-569); Vx_a, Vz_a, Rx, Rz padded by padr (lpml,
+// or 0 on a domain model, whose image is the whole padded domain:
+// fwi.cpp:535-544).  The rho-gradient velocity terms keep the reference's
+// double temporaries.  A batch of nshot shots (fields fstride apart) is
+// summed into the one image, shot 0 first.
+// Pf is either the padded forward pressure (pf_pad = padr) or a stored
 // unpadded snapshot (pf_pad = 0), FP32 or (F16) FP16 scaled by amax[shot];
-// pf_stride separates its shots.
+// pf_stride separates its shots.  srcilum (optional) takes the source
+// illumination -(-2/Vp) Pf^2 (fwi.cpp:567-569).
 template <bool F16>
 __global__ void crosscorr_a2d_kernel(const void *__restrict__ Pf_, const unsigned *__restrict__ amax,
                                      const float *__restrict__ Vxa, const float *__restrict__ Vza,
                                      const float *__restrict__ Rx, const float *__restrict__ Rz,
                                      const float *__restrict__ Vp, const float *__restrict__ Rho, int nx,
-                                     int nz, int lpml, float dx, float dz, float *__restrict__ vpgrad,
-                                     float *__restrict__ rhograd, int nshot, size_t fstride, int pf_pad,
-                                     size_t pf_stride) {
+                                     int nz, int padr, float dx, float dz, float *__restrict__ vpgrad,
+                                     float *__restrict__ rhograd, float *__restrict__ srcilum, int nshot,
+                                     size_t fstride, int pf_pad, size_t pf_stride) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int iz = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (ix >= nx - 1 || iz >= nz - 1) return;
-    const int nxp = nx + 2 * lpml;
+    const int nxp = nx + 2 * padr;
     const size_t m = (size_t)iz * nx + ix;
-    const size_t q = (size_t)(iz + lpml) * nxp + ix + lpml;
+    const size_t q = (size_t)(iz + padr) * nxp + ix + padr;
     const int pld = nx + 2 * pf_pad;
     const size_t qp = (size_t)(iz + pf_pad) * pld + ix + pf_pad;
     const float vpscale = (float)(-2.0 / (double)Vp[m]);
     const float rhoscale = (float)(-1.0 / (double)Rho[m]);
     const double rx0 = Rx[q], rx1 = Rx[q - 1], rz0 = Rz[q], rz1 = Rz[q - nxp];
-    float vg = vpgrad[m], rg = rhograd[m];
+    float vg = vpgrad[m], rg = rhograd[m], si = srcilum ? srcilum[m] : 0.f;
     for (int k = 0; k < nshot; k++) {
         float pc, pe, pw, ps, pn;
         if constexpr (F16) {
@@ -219,9 +221,11 @@ __global__ void crosscorr_a2d_kernel(const void *__restrict__ Pf_, const unsigne
         float uz = (float)(0.5 * (double)Vz[q] * rz0 * (double)(ps - pc) / (double)dz);
         uz = (float)((double)uz + 0.5 * (double)Vz[q - nxp] * rz1 * (double)(pc - pn) / (double)dz);
         rg += ux + uz;
+        if (srcilum) si -= vpscale * pc * pc;
     }
     vpgrad[m] = vg;
     rhograd[m] = rg;
+    if (srcilum) srcilum[m] = si;
 }
 
 // ---------------------------------------------------------------------------
@@ -289,8 +293,15 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
     const int nt = f->src.nt;
     if (f->obs.nt != nt || f->obs.field != RSG_PRESSURE || f->src.field != RSG_PRESSURE)
         return fail(RSG_EINVAL, "fwi: source and data must be PRESSURE traces with equal nt");
-    if (f->nx + 2 * g->lpml != g->nx || f->nz + 2 * g->lpml != g->nz)
-        return fail(RSG_EINVAL, "fwi: image size must be the unpadded grid");
+    // image offset on the grid: lpml, or none on a domain model (fwi.cpp:535-544)
+    const int pad = g->domdec ? 0 : g->lpml;
+    if (f->nx + 2 * pad != g->nx || f->nz + 2 * pad != g->nz)
+        return fail(RSG_EINVAL, g->domdec ? "fwi: image size must be the grid of the domain model"
+                                          : "fwi: image size must be the unpadded grid");
+    if (g->domdec && f->adjoint == RSG_ADJOINT_EXACT)
+        return fail(RSG_EINVAL, "fwi: the exact adjoint takes models padded by the library (no domain model)");
+    if (f->srcilum && f->adjoint == RSG_ADJOINT_EXACT)
+        return fail(RSG_EINVAL, "fwi: source illumination is computed by the reference imaging condition only");
     if (f->mode == RSG_FWI_FULL && f->snapinc < 1) return fail(RSG_EINVAL, "fwi: snapinc must be >= 1");
     if (f->adjoint != RSG_ADJOINT_REFERENCE && f->adjoint != RSG_ADJOINT_EXACT)
         return fail(RSG_EINVAL, "fwi: adjoint must be RSG_ADJOINT_REFERENCE or RSG_ADJOINT_EXACT");
@@ -317,7 +328,7 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
     wg.data = f->wavgrad;
     OrderedInject2D oi;
     if (f->obs.ntrace > 0)
-        if (int rc = oi.build(db, f->obs.map, f->obs.ntrace, ns, g->nx, g->lpml)) return rc;
+        if (int rc = oi.build(db, f->obs.map, f->obs.ntrace, ns, g->nx, g->nz, map_pad(g), st)) return rc;
     const dim3 xb(64, 4), xg((f->nx + 63) / 64, (f->nz + 3) / 4);
     const size_t fs = fw.n;
 
@@ -349,13 +360,14 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
         if (Pf) {
             if (amax)
                 crosscorr_a2d_kernel<true><<<xg, xb, 0, st>>>(Pf, amax, bw.st.Vx, bw.st.Vz, f->Rx, f->Rz, f->Vp,
-                                                              f->Rho, f->nx, f->nz, g->lpml, g->dx, g->dz,
-                                                              f->vpgrad, f->rhograd, ns, fs, pf_pad, pf_stride);
+                                                              f->Rho, f->nx, f->nz, pad, g->dx, g->dz,
+                                                              f->vpgrad, f->rhograd, f->srcilum, ns, fs, pf_pad,
+                                                              pf_stride);
             else
                 crosscorr_a2d_kernel<false><<<xg, xb, 0, st>>>(Pf, nullptr, bw.st.Vx, bw.st.Vz, f->Rx, f->Rz,
-                                                               f->Vp, f->Rho, f->nx, f->nz, g->lpml, g->dx,
-                                                               g->dz, f->vpgrad, f->rhograd, ns, fs, pf_pad,
-                                                               pf_stride);
+                                                               f->Vp, f->Rho, f->nx, f->nz, pad, g->dx,
+                                                               g->dz, f->vpgrad, f->rhograd, f->srcilum, ns, fs,
+                                                               pf_pad, pf_stride);
             RSG_LAUNCH_CHECK("crosscorr_a2d_kernel");
         }
         if (f->wavgrad)
@@ -389,7 +401,7 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
     auto snap_at = [&](int k) -> void * { return hist + slot * (size_t)k * esz; };
     auto amax_at = [&](int k) -> unsigned * { return f16 ? hamax + (size_t)ns * k : nullptr; };
     auto take_snap = [&](int k) -> int {
-        return pack_interior(fw.st.P, snap_at(k), amax_at(k), f->nx, f->nz, g->lpml, fw.n, ns, f16, st);
+        return pack_interior(fw.st.P, snap_at(k), amax_at(k), f->nx, f->nz, pad, fw.n, ns, f16, st);
     };
 
     if (mode == RSG_FWI_STORE) {
@@ -406,7 +418,7 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
                 if (int rc = reverse_tail(capo, snap_at(capo - 1), 0, nint, amax_at(capo - 1))) return rc;
             } else {
                 if (int rc = launch_zero(fw.st.P, (size_t)ns * fw.n * sizeof(float), st)) return rc;
-                if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n, nullptr)) return rc;
+                if (int rc = reverse_tail(capo, fw.st.P, pad, fw.n, nullptr)) return rc;
             }
         }
     } else if (mode == RSG_FWI_FULL) {
@@ -492,7 +504,7 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
                     RSG_HIP(hipMemcpyAsync(fw.base, turn_state, fw.total * sizeof(float), hipMemcpyDeviceToDevice,
                                            st));
                     if (int rc = exact_step(capo)) return rc;
-                } else if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n, nullptr)) {
+                } else if (int rc = reverse_tail(capo, fw.st.P, pad, fw.n, nullptr)) {
                     return rc;
                 }
                 reverse = true;
@@ -503,7 +515,7 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
                     continue;
                 }
                 if (int rc = bstep()) return rc;
-                if (int rc = reverse_tail(capo, fw.st.P, g->lpml, fw.n, nullptr)) return rc;
+                if (int rc = reverse_tail(capo, fw.st.P, pad, fw.n, nullptr)) return rc;
             } else if (a == Revolve::TAKESHOT) {
                 if (f16) {
                     if (int rc = ck_pack(rev.check)) return rc;
@@ -558,6 +570,22 @@ extern "C" int rsg_fwi_a2d_gradient_batch(const rsg_grid *g, const rsg_fwi_a2d *
     return rsg::fwi_a2d_impl(g, f, nshot, (hipStream_t)stream);
 }
 
+extern "C" int rsg_crosscorr_a2d(const float *wsp, int pads, const float *wrx, const float *wrz, int padr,
+                                 const float *Vp, const float *Rho, const float *Rx, const float *Rz, int nx, int nz,
+                                 float dx, float dz, int domdec, float *vpgrad, float *rhograd, float *srcilum,
+                                 void *stream) {
+    if (!wsp || !wrx || !wrz || !Vp || !Rho || !Rx || !Rz || !vpgrad || !rhograd || nx < 1 || nz < 1 || pads < 0 ||
+        padr < 0)
+        return rsg::fail(RSG_EINVAL, "rsg_crosscorr_a2d: bad arguments");
+    if (domdec) pads = padr = 0;
+    if (nx < 3 || nz < 3) return RSG_OK;  // no interior cell
+    const dim3 xb(64, 4), xg((nx + 63) / 64, (nz + 3) / 4);
+    rsg::crosscorr_a2d_kernel<false><<<xg, xb, 0, (hipStream_t)stream>>>(
+        wsp, nullptr, wrx, wrz, Rx, Rz, Vp, Rho, nx, nz, padr, dx, dz, vpgrad, rhograd, srcilum, 1, 0, pads, 0);
+    RSG_LAUNCH_CHECK("crosscorr_a2d_kernel");
+    return RSG_OK;
+}
+
 extern "C" int rsg_revolve_schedule(int nt, int snaps, int *actions, int *capos, int *checks, int maxn,
                                     int *n) {
     if (!actions || !capos || !checks || !n || nt < 1) return rsg::fail(RSG_EINVAL, "bad arguments");
@@ -577,3 +605,41 @@ extern "C" int rsg_revolve_schedule(int nt, int snaps, int *actions, int *capos,
 }
 
 extern "C" int rsg_revolve_numforw(int nt, int snaps) { return rsg::Revolve::numforw(nt, snaps); }
+
+extern "C" int rsg_revolve_adjust(int nt) { return rsg::Revolve::adjust(nt); }
+
+extern "C" int rsg_revolve_init(rsg_revolve_state *st, int nt, int snaps, int *adjusted) {
+    if (!st || nt < 1) return rsg::fail(RSG_EINVAL, "rsg_revolve_init: bad arguments");
+    rsg::Revolve r(nt, snaps);
+    if (adjusted) *adjusted = r.snaps != snaps;
+    st->steps = r.steps;
+    st->snaps = r.snaps;
+    st->capo = r.capo;
+    st->fine = r.fine;
+    st->check = r.check;
+    st->turn = r.turn;
+    st->oldfine = r.oldfine;
+    for (int i = 0; i < rsg::Revolve::kCheckUp; i++) st->ch[i] = r.ch[i];
+    return RSG_OK;
+}
+
+extern "C" int rsg_revolve_next(rsg_revolve_state *st) {
+    if (!st) return rsg::Revolve::ERROR;
+    rsg::Revolve r(st->steps > 1 ? st->steps : 2, 2);
+    r.steps = st->steps;
+    r.snaps = st->snaps;
+    r.capo = st->capo;
+    r.fine = st->fine;
+    r.check = st->check;
+    r.turn = st->turn;
+    r.oldfine = st->oldfine;
+    r.ch.assign(st->ch, st->ch + rsg::Revolve::kCheckUp);
+    const int a = (int)r.next();
+    st->capo = r.capo;
+    st->fine = r.fine;
+    st->check = r.check;
+    st->turn = r.turn;
+    st->oldfine = r.oldfine;
+    for (int i = 0; i < rsg::Revolve::kCheckUp; i++) st->ch[i] = r.ch[i];
+    return a;
+}

@@ -171,7 +171,7 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
     wg.data = f->wavgrad;
     OrderedInject3D oi;
     if (f->obs.ntrace > 0)
-        if (int rc = oi.build(dm, f->obs.map, f->obs.ntrace, g->nx, g->ny, g->nz, lpml, st)) return rc;
+        if (int rc = oi.build(dm, f->obs.map, f->obs.ntrace, g->nx, g->ny, g->nz, map_pad(g), st)) return rc;
     const dim3 xb(64, 4), xg((f->nx - 1 + 63) / 64, (f->ny - 1 + 3) / 4, std::max(f->nz - 1, 1));
 
     auto fstep = [&](int it, bool rec) -> int {

@@ -260,4 +260,133 @@ int rsg_snap_cadence(int nt, float dt, int snapinc, int *snapnt, int *enddiff) {
     return RSG_OK;
 }
 
+
+// Model<T>::getLocalsize2d (model.cpp:271-340).  The extremes and offsets
+// are doubles, the window sizes go through rintf of the double quotients
+// (rintf converts its argument to float first), as in the reference.
+int rsg_local_window_2d(const float *coords, int ntrace, float dx, float ox, float aperture, int map,
+                        long long *start, long long *size) {
+    if (!coords || ntrace < 1 || !start || !size || !(dx > 0.f))
+        return rsg::fail(RSG_EINVAL, "rsg_local_window_2d: bad arguments");
+    auto sx = [&](int i) { return (double)coords[4 * i]; };
+    auto gx = [&](int i) { return (double)coords[4 * i + 2]; };
+    size_t sz = 0;
+    long long st = 0;
+    if (aperture >= 0) {
+        double mn = map == 0 ? sx(0) : gx(0), mx = mn, off = std::fabs(gx(0) - sx(0));
+        for (int i = 1; i < ntrace; i++) {
+            const double v = map == 0 ? sx(i) : gx(i);
+            if (v < mn) mn = v;
+            if (v > mx) mx = v;
+            if (std::fabs(gx(i) - sx(i)) > off) off = std::fabs(gx(i) - sx(i));
+        }
+        if (aperture > 0) {
+            sz = (size_t)(rintf((float)((mx - mn + aperture) / dx)) + 1);
+            if (sz % 2 == 0) sz = sz + 1;
+        } else {
+            sz = (size_t)(2 * rintf((float)(off / dx)) + 1);
+        }
+        st = (long long)(rintf((float)((mn - ox) / dx)) - (float)((sz - 1) / 2));
+    } else {
+        double mn = sx(0), mx = sx(0);
+        for (int i = 0; i < ntrace; i++) {
+            if (sx(i) < mn) mn = sx(i);
+            if (sx(i) > mx) mx = sx(i);
+            if (gx(i) < mn) mn = gx(i);
+            if (gx(i) > mx) mx = gx(i);
+        }
+        const double da = aperture;
+        sz = (size_t)(rintf((float)((mx - mn + 2 * std::fabs(da)) / dx)) + 2);
+        st = (long long)(rintf((float)((mn - ox) / dx)) - rintf((float)std::fabs(da / dx))) - 1;
+    }
+    *start = st;
+    *size = (long long)sz;
+    return RSG_OK;
+}
+
+namespace {
+// Clamped source column / row of the global model for a padded domain cell
+// (the lpos clamps of model.cpp:1573-1590).
+inline long long clampi(long long v, long long n) { return v < 0 ? 0 : (v > n - 1 ? n - 1 : v); }
+}  // namespace
+
+int rsg_domain_model_a2d(const float *vp, const float *rho, int nx, int nz, int lpml, int fs, float dx,
+                         float dz, float ox, float oz, long long start, long long size, float *Vp, float *R,
+                         float *L, float *Rx, float *Rz, float *ox_out, float *oz_out) {
+    if (!vp || !rho || !Vp || !R || !L || !Rx || !Rz || nx < 1 || nz < 1 || lpml < 0 || size < 1)
+        return rsg::fail(RSG_EINVAL, "rsg_domain_model_a2d: bad arguments");
+    const long long nxd = size + 2 * lpml, nzd = nz + 2 * (long long)lpml;
+    for (long long i1 = 0; i1 < nzd; i1++) {
+        const float *vrow = vp + clampi(i1 - lpml, nz) * nx;
+        const float *rrow = rho + clampi(i1 - lpml, nz) * nx;
+        const float *radv = rho + clampi(i1 - lpml + 1, nz) * nx;
+        for (long long i2 = 0; i2 < nxd; i2++) {
+            const long long lp = clampi(start + i2 - lpml, nx);
+            const size_t q = (size_t)(i1 * nxd + i2);
+            const float r = rrow[lp], v = vrow[lp];
+            if (r <= 0.0f) return rsg::fail(RSG_EINVAL, "getDomainmodel: Zero density found.");
+            Vp[q] = v;
+            R[q] = r;
+            L[q] = r * v * v;
+            Rx[q] = lp < nx - 1 ? (float)(2.0 / (double)(r + rrow[lp + 1])) : (float)(1.0 / (double)r);
+            Rz[q] = (float)(2.0 / (double)(r + radv[lp]));
+        }
+    }
+    if (fs)
+        for (long long x = 0; x < nxd; x++) {
+            Rx[(size_t)(lpml * nxd + x)] *= 2.0f;
+            L[(size_t)(lpml * nxd + x)] *= 0.0f;
+        }
+    if (ox_out) *ox_out = ox + (float)(start - lpml) * dx;
+    if (oz_out) *oz_out = oz + (float)(-lpml) * dz;
+    return RSG_OK;
+}
+
+int rsg_domain_model_e2d(const float *vp, const float *vs, const float *rho, int nx, int nz, int lpml, int fs,
+                         float dx, float dz, float ox, float oz, long long start, long long size, float *Vp,
+                         float *Vs, float *R, float *L, float *L2M, float *M, float *Rx, float *Rz,
+                         float *ox_out, float *oz_out) {
+    if (!vp || !vs || !rho || !Vp || !Vs || !R || !L || !L2M || !M || !Rx || !Rz || nx < 1 || nz < 1 ||
+        lpml < 0 || size < 1)
+        return rsg::fail(RSG_EINVAL, "rsg_domain_model_e2d: bad arguments");
+    const long long nxd = size + 2 * lpml, nzd = nz + 2 * (long long)lpml;
+    for (long long i1 = 0; i1 < nzd; i1++) {
+        const long long z0 = clampi(i1 - lpml, nz), z1 = clampi(i1 - lpml + 1, nz);
+        const float *vprow = vp + z0 * nx, *vsrow = vs + z0 * nx, *rrow = rho + z0 * nx;
+        const float *vsadv = vs + z1 * nx, *radv = rho + z1 * nx;
+        for (long long i2 = 0; i2 < nxd; i2++) {
+            const long long lp = clampi(start + i2 - lpml, nx);
+            const size_t q = (size_t)(i1 * nxd + i2);
+            const float r = rrow[lp], v = vprow[lp], w = vsrow[lp];
+            if (r <= 0.0f) return rsg::fail(RSG_EINVAL, "getDomainmodel: Zero density found.");
+            Vp[q] = v;
+            Vs[q] = w;
+            R[q] = r;
+            L2M[q] = r * v * v;
+            L[q] = (float)((double)(r * v * v) - 2.0 * (double)r * (double)w * (double)w);
+            Rx[q] = lp < nx - 1 ? (float)(2.0 / (double)(r + rrow[lp + 1])) : (float)(1.0 / (double)r);
+            Rz[q] = (float)(2.0 / (double)(r + radv[lp]));
+            const float m1 = r * w * w, m3 = radv[lp] * vsadv[lp] * vsadv[lp];
+            const long long l2 = lp < nx - 1 ? lp + 1 : lp;
+            const float m2 = rrow[l2] * vsrow[l2] * vsrow[l2], m4 = radv[l2] * vsadv[l2] * vsadv[l2];
+            M[q] = (float)(0.25 * (double)(m1 + m2 + m3 + m4));
+        }
+    }
+    if (fs)
+        for (long long x = 0; x < nxd; x++) {
+            const size_t q = (size_t)(lpml * nxd + x);
+            Rx[q] *= 2.0f;
+            L2M[q] = (float)((double)(L2M[q] - L[q]) / 2.0);
+            L[q] = 0.0f;
+        }
+    if (ox_out) *ox_out = ox + (float)(start - lpml) * dx;
+    if (oz_out) *oz_out = oz + (float)(-lpml) * dz;
+    return RSG_OK;
+}
+
+int rsg_image_offset(float ol, float dl, float og, float dg) {
+    const volatile float a = ol / dl, b = og / dg;
+    return (int)(a - b);
+}
+
 }  // extern "C"

@@ -11,6 +11,11 @@
 
 namespace rsg {
 
+// Offset of source/receiver maps on the grid: lpml, or 0 for a domain model
+// whose arrays already hold the padding (Waves*::insertSource/recordData
+// take lpml = 0 when getDomdec(), waves.cpp:411-418).
+inline int map_pad(const rsg_grid *g) { return g->domdec ? 0 : g->lpml; }
+
 // ---- error state ----------------------------------------------------------
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);

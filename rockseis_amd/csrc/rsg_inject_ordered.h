@@ -23,10 +23,26 @@ namespace {
 struct OrderedInject2D {
     const int *cells = nullptr, *starts = nullptr, *ents = nullptr, *coff = nullptr;
     int maxcells = 0;
+    // map_dev is read on stream st (the caller may have written it there);
+    // nxp x nzp is the padded grid, pad the map offset (lpml, or 0 on a
+    // domain model).  A receiver whose 2x2 cells leave the grid is an error.
     template <class Mem>
-    int build(Mem &db, const int *map_dev, int ntr, int ns, int nxp, int lpml) {
+    int build(Mem &db, const int *map_dev, int ntr, int ns, int nxp, int nzp, int pad, hipStream_t st) {
         std::vector<int> mp(2 * (size_t)ntr * ns), hc, hs, he, ho(ns + 1, 0);
-        if (!mp.empty()) RSG_HIP(hipMemcpy(mp.data(), map_dev, sizeof(int) * mp.size(), hipMemcpyDeviceToHost));
+        if (!mp.empty()) {
+            RSG_HIP(hipMemcpyAsync(mp.data(), map_dev, sizeof(int) * mp.size(), hipMemcpyDeviceToHost, st));
+            RSG_HIP(hipStreamSynchronize(st));
+        }
+        // a receiver on the last column of a domain model puts its x+1
+        // corners on the next row's first cell, as the reference's flat
+        // index does; only cells past the array are an error
+        for (size_t q = 0; q < mp.size(); q += 2) {
+            const int mx = mp[q], mz = mp[q + 1];
+            if (mx < 0 || mz < 0) continue;
+            if (((long)pad + mz + 1) * nxp + pad + mx + 1 >= (long)nxp * nzp)
+                return fail(RSG_EINVAL, "2D injection: receiver map outside the padded grid");
+        }
+        const int lpml = pad;
         for (int k = 0; k < ns; k++) {
             std::vector<std::pair<int, int>> e;  // (cell, trace*4 + corner)
             e.reserve(4 * (size_t)ntr);
@@ -58,7 +74,9 @@ struct OrderedInject2D {
         all.insert(all.end(), ho.begin(), ho.end());
         int *d = nullptr;
         if (int rc = db.alloc(&d, all.size())) return rc;
-        RSG_HIP(hipMemcpy(d, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice));
+        RSG_HIP(hipMemcpyAsync(d, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice, st));
+        // the host vector goes out of scope with this call
+        RSG_HIP(hipStreamSynchronize(st));
         cells = d;
         starts = cells + hc.size();
         ents = starts + hs.size();
@@ -114,7 +132,7 @@ struct OrderedInject3D {
         for (int i = 0; i < ntr; i++) {
             const int mx = mp[3 * (size_t)i], my = mp[3 * (size_t)i + 1], mz = mp[3 * (size_t)i + 2];
             if (mx < 0 || my < 0 || mz < 0) continue;
-            if (lpml + mx + 1 >= nx || lpml + my + 1 >= ny || lpml + mz + 1 >= nz)
+            if ((size_t)(lpml + mz + 1) * nxy + (size_t)(lpml + my + 1) * nx + (lpml + mx + 1) >= nxy * nz)
                 return fail(RSG_EINVAL, "3D injection: receiver map outside the padded grid");
             for (int i1 = 0; i1 < 2; i1++)
                 for (int i2 = 0; i2 < 2; i2++)

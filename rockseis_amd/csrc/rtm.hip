@@ -221,7 +221,7 @@ static int rtm_a2d_batch_impl(const rsg_grid *g, const rsg_rtm_a2d *r, int ns, h
     if (int rc = dm.alloc(&ck, total * (size_t)rev.snaps)) return rc;
     OrderedInject2D oi;
     if (r->data.ntrace > 0)
-        if (int rc = oi.build(dm, r->data.map, r->data.ntrace, ns, g->nx, g->lpml)) return rc;
+        if (int rc = oi.build(dm, r->data.map, r->data.ntrace, ns, g->nx, g->nz, map_pad(g), st)) return rc;
     const size_t nimg = (size_t)r->nhx * r->nhz * r->nx * r->nz, fs = (size_t)g->nx * g->nz;
     if (int rc = launch_zero(r->image, nimg * sizeof(float), st)) return rc;
     const dim3 xb(64, 4), xg((r->nx + 63) / 64, (r->nz + 3) / 4, r->nhx * r->nhz);
@@ -296,7 +296,7 @@ static int rtm_a2d_impl(const rsg_grid *g, const rsg_rtm_a2d *r, hipStream_t st)
     if (int rc = dm.alloc(&ck, total * (size_t)rev.snaps)) return rc;
     OrderedInject2D oi;
     if (r->data.ntrace > 0)
-        if (int rc = oi.build(dm, r->data.map, r->data.ntrace, 1, g->nx, g->lpml)) return rc;
+        if (int rc = oi.build(dm, r->data.map, r->data.ntrace, 1, g->nx, g->nz, map_pad(g), st)) return rc;
     // RtmAcoustic2D::run_optimal allocates (zeroes) the image before the run
     const size_t nimg = (size_t)r->nhx * r->nhz * r->nx * r->nz;
     if (int rc = launch_zero(r->image, nimg * sizeof(float), st)) return rc;
@@ -377,7 +377,7 @@ static int rtm_a3d_impl(const rsg_grid *g, const rsg_rtm_a3d *r, hipStream_t st)
     if (int rc = dm.alloc(&ck, total * (size_t)rev.snaps)) return rc;
     OrderedInject3D oi;
     if (r->data.ntrace > 0)
-        if (int rc = oi.build(dm, r->data.map, r->data.ntrace, g->nx, g->ny, g->nz, g->lpml, st)) return rc;
+        if (int rc = oi.build(dm, r->data.map, r->data.ntrace, g->nx, g->ny, g->nz, map_pad(g), st)) return rc;
     const size_t nimg = (size_t)r->nhx * r->nhy * r->nhz * r->nx * r->ny * r->nz;
     if (int rc = launch_zero(r->image, nimg * sizeof(float), st)) return rc;
     const dim3 xb(64, 4), xg((r->nx + 63) / 64, (r->ny + 3) / 4, r->nz);

@@ -21,7 +21,7 @@ __device__ __forceinline__ int time_index(const int *it_dev, int it) {
 // from shot 0's and its traces follow shot 0's (maps, wavelets).
 __global__ void inject2d_kernel(float *F, float *F2, const float *Mod, float scale, int nx, int lpml,
                                 int ntrace, const int *map, const float *shift, const float *wav,
-                                int nt, const int *it_dev, int it0, size_t fstride) {
+                                int nt, const int *it_dev, int it0, size_t fstride, size_t ncell) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntrace) return;
     const int it = time_index(it_dev, it0);
@@ -29,13 +29,13 @@ __global__ void inject2d_kernel(float *F, float *F2, const float *Mod, float sca
     const int *m = map + k * 2 * ntrace;
     const float *sh = shift + k * 2 * ntrace;
     const float *w = wav + k * (size_t)ntrace * nt;
-    inject2d_trace(i, F + k * fstride, Mod, scale, nx, lpml, m, sh, w, nt, it);
-    if (F2) inject2d_trace(i, F2 + k * fstride, Mod, scale, nx, lpml, m, sh, w, nt, it);
+    inject2d_trace(i, F + k * fstride, Mod, scale, nx, lpml, m, sh, w, nt, it, ncell);
+    if (F2) inject2d_trace(i, F2 + k * fstride, Mod, scale, nx, lpml, m, sh, w, nt, it, ncell);
 }
 
 __device__ __forceinline__ void inject3d_trace(int i, float *F, const float *Mod, float scale, int nx,
                                                int ny, int lpml, const int *map, const float *shift,
-                                               const float *wav, int nt, int it) {
+                                               const float *wav, int nt, int it, size_t ncell) {
     const int mx = map[3 * i], my = map[3 * i + 1], mz = map[3 * i + 2];
     if (mx < 0 || my < 0 || mz < 0) return;
     const float sx = shift[3 * i], sy = shift[3 * i + 1], sz = shift[3 * i + 2];
@@ -50,6 +50,7 @@ __device__ __forceinline__ void inject3d_trace(int i, float *F, const float *Mod
             for (int i3 = 0; i3 < 2; i3++) {
                 const size_t p = (size_t)(lpml + mz + i1) * nxy + (size_t)(lpml + my + i2) * nx +
                                  (lpml + mx + i3);
+                if (p >= ncell) continue;
                 const float m = Mod ? scale * Mod[p] : scale;
                 unsafeAtomicAdd(F + p, m * w * xtri[i3] * ytri[i2] * ztri[i1]);
             }
@@ -57,26 +58,26 @@ __device__ __forceinline__ void inject3d_trace(int i, float *F, const float *Mod
 
 __global__ void inject3d_kernel(float *F, const float *Mod, float scale, int nx, int ny, int lpml,
                                 int ntrace, const int *map, const float *shift, const float *wav,
-                                int nt, const int *it_dev, int it0) {
+                                int nt, const int *it_dev, int it0, size_t ncell) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntrace) return;
-    inject3d_trace(i, F, Mod, scale, nx, ny, lpml, map, shift, wav, nt, time_index(it_dev, it0));
+    inject3d_trace(i, F, Mod, scale, nx, ny, lpml, map, shift, wav, nt, time_index(it_dev, it0), ncell);
 }
 
 __global__ void record2d_kernel(const float *F, const float *F2, const float *factor, int nx,
                                 int lpml, int ntrace, const int *map, const float *shift,
-                                float *data, int nt, const int *it_dev, int it0, size_t fstride) {
+                                float *data, int nt, const int *it_dev, int it0, size_t fstride, size_t ncell) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntrace) return;
     const size_t k = blockIdx.y;
     record2d_trace(i, F + k * fstride, F2 ? F2 + k * fstride : nullptr, factor ? factor + k * ntrace : nullptr,
                    nx, lpml, map + k * 2 * ntrace, shift + k * 2 * ntrace, data + k * (size_t)ntrace * nt, nt,
-                   time_index(it_dev, it0));
+                   time_index(it_dev, it0), ncell);
 }
 
 __device__ __forceinline__ void record3d_trace(int i, const float *F, int nx, int ny, int lpml,
                                                const int *map, const float *shift, float *data,
-                                               int nt, int it) {
+                                               int nt, int it, size_t ncell) {
     const int mx = map[3 * i], my = map[3 * i + 1], mz = map[3 * i + 2];
     if (mx < 0 || my < 0 || mz < 0) return;
     const float sx = shift[3 * i], sy = shift[3 * i + 1], sz = shift[3 * i + 2];
@@ -89,17 +90,18 @@ __device__ __forceinline__ void record3d_trace(int i, const float *F, int nx, in
 #pragma unroll
         for (int i2 = 0; i2 < 2; i2++)
 #pragma unroll
-            for (int i3 = 0; i3 < 2; i3++)
-                acc += xtri[i3] * ytri[i2] * ztri[i1] *
-                       F[(size_t)(lpml + mz + i1) * nxy + (size_t)(lpml + my + i2) * nx + (lpml + mx + i3)];
+            for (int i3 = 0; i3 < 2; i3++) {
+                const size_t p = (size_t)(lpml + mz + i1) * nxy + (size_t)(lpml + my + i2) * nx + (lpml + mx + i3);
+                if (p < ncell) acc += xtri[i3] * ytri[i2] * ztri[i1] * F[p];
+            }
     *d = acc;
 }
 
 __global__ void record3d_kernel(const float *F, int nx, int ny, int lpml, int ntrace, const int *map,
-                                const float *shift, float *data, int nt, const int *it_dev, int it0) {
+                                const float *shift, float *data, int nt, const int *it_dev, int it0, size_t ncell) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntrace) return;
-    record3d_trace(i, F, nx, ny, lpml, map, shift, data, nt, time_index(it_dev, it0));
+    record3d_trace(i, F, nx, ny, lpml, map, shift, data, nt, time_index(it_dev, it0), ncell);
 }
 
 // All receiver sets of a step in one launch: lane g of the grid gathers
@@ -114,6 +116,7 @@ struct RecDesc {
 constexpr int kMaxFusedRec = 4;
 struct RecSets {
     int dim, nx, ny, lpml, nrec;
+    size_t ncell;
     int first[kMaxFusedRec + 1];
     RecDesc rec[kMaxFusedRec];
 };
@@ -129,9 +132,9 @@ __global__ void record_sets_kernel(const RecSets rs, const int *it_dev, int it0)
     const int i = g - rs.first[r];
     const int it = time_index(it_dev, it0);
     if (rs.dim == 3)
-        record3d_trace(i, rd.F, rs.nx, rs.ny, rs.lpml, rd.map, rd.shift, rd.data, rd.nt, it);
+        record3d_trace(i, rd.F, rs.nx, rs.ny, rs.lpml, rd.map, rd.shift, rd.data, rd.nt, it, rs.ncell);
     else
-        record2d_trace(i, rd.F, rd.F2, rd.factor, rs.nx, rs.lpml, rd.map, rd.shift, rd.data, rd.nt, it);
+        record2d_trace(i, rd.F, rd.F2, rd.factor, rs.nx, rs.lpml, rd.map, rd.shift, rd.data, rd.nt, it, rs.ncell);
 }
 
 __global__ void add_int_kernel(int *p, int v) {
@@ -204,7 +207,8 @@ int step_io(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *s
     rs.dim = kind == RSG_KIND_A3D ? 3 : 2;
     rs.nx = g->nx;
     rs.ny = g->ny;
-    rs.lpml = g->lpml;
+    rs.lpml = map_pad(g);
+    rs.ncell = (size_t)g->nx * g->ny * g->nz;
     rs.nrec = 0;
     rs.first[0] = 0;
     for (int r = 0; r < nrec; r++) {
@@ -244,12 +248,14 @@ int insert_source(int kind, const rsg_grid *g, const rsg_state *s, const rsg_tra
     const int threads = 64, blocks = (src->ntrace + threads - 1) / threads;
     if (kind == RSG_KIND_A3D) {
         if (nshot != 1) return fail(RSG_EINVAL, "3D batches are not supported");
-        inject3d_kernel<<<blocks, threads, 0, st>>>(F, Mod, scale, g->nx, g->ny, g->lpml, src->ntrace,
-                                                    src->map, src->shift, src->data, src->nt, it_dev, it);
+        inject3d_kernel<<<blocks, threads, 0, st>>>(F, Mod, scale, g->nx, g->ny, map_pad(g), src->ntrace,
+                                                    src->map, src->shift, src->data, src->nt, it_dev, it,
+                                                    (size_t)g->nx * g->ny * g->nz);
     } else {
-        inject2d_kernel<<<dim3(blocks, nshot), threads, 0, st>>>(F, F2, Mod, scale, g->nx, g->lpml,
+        inject2d_kernel<<<dim3(blocks, nshot), threads, 0, st>>>(F, F2, Mod, scale, g->nx, map_pad(g),
                                                                  src->ntrace, src->map, src->shift, src->data,
-                                                                 src->nt, it_dev, it, fstride);
+                                                                 src->nt, it_dev, it, fstride,
+                                                                 (size_t)g->nx * g->nz);
     }
     RSG_LAUNCH_CHECK("inject kernel");
     return RSG_OK;
@@ -264,12 +270,14 @@ int record_data(int kind, const rsg_grid *g, const rsg_state *s, const rsg_trace
     const int threads = 64, blocks = (rec->ntrace + threads - 1) / threads;
     if (kind == RSG_KIND_A3D && nshot != 1) return fail(RSG_EINVAL, "3D batches are not supported");
     if (kind == RSG_KIND_A3D)
-        record3d_kernel<<<blocks, threads, 0, st>>>(F, g->nx, g->ny, g->lpml, rec->ntrace, rec->map,
-                                                    rec->shift, rec->data, rec->nt, it_dev, it);
+        record3d_kernel<<<blocks, threads, 0, st>>>(F, g->nx, g->ny, map_pad(g), rec->ntrace, rec->map,
+                                                    rec->shift, rec->data, rec->nt, it_dev, it,
+                                                    (size_t)g->nx * g->ny * g->nz);
     else
-        record2d_kernel<<<dim3(blocks, nshot), threads, 0, st>>>(F, F2, rec->factor, g->nx, g->lpml,
+        record2d_kernel<<<dim3(blocks, nshot), threads, 0, st>>>(F, F2, rec->factor, g->nx, map_pad(g),
                                                                  rec->ntrace, rec->map, rec->shift, rec->data,
-                                                                 rec->nt, it_dev, it, fstride);
+                                                                 rec->nt, it_dev, it, fstride,
+                                                                 (size_t)g->nx * g->nz);
     RSG_LAUNCH_CHECK("record kernel");
     return RSG_OK;
 }
