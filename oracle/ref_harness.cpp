@@ -545,4 +545,71 @@ int ref_revolve_adjust(int nt) {
     return r.adjust(nt);
 }
 
+// Model<T>::getLocalsize2d and ModelAcoustic2D / ModelElastic2D::
+// getDomainmodel(data, aperture, map, 0, 1, 1, order) for one domain, on the
+// model files vpfile / rhofile (/ vsfile) written by the caller.  coords
+// [ntr][4] = sx, sz, gx, gz.  With the output arrays null only the sizes are
+// returned: out_i = {start, size, nxd, nzd}, out_f = {ox, oz} of the domain.
+int ref_domain_model(const char *vpfile, const char *vsfile, const char *rhofile, int lpml, int fs,
+                     const float *coords, int ntr, float aperture, int map, int order, long long *out_i,
+                     float *out_f, float *Vp, float *Vs, float *R, float *L, float *L2M, float *M, float *Rx,
+                     float *Rz) {
+    auto d = std::make_shared<Data2D<float>>(ntr, 1, 1.0f, 0.0f);
+    Point2D<float> *sc = d->getGeom()->getScoords(), *gc = d->getGeom()->getGcoords();
+    for (int i = 0; i < ntr; i++) {
+        sc[i].x = coords[4 * i];
+        sc[i].y = coords[4 * i + 1];
+        gc[i].x = coords[4 * i + 2];
+        gc[i].y = coords[4 * i + 3];
+    }
+    off_t start = 0;
+    size_t size = 0;
+    const bool m = map != 0;
+    if (vsfile && vsfile[0]) {
+        auto g = std::make_shared<ModelElastic2D<float>>(vpfile, vsfile, rhofile, lpml, fs);
+        g->getLocalsize2d(d, aperture, m, &start, &size);
+        auto l = g->getDomainmodel(d, aperture, m, 0, 1, 1, order);
+        const size_t n = (size_t)l->getNx() * l->getNz();
+        out_i[0] = start;
+        out_i[1] = (long long)size;
+        out_i[2] = l->getNx();
+        out_i[3] = l->getNz();
+        out_f[0] = l->getOx();
+        out_f[1] = l->getOz();
+        if (!Vp) return 0;
+        std::memcpy(Vp, l->getVp(), sizeof(float) * n);
+        std::memcpy(Vs, l->getVs(), sizeof(float) * n);
+        std::memcpy(R, l->getR(), sizeof(float) * n);
+        std::memcpy(L, l->getL(), sizeof(float) * n);
+        std::memcpy(L2M, l->getL2M(), sizeof(float) * n);
+        std::memcpy(M, l->getM(), sizeof(float) * n);
+        std::memcpy(Rx, l->getRx(), sizeof(float) * n);
+        std::memcpy(Rz, l->getRz(), sizeof(float) * n);
+        return 0;
+    }
+    auto g = std::make_shared<ModelAcoustic2D<float>>(vpfile, rhofile, lpml, fs);
+    g->getLocalsize2d(d, aperture, m, &start, &size);
+    auto l = g->getDomainmodel(d, aperture, m, 0, 1, 1, order);
+    const size_t n = (size_t)l->getNx() * l->getNz();
+    out_i[0] = start;
+    out_i[1] = (long long)size;
+    out_i[2] = l->getNx();
+    out_i[3] = l->getNz();
+    out_f[0] = l->getOx();
+    out_f[1] = l->getOz();
+    if (!Vp) return 0;
+    std::memcpy(Vp, l->getVp(), sizeof(float) * n);
+    std::memcpy(R, l->getR(), sizeof(float) * n);
+    std::memcpy(L, l->getL(), sizeof(float) * n);
+    std::memcpy(Rx, l->getRx(), sizeof(float) * n);
+    std::memcpy(Rz, l->getRz(), sizeof(float) * n);
+    return 0;
+}
+
+// Image2D::stackImage_parallel's start column: (int)(oxl/dxl - oxg/dxg) in T.
+int ref_image_offset(float ol, float dl, float og, float dg) {
+    float a = ol / dl, b = og / dg;
+    return (int)(a - b);
+}
+
 }  // extern "C"

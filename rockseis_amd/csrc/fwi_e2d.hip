@@ -63,12 +63,16 @@ __global__ void e2d_source_kernel(const ShotDev *__restrict__ sd, int it, int nt
     const int i = blockIdx.x * kIoThreads + threadIdx.x;
     if (i >= s.nsrc) return;
     if (s.sfield == RSG_PRESSURE) {
-        inject2d_trace(i, s.fSxx, nullptr, -dt, s.nxp, s.lpml, s.smap, s.sshift, s.wav, nt, it);
-        inject2d_trace(i, s.fSzz, nullptr, -dt, s.nxp, s.lpml, s.smap, s.sshift, s.wav, nt, it);
+        inject2d_trace(i, s.fSxx, nullptr, -dt, s.nxp, s.lpml, s.smap, s.sshift, s.wav, nt, it,
+                       (size_t)s.nxp * s.nzp);
+        inject2d_trace(i, s.fSzz, nullptr, -dt, s.nxp, s.lpml, s.smap, s.sshift, s.wav, nt, it,
+                       (size_t)s.nxp * s.nzp);
     } else if (s.sfield == RSG_VX) {
-        inject2d_trace(i, s.fVx, s.Rx, dt, s.nxp, s.lpml, s.smap, s.sshift, s.wav, nt, it);
+        inject2d_trace(i, s.fVx, s.Rx, dt, s.nxp, s.lpml, s.smap, s.sshift, s.wav, nt, it,
+                       (size_t)s.nxp * s.nzp);
     } else {
-        inject2d_trace(i, s.fVz, s.Rz, dt, s.nxp, s.lpml, s.smap, s.sshift, s.wav, nt, it);
+        inject2d_trace(i, s.fVz, s.Rz, dt, s.nxp, s.lpml, s.smap, s.sshift, s.wav, nt, it,
+                       (size_t)s.nxp * s.nzp);
     }
 }
 
@@ -78,9 +82,12 @@ __global__ void e2d_record_kernel(const ShotDev *__restrict__ sd, int it, int nt
     const ShotDev &s = sd[blockIdx.y];
     const int i = blockIdx.x * kIoThreads + threadIdx.x;
     if (i >= s.nrec) return;
-    record2d_trace(i, s.fSxx, s.fSzz, nullptr, s.nxp, s.lpml, s.gmap[0], s.gshift[0], s.dmod[0], nt, it);
-    record2d_trace(i, s.fVx, nullptr, nullptr, s.nxp, s.lpml, s.gmap[1], s.gshift[1], s.dmod[1], nt, it);
-    record2d_trace(i, s.fVz, nullptr, nullptr, s.nxp, s.lpml, s.gmap[2], s.gshift[2], s.dmod[2], nt, it);
+    record2d_trace(i, s.fSxx, s.fSzz, nullptr, s.nxp, s.lpml, s.gmap[0], s.gshift[0], s.dmod[0], nt, it,
+                   (size_t)s.nxp * s.nzp);
+    record2d_trace(i, s.fVx, nullptr, nullptr, s.nxp, s.lpml, s.gmap[1], s.gshift[1], s.dmod[1], nt, it,
+                   (size_t)s.nxp * s.nzp);
+    record2d_trace(i, s.fVz, nullptr, nullptr, s.nxp, s.lpml, s.gmap[2], s.gshift[2], s.dmod[2], nt, it,
+                   (size_t)s.nxp * s.nzp);
 }
 
 // The three residuals into the adjoint wavefield (insertSource of dataresP,
@@ -129,6 +136,7 @@ __global__ void e2d_wavgrad_kernel(const ShotDev *__restrict__ sd, int it, int n
     for (int i1 = 0; i1 < 2; i1++)
         for (int i2 = 0; i2 < 2; i2++) {
             const size_t q = (size_t)(mz + s.lpml + i1) * s.nxp + (mx + s.lpml + i2);
+            if (q >= (size_t)s.nxp * s.nzp) continue;
             const float v = s.sfield == RSG_PRESSURE ? s.bSxx[q] + s.bSzz[q]
                                                      : (s.sfield == RSG_VX ? s.bVx[q] : s.bVz[q]);
             acc += xtri[i2] * ytri[i1] * v;
@@ -240,6 +248,10 @@ inline bool e2d_xstrip(int i) { return (i / 4) % 2 == 0; }
 // the reverse sweep with its recomputations and stacks the windows' kernels.
 // Both phases only enqueue kernels and device-to-device copies on the
 // stream, so a plan captures each as a hipGraph once and replays it.
+// Map and image offset of a window: lpml, or 0 on a domain model whose
+// arrays already hold the padding (FwiElastic2D::crossCorr, fwi.cpp:2666-2675).
+inline int shot_pad(const rsg_grid &g) { return g.domdec ? 0 : g.lpml; }
+
 struct E2DFwi {
     const rsg_fwi_e2d *f = nullptr;
     const rsg_fwi_e2d_shot *sh = nullptr;
@@ -277,7 +289,7 @@ struct E2DFwi {
         // ---- validation
         for (int k = 0; k < ns; k++) {
             const rsg_fwi_e2d_shot &s = sh[k];
-            const int nx = s.g.nx - 2 * lpml, nz = s.g.nz - 2 * lpml;
+            const int pad = shot_pad(s.g), nx = s.g.nx - 2 * pad, nz = s.g.nz - 2 * pad;
             if (s.g.lpml != lpml || s.g.order != order || s.g.dt != dt || s.g.dx != g0.dx || s.g.dz != g0.dz ||
                 s.g.ny != 1)
                 return fail(RSG_EINVAL, "fwi_e2d: shots must share lpml, order, dx, dz, dt (2D grids)");
@@ -326,7 +338,10 @@ struct E2DFwi {
             size_t free_b = 0, total_b = 0;
             RSG_HIP(hipMemGetInfo(&free_b, &total_b));
             size_t img = 0;
-            for (int k = 0; k < ns; k++) img += 3 * (size_t)(sh[k].g.nx - 2 * lpml) * (sh[k].g.nz - 2 * lpml);
+            for (int k = 0; k < ns; k++) {
+                const int pad = shot_pad(sh[k].g);
+                img += 3 * (size_t)(sh[k].g.nx - 2 * pad) * (sh[k].g.nz - 2 * pad);
+            }
             const size_t need = total * 4 * 2 + img * 4 + total * (f16 ? 2 : 4) * (size_t)rev->snaps;
             if (need + ((size_t)1 << 30) > free_b)
                 return fail(RSG_ENOMEM, "fwi_e2d: states and checkpoints do not fit in device memory");
@@ -334,8 +349,10 @@ struct E2DFwi {
         if (int rc = dm.alloc(&fwb, total)) return rc;
         if (int rc = dm.alloc(&bwb, total)) return rc;
         img_off.assign(ns + 1, 0);
-        for (int k = 0; k < ns; k++)
-            img_off[k + 1] = img_off[k] + 3 * (size_t)(sh[k].g.nx - 2 * lpml) * (sh[k].g.nz - 2 * lpml);
+        for (int k = 0; k < ns; k++) {
+            const int pad = shot_pad(sh[k].g);
+            img_off[k + 1] = img_off[k] + 3 * (size_t)(sh[k].g.nx - 2 * pad) * (sh[k].g.nz - 2 * pad);
+        }
         if (int rc = dm.alloc(&imgs, img_off[ns])) return rc;
         // ---- per-shot states, ragged plans, kernel descriptors
         std::vector<rsg_grid> grids(ns);
@@ -369,9 +386,11 @@ struct E2DFwi {
             ShotDev &d = hd[k];
             d.nxp = s.g.nx;
             d.nzp = s.g.nz;
-            d.nx = s.g.nx - 2 * lpml;
-            d.nz = s.g.nz - 2 * lpml;
-            d.lpml = lpml;
+            // image window and map offset: the unpadded window, or the whole
+            // padded domain of a domain model (fwi.cpp:2666-2675)
+            d.nx = s.g.nx - 2 * shot_pad(s.g);
+            d.nz = s.g.nz - 2 * shot_pad(s.g);
+            d.lpml = shot_pad(s.g);
             d.ox = s.ox;
             d.fVx = sf[k].Vx;
             d.fVz = sf[k].Vz;
@@ -414,20 +433,25 @@ struct E2DFwi {
             std::vector<size_t> base(ns * 3 * 3);  // offsets of cells, starts, entries
             std::vector<int> mp;
             for (int k = 0; k < ns; k++) {
-                const int nrec = sh[k].obs[0].ntrace, nxp = sh[k].g.nx;
+                const int nrec = sh[k].obs[0].ntrace, nxp = sh[k].g.nx, pad = shot_pad(sh[k].g);
+                const long ncell = (long)sh[k].g.nx * sh[k].g.nz;
                 for (int c = 0; c < 3; c++) {
                     mp.resize(2 * (size_t)nrec);
-                    if (nrec)
-                        RSG_HIP(hipMemcpy(mp.data(), sh[k].obs[c].map, sizeof(int) * 2 * nrec,
-                                          hipMemcpyDeviceToHost));
+                    if (nrec) {  // the caller may have written the maps on st
+                        RSG_HIP(hipMemcpyAsync(mp.data(), sh[k].obs[c].map, sizeof(int) * 2 * nrec,
+                                               hipMemcpyDeviceToHost, st));
+                        RSG_HIP(hipStreamSynchronize(st));
+                    }
                     std::vector<std::pair<int, int>> ent;  // (cell, trace*4 + corner)
                     ent.reserve(4 * (size_t)nrec);
                     for (int i = 0; i < nrec; i++) {
                         const int mx = mp[2 * i], mz = mp[2 * i + 1];
                         if (mx < 0 || mz < 0) continue;
+                        if ((long)(pad + mz + 1) * nxp + pad + mx + 1 >= ncell)
+                            return fail(RSG_EINVAL, "fwi_e2d: receiver map outside the padded grid");
                         for (int i1 = 0; i1 < 2; i1++)
                             for (int i2 = 0; i2 < 2; i2++)
-                                ent.emplace_back((lpml + mz + i1) * nxp + (lpml + mx + i2), i * 4 + i1 * 2 + i2);
+                                ent.emplace_back((pad + mz + i1) * nxp + (pad + mx + i2), i * 4 + i1 * 2 + i2);
                     }
                     std::stable_sort(ent.begin(), ent.end(),
                                      [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
@@ -455,7 +479,8 @@ struct E2DFwi {
             }
             int *dtab = nullptr;
             if (int rc = dm.alloc(&dtab, tab.size())) return rc;
-            RSG_HIP(hipMemcpy(dtab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+            RSG_HIP(hipMemcpyAsync(dtab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice, st));
+            RSG_HIP(hipStreamSynchronize(st));  // the host table goes out of scope
             for (int k = 0; k < ns; k++)
                 for (int c = 0; c < 3; c++) {
                     const size_t b = 9 * (size_t)k + 3 * c;

@@ -422,6 +422,64 @@ GROUPS = dict(der=der_fixture, pml=pml_fixture, stagger=stagger_fixture, makemap
               fwi_e2d_misfit=fwi_e2d_misfit_fixtures, fwi_a3d_misfit=fwi_a3d_misfit_fixtures, rtm=rtm_fixtures,
               rtm_a3d=rtm_a3d_fixtures)
 
+DOMDEC_CASES = [  # elastic, fs, map (0 SMAP, 1 GMAP), aperture, shot x, receiver x0, dx of the spread
+    (0, 0, 0, 400.0, 230.0, 13.0, 20.0), (0, 1, 1, 150.0, 355.0, 101.0, 7.5), (0, 0, 0, 0.0, 230.0, 40.0, 25.0),
+    (0, 1, 0, -120.0, 90.0, 3.0, 30.0), (1, 1, 0, 400.0, 230.0, 13.0, 20.0), (1, 0, 1, 250.0, 355.0, 101.0, 7.5),
+]
+
+
+def domdec_fixture(R):
+    """Model::getLocalsize2d and Model*::getDomainmodel (one domain) of the
+    reference on model files, for both map types, positive, zero and
+    negative apertures, with and without free surface."""
+    import ctypes as C
+    import tempfile
+
+    from rockseis_amd import rss
+    nx, nz, lpml, dx = 47, 31, 8, 10.0
+    vp, vs, rho = _model2d(nx, nz, dx, 9)
+    out = {"vp": vp, "vs": vs, "rho": rho, "grid": np.float32([nx, nz, lpml, dx]),
+           "cases": np.float32(DOMDEC_CASES)}
+    fp = C.POINTER(C.c_float)
+    with tempfile.TemporaryDirectory() as d:
+        files = [os.path.join(d, f) for f in ("vp.rss", "vs.rss", "rho.rss")]
+        for f, a in zip(files, (vp, vs, rho)):
+            rss.write_model2d(f, a, dx, dx, 15.0, 5.0)
+        for k, (el, fs, mp, ap, sx, g0, gd) in enumerate(DOMDEC_CASES):
+            ntr = 12
+            coords = np.zeros((ntr, 4), np.float32)
+            coords[:, 0], coords[:, 1] = sx, 10.0
+            coords[:, 2], coords[:, 3] = g0 + gd * np.arange(ntr), 20.0
+            oi = (C.c_longlong * 4)()
+            of = (C.c_float * 2)()
+            vsf = files[1].encode() if el else b""
+            args = (files[0].encode(), vsf, files[2].encode(), lpml, int(fs), coords.ctypes.data_as(fp), ntr,
+                    C.c_float(ap), int(mp), 8, oi, of)
+            R.ref_domain_model(*args, *([None] * 8))
+            nxd, nzd = int(oi[2]), int(oi[3])
+            arrs = {n: np.zeros((nzd, nxd), np.float32) for n in ("Vp", "Vs", "R", "L", "L2M", "M", "Rx", "Rz")}
+            R.ref_domain_model(*args, *[arrs[n].ctypes.data_as(fp) for n in ("Vp", "Vs", "R", "L", "L2M", "M",
+                                                                              "Rx", "Rz")])
+            out[f"c{k}_coords"] = coords
+            out[f"c{k}_int"] = np.int64(list(oi))
+            out[f"c{k}_origin"] = np.float32(list(of))
+            for n, a in arrs.items():
+                if el or n in ("Vp", "R", "L", "Rx", "Rz"):
+                    out[f"c{k}_{n}"] = a
+    offs = []
+    rng = np.random.default_rng(4)
+    for _ in range(64):
+        ol, og = np.float32(rng.uniform(-500, 500)), np.float32(rng.uniform(-500, 500))
+        dl = np.float32(rng.choice([2.5, 5.0, 10.0, 12.5]))
+        offs.append((ol, dl, og, dl, R.ref_image_offset(C.c_float(ol), C.c_float(dl), C.c_float(og),
+                                                        C.c_float(dl))))
+    out["image_offsets"] = np.float64(offs)
+    save("domdec", **out)
+
+
+GROUPS["domdec"] = domdec_fixture
+
+
 if __name__ == "__main__":
     # python -m tests.golden.make_golden [group ...]   (default: every group)
     R = oracle.ref()
