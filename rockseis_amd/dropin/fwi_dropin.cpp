@@ -311,3 +311,230 @@ void FwiAcoustic2D<float>::computeResiduals() {
 }
 
 }  // namespace rockseis
+
+// ---- FwiElastic2D<float> (fwi.h:243-312; fwi.cpp:2591-3773, 3937-4143) -------
+//
+// run_optimal() hands the shot to rsg_fwi_e2d_gradient_batch as a batch of
+// one window at offset 0: forward steps, the P / Ux / Uz records, misfit and
+// residuals, the adjoint-state sweep with Revolve checkpoints in HBM, the
+// lambda / mu / rho imaging condition, and scaleGrad into Vp / Vs / rho
+// gradients, as run_optimal ends (fwi.cpp:4132).  Domain models keep the
+// reference's imaging window (fwi.cpp:2666-2675).
+
+namespace {
+
+using rockseis::ModelElastic2D;
+
+struct E2DJob {
+    ModelElastic2D<float> *model;
+    std::shared_ptr<Data2D<float>> source, obs[3], mod[3], res[3], wei[3], wavgrad;
+    std::shared_ptr<Image2D<float>> grad[3];
+    int order, ncheck, misfit_type;
+};
+
+double run_e2d(const E2DJob &j) {
+    ModelElastic2D<float> *m = j.model;
+    const int want[3] = {rockseis::PRESSURE, rockseis::VX, rockseis::VZ};
+    for (int c = 0; c < 3; c++) {
+        if (!j.obs[c] || j.obs[c]->getField() != want[c])
+            rsgd::fatal("FwiElastic2D: ", "the device gradient takes the P, Ux (VX) and Uz (VZ) records together");
+        if (j.obs[c]->getReciprocity())
+            rsgd::fatal("FwiElastic2D: ", "reciprocity-scaled pressure records are not supported");
+        if (!j.grad[c]) rsgd::fatal("FwiElastic2D: ", "the Vp, Vs and rho gradient images must all be set");
+    }
+    const int sf = (int)j.source->getField();
+    if (sf != rockseis::PRESSURE && sf != rockseis::VX && sf != rockseis::VZ)
+        rsgd::fatal("FwiElastic2D: ", "source field must be PRESSURE, VX or VZ");
+    const bool domdec = m->getDomain()->getStatus();
+    const int lpml = m->getLpml(), nx = m->getNx(), nz = m->getNz();
+    rsg_fwi_e2d_shot s{};
+    s.g.nx = domdec ? nx : nx + 2 * lpml;
+    s.g.ny = 1;
+    s.g.nz = domdec ? nz : nz + 2 * lpml;
+    s.g.lpml = lpml;
+    s.g.order = j.order < 1 ? 1 : (j.order > 8 ? 8 : j.order);
+    s.g.fs = m->getFs() ? 1 : 0;
+    s.g.dx = m->getDx();
+    s.g.dy = 1.0f;
+    s.g.dz = m->getDz();
+    s.g.dt = j.source->getDt();
+    s.g.domdec = domdec ? 1 : 0;
+    const size_t npad = (size_t)s.g.nx * s.g.nz, nimg = (size_t)nx * nz;
+    rsgd::DevMem model, tab;
+    model.alloc(sizeof(float) * (5 * npad + 3 * nimg));
+    const float *padded[5] = {m->getL(), m->getL2M(), m->getM(), m->getRx(), m->getRz()};
+    for (int k = 0; k < 5; k++) model.upload(padded[k], sizeof(float) * npad, k * sizeof(float) * npad);
+    const float *img[3] = {m->getVp(), m->getVs(), m->getR()};
+    for (int k = 0; k < 3; k++) model.upload(img[k], sizeof(float) * nimg, sizeof(float) * (5 * npad + k * nimg));
+    float *b = model.f();
+    s.L = b;
+    s.L2M = b + npad;
+    s.M = b + 2 * npad;
+    s.Rx = b + 3 * npad;
+    s.Rz = b + 4 * npad;
+    s.Vp = b + 5 * npad;
+    s.Vs = b + 5 * npad + nimg;
+    s.Rho = b + 5 * npad + 2 * nimg;
+    s.ox = 0;
+    std::vector<float> t(12 * (size_t)lpml);
+    rsgd::ok(rsg_pml_tables(lpml, s.g.dt, SMAX, 1, t.data()), "FwiElastic2D: PML tables");
+    tab.alloc(sizeof(float) * t.size());
+    tab.upload(t.data(), sizeof(float) * t.size());
+
+    DevTraces src, obs[3];
+    src.from(j.source, false, "FwiElastic2D source");
+    s.src = src.t;
+    rsgd::DevMem dmod[3], dres[3], wei[3], wgrad, grads;
+    size_t ndat = 0;
+    for (int c = 0; c < 3; c++) {
+        obs[c].from(j.obs[c], true, "FwiElastic2D data");
+        s.obs[c] = obs[c].t;
+        ndat = (size_t)obs[c].t.ntrace * obs[c].t.nt;
+        dmod[c].alloc(sizeof(float) * ndat);
+        dres[c].alloc(sizeof(float) * ndat);
+        s.dmod[c] = dmod[c].f();
+        s.dres[c] = dres[c].f();
+        if (j.wei[c]) {
+            if ((size_t)j.wei[c]->getNtrace() * j.wei[c]->getNt() != ndat)
+                rsgd::fatal("FwiElastic2D: ", "data weights must match the recorded data");
+            wei[c].alloc(sizeof(float) * ndat);
+            wei[c].upload(j.wei[c]->getData(), sizeof(float) * ndat);
+            s.weight[c] = wei[c].f();
+        }
+    }
+    if (j.wavgrad) {
+        same_map(j.source, j.wavgrad);
+        wgrad.alloc(sizeof(float) * src.t.ntrace * src.t.nt);
+        s.wavgrad = wgrad.f();
+    }
+    // run_optimal allocates fresh (zero) images (fwi.cpp:3973-3975)
+    grads.alloc(sizeof(float) * 3 * nimg);
+    for (int c = 0; c < 3; c++) {
+        if (!j.grad[c]->getAllocated()) j.grad[c]->allocateImage();
+        if ((size_t)j.grad[c]->getNx() * j.grad[c]->getNz() * j.grad[c]->getNhx() * j.grad[c]->getNhz() != nimg)
+            rsgd::fatal("FwiElastic2D: ", "gradient image size does not match the model");
+    }
+    rsg_fwi_e2d f{};
+    f.tab = tab.f();
+    f.ncheck = j.ncheck;
+    f.fp16 = 0;
+    f.scale = 1;
+    f.nx = nx;
+    f.nz = nz;
+    for (int c = 0; c < 3; c++) f.grad[c] = grads.f() + c * nimg;
+    double misfit = 0.0;
+    f.misfit = &misfit;
+    f.misfit_type = j.misfit_type;
+    rsgd::ok(rsg_fwi_e2d_gradient_batch(&f, &s, 1, nullptr), "FwiElastic2D gradient");
+    for (int c = 0; c < 3; c++) {
+        grads.download(j.grad[c]->getImagedata(), sizeof(float) * nimg, sizeof(float) * c * nimg);
+        if (j.mod[c]) dmod[c].download(j.mod[c]->getData(), sizeof(float) * ndat);
+        if (j.res[c]) dres[c].download(j.res[c]->getData(), sizeof(float) * ndat);
+    }
+    if (j.wavgrad) wgrad.download(j.wavgrad->getData(), sizeof(float) * src.t.ntrace * src.t.nt);
+    return misfit;
+}
+
+// FwiElastic2D::computeMisfit / computeResiduals on the device.
+double e2d_misfit(int type, const std::shared_ptr<Data2D<float>> mod[3], const std::shared_ptr<Data2D<float>> rec[3],
+                  const std::shared_ptr<Data2D<float>> wei[3], const std::shared_ptr<Data2D<float>> res[3]) {
+    const size_t ntr = mod[1]->getNtrace(), nt = mod[1]->getNt(), n = ntr * nt;
+    const void *map = mod[1]->getGeom()->getGmap();
+    if (!map) rsgd::fatal("FwiElastic2D: ", "modelled data not mapped (Data2D::makeMap)");
+    rsgd::DevMem buf, dmap;
+    buf.alloc(sizeof(float) * 12 * n);
+    dmap.alloc(sizeof(int) * 2 * ntr);
+    dmap.upload(map, sizeof(int) * 2 * ntr);
+    rsg_e2d_records r{};
+    r.nrec = (int)ntr;
+    r.map = dmap.i();
+    for (int c = 0; c < 3; c++) {
+        if ((size_t)mod[c]->getNtrace() != ntr || (size_t)rec[c]->getNtrace() != ntr || (size_t)rec[c]->getNt() != nt)
+            rockseis::rs_error("Mismatch between number of traces or samples in the modelled and recorded data.");
+        float *pm = buf.f() + (4 * c) * n, *pr = pm + n, *pw = pr + n, *ps = pw + n;
+        buf.upload(mod[c]->getData(), sizeof(float) * n, sizeof(float) * (4 * c) * n);
+        buf.upload(rec[c]->getData(), sizeof(float) * n, sizeof(float) * (4 * c + 1) * n);
+        r.mod[c] = pm;
+        r.obs[c] = pr;
+        if (wei[c]) {
+            buf.upload(wei[c]->getData(), sizeof(float) * n, sizeof(float) * (4 * c + 2) * n);
+            r.weight[c] = pw;
+        }
+        r.res[c] = ps;
+    }
+    double misfit = 0.0;
+    rsgd::ok(rsg_misfit_residuals_e2d(type, &r, 1, (int)nt, &misfit, nullptr), "FwiElastic2D misfit");
+    for (int c = 0; c < 3; c++)
+        if (res[c]) buf.download(res[c]->getData(), sizeof(float) * n, sizeof(float) * (4 * c + 3) * n);
+    return misfit;
+}
+
+}  // namespace
+
+namespace rockseis {
+
+template <>
+int FwiElastic2D<float>::run_optimal() {
+    if (!vpgradset && !vsgradset && !wavgradset) {
+        rs_warning("FwiElastic2D::run: No image set");
+        return FWI_ERR;
+    }
+    this->createLog(this->getLogfile());
+    this->writeLog("Running 2D Elastic full-waveform inversion gradient with optimal checkpointing (rsgpu).");
+    E2DJob j{};
+    j.model = model.get();
+    j.source = source;
+    j.obs[0] = dataPset ? dataP : nullptr;
+    j.obs[1] = dataUxset ? dataUx : nullptr;
+    j.obs[2] = dataUzset ? dataUz : nullptr;
+    j.mod[0] = datamodPset ? datamodP : nullptr;
+    j.mod[1] = datamodUxset ? datamodUx : nullptr;
+    j.mod[2] = datamodUzset ? datamodUz : nullptr;
+    j.res[0] = dataresPset ? dataresP : nullptr;
+    j.res[1] = dataresUxset ? dataresUx : nullptr;
+    j.res[2] = dataresUzset ? dataresUz : nullptr;
+    j.wei[0] = dataweightpset ? dataweightp : nullptr;
+    j.wei[1] = dataweightxset ? dataweightx : nullptr;
+    j.wei[2] = dataweightzset ? dataweightz : nullptr;
+    j.wavgrad = wavgradset ? wavgrad : nullptr;
+    j.grad[0] = vpgradset ? vpgrad : nullptr;
+    j.grad[1] = vsgradset ? vsgrad : nullptr;
+    j.grad[2] = rhogradset ? rhograd : nullptr;
+    j.order = this->getOrder();
+    j.ncheck = this->getNcheck();
+    j.misfit_type = (int)this->getMisfit_type();
+    this->setMisfit((float)run_e2d(j));
+    this->writeLog("\nGradient computation completed.");
+    return FWI_OK;
+}
+
+template <>
+int FwiElastic2D<float>::run() {
+    rs_error("FwiElastic2D::run: ", "full snapshot saving is not provided by the rsgpu drop-in; use run_optimal "
+                                    "(snapmethod OPTIMAL)");
+    return FWI_ERR;
+}
+
+template <>
+void FwiElastic2D<float>::computeMisfit() {
+    const std::shared_ptr<Data2D<float>> mod[3] = {datamodP, datamodUx, datamodUz};
+    const std::shared_ptr<Data2D<float>> rec[3] = {dataP, dataUx, dataUz};
+    const std::shared_ptr<Data2D<float>> wei[3] = {dataweightpset ? dataweightp : nullptr,
+                                                   dataweightxset ? dataweightx : nullptr,
+                                                   dataweightzset ? dataweightz : nullptr};
+    const std::shared_ptr<Data2D<float>> none[3] = {nullptr, nullptr, nullptr};
+    this->setMisfit((float)e2d_misfit((int)this->getMisfit_type(), mod, rec, wei, none));
+}
+
+template <>
+void FwiElastic2D<float>::computeResiduals() {
+    const std::shared_ptr<Data2D<float>> mod[3] = {datamodP, datamodUx, datamodUz};
+    const std::shared_ptr<Data2D<float>> rec[3] = {dataP, dataUx, dataUz};
+    const std::shared_ptr<Data2D<float>> wei[3] = {dataweightpset ? dataweightp : nullptr,
+                                                   dataweightxset ? dataweightx : nullptr,
+                                                   dataweightzset ? dataweightz : nullptr};
+    const std::shared_ptr<Data2D<float>> res[3] = {dataresP, dataresUx, dataresUz};
+    e2d_misfit((int)this->getMisfit_type(), mod, rec, wei, res);
+}
+
+}  // namespace rockseis

@@ -1,5 +1,5 @@
 """Cases that run the reference's own driver programs (Acousticmod2d,
-Elasticmod2d, Acousticfwi2d) end to end.
+Elasticmod2d, Acousticfwi2d, Elasticfwi2d) end to end.
 
 The same seeded inputs are written as RSS files for two builds of each
 driver (oracle/Makefile target `drivers`): the reference as it builds
@@ -176,3 +176,44 @@ def fresh_dir(base, name):
         shutil.rmtree(d)
     os.makedirs(d)
     return d
+
+
+def efwi_config(d):
+    items = [("freesurface", "true"), ("order", 8), ("lpml", LPML), ("ndomain0", 1), ("ndomain1", 1),
+             ("snapinc", 1), ("apertx", 900), ("kvp", 1.0), ("kvs", 1.0), ("krho", 1.0), ("ksource", 1.0),
+             ("paramtype", 0), ("outputhess", "false"), ("source_type", 0), ("snapmethod", 1),
+             ("nsnaps", FWI_NSNAPS), ("incore", "true"), ("misfit_type", 0), ("dataweightp", "false"),
+             ("dataweightx", "false"), ("dataweightz", "false"), ("modmute", "false"), ("srcmute", "false"),
+             ("vpregalpha", 0.0), ("vsregalpha", 0.0), ("rhoregalpha", 0.0), ("max_linesearch", 1),
+             ("max_iterations", 0), ("update_vp", "true"), ("update_vs", "true"), ("update_rho", "true"),
+             ("update_source", "true"), ("linesearch", 3), ("optmethod", 1), ("reciprocity", "false"),
+             ("Vp", "vp0.rss"), ("Vs", "vs0.rss"), ("Rho", "rho0.rss"), ("Wavelet", "wav.rss"),
+             ("Precordfile", "pobs.rss"), ("Uxrecordfile", "uxobs.rss"), ("Uzrecordfile", "uzobs.rss"),
+             ("Snapfile", "efwisnap.rss")]
+    _cfg(os.path.join(d, "efwi.cfg"), items)
+
+
+def efwi(flavour, d, obs, np_=2):
+    """The elastic FWI driver (Elasticfwi2d) on the smoothed model with the
+    elastic modelling case's P / Vx / Vz records as observed data: one
+    gradient evaluation."""
+    vp, vs, rho = model()
+    for name, a in (("vp0", smooth(vp)), ("vs0", smooth(vs)), ("rho0", smooth(rho))):
+        rss.write_model2d(os.path.join(d, name + ".rss"), a, DX, DX)
+    wav = synth.ricker(NT, DT, F0)[None]
+    rss.write_data2d(os.path.join(d, "wav.rss"), wav, np.zeros((1, 2)), np.zeros((1, 2)), DT)
+    sc, gc = survey()
+    for f, key in (("pobs", "p"), ("uxobs", "vx"), ("uzobs", "vz")):
+        rss.write_data2d(os.path.join(d, f + ".rss"), obs[key], sc, gc, 2e-3)
+    efwi_config(d)
+    run([MPIEXEC, "-n", str(np_), binary(flavour, "Elasticfwi2d"), "efwi.cfg"], d)
+    out = {}
+    for name in ("vp_grad", "vs_grad", "rho_grad"):
+        a, _ = rss.read_image2d(os.path.join(d, name + ".rss"))
+        out[name] = a[0, 0].astype(np.float32)
+    out["source_grad"] = rss.read_data(os.path.join(d, "source_grad.rss"))["data"].astype(np.float32)
+    h, m = rss.read(os.path.join(d, "misfit.rss"))
+    out["misfit"] = np.array(m, np.float32)
+    for f in ("pmod", "uxmod", "uzmod", "pres", "uxres", "uzres"):
+        out[f] = rss.read_data(os.path.join(d, f + ".rss"))["data"].astype(np.float32)
+    return out

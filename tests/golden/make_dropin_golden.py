@@ -12,6 +12,10 @@ oracle/_ref/bin/ref):
                    illumination, the wavelet gradient, the misfit of each
                    shot and the modelled and residual records; the observed
                    data (the acoustic case's P records) ride along as input
+  dropin_efwi.npz  mpiexec -n 2 rsElasticfwi2d, one gradient evaluation on
+                   the smoothed elastic model with the elastic case's P / Vx /
+                   Vz records as observed data: Vp / Vs / rho gradients, the
+                   wavelet gradient, misfits, modelled and residual records
 
 The inputs are the seeded arrays of tests/dropin_cases.py written as RSS
 files; run in a container where /root/reference is built:
@@ -46,6 +50,8 @@ def main():
         save("dropin_emod", **emod)
         fwi = dc.fwi("ref", dc.fresh_dir(tmp, "fwi"), amod["p"])
         save("dropin_fwi", obs=amod["p"], **fwi)
+        efwi = dc.efwi("ref", dc.fresh_dir(tmp, "efwi"), emod)
+        save("dropin_efwi", obs_p=emod["p"], obs_vx=emod["vx"], obs_vz=emod["vz"], **efwi)
 
 
 if __name__ == "__main__":

@@ -26,6 +26,7 @@ REPLACED = (
     "rockseis::Snapshot2D<float>::writeSnap(int)",
     "rockseis::Revolve<float>::revolve()",
     "rockseis::FwiAcoustic2D<float>::run_optimal()",
+    "rockseis::FwiElastic2D<float>::run_optimal()",
 )
 
 have = all(os.path.exists(dc.binary(f, d)) for f in ("ref", "gpu") for d in DRIVERS)
@@ -48,7 +49,7 @@ def _binding(text, member):
     return kinds
 
 
-@pytest.mark.parametrize("name", ("Acousticmod2d", "Acousticfwi2d"))
+@pytest.mark.parametrize("name", ("Acousticmod2d", "Acousticfwi2d", "Elasticfwi2d"))
 def test_dropin_members_replace_the_reference(name):
     gpu, ref = _nm(dc.binary("gpu", name), "-C"), _nm(dc.binary("ref", name), "-C")
     for m in REPLACED:
@@ -78,7 +79,9 @@ def test_reference_programs_reproduce_the_fixtures(golden):
         amod = dc.modelling("ref", dc.fresh_dir(tmp, "amod"), elastic=False)
         emod = dc.modelling("ref", dc.fresh_dir(tmp, "emod"), elastic=True)
         fwi = dc.fwi("ref", dc.fresh_dir(tmp, "fwi"), golden("dropin_fwi")["obs"])
-    for name, out in (("dropin_amod", amod), ("dropin_emod", emod), ("dropin_fwi", fwi)):
+        ge = golden("dropin_efwi")
+        efwi = dc.efwi("ref", dc.fresh_dir(tmp, "efwi"), {"p": ge["obs_p"], "vx": ge["obs_vx"], "vz": ge["obs_vz"]})
+    for name, out in (("dropin_amod", amod), ("dropin_emod", emod), ("dropin_fwi", fwi), ("dropin_efwi", efwi)):
         g = golden(name)
         for k, v in out.items():
             np.testing.assert_array_equal(v, g[k], err_msg=f"{name}:{k}")

@@ -51,3 +51,13 @@ def test_fwi_driver_dropin(golden):
         out = dc.fwi("gpu", d, g["obs"])
     _compare(out, g, ("pmod", "pres"), RTOL_L2, "dropin_fwi")
     _compare(out, g, ("vp_grad", "rho_grad", "src_ilum", "source_grad", "misfit"), 1e-4, "dropin_fwi")
+
+
+def test_elastic_fwi_driver_dropin(golden):
+    print("dropin_efwi: mpiexec -n 2 rsElasticfwi2d with the drop-in (domain model, free surface)", flush=True)
+    g = golden("dropin_efwi")
+    obs = {"p": g["obs_p"], "vx": g["obs_vx"], "vz": g["obs_vz"]}
+    with tempfile.TemporaryDirectory() as d:
+        out = dc.efwi("gpu", d, obs)
+    _compare(out, g, ("pmod", "uxmod", "uzmod", "pres", "uxres", "uzres"), RTOL_L2, "dropin_efwi")
+    _compare(out, g, ("vp_grad", "vs_grad", "rho_grad", "source_grad", "misfit"), 1e-4, "dropin_efwi")
