@@ -41,7 +41,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--graph-chunk", type=int, default=100)
+    ap.add_argument("--graph-chunk", type=int, default=128,
+                    help="largest hipGraph of the time loop (graphs of 1, 2, 4, ... steps up to it are "
+                         "captured before the timed region; K steps replay its binary decomposition)")
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=80, help="time steps of the CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -101,6 +103,22 @@ def roofline_only(reps):
 
 
 _T0 = time.perf_counter()
+
+
+def _pow2_floor(n):
+    return 1 << (max(int(n), 1).bit_length() - 1)
+
+
+def _pow2_split(k, chunk):
+    """The replays rsg_modelling_plan_run makes for k steps, as text."""
+    top, parts = _pow2_floor(chunk), []
+    n = top
+    while n >= 1:
+        c, k = divmod(k, n) if n == top else (k // n, k % n)
+        if c:
+            parts.append(f"{c} x {n}" if c > 1 else f"{n}")
+        n //= 2
+    return " + ".join(parts) if parts else "0"
 
 
 def _progress(msg):
@@ -220,6 +238,7 @@ def main():
     from rockseis_amd import autotune
     tuned = autotune.tune_waves(mod.waves, cfg["order"], nshot=1)
     _progress(f"modelling workload built, tile rows per thread {tuned['rz2d']}, PML tile cost {tuned.get('pml_cost')}")
+    mod.capture()  # the time loop's graphs, with the tuned launch shape
     if W:
         mod.launch(0, W)
     torch.cuda.synchronize()
@@ -257,6 +276,11 @@ def main():
                    "grid": f"{cfg['nx']}x{cfg['nz']} + lpml {cfg['lpml']} -> {cfg['nx_pml']}x{cfg['nz_pml']}",
                    "dx_m": cfg["dx"], "order": cfg["order"], "dt_s": cfg["dt"], "nt": cfg["nt"],
                    "source": f"explosive {cfg['f0']:g} Hz Ricker", "receivers": "2001 x (vx, vz)",
+                   "time_loop": (f"hipGraph replays only: graphs of 1, 2, 4, ..., {_pow2_floor(args.graph_chunk)} "
+                                 f"steps captured before the timed region, {K} steps = "
+                                 f"{_pow2_split(K, args.graph_chunk)}; source injection and both receiver "
+                                 "gathers fused into the stress kernel (2 kernels per step)")
+                   if args.graph_chunk > 0 else "eager launches, one step at a time",
                    "graph_chunk": args.graph_chunk, "rows_per_thread": tuned["rz2d"],
                    "pml_tile_cost": tuned.get("pml_cost"),
                    "parallelism": "single GPU"},
@@ -428,8 +452,9 @@ def a3d_leg():
     m.staggerModels()
     mod = core.ModellingAcoustic3D(m, src, order, 1)
     mod.setRecP(rec)
-    mod.setGraphChunk(50)
+    mod.setGraphChunk(64)
     mod.prepare()
+    mod.capture()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     mod.launch(0, nt)

@@ -145,6 +145,15 @@ int rsg_stagger_e2d(const float *vp, const float *vs, const float *rho, int nx, 
                     int fs, float *L, float *L2M, float *M, float *Rx, float *Rz);
 int rsg_stagger_a3d(const float *vp, const float *rho, int nx, int ny, int nz, int lpml, int fs,
                     float *L, float *Rx, float *Ry, float *Rz);
+/* The same staggering (model.cpp:1350-1411, 1965-2040) on the device, equal
+ * to rsg_stagger_a2d / rsg_stagger_e2d bit for bit: device inputs of nx x nz
+ * cells with row stride ld >= nx (a window of a wider model), device outputs
+ * of (nx+2*lpml) x (nz+2*lpml).  Queued on `stream`; returns after the
+ * stream has drained (the zero-density check reads one flag back). */
+int rsg_stagger_a2d_dev(const float *vp, const float *rho, int ld, int nx, int nz, int lpml, int fs,
+                        float *L, float *Rx, float *Rz, void *stream);
+int rsg_stagger_e2d_dev(const float *vp, const float *vs, const float *rho, int ld, int nx, int nz, int lpml,
+                        int fs, float *L, float *L2M, float *M, float *Rx, float *Rz, void *stream);
 /* Geometry2D::makeMap for one coordinate set (geometry.cpp:137-235), with
  * the field shift of Data2D::makeMap (-0.5 along x for VX, z for VZ). */
 int rsg_makemap2d(const float *coords, int ntrace, int nx, int nz, float dx, float dz, float ox,
@@ -243,6 +252,24 @@ typedef struct {
 int rsg_modelling_run_snap(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src,
                            int nrec, const rsg_traces *recs, int nsnap, const rsg_snapset *snaps, int it0,
                            int nsteps, int graph_chunk, void *stream);
+
+/* ---- modelling plan: the same time loop (Modelling*::run, modelling.cpp:
+ * 242-286, with the snapshots of rsg_modelling_run_snap) captured once as
+ * hipGraphs of 1, 2, 4, ..., max_chunk steps (max_chunk rounded down to a
+ * power of two, 1..4096) and replayed for any range of steps: run() replays
+ * the largest graph while it fits and then the binary digits of the rest,
+ * so every step runs from a graph and no capture happens after create.
+ * The graphs hold the pointers and launch shapes in force at create: the
+ * fields, model, traces and snapshot buffers must outlive the plan, and a
+ * retuned launch shape needs a new plan.  One run at a time per plan;
+ * runs are queued on `stream` and ordered with the caller's other work
+ * there.  destroy() waits for the device before it releases the graphs. */
+typedef struct rsg_modelling_plan rsg_modelling_plan;
+int rsg_modelling_plan_create(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src,
+                              int nrec, const rsg_traces *recs, int nsnap, const rsg_snapset *snaps,
+                              int max_chunk, void *stream, rsg_modelling_plan **out);
+int rsg_modelling_plan_run(rsg_modelling_plan *p, int it0, int nsteps, void *stream);
+int rsg_modelling_plan_destroy(rsg_modelling_plan *p);
 
 /* ---- FWI gradient of one shot (FwiAcoustic2D::run_optimal, fwi.cpp:1159-1332,
  * and FwiAcoustic2D::run, fwi.cpp:1030-1156) */
@@ -378,8 +405,13 @@ typedef struct {
 int rsg_rtm_a2d_image(const rsg_grid *g, const rsg_rtm_a2d *r, void *stream);
 /* nshot shots on one grid in one set of launches per step (the reference
  * migrates one shot per MPI worker and stacks the images, image.cpp): traces
- * of shot k follow shot k-1 in src/data (map, shift, data); the image is the
- * stack over the shots, added in shot order at every step. */
+ * of shot k follow shot k-1 in src/data (map, shift, data).  The image is the
+ * sum over steps of the per-step sum over shots (shot order), whereas the
+ * reference's stack is the sum over shots of whole per-shot images: the
+ * two differ in rounding only (relative L2 5.7e-7 for 3 shots in the GPU
+ * test), not bit for bit.  Every shot uses the same full grid: the per-shot
+ * window offsets of stackImage (image.cpp:577-578) are not supported; shots
+ * on their own windows take rsg_rtm_a2d_image one at a time. */
 int rsg_rtm_a2d_batch(const rsg_grid *g, const rsg_rtm_a2d *r, int nshot, void *stream);
 
 /* ---- Acoustic 3D reverse-time migration of one shot

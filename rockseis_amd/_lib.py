@@ -182,6 +182,12 @@ def _declare(L):
         "rsg_revolve_adjust": ([ip], ip),
         "rsg_revolve_init": ([vp, ip, ip, C.POINTER(C.c_int)], ip),
         "rsg_revolve_next": ([vp], ip),
+        "rsg_modelling_plan_create": ([ip, GP, SP, TP, ip, TP, ip, C.POINTER(SnapSet), ip, vp,
+                                       C.POINTER(C.c_void_p)], ip),
+        "rsg_modelling_plan_run": ([vp, ip, ip, vp], ip),
+        "rsg_modelling_plan_destroy": ([vp], ip),
+        "rsg_stagger_a2d_dev": ([vp, vp, ip, ip, ip, ip, ip, vp, vp, vp, vp], ip),
+        "rsg_stagger_e2d_dev": ([vp, vp, vp, ip, ip, ip, ip, ip, vp, vp, vp, vp, vp, vp], ip),
     }
     for name in ("a2d_forward_velocity", "a2d_forward_stress", "e2d_forward_velocity",
                  "e2d_forward_stress", "e2d_backward_velocity", "e2d_backward_stress",
@@ -211,6 +217,8 @@ EXPORTS = (
     "rsg_modelling_run_snap", "rsg_snap_interior", "rsg_crosscorr_a2d", "rsg_local_window_2d",
     "rsg_domain_model_a2d", "rsg_domain_model_e2d", "rsg_image_offset", "rsg_der_apply",
     "rsg_revolve_adjust", "rsg_revolve_init", "rsg_revolve_next",
+    "rsg_modelling_plan_create", "rsg_modelling_plan_run", "rsg_modelling_plan_destroy",
+    "rsg_stagger_a2d_dev", "rsg_stagger_e2d_dev",
 )
 
 

@@ -72,6 +72,21 @@ class Der:
         return out
 
 
+def _dev_window(t, nz, nx):
+    """A float32 device tensor [nz, >= nx] with unit column stride (a window
+    of a wider model may keep the wider row stride)."""
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise R.RsgError("staggerModelsDevice: model arrays must be CUDA tensors")
+    t = t.detach()
+    if t.dtype != torch.float32:
+        t = t.float()
+    if t.dim() != 2 or t.shape[0] != nz or t.shape[1] < nx:
+        raise R.RsgError(f"staggerModelsDevice: expected [{nz}, >= {nx}], got {tuple(t.shape)}")
+    if t.stride(1) != 1 or t.stride(0) < t.shape[1] and t.shape[0] > 1:
+        t = t.contiguous()
+    return t
+
+
 class Pml:
     """CFS-PML constants (Pml<T>, pml.h:31-81); tables computed on the host
     exactly as computeABC() and kept on the device."""
@@ -112,6 +127,19 @@ class _Model:
     def getFs(self):
         return self.fs
 
+    def _device_arrays(self, names):
+        """The padded device arrays _d<name>, allocated unless they exist
+        with this model's shape (captured launches keep their pointers)."""
+        shp = (self.getNz_pml(), self.getNx_pml())
+        out = []
+        for n in names:
+            d = getattr(self, "_d" + n, None)
+            if d is None or tuple(d.shape) != shp or d.dtype != torch.float32:
+                d = torch.empty(shp, dtype=torch.float32, device=_dev())
+                setattr(self, "_d" + n, d)
+            out.append(d)
+        return out
+
     def _upload(self, names):
         # in place when the device array exists with the same shape, so that
         # captured launches (FWI plans) keep pointing at the model
@@ -149,6 +177,17 @@ class ModelAcoustic2D(_Model):
                                         _np_ptr(self.Rz)), "staggerModels")
         self._upload(("L", "Rx", "Rz"))
 
+    def staggerModelsDevice(self, vp, rho):
+        """staggerModels() from device tensors: vp, rho [nz, ld >= nx] float32
+        (the first nx columns are this model's), staggered on the device into
+        the model's device arrays (in place when they exist), bit-identical
+        to the host set-up; the host arrays are left as they are."""
+        vp, rho = _dev_window(vp, self.nz, self.nx), _dev_window(rho, self.nz, self.nx)
+        d = self._device_arrays(("L", "Rx", "Rz"))
+        R.check(R.lib().rsg_stagger_a2d_dev(vp.data_ptr(), rho.data_ptr(), vp.stride(0), self.nx, self.nz,
+                                            self.lpml, int(self.fs), *(t.data_ptr() for t in d), _stream()),
+                "staggerModels")
+
 
 class ModelElastic2D(_Model):
     def __init__(self, nx, nz, lpml, dx, dz, ox=0.0, oz=0.0, fs=False):
@@ -173,6 +212,16 @@ class ModelElastic2D(_Model):
                                         _np_ptr(self.M), _np_ptr(self.Rx), _np_ptr(self.Rz)),
                 "staggerModels")
         self._upload(("L", "L2M", "M", "Rx", "Rz"))
+
+    def staggerModelsDevice(self, vp, vs, rho):
+        """staggerModels() from device tensors (ModelAcoustic2D.staggerModelsDevice)."""
+        a = [_dev_window(t, self.nz, self.nx) for t in (vp, vs, rho)]
+        if len({t.stride(0) for t in a}) != 1:
+            raise R.RsgError("staggerModelsDevice: vp, vs and rho must share their row stride")
+        d = self._device_arrays(("L", "L2M", "M", "Rx", "Rz"))
+        R.check(R.lib().rsg_stagger_e2d_dev(*(t.data_ptr() for t in a), a[0].stride(0), self.nx, self.nz,
+                                            self.lpml, int(self.fs), *(t.data_ptr() for t in d), _stream()),
+                "staggerModels")
 
 
 class ModelAcoustic3D(_Model):
@@ -577,6 +626,7 @@ class _Modelling:
         self.snapfiles = {}
         self.graph_chunk = 0
         self.waves = None
+        self._plan = None
 
     def getOrder(self):
         return self.order
@@ -621,8 +671,24 @@ class _Modelling:
         self.recs["VZ"] = d
 
     def setGraphChunk(self, n):
-        """Capture n steps per hipGraph (0 = plain per-step launches)."""
+        """Run the time loop from hipGraphs of up to n steps (0 = plain
+        per-step launches).  The graphs (1, 2, 4, ... n steps) are captured
+        once, at the first launch after prepare(), with the launch shapes in
+        force then, and replayed for every later launch."""
         self.graph_chunk = int(n)
+        self._drop_plan()
+
+    def _drop_plan(self):
+        plan = getattr(self, "_plan", None)
+        self._plan = None
+        if plan:
+            R.check(R.lib().rsg_modelling_plan_destroy(plan), "rsg_modelling_plan_destroy")
+
+    def __del__(self):
+        try:
+            self._drop_plan()
+        except Exception:
+            pass
 
     def _vpmax(self):
         return float(np.max(self.model.Vp))
@@ -635,6 +701,7 @@ class _Modelling:
         arguments of rsg_modelling_run (kept so run() can be timed alone)."""
         if not self.checkStability():
             raise R.RsgError("Wavelet sampling interval (dt) does not match the stability criteria.")
+        self._drop_plan()
         m, src = self.model, self.source
         self.waves = self.waves_cls(m, src.nt, src.dt, src.ot)
         w = self.waves
@@ -674,9 +741,27 @@ class _Modelling:
             self._snap_dev[name] = out
         return sets
 
+    def capture(self):
+        """Capture the time loop's hipGraphs (setGraphChunk) now rather than
+        at the first launch; a no-op without graphs or when captured."""
+        if self.graph_chunk <= 0 or self._plan is not None:
+            return
+        g, s, t_src, recs, nrec = self._args
+        plan = C.c_void_p()
+        snaps = self._snapsets
+        R.check(R.lib().rsg_modelling_plan_create(
+            self.waves.kind, C.byref(g), C.byref(s), C.byref(t_src), nrec, recs,
+            0 if snaps is None else len(snaps), snaps, self.graph_chunk, _stream(), C.byref(plan)),
+            "rsg_modelling_plan_create")
+        self._plan = plan
+
     def launch(self, it0=0, nsteps=None):
         g, s, t_src, recs, nrec = self._args
         nsteps = self.source.nt - it0 if nsteps is None else nsteps
+        if self.graph_chunk > 0:
+            self.capture()
+            R.check(R.lib().rsg_modelling_plan_run(self._plan, it0, nsteps, _stream()), "rsg_modelling_plan_run")
+            return
         if self._snapsets is not None:
             R.check(R.lib().rsg_modelling_run_snap(self.waves.kind, C.byref(g), C.byref(s), C.byref(t_src), nrec,
                                                    recs, len(self._snapsets), self._snapsets, it0, nsteps,

@@ -7,6 +7,8 @@
 // the loop can be captured as a hipGraph of `graph_chunk` steps and replayed;
 // inside the graph the time index comes from a device counter that the last
 // node of each chunk advances.
+#include <algorithm>
+#include <new>
 #include <vector>
 
 #include "rsg_common.h"
@@ -340,5 +342,126 @@ extern "C" int rsg_repeat_step(int kind, int which, const rsg_grid *g, const rsg
         }
         if (rc) return rc;
     }
+    return RSG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Modelling plan: the time loop of one modelling run captured once as
+// hipGraphs of 1, 2, 4, ..., 2^(nlev-1) steps and replayed for any step
+// range.  A range of n steps replays the largest graph while it fits, then
+// the binary digits of the rest, so no step of a run is launched eagerly and
+// no capture happens after set-up (a caller timing short ranges times the
+// replay, not the capture).  The time index comes from a device counter that
+// run() sets before the first replay and that every graph's last node
+// advances by its own step count.
+struct rsg_modelling_plan {
+    int kind = 0;
+    rsg_grid g{};
+    rsg_state s{};
+    rsg_traces src{};
+    bool has_src = false;
+    std::vector<rsg_traces> recs;
+    std::vector<rsg_snapset> snaps;
+    int nt_max = 0;  // steps it0 + nsteps may not pass (wavelet and record lengths)
+    rsg::IoPlan io;
+    int *it_dev = nullptr;
+    std::vector<hipGraphExec_t> exec;  // exec[k] runs 2^k steps
+    ~rsg_modelling_plan() {
+        // replays may still run on the caller's stream: drain before the
+        // graphs, the counter and the receiver tables (io) are released
+        (void)hipDeviceSynchronize();
+        for (hipGraphExec_t e : exec) (void)hipGraphExecDestroy(e);
+        if (it_dev) (void)hipFree(it_dev);
+    }
+};
+
+extern "C" int rsg_modelling_plan_create(int kind, const rsg_grid *g, const rsg_state *s, const rsg_traces *src,
+                                         int nrec, const rsg_traces *recs, int nsnap, const rsg_snapset *snaps,
+                                         int max_chunk, void *stream, rsg_modelling_plan **out) {
+    if (!out) return fail(RSG_EINVAL, "rsg_modelling_plan_create: out is NULL");
+    *out = nullptr;
+    if (!g || !s || nrec < 0 || (nrec > 0 && !recs) || nsnap < 0 || (nsnap > 0 && !snaps) || max_chunk < 1 ||
+        max_chunk > (1 << 12))
+        return fail(RSG_EINVAL, "rsg_modelling_plan_create: bad arguments (max_chunk 1..4096)");
+    if (kind != RSG_KIND_A2D && kind != RSG_KIND_E2D && kind != RSG_KIND_A3D)
+        return fail(RSG_EINVAL, "rsg_modelling_plan_create: unknown propagator kind");
+    for (int k = 0; k < nsnap; k++)
+        if (!snaps[k].out || snaps[k].snapinc < 1 || (!snaps[k].f[0] && !snaps[k].f[1] && !snaps[k].f[2]))
+            return fail(RSG_EINVAL, "rsg_modelling_plan_create: a snapshot set needs a field, out and snapinc >= 1");
+    rsg_modelling_plan *p = new (std::nothrow) rsg_modelling_plan;
+    if (!p) return fail(RSG_ENOMEM, "rsg_modelling_plan_create: host allocation");
+    p->kind = kind;
+    p->g = *g;
+    p->s = *s;
+    p->has_src = src != nullptr;
+    if (src) p->src = *src;
+    p->recs.assign(recs, recs + nrec);
+    p->snaps.assign(snaps, snaps + nsnap);
+    p->nt_max = src ? src->nt : 0x7fffffff;
+    for (int r = 0; r < nrec; r++) p->nt_max = std::min(p->nt_max, recs[r].nt);
+    hipStream_t cst = (hipStream_t)stream;
+    hipStream_t cap = nullptr;
+    int rc = RSG_OK;
+    do {
+        if ((rc = make_io_plan(kind, &p->g, p->has_src ? &p->src : nullptr, nrec, p->recs.data(), cst, p->io)))
+            break;
+        if (!p->io.active) p->io.rest = p->recs;
+        if (hipMalloc((void **)&p->it_dev, sizeof(int)) != hipSuccess) {
+            rc = fail(RSG_ENOMEM, "rsg_modelling_plan_create: counter allocation");
+            break;
+        }
+        // captured work does not run: a private stream serves every capture
+        if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
+            rc = fail(RSG_EHIP, "rsg_modelling_plan_create: capture stream");
+            break;
+        }
+        for (int n = 1; n <= max_chunk && rc == RSG_OK; n *= 2) {
+            hipGraph_t graph = nullptr;
+            if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                rc = fail(RSG_EHIP, "hipStreamBeginCapture failed");
+                break;
+            }
+            for (int k = 0; k < n && rc == RSG_OK; k++)
+                rc = one_step(kind, &p->g, &p->s, p->has_src ? &p->src : nullptr, p->io, nsnap, p->snaps.data(),
+                              p->it_dev, k, cap);
+            if (rc == RSG_OK) rc = launch_add_int(p->it_dev, n, cap);
+            const hipError_t e = hipStreamEndCapture(cap, &graph);
+            if (rc == RSG_OK && e != hipSuccess) rc = check_hip(e, "hipStreamEndCapture");
+            if (rc == RSG_OK) {
+                hipGraphExec_t ex = nullptr;
+                const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+                if (ei != hipSuccess) rc = check_hip(ei, "hipGraphInstantiate");
+                else p->exec.push_back(ex);
+            }
+            if (graph) (void)hipGraphDestroy(graph);
+        }
+    } while (0);
+    if (cap) (void)hipStreamDestroy(cap);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return RSG_OK;
+}
+
+extern "C" int rsg_modelling_plan_run(rsg_modelling_plan *p, int it0, int nsteps, void *stream) {
+    if (!p || it0 < 0 || nsteps < 0) return fail(RSG_EINVAL, "rsg_modelling_plan_run: bad arguments");
+    if ((long)it0 + nsteps > p->nt_max) return fail(RSG_EINVAL, "rsg_modelling_plan_run: past wavelet or record end");
+    if (nsteps == 0) return RSG_OK;
+    hipStream_t st = (hipStream_t)stream;
+    if (int rc = launch_set_int(p->it_dev, it0, st)) return rc;
+    for (int k = (int)p->exec.size() - 1; k >= 0; k--) {
+        const int n = 1 << k;
+        for (; nsteps >= n; nsteps -= n) {
+            const hipError_t e = hipGraphLaunch(p->exec[k], st);
+            if (e != hipSuccess) return check_hip(e, "hipGraphLaunch");
+        }
+    }
+    return RSG_OK;
+}
+
+extern "C" int rsg_modelling_plan_destroy(rsg_modelling_plan *p) {
+    delete p;
     return RSG_OK;
 }

@@ -349,6 +349,7 @@ def batch_gradient(vp, rho, survey: Survey, mode=None):
     step runs all shots in one launch.  Shots must share nt and their source
     and receiver counts.  Returns (misfit, vpgrad, rhograd, dmod)."""
     dev = _dev()
+    on_dev = isinstance(vp, torch.Tensor) and vp.is_cuda
     nz, nx = vp.shape
     shots = survey.shots
     ns = len(shots)
@@ -357,8 +358,9 @@ def batch_gradient(vp, rho, survey: Survey, mode=None):
         if sh.scoords.shape[0] != nsrc or sh.gcoords.shape[0] != nrec or sh.wavelet.shape[1] != nt:
             raise R.RsgError("batch_gradient: shots must share nt and trace counts")
     m = ModelAcoustic2D(nx, nz, survey.lpml, survey.dx, survey.dz, 0.0, 0.0, survey.fs)
-    m.Vp[:] = vp
-    m.R[:] = rho
+    if not on_dev:
+        m.Vp[:] = vp
+        m.R[:] = rho
     smaps, sshifts, gmaps, gshifts = [], [], [], []
     for sh in shots:
         d = Data2D(nsrc, nt, survey.dt, 0.0, "P")
@@ -371,7 +373,14 @@ def batch_gradient(vp, rho, survey: Survey, mode=None):
         d.makeMap(m, GMAP)
         gmaps.append(d.gmap)
         gshifts.append(d.gshift)
-    m.staggerModels()
+    if on_dev:
+        # the model stays in HBM: staggered on the device, bit-identical to
+        # the host set-up
+        vp_d = vp.detach().to(dev, torch.float32).contiguous()
+        rho_d = rho.detach().to(dev, torch.float32).contiguous()
+        m.staggerModelsDevice(vp_d, rho_d)
+    else:
+        m.staggerModels()
     pml = Pml(m.lpml, survey.dt)
     pml.setSmax(SMAX)
     pml.computeABC()
@@ -383,7 +392,7 @@ def batch_gradient(vp, rho, survey: Survey, mode=None):
                 gmap=dev_t(np.stack(gmaps), torch.int32), gshift=dev_t(np.stack(gshifts)),
                 wav=dev_t(np.stack([sh.wavelet for sh in shots])),
                 dobs=dev_t(np.stack([sh.dobs for sh in shots])),
-                vp=dev_t(m.Vp), rho=dev_t(m.R))
+                vp=vp_d if on_dev else dev_t(m.Vp), rho=rho_d if on_dev else dev_t(m.R))
     dmod = torch.zeros((ns, nrec, nt), dtype=torch.float32, device=dev)
     dres = torch.zeros_like(dmod)
     vg = torch.zeros((nz, nx), dtype=torch.float32, device=dev)
@@ -422,12 +431,16 @@ class AcousticFwiMisfit(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, vp: torch.Tensor, rho: torch.Tensor, survey: Survey):
-        v = vp.detach().to("cpu", torch.float32).numpy()
-        r = rho.detach().to("cpu", torch.float32).numpy()
         if getattr(survey, "batched", True):
-            total, gv, gr, _ = batch_gradient(v, r, survey)
-            gv, gr = gv.to(vp.dtype), gr.to(rho.dtype)
+            # CUDA inputs stay on the device (staggered there); CPU inputs
+            # go through the host set-up
+            a = (vp, rho) if vp.is_cuda and rho.is_cuda else \
+                tuple(t.detach().to("cpu", torch.float32).numpy() for t in (vp, rho))
+            total, gv, gr, _ = batch_gradient(*a, survey)
+            gv, gr = gv.to(vp.device, vp.dtype), gr.to(rho.device, rho.dtype)
         else:
+            v = vp.detach().to("cpu", torch.float32).numpy()
+            r = rho.detach().to("cpu", torch.float32).numpy()
             total = 0.0
             gvn = np.zeros_like(v)
             grn = np.zeros_like(r)
@@ -505,7 +518,11 @@ class _E2DBatch:
 
     def __init__(self, vp, vs, rho, survey: ElasticSurvey, scale=True, wavgrad=False):
         dev = _dev()
-        vp, vs, rho = (np.ascontiguousarray(a, np.float32) for a in (vp, vs, rho))
+        self.on_dev = isinstance(vp, torch.Tensor) and vp.is_cuda
+        if self.on_dev:
+            vp, vs, rho = (t.detach().to(dev, torch.float32).contiguous() for t in (vp, vs, rho))
+        else:
+            vp, vs, rho = (np.ascontiguousarray(a, np.float32) for a in (vp, vs, rho))
         nz, nx = vp.shape
         self.survey, self.nx, self.nz = survey, nx, nz
         ns = len(survey.shots)
@@ -523,8 +540,12 @@ class _E2DBatch:
                 raise R.RsgError(f"elastic_batch_gradient: window of shot {k} outside the model")
             m = ModelElastic2D(nxw, nz, survey.lpml, survey.dx, survey.dz, survey.ox + sh.x0 * survey.dx,
                                survey.oz, survey.fs)
-            m.Vp[:], m.Vs[:], m.R[:] = vp[:, sh.x0:sh.x0 + nxw], vs[:, sh.x0:sh.x0 + nxw], rho[:, sh.x0:sh.x0 + nxw]
-            m.staggerModels()
+            win = [a[:, sh.x0:sh.x0 + nxw] for a in (vp, vs, rho)]
+            if self.on_dev:
+                m.staggerModelsDevice(*win)
+            else:
+                m.Vp[:], m.Vs[:], m.R[:] = win
+                m.staggerModels()
             self.models.append(m)
             self.windows.append((sh.x0, nxw))
             nt = sh.wavelet.shape[1]
@@ -538,7 +559,11 @@ class _E2DBatch:
             s.g = R.Grid(nx=m.getNx_pml(), ny=1, nz=m.getNz_pml(), lpml=m.lpml, order=survey.order, fs=int(m.fs),
                          dx=m.dx, dy=1.0, dz=m.dz, dt=survey.dt)
             s.L, s.L2M, s.M, s.Rx, s.Rz = (getattr(m, "_d" + n).data_ptr() for n in ("L", "L2M", "M", "Rx", "Rz"))
-            tv = [dev_t(a) for a in (m.Vp, m.Vs, m.R)]
+            if self.on_dev:
+                tv = [w.contiguous() for w in win]
+                self.keep.extend(tv)
+            else:
+                tv = [dev_t(a) for a in (m.Vp, m.Vs, m.R)]
             m._dwin = tv  # the unpadded window model the gradient scaling reads
             s.Vp, s.Vs, s.Rho = (t.data_ptr() for t in tv)
             s.ox = sh.x0
@@ -582,7 +607,18 @@ class _E2DBatch:
         return autotune.tune_ragged(R.KIND_E2D, [s.g for s in self.shots], self.f.tab, reps)
 
     def set_model(self, vp, vs, rho):
-        """A new model into the same device arrays (restaggered in place)."""
+        """A new model into the same device arrays (restaggered in place;
+        on the device when the model is given as CUDA tensors)."""
+        if isinstance(vp, torch.Tensor) and vp.is_cuda:
+            vp, vs, rho = (t.detach().to(vp.device, torch.float32).contiguous() for t in (vp, vs, rho))
+            if tuple(vp.shape) != (self.nz, self.nx):
+                raise R.RsgError("set_model: model shape differs from the plan's")
+            for m, (x0, nxw) in zip(self.models, self.windows):
+                win = [a[:, x0:x0 + nxw] for a in (vp, vs, rho)]
+                m.staggerModelsDevice(*win)
+                for t, w in zip(m._dwin, win):
+                    t.copy_(w)
+            return
         vp, vs, rho = (np.ascontiguousarray(a, np.float32) for a in (vp, vs, rho))
         if vp.shape != (self.nz, self.nx):
             raise R.RsgError("set_model: model shape differs from the plan's")
@@ -770,9 +806,10 @@ class ElasticFwiMisfit(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, vp: torch.Tensor, vs: torch.Tensor, rho: torch.Tensor, survey: ElasticSurvey):
-        a = [t.detach().to("cpu", torch.float32).numpy() for t in (vp, vs, rho)]
+        t3 = (vp, vs, rho)
+        a = t3 if all(t.is_cuda for t in t3) else [t.detach().to("cpu", torch.float32).numpy() for t in t3]
         total, g0, g1, g2, _ = elastic_batch_gradient(*a, survey, scale=True)
-        ctx.save_for_backward(g0.to(vp.dtype), g1.to(vs.dtype), g2.to(rho.dtype))
+        ctx.save_for_backward(g0.to(vp.device, vp.dtype), g1.to(vs.device, vs.dtype), g2.to(rho.device, rho.dtype))
         return torch.tensor(total, dtype=torch.float64, device=vp.device)
 
     @staticmethod

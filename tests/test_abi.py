@@ -253,3 +253,32 @@ def test_fwi_e2d_plan_rejects_bad_arguments():
     assert h.value is None
     assert L.rsg_fwi_e2d_plan_run(None, None, None) != 0
     assert L.rsg_fwi_e2d_plan_destroy(None) == 0
+
+
+def test_modelling_plan_rejects_bad_arguments():
+    """rsg_modelling_plan_create / _run validate their arguments before any
+    device work: null output, null grid or state, a chunk outside 1..4096,
+    an unknown kind, a snapshot set without a field; run and destroy take
+    a null plan."""
+    L = R.lib()
+    h = C.c_void_p()
+    g, s = R.Grid(), R.State()
+    assert L.rsg_modelling_plan_create(R.KIND_A2D, C.byref(g), C.byref(s), None, 0, None, 0, None, 8,
+                                       None, None) != 0
+    assert L.rsg_modelling_plan_create(R.KIND_A2D, None, C.byref(s), None, 0, None, 0, None, 8,
+                                       None, C.byref(h)) != 0
+    for chunk in (0, 5000):
+        assert L.rsg_modelling_plan_create(R.KIND_A2D, C.byref(g), C.byref(s), None, 0, None, 0, None,
+                                           chunk, None, C.byref(h)) != 0
+        assert b"max_chunk" in L.rsg_last_error()
+    assert L.rsg_modelling_plan_create(9, C.byref(g), C.byref(s), None, 0, None, 0, None, 8,
+                                       None, C.byref(h)) != 0
+    sn = (R.SnapSet * 1)()
+    sn[0].snapinc = 1
+    sn[0].out = 1
+    assert L.rsg_modelling_plan_create(R.KIND_A2D, C.byref(g), C.byref(s), None, 0, None, 1, sn, 8,
+                                       None, C.byref(h)) != 0
+    assert b"snapshot" in L.rsg_last_error()
+    assert h.value is None
+    assert L.rsg_modelling_plan_run(None, 0, 1, None) != 0
+    assert L.rsg_modelling_plan_destroy(None) == 0

@@ -137,7 +137,11 @@ def _run_full_on_case0(g, fp16):
     return dict(vpgrad=vg.imagedata, rhograd=rg.imagedata)
 
 
-def test_autograd_function_matches_driver(golden):
+@pytest.mark.parametrize("device_inputs", (True, False), ids=("cuda", "cpu"))
+def test_autograd_function_matches_driver(golden, device_inputs):
+    """AcousticFwiMisfit against the reference fixture, with CUDA inputs
+    (model staggered on the device) and CPU inputs (gradients back on the
+    CPU)."""
     import torch
     from rockseis_amd import fwi
     g = golden("fwi_case0")
@@ -145,10 +149,12 @@ def test_autograd_function_matches_driver(golden):
     nx, nz, lpml, dx, dt, nt = g["grid"]
     survey = fwi.Survey([fwi.Shot(g["scoords"], g["wav"], g["gcoords"], g["dobs"])], int(lpml), dx, dx, dt,
                         order=order, fs=bool(fs), ncheck=ncheck, mode="store")
-    vp = torch.tensor(g["vp"], device="cuda", requires_grad=True)
-    rho = torch.tensor(g["rho"], device="cuda", requires_grad=True)
+    dev = "cuda" if device_inputs else "cpu"
+    vp = torch.tensor(g["vp"], device=dev, requires_grad=True)
+    rho = torch.tensor(g["rho"], device=dev, requires_grad=True)
     f = fwi.misfit(vp, rho, survey)
     f.backward()
+    assert vp.grad.device == vp.device and rho.grad.device == rho.device
     exact = 0.5 * np.sum((g["dmod"].astype(np.float64) - g["dobs"].astype(np.float64)) ** 2)
     assert abs(float(f) - exact) <= RTOL_L2 * exact
     assert rel_l2(vp.grad.cpu().numpy(), g["vpgrad"]) <= RTOL_L2

@@ -55,9 +55,50 @@ def test_a2d_graph_replay_matches_eager(golden):
     args = (g["vp"], g["rho"], int(lpml), dx, dt, int(nt), order, fs, sf, g["scoords"], g["wav"], rf,
             g["gcoords"])
     r0, w0 = _a2d_modelling(core, *args, graph=0)
-    r1, w1 = _a2d_modelling(core, *args, graph=32)   # 9 replays of 32 steps + 12 eager steps
+    r1, w1 = _a2d_modelling(core, *args, graph=32)   # graphs of 1..32 steps: 9 x 32 + 8 + 4
     assert np.array_equal(r0, r1)
     assert np.array_equal(w0.P.cpu().numpy(), w1.P.cpu().numpy())
+
+
+@pytest.mark.parametrize("chunk", (1, 16, 128))
+def test_modelling_plan_ranges_match_eager(chunk):
+    """rsg_modelling_plan_run over consecutive step ranges of awkward lengths
+    (1, 19, 37, the rest) from one plan, with a snapshot set, gives the
+    records, fields and snapshots of the eager per-step loop bit for bit."""
+    from rockseis_amd import synth
+    core = _core()
+    nx, nz, lpml, dx, dt, nt = 150, 90, 12, 5.0, 5e-4, 140
+    vp = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1500, 3500, nanom=4, seed=5)
+    out = {}
+    for graph in (0, chunk):
+        m = core.ModelElastic2D(nx, nz, lpml, dx, dx, 0.0, 0.0, True)
+        m.Vp[:], m.Vs[:], m.R[:] = vp, vp / 1.73, synth.gardner(vp)
+        src = data2d(np.float32([[370.0, 20.0]]), nt, dt, "P", synth.ricker(nt, dt, 20.0))
+        src.makeMap(m, core.SMAP)
+        recs = {}
+        for f in ("VX", "VZ", "P"):
+            r = data2d(synth.surface_line(nx, dx, 12.5), nt, dt, f)
+            r.makeMap(m, core.GMAP)
+            recs[f] = r
+        m.staggerModels()
+        mod = core.ModellingElastic2D(m, src, 8, 3)
+        mod.setRecVx(recs["VX"])
+        mod.setRecVz(recs["VZ"])
+        mod.setRecP(recs["P"])
+        mod.setSnapSxz()
+        mod.setGraphChunk(graph)
+        mod.prepare()
+        it = 0
+        for n in (1, 19, 37, nt - 57):
+            mod.launch(it, n)
+            it += n
+        mod.finish()
+        w = mod.waves
+        out[graph] = ([recs[f].data.copy() for f in ("VX", "VZ", "P")] + [mod.snaps["SXZ"]] +
+                      [getattr(w, k).cpu().numpy() for k in ("Vx", "Vz", "Sxx", "Szz", "Sxz")])
+    print(f"modelling plan, graphs up to {chunk} steps: {len(out[0])} arrays compared", flush=True)
+    for a, b in zip(out[0], out[chunk]):
+        assert np.array_equal(a, b)
 
 
 def test_a2d_per_step_api_and_snapshots(golden):
