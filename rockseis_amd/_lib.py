@@ -113,6 +113,11 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBPATH):
             raise RsgError(f"librsgpu.so not built ({LIBPATH}); run __graft_entry__.build()")
+        # torch ships its own HIP runtime under the soname the library needs
+        # (libamdhip64.so.7).  With torch loaded first the library binds to
+        # that one runtime; loaded the other way round, the process holds two
+        # HIP/HSA runtimes and the library's finds no device.
+        import torch  # noqa: F401
         L = C.CDLL(LIBPATH)
         _declare(L)
         _lib = L
