@@ -271,6 +271,22 @@ int rsg_modelling_plan_create(int kind, const rsg_grid *g, const rsg_state *s, c
 int rsg_modelling_plan_run(rsg_modelling_plan *p, int it0, int nsteps, void *stream);
 int rsg_modelling_plan_destroy(rsg_modelling_plan *p);
 
+/* ---- storage of FWI snapshots and Revolve checkpoints (the fp16 member of
+ * the FWI settings below; the reference keeps them as T = float, revolve.cpp
+ * 648-1271): stencil arithmetic stays FP32 in every case.
+ *   RSG_STORE_FP32  plain copies;
+ *   RSG_STORE_FP16  FP16 with one power-of-two scale per stored array (per
+ *                   field or PML strip of a shot): 11 significant bits;
+ *   RSG_STORE_S16   16-bit block floating point: blocks of 256 consecutive
+ *                   values share a power-of-two exponent, each value is a
+ *                   signed 16-bit integer (error <= 2^-16 of the block's
+ *                   largest magnitude), 2 bytes a value plus 2 a block.
+ * Both 16-bit kinds halve the memory of FP32; S16 keeps the gradient error
+ * against an FP64 computation about five times lower than FP16. */
+#define RSG_STORE_FP32 0
+#define RSG_STORE_FP16 1
+#define RSG_STORE_S16 2
+
 /* ---- FWI gradient of one shot (FwiAcoustic2D::run_optimal, fwi.cpp:1159-1332,
  * and FwiAcoustic2D::run, fwi.cpp:1030-1156) */
 #define RSG_FWI_REVOLVE 0 /* run_optimal: Revolve checkpoints in HBM (ncheck)        */
@@ -310,8 +326,9 @@ typedef struct {
     int *mode_used;                 /* out (host, optional): STORE falls back to
                                        REVOLVE when the history does not fit        */
     int snapinc;                    /* RSG_FWI_FULL: snapshot increment (Fwi snapinc) */
-    int fp16;                       /* store snapshots / checkpoints as FP16 with a
-                                       power-of-two scale per stored array          */
+    int fp16;                       /* storage of snapshots / checkpoints
+                                       (RSG_STORE_*): FP32, FP16 with a power-of-two
+                                       scale per stored array, or S16              */
     int misfit_type;                /* rs_fwimisfit (utils.h:26): 0 DIFFERENCE,
                                        1 CORRELATION, 2 ADAPTIVE_GAUSS,
                                        3 ADAPTIVE_LINEAR, 4 ADAPTIVE_SINGLE (the
@@ -373,7 +390,7 @@ typedef struct {
     float *vpgrad, *rhograd;             /* in/out: accumulated gradients            */
     double *misfit;                      /* out (host)                               */
     int ncheck;                          /* Revolve snapshots                        */
-    int fp16;                            /* FP16 checkpoints                         */
+    int fp16;                            /* checkpoint storage, RSG_STORE_*          */
     int misfit_type;                     /* 0 DIFFERENCE, 1 CORRELATION, 2 ADAPTIVE_GAUSS,
                                             3 ADAPTIVE_LINEAR, 4 (= DIFFERENCE); the
                                             residuals are not percentile-clipped
@@ -459,7 +476,7 @@ typedef struct {
 typedef struct {
     const float *tab;  /* PML tables of the shared lpml and dt (device)              */
     int ncheck;        /* Revolve snapshots                                            */
-    int fp16;          /* checkpoints as FP16 with one power-of-two scale per array   */
+    int fp16;          /* checkpoint storage, RSG_STORE_*                              */
     int scale;         /* 1: Vp / Vs / rho gradients; 0: lambda / mu / rho kernels     */
     int nx, nz;        /* stacked image size                                           */
     float *grad[3];    /* in/out (device): gradients added here, nz*nx each            */

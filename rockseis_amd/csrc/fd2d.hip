@@ -45,6 +45,47 @@ extern "C" int rsg_a2d_forward_stress_batch(const rsg_grid *g, const rsg_state *
     return RSG_OK;
 }
 
+namespace rsg {
+
+// One forward acoustic step of a batch (velocity, then stress) out of place:
+// the previous state is read from `in` (a checkpoint buffer, left as it is)
+// and the new one written to `out`, bit for bit what the in-place pair of
+// launches gives on a copy of `in`.
+int a2d_forward_step_from(const rsg_grid *g, const rsg_state *in, const rsg_state *out, const rsg_batch *b,
+                          hipStream_t stream) {
+    if (int rc = check_grid2d(g, out)) return rc;
+    if (int rc = check_batch(b)) return rc;
+    if (!in || !in->P || !in->Vx || !in->Vz || !out->P || !out->Vx || !out->Vz || !out->Rx || !out->Rz || !out->L)
+        return fail(RSG_EINVAL, "a2d: null array");
+    for (int i = 0; i < RSG_A2D_NSTRIP; i++)
+        if (!in->psi[i] || !out->psi[i]) return fail(RSG_EINVAL, "a2d: null PML strip");
+    const rsg_state *s = out;
+#define GOI(KERNEL, O, RZ)                                                                        \
+    do {                                                                                          \
+        K2D a = make_k2d(g, s, BZ * RZ);                                                          \
+        a.sf = (long)b->field_stride;                                                             \
+        a.sxs = (long)b->xstrip_stride;                                                           \
+        a.szs = (long)b->zstrip_stride;                                                           \
+        a.iP = in->P;                                                                             \
+        a.iVx = in->Vx;                                                                           \
+        a.iVz = in->Vz;                                                                           \
+        for (int i = 0; i < RSG_A2D_NSTRIP; i++) a.ipsi[i] = in->psi[i];                          \
+        KERNEL<<<dim3(nblocks(a), b->nshot), dim3(BX, BZ), 0, stream>>>(a);                       \
+    } while (0)
+#define L_(O, RZ) GOI((a2d_velocity_kernel<O, RZ>), O, RZ)
+    RSG_DISPATCH(L_);
+#undef L_
+    RSG_LAUNCH_CHECK("a2d_velocity_kernel (out of place)");
+#define L_(O, RZ) GOI((a2d_stress_kernel<O, RZ>), O, RZ)
+    RSG_DISPATCH(L_);
+#undef L_
+#undef GOI
+    RSG_LAUNCH_CHECK("a2d_stress_kernel (out of place)");
+    return RSG_OK;
+}
+
+}  // namespace rsg
+
 static int e2d_check(const rsg_grid *g, const rsg_state *s) {
     if (int rc = check_grid2d(g, s)) return rc;
     if (!s->Vx || !s->Vz || !s->Sxx || !s->Szz || !s->Sxz || !s->L || !s->L2M || !s->M || !s->Rx ||

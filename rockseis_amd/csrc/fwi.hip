@@ -170,11 +170,12 @@ Revolve::Action Revolve::next() {
 // double temporaries.  A batch of nshot shots (fields fstride apart) is
 // summed into the one image, shot 0 first.
 // Pf is either the padded forward pressure (pf_pad = padr) or a stored
-// unpadded snapshot (pf_pad = 0), FP32 or (F16) FP16 scaled by amax[shot];
+// unpadded snapshot (pf_pad = 0): FP32, FP16 scaled by aux = amax[shot], or
+// S16 with aux = the block exponents of the flat [nshot][nz][nx] snapshot;
 // pf_stride separates its shots.  srcilum (optional) takes the source
 // illumination -(-2/Vp) Pf^2 (fwi.cpp:567-569).
-template <bool F16>
-__global__ void crosscorr_a2d_kernel(const void *__restrict__ Pf_, const unsigned *__restrict__ amax,
+template <int STORE>
+__global__ void crosscorr_a2d_kernel(const void *__restrict__ Pf_, const void *__restrict__ aux,
                                      const float *__restrict__ Vxa, const float *__restrict__ Vza,
                                      const float *__restrict__ Rx, const float *__restrict__ Rz,
                                      const float *__restrict__ Vp, const float *__restrict__ Rho, int nx,
@@ -195,9 +196,18 @@ __global__ void crosscorr_a2d_kernel(const void *__restrict__ Pf_, const unsigne
     float vg = vpgrad[m], rg = rhograd[m], si = srcilum ? srcilum[m] : 0.f;
     for (int k = 0; k < nshot; k++) {
         float pc, pe, pw, ps, pn;
-        if constexpr (F16) {
+        if constexpr (STORE == kStoreS16) {
+            const short *Q = (const short *)Pf_;
+            const short *E = (const short *)aux;
+            const size_t o = k * pf_stride + qp;
+            pc = s16_value(Q, E, o);
+            pe = s16_value(Q, E, o + 1);
+            pw = s16_value(Q, E, o - 1);
+            ps = s16_value(Q, E, o + pld);
+            pn = s16_value(Q, E, o - pld);
+        } else if constexpr (STORE == kStoreF16) {
             const half_t *P = (const half_t *)Pf_ + k * pf_stride;
-            const float us = store16_unscale(amax[k]);
+            const float us = store16_unscale(((const unsigned *)aux)[k]);
             pc = (float)P[qp] * us;
             pe = (float)P[qp + 1] * us;
             pw = (float)P[qp - 1] * us;
@@ -267,23 +277,30 @@ struct A2DWaves {
         if (int rc = db.alloc(&base, total)) return rc;
         if (int rc = launch_zero(base, total * sizeof(float), s)) return rc;
         st = rsg_state{};
-        st.P = base;
-        st.Vx = base + ns * n;
-        st.Vz = base + 2 * ns * n;
-        float *p = base + 3 * ns * n;
-        const int xs[4] = {RSG_A2D_P_LEFT, RSG_A2D_VXX_LEFT, RSG_A2D_P_RIGHT, RSG_A2D_VXX_RIGHT};
-        const int zs[4] = {RSG_A2D_P_TOP, RSG_A2D_VZZ_TOP, RSG_A2D_P_BOTTOM, RSG_A2D_VZZ_BOTTOM};
-        for (int i = 0; i < 4; i++, p += ns * nstrip_x) st.psi[xs[i]] = p;
-        for (int i = 0; i < 4; i++, p += ns * nstrip_z) st.psi[zs[i]] = p;
         st.L = f->L;
         st.Rx = f->Rx;
         st.Rz = f->Rz;
         st.tab = f->tab;
+        st = at(base);
         b.nshot = ns;
         b.field_stride = n;
         b.xstrip_stride = nstrip_x;
         b.zstrip_stride = nstrip_z;
         return RSG_OK;
+    }
+    // The state laid out over any buffer of `total` values (the model and
+    // PML tables of st): P, Vx, Vz, then the x strips and the z strips.
+    rsg_state at(float *buf) const {
+        rsg_state r = st;
+        r.P = buf;
+        r.Vx = buf + ns * n;
+        r.Vz = buf + 2 * ns * n;
+        float *p = buf + 3 * ns * n;
+        const int xs[4] = {RSG_A2D_P_LEFT, RSG_A2D_VXX_LEFT, RSG_A2D_P_RIGHT, RSG_A2D_VXX_RIGHT};
+        const int zs[4] = {RSG_A2D_P_TOP, RSG_A2D_VZZ_TOP, RSG_A2D_P_BOTTOM, RSG_A2D_VZZ_BOTTOM};
+        for (int i = 0; i < 4; i++, p += ns * nstrip_x) r.psi[xs[i]] = p;
+        for (int i = 0; i < 4; i++, p += ns * nstrip_z) r.psi[zs[i]] = p;
+        return r;
     }
 };
 
@@ -308,8 +325,11 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
     const bool exact = f->adjoint == RSG_ADJOINT_EXACT;
     if (exact && (f->mode != RSG_FWI_REVOLVE || (f->misfit_type != 0 && f->misfit_type != 4)))
         return fail(RSG_EINVAL, "fwi: the exact adjoint takes RSG_FWI_REVOLVE and the DIFFERENCE misfit");
-    const bool f16 = f->fp16 != 0;
-    if (f16 && 4 * ns > 65535) return fail(RSG_EINVAL, "fwi: FP16 storage takes at most 16383 shots");
+    const int store = f->fp16;
+    if (store != kStoreF32 && store != kStoreF16 && store != kStoreS16)
+        return fail(RSG_EINVAL, "fwi: storage must be 0 (FP32), 1 (FP16) or 2 (S16)");
+    const bool f16 = store != kStoreF32;  // 2 bytes a stored value
+    if (store == kStoreF16 && 4 * ns > 65535) return fail(RSG_EINVAL, "fwi: FP16 storage takes at most 16383 shots");
     DevBuf db;
     A2DWaves fw, bw;
     if (int rc = fw.init(db, g, f, ns, st)) return rc;
@@ -349,8 +369,8 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
                                          f->obs.ntrace, nt, ns, 1, misfit_d, st);
     };
     // Residual injection at time t, crossCorr with Pf (skipped when null;
-    // FP16 when amax is set) and the wavelet gradient at t.
-    auto reverse_tail = [&](int t, const void *Pf, int pf_pad, size_t pf_stride, const unsigned *amax) -> int {
+    // FP16 or S16 as `store` when aux is set) and the wavelet gradient at t.
+    auto reverse_tail = [&](int t, const void *Pf, int pf_pad, size_t pf_stride, const void *aux) -> int {
         if (oi.maxcells > 0) {
             inject2d_ordered_kernel<<<dim3((oi.maxcells + 63) / 64, ns), 64, 0, st>>>(
                 bw.st.P, f->L, g->dt, oi.cells, oi.starts, oi.ents, oi.coff, res.shift, res.data, res.ntrace, nt, t,
@@ -358,16 +378,21 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
             RSG_LAUNCH_CHECK("inject2d_ordered_kernel");
         }
         if (Pf) {
-            if (amax)
-                crosscorr_a2d_kernel<true><<<xg, xb, 0, st>>>(Pf, amax, bw.st.Vx, bw.st.Vz, f->Rx, f->Rz, f->Vp,
-                                                              f->Rho, f->nx, f->nz, pad, g->dx, g->dz,
-                                                              f->vpgrad, f->rhograd, f->srcilum, ns, fs, pf_pad,
-                                                              pf_stride);
+            if (aux && store == kStoreS16)
+                crosscorr_a2d_kernel<kStoreS16><<<xg, xb, 0, st>>>(Pf, aux, bw.st.Vx, bw.st.Vz, f->Rx, f->Rz, f->Vp,
+                                                                   f->Rho, f->nx, f->nz, pad, g->dx, g->dz,
+                                                                   f->vpgrad, f->rhograd, f->srcilum, ns, fs, pf_pad,
+                                                                   pf_stride);
+            else if (aux)
+                crosscorr_a2d_kernel<kStoreF16><<<xg, xb, 0, st>>>(Pf, aux, bw.st.Vx, bw.st.Vz, f->Rx, f->Rz, f->Vp,
+                                                                   f->Rho, f->nx, f->nz, pad, g->dx, g->dz,
+                                                                   f->vpgrad, f->rhograd, f->srcilum, ns, fs, pf_pad,
+                                                                   pf_stride);
             else
-                crosscorr_a2d_kernel<false><<<xg, xb, 0, st>>>(Pf, nullptr, bw.st.Vx, bw.st.Vz, f->Rx, f->Rz,
-                                                               f->Vp, f->Rho, f->nx, f->nz, pad, g->dx,
-                                                               g->dz, f->vpgrad, f->rhograd, f->srcilum, ns, fs,
-                                                               pf_pad, pf_stride);
+                crosscorr_a2d_kernel<kStoreF32><<<xg, xb, 0, st>>>(Pf, nullptr, bw.st.Vx, bw.st.Vz, f->Rx, f->Rz,
+                                                                   f->Vp, f->Rho, f->nx, f->nz, pad, g->dx,
+                                                                   g->dz, f->vpgrad, f->rhograd, f->srcilum, ns, fs,
+                                                                   pf_pad, pf_stride);
             RSG_LAUNCH_CHECK("crosscorr_a2d_kernel");
         }
         if (f->wavgrad)
@@ -376,7 +401,8 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
     };
 
     // Snapshot history of the STORE and FULL modes: nslot slots of the
-    // unpadded pressure of every shot, FP32 or FP16 (+ one amax per shot).
+    // unpadded pressure of every shot, FP32, FP16 (+ one amax per shot) or
+    // S16 (+ the exponents of the slot's blocks).
     const size_t nint = (size_t)f->nx * f->nz, slot = (size_t)ns * nint;
     const size_t esz = f16 ? sizeof(half_t) : sizeof(float);
     int mode = f->mode, snapnt = nt, enddiff = 0;
@@ -393,15 +419,22 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
     if (f->mode_used) *f->mode_used = mode;
     char *hist = nullptr;
     unsigned *hamax = nullptr;
+    short *hexp = nullptr;
+    const size_t sblk = s16_blocks(slot);
     if (mode != RSG_FWI_REVOLVE) {
         if (int rc = db.alloc(&hist, slot * (size_t)snapnt * esz)) return rc;
-        if (f16)
+        if (store == kStoreF16)
             if (int rc = db.alloc(&hamax, (size_t)ns * snapnt)) return rc;
+        if (store == kStoreS16)
+            if (int rc = db.alloc(&hexp, sblk * snapnt)) return rc;
     }
     auto snap_at = [&](int k) -> void * { return hist + slot * (size_t)k * esz; };
-    auto amax_at = [&](int k) -> unsigned * { return f16 ? hamax + (size_t)ns * k : nullptr; };
+    auto amax_at = [&](int k) -> const void * {
+        return store == kStoreF16 ? (const void *)(hamax + (size_t)ns * k)
+                                  : store == kStoreS16 ? (const void *)(hexp + sblk * k) : nullptr;
+    };
     auto take_snap = [&](int k) -> int {
-        return pack_interior(fw.st.P, snap_at(k), amax_at(k), f->nx, f->nz, pad, fw.n, ns, f16, st);
+        return pack_interior(fw.st.P, snap_at(k), (void *)amax_at(k), f->nx, f->nz, pad, fw.n, ns, store, st);
     };
 
     if (mode == RSG_FWI_STORE) {
@@ -463,7 +496,25 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
         half_t *ck16 = nullptr;
         unsigned *ckamax = nullptr;
         const int nseg = 11 * ns;
-        if (f16) {
+        // S16: one segment per array of each shot (blocks restart at every
+        // field and strip); a checkpoint is cblk blocks of kS16Block values
+        std::vector<size_t> soff(1, 0), sblkoff;
+        short *cks = nullptr, *ckexp = nullptr;
+        size_t *dsoff = nullptr, *dsblk = nullptr, cblk = 0;
+        if (store == kStoreS16) {
+            const size_t lens[3] = {fw.n, fw.nstrip_x, fw.nstrip_z};
+            const int narr[3] = {3, 4, 4};
+            for (int g3 = 0; g3 < 3; g3++)
+                for (int a = 0; a < narr[g3] * ns; a++) soff.push_back(soff.back() + lens[g3]);
+            sblkoff = s16_block_offsets(soff);
+            cblk = sblkoff.back();
+            if (int rc = db.alloc(&cks, cblk * kS16Block * (size_t)rev.snaps)) return rc;
+            if (int rc = db.alloc(&ckexp, cblk * rev.snaps)) return rc;
+            if (int rc = db.alloc(&dsoff, soff.size())) return rc;
+            if (int rc = db.alloc(&dsblk, sblkoff.size())) return rc;
+            RSG_HIP(hipMemcpyAsync(dsoff, soff.data(), sizeof(size_t) * soff.size(), hipMemcpyHostToDevice, st));
+            RSG_HIP(hipMemcpyAsync(dsblk, sblkoff.data(), sizeof(size_t) * sblkoff.size(), hipMemcpyHostToDevice, st));
+        } else if (f16) {
             if (int rc = db.alloc(&ck16, fw.total * (size_t)rev.snaps)) return rc;
             if (int rc = db.alloc(&ckamax, (size_t)nseg * rev.snaps)) return rc;
         } else {
@@ -471,6 +522,9 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
         }
         const size_t xoff = 3 * fw.n * ns, zoff = xoff + 4 * fw.nstrip_x * ns;
         auto ck_pack = [&](int c) -> int {
+            if (store == kStoreS16)
+                return pack_s16(fw.base, cks + cblk * kS16Block * c, ckexp + cblk * c, dsoff, dsblk,
+                                (int)soff.size() - 1, cblk, st);
             half_t *d = ck16 + fw.total * c;
             unsigned *am = ckamax + (size_t)nseg * c;
             if (int rc = pack16_segments(fw.base, d, am, fw.n, 3 * ns, st)) return rc;
@@ -479,6 +533,9 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
             return pack16_segments(fw.base + zoff, d + zoff, am + 7 * ns, fw.nstrip_z, 4 * ns, st);
         };
         auto ck_unpack = [&](int c) -> int {
+            if (store == kStoreS16)
+                return unpack_s16(cks + cblk * kS16Block * c, ckexp + cblk * c, fw.base, dsoff, dsblk,
+                                  (int)soff.size() - 1, cblk, st);
             const half_t *d = ck16 + fw.total * c;
             const unsigned *am = ckamax + (size_t)nseg * c;
             if (int rc = unpack16_segments(d, am, fw.base, fw.n, 3 * ns, st)) return rc;
@@ -486,6 +543,52 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
                 return rc;
             return unpack16_segments(d + zoff, am + 7 * ns, fw.base + zoff, fw.nstrip_z, 4 * ns, st);
         };
+        // FP32 checkpoints of the reference adjoint are kept by reference, not
+        // copied: the forward state lives in one of rev.snaps + 2 buffers
+        // (fw.base, a spare, the checkpoint slots).  TAKESHOT hands the
+        // current buffer to the checkpoint, RESTORE points the current state
+        // at it, and the next forward step after either runs out of place
+        // from that buffer into a free one, so the checkpoint is never
+        // written.  A turn only reads the forward pressure, straight from the
+        // checkpoint (1674 of the 2999 restores at nt 3000 / 50 snapshots
+        // are followed by a turn, the rest by a forward step).
+        const bool zc = store == kStoreF32 && !exact;
+        std::vector<float *> pool, slot(rev.snaps, nullptr);
+        float *cur = fw.base;
+        bool cur_ro = false;  // cur is a checkpoint's buffer
+        if (zc) {
+            float *spare = nullptr;
+            if (int rc = db.alloc(&spare, fw.total)) return rc;
+            pool.push_back(spare);
+            for (int c = rev.snaps - 1; c >= 0; c--) pool.push_back(ck + fw.total * c);
+        }
+        auto take_free = [&]() -> float * {
+            if (pool.empty()) return nullptr;
+            float *p = pool.back();
+            pool.pop_back();
+            return p;
+        };
+        auto zstep = [&](int it, bool rec) -> int {
+            rsg_state cs = fw.at(cur);
+            if (cur_ro) {
+                float *nb = take_free();
+                if (!nb) return fail(RSG_EINVAL, "fwi: no free checkpoint buffer");
+                const rsg_state out = fw.at(nb);
+                if (int rc = a2d_forward_step_from(g, &cs, &out, &fw.b, st)) return rc;
+                cur = nb;
+                cur_ro = false;
+                cs = out;
+            } else {
+                if (int rc = rsg_a2d_forward_velocity_batch(g, &cs, &fw.b, st)) return rc;
+                if (int rc = rsg_a2d_forward_stress_batch(g, &cs, &fw.b, st)) return rc;
+            }
+            if (int rc = insert_source(RSG_KIND_A2D, g, &cs, &f->src, nullptr, it, st, ns, fs)) return rc;
+            if (rec)
+                if (int rc = record_data(RSG_KIND_A2D, g, &cs, &mod, nullptr, it, st, ns, fs)) return rc;
+            return RSG_OK;
+        };
+        auto step = [&](int it, bool rec) -> int { return zc ? zstep(it, rec) : fstep(it, rec); };
+        const float *const *Pcur = zc ? (const float *const *)&cur : (const float *const *)&fw.st.P;
         bool reverse = false;
         for (;;) {
             const int oldcapo = rev.capo;
@@ -493,18 +596,18 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
             const int capo = rev.capo;
             if (a == Revolve::ADVANCE) {
                 for (int it = oldcapo; it < capo; it++)
-                    if (int rc = fstep(it, !reverse)) return rc;
+                    if (int rc = step(it, !reverse)) return rc;
             } else if (a == Revolve::FIRSTURN) {
                 if (exact)
                     RSG_HIP(hipMemcpyAsync(turn_state, fw.base, fw.total * sizeof(float), hipMemcpyDeviceToDevice,
                                            st));
-                if (int rc = fstep(capo, true)) return rc;
+                if (int rc = step(capo, true)) return rc;
                 if (int rc = first_turn()) return rc;
                 if (exact) {
                     RSG_HIP(hipMemcpyAsync(fw.base, turn_state, fw.total * sizeof(float), hipMemcpyDeviceToDevice,
                                            st));
                     if (int rc = exact_step(capo)) return rc;
-                } else if (int rc = reverse_tail(capo, fw.st.P, pad, fw.n, nullptr)) {
+                } else if (int rc = reverse_tail(capo, *Pcur, pad, fw.n, nullptr)) {
                     return rc;
                 }
                 reverse = true;
@@ -515,7 +618,25 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
                     continue;
                 }
                 if (int rc = bstep()) return rc;
-                if (int rc = reverse_tail(capo, fw.st.P, pad, fw.n, nullptr)) return rc;
+                if (int rc = reverse_tail(capo, *Pcur, pad, fw.n, nullptr)) return rc;
+            } else if (a == Revolve::TAKESHOT && zc) {
+                float *&sl = slot[rev.check];
+                if (!cur_ro) {
+                    if (sl) pool.push_back(sl);
+                    sl = cur;
+                    cur_ro = true;
+                } else if (sl != cur) {
+                    float *nb = take_free();
+                    if (!nb) return fail(RSG_EINVAL, "fwi: no free checkpoint buffer");
+                    RSG_HIP(hipMemcpyAsync(nb, cur, fw.total * sizeof(float), hipMemcpyDeviceToDevice, st));
+                    if (sl) pool.push_back(sl);
+                    sl = nb;
+                }
+            } else if (a == Revolve::RESTORE && zc) {
+                if (!slot[rev.check]) return fail(RSG_EINVAL, "fwi: Revolve restored an empty checkpoint");
+                if (!cur_ro) pool.push_back(cur);
+                cur = slot[rev.check];
+                cur_ro = true;
             } else if (a == Revolve::TAKESHOT) {
                 if (f16) {
                     if (int rc = ck_pack(rev.check)) return rc;
@@ -580,7 +701,7 @@ extern "C" int rsg_crosscorr_a2d(const float *wsp, int pads, const float *wrx, c
     if (domdec) pads = padr = 0;
     if (nx < 3 || nz < 3) return RSG_OK;  // no interior cell
     const dim3 xb(64, 4), xg((nx + 63) / 64, (nz + 3) / 4);
-    rsg::crosscorr_a2d_kernel<false><<<xg, xb, 0, (hipStream_t)stream>>>(
+    rsg::crosscorr_a2d_kernel<rsg::kStoreF32><<<xg, xb, 0, (hipStream_t)stream>>>(
         wsp, nullptr, wrx, wrz, Rx, Rz, Vp, Rho, nx, nz, padr, dx, dz, vpgrad, rhograd, srcilum, 1, 0, pads, 0);
     RSG_LAUNCH_CHECK("crosscorr_a2d_kernel");
     return RSG_OK;

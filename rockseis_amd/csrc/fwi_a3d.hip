@@ -108,7 +108,10 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
     segs.push_back(o);
     const size_t total = o;
     const int nseg = (int)segs.size() - 1;
-    const bool f16 = f->fp16 != 0;
+    const int store = f->fp16;
+    if (store != kStoreF32 && store != kStoreF16 && store != kStoreS16)
+        return fail(RSG_EINVAL, "fwi_a3d: checkpoint storage must be 0 (FP32), 1 (FP16) or 2 (S16)");
+    const bool f16 = store != kStoreF32;  // 2 bytes a value
     Revolve rev(nt, f->ncheck);
     {
         size_t free_b = 0, total_b = 0;
@@ -152,9 +155,22 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
     unsigned *ckpart = nullptr;  // per-chunk maxima of the FP16 packing
     // FP16 checkpoint slots start at multiples of 4 values (8-byte aligned)
     const size_t cstride = (total + 3) & ~(size_t)3;
-    const std::vector<Chunk16> chunks = f16 ? store16_chunks(segs) : std::vector<Chunk16>();
+    const std::vector<Chunk16> chunks = store == kStoreF16 ? store16_chunks(segs) : std::vector<Chunk16>();
     const int nchunk = (int)chunks.size();
-    if (f16) {
+    // S16: blocks restart at every field and PML face (segs); a checkpoint is
+    // nblk blocks of kS16Block values
+    short *cks = nullptr, *ckexp = nullptr;
+    size_t *dsblk = nullptr;
+    const std::vector<size_t> sblkoff = store == kStoreS16 ? s16_block_offsets(segs) : std::vector<size_t>();
+    const size_t nblk = store == kStoreS16 ? sblkoff.back() : 0;
+    if (store == kStoreS16) {
+        if (int rc = dm.alloc(&cks, nblk * kS16Block * (size_t)rev.snaps)) return rc;
+        if (int rc = dm.alloc(&ckexp, nblk * rev.snaps)) return rc;
+        if (int rc = dm.alloc(&dsegs, segs.size())) return rc;
+        if (int rc = dm.alloc(&dsblk, sblkoff.size())) return rc;
+        RSG_HIP(hipMemcpyAsync(dsegs, segs.data(), sizeof(size_t) * segs.size(), hipMemcpyHostToDevice, st));
+        RSG_HIP(hipMemcpyAsync(dsblk, sblkoff.data(), sizeof(size_t) * sblkoff.size(), hipMemcpyHostToDevice, st));
+    } else if (f16) {
         if (int rc = dm.alloc(&ck16, cstride * (size_t)rev.snaps)) return rc;
         if (int rc = dm.alloc(&ckamax, (size_t)nseg * rev.snaps)) return rc;
         if (int rc = dm.alloc(&dsegs, segs.size())) return rc;
@@ -216,7 +232,11 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
             if (int rc = rsg_a3d_forward_stress(g, &sb, st)) return rc;
             if (int rc = reverse_tail(capo)) return rc;
         } else if (a == Revolve::TAKESHOT) {
-            if (f16) {
+            if (store == kStoreS16) {
+                if (int rc = pack_s16(fwb, cks + nblk * kS16Block * rev.check, ckexp + nblk * rev.check, dsegs, dsblk,
+                                      nseg, nblk, st))
+                    return rc;
+            } else if (f16) {
                 if (int rc = pack16_table(fwb, ck16 + cstride * rev.check, ckamax + (size_t)nseg * rev.check, ckpart,
                                           dsegs, dchunks, nchunk, nseg, total, st))
                     return rc;
@@ -224,7 +244,11 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
                 RSG_HIP(hipMemcpyAsync(ck + total * rev.check, fwb, total * sizeof(float), hipMemcpyDeviceToDevice, st));
             }
         } else if (a == Revolve::RESTORE) {
-            if (f16) {
+            if (store == kStoreS16) {
+                if (int rc = unpack_s16(cks + nblk * kS16Block * rev.check, ckexp + nblk * rev.check, fwb, dsegs, dsblk,
+                                        nseg, nblk, st))
+                    return rc;
+            } else if (f16) {
                 if (int rc = unpack16_table(ck16 + cstride * rev.check, ckamax + (size_t)nseg * rev.check, fwb, dsegs,
                                             dchunks, nchunk, total, st))
                     return rc;

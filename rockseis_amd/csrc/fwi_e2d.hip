@@ -262,7 +262,12 @@ struct E2DFwi {
     std::vector<size_t> off, segs, img_off;
     size_t total = 0, cstride = 0;
     int nseg = 0, nchunk = 0;
-    bool f16 = false;
+    bool f16 = false;  // FP16 or S16 checkpoints (2 bytes a value)
+    int store = kStoreF32;
+    // S16: blocks restart at every field and strip of every shot (segs); a
+    // checkpoint is sblk blocks of kS16Block values and sblk exponents
+    short *cks = nullptr, *ckexp = nullptr;
+    size_t *dsblk = nullptr, sblk = 0;
     std::unique_ptr<Revolve> rev;
     bool reverse = false;
     DevMem dm;
@@ -332,8 +337,12 @@ struct E2DFwi {
         total = off[ns];
         nseg = (int)segs.size() - 1;
         rev.reset(new Revolve(nt, f->ncheck));
-        f16 = f->fp16 != 0;
-        if (f16 && nseg > 65535) return fail(RSG_EINVAL, "fwi_e2d: FP16 checkpoints take at most 3120 shots");
+        store = f->fp16;
+        if (store != kStoreF32 && store != kStoreF16 && store != kStoreS16)
+            return fail(RSG_EINVAL, "fwi_e2d: checkpoint storage must be 0 (FP32), 1 (FP16) or 2 (S16)");
+        f16 = store != kStoreF32;
+        if (store == kStoreF16 && nseg > 65535)
+            return fail(RSG_EINVAL, "fwi_e2d: FP16 checkpoints take at most 3120 shots");
         {
             size_t free_b = 0, total_b = 0;
             RSG_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -509,9 +518,18 @@ struct E2DFwi {
         RSG_HIP(hipMemcpyAsync(dsd, hd.data(), sizeof(ShotDev) * ns, hipMemcpyHostToDevice, st));
         // FP16 checkpoint slots start at multiples of 4 values (8-byte aligned)
         cstride = (total + 3) & ~(size_t)3;
-        const std::vector<Chunk16> chunks = f16 ? store16_chunks(segs) : std::vector<Chunk16>();
+        const std::vector<Chunk16> chunks = store == kStoreF16 ? store16_chunks(segs) : std::vector<Chunk16>();
         nchunk = (int)chunks.size();
-        if (f16) {
+        const std::vector<size_t> sblkoff = store == kStoreS16 ? s16_block_offsets(segs) : std::vector<size_t>();
+        if (store == kStoreS16) {
+            sblk = sblkoff.back();
+            if (int rc = dm.alloc(&cks, sblk * kS16Block * (size_t)rev->snaps)) return rc;
+            if (int rc = dm.alloc(&ckexp, sblk * rev->snaps)) return rc;
+            if (int rc = dm.alloc(&dsegs, segs.size())) return rc;
+            if (int rc = dm.alloc(&dsblk, sblkoff.size())) return rc;
+            RSG_HIP(hipMemcpyAsync(dsegs, segs.data(), sizeof(size_t) * segs.size(), hipMemcpyHostToDevice, st));
+            RSG_HIP(hipMemcpyAsync(dsblk, sblkoff.data(), sizeof(size_t) * sblkoff.size(), hipMemcpyHostToDevice, st));
+        } else if (f16) {
             if (int rc = dm.alloc(&dsegs, segs.size())) return rc;
             if (int rc = dm.alloc(&dchunks, chunks.size())) return rc;
             if (int rc = dm.alloc(&ckpart, chunks.size())) return rc;
@@ -595,6 +613,9 @@ struct E2DFwi {
     }
 
     int takeshot(hipStream_t st) {
+        if (store == kStoreS16)
+            return pack_s16(fwb, cks + sblk * kS16Block * rev->check, ckexp + sblk * rev->check, dsegs, dsblk, nseg,
+                            sblk, st);
         if (f16)
             return pack16_table(fwb, ck16 + cstride * rev->check, ckamax + (size_t)nseg * rev->check, ckpart, dsegs,
                                 dchunks, nchunk, nseg, total, st);
@@ -626,7 +647,11 @@ struct E2DFwi {
             } else if (a == Revolve::TAKESHOT) {
                 if (int rc = takeshot(st)) return rc;
             } else if (a == Revolve::RESTORE) {
-                if (f16) {
+                if (store == kStoreS16) {
+                    if (int rc = unpack_s16(cks + sblk * kS16Block * rev->check, ckexp + sblk * rev->check, fwb,
+                                            dsegs, dsblk, nseg, sblk, st))
+                        return rc;
+                } else if (f16) {
                     if (int rc = unpack16_table(ck16 + cstride * rev->check, ckamax + (size_t)nseg * rev->check, fwb,
                                                 dsegs, dchunks, nchunk, total, st))
                         return rc;

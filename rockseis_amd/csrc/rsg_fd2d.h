@@ -86,6 +86,13 @@ struct K2D {
     float *P, *Vx, *Vz, *Sxx, *Szz, *Sxz;
     const float *L, *L2M, *M, *Rx, *Rz;
     float *psi[RSG_MAX_STRIPS];
+    // Acoustic kernels read the updated arrays' previous values from these
+    // (the fields and strips above, unless a step runs out of place: from a
+    // checkpoint buffer into the working state).  The velocity step reads
+    // P (stencil), Vx, Vz and the P strips here; the stress step reads P and
+    // the Vxx / Vzz strips here and the new Vx, Vz from the outputs.
+    const float *iP, *iVx, *iVz;
+    const float *ipsi[RSG_A2D_NSTRIP];
     Coefs cx, cz;
     FusedIO io;  // stress kernels of single-shot modelling runs only
 };
@@ -298,6 +305,21 @@ __device__ __forceinline__ void term(float &F, float dtM, float d, const PmlPos 
     }
 }
 
+// The same with the strip's previous value read from ilo / ihi (equal to
+// lo / hi in place).
+template <bool FAST>
+__device__ __forceinline__ void term(float &F, float dtM, float d, const PmlPos &pp, const Tab &tb,
+                                     int stag, float *lo, float *hi, const float *ilo, const float *ihi,
+                                     size_t sidx) {
+    F += dtM * d;
+    if (!FAST && pp.side >= 0) {
+        const float A = tb.A(pp.side, stag, pp.i), B = tb.B(pp.side, stag, pp.i), C = tb.C(pp.side, stag, pp.i);
+        const float p = B * (pp.side ? ihi : ilo)[sidx] + A * d;
+        (pp.side ? hi : lo)[sidx] = p;
+        F -= dtM * (p + C * d);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Acoustic 2D
 // ---------------------------------------------------------------------------
@@ -316,16 +338,17 @@ __device__ __forceinline__ void a2d_velocity_body(const K2D &a, int x0, int z0, 
     const unsigned cb = (unsigned)(((z0 + ty * RZ) * nx + x) * 4);
     const Buf bVx = mkbuf(a.Vx + fo, nb), bVz = mkbuf(a.Vz + fo, nb), bRx = mkbuf(a.Rx, nb),
               bRz = mkbuf(a.Rz, nb);
+    const Buf iVx = mkbuf(a.iVx + fo, nb), iVz = mkbuf(a.iVz + fo, nb);
     float vx[RZ], vz[RZ], rx[RZ], rz[RZ];
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
-        vx[r] = cell_ld<FAST>(bVx, cb, r, rowb);
-        vz[r] = cell_ld<FAST>(bVz, cb, r, rowb);
+        vx[r] = cell_ld<FAST>(iVx, cb, r, rowb);
+        vz[r] = cell_ld<FAST>(iVz, cb, r, rowb);
         rx[r] = cell_ld<FAST>(bRx, cb, r, rowb);
         rz[r] = cell_ld<FAST>(bRz, cb, r, rowb);
     }
     Tile<TZ, H, H> tp;
-    tile_fetch<TZ, H, H, FAST>(tp, Ld1{mkbuf(a.P + fo, nb)}, nx, nz, x0, z0);
+    tile_fetch<TZ, H, H, FAST>(tp, Ld1{mkbuf(a.iP + fo, nb)}, nx, nz, x0, z0);
     tile_store(tp, sP);
     __syncthreads();
     if (!xin) return;
@@ -341,9 +364,11 @@ __device__ __forceinline__ void a2d_velocity_body(const K2D &a, int x0, int z0, 
         const float dPz = ddfw<ORDER, FAST>(s, LD, a.cz.c, FAST ? ORDER : kfw(z, nz, ORDER), a.invdz);
         const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
         term<FAST>(vx[r], a.dt * rx[r], dPx, px, tb, 1, (a.psi[RSG_A2D_P_LEFT] + xo),
-                   (a.psi[RSG_A2D_P_RIGHT] + xo), (size_t)z * a.lpml + px.i);
+                   (a.psi[RSG_A2D_P_RIGHT] + xo), (a.ipsi[RSG_A2D_P_LEFT] + xo), (a.ipsi[RSG_A2D_P_RIGHT] + xo),
+                   (size_t)z * a.lpml + px.i);
         term<FAST>(vz[r], a.dt * rz[r], dPz, pz, tb, 1, (a.psi[RSG_A2D_P_TOP] + zo),
-                   (a.psi[RSG_A2D_P_BOTTOM] + zo), (size_t)pz.i * nx + x);
+                   (a.psi[RSG_A2D_P_BOTTOM] + zo), (a.ipsi[RSG_A2D_P_TOP] + zo), (a.ipsi[RSG_A2D_P_BOTTOM] + zo),
+                   (size_t)pz.i * nx + x);
         cell_st<FAST>(bVx, vx[r], cb, r, rowb);
         cell_st<FAST>(bVz, vz[r], cb, r, rowb);
     }
@@ -371,11 +396,11 @@ __device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, fl
     const unsigned nb = (unsigned)(nx * nz * 4);
     const int rowb = nx * 4;
     const unsigned cb = (unsigned)(((z0 + ty * RZ) * nx + x) * 4);
-    const Buf bP = mkbuf(a.P + fo, nb), bL = mkbuf(a.L, nb);
+    const Buf bP = mkbuf(a.P + fo, nb), bL = mkbuf(a.L, nb), iP = mkbuf(a.iP + fo, nb);
     float pv[RZ], lv[RZ];
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
-        pv[r] = cell_ld<FAST>(bP, cb, r, rowb);
+        pv[r] = cell_ld<FAST>(iP, cb, r, rowb);
         lv[r] = cell_ld<FAST>(bL, cb, r, rowb);
     }
     Tile<TZ, H, 0> tvx;
@@ -400,9 +425,9 @@ __device__ __forceinline__ void a2d_stress_body(const K2D &a, int x0, int z0, fl
         const PmlPos pz = FAST ? PmlPos{-1, 0} : pml_pos(z, nz, a.lpml);
         const float dtL = a.dt * lv[r];
         term<FAST>(pv[r], dtL, dVx, px, tb, 0, (a.psi[RSG_A2D_VXX_LEFT] + xo), (a.psi[RSG_A2D_VXX_RIGHT] + xo),
-                   (size_t)z * a.lpml + px.i);
+                   (a.ipsi[RSG_A2D_VXX_LEFT] + xo), (a.ipsi[RSG_A2D_VXX_RIGHT] + xo), (size_t)z * a.lpml + px.i);
         term<FAST>(pv[r], dtL, dVz, pz, tb, 0, (a.psi[RSG_A2D_VZZ_TOP] + zo), (a.psi[RSG_A2D_VZZ_BOTTOM] + zo),
-                   (size_t)pz.i * nx + x);
+                   (a.ipsi[RSG_A2D_VZZ_TOP] + zo), (a.ipsi[RSG_A2D_VZZ_BOTTOM] + zo), (size_t)pz.i * nx + x);
         cell_st<FAST>(bP, pv[r], cb, r, rowb);
     }
     if (IO && a.io.nsrc) fused_inject<RZ, FAST>(a.io, bP, nullptr, &bL, x, z0 + ty * RZ, nz, cb, rowb, it);
@@ -762,6 +787,10 @@ static K2D make_k2d(const rsg_grid *g, const rsg_state *s, int tz) {
     a.Rx = s->Rx;
     a.Rz = s->Rz;
     for (int i = 0; i < RSG_MAX_STRIPS; i++) a.psi[i] = s->psi[i];
+    a.iP = s->P;
+    a.iVx = s->Vx;
+    a.iVz = s->Vz;
+    for (int i = 0; i < RSG_A2D_NSTRIP; i++) a.ipsi[i] = s->psi[i];
     a.cx = scaled_coefs(g->dx);
     a.cz = scaled_coefs(g->dz);
     a.io = FusedIO{};

@@ -22,6 +22,11 @@ struct Revolve {
     static int numforw(int steps, int snaps);
 };
 
+// One forward acoustic 2D step of a batch out of place: the previous state
+// is read from `in` (left unchanged), the new one written to `out` (fd2d.hip).
+int a2d_forward_step_from(const rsg_grid *g, const rsg_state *in, const rsg_state *out, const rsg_batch *b,
+                          hipStream_t stream);
+
 // Misfit (added to *misfit in double) and residuals of nshot shots of ntr
 // traces (shot k's traces follow shot k-1's) for type 0 DIFFERENCE,
 // 1 CORRELATION, 2 ADAPTIVE_GAUSS, 3 ADAPTIVE_LINEAR (4 ADAPTIVE_SINGLE is

@@ -59,10 +59,47 @@ int pack16_table(const float *src, half_t *dst, unsigned *amax, unsigned *part, 
 int unpack16_table(const half_t *src, const unsigned *amax, float *dst, const size_t *off, const Chunk16 *cs,
                    int nchunk, size_t total, hipStream_t st);
 
+// 16-bit block floating point (S16) storage: blocks of kS16Block
+// consecutive values share one power-of-two exponent E, the block's largest
+// magnitude being m * 2^E with m in [0.5, 1); each value v is kept as the
+// signed 16-bit integer rint(v * 2^(15-E)) (|q| <= 32767), so its error is at
+// most 2^(E-16), about 1.5e-5 of the block maximum.  A smooth wavefield varies
+// little within 256 cells, so most values keep 14-15 significant bits where
+// FP16 keeps 11.  An all-zero block stores exponent kS16Zero.
+//
+// Blocks never straddle two stored arrays: a checkpoint holds fields and PML
+// memory variables whose magnitudes differ by many orders, and a block shared
+// by the end of one and the start of the next would round the smaller one
+// away.  The source is a flat buffer of nseg segments (segment s =
+// src[off[s] .. off[s+1])); segment s owns blocks blk[s] .. blk[s+1]-1
+// (s16_block_offsets), and block b's 256 values sit at q[256 b ..] (the
+// last block of a segment padded), its exponent at e[b].
+constexpr int kS16Block = 256;
+constexpr short kS16Zero = -32768;
+inline size_t s16_blocks(size_t n) { return (n + kS16Block - 1) / kS16Block; }
+std::vector<size_t> s16_block_offsets(const std::vector<size_t> &off);
+// off, blk: device copies of the two offset tables (nseg + 1 entries each);
+// q holds blk[nseg] * kS16Block values, e blk[nseg] exponents.
+int pack_s16(const float *src, short *q, short *e, const size_t *off, const size_t *blk, int nseg, size_t nblk,
+             hipStream_t st);
+int unpack_s16(const short *q, const short *e, float *dst, const size_t *off, const size_t *blk, int nseg,
+               size_t nblk, hipStream_t st);
+
+// Storage kinds of checkpoints (the fp16 member of the FWI settings):
+// 0 FP32, 1 FP16 with one power-of-two scale per array, 2 S16.
+enum { kStoreF32 = 0, kStoreF16 = 1, kStoreS16 = 2 };
+
 // Unpadded nx x nz window of every shot's padded (lpml) field, shots sstride
-// apart -> dst [nshot][nz][nx] in FP32 (amax == nullptr) or FP16 with one
-// scale per shot in amax[nshot].
-int pack_interior(const float *src, void *dst, unsigned *amax, int nx, int nz, int lpml, size_t sstride,
-                  int nshot, bool fp16, hipStream_t st);
+// apart -> dst [nshot][nz][nx] as `store`: FP32; FP16 with one scale per
+// shot in aux = unsigned amax[nshot]; or S16 over the flat snapshot with
+// aux = short exponents[s16_blocks(nshot*nz*nx)].
+int pack_interior(const float *src, void *dst, void *aux, int nx, int nz, int lpml, size_t sstride,
+                  int nshot, int store, hipStream_t st);
+
+// One value of a single-segment S16 buffer (q, exponents e) at flat index i.
+__device__ inline float s16_value(const short *q, const short *e, size_t i) {
+    const short x = e[i / kS16Block];
+    return x == kS16Zero ? 0.f : ldexpf((float)q[i], x - 15);
+}
 
 }  // namespace rsg

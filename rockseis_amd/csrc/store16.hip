@@ -228,6 +228,53 @@ __global__ __launch_bounds__(kThreads) void pack_interior_kernel(const float *__
     }
 }
 
+// S16 blocks: one wave per block of kS16Block values, four values a lane.
+constexpr int kS16PerWg = kThreads / 64;  // blocks per workgroup
+
+__device__ inline float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// S16 interior window: one wave per block of kS16Block values of the flat
+// [nshot][nz][nx] snapshot; lane l takes values l, l+64, l+128, l+192 of
+// the block, so each store instruction writes 64 consecutive values.
+__global__ __launch_bounds__(kThreads) void pack_interior_s16_kernel(const float *__restrict__ src,
+                                                                      short *__restrict__ q, short *__restrict__ ex,
+                                                                      int nx, int nz, int lpml, size_t sstride,
+                                                                      size_t n, size_t nblk) {
+    const size_t b = (size_t)blockIdx.x * kS16PerWg + (threadIdx.x >> 6);
+    if (b >= nblk) return;
+    const int lane = threadIdx.x & 63;
+    const int nxp = nx + 2 * lpml;
+    const size_t nint = (size_t)nx * nz;
+    float v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const size_t i = b * kS16Block + lane + 64 * j;
+        v[j] = 0.f;
+        if (i < n) {
+            const size_t k = i / nint, r = i - k * nint;
+            const size_t iz = r / nx, ix = r - iz * nx;
+            v[j] = src[k * sstride + (iz + lpml) * nxp + ix + lpml];
+        }
+    }
+    float m = fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3])));
+    m = wave_max(m);
+    int e = 0;
+    const bool zero = !(m > 0.f);
+    if (!zero) (void)frexpf(m, &e);
+    if (lane == 0) ex[b] = zero ? kS16Zero : (short)e;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const size_t i = b * kS16Block + lane + 64 * j;
+        if (i < n) {
+            const float t = zero ? 0.f : rintf(ldexpf(v[j], 15 - e));
+            q[i] = (short)fminf(fmaxf(t, -32767.f), 32767.f);
+        }
+    }
+}
+
 int zero_u32(unsigned *p, int n, hipStream_t st) {
     fill_u32_kernel<<<(n + kThreads - 1) / kThreads, kThreads, 0, st>>>(p, n);
     RSG_LAUNCH_CHECK("fill_u32_kernel");
@@ -241,7 +288,93 @@ dim3 seg_grid(size_t seglen, int nseg) {
     return dim3((unsigned)bx, (unsigned)nseg);
 }
 
+// Block b of a segmented S16 buffer: its segment (binary search of the
+// block offsets), first source value and length.
+__device__ inline void s16_block(const size_t *__restrict__ off, const size_t *__restrict__ blk, int nseg, size_t b,
+                                 size_t &first, int &len) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blk[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    first = off[lo] + (b - blk[lo]) * kS16Block;
+    const size_t rest = off[lo + 1] - first;
+    len = rest < (size_t)kS16Block ? (int)rest : kS16Block;
+}
+
+// One wave per block; lane l takes values l, l+64, l+128, l+192 of the block,
+// so each load and store instruction covers 64 consecutive values.
+__global__ __launch_bounds__(kThreads) void pack_s16_kernel(const float *__restrict__ src, short *__restrict__ q,
+                                                             short *__restrict__ ex, const size_t *__restrict__ off,
+                                                             const size_t *__restrict__ blk, int nseg, size_t nblk) {
+    const size_t b = (size_t)blockIdx.x * kS16PerWg + (threadIdx.x >> 6);
+    if (b >= nblk) return;  // whole waves leave together
+    const int lane = threadIdx.x & 63;
+    size_t first;
+    int len;
+    s16_block(off, blk, nseg, b, first, len);
+    float v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = lane + 64 * j < len ? src[first + lane + 64 * j] : 0.f;
+    float m = fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3])));
+    m = wave_max(m);
+    int e = 0;
+    const bool zero = !(m > 0.f);
+    if (!zero) (void)frexpf(m, &e);
+    if (lane == 0) ex[b] = zero ? kS16Zero : (short)e;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const float t = zero ? 0.f : rintf(ldexpf(v[j], 15 - e));
+        q[b * kS16Block + lane + 64 * j] = (short)fminf(fmaxf(t, -32767.f), 32767.f);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void unpack_s16_kernel(const short *__restrict__ q,
+                                                               const short *__restrict__ ex, float *__restrict__ dst,
+                                                               const size_t *__restrict__ off,
+                                                               const size_t *__restrict__ blk, int nseg, size_t nblk) {
+    const size_t b = (size_t)blockIdx.x * kS16PerWg + (threadIdx.x >> 6);
+    if (b >= nblk) return;
+    const int lane = threadIdx.x & 63;
+    size_t first;
+    int len;
+    s16_block(off, blk, nseg, b, first, len);
+    const short e = ex[b];
+    const int sh = e == kS16Zero ? 0 : e - 15;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (lane + 64 * j < len)
+            dst[first + lane + 64 * j] = e == kS16Zero ? 0.f : ldexpf((float)q[b * kS16Block + lane + 64 * j], sh);
+}
+
 }  // namespace
+
+std::vector<size_t> s16_block_offsets(const std::vector<size_t> &off) {
+    std::vector<size_t> blk(off.size(), 0);
+    for (size_t s = 0; s + 1 < off.size(); s++) blk[s + 1] = blk[s] + s16_blocks(off[s + 1] - off[s]);
+    return blk;
+}
+
+int pack_s16(const float *src, short *q, short *e, const size_t *off, const size_t *blk, int nseg, size_t nblk,
+             hipStream_t st) {
+    if (nblk == 0) return RSG_OK;
+    if (nseg < 1) return fail(RSG_EINVAL, "pack_s16: empty segment table");
+    pack_s16_kernel<<<(unsigned)((nblk + kS16PerWg - 1) / kS16PerWg), kThreads, 0, st>>>(src, q, e, off, blk, nseg,
+                                                                                          nblk);
+    RSG_LAUNCH_CHECK("pack_s16_kernel");
+    return RSG_OK;
+}
+
+int unpack_s16(const short *q, const short *e, float *dst, const size_t *off, const size_t *blk, int nseg,
+               size_t nblk, hipStream_t st) {
+    if (nblk == 0) return RSG_OK;
+    if (nseg < 1) return fail(RSG_EINVAL, "unpack_s16: empty segment table");
+    unpack_s16_kernel<<<(unsigned)((nblk + kS16PerWg - 1) / kS16PerWg), kThreads, 0, st>>>(q, e, dst, off, blk,
+                                                                                            nseg, nblk);
+    RSG_LAUNCH_CHECK("unpack_s16_kernel");
+    return RSG_OK;
+}
 
 int pack16_segments(const float *src, half_t *dst, unsigned *amax, size_t seglen, int nseg, hipStream_t st) {
     if (nseg < 1 || nseg > 65535) return fail(RSG_EINVAL, "pack16: 1..65535 segments");
@@ -302,11 +435,17 @@ int unpack16_table(const half_t *src, const unsigned *amax, float *dst, const si
     return RSG_OK;
 }
 
-int pack_interior(const float *src, void *dst, unsigned *amax, int nx, int nz, int lpml, size_t sstride,
-                  int nshot, bool fp16, hipStream_t st) {
+int pack_interior(const float *src, void *dst, void *aux, int nx, int nz, int lpml, size_t sstride,
+                  int nshot, int store, hipStream_t st) {
     if (nshot < 1 || nshot > 65535) return fail(RSG_EINVAL, "pack_interior: 1..65535 shots");
     const dim3 g((nx + kThreads - 1) / kThreads, (nz + kRows - 1) / kRows, nshot);
-    if (fp16) {
+    unsigned *amax = (unsigned *)aux;
+    if (store == kStoreS16) {
+        const size_t n = (size_t)nx * nz * nshot, nblk = s16_blocks(n);
+        pack_interior_s16_kernel<<<(unsigned)((nblk + kS16PerWg - 1) / kS16PerWg), kThreads, 0, st>>>(
+            src, (short *)dst, (short *)aux, nx, nz, lpml, sstride, n, nblk);
+        RSG_LAUNCH_CHECK("pack_interior_s16_kernel");
+    } else if (store == kStoreF16) {
         if (int rc = zero_u32(amax, nshot, st)) return rc;
         amax_interior_kernel<<<g, kThreads, 0, st>>>(src, amax, nx, nz, lpml, sstride);
         RSG_LAUNCH_CHECK("amax_interior_kernel");

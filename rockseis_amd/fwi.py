@@ -45,6 +45,24 @@ def _adjoint_kind(kind):
     return _ADJOINTS[kind]
 
 
+
+# Storage of snapshots and Revolve checkpoints (RSG_STORE_* of rsgpu.h):
+# False / 0 / "fp32", True / 1 / "fp16" (one power-of-two scale per array),
+# 2 / "s16" (16-bit block floating point, 256 values per exponent).
+STORE_FP32, STORE_FP16, STORE_S16 = 0, 1, 2
+
+
+def storage_kind(v):
+    if isinstance(v, str):
+        kinds = {"fp32": STORE_FP32, "fp16": STORE_FP16, "s16": STORE_S16}
+        if v.lower() not in kinds:
+            raise R.RsgError(f"storage kind {v!r}: use 'fp32', 'fp16' or 's16'")
+        return kinds[v.lower()]
+    k = int(v)
+    if k not in (STORE_FP32, STORE_FP16, STORE_S16):
+        raise R.RsgError(f"storage kind {v!r}: use 0 (FP32), 1 (FP16) or 2 (S16)")
+    return k
+
 class Image2D:
     """Image on the unpadded model grid: [nz, nx] float32, or
     [nhz, nhx, nz, nx] with subsurface offsets (Image2D's ki2D order)."""
@@ -129,7 +147,7 @@ class FwiAcoustic2D:
         self.dataweight = d
 
     def setFp16Storage(self, v):
-        self.fp16 = bool(v)
+        self.fp16 = storage_kind(v)
 
     def setAdjoint(self, kind):
         """"reference" (the reference's gradient) or "exact" (the exact
@@ -304,7 +322,7 @@ class Survey:
         self.shots = list(shots)
         _adjoint_kind(adjoint)
         self.adjoint = adjoint
-        self.snapinc, self.fp16 = int(snapinc), bool(fp16)
+        self.snapinc, self.fp16 = int(snapinc), storage_kind(fp16)
         self.batched = bool(batched)
         self.lpml, self.dx, self.dz, self.dt = int(lpml), float(dx), float(dz), float(dt)
         self.order, self.fs, self.ncheck = int(order), bool(fs), int(ncheck)
@@ -506,7 +524,7 @@ class ElasticSurvey:
         self.misfit_type = int(misfit_type)
         self.shots = list(shots)
         self.lpml, self.dx, self.dz, self.dt = int(lpml), float(dx), float(dz), float(dt)
-        self.order, self.fs, self.ncheck, self.fp16 = int(order), bool(fs), int(ncheck), bool(fp16)
+        self.order, self.fs, self.ncheck, self.fp16 = int(order), bool(fs), int(ncheck), storage_kind(fp16)
         self.ox, self.oz = float(ox), float(oz)  # origin of the full model
 
 
@@ -756,7 +774,7 @@ class FwiElastic2D:
         self.incore = bool(v)
 
     def setFp16Storage(self, v):
-        self.fp16 = bool(v)
+        self.fp16 = storage_kind(v)
 
     def setMisfit_type(self, t):
         if int(t) not in _ELASTIC_MISFITS:
@@ -863,7 +881,7 @@ class FwiAcoustic3D:
         self.incore = bool(v)
 
     def setFp16Storage(self, v):
-        self.fp16 = bool(v)
+        self.fp16 = storage_kind(v)
 
     def setMisfit_type(self, t):
         if int(t) not in _ACOUSTIC_MISFITS:

@@ -150,17 +150,27 @@ def test_exact_batch_and_autograd():
 
 
 def test_exact_fp16_checkpoints():
-    """FP16 Revolve checkpoints (power-of-two scale per array): the measured
-    gradient error vs the FP64 oracle is printed and bounded at 2e-3."""
+    """16-bit Revolve checkpoints, gradient error vs the FP64 oracle.  This
+    case's residual is a small difference of FP32 records, so FP32
+    checkpoints already leave 1.5e-4 (measured on MI355X): FP16 (power-of-two
+    scale per array, measured 3.4e-4 / 3.9e-4) is held to 6e-4, S16 (16-bit
+    block floating point, measured 1.6e-4 / 1.7e-4) to the larger of 1e-4 and
+    1.25x the FP32-checkpoint error."""
     from oracle.adjoint import fwi_a2d_exact
     c = exact_case(seed=41, order=8)
     dobs = observed(c)
     ref = fwi_a2d_exact(*oracle_args(c, dobs))
-    r = _gpu(c, dobs, ncheck=5, fp16=True)
-    for k in ("vpgrad", "rhograd", "wavgrad"):
-        e = rel_l2(r[k], ref[k])
-        print(f"  fp16 {k}: rel L2 {e:.2e}")
-        assert e <= 2e-3, k
+    keys = ("vpgrad", "rhograd", "wavgrad")
+    r32 = _gpu(c, dobs, ncheck=5, fp16=0)
+    e32 = {k: rel_l2(r32[k], ref[k]) for k in keys}
+    for kind in (1, 2):
+        r = _gpu(c, dobs, ncheck=5, fp16=kind)
+        for k in keys:
+            bound = 6e-4 if kind == 1 else max(1e-4, 1.25 * e32[k])
+            e = rel_l2(r[k], ref[k])
+            print(f"  {('fp32', 'fp16', 's16')[kind]} {k}: rel L2 {e:.2e} (FP32 checkpoints {e32[k]:.2e}, "
+                  f"bound {bound:.1e})")
+            assert e <= bound, (kind, k)
 
 
 def test_exact_rejects_other_modes():

@@ -85,14 +85,16 @@ def test_fwi_a2d_full_vs_reference(golden, case):
 
 
 def test_fp16_storage_error_vs_fp64_oracle(golden):
-    """Gradient error against the FP64 oracle for FP32 and FP16 storage of
+    """Gradient error against the FP64 oracle for FP32, FP16 and S16 storage of
     Revolve checkpoints, the store-every-step history and run()'s snapshots
     (order 4, 60x50, nt 300).  FP16 keeps 11 significant bits, so every
     stored value carries a relative rounding error up to 2^-11; the errors
     average out over the correlated time steps.  Measured on MI355X
     (vpgrad / rhograd): FP32 2.7e-6 / 2.6e-6 in every mode; FP16 Revolve
     7.4e-5 / 8.2e-5, FP16 store-every-step 3.9e-5 / 5.5e-5, FP16 run() with
-    snapinc 3 8.1e-5 / 1.07e-4 (a third as many snapshots to average over)."""
+    snapinc 3 8.1e-5 / 1.07e-4 (a third as many snapshots to average over).
+    S16 (int16 mantissas, one power-of-two exponent per 256 values) keeps
+    about four more bits of a smooth wavefield."""
     from oracle import fwi as ofwi
     g = golden("fwi_case0")
     order, fs, ncheck = FWI_CASES[0]
@@ -101,8 +103,11 @@ def test_fp16_storage_error_vs_fp64_oracle(golden):
     tail = (g["scoords"], g["wav"], g["gcoords"], g["dobs"])
     o_opt = ofwi.fwi_a2d(*args, ncheck, *tail, real=np.float64)
     o_full = ofwi.fwi_a2d_full(*args, 3, *tail, real=np.float64)
-    bounds = {("revolve", False): 1e-5, ("store", False): 1e-5, ("full", False): 1e-5,
-              ("revolve", True): 2e-4, ("store", True): 1e-4, ("full", True): 2e-4}
+    # FP16 bounds: 1.5x the measured figures; S16 (16-bit block floating
+    # point): the 1e-4 target
+    bounds = {("revolve", 0): 1e-5, ("store", 0): 1e-5, ("full", 0): 1e-5,
+              ("revolve", 1): 1.3e-4, ("store", 1): 1e-4, ("full", 1): 1.6e-4,
+              ("revolve", 2): 1e-4, ("store", 2): 1e-4, ("full", 2): 1e-4}
     for (mode, fp16), bound in bounds.items():
         if mode == "full":
             r = _run_full_on_case0(g, fp16)
@@ -112,7 +117,8 @@ def test_fp16_storage_error_vs_fp64_oracle(golden):
             ref = o_opt
         ev = rel_l2(r["vpgrad"], ref["vpgrad"])
         er = rel_l2(r["rhograd"], ref["rhograd"])
-        print(f"  {mode:8s} fp16={fp16!s:5s} vpgrad {ev:.2e} rhograd {er:.2e} (bound {bound:.0e})", flush=True)
+        kind = ("fp32", "fp16", "s16")[fp16]
+        print(f"  {mode:8s} {kind:4s} vpgrad {ev:.2e} rhograd {er:.2e} (bound {bound:.1e})", flush=True)
         assert ev <= bound and er <= bound, (mode, fp16)
 
 

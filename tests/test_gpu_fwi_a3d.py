@@ -58,7 +58,7 @@ def test_fwi_a3d_vs_reference(golden, case):
 
 
 def test_fp16_checkpoints_error_vs_fp64_oracle(golden):
-    """Gradient error against the FP64 oracle with FP32 and FP16 Revolve
+    """Gradient error against the FP64 oracle with FP32, FP16 and S16 Revolve
     checkpoints (order 4, 26x24x22, nt 120, 10 snapshots)."""
     from oracle import fwi as ofwi
     g = golden("fwi_a3d_case0")
@@ -66,10 +66,11 @@ def test_fp16_checkpoints_error_vs_fp64_oracle(golden):
     nx, ny, nz, lpml, dx, dt, nt = g["grid"]
     o64 = ofwi.fwi_a3d(g["vp"], g["rho"], int(lpml), (dx, dx, dx), dt, int(nt), order, fs, ncheck, g["scoords"],
                        g["wav"], g["gcoords"], g["dobs"], real=np.float64)
-    for fp16, bound in ((False, 1e-5), (True, 1e-3)):
+    # FP16: 1.5x the measured 1.32e-4; S16 (16-bit block floating point): the 1e-4 target
+    for fp16, bound in ((0, 1e-5), (1, 2e-4), (2, 1e-4)):
         r = _driver(g, 0, fp16=fp16)
         ev, er = rel_l2(r["vpgrad"], o64["vpgrad"]), rel_l2(r["rhograd"], o64["rhograd"])
-        print(f"  fp16={fp16}: vpgrad {ev:.2e} rhograd {er:.2e} (bound {bound:.0e})", flush=True)
+        print(f"  {('fp32', 'fp16', 's16')[fp16]}: vpgrad {ev:.2e} rhograd {er:.2e} (bound {bound:.1e})", flush=True)
         assert max(ev, er) <= bound, fp16
 
 

@@ -122,18 +122,20 @@ def test_ragged_windows_equal_single_windows(golden):
 
 
 def test_fp16_checkpoints_error_vs_fp64_oracle(golden):
-    """Vp / Vs / rho gradient error against the FP64 oracle with FP32 and
-    FP16 Revolve checkpoints (order 8, free surface, Uz source)."""
+    """Vp / Vs / rho gradient error against the FP64 oracle with FP32, FP16
+    and S16 Revolve checkpoints (order 8, free surface, Uz source)."""
     from oracle import fwi as ofwi
     g = golden("fwi_e2d_case1")
     order, fs, ncheck, sf = FWI_E2D_CASES[1]
     nx, nz, lpml, dx, dt, nt = g["grid"]
     o64 = ofwi.fwi_e2d(g["vp"], g["vs"], g["rho"], int(lpml), dx, dx, dt, int(nt), order, fs, ncheck, sf,
                        g["scoords"], g["wav"], g["gcoords"], g["dobs"], real=np.float64)
-    for fp16, bound in ((False, 1e-5), (True, 1e-3)):
+    # FP16: 1.5x the measured 3.04e-4; S16 (16-bit block floating point): the 1e-4 target
+    for fp16, bound in ((0, 1e-5), (1, 4.6e-4), (2, 1e-4)):
         r = _driver(g, 1, fp16=fp16)
         errs = [rel_l2(r[k], o64[k]) for k in ("vpgrad", "vsgrad", "rhograd")]
-        print(f"  fp16={fp16}: vp {errs[0]:.2e} vs {errs[1]:.2e} rho {errs[2]:.2e} (bound {bound:.0e})", flush=True)
+        print(f"  {('fp32', 'fp16', 's16')[fp16]}: vp {errs[0]:.2e} vs {errs[1]:.2e} rho {errs[2]:.2e} "
+              f"(bound {bound:.1e})", flush=True)
         assert max(errs) <= bound, fp16
 
 
