@@ -353,6 +353,22 @@ int rsg_fwi_a2d_gradient(const rsg_grid *g, const rsg_fwi_a2d *f, void *stream);
  * has none. */
 int rsg_misfit_residuals(int type, const float *mod, const float *rec, const float *wei, float *res, const int *map,
                          int dim, int ntr, int nt, int nshot, int scaling, double *misfit, void *stream);
+/* No counterpart in the reference.  rsg_misfit_residuals for callers that
+ * cannot allocate device memory (code under hipGraph capture, serving loops,
+ * harnesses that forbid hipMalloc): same arguments and results, no
+ * hipMalloc / hipFree and no host allocation.  Its device scratch is one
+ * fixed array of RSG_NOALLOC_SCRATCH_BYTES per GPU, held from the library's
+ * load to the end of the process.  Limits: types 0 (DIFFERENCE),
+ * 1 (CORRELATION) and 4 (its DIFFERENCE branch) -- the ADAPTIVE types need
+ * 7*nt floats per trace and take rsg_misfit_residuals; at most 64 shots; the
+ * scratch must hold 3*ntr*nshot floats of CORRELATION norms plus about
+ * 1 KiB per shot (larger inputs return RSG_EINVAL).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams or
+ * threads) race and must be serialized by the caller. */
+#define RSG_NOALLOC_SCRATCH_BYTES (4u << 20)
+int rsg_misfit_residuals_noalloc(int type, const float *mod, const float *rec, const float *wei, float *res,
+                                 const int *map, int dim, int ntr, int nt, int nshot, int scaling, double *misfit,
+                                 void *stream);
 /* nshot shots on one grid in one set of launches per step (the reference
  * runs one shot per MPI worker, inversion.cpp:332-600, and stacks the shot
  * gradients, image.cpp:533-600): traces of shot k follow shot k-1 in

@@ -193,6 +193,8 @@ def _declare(L):
         "rsg_modelling_plan_destroy": ([vp], ip),
         "rsg_stagger_a2d_dev": ([vp, vp, ip, ip, ip, ip, ip, vp, vp, vp, vp], ip),
         "rsg_stagger_e2d_dev": ([vp, vp, vp, ip, ip, ip, ip, ip, vp, vp, vp, vp, vp, vp], ip),
+        "rsg_misfit_residuals_noalloc": ([ip, vp, vp, vp, vp, vp, ip, ip, ip, ip, ip, C.POINTER(C.c_double), vp],
+                                         ip),
     }
     for name in ("a2d_forward_velocity", "a2d_forward_stress", "e2d_forward_velocity",
                  "e2d_forward_stress", "e2d_backward_velocity", "e2d_backward_stress",
@@ -223,7 +225,7 @@ EXPORTS = (
     "rsg_domain_model_a2d", "rsg_domain_model_e2d", "rsg_image_offset", "rsg_der_apply",
     "rsg_revolve_adjust", "rsg_revolve_init", "rsg_revolve_next",
     "rsg_modelling_plan_create", "rsg_modelling_plan_run", "rsg_modelling_plan_destroy",
-    "rsg_stagger_a2d_dev", "rsg_stagger_e2d_dev",
+    "rsg_stagger_a2d_dev", "rsg_stagger_e2d_dev", "rsg_misfit_residuals_noalloc",
 )
 
 

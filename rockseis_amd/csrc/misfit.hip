@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "rsg_common.h"
@@ -630,40 +631,58 @@ struct FreeList {
     }
 };
 
+// Device scratch taken from a fixed region (the no-allocation entry point):
+// bump allocation in 256-byte steps, an error past the region's end.
+struct Arena {
+    char *base;
+    size_t cap, used;
+    template <class T>
+    int alloc(T **out, size_t n, const char *) {
+        const size_t b = (sizeof(T) * (n ? n : 1) + 255) & ~(size_t)255;
+        if (used + b > cap)
+            return fail(RSG_EINVAL, "rsg_misfit_residuals_noalloc: these sizes need more device scratch than the "
+                                    "fixed RSG_NOALLOC_SCRATCH_BYTES");
+        *out = (T *)(base + used);
+        used += b;
+        return RSG_OK;
+    }
+};
+
 // k[s]-th smallest |v| (0-based) of every segment, exactly: a radix select
 // over the bit patterns of |v| (non-negative floats order as unsigned
 // integers), four 8-bit passes, all segments per launch (at most 65535 a
-// launch).  out[s] = the bit pattern.
-int select_kth(const std::vector<SelSeg> &segs, const std::vector<unsigned long long> &kth, std::vector<unsigned> &out,
-               hipStream_t st) {
-    const int nseg = (int)segs.size();
-    out.assign(nseg, 0u);
+// launch).  out[s] = the bit pattern.  Host work arrays: out and k of nseg
+// entries, h of 256 * min(nseg, 65535); device scratch from fr.
+template <class Alloc>
+int select_kth_impl(Alloc &fr, const SelSeg *segs, const unsigned long long *kth, int nseg, unsigned *out,
+                    unsigned long long *k, unsigned *h, hipStream_t st) {
+    for (int i = 0; i < nseg; i++) {
+        out[i] = 0u;
+        k[i] = kth[i];
+    }
     if (nseg == 0) return RSG_OK;
-    FreeList fr;
     SelSeg *dsegs = nullptr;
     unsigned *hist = nullptr, *prefix = nullptr;
     const int cap = std::min(nseg, 65535);
     if (int rc = fr.alloc(&dsegs, nseg, "fwi: misfit scratch allocation failed")) return rc;
     if (int rc = fr.alloc(&hist, 256 * (size_t)cap, "fwi: misfit scratch allocation failed")) return rc;
     if (int rc = fr.alloc(&prefix, nseg, "fwi: misfit scratch allocation failed")) return rc;
-    RSG_HIP(hipMemcpyAsync(dsegs, segs.data(), sizeof(SelSeg) * nseg, hipMemcpyHostToDevice, st));
+    RSG_HIP(hipMemcpyAsync(dsegs, segs, sizeof(SelSeg) * nseg, hipMemcpyHostToDevice, st));
     size_t maxn = 1;
-    for (const SelSeg &g : segs) maxn = std::max(maxn, g.n);
+    for (int i = 0; i < nseg; i++) maxn = std::max(maxn, segs[i].n);
     unsigned blocks = (unsigned)std::min<size_t>((maxn + kThreads * 8 - 1) / (kThreads * 8), 1024);
     if (blocks < 1) blocks = 1;
-    std::vector<unsigned long long> k(kth);
-    std::vector<unsigned> h(256 * (size_t)cap);
     for (int pass = 0; pass < 4; pass++) {
         const int shift = 24 - 8 * pass;
         const unsigned mask = pass == 0 ? 0u : (0xffffffffu << (32 - 8 * pass));
-        RSG_HIP(hipMemcpyAsync(prefix, out.data(), sizeof(unsigned) * nseg, hipMemcpyHostToDevice, st));
+        RSG_HIP(hipMemcpyAsync(prefix, out, sizeof(unsigned) * nseg, hipMemcpyHostToDevice, st));
         for (int s0 = 0; s0 < nseg; s0 += cap) {
             const int ng = std::min(cap, nseg - s0);
             fill_u32<<<(256 * ng + kThreads - 1) / kThreads, kThreads, 0, st>>>(hist, 256 * ng);
             RSG_LAUNCH_CHECK("fill_u32");
             radix_hist_seg_kernel<<<dim3(blocks, ng), kThreads, 0, st>>>(dsegs + s0, prefix + s0, mask, shift, hist);
             RSG_LAUNCH_CHECK("radix_hist_seg_kernel");
-            RSG_HIP(hipMemcpyAsync(h.data(), hist, sizeof(unsigned) * 256 * ng, hipMemcpyDeviceToHost, st));
+            RSG_HIP(hipMemcpyAsync(h, hist, sizeof(unsigned) * 256 * ng, hipMemcpyDeviceToHost, st));
             RSG_HIP(hipStreamSynchronize(st));
             for (int j = 0; j < ng; j++) {
                 const int sg = s0 + j;
@@ -683,6 +702,16 @@ int select_kth(const std::vector<SelSeg> &segs, const std::vector<unsigned long 
     return RSG_OK;
 }
 
+int select_kth(const std::vector<SelSeg> &segs, const std::vector<unsigned long long> &kth, std::vector<unsigned> &out,
+               hipStream_t st) {
+    const int nseg = (int)segs.size();
+    out.assign(nseg, 0u);
+    FreeList fr;
+    std::vector<unsigned long long> k(nseg);
+    std::vector<unsigned> h(256 * (size_t)std::max(1, std::min(nseg, 65535)));
+    return select_kth_impl(fr, segs.data(), kth.data(), nseg, out.data(), k.data(), h.data(), st);
+}
+
 // Host table of the ADAPTIVE time weight (fwi.cpp:397-420) in the
 // reference's arithmetic: type 2 Gaussian exp(-it^2/(2 s^2)), s = (float)(0.01
 // nt); otherwise the ramp min(it, 500).
@@ -700,9 +729,20 @@ std::vector<float> adaptive_h(int type, int nt) {
 
 }  // namespace
 
-int acoustic_misfit_residuals(int type, const float *mod, const float *rec, const float *wei, float *res,
-                              const int *map, int dim, int ntr, int nt, int nshot, int scaling, double *misfit,
-                              hipStream_t st) {
+namespace {
+
+// Largest shot count of the no-allocation entry point: its per-shot host
+// work arrays are fixed-size arrays on the stack.
+constexpr int kNoallocShots = 64;
+
+// The acoustic misfit and residuals with device scratch from fr; the clip's
+// host work arrays in hk (kth, out, k, h as select_kth_impl takes them,
+// nshot and 256 * nshot entries).
+template <class Alloc>
+int acoustic_misfit_residuals_impl(Alloc &fr, int type, const float *mod, const float *rec, const float *wei,
+                                   float *res, const int *map, int dim, int ntr, int nt, int nshot, int scaling,
+                                   double *misfit, SelSeg *segs, unsigned long long *kth, unsigned *out,
+                                   unsigned long long *k, unsigned *h, hipStream_t st) {
     if (type == 4) type = 0;  // ADAPTIVE_SINGLE has no acoustic branch: the default one is DIFFERENCE
     if (type < 0 || type > 3)
         return fail(RSG_EINVAL,
@@ -711,15 +751,14 @@ int acoustic_misfit_residuals(int type, const float *mod, const float *rec, cons
     const size_t nper = (size_t)ntr * nt, n = nper * nshot;
     if (n == 0) return RSG_OK;
     float *norms = nullptr;
-    FreeList fr;
     if (type >= 2) {
-        const std::vector<float> h = adaptive_h(type, nt);
+        const std::vector<float> hh = adaptive_h(type, nt);
         float *hd = nullptr, *scr = nullptr;
         const size_t ntr_all = (size_t)ntr * nshot;
         if (int rc = fr.alloc(&hd, nt, "fwi: ADAPTIVE misfit scratch allocation failed")) return rc;
         if (int rc = fr.alloc(&scr, 7 * (size_t)nt * ntr_all, "fwi: ADAPTIVE misfit scratch allocation failed"))
             return rc;
-        RSG_HIP(hipMemcpyAsync(hd, h.data(), sizeof(float) * nt, hipMemcpyHostToDevice, st));
+        RSG_HIP(hipMemcpyAsync(hd, hh.data(), sizeof(float) * nt, hipMemcpyHostToDevice, st));
         adaptive_kernel<<<(unsigned)((ntr_all + 63) / 64), 64, 0, st>>>(type, mod, rec, wei, res, map, dim,
                                                                          (int)ntr_all, nt, scaling, hd, scr, misfit);
         RSG_LAUNCH_CHECK("adaptive_kernel");
@@ -737,20 +776,34 @@ int acoustic_misfit_residuals(int type, const float *mod, const float *rec, cons
     // FwiAcoustic3D::computeResiduals (fwi.cpp:1651-1853) has no clip step
     if (type == 0 || dim == 3) return RSG_OK;
     // 97th-percentile clip per shot (FwiAcoustic2D): element (int)(97*nt*ntr/100) of the sorted |res|
-    std::vector<SelSeg> segs(nshot);
-    for (int k = 0; k < nshot; k++) segs[k] = SelSeg{res + k * nper, nper};
-    std::vector<unsigned long long> kth(nshot, (unsigned long long)((97LL * nt * ntr) / 100));
-    std::vector<unsigned> pf;
-    if (int rc = select_kth(segs, kth, pf, st)) return rc;
+    for (int q = 0; q < nshot; q++) {
+        segs[q] = SelSeg{res + q * nper, nper};
+        kth[q] = (unsigned long long)((97LL * nt * ntr) / 100);
+    }
+    if (int rc = select_kth_impl(fr, segs, kth, nshot, out, k, h, st)) return rc;
     unsigned *dpf = nullptr;
     if (int rc = fr.alloc(&dpf, nshot, "fwi: misfit scratch allocation failed")) return rc;
-    RSG_HIP(hipMemcpyAsync(dpf, pf.data(), sizeof(unsigned) * nshot, hipMemcpyHostToDevice, st));
+    RSG_HIP(hipMemcpyAsync(dpf, out, sizeof(unsigned) * nshot, hipMemcpyHostToDevice, st));
     unsigned blocks = (unsigned)std::min<size_t>((nper + kThreads * 8 - 1) / (kThreads * 8), 1024);
     if (blocks < 1) blocks = 1;
     clip_kernel<<<dim3(blocks, nshot), kThreads, 0, st>>>(res, nper, dpf);
     RSG_LAUNCH_CHECK("clip_kernel");
-    RSG_HIP(hipStreamSynchronize(st));  // the scratch is freed on return
+    RSG_HIP(hipStreamSynchronize(st));  // the scratch is released (or reused) on return
     return RSG_OK;
+}
+
+}  // namespace
+
+int acoustic_misfit_residuals(int type, const float *mod, const float *rec, const float *wei, float *res,
+                              const int *map, int dim, int ntr, int nt, int nshot, int scaling, double *misfit,
+                              hipStream_t st) {
+    FreeList fr;
+    const size_t ns = (size_t)std::max(nshot, 1);
+    std::vector<SelSeg> segs(ns);
+    std::vector<unsigned long long> kth(ns), k(ns);
+    std::vector<unsigned> out(ns), h(256 * std::min<size_t>(ns, 65535));
+    return acoustic_misfit_residuals_impl(fr, type, mod, rec, wei, res, map, dim, ntr, nt, nshot, scaling, misfit,
+                                          segs.data(), kth.data(), out.data(), k.data(), h.data(), st);
 }
 
 int elastic_misfit_residuals(int type, const std::vector<E2DRecs> &shots_in, int nt, double *misfit,
@@ -921,5 +974,77 @@ extern "C" int rsg_misfit_residuals_e2d(int type, const rsg_e2d_records *shots, 
     }
     (void)hipFree(d);
     if (rc == RSG_OK && misfit) *misfit = h;
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// rsg_misfit_residuals without device allocation.  Its scratch (misfit
+// accumulator, CORRELATION norms, radix-select tables, clip values) lives in
+// one file-scope device array per GPU.  In host code the array's name is a
+// stand-in, not a device address: the address each device holds it at comes
+// from hipGetSymbolAddress, taken once per device with that device current,
+// and only that address is passed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[RSG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64] = {};
+
+int scratch_of_current_device(char **out) {
+    int dev = -1;
+    RSG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return rsg::fail(RSG_EINVAL, "rsg_misfit_residuals_noalloc: device index past 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return rsg::check_hip(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                                         "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = (char *)g_scratch_ptr[dev];
+    return RSG_OK;
+}
+
+}  // namespace
+
+extern "C" int rsg_misfit_residuals_noalloc(int type, const float *mod, const float *rec, const float *wei,
+                                            float *res, const int *map, int dim, int ntr, int nt, int nshot,
+                                            int scaling, double *misfit, void *stream) {
+    if (!mod || !rec || !res || !map || ntr < 0 || nt < 0 || nshot < 1 || (dim != 2 && dim != 3) ||
+        (scaling != 1 && scaling != -1))
+        return rsg::fail(RSG_EINVAL, "rsg_misfit_residuals_noalloc: bad arguments");
+    if (type != 0 && type != 1 && type != 4)
+        return rsg::fail(RSG_EINVAL, "rsg_misfit_residuals_noalloc: DIFFERENCE (0), CORRELATION (1) or "
+                                     "ADAPTIVE_SINGLE (4, the DIFFERENCE branch) only; the ADAPTIVE types take "
+                                     "rsg_misfit_residuals");
+    if (nshot > rsg::kNoallocShots)
+        return rsg::fail(RSG_EINVAL, "rsg_misfit_residuals_noalloc: at most 64 shots per call");
+    char *base = nullptr;
+    if (int rc = scratch_of_current_device(&base)) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    rsg::Arena ar{base, (size_t)RSG_NOALLOC_SCRATCH_BYTES, 0};
+    double *d = nullptr;
+    if (int rc = ar.alloc(&d, 1, "")) return rc;
+    // the array keeps the previous call's contents: clear the accumulator
+    // with a fill kernel
+    int rc = rsg::launch_zero(d, sizeof(double), st);
+    rsg::SelSeg segs[rsg::kNoallocShots];
+    unsigned long long kth[rsg::kNoallocShots], k[rsg::kNoallocShots];
+    unsigned out[rsg::kNoallocShots], h[256 * rsg::kNoallocShots];
+    if (rc == RSG_OK)
+        rc = rsg::acoustic_misfit_residuals_impl(ar, type, mod, rec, wei, res, map, dim, ntr, nt, nshot, scaling, d,
+                                                 segs, kth, out, k, h, st);
+    double hm = 0.0;
+    if (rc == RSG_OK) {
+        const hipError_t e = hipMemcpyAsync(&hm, d, sizeof(double), hipMemcpyDeviceToHost, st);
+        const hipError_t e2 = hipStreamSynchronize(st);
+        if (e != hipSuccess || e2 != hipSuccess)
+            rc = rsg::check_hip(e != hipSuccess ? e : e2, "rsg_misfit_residuals_noalloc");
+    } else {
+        (void)hipStreamSynchronize(st);
+    }
+    if (rc == RSG_OK && misfit) *misfit = hm;
     return rc;
 }
