@@ -45,7 +45,13 @@ def parse():
                     help="largest hipGraph of the time loop (graphs of 1, 2, 4, ... steps up to it are "
                          "captured before the timed region; K steps replay its binary decomposition)")
     ap.add_argument("--kernel-reps", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=80, help="time steps of the CPU sample")
+    ap.add_argument("--cpu-steps", type=int, default=40, help="time steps of the CPU sample")
+    ap.add_argument("--cpu-start", type=int, default=3000,
+                    help="first step of the CPU sample (the GPU run's state at that step is its start)")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="most threads of the shot-parallel CPU figure (the box's CPU share)")
+    ap.add_argument("--cpu-config-steps", type=int, default=10,
+                    help="time steps of the single-thread CPU samples of configs 3 and 5")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fwi", action="store_true", help="skip the FWI shots/s leg")
     ap.add_argument("--legs", default="a3d,e2dfwi",
@@ -61,6 +67,10 @@ def parse():
                     help="only the roofline kernel timing (20 modelling steps for a non-zero state, then "
                          "--kernel-reps back-to-back launches of each half-step kernel): the launches a "
                          "rocprofv3 --kernel-trace --stats run of this command summarizes")
+    ap.add_argument("--pmc-a3d", action="store_true",
+                    help="no timing: the acoustic 3D workload of the a3d leg (440^3 padded, order 4), 20 "
+                         "modelling steps, 10 launches of each half-step kernel and 5 calibration copies "
+                         "(torch.neg of 256 MiB), for rocprofv3 --pmc passes")
     ap.add_argument("--pmc-dispatch", action="store_true",
                     help="no timing: 100 modelling steps, 10 launches of each half-step kernel and 5 "
                          "calibration copies (torch.neg of 256 MiB), for rocprofv3 --pmc passes")
@@ -87,6 +97,50 @@ def pmc_dispatch():
         torch.neg(x, out=y)
     torch.cuda.synchronize()
     print("pmc dispatch done", flush=True)
+
+
+def _a3d_model():
+    """The acoustic 3D modelling run of the a3d leg, prepared (not run)."""
+    import numpy as np
+    from rockseis_amd import core, synth
+    n, lpml, dx, nt, dt, order = 400, 20, 10.0, 2500, 1e-3, 4
+    vp = synth.layered_anomaly_vp3d(n, n, n, dx, dx, dx, nanom=10, seed=44)
+    rho = synth.gardner(vp)
+    m = core.ModelAcoustic3D(n, n, n, lpml, dx, dx, dx)
+    m.Vp[:], m.R[:] = vp, rho
+    src = core.Data3D(1, nt, dt, 0.0, "P")
+    src.scoords[:] = np.float32([[n // 2 * dx, n // 2 * dx, 10.0]])
+    src.data[:] = synth.ricker(nt, dt, 12.0)
+    src.makeMap(m, core.SMAP)
+    gx, gy = np.meshgrid(np.arange(200) * 20.0, np.arange(200) * 20.0)
+    gc = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, 10.0)], 1).astype(np.float32)
+    rec = core.Data3D(gc.shape[0], nt, dt, 0.0, "P")
+    rec.gcoords[:] = gc
+    rec.makeMap(m, core.GMAP)
+    m.staggerModels()
+    mod = core.ModellingAcoustic3D(m, src, order, 1)
+    mod.setRecP(rec)
+    return mod, m, src, rec, gc, (n, lpml, dx, nt, dt, order)
+
+
+def pmc_a3d():
+    import torch
+    from rockseis_amd import _lib as R
+    mod = _a3d_model()[0]
+    mod.prepare()
+    mod.launch(0, 20)
+    torch.cuda.synchronize()
+    g, s = mod._args[0], mod._args[1]
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for which in (0, 1):
+        R.check(R.lib().rsg_repeat_step(R.KIND_A3D, which, C.byref(g), C.byref(s), 10, st))
+    torch.cuda.synchronize()
+    x = torch.ones(64 * 1024 * 1024, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print("pmc a3d dispatch done", flush=True)
 
 
 def roofline_only(reps):
@@ -182,37 +236,153 @@ def kernel_time_ms(mod, which, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def cpu_baseline(cfg, inputs, nsteps):
-    """The reference WavesElastic2D (oracle/_ref, compiled from the reference
-    sources, single thread) or, where that build is absent, the oracle's C
-    restatement: full padded grid, `nsteps` time steps; the one-step run is
-    subtracted to remove set-up."""
+def _host_cpu():
+    """The host CPU's model name and the cores this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def _e2d_state(mod):
+    """The elastic state of a modelling run as one flat host array: Vx, Vz,
+    Sxx, Szz, Sxz, then the 16 PML strips in the reference's member order."""
     import numpy as np
+    w = mod.waves
+    parts = [getattr(w, f).reshape(-1) for f in ("Vx", "Vz", "Sxx", "Szz", "Sxz")]
+    parts += [w.psi[n].reshape(-1) for n in w.strips]
+    import torch
+    return torch.cat(parts).cpu().numpy().astype(np.float32)
+
+
+def cpu_baseline(cfg, inputs, state, it0, nsteps, max_threads):
+    """The reference WavesElastic2D (oracle/_ref, compiled from the reference
+    sources) on this host, on a bounded sample of the same workload: time
+    steps it0 .. it0+nsteps-1 of the run, started from the state the GPU run
+    reached at step it0 (late in the run the wavefield covers most of the
+    grid; the first steps of a run, mostly exact zeros, run faster on the
+    CPU).  value: one thread.  shot_parallel: min(cores, max_threads) threads
+    each running the same window as an independent shot at once (the
+    reference's MPI task farm runs one shot per rank,
+    parallel.cpp:690-732), summed.  Where the reference build is absent,
+    the oracle's C restatement runs the first nsteps steps instead."""
+    import concurrent.futures as cf
     import oracle
     from oracle import runs
     vp, vs, rho, src, recs = inputs
-    sc = src.scoords
-    gc = recs[0].gcoords
-    wav = src.data[:, :nsteps + 1]
-    args = (vp, vs, rho, cfg["lpml"], cfg["dx"], cfg["dx"], cfg["dt"])
-    kind = "reference" if oracle.ref_available() else "port"
-    fn = runs.ref_e2d_run if kind == "reference" else runs.e2d_run
-
-    def run(n):
-        t = time.perf_counter()
-        fn(*args, n, cfg["order"], False, "P", sc, np.ascontiguousarray(wav[:, :n]), "VX", gc)
-        return time.perf_counter() - t
-
-    t1 = run(1)
-    tn = run(nsteps + 1)
-    per_step = (tn - t1) / nsteps
     cells = cfg["nx_pml"] * cfg["nz_pml"]
-    return {"value": round(cells / per_step / 1e9, 6), "unit": "Gcell-updates/s", "cores": 1,
-            "kind": kind,
-            "sample": (f"{nsteps} time steps of the same elastic 2D workload on the full "
-                       f"{cfg['nx_pml']}x{cfg['nz_pml']} padded grid, order {cfg['order']}, "
-                       f"single thread ({'reference WavesElastic2D built from its sources' if kind == 'reference' else 'oracle C restatement'}), "
-                       f"{per_step:.3f} s per step")}
+    cpu_model, cores = _host_cpu()
+    args = (vp, vs, rho, cfg["lpml"], cfg["dx"], cfg["dx"], cfg["dt"], cfg["nt"], cfg["order"], False,
+            src.scoords, src.data, recs[0].gcoords, it0, nsteps, state)
+    if not oracle.ref_available():
+        t0 = time.perf_counter()
+        runs.e2d_run(vp, vs, rho, cfg["lpml"], cfg["dx"], cfg["dx"], cfg["dt"], nsteps, cfg["order"], False, "P",
+                     src.scoords, src.data[:, :nsteps], "VX", recs[0].gcoords)
+        t = time.perf_counter() - t0
+        return {"value": round(cells * nsteps / t / 1e9, 6), "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
+                "cpu_model": cpu_model, "host_cores": cores,
+                "sample": f"first {nsteps} steps of the workload, oracle C restatement, one thread (set-up included)"}
+    t1 = runs.ref_e2d_time_window(*args)
+    nthr = max(1, min(cores, max_threads))
+    with cf.ThreadPoolExecutor(nthr) as ex:
+        ts = list(ex.map(lambda _: runs.ref_e2d_time_window(*args), range(nthr)))
+    par = sum(cells * nsteps / t for t in ts)
+    return {"value": round(cells * nsteps / t1 / 1e9, 6), "unit": "Gcell-updates/s", "cores": 1,
+            "kind": "reference", "cpu_model": cpu_model, "host_cores": cores,
+            "seconds_per_step": round(t1 / nsteps, 4),
+            "shot_parallel": {"value": round(par / 1e9, 6), "unit": "Gcell-updates/s", "threads": nthr,
+                              "seconds_per_step_per_shot": round(max(ts) / nsteps, 4)},
+            "sample": (f"steps {it0}..{it0 + nsteps - 1} of the same elastic 2D workload (nt {cfg['nt']}) on the "
+                       f"full {cfg['nx_pml']}x{cfg['nz_pml']} padded grid, order {cfg['order']}, started from the "
+                       "GPU run's state at that step; reference WavesElastic2D built from its sources, "
+                       "velocity + stress + source + 2 receiver sets per step, set-up excluded")}
+
+
+def cpu_configs(n_window):
+    """Single-thread reference step times on configs 3 and 5 (late windows
+    from GPU states), and the per-shot FWI gradient times they imply:
+    (numforw(nt, 50) + nt) steps of the Revolve schedule (the imaging
+    condition and checkpoint copies not counted).  Config 4's 440^3 grid
+    takes about a minute per reference step on one core and is left out of
+    the default run."""
+    import numpy as np
+    import torch
+    from oracle import runs
+    from rockseis_amd import core, synth
+    out = {}
+    # config 3: acoustic 2D 801 x 401 at 10 m (861 x 461 padded), order 8, smoothed model
+    nx, nz, dx, lpml, nt, dt, order = 801, 401, 10.0, 30, 3000, 1e-3, 8
+    vt = synth.gradient_anomaly_vp(nx, nz, dx, dx, 1500.0, 4000.0, nanom=20, seed=7)
+    vp = synth.gaussian_smooth(vt, 250.0, dx, dx)
+    rho = synth.gardner(vp)
+    m = core.ModelAcoustic2D(nx, nz, lpml, dx, dx, 0.0, 0.0, False)
+    m.Vp[:], m.R[:] = vp, rho
+    src = core.Data2D(1, nt, dt, 0.0, "P")
+    src.scoords[:] = np.float32([[125.0 + 250.0 * 16, 10.0]])
+    src.data[:] = synth.ricker(nt, dt, 10.0)
+    src.makeMap(m, core.SMAP)
+    rec = core.Data2D(nx, nt, dt, 0.0, "P")
+    rec.gcoords[:] = synth.surface_line(nx, dx, 10.0)
+    rec.makeMap(m, core.GMAP)
+    m.staggerModels()
+    mod = core.ModellingAcoustic2D(m, src, order, 1)
+    mod.setRecP(rec)
+    mod.prepare()
+    it0 = nt // 2
+    mod.launch(0, it0)
+    torch.cuda.synchronize()
+    w = mod.waves
+    st = torch.cat([w.P.reshape(-1), w.Vx.reshape(-1), w.Vz.reshape(-1)] +
+                   [w.psi[n].reshape(-1) for n in w.strips]).cpu().numpy()
+    t = runs.ref_a2d_time_window(vp, rho, lpml, dx, dx, dt, nt, order, src.scoords, src.data, rec.gcoords, it0,
+                                 n_window, st)
+    steps = _numforw(nt, 50) + nt
+    out["config3"] = {"seconds_per_step": round(t / n_window, 4),
+                      "fwi_gradient_s_per_shot_estimate": round(t / n_window * steps, 1),
+                      "sample": (f"acoustic 2D 861x461 padded, order 8: steps {it0}..{it0 + n_window - 1} of a "
+                                 f"shot from the GPU state, one thread; gradient = {steps} steps")}
+    del mod
+    # config 5: one elastic window of 1300 x 600 cells (+lpml 30) at 5 m, order 8
+    nxw, nz, dx, lpml, nt, dt = 1300, 600, 5.0, 30, 4000, 4.5e-4
+    vp = np.clip(synth.gradient_anomaly_vp(nxw, nz, dx, dx, 1500.0, 4500.0, nanom=12, seed=5), 1500.0, 4500.0)
+    vp = vp.astype(np.float32)
+    vs, rho = (vp / 1.8).astype(np.float32), synth.gardner(vp)
+    m = core.ModelElastic2D(nxw, nz, lpml, dx, dx, 0.0, 0.0, False)
+    m.Vp[:], m.Vs[:], m.R[:] = vp, vs, rho
+    src = core.Data2D(1, nt, dt, 0.0, "P")
+    src.scoords[:] = np.float32([[nxw // 2 * dx, 10.0]])
+    src.data[:] = synth.ricker(nt, dt, 15.0)
+    src.makeMap(m, core.SMAP)
+    recs = []
+    for f in ("VX", "VZ"):
+        r = core.Data2D(nxw, nt, dt, 0.0, f)
+        r.gcoords[:] = synth.surface_line(nxw, dx, 10.0)
+        r.makeMap(m, core.GMAP)
+        recs.append(r)
+    m.staggerModels()
+    mod = core.ModellingElastic2D(m, src, 8, 1)
+    mod.setRecVx(recs[0])
+    mod.setRecVz(recs[1])
+    mod.prepare()
+    it0 = nt // 2
+    mod.launch(0, it0)
+    torch.cuda.synchronize()
+    st = _e2d_state(mod)
+    t = runs.ref_e2d_time_window(vp, vs, rho, lpml, dx, dx, dt, nt, 8, False, src.scoords, src.data,
+                                 recs[0].gcoords, it0, n_window, st)
+    steps = _numforw(nt, 50) + nt
+    out["config5"] = {"seconds_per_step": round(t / n_window, 4),
+                      "fwi_gradient_s_per_window_estimate": round(t / n_window * steps, 1),
+                      "sample": (f"one elastic window of 1300 x 600 cells (1360 x 660 padded), order 8: steps "
+                                 f"{it0}..{it0 + n_window - 1} from the GPU state, one thread; gradient = {steps} "
+                                 "steps at the forward step's cost")}
+    return out
 
 
 def main():
@@ -223,6 +393,9 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if args.pmc_a3d:
+        pmc_a3d()
+        return
     if args.pmc_dispatch:
         pmc_dispatch()
         return
@@ -259,6 +432,14 @@ def main():
     b_str = 44 * cells + 2 * 8 * strip_cells
     t_vel = kernel_time_ms(mod, 0, args.kernel_reps)
     t_str = kernel_time_ms(mod, 1, args.kernel_reps)
+    # the CPU sample starts from the GPU run's state at step cpu_it0
+    cpu_it0 = max(0, min(args.cpu_start, cfg["nt"] - args.cpu_steps))
+    cpu_state = None
+    if not args.no_cpu:
+        reset(mod)
+        mod.launch(0, cpu_it0)
+        torch.cuda.synchronize()
+        cpu_state = _e2d_state(mod)
     step_bytes = b_vel + b_str
     roof = {"bound": "hbm", "kernel": "e2d_stress_kernel<8>", "achieved": round(b_str / (t_str * 1e-3) / 1e9, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -299,7 +480,11 @@ def main():
         torch.cuda.empty_cache()
         out["e2dfwi"] = e2dfwi_leg(args.e2dfwi_shots, eager=args.e2dfwi_eager)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg, inputs, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(cfg, inputs, cpu_state, cpu_it0, args.cpu_steps, args.cpu_threads)
+        _progress("cpu baseline timed")
+        if not args.no_fwi:
+            out["cpu_baseline"]["per_config"] = cpu_configs(args.cpu_config_steps)
+            _progress("cpu per-config windows timed")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
@@ -431,27 +616,10 @@ def a3d_leg():
     200 x 200 receivers every 20 m at z = 10 m, dt 1 ms, nt 2500.  The
     modelled records of the true model are the observed data of the gradient,
     which runs on the layered background (the true model without its
-    anomalies), with 50 Revolve checkpoints in FP16."""
-    import numpy as np
+    anomalies), with 50 Revolve checkpoints in 16 bits (S16)."""
     import torch
     from rockseis_amd import _lib as R, core, fwi, synth
-    n, lpml, dx, nt, dt, order = 400, 20, 10.0, 2500, 1e-3, 4
-    vp = synth.layered_anomaly_vp3d(n, n, n, dx, dx, dx, nanom=10, seed=44)
-    rho = synth.gardner(vp)
-    m = core.ModelAcoustic3D(n, n, n, lpml, dx, dx, dx)
-    m.Vp[:], m.R[:] = vp, rho
-    src = core.Data3D(1, nt, dt, 0.0, "P")
-    src.scoords[:] = np.float32([[n // 2 * dx, n // 2 * dx, 10.0]])
-    src.data[:] = synth.ricker(nt, dt, 12.0)
-    src.makeMap(m, core.SMAP)
-    gx, gy = np.meshgrid(np.arange(200) * 20.0, np.arange(200) * 20.0)
-    gc = np.stack([gx.ravel(), gy.ravel(), np.full(gx.size, 10.0)], 1).astype(np.float32)
-    rec = core.Data3D(gc.shape[0], nt, dt, 0.0, "P")
-    rec.gcoords[:] = gc
-    rec.makeMap(m, core.GMAP)
-    m.staggerModels()
-    mod = core.ModellingAcoustic3D(m, src, order, 1)
-    mod.setRecP(rec)
+    mod, m, src, rec, gc, (n, lpml, dx, nt, dt, order) = _a3d_model()
     mod.setGraphChunk(64)
     mod.prepare()
     mod.capture()
@@ -495,7 +663,7 @@ def a3d_leg():
     f.setVpgrad(vg)
     f.setRhograd(rg)
     f.setNcheck(50)
-    f.setFp16Storage(True)
+    f.setFp16Storage("s16")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     f.run_optimal()
@@ -505,7 +673,8 @@ def a3d_leg():
     return {"workload": "acoustic 3D, 400^3 at 10 m (440^3 padded), order 4, nt 2500, 200x200 receivers",
             "modelling_gcell_per_s": round(N * nt / wall / 1e9, 3), "modelling_s": round(wall, 3),
             "kernels": kt, "fwi_gradient_s": round(tg, 2),
-            "fwi": "FwiAcoustic3D::run_optimal, 50 Revolve checkpoints in FP16, layered background model",
+            "fwi": ("FwiAcoustic3D::run_optimal, 50 Revolve checkpoints in 16 bits (S16: int16 mantissas, one "
+                    "power-of-two exponent per 256 values), layered background model"),
             "misfit": f.misfit64}
 
 
@@ -515,7 +684,7 @@ def e2dfwi_leg(nshot, n_plain=4, eager=False):
     seeded widths 600..2000 x 600 cells, order 8, lpml 30, 15 Hz Ricker
     explosive source at the window centre (z = 10 m), receivers at every cell
     of the window at z = 10 m (P, Ux, Uz), dt 0.45 ms, nt 4000, 50 Revolve
-    checkpoints in FP16; lambda / mu / rho kernels.  The observed records are
+    checkpoints in 16 bits (S16); lambda / mu / rho kernels.  The observed records are
     zero (the timed work does not depend on their values).  Plain = the first
     n_plain windows one at a time."""
     import numpy as np
@@ -537,7 +706,7 @@ def e2dfwi_leg(nshot, n_plain=4, eager=False):
         sc = np.float32([[(x0 + w // 2) * dx, 10.0]])
         shots.append(fwi.ElasticShot(sc, wav, gc, np.zeros((3, w, nt), np.float32), x0, w, "P"))
         cells += (w + 2 * lpml) * (nz + 2 * lpml)
-    sv = fwi.ElasticSurvey(shots, lpml, dx, dx, dt, order, False, 50, fp16=True)
+    sv = fwi.ElasticSurvey(shots, lpml, dx, dx, dt, order, False, 50, fp16="s16")
     _progress(f"e2dfwi: {nshot} windows set up ({cells} padded cells)")
     torch.cuda.synchronize()
     te = mis = None
@@ -568,14 +737,14 @@ def e2dfwi_leg(nshot, n_plain=4, eager=False):
     n_plain = max(1, min(n_plain, nshot))
     t0 = time.perf_counter()
     for sh in shots[:n_plain]:
-        one = fwi.ElasticSurvey([sh], lpml, dx, dx, dt, order, False, 50, fp16=True)
+        one = fwi.ElasticSurvey([sh], lpml, dx, dx, dt, order, False, 50, fp16="s16")
         fwi.elastic_batch_gradient(vp, vs, rho, one, scale=False)
     torch.cuda.synchronize()
     tp = time.perf_counter() - t0
     _progress(f"e2dfwi: plain gradients of {n_plain} windows {tp:.2f} s")
     steps = nt + _numforw(nt, 50)  # forward steps incl. recomputation + adjoint steps
     return {"workload": f"{nshot} elastic windows of 600-2000 x 600 cells (+lpml 30) from a 6000x600 model at "
-                        "5 m, order 8, nt 4000, FP16 Revolve (50)",
+                        "5 m, order 8, nt 4000, 16-bit (S16) Revolve checkpoints (50)",
             "padded_cells_per_step": cells, "batched_s": round(tb, 2),
             "batched": "persistent plan: the Revolve sweep as two hipGraphs replayed (rsg_fwi_e2d_plan_run)",
             "plan_build_s": round(t_build, 2), "rows_per_thread": rz_ragged,

@@ -113,10 +113,12 @@ int rsg_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
 int rsg_zero(void *ptr, size_t bytes, void *stream);
 
 /* Launch-shape knobs: "rz2d" (rows per thread of the 2D tiles: 1, 2, 4, 8),
- * "rz2d_ragged" (the same for ragged batches, default 2), "pml_cost" (2D:
- * weight in percent of a PML / edge tile against an interior one when the
- * tiles are split into eight XCD ranges of equal cost, default 200), "zc3d"
- * (z planes per 3D block, 0 = automatic), "fuse_io" (modelling runs: sources
+ * "rz2d_ragged" (the same for ragged batches, default 2), "pml_cost" (weight
+ * in percent of a PML / edge cell or tile against an interior one when the
+ * tiles are split into eight XCD ranges of equal cost, 2D and 3D, default
+ * 200), "zc3d" (z planes per 3D block, 0 = automatic), "xcd3d" (3D tile
+ * order: 1 = each XCD marches its own contiguous group of (x, y) columns
+ * through z, 0 = grid order; default 1), "fuse_io" (modelling runs: sources
  * and velocity gathers in the stress kernel, 1 = on).  Results do not depend
  * on them; rockseis_amd.autotune picks rz2d, pml_cost and zc3d per
  * workload. */

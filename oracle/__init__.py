@@ -201,6 +201,13 @@ def _declare_ref(R):
     R.ref_e2d_run.argtypes = ([_f32p, _f32p, _f32p] + [C.c_int] * 3 + [C.c_float] * 3
                               + [C.c_int] * 5 + [_f32p, _f32p, C.c_int, C.c_int, C.c_int, _f32p,
                                                  _f32p, C.c_int, _f32p])
+    R.ref_a2d_time_window.argtypes = ([_f32p, _f32p] + [C.c_int] * 3 + [C.c_float] * 3 + [C.c_int] * 3
+                                      + [_f32p, _f32p, C.c_int, _f32p, C.c_int, C.c_int, _f32p])
+    R.ref_a2d_time_window.restype = C.c_double
+    R.ref_e2d_time_window.argtypes = ([_f32p, _f32p, _f32p] + [C.c_int] * 3 + [C.c_float] * 3
+                                      + [C.c_int] * 4 + [_f32p, _f32p, C.c_int, _f32p, C.c_int, C.c_int,
+                                                         _f32p])
+    R.ref_e2d_time_window.restype = C.c_double
     R.ref_a3d_run.argtypes = ([_f32p, _f32p] + [C.c_int] * 4 + [C.c_float] * 4 + [C.c_int] * 5
                               + [_f32p, _f32p, C.c_int, C.c_int, _f32p, _f32p, _f32p])
     R.ref_fwi_a2d.argtypes = ([_f32p, _f32p] + [C.c_int] * 3 + [C.c_float] * 3 + [C.c_int] * 5

@@ -8,6 +8,7 @@
 //
 // Every entry point is extern "C" with plain pointers; all arrays are
 // float32, x fastest.
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -226,6 +227,118 @@ void ref_e2d_run(const float *vp, const float *vs, const float *rho, int nx, int
     std::memcpy(rec, data->getData(), sizeof(float) * nrec * nt);
     float *f[5] = {waves->getVx(), waves->getVz(), waves->getSxx(), waves->getSzz(), waves->getSxz()};
     for (int j = 0; j < 5; j++) std::memcpy(fields + np * j, f[j], sizeof(float) * np);
+}
+
+// Timing of the reference's acoustic 2D time step (WavesAcoustic2D::
+// forwardstepVelocity / forwardstepStress, insertSource, recordData of one
+// PRESSURE receiver set, modelling.cpp:242-286) on a given state: P, Vx, Vz
+// (each nx_pml*nz_pml), then the 8 PML strips in PmlAcoustic2D member order
+// (pml.h:103-111: P_left, P_right, P_top, P_bottom, Vxx_left, Vxx_right,
+// Vzz_top, Vzz_bottom).  Returns the seconds of steps it0 .. it0+n-1.
+double ref_a2d_time_window(const float *vp, const float *rho, int nx, int nz, int lpml, float dx, float dz,
+                           float dt, int nt, int order, int nsrc, const float *scoords, const float *wav, int nrec,
+                           const float *gcoords, int it0, int n, const float *state) {
+    auto model = std::make_shared<ModelAcoustic2D<float>>(nx, nz, lpml, dx, dz, 0.f, 0.f, false);
+    model->createModel();
+    std::memcpy(model->getVp(), vp, sizeof(float) * nx * nz);
+    std::memcpy(model->getR(), rho, sizeof(float) * nx * nz);
+    auto src = std::make_shared<Data2D<float>>(nsrc, nt, dt, 0.f);
+    src->setField(PRESSURE);
+    fill2<float>(src, scoords, true);
+    std::memcpy(src->getData(), wav, sizeof(float) * nsrc * nt);
+    src->makeMap(model->getGeom(), SMAP);
+    auto data = std::make_shared<Data2D<float>>(nrec, nt, dt, 0.f);
+    data->setField(PRESSURE);
+    fill2<float>(data, gcoords, false);
+    data->makeMap(model->getGeom(), GMAP);
+    model->staggerModels();
+    auto waves = std::make_shared<WavesAcoustic2D<float>>(model, nt, dt, 0.f);
+    auto der = std::make_shared<Der<float>>(waves->getNx_pml(), 1, waves->getNz_pml(), waves->getDx(), 1.0,
+                                            waves->getDz(), order);
+    waves->getPml()->setSmax(SMAX);
+    waves->getPml()->computeABC();
+    const size_t nxp = waves->getNx_pml(), nzp = waves->getNz_pml(), np = nxp * nzp;
+    float *f[3] = {waves->getP(), waves->getVx(), waves->getVz()};
+    for (int j = 0; j < 3; j++) std::memcpy(f[j], state + np * j, sizeof(float) * np);
+    auto pm = waves->getPml();
+    float *strips[8] = {pm->P_left, pm->P_right, pm->P_top, pm->P_bottom,
+                        pm->Vxx_left, pm->Vxx_right, pm->Vzz_top, pm->Vzz_bottom};
+    const float *p = state + 3 * np;
+    for (int j = 0; j < 8; j++) {
+        const size_t len = (j % 4) < 2 ? (size_t)lpml * nzp : nxp * (size_t)lpml;
+        std::memcpy(strips[j], p, sizeof(float) * len);
+        p += len;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int it = it0; it < it0 + n; it++) {
+        waves->forwardstepVelocity(model, der);
+        waves->forwardstepStress(model, der);
+        waves->insertSource(model, src, SMAP, it);
+        waves->recordData(data, GMAP, it);
+    }
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Timing of the reference's elastic 2D time step (WavesElastic2D::
+// forwardstepVelocity / forwardstepStress, insertSource, recordData of a VX
+// and a VZ receiver set, as Modelling::run steps, modelling.cpp:242-286) on
+// a given state: state holds the 5 fields (Vx, Vz, Sxx, Szz, Sxz, each
+// nx_pml*nz_pml) then the 16 PML strips in PmlElastic2D member order
+// (pml.h:159-178; x strips lpml*nz_pml, z strips nx_pml*lpml).  Steps
+// it0 .. it0+n-1 run from that state; the return value is their wall time in
+// seconds, set-up excluded.  Each call builds its own objects, so calls on
+// separate threads run independently (the CPU baseline's shot-parallel
+// figure).
+double ref_e2d_time_window(const float *vp, const float *vs, const float *rho, int nx, int nz, int lpml,
+                           float dx, float dz, float dt, int nt, int order, int fs, int nsrc,
+                           const float *scoords, const float *wav, int nrec, const float *gcoords, int it0,
+                           int n, const float *state) {
+    auto model = std::make_shared<ModelElastic2D<float>>(nx, nz, lpml, dx, dz, 0.f, 0.f, fs != 0);
+    model->createModel();
+    std::memcpy(model->getVp(), vp, sizeof(float) * nx * nz);
+    std::memcpy(model->getVs(), vs, sizeof(float) * nx * nz);
+    std::memcpy(model->getR(), rho, sizeof(float) * nx * nz);
+    auto src = std::make_shared<Data2D<float>>(nsrc, nt, dt, 0.f);
+    src->setField(PRESSURE);
+    fill2<float>(src, scoords, true);
+    std::memcpy(src->getData(), wav, sizeof(float) * nsrc * nt);
+    src->makeMap(model->getGeom(), SMAP);
+    std::shared_ptr<Data2D<float>> rec[2];
+    const rs_field rf[2] = {VX, VZ};
+    for (int k = 0; k < 2; k++) {
+        rec[k] = std::make_shared<Data2D<float>>(nrec, nt, dt, 0.f);
+        rec[k]->setField(rf[k]);
+        fill2<float>(rec[k], gcoords, false);
+        rec[k]->makeMap(model->getGeom(), GMAP);
+    }
+    model->staggerModels();
+    auto waves = std::make_shared<WavesElastic2D<float>>(model, nt, dt, 0.f);
+    auto der = std::make_shared<Der<float>>(waves->getNx_pml(), 1, waves->getNz_pml(), waves->getDx(), 1.0,
+                                            waves->getDz(), order);
+    waves->getPml()->setSmax(SMAX);
+    waves->getPml()->computeABC();
+    const size_t nxp = waves->getNx_pml(), nzp = waves->getNz_pml(), np = nxp * nzp;
+    float *f[5] = {waves->getVx(), waves->getVz(), waves->getSxx(), waves->getSzz(), waves->getSxz()};
+    for (int j = 0; j < 5; j++) std::memcpy(f[j], state + np * j, sizeof(float) * np);
+    auto pm = waves->getPml();
+    float *strips[16] = {pm->Sxx_left, pm->Sxx_right, pm->Sxzx_left, pm->Sxzx_right, pm->Szz_top, pm->Szz_bottom,
+                         pm->Sxzz_top, pm->Sxzz_bottom, pm->Vxx_left, pm->Vxx_right, pm->Vzx_left, pm->Vzx_right,
+                         pm->Vzz_top, pm->Vzz_bottom, pm->Vxz_top, pm->Vxz_bottom};
+    const float *p = state + 5 * np;
+    for (int j = 0; j < 16; j++) {
+        const size_t len = (j % 8) < 4 ? (size_t)lpml * nzp : nxp * (size_t)lpml;
+        std::memcpy(strips[j], p, sizeof(float) * len);
+        p += len;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int it = it0; it < it0 + n; it++) {
+        waves->forwardstepVelocity(model, der);
+        waves->forwardstepStress(model, der);
+        waves->insertSource(model, src, SMAP, it);
+        waves->recordData(model, rec[0], GMAP, it);
+        waves->recordData(model, rec[1], GMAP, it);
+    }
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // Acoustic 3D: coords are (x,y,z) triples.  fields: P,Vx,Vy,Vz.

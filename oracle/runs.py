@@ -291,6 +291,35 @@ def ref_e2d_run(vp, vs, rho, lpml, dx, dz, dt, nt, order, fs, srcfield, scoords,
     return dict(rec=rec, **dict(zip(E2DState.FIELDS, fields)))
 
 
+def ref_a2d_time_window(vp, rho, lpml, dx, dz, dt, nt, order, scoords, wav, gcoords, it0, n, state):
+    """Seconds the compiled reference takes for time steps it0 .. it0+n-1 of
+    an acoustic 2D run (PRESSURE source and receivers, no free surface)
+    started from `state`: P, Vx, Vz then the 8 PML strips, flat float32
+    (ref_harness.cpp ref_a2d_time_window)."""
+    nz, nx = vp.shape
+    sc = np.ascontiguousarray(scoords, np.float32).reshape(-1, 2)
+    gc = np.ascontiguousarray(gcoords, np.float32).reshape(-1, 2)
+    return ref().ref_a2d_time_window(*(np.ascontiguousarray(a, np.float32).ravel() for a in (vp, rho)),
+                                     nx, nz, lpml, dx, dz, dt, nt, order, sc.shape[0], sc.ravel(),
+                                     np.ascontiguousarray(wav, np.float32).ravel(), gc.shape[0], gc.ravel(),
+                                     int(it0), int(n), np.ascontiguousarray(state, np.float32).ravel())
+
+
+def ref_e2d_time_window(vp, vs, rho, lpml, dx, dz, dt, nt, order, fs, scoords, wav, gcoords, it0, n, state):
+    """Seconds the compiled reference takes for time steps it0 .. it0+n-1 of
+    an elastic 2D run (explosive source, VX and VZ receivers) started from
+    `state`: Vx, Vz, Sxx, Szz, Sxz then the 16 PML strips, flat float32
+    (ref_harness.cpp ref_e2d_time_window).  Releases the GIL: calls on
+    separate threads run in parallel."""
+    nz, nx = vp.shape
+    sc = np.ascontiguousarray(scoords, np.float32).reshape(-1, 2)
+    gc = np.ascontiguousarray(gcoords, np.float32).reshape(-1, 2)
+    return ref().ref_e2d_time_window(*(np.ascontiguousarray(a, np.float32).ravel() for a in (vp, vs, rho)),
+                                     nx, nz, lpml, dx, dz, dt, nt, order, int(fs), sc.shape[0], sc.ravel(),
+                                     np.ascontiguousarray(wav, np.float32).ravel(), gc.shape[0], gc.ravel(),
+                                     int(it0), int(n), np.ascontiguousarray(state, np.float32).ravel())
+
+
 # --------------------------------------------------------------------------
 # Acoustic 3D
 # --------------------------------------------------------------------------

@@ -430,13 +430,24 @@ extern "C" int rsg_modelling_plan_create(int kind, const rsg_grid *g, const rsg_
             if (rc == RSG_OK) {
                 hipGraphExec_t ex = nullptr;
                 const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
-                if (ei != hipSuccess) rc = check_hip(ei, "hipGraphInstantiate");
-                else p->exec.push_back(ex);
+                if (ei != hipSuccess) {
+                    rc = check_hip(ei, "hipGraphInstantiate");
+                } else {
+                    p->exec.push_back(ex);
+                    // upload now: a graph's first launch otherwise pays for
+                    // it, inside the caller's first run of that size
+                    const hipError_t eu = hipGraphUpload(ex, cap);
+                    if (eu != hipSuccess) rc = check_hip(eu, "hipGraphUpload");
+                }
             }
             if (graph) (void)hipGraphDestroy(graph);
         }
     } while (0);
-    if (cap) (void)hipStreamDestroy(cap);
+    if (cap) {
+        const hipError_t e = hipStreamSynchronize(cap);  // uploads done before any replay
+        if (rc == RSG_OK && e != hipSuccess) rc = check_hip(e, "hipStreamSynchronize");
+        (void)hipStreamDestroy(cap);
+    }
     if (rc) {
         delete p;
         return rc;

@@ -11,6 +11,9 @@
 // current plane is computed, so memory latency is paid once per chunk rather
 // than once per plane.  Raw buffer loads with per-plane descriptors and fixed
 // 32-bit in-plane offsets keep the address arithmetic out of the loop.
+#include <algorithm>
+#include <vector>
+
 #include "rsg_common.h"
 
 namespace rsg {
@@ -19,6 +22,16 @@ constexpr int B3X = 64, B3Y = 4, R3Y = 2, T3Y = B3Y * R3Y;
 
 struct K3D {
     int nx, ny, nz, lpml;
+    // Tile order.  ncx x ncy columns of B3X x T3Y cells, nzc z chunks.  With
+    // xgroups set the launch is one-dimensional: blocks are dealt round-robin
+    // to the 8 XCDs (XCD k runs blocks k, k+8, ...), and XCD k marches its own
+    // contiguous group of columns cg[k] .. cg[k+1]-1 (x fastest, then y)
+    // through the z chunks, columns fastest.  The blocks an XCD holds at once
+    // are then neighbours in x and y at the same depth: the cache lines their
+    // tile rows share (rows of a padded grid are rarely 128-byte aligned) and
+    // their y halos are fetched into that XCD's L2 once, not once per XCD.
+    int xgroups, ncx, ncy, nzc, cgmax;
+    int cg[9];
     float dt, invdx, invdy, invdz;
     const float *tab;
     float *P, *Vx, *Vy, *Vz;
@@ -118,6 +131,28 @@ struct Plane {
     }
 };
 
+// Tile (x0, y0, chunk) of this block; false for the surplus blocks of an XCD
+// whose column group is one column shorter.
+__device__ __forceinline__ bool tile3_of(const K3D &a, int &x0, int &y0, int &zchunk) {
+    int cx, cy;
+    if (!a.xgroups) {
+        cx = blockIdx.x;
+        cy = blockIdx.y;
+        zchunk = blockIdx.z;
+    } else {
+        const int k = blockIdx.x % 8, j = blockIdx.x / 8;
+        const int ncol = a.cg[k + 1] - a.cg[k];
+        if (ncol <= 0 || j >= ncol * a.nzc) return false;
+        zchunk = j / ncol;
+        const int col = a.cg[k] + (j - zchunk * ncol);
+        cy = col / a.ncx;
+        cx = col - cy * a.ncx;
+    }
+    x0 = cx * B3X;
+    y0 = cy * T3Y;
+    return true;
+}
+
 // Per-thread column set-up shared by both half steps: in-plane offsets of the
 // thread's R3Y cells (kOut outside the grid) and their PML positions and
 // edge stencil orders in x and y, fixed for the whole z march.
@@ -151,8 +186,9 @@ template <int ORDER>
 __global__ __launch_bounds__(B3X *B3Y) void a3d_velocity_kernel(const K3D a, int zc) {
     constexpr int H = ORDER, LD = B3X + 2 * H;
     __shared__ float sP[(T3Y + 2 * H) * LD];
-    const int x0 = blockIdx.x * B3X, y0 = blockIdx.y * T3Y;
-    const int zb = blockIdx.z * zc, ze = min(zb + zc, a.nz);
+    int x0, y0, zck;
+    if (!tile3_of(a, x0, y0, zck)) return;
+    const int zb = zck * zc, ze = min(zb + zc, a.nz);
     const int tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
     const size_t nxy = (size_t)a.nx * a.ny;
     const Tab3 tb{a.tab, a.lpml};
@@ -258,8 +294,9 @@ __global__ __launch_bounds__(B3X *B3Y) void a3d_stress_kernel(const K3D a, int z
     constexpr int H = ORDER, LDX = B3X + 2 * H, LDY = B3X;
     __shared__ float sVx[T3Y * LDX];
     __shared__ float sVy[(T3Y + 2 * H) * LDY];
-    const int x0 = blockIdx.x * B3X, y0 = blockIdx.y * T3Y;
-    const int zb = blockIdx.z * zc, ze = min(zb + zc, a.nz);
+    int x0, y0, zck;
+    if (!tile3_of(a, x0, y0, zck)) return;
+    const int zb = zck * zc, ze = min(zb + zc, a.nz);
     const int tx = threadIdx.x, ty = threadIdx.y, x = x0 + tx;
     const size_t nxy = (size_t)a.nx * a.ny;
     const Tab3 tb{a.tab, a.lpml};
@@ -386,7 +423,52 @@ static K3D make_k3d(const rsg_grid *g, const rsg_state *s) {
     a.cx = scaled_coefs(g->dx);
     a.cy = scaled_coefs(g->dy);
     a.cz = scaled_coefs(g->dz);
+    a.xgroups = 0;
+    a.ncx = (g->nx + B3X - 1) / B3X;
+    a.ncy = (g->ny + T3Y - 1) / T3Y;
+    a.nzc = a.cgmax = 0;
+    for (int k = 0; k < 9; k++) a.cg[k] = 0;
     return a;
+}
+
+// Launch grid of a 3D half step with z chunks of zc planes, and the tile
+// order it implies (K3D.xgroups, cg).
+static dim3 grid3(K3D &a, int zc) {
+    a.nzc = (a.nz + zc - 1) / zc;
+    if (!tuning().xcd3d) {
+        a.xgroups = 0;
+        return dim3(a.ncx, a.ncy, a.nzc);
+    }
+    a.xgroups = 1;
+    // groups of equal estimated cost: a cell in an x or y PML column counts
+    // pml_cost percent of an interior one (it updates memory variables on
+    // every plane), so the PML-heavy groups at the ends of the y range are
+    // shorter
+    const int ncol = a.ncx * a.ncy;
+    const long pc = tuning().pml_cost;
+    std::vector<long long> pre(ncol + 1, 0);
+    for (int c = 0; c < ncol; c++) {
+        const int x0 = (c % a.ncx) * B3X, y0 = (c / a.ncx) * T3Y;
+        const int x1 = std::min(x0 + B3X, a.nx), y1 = std::min(y0 + T3Y, a.ny);
+        long cost = 0;
+        for (int y = y0; y < y1; y++)
+            for (int x = x0; x < x1; x++) {
+                const bool pml = x < a.lpml || x >= a.nx - a.lpml || y < a.lpml || y >= a.ny - a.lpml;
+                cost += pml ? pc : 100;
+            }
+        pre[c + 1] = pre[c] + cost;
+    }
+    a.cg[0] = 0;
+    a.cgmax = 0;
+    int c = 0;
+    for (int k = 1; k <= 8; k++) {
+        const long long target = pre[ncol] * k / 8;
+        while (c < ncol && pre[c + 1] <= target) c++;
+        if (k == 8) c = ncol;
+        a.cg[k] = c;
+        a.cgmax = std::max(a.cgmax, a.cg[k] - a.cg[k - 1]);
+    }
+    return dim3((unsigned)(8L * a.cgmax * a.nzc), 1, 1);
 }
 
 // z-chunk length: short chunks give many blocks (the pipelined march keeps
@@ -420,9 +502,9 @@ using namespace rsg;
 extern "C" int rsg_a3d_forward_velocity(const rsg_grid *g, const rsg_state *s, void *stream) {
     if (int rc = check_grid3d(g, s)) return rc;
     if (!s->Rx || !s->Ry || !s->Rz) return fail(RSG_EINVAL, "a3d: null model");
-    const K3D a = make_k3d(g, s);
+    K3D a = make_k3d(g, s);
     const int zc = zchunk(g);
-    const dim3 grid((g->nx + B3X - 1) / B3X, (g->ny + T3Y - 1) / T3Y, (g->nz + zc - 1) / zc);
+    const dim3 grid = grid3(a, zc);
 #define L_(O) a3d_velocity_kernel<O><<<grid, dim3(B3X, B3Y), 0, (hipStream_t)stream>>>(a, zc)
     RSG3_ORDER_CASES(L_)
 #undef L_
@@ -433,9 +515,9 @@ extern "C" int rsg_a3d_forward_velocity(const rsg_grid *g, const rsg_state *s, v
 extern "C" int rsg_a3d_forward_stress(const rsg_grid *g, const rsg_state *s, void *stream) {
     if (int rc = check_grid3d(g, s)) return rc;
     if (!s->L) return fail(RSG_EINVAL, "a3d: null model");
-    const K3D a = make_k3d(g, s);
+    K3D a = make_k3d(g, s);
     const int zc = zchunk(g);
-    const dim3 grid((g->nx + B3X - 1) / B3X, (g->ny + T3Y - 1) / T3Y, (g->nz + zc - 1) / zc);
+    const dim3 grid = grid3(a, zc);
 #define L_(O) a3d_stress_kernel<O><<<grid, dim3(B3X, B3Y), 0, (hipStream_t)stream>>>(a, zc)
     RSG3_ORDER_CASES(L_)
 #undef L_

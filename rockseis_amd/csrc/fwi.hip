@@ -501,6 +501,8 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
         std::vector<size_t> soff(1, 0), sblkoff;
         short *cks = nullptr, *ckexp = nullptr;
         size_t *dsoff = nullptr, *dsblk = nullptr, cblk = 0;
+        int *dsseg = nullptr;
+        std::vector<int> sseg;
         if (store == kStoreS16) {
             const size_t lens[3] = {fw.n, fw.nstrip_x, fw.nstrip_z};
             const int narr[3] = {3, 4, 4};
@@ -514,6 +516,9 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
             if (int rc = db.alloc(&dsblk, sblkoff.size())) return rc;
             RSG_HIP(hipMemcpyAsync(dsoff, soff.data(), sizeof(size_t) * soff.size(), hipMemcpyHostToDevice, st));
             RSG_HIP(hipMemcpyAsync(dsblk, sblkoff.data(), sizeof(size_t) * sblkoff.size(), hipMemcpyHostToDevice, st));
+            sseg = s16_block_segments(sblkoff);
+            if (int rc = db.alloc(&dsseg, sseg.size())) return rc;
+            RSG_HIP(hipMemcpyAsync(dsseg, sseg.data(), sizeof(int) * sseg.size(), hipMemcpyHostToDevice, st));
         } else if (f16) {
             if (int rc = db.alloc(&ck16, fw.total * (size_t)rev.snaps)) return rc;
             if (int rc = db.alloc(&ckamax, (size_t)nseg * rev.snaps)) return rc;
@@ -523,8 +528,7 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
         const size_t xoff = 3 * fw.n * ns, zoff = xoff + 4 * fw.nstrip_x * ns;
         auto ck_pack = [&](int c) -> int {
             if (store == kStoreS16)
-                return pack_s16(fw.base, cks + cblk * kS16Block * c, ckexp + cblk * c, dsoff, dsblk,
-                                (int)soff.size() - 1, cblk, st);
+                return pack_s16(fw.base, cks + cblk * kS16Block * c, ckexp + cblk * c, dsoff, dsblk, dsseg, cblk, st);
             half_t *d = ck16 + fw.total * c;
             unsigned *am = ckamax + (size_t)nseg * c;
             if (int rc = pack16_segments(fw.base, d, am, fw.n, 3 * ns, st)) return rc;
@@ -534,8 +538,7 @@ static int fwi_a2d_impl(const rsg_grid *g, const rsg_fwi_a2d *f, int ns, hipStre
         };
         auto ck_unpack = [&](int c) -> int {
             if (store == kStoreS16)
-                return unpack_s16(cks + cblk * kS16Block * c, ckexp + cblk * c, fw.base, dsoff, dsblk,
-                                  (int)soff.size() - 1, cblk, st);
+                return unpack_s16(cks + cblk * kS16Block * c, ckexp + cblk * c, fw.base, dsoff, dsblk, dsseg, cblk, st);
             const half_t *d = ck16 + fw.total * c;
             const unsigned *am = ckamax + (size_t)nseg * c;
             if (int rc = unpack16_segments(d, am, fw.base, fw.n, 3 * ns, st)) return rc;

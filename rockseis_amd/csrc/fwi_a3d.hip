@@ -161,6 +161,7 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
     // nblk blocks of kS16Block values
     short *cks = nullptr, *ckexp = nullptr;
     size_t *dsblk = nullptr;
+    int *dsseg = nullptr;
     const std::vector<size_t> sblkoff = store == kStoreS16 ? s16_block_offsets(segs) : std::vector<size_t>();
     const size_t nblk = store == kStoreS16 ? sblkoff.back() : 0;
     if (store == kStoreS16) {
@@ -170,6 +171,10 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
         if (int rc = dm.alloc(&dsblk, sblkoff.size())) return rc;
         RSG_HIP(hipMemcpyAsync(dsegs, segs.data(), sizeof(size_t) * segs.size(), hipMemcpyHostToDevice, st));
         RSG_HIP(hipMemcpyAsync(dsblk, sblkoff.data(), sizeof(size_t) * sblkoff.size(), hipMemcpyHostToDevice, st));
+        const std::vector<int> sseg = s16_block_segments(sblkoff);
+        if (int rc = dm.alloc(&dsseg, sseg.size())) return rc;
+        RSG_HIP(hipMemcpyAsync(dsseg, sseg.data(), sizeof(int) * sseg.size(), hipMemcpyHostToDevice, st));
+        RSG_HIP(hipStreamSynchronize(st));  // sseg goes out of scope
     } else if (f16) {
         if (int rc = dm.alloc(&ck16, cstride * (size_t)rev.snaps)) return rc;
         if (int rc = dm.alloc(&ckamax, (size_t)nseg * rev.snaps)) return rc;
@@ -234,7 +239,7 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
         } else if (a == Revolve::TAKESHOT) {
             if (store == kStoreS16) {
                 if (int rc = pack_s16(fwb, cks + nblk * kS16Block * rev.check, ckexp + nblk * rev.check, dsegs, dsblk,
-                                      nseg, nblk, st))
+                                      dsseg, nblk, st))
                     return rc;
             } else if (f16) {
                 if (int rc = pack16_table(fwb, ck16 + cstride * rev.check, ckamax + (size_t)nseg * rev.check, ckpart,
@@ -246,7 +251,7 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
         } else if (a == Revolve::RESTORE) {
             if (store == kStoreS16) {
                 if (int rc = unpack_s16(cks + nblk * kS16Block * rev.check, ckexp + nblk * rev.check, fwb, dsegs, dsblk,
-                                        nseg, nblk, st))
+                                      dsseg, nblk, st))
                     return rc;
             } else if (f16) {
                 if (int rc = unpack16_table(ck16 + cstride * rev.check, ckamax + (size_t)nseg * rev.check, fwb, dsegs,

@@ -268,6 +268,7 @@ struct E2DFwi {
     // checkpoint is sblk blocks of kS16Block values and sblk exponents
     short *cks = nullptr, *ckexp = nullptr;
     size_t *dsblk = nullptr, sblk = 0;
+    int *dsseg = nullptr;
     std::unique_ptr<Revolve> rev;
     bool reverse = false;
     DevMem dm;
@@ -529,6 +530,10 @@ struct E2DFwi {
             if (int rc = dm.alloc(&dsblk, sblkoff.size())) return rc;
             RSG_HIP(hipMemcpyAsync(dsegs, segs.data(), sizeof(size_t) * segs.size(), hipMemcpyHostToDevice, st));
             RSG_HIP(hipMemcpyAsync(dsblk, sblkoff.data(), sizeof(size_t) * sblkoff.size(), hipMemcpyHostToDevice, st));
+            const std::vector<int> sseg = s16_block_segments(sblkoff);
+            if (int rc = dm.alloc(&dsseg, sseg.size())) return rc;
+            RSG_HIP(hipMemcpyAsync(dsseg, sseg.data(), sizeof(int) * sseg.size(), hipMemcpyHostToDevice, st));
+            RSG_HIP(hipStreamSynchronize(st));  // sseg goes out of scope
         } else if (f16) {
             if (int rc = dm.alloc(&dsegs, segs.size())) return rc;
             if (int rc = dm.alloc(&dchunks, chunks.size())) return rc;
@@ -614,7 +619,7 @@ struct E2DFwi {
 
     int takeshot(hipStream_t st) {
         if (store == kStoreS16)
-            return pack_s16(fwb, cks + sblk * kS16Block * rev->check, ckexp + sblk * rev->check, dsegs, dsblk, nseg,
+            return pack_s16(fwb, cks + sblk * kS16Block * rev->check, ckexp + sblk * rev->check, dsegs, dsblk, dsseg,
                             sblk, st);
         if (f16)
             return pack16_table(fwb, ck16 + cstride * rev->check, ckamax + (size_t)nseg * rev->check, ckpart, dsegs,
@@ -649,7 +654,7 @@ struct E2DFwi {
             } else if (a == Revolve::RESTORE) {
                 if (store == kStoreS16) {
                     if (int rc = unpack_s16(cks + sblk * kS16Block * rev->check, ckexp + sblk * rev->check, fwb,
-                                            dsegs, dsblk, nseg, sblk, st))
+                                            dsegs, dsblk, dsseg, sblk, st))
                         return rc;
                 } else if (f16) {
                     if (int rc = unpack16_table(ck16 + cstride * rev->check, ckamax + (size_t)nseg * rev->check, fwb,

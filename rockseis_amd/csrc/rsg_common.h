@@ -142,6 +142,7 @@ struct Tuning {
     int zc3d = 0;   // z planes per 3D block (0: automatic)
     int fuse_io = 1;  // modelling runs: source injection / velocity gathers in the stress kernel
     int pml_cost = 200;  // weight (percent of an interior tile) of a PML / edge tile in the XCD split
+    int xcd3d = 1;  // 3D tiles: 1 = each XCD marches its own contiguous group of (x, y) columns, 0 = grid order
 };
 Tuning &tuning();
 

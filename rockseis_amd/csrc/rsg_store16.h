@@ -72,17 +72,19 @@ int unpack16_table(const half_t *src, const unsigned *amax, float *dst, const si
 // by the end of one and the start of the next would round the smaller one
 // away.  The source is a flat buffer of nseg segments (segment s =
 // src[off[s] .. off[s+1])); segment s owns blocks blk[s] .. blk[s+1]-1
-// (s16_block_offsets), and block b's 256 values sit at q[256 b ..] (the
-// last block of a segment padded), its exponent at e[b].
+// (s16_block_offsets), bseg[b] is block b's segment (s16_block_segments),
+// and block b's 256 values sit at q[256 b ..] (the last block of a segment
+// padded), its exponent at e[b].
 constexpr int kS16Block = 256;
 constexpr short kS16Zero = -32768;
 inline size_t s16_blocks(size_t n) { return (n + kS16Block - 1) / kS16Block; }
 std::vector<size_t> s16_block_offsets(const std::vector<size_t> &off);
-// off, blk: device copies of the two offset tables (nseg + 1 entries each);
-// q holds blk[nseg] * kS16Block values, e blk[nseg] exponents.
-int pack_s16(const float *src, short *q, short *e, const size_t *off, const size_t *blk, int nseg, size_t nblk,
-             hipStream_t st);
-int unpack_s16(const short *q, const short *e, float *dst, const size_t *off, const size_t *blk, int nseg,
+std::vector<int> s16_block_segments(const std::vector<size_t> &blk);
+// off, blk, bseg: device copies of the tables (nseg + 1, nseg + 1 and nblk
+// entries); q holds nblk * kS16Block values, e nblk exponents.
+int pack_s16(const float *src, short *q, short *e, const size_t *off, const size_t *blk, const int *bseg,
+             size_t nblk, hipStream_t st);
+int unpack_s16(const short *q, const short *e, float *dst, const size_t *off, const size_t *blk, const int *bseg,
                size_t nblk, hipStream_t st);
 
 // Storage kinds of checkpoints (the fp16 member of the FWI settings):

@@ -148,6 +148,9 @@ int rsg_set_tuning(const char *key, int value) {
     } else if (k == "pml_cost") {
         if (value < 1 || value > 10000) return fail(RSG_EINVAL, "pml_cost must be 1..10000");
         tuning().pml_cost = value;
+    } else if (k == "xcd3d") {
+        if (value != 0 && value != 1) return fail(RSG_EINVAL, "xcd3d must be 0 or 1");
+        tuning().xcd3d = value;
     } else {
         return fail(RSG_EINVAL, "unknown tuning key " + k);
     }
@@ -167,6 +170,8 @@ int rsg_get_tuning(const char *key, int *value) {
         *value = tuning().fuse_io;
     else if (k == "pml_cost")
         *value = tuning().pml_cost;
+    else if (k == "xcd3d")
+        *value = tuning().xcd3d;
     else
         return fail(RSG_EINVAL, "unknown tuning key " + k);
     return RSG_OK;

@@ -288,18 +288,13 @@ dim3 seg_grid(size_t seglen, int nseg) {
     return dim3((unsigned)bx, (unsigned)nseg);
 }
 
-// Block b of a segmented S16 buffer: its segment (binary search of the
-// block offsets), first source value and length.
-__device__ inline void s16_block(const size_t *__restrict__ off, const size_t *__restrict__ blk, int nseg, size_t b,
-                                 size_t &first, int &len) {
-    int lo = 0, hi = nseg - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (blk[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    first = off[lo] + (b - blk[lo]) * kS16Block;
-    const size_t rest = off[lo + 1] - first;
+// Block b of a segmented S16 buffer: its segment (from the per-block table),
+// first source value and length.
+__device__ __forceinline__ void s16_block(const size_t *__restrict__ off, const size_t *__restrict__ blk,
+                                          const int *__restrict__ bseg, size_t b, size_t &first, int &len) {
+    const int sg = bseg[b];
+    first = off[sg] + (b - blk[sg]) * kS16Block;
+    const size_t rest = off[sg + 1] - first;
     len = rest < (size_t)kS16Block ? (int)rest : kS16Block;
 }
 
@@ -307,13 +302,14 @@ __device__ inline void s16_block(const size_t *__restrict__ off, const size_t *_
 // so each load and store instruction covers 64 consecutive values.
 __global__ __launch_bounds__(kThreads) void pack_s16_kernel(const float *__restrict__ src, short *__restrict__ q,
                                                              short *__restrict__ ex, const size_t *__restrict__ off,
-                                                             const size_t *__restrict__ blk, int nseg, size_t nblk) {
+                                                             const size_t *__restrict__ blk,
+                                                             const int *__restrict__ bseg, size_t nblk) {
     const size_t b = (size_t)blockIdx.x * kS16PerWg + (threadIdx.x >> 6);
     if (b >= nblk) return;  // whole waves leave together
     const int lane = threadIdx.x & 63;
     size_t first;
     int len;
-    s16_block(off, blk, nseg, b, first, len);
+    s16_block(off, blk, bseg, b, first, len);
     float v[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) v[j] = lane + 64 * j < len ? src[first + lane + 64 * j] : 0.f;
@@ -333,13 +329,14 @@ __global__ __launch_bounds__(kThreads) void pack_s16_kernel(const float *__restr
 __global__ __launch_bounds__(kThreads) void unpack_s16_kernel(const short *__restrict__ q,
                                                                const short *__restrict__ ex, float *__restrict__ dst,
                                                                const size_t *__restrict__ off,
-                                                               const size_t *__restrict__ blk, int nseg, size_t nblk) {
+                                                               const size_t *__restrict__ blk,
+                                                               const int *__restrict__ bseg, size_t nblk) {
     const size_t b = (size_t)blockIdx.x * kS16PerWg + (threadIdx.x >> 6);
     if (b >= nblk) return;
     const int lane = threadIdx.x & 63;
     size_t first;
     int len;
-    s16_block(off, blk, nseg, b, first, len);
+    s16_block(off, blk, bseg, b, first, len);
     const short e = ex[b];
     const int sh = e == kS16Zero ? 0 : e - 15;
 #pragma unroll
@@ -356,22 +353,27 @@ std::vector<size_t> s16_block_offsets(const std::vector<size_t> &off) {
     return blk;
 }
 
-int pack_s16(const float *src, short *q, short *e, const size_t *off, const size_t *blk, int nseg, size_t nblk,
-             hipStream_t st) {
+std::vector<int> s16_block_segments(const std::vector<size_t> &blk) {
+    std::vector<int> seg(blk.empty() ? 0 : blk.back());
+    for (size_t s = 0; s + 1 < blk.size(); s++)
+        for (size_t b = blk[s]; b < blk[s + 1]; b++) seg[b] = (int)s;
+    return seg;
+}
+
+int pack_s16(const float *src, short *q, short *e, const size_t *off, const size_t *blk, const int *bseg,
+             size_t nblk, hipStream_t st) {
     if (nblk == 0) return RSG_OK;
-    if (nseg < 1) return fail(RSG_EINVAL, "pack_s16: empty segment table");
-    pack_s16_kernel<<<(unsigned)((nblk + kS16PerWg - 1) / kS16PerWg), kThreads, 0, st>>>(src, q, e, off, blk, nseg,
+    pack_s16_kernel<<<(unsigned)((nblk + kS16PerWg - 1) / kS16PerWg), kThreads, 0, st>>>(src, q, e, off, blk, bseg,
                                                                                           nblk);
     RSG_LAUNCH_CHECK("pack_s16_kernel");
     return RSG_OK;
 }
 
-int unpack_s16(const short *q, const short *e, float *dst, const size_t *off, const size_t *blk, int nseg,
+int unpack_s16(const short *q, const short *e, float *dst, const size_t *off, const size_t *blk, const int *bseg,
                size_t nblk, hipStream_t st) {
     if (nblk == 0) return RSG_OK;
-    if (nseg < 1) return fail(RSG_EINVAL, "unpack_s16: empty segment table");
     unpack_s16_kernel<<<(unsigned)((nblk + kS16PerWg - 1) / kS16PerWg), kThreads, 0, st>>>(q, e, dst, off, blk,
-                                                                                            nseg, nblk);
+                                                                                            bseg, nblk);
     RSG_LAUNCH_CHECK("unpack_s16_kernel");
     return RSG_OK;
 }

@@ -87,19 +87,24 @@ def test_pml_cost_values_give_identical_fields():
 
 
 def test_zc3d_values_give_identical_fields():
+    """Every z chunk length and both 3D tile orders (grid order, XCD column
+    groups) give bit-identical records and fields."""
     from rockseis_amd import _lib as R, core
-    old = R.get_tuning("zc3d")
+    old = R.get_tuning("zc3d"), R.get_tuning("xcd3d")
     try:
         out = {}
-        for v in (0, 3, 16, 64):
-            R.set_tuning("zc3d", v)
-            out[v] = _a3d(core)
-        for v in (3, 16, 64):
-            assert np.array_equal(out[v][0], out[0][0]), v
-            assert np.array_equal(out[v][1], out[0][1]), v
-        assert np.abs(out[0][0]).max() > 0
+        for xcd in (0, 1):
+            R.set_tuning("xcd3d", xcd)
+            for v in (0, 3, 16, 64):
+                R.set_tuning("zc3d", v)
+                out[(xcd, v)] = _a3d(core)
+        for key in out:
+            assert np.array_equal(out[key][0], out[(0, 0)][0]), key
+            assert np.array_equal(out[key][1], out[(0, 0)][1]), key
+        assert np.abs(out[(0, 0)][0]).max() > 0
     finally:
-        R.set_tuning("zc3d", old)
+        R.set_tuning("zc3d", old[0])
+        R.set_tuning("xcd3d", old[1])
 
 
 @pytest.mark.parametrize("kind", ["a2d", "e2d", "a3d", "a2d_batch"])

@@ -318,3 +318,33 @@ def test_rtm_a3d_image(golden, case):
     img = rtm.rtm_a3d(g["vp"], g["rho"], int(lpml), (dx, dx, dx), dt, int(nt), order, fs, ncheck, g["scoords"],
                       g["wav"], g["gcoords"], g["dobs"], nhx, nhy, nhz)
     assert np.array_equal(img, f["image"])
+
+
+@pytest.mark.skipif(not __import__("oracle").ref_available(), reason="reference build absent")
+def test_reference_window_timers_run():
+    """The CPU-baseline timers (ref_a2d_time_window, ref_e2d_time_window:
+    the compiled reference stepping from a given state) run on a small grid
+    from a state a reference run produced and return a positive time; two
+    threads at once also work (the shot-parallel figure)."""
+    import concurrent.futures as cf
+    from oracle import runs
+    rng = np.random.default_rng(3)
+    nx, nz, lpml, dx, dt, nt = 40, 30, 8, 10.0, 1e-3, 60
+    vp = (2000.0 + 200.0 * rng.random((nz, nx))).astype(np.float32)
+    rho = (2000.0 * np.ones((nz, nx))).astype(np.float32)
+    vs = (vp / 1.8).astype(np.float32)
+    sc = np.float32([[200.0, 20.0]])
+    wav = rng.standard_normal((1, nt)).astype(np.float32)
+    gc = np.stack([np.arange(nx) * dx, np.full(nx, 20.0)], 1).astype(np.float32)
+    nxp, nzp = nx + 2 * lpml, nz + 2 * lpml
+    e = runs.ref_e2d_run(vp, vs, rho, lpml, dx, dx, dt, 20, 4, False, "P", sc, wav[:, :20], "VX", gc)
+    strips = np.zeros(8 * lpml * nzp + 8 * nxp * lpml, np.float32)
+    st = np.concatenate([e[k].ravel() for k in ("Vx", "Vz", "Sxx", "Szz", "Sxz")] + [strips])
+    with cf.ThreadPoolExecutor(2) as ex:
+        ts = list(ex.map(lambda _: runs.ref_e2d_time_window(vp, vs, rho, lpml, dx, dx, dt, nt, 4, False, sc, wav,
+                                                            gc, 20, 10, st), range(2)))
+    assert all(t > 0 for t in ts)
+    a = runs.ref_a2d_run(vp, rho, lpml, dx, dx, dt, 20, 4, False, "P", sc, wav[:, :20], "P", gc)
+    st2 = np.concatenate([a[k].ravel() for k in ("P", "Vx", "Vz")] +
+                         [np.zeros(4 * lpml * nzp + 4 * nxp * lpml, np.float32)])
+    assert runs.ref_a2d_time_window(vp, rho, lpml, dx, dx, dt, nt, 4, sc, wav, gc, 20, 10, st2) > 0
