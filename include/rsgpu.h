@@ -76,6 +76,22 @@ enum {
 };
 #define RSG_MAX_STRIPS 16
 
+/* PML memory strip slots of rsg_state_e3d.psi[] (WavesElastic3D /
+ * PmlElastic3D, pml.h:197-234): x faces (lpml*ny*nz each, [z][y][i]), y
+ * faces (nx*lpml*nz, [z][i][x]), z faces (nx*ny*lpml, [i][y][x]). */
+enum {
+    RSG_E3D_SXX_LEFT = 0, RSG_E3D_SXX_RIGHT, RSG_E3D_SXZX_LEFT, RSG_E3D_SXZX_RIGHT,
+    RSG_E3D_SXYX_LEFT, RSG_E3D_SXYX_RIGHT, RSG_E3D_VXX_LEFT, RSG_E3D_VXX_RIGHT,
+    RSG_E3D_VZX_LEFT, RSG_E3D_VZX_RIGHT, RSG_E3D_VYX_LEFT, RSG_E3D_VYX_RIGHT,
+    RSG_E3D_SYY_FRONT, RSG_E3D_SYY_BACK, RSG_E3D_SXYY_FRONT, RSG_E3D_SXYY_BACK,
+    RSG_E3D_SYZY_FRONT, RSG_E3D_SYZY_BACK, RSG_E3D_VYY_FRONT, RSG_E3D_VYY_BACK,
+    RSG_E3D_VXY_FRONT, RSG_E3D_VXY_BACK, RSG_E3D_VZY_FRONT, RSG_E3D_VZY_BACK,
+    RSG_E3D_SZZ_TOP, RSG_E3D_SZZ_BOTTOM, RSG_E3D_SXZZ_TOP, RSG_E3D_SXZZ_BOTTOM,
+    RSG_E3D_SYZZ_TOP, RSG_E3D_SYZZ_BOTTOM, RSG_E3D_VZZ_TOP, RSG_E3D_VZZ_BOTTOM,
+    RSG_E3D_VXZ_TOP, RSG_E3D_VXZ_BOTTOM, RSG_E3D_VYZ_TOP, RSG_E3D_VYZ_BOTTOM,
+    RSG_E3D_NSTRIP
+};
+
 /* Device pointers of one propagator's state (a Waves<T> object plus the
  * staggered model arrays it reads).  Members a kind does not use are NULL. */
 typedef struct {
@@ -273,6 +289,32 @@ int rsg_modelling_plan_create(int kind, const rsg_grid *g, const rsg_state *s, c
 int rsg_modelling_plan_run(rsg_modelling_plan *p, int it0, int nsteps, void *stream);
 int rsg_modelling_plan_destroy(rsg_modelling_plan *p);
 
+/* ---- WavesElastic3D (waves.cpp:2310-4403): the nine fields, the
+ * staggered model of ModelElastic3D::staggerModels (model.cpp:3371-3473)
+ * and the 36 strips; padded grids [z][y][x]. */
+typedef struct {
+    float *Vx, *Vy, *Vz, *Sxx, *Syy, *Szz, *Syz, *Sxz, *Sxy;
+    const float *L, *L2M, *Mxz, *Myz, *Mxy, *Rx, *Ry, *Rz;
+    const float *tab;                 /* PML tables as in rsg_state              */
+    float *psi[RSG_E3D_NSTRIP];
+} rsg_state_e3d;
+/* ModelElastic3D::staggerModels on the host, bit-exact: L, L2M, M_xz, M_yz,
+ * M_xy, Rx, Ry, Rz of the (nx+2lpml)(ny+2lpml)(nz+2lpml) padded grid. */
+int rsg_stagger_e3d(const float *vp, const float *vs, const float *rho, int nx, int ny, int nz, int lpml, int fs,
+                    float *L, float *L2M, float *Mxz, float *Myz, float *Mxy, float *Rx, float *Ry, float *Rz);
+/* forwardstepVelocity / forwardstepStress (waves.cpp:2465-3258); g: padded
+ * sizes (2 lpml < n, nz <= 65535), fs zeroes Szz on plane lpml. */
+int rsg_e3d_forward_velocity(const rsg_grid *g, const rsg_state_e3d *s, void *stream);
+int rsg_e3d_forward_stress(const rsg_grid *g, const rsg_state_e3d *s, void *stream);
+/* insertSource (waves.cpp:4104-4246): PRESSURE explosive into Sxx, Syy,
+ * Szz (-dt wav w), VX / VY / VZ with dt R; traces add with atomics. */
+int rsg_e3d_insert_source(const rsg_grid *g, const rsg_state_e3d *s, const rsg_traces *src, int it, void *stream);
+/* recordData (waves.cpp:4247-4403): VX / VY / VZ gathers, or PRESSURE
+ * (Sxx + Syy + Szz) / 3 divided by (3 rho vp^2 - 4 rho vs^2) / 3 at the
+ * trace's cell of the unpadded model Vp, Vs, Rho (device, [nz][ny][nx]). */
+int rsg_e3d_record_data(const rsg_grid *g, const rsg_state_e3d *s, const rsg_traces *rec, const float *Vp,
+                        const float *Vs, const float *Rho, int it, void *stream);
+
 /* ---- storage of FWI snapshots and Revolve checkpoints (the fp16 member of
  * the FWI settings below; the reference keeps them as T = float, revolve.cpp
  * 648-1271): stencil arithmetic stays FP32 in every case.
@@ -415,6 +457,12 @@ typedef struct {
                                             (FwiAcoustic3D::computeResiduals,
                                             fwi.cpp:1651-1853)                        */
     const float *weight;                 /* data weights like obs, or NULL           */
+    int adjoint;                         /* RSG_ADJOINT_REFERENCE (0) or RSG_ADJOINT_EXACT
+                                            (no reference counterpart: the exact discrete
+                                            adjoint of the forward step and of
+                                            ModelAcoustic3D::staggerModels, the gradient
+                                            of the DIFFERENCE misfit; misfit_type 0 or 4,
+                                            models padded by the library)              */
 } rsg_fwi_a3d;
 
 int rsg_fwi_a3d_gradient(const rsg_grid *g, const rsg_fwi_a3d *f, void *stream);
@@ -503,6 +551,14 @@ typedef struct {
                           4 ADAPTIVE_SINGLE (rs_fwimisfit, utils.h:26); the 97th-percentile
                           clips of types 1-3 are per shot, as the reference runs one
                           shot at a time; types 2-4 leave the P residual at zero      */
+    int adjoint;       /* RSG_ADJOINT_REFERENCE (0): the reference's adjoint-state steps
+                          and crossCorr.  RSG_ADJOINT_EXACT (no reference counterpart):
+                          the exact discrete adjoint of the forward step and of
+                          ModelElastic2D::staggerModels, the gradient of the DIFFERENCE
+                          misfit of the discrete forward map (misfit_type 0 and windows
+                          padded by the library, no domain models); scale 1 gives the
+                          Vp / Vs / rho gradients, 0 those of lambda / mu / rho as
+                          independent parameters; wavgrad = dJ/d(wavelet)            */
 } rsg_fwi_e2d;
 
 int rsg_fwi_e2d_gradient_batch(const rsg_fwi_e2d *f, const rsg_fwi_e2d_shot *shots, int nshot,
@@ -545,6 +601,59 @@ typedef struct {
 
 int rsg_misfit_residuals_e2d(int type, const rsg_e2d_records *shots, int nshot, int nt, double *misfit,
                              void *stream);
+
+/* ---- RSS files (rockseis::File, file.cpp:184-260; data.cpp:463-634) into
+ * and out of device memory: one read / write of the whole file through
+ * pinned host memory and one strided copy, where the reference's
+ * Data2D::read / write go trace by trace.  The layout is the reference's:
+ * "R0CKS", int data_format, header_format, type, size_t Nheader, Ndims,
+ * 9 size_t n, 9 double d, 9 double o (start of data at byte 249), then the
+ * data; DATA2D / DATA3D traces carry 4 (sx, sz, gx, gz) / 6 (sx, sy, sz,
+ * gx, gy, gz) header floats before their nt samples.  Errors: RSG_EINVAL
+ * for a missing, truncated or mismatching file. */
+#define RSG_RSS_GENERIC 0
+#define RSG_RSS_REGULAR 1
+#define RSG_RSS_DATA2D 2
+#define RSG_RSS_DATA3D 3
+typedef struct {
+    int data_format, header_format, type;  /* bytes per value / header value, rs_datatype */
+    size_t nheader, ndims;
+    size_t n[9];
+    double d[9], o[9];
+    size_t start;                          /* byte offset of the data                     */
+} rsg_rss_header;
+int rsg_rss_read_header(const char *path, rsg_rss_header *h);
+/* coords (host, [ntrace][4] or [ntrace][6], may be NULL), data (device,
+ * [ntrace][nt]); ntrace and nt must equal the file's n2 and n1. */
+int rsg_rss_read_data2d(const char *path, float *coords, float *data, int ntrace, int nt, void *stream);
+int rsg_rss_read_data3d(const char *path, float *coords, float *data, int ntrace, int nt, void *stream);
+/* Data2D<float>::write / Data3D<float>::write: dt and ot are the float
+ * members the reference widens to the header's doubles (the 3D file records
+ * o1 = 0, as the reference's does). */
+int rsg_rss_write_data2d(const char *path, const float *coords, const float *data, int ntrace, int nt, float dt,
+                         float ot, void *stream);
+int rsg_rss_write_data3d(const char *path, const float *coords, const float *data, int ntrace, int nt, float dt,
+                         void *stream);
+/* A REGULAR float array (models, images, Nheader 0) of n values (the product
+ * of the file's non-zero sizes) into device memory, and back out with the
+ * header h (data_format 4, nheader 0). */
+int rsg_rss_read_regular(const char *path, float *data, size_t n, void *stream);
+int rsg_rss_write_regular(const char *path, const rsg_rss_header *h, const float *data, void *stream);
+
+/* ---- in-memory image stacking: Image2D::stackImage_parallel
+ * (image.cpp:533-600) with both images in device memory.  Each local cell
+ * (ix, iz) with padlx <= ix < nx-padhx, padlz <= iz < nz-padhz, for every
+ * subsurface offset, is added to the global cell (ix + ix0, iz + iz0) when
+ * that lies inside the global image; ix0 = rsg_image_offset(l->ox, l->dx,
+ * g->ox, g->dx), iz0 likewise.  Images are [nhz][nhx][nz][nx]; they must
+ * share nhx, nhz, dx and dz (RSG_EINVAL otherwise).  Stacking the shots one
+ * call after another adds in the reference's order. */
+typedef struct {
+    int nx, nz, nhx, nhz;
+    float dx, dz, ox, oz;
+} rsg_image2d;
+int rsg_stack_image_2d(float *gimg, const rsg_image2d *g, const float *limg, const rsg_image2d *l, int padlx,
+                       int padhx, int padlz, int padhz, void *stream);
 
 /* Model<T>::getLocalsize2d (model.cpp:271-340): the aperture window of a
  * gather.  coords [ntrace][4] hold sx, sz, gx, gz per trace; map 0 (SMAP)

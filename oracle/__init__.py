@@ -95,7 +95,14 @@ def _make_structs(real):
                                    "Vxx_left", "Vxx_right", "Vyy_front", "Vyy_back",
                                    "Vzz_top", "Vzz_bottom", "L", "Rx", "Ry", "Rz", "df"], ct))
 
-    return A2D, E2D, A3D
+    class E3D(C.Structure):
+        _fields_ = ([("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("lpml", C.c_int), ("order", C.c_int),
+                     ("fs", C.c_int), ("dx", ct), ("dy", ct), ("dz", ct), ("dt", ct)]
+                    + _ptr_fields(["tab", "Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy"], ct)
+                    + [("psi", C.POINTER(ct) * 36)]
+                    + _ptr_fields(["L", "L2M", "Mxz", "Myz", "Mxy", "Rx", "Ry", "Rz", "df"], ct))
+
+    return A2D, E2D, A3D, E3D
 
 
 STRUCTS = {np.float32: _make_structs(np.float32), np.float64: _make_structs(np.float64)}
@@ -106,7 +113,7 @@ def _declare(L):
     for real, suf in _SUF.items():
         rp = _f32p if real == np.float32 else _f64p
         ct = C.c_float if real == np.float32 else C.c_double
-        A2D, E2D, A3D = STRUCTS[real]
+        A2D, E2D, A3D, E3D = STRUCTS[real]
         f = getattr(L, "orc_der_coeffs" + suf)
         f.argtypes = [rp]
         f = getattr(L, "orc_deriv" + suf)
@@ -147,6 +154,25 @@ def _declare(L):
         f.argtypes = [C.POINTER(A2D), C.POINTER(A2D), rp]
         f = getattr(L, "orc_a2d_adj_velocity" + suf)
         f.argtypes = [C.POINTER(A2D), C.POINTER(A2D), rp, rp]
+        f = getattr(L, "orc_e2d_adj_stress" + suf)
+        f.argtypes = [C.POINTER(E2D), C.POINTER(E2D), rp, rp, rp]
+        f = getattr(L, "orc_e2d_adj_velocity" + suf)
+        f.argtypes = [C.POINTER(E2D), C.POINTER(E2D), rp, rp]
+        for nm in ("e3d_velocity", "e3d_stress"):
+            f = getattr(L, "orc_" + nm + suf)
+            f.argtypes = [C.POINTER(E3D)]
+        f = getattr(L, "orc_inject3d_explosive" + suf)
+        f.argtypes = [rp, rp, rp, C.c_int, C.c_int, C.c_int, ct, _i32p, rp, rp, C.c_int, C.c_int, C.c_int]
+        f = getattr(L, "orc_record3d_pressure_el" + suf)
+        f.argtypes = [rp, rp, rp, C.c_int, C.c_int, C.c_int, rp, rp, rp, C.c_int, C.c_int, _i32p, rp, rp, C.c_int,
+                      C.c_int, C.c_int]
+        f = getattr(L, "orc_stagger_e3d" + suf)
+        f.argtypes = [rp, rp, rp] + [C.c_int] * 5 + [rp] * 8
+        f.restype = C.c_int
+        f = getattr(L, "orc_a3d_adj_stress" + suf)
+        f.argtypes = [C.POINTER(A3D), C.POINTER(A3D), rp]
+        f = getattr(L, "orc_a3d_adj_velocity" + suf)
+        f.argtypes = [C.POINTER(A3D), C.POINTER(A3D), rp, rp, rp]
         f = getattr(L, "orc_a2d_crosscorr" + suf)
         f.argtypes = [rp, C.c_int, rp, rp, C.c_int, rp, rp, rp, rp, C.c_int, C.c_int, ct, ct, rp, rp]
         f = getattr(L, "orc_misfit_a" + suf)
@@ -185,6 +211,9 @@ def _declare(L):
 
 
 def _declare_ref(R):
+    R.ref_e3d_run.argtypes = ([_f32p, _f32p, _f32p] + [C.c_int] * 4 + [C.c_float] * 4 + [C.c_int] * 5
+                              + [_f32p, _f32p, C.c_int, C.c_int, _f32p, _f32p, _f32p])
+    R.ref_stagger_e3d.argtypes = [_f32p, _f32p, _f32p] + [C.c_int] * 5 + [_f32p]
     R.ref_der.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                           C.c_int, C.c_int]
     R.ref_pml_abc.argtypes = [C.c_int, C.c_float, C.c_float, C.c_int, _f32p]

@@ -389,6 +389,72 @@ void ref_a3d_run(const float *vp, const float *rho, int nx, int ny, int nz, int 
     for (int j = 0; j < 4; j++) std::memcpy(fields + np * j, f[j], sizeof(float) * np);
 }
 
+// WavesElastic3D modelling (Modelling loop: forwardstepVelocity,
+// forwardstepStress, insertSource, recordData) on a ModelElastic3D; fields
+// out in the order Vx, Vy, Vz, Sxx, Syy, Szz, Syz, Sxz, Sxy (padded grids).
+void ref_e3d_run(const float *vp, const float *vs, const float *rho, int nx, int ny, int nz, int lpml, float dx,
+                 float dy, float dz, float dt, int nt, int order, int fs, int srcfield, int nsrc,
+                 const float *scoords, const float *wav, int recfield, int nrec, const float *gcoords, float *rec,
+                 float *fields) {
+    auto model = std::make_shared<ModelElastic3D<float>>(nx, ny, nz, lpml, dx, dy, dz, 0.f, 0.f, 0.f, fs != 0);
+    model->createModel();
+    std::memcpy(model->getVp(), vp, sizeof(float) * nx * ny * nz);
+    std::memcpy(model->getVs(), vs, sizeof(float) * nx * ny * nz);
+    std::memcpy(model->getR(), rho, sizeof(float) * nx * ny * nz);
+    auto src = std::make_shared<Data3D<float>>(nsrc, nt, dt, 0.f);
+    src->setField(field_of(srcfield));
+    Point3D<float> *sc = src->getGeom()->getScoords();
+    for (int i = 0; i < nsrc; i++) {
+        sc[i].x = scoords[3 * i];
+        sc[i].y = scoords[3 * i + 1];
+        sc[i].z = scoords[3 * i + 2];
+    }
+    std::memcpy(src->getData(), wav, sizeof(float) * nsrc * nt);
+    src->makeMap(model->getGeom(), SMAP);
+    auto data = std::make_shared<Data3D<float>>(nrec, nt, dt, 0.f);
+    data->setField(field_of(recfield));
+    Point3D<float> *gc = data->getGeom()->getGcoords();
+    for (int i = 0; i < nrec; i++) {
+        gc[i].x = gcoords[3 * i];
+        gc[i].y = gcoords[3 * i + 1];
+        gc[i].z = gcoords[3 * i + 2];
+    }
+    data->makeMap(model->getGeom(), GMAP);
+    model->staggerModels();
+    auto waves = std::make_shared<WavesElastic3D<float>>(model, nt, dt, 0.f);
+    auto der = std::make_shared<Der<float>>(waves->getNx_pml(), waves->getNy_pml(), waves->getNz_pml(),
+                                            waves->getDx(), waves->getDy(), waves->getDz(), order);
+    waves->getPml()->setSmax(SMAX);
+    waves->getPml()->computeABC();
+    for (int it = 0; it < nt; it++) {
+        waves->forwardstepVelocity(model, der);
+        waves->forwardstepStress(model, der);
+        waves->insertSource(model, src, SMAP, it);
+        waves->recordData(model, data, GMAP, it);
+    }
+    size_t np = (size_t)waves->getNx_pml() * waves->getNy_pml() * waves->getNz_pml();
+    std::memcpy(rec, data->getData(), sizeof(float) * nrec * nt);
+    float *f[9] = {waves->getVx(),  waves->getVy(),  waves->getVz(),  waves->getSxx(), waves->getSyy(),
+                   waves->getSzz(), waves->getSyz(), waves->getSxz(), waves->getSxy()};
+    for (int j = 0; j < 9; j++) std::memcpy(fields + np * j, f[j], sizeof(float) * np);
+}
+
+// ModelElastic3D::staggerModels: out in the order L, L2M, M_xz, M_yz, M_xy,
+// Rx, Ry, Rz (padded grids).
+void ref_stagger_e3d(const float *vp, const float *vs, const float *rho, int nx, int ny, int nz, int lpml, int fs,
+                     float *out) {
+    auto model = std::make_shared<ModelElastic3D<float>>(nx, ny, nz, lpml, 10.f, 10.f, 10.f, 0.f, 0.f, 0.f, fs != 0);
+    model->createModel();
+    std::memcpy(model->getVp(), vp, sizeof(float) * nx * ny * nz);
+    std::memcpy(model->getVs(), vs, sizeof(float) * nx * ny * nz);
+    std::memcpy(model->getR(), rho, sizeof(float) * nx * ny * nz);
+    model->staggerModels();
+    const size_t np = (size_t)(nx + 2 * lpml) * (ny + 2 * lpml) * (nz + 2 * lpml);
+    float *f[8] = {model->getL(), model->getL2M(), model->getM_xz(), model->getM_yz(), model->getM_xy(),
+                   model->getRx(), model->getRy(),  model->getRz()};
+    for (int j = 0; j < 8; j++) std::memcpy(out + np * j, f[j], sizeof(float) * np);
+}
+
 // FwiAcoustic2D::run_optimal on a staggered (non-domain) model: the
 // reference's gradient of one shot (fwi.cpp:1159-1332) with ncheck in-core
 // Revolve checkpoints and the DIFFERENCE misfit.  Outputs: vpgrad, rhograd
@@ -723,6 +789,90 @@ int ref_domain_model(const char *vpfile, const char *vsfile, const char *rhofile
 int ref_image_offset(float ol, float dl, float og, float dg) {
     float a = ol / dl, b = og / dg;
     return (int)(a - b);
+}
+
+// Data2D<float>::write / Data3D<float>::write of ntrace traces (coords per
+// trace: sx, sz, gx, gz, or sx, sy, sz, gx, gy, gz); returns 0 on success.
+int ref_rss_write_data(const char *path, int dim3, const float *coords, const float *data, int ntrace, int nt,
+                       float dt, float ot) {
+    if (dim3) {
+        auto d = std::make_shared<Data3D<float>>(ntrace, nt, dt, ot);
+        d->setFile(path);
+        Point3D<float> *s = d->getGeom()->getScoords(), *g = d->getGeom()->getGcoords();
+        for (int i = 0; i < ntrace; i++) {
+            s[i].x = coords[6 * i];
+            s[i].y = coords[6 * i + 1];
+            s[i].z = coords[6 * i + 2];
+            g[i].x = coords[6 * i + 3];
+            g[i].y = coords[6 * i + 4];
+            g[i].z = coords[6 * i + 5];
+        }
+        std::memcpy(d->getData(), data, sizeof(float) * ntrace * nt);
+        return d->write() == FILE_OK ? 0 : 1;
+    }
+    auto d = std::make_shared<Data2D<float>>(ntrace, nt, dt, ot);
+    d->setFile(path);
+    Point2D<float> *s = d->getGeom()->getScoords(), *g = d->getGeom()->getGcoords();
+    for (int i = 0; i < ntrace; i++) {
+        s[i].x = coords[4 * i];
+        s[i].y = coords[4 * i + 1];
+        g[i].x = coords[4 * i + 2];
+        g[i].y = coords[4 * i + 3];
+    }
+    std::memcpy(d->getData(), data, sizeof(float) * ntrace * nt);
+    return d->write() == FILE_OK ? 0 : 1;
+}
+
+// Data2D<float>::read / Data3D<float>::read into coords and data.
+int ref_rss_read_data(const char *path, int dim3, float *coords, float *data, int ntrace, int nt) {
+    if (dim3) {
+        auto d = std::make_shared<Data3D<float>>(ntrace, nt, 1.0f, 0.0f);
+        d->setFile(path);
+        if (d->read() != FILE_OK) return 1;
+        Point3D<float> *s = d->getGeom()->getScoords(), *g = d->getGeom()->getGcoords();
+        for (int i = 0; i < ntrace; i++) {
+            const float v[6] = {s[i].x, s[i].y, s[i].z, g[i].x, g[i].y, g[i].z};
+            std::memcpy(coords + 6 * i, v, sizeof(v));
+        }
+        std::memcpy(data, d->getData(), sizeof(float) * ntrace * nt);
+        return 0;
+    }
+    auto d = std::make_shared<Data2D<float>>(ntrace, nt, 1.0f, 0.0f);
+    d->setFile(path);
+    if (d->read() != FILE_OK) return 1;
+    Point2D<float> *s = d->getGeom()->getScoords(), *g = d->getGeom()->getGcoords();
+    for (int i = 0; i < ntrace; i++) {
+        const float v[4] = {s[i].x, s[i].y, g[i].x, g[i].y};
+        std::memcpy(coords + 4 * i, v, sizeof(v));
+    }
+    std::memcpy(data, d->getData(), sizeof(float) * ntrace * nt);
+    return 0;
+}
+
+// The inversion driver's file stacking: Image2D::createEmpty of the global
+// image at path, then Image2D::stackImage_parallel of each local image
+// (images [nhz][nhx][nz][nx], concatenated in `locals`), then
+// Image2D::read of the result into out.
+int ref_stack_images_2d(const char *path, int nxg, int nzg, int nhx, int nhz, float dx, float dz, float oxg, float ozg,
+                        int nloc, const int *nxl, const int *nzl, const float *oxl, const float *ozl,
+                        const float *locals, int padlx, int padhx, int padlz, int padhz, float *out) {
+    auto g = std::make_shared<Image2D<float>>(nxg, nzg, nhx, nhz, dx, dz, oxg, ozg);
+    g->setImagefile(path);
+    if (g->createEmpty() != FILE_OK) return 1;
+    size_t off = 0;
+    for (int k = 0; k < nloc; k++) {
+        auto l = std::make_shared<Image2D<float>>(nxl[k], nzl[k], nhx, nhz, dx, dz, oxl[k], ozl[k]);
+        l->allocateImage();
+        const size_t n = (size_t)nxl[k] * nzl[k] * nhx * nhz;
+        std::memcpy(l->getImagedata(), locals + off, sizeof(float) * n);
+        off += n;
+        if (l->stackImage_parallel(path, padlx, padhx, padlz, padhz) != FILE_OK) return 1;
+    }
+    auto r = std::make_shared<Image2D<float>>(nxg, nzg, nhx, nhz, dx, dz, oxg, ozg);
+    r->setImagefile(path);
+    if (r->read() != FILE_OK) return 1;
+    std::memcpy(out, r->getImagedata(), sizeof(float) * nxg * nzg * nhx * nhz);
+    return 0;
 }
 
 }  // extern "C"

@@ -533,6 +533,119 @@ void FN(orc_e2d_stress_bw)(FN(orc_e2d_t) *s)
 }
 
 /* ---------------------------------------------------------------------- */
+/* Exact discrete adjoint of the elastic 2D step (no reference counterpart: */
+/* FwiElastic2D back-propagates with the adjoint-state steps above, which   */
+/* are not the transpose of the discrete forward step).  Every linear map   */
+/* of orc_e2d_velocity / orc_e2d_stress is transposed as it stands.         */
+/* ---------------------------------------------------------------------- */
+
+/*
+ * Reverse of one derivative term of upd_x2d_ (axis 0) / upd_z2d_ (axis 2)
+ * with up to two targets sharing the strip: F += dt*M*d, F2 += dt*M2*d
+ * (outside the strips), minus dt*M*(psi' + C*d) (and the same for F2) inside,
+ * psi' = B*psi + A*d.  lF, lF2 are the adjoints of F', F2' (lF2 NULL: one
+ * target).  With S = M*lF + M2*lF2 this writes w = adjoint of d, replaces
+ * lpsi by the adjoint of the old psi and adds lF * dF'/dM to g and
+ * lF2 * dF2'/dM2 to g2.  psi holds the forward strip before (psi_old=1) or
+ * after (psi_old=0) the update.
+ */
+static void FN(adj_upd2_2d_)(int axis, int nx, int nz, int L, REAL dt, const REAL *tab, int stag,
+                             const REAL *df, const REAL *lF, const REAL *M, const REAL *lF2, const REAL *M2,
+                             const REAL *psiLo, const REAL *psiHi, int psi_old, REAL *lpsiLo, REAL *lpsiHi,
+                             REAL *w, REAL *g, REAL *g2)
+{
+    long n = (long)nx * nz;
+    for (long p = 0; p < n; p++) {
+        REAL S = M[p] * lF[p] + (lF2 ? M2[p] * lF2[p] : 0.0);
+        w[p] = dt * S;
+        g[p] += lF[p] * dt * df[p];
+        if (lF2) g2[p] += lF2[p] * dt * df[p];
+    }
+    int n1 = axis == 0 ? nz : L, n2 = axis == 0 ? L : nx;
+    for (int side = 0; side < 2; side++) {
+        const REAL *At = side ? TAB_A_R(tab, L, stag) : TAB_A_L(tab, L, stag);
+        const REAL *Bt = side ? TAB_B_R(tab, L, stag) : TAB_B_L(tab, L, stag);
+        const REAL *Ct = side ? TAB_C_R(tab, L, stag) : TAB_C_L(tab, L, stag);
+        const REAL *psi = side ? psiHi : psiLo;
+        REAL *lpsi = side ? lpsiHi : lpsiLo;
+        for (int i1 = 0; i1 < n1; i1++)
+            for (int i2 = 0; i2 < n2; i2++) {
+                long q = (long)i1 * n2 + i2, p;
+                int t;
+                if (axis == 0) {
+                    t = i2;
+                    p = (long)i1 * nx + i2 + (side ? nx - L : 0);
+                } else {
+                    t = i1;
+                    p = (long)(i1 + (side ? nz - L : 0)) * nx + i2;
+                }
+                REAL A = At[t], B = Bt[t], C = Ct[t];
+                REAL pn = psi_old ? B * psi[q] + A * df[p] : psi[q];
+                REAL S = M[p] * lF[p] + (lF2 ? M2[p] * lF2[p] : 0.0);
+                REAL lq = lpsi[q], dtS = dt * S, e = dt * (pn + C * df[p]);
+                w[p] = dtS * (1 - A - C) + A * lq;
+                lpsi[q] = B * lq - dtS * B;
+                g[p] -= lF[p] * e;
+                if (lF2) g2[p] -= lF2[p] * e;
+            }
+    }
+}
+
+/* Reverse of the stress half step.  s: forward state holding V of this
+ * step (after the velocity half step) and the V strips before the stress
+ * half step; a: adjoint state (stresses = adjoints after the step; the
+ * velocities and V strips carried back).  The free surface zeroes Szz on
+ * row lpml after the normal-stress update, so its adjoint is dropped there
+ * first.  gL, gL2M, gM += the gradients of the moduli. */
+void FN(orc_e2d_adj_stress)(FN(orc_e2d_t) *s, FN(orc_e2d_t) *a, REAL *gL, REAL *gL2M, REAL *gM)
+{
+    int nx = s->nx, nz = s->nz, L = s->lpml;
+    if (s->fs)
+        for (int ix = 0; ix < nx; ix++) a->Szz[(long)L * nx + ix] = 0.0;
+    FN(orc_deriv)(s->Vx, s->df, nx, 1, nz, 0, 0, s->order, s->dx);
+    FN(adj_upd2_2d_)(0, nx, nz, L, s->dt, s->tab, 0, s->df, a->Sxx, s->L2M, a->Szz, s->L, s->Vxx_left,
+                     s->Vxx_right, 1, a->Vxx_left, a->Vxx_right, a->df, gL2M, gL);
+    FN(orc_deriv_t)(a->df, a->Vx, nx, 1, nz, 0, 0, s->order, s->dx);
+    FN(orc_deriv)(s->Vz, s->df, nx, 1, nz, 2, 0, s->order, s->dz);
+    FN(adj_upd2_2d_)(2, nx, nz, L, s->dt, s->tab, 0, s->df, a->Sxx, s->L, a->Szz, s->L2M, s->Vzz_top,
+                     s->Vzz_bottom, 1, a->Vzz_top, a->Vzz_bottom, a->df, gL, gL2M);
+    FN(orc_deriv_t)(a->df, a->Vz, nx, 1, nz, 2, 0, s->order, s->dz);
+    FN(orc_deriv)(s->Vz, s->df, nx, 1, nz, 0, 1, s->order, s->dx);
+    FN(adj_upd2_2d_)(0, nx, nz, L, s->dt, s->tab, 1, s->df, a->Sxz, s->M, 0, 0, s->Vzx_left, s->Vzx_right, 1,
+                     a->Vzx_left, a->Vzx_right, a->df, gM, 0);
+    FN(orc_deriv_t)(a->df, a->Vz, nx, 1, nz, 0, 1, s->order, s->dx);
+    FN(orc_deriv)(s->Vx, s->df, nx, 1, nz, 2, 1, s->order, s->dz);
+    FN(adj_upd2_2d_)(2, nx, nz, L, s->dt, s->tab, 1, s->df, a->Sxz, s->M, 0, 0, s->Vxz_top, s->Vxz_bottom, 1,
+                     a->Vxz_top, a->Vxz_bottom, a->df, gM, 0);
+    FN(orc_deriv_t)(a->df, a->Vx, nx, 1, nz, 2, 1, s->order, s->dz);
+}
+
+/* Reverse of the velocity half step.  s: forward state with the stresses
+ * before the step and the stress strips after the velocity half step; a:
+ * adjoint state (velocities = adjoints after the half step, unchanged;
+ * stresses and stress strips carried back).  gRx, gRz += dJ/dRx, dJ/dRz. */
+void FN(orc_e2d_adj_velocity)(FN(orc_e2d_t) *s, FN(orc_e2d_t) *a, REAL *gRx, REAL *gRz)
+{
+    int nx = s->nx, nz = s->nz, L = s->lpml;
+    FN(orc_deriv)(s->Sxx, s->df, nx, 1, nz, 0, 1, s->order, s->dx);
+    FN(adj_upd2_2d_)(0, nx, nz, L, s->dt, s->tab, 1, s->df, a->Vx, s->Rx, 0, 0, s->Sxx_left, s->Sxx_right, 0,
+                     a->Sxx_left, a->Sxx_right, a->df, gRx, 0);
+    FN(orc_deriv_t)(a->df, a->Sxx, nx, 1, nz, 0, 1, s->order, s->dx);
+    FN(orc_deriv)(s->Sxz, s->df, nx, 1, nz, 2, 0, s->order, s->dz);
+    FN(adj_upd2_2d_)(2, nx, nz, L, s->dt, s->tab, 0, s->df, a->Vx, s->Rx, 0, 0, s->Sxzz_top, s->Sxzz_bottom, 0,
+                     a->Sxzz_top, a->Sxzz_bottom, a->df, gRx, 0);
+    FN(orc_deriv_t)(a->df, a->Sxz, nx, 1, nz, 2, 0, s->order, s->dz);
+    FN(orc_deriv)(s->Sxz, s->df, nx, 1, nz, 0, 0, s->order, s->dx);
+    FN(adj_upd2_2d_)(0, nx, nz, L, s->dt, s->tab, 0, s->df, a->Vz, s->Rz, 0, 0, s->Sxzx_left, s->Sxzx_right, 0,
+                     a->Sxzx_left, a->Sxzx_right, a->df, gRz, 0);
+    FN(orc_deriv_t)(a->df, a->Sxz, nx, 1, nz, 0, 0, s->order, s->dx);
+    FN(orc_deriv)(s->Szz, s->df, nx, 1, nz, 2, 1, s->order, s->dz);
+    FN(adj_upd2_2d_)(2, nx, nz, L, s->dt, s->tab, 1, s->df, a->Vz, s->Rz, 0, 0, s->Szz_top, s->Szz_bottom, 0,
+                     a->Szz_top, a->Szz_bottom, a->df, gRz, 0);
+    FN(orc_deriv_t)(a->df, a->Szz, nx, 1, nz, 2, 1, s->order, s->dz);
+}
+
+/* ---------------------------------------------------------------------- */
 /* WavesAcoustic3D (waves.cpp:710-1275)                                    */
 /* ---------------------------------------------------------------------- */
 
@@ -607,6 +720,96 @@ void FN(orc_a3d_stress)(FN(orc_a3d_t) *s)
     FN(upd3d_)(s, 2, 0, s->P, s->L, s->Vzz_top, s->Vzz_bottom);
 }
 
+/* Exact discrete adjoint of the acoustic 3D step (no reference counterpart:
+ * FwiAcoustic3D back-propagates with the forward operator).  Reverse of one
+ * derivative term of upd3d_: F' = F + dt*M*d outside the strips, inside
+ * psi' = B*psi + A*d and F' -= dt*M*(psi' + C*d).  lF = adjoint of F',
+ * lpsi = adjoint of psi' (replaced by the adjoint of the old psi), w = the
+ * adjoint of d, g += lF dF'/dM; psi holds the forward strip before
+ * (psi_old=1) or after (psi_old=0) the update; d in s->df. */
+static void FN(adj_upd3d_)(FN(orc_a3d_t) *s, int axis, int stag, const REAL *lF, const REAL *M, const REAL *psiLo,
+                           const REAL *psiHi, int psi_old, REAL *lpsiLo, REAL *lpsiHi, REAL *w, REAL *g)
+{
+    int nx = s->nx, ny = s->ny, nz = s->nz, L = s->lpml;
+    REAL dt = s->dt;
+    const REAL *df = s->df, *tab = s->tab;
+    long nxy = (long)nx * ny, n = nxy * nz;
+    for (long p = 0; p < n; p++) {
+        w[p] = dt * M[p] * lF[p];
+        g[p] += lF[p] * dt * df[p];
+    }
+    int ex = axis == 0 ? L : nx, ey = axis == 1 ? L : ny, ez = axis == 2 ? L : nz;
+    for (int side = 0; side < 2; side++) {
+        const REAL *At = side ? TAB_A_R(tab, L, stag) : TAB_A_L(tab, L, stag);
+        const REAL *Bt = side ? TAB_B_R(tab, L, stag) : TAB_B_L(tab, L, stag);
+        const REAL *Ct = side ? TAB_C_R(tab, L, stag) : TAB_C_L(tab, L, stag);
+        const REAL *psi = side ? psiHi : psiLo;
+        REAL *lpsi = side ? lpsiHi : lpsiLo;
+        for (int iz = 0; iz < ez; iz++)
+            for (int iy = 0; iy < ey; iy++)
+                for (int ix = 0; ix < ex; ix++) {
+                    long q, p;
+                    int t;
+                    if (axis == 0) {
+                        q = (long)iz * L * ny + (long)iy * L + ix;
+                        p = iz * nxy + (long)iy * nx + ix + (side ? nx - L : 0);
+                        t = ix;
+                    } else if (axis == 1) {
+                        q = (long)iz * nx * L + (long)iy * nx + ix;
+                        p = iz * nxy + (long)(iy + (side ? ny - L : 0)) * nx + ix;
+                        t = iy;
+                    } else {
+                        q = iz * nxy + (long)iy * nx + ix;
+                        p = (iz + (side ? nz - L : 0)) * nxy + (long)iy * nx + ix;
+                        t = iz;
+                    }
+                    REAL A = At[t], B = Bt[t], C = Ct[t];
+                    REAL pn = psi_old ? B * psi[q] + A * df[p] : psi[q];
+                    REAL lf = lF[p], lq = lpsi[q], dtM = dt * M[p];
+                    w[p] = dtM * (1 - A - C) * lf + A * lq;
+                    lpsi[q] = B * lq - dtM * B * lf;
+                    g[p] -= lf * dt * (pn + C * df[p]);
+                }
+    }
+}
+
+/* Reverse of the stress half step: s holds V of this step and the Vxx /
+ * Vyy / Vzz strips before the stress half step; a: adjoint state (P =
+ * adjoint of the pressure after the step, unchanged; velocities and V
+ * strips carried back).  gL += dJ/dL. */
+void FN(orc_a3d_adj_stress)(FN(orc_a3d_t) *s, FN(orc_a3d_t) *a, REAL *gL)
+{
+    int nx = s->nx, ny = s->ny, nz = s->nz;
+    REAL *V[3] = {s->Vx, s->Vy, s->Vz}, *lV[3] = {a->Vx, a->Vy, a->Vz};
+    REAL *pl[3] = {s->Vxx_left, s->Vyy_front, s->Vzz_top}, *ph[3] = {s->Vxx_right, s->Vyy_back, s->Vzz_bottom};
+    REAL *ll[3] = {a->Vxx_left, a->Vyy_front, a->Vzz_top}, *lh[3] = {a->Vxx_right, a->Vyy_back, a->Vzz_bottom};
+    REAL d[3] = {s->dx, s->dy, s->dz};
+    for (int ax = 0; ax < 3; ax++) {
+        FN(orc_deriv)(V[ax], s->df, nx, ny, nz, ax, 0, s->order, d[ax]);
+        FN(adj_upd3d_)(s, ax, 0, a->P, s->L, pl[ax], ph[ax], 1, ll[ax], lh[ax], a->df, gL);
+        FN(orc_deriv_t)(a->df, lV[ax], nx, ny, nz, ax, 0, s->order, d[ax]);
+    }
+}
+
+/* Reverse of the velocity half step: s holds P before the step and the P
+ * strips after the velocity half step; a: adjoint state (velocities =
+ * adjoints after the half step, unchanged; P and the P strips carried
+ * back).  gRx, gRy, gRz += dJ/dRx, dJ/dRy, dJ/dRz. */
+void FN(orc_a3d_adj_velocity)(FN(orc_a3d_t) *s, FN(orc_a3d_t) *a, REAL *gRx, REAL *gRy, REAL *gRz)
+{
+    int nx = s->nx, ny = s->ny, nz = s->nz;
+    REAL *lV[3] = {a->Vx, a->Vy, a->Vz}, *g[3] = {gRx, gRy, gRz};
+    const REAL *R[3] = {s->Rx, s->Ry, s->Rz};
+    REAL *pl[3] = {s->P_left, s->P_front, s->P_top}, *ph[3] = {s->P_right, s->P_back, s->P_bottom};
+    REAL *ll[3] = {a->P_left, a->P_front, a->P_top}, *lh[3] = {a->P_right, a->P_back, a->P_bottom};
+    REAL d[3] = {s->dx, s->dy, s->dz};
+    for (int ax = 0; ax < 3; ax++) {
+        FN(orc_deriv)(s->P, s->df, nx, ny, nz, ax, 1, s->order, d[ax]);
+        FN(adj_upd3d_)(s, ax, 1, lV[ax], R[ax], pl[ax], ph[ax], 0, ll[ax], lh[ax], a->df, g[ax]);
+        FN(orc_deriv_t)(a->df, a->P, nx, ny, nz, ax, 1, s->order, d[ax]);
+    }
+}
+
 /* Trilinear injection / gather (waves.cpp:998-1275); map/shift (x,y,z). */
 void FN(orc_inject3d)(REAL *F, const REAL *Mod, int nx, int ny, int lpml, REAL dt,
                       const int *map, const REAL *shift, const REAL *wav,
@@ -644,6 +847,179 @@ void FN(orc_record3d)(const REAL *F, int nx, int ny, int lpml, const int *map,
                         data[(long)i * nt + it] += xtri[i3] * ytri[i2] * ztri[i1] * F[p];
                     }
         }
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* WavesElastic3D (waves.cpp:2310-3258, 4104-4403)                          */
+/* ---------------------------------------------------------------------- */
+
+/* Strip slots (PmlElastic3D, pml.h:197-234): x faces (lpml*ny*nz each)
+ * 0..11 Sxx, Sxzx, Sxyx, Vxx, Vzx, Vyx (left, right); y faces (nx*lpml*nz)
+ * 12..23 Syy, Sxyy, Syzy, Vyy, Vxy, Vzy (front, back); z faces (nx*ny*lpml)
+ * 24..35 Szz, Sxzz, Syzz, Vzz, Vxz, Vyz (top, bottom). */
+typedef struct {
+    int nx, ny, nz, lpml, order, fs;
+    REAL dx, dy, dz, dt;
+    const REAL *tab;
+    REAL *Vx, *Vy, *Vz, *Sxx, *Syy, *Szz, *Syz, *Sxz, *Sxy;
+    REAL *psi[36];
+    const REAL *L, *L2M, *Mxz, *Myz, *Mxy, *Rx, *Ry, *Rz;
+    REAL *df;
+} FN(orc_e3d_t);
+
+/* F_k += dt*M_k*df over the grid (k = 1..3, F2/F3 optional), then the two
+ * strips of `axis`: psi = B*psi + A*df, F_k -= dt*M_k*(psi + C*df). */
+static void FN(upd3e_)(FN(orc_e3d_t) *s, int axis, int stag, REAL *F1, const REAL *M1, REAL *F2, const REAL *M2,
+                       REAL *F3, const REAL *M3, int slot)
+{
+    int nx = s->nx, ny = s->ny, nz = s->nz, L = s->lpml;
+    REAL dt = s->dt;
+    const REAL *df = s->df, *tab = s->tab;
+    REAL *psiLo = s->psi[slot], *psiHi = s->psi[slot + 1];
+    long nxy = (long)nx * ny, n = nxy * nz;
+    for (long p = 0; p < n; p++) {
+        F1[p] += dt * M1[p] * df[p];
+        if (F2) F2[p] += dt * M2[p] * df[p];
+        if (F3) F3[p] += dt * M3[p] * df[p];
+    }
+    const REAL *Al = TAB_A_L(tab, L, stag), *Bl = TAB_B_L(tab, L, stag), *Cl = TAB_C_L(tab, L, stag);
+    const REAL *Ar = TAB_A_R(tab, L, stag), *Br = TAB_B_R(tab, L, stag), *Cr = TAB_C_R(tab, L, stag);
+    int ex = axis == 0 ? L : nx, ey = axis == 1 ? L : ny, ez = axis == 2 ? L : nz;
+    for (int iz = 0; iz < ez; iz++)
+        for (int iy = 0; iy < ey; iy++)
+            for (int ix = 0; ix < ex; ix++) {
+                long q, p, ph;
+                int t;
+                if (axis == 0) {
+                    q = (long)iz * L * ny + (long)iy * L + ix;
+                    p = iz * nxy + (long)iy * nx + ix;
+                    ph = p + nx - L;
+                    t = ix;
+                } else if (axis == 1) {
+                    q = (long)iz * nx * L + (long)iy * nx + ix;
+                    p = iz * nxy + (long)iy * nx + ix;
+                    ph = p + (long)(ny - L) * nx;
+                    t = iy;
+                } else {
+                    q = iz * nxy + (long)iy * nx + ix;
+                    p = q;
+                    ph = p + (long)(nz - L) * nxy;
+                    t = iz;
+                }
+                for (int side = 0; side < 2; side++) {
+                    long c = side ? ph : p;
+                    REAL *psi = side ? psiHi : psiLo;
+                    REAL A = side ? Ar[t] : Al[t], B = side ? Br[t] : Bl[t], C = side ? Cr[t] : Cl[t];
+                    psi[q] = B * psi[q] + A * df[c];
+                    F1[c] -= dt * M1[c] * (psi[q] + C * df[c]);
+                    if (F2) F2[c] -= dt * M2[c] * (psi[q] + C * df[c]);
+                    if (F3) F3[c] -= dt * M3[c] * (psi[q] + C * df[c]);
+                }
+            }
+}
+
+void FN(orc_e3d_velocity)(FN(orc_e3d_t) *s)
+{
+    int nx = s->nx, ny = s->ny, nz = s->nz, o = s->order;
+    FN(orc_deriv)(s->Sxx, s->df, nx, ny, nz, 0, 1, o, s->dx);
+    FN(upd3e_)(s, 0, 1, s->Vx, s->Rx, 0, 0, 0, 0, 0);
+    FN(orc_deriv)(s->Sxy, s->df, nx, ny, nz, 1, 0, o, s->dy);
+    FN(upd3e_)(s, 1, 0, s->Vx, s->Rx, 0, 0, 0, 0, 14);
+    FN(orc_deriv)(s->Sxz, s->df, nx, ny, nz, 2, 0, o, s->dz);
+    FN(upd3e_)(s, 2, 0, s->Vx, s->Rx, 0, 0, 0, 0, 26);
+    FN(orc_deriv)(s->Sxy, s->df, nx, ny, nz, 0, 0, o, s->dx);
+    FN(upd3e_)(s, 0, 0, s->Vy, s->Ry, 0, 0, 0, 0, 4);
+    FN(orc_deriv)(s->Syy, s->df, nx, ny, nz, 1, 1, o, s->dy);
+    FN(upd3e_)(s, 1, 1, s->Vy, s->Ry, 0, 0, 0, 0, 12);
+    FN(orc_deriv)(s->Syz, s->df, nx, ny, nz, 2, 0, o, s->dz);
+    FN(upd3e_)(s, 2, 0, s->Vy, s->Ry, 0, 0, 0, 0, 28);
+    FN(orc_deriv)(s->Sxz, s->df, nx, ny, nz, 0, 0, o, s->dx);
+    FN(upd3e_)(s, 0, 0, s->Vz, s->Rz, 0, 0, 0, 0, 2);
+    FN(orc_deriv)(s->Syz, s->df, nx, ny, nz, 1, 0, o, s->dy);
+    FN(upd3e_)(s, 1, 0, s->Vz, s->Rz, 0, 0, 0, 0, 16);
+    FN(orc_deriv)(s->Szz, s->df, nx, ny, nz, 2, 1, o, s->dz);
+    FN(upd3e_)(s, 2, 1, s->Vz, s->Rz, 0, 0, 0, 0, 24);
+}
+
+void FN(orc_e3d_stress)(FN(orc_e3d_t) *s)
+{
+    int nx = s->nx, ny = s->ny, nz = s->nz, o = s->order;
+    long nxy = (long)nx * ny;
+    FN(orc_deriv)(s->Vx, s->df, nx, ny, nz, 0, 0, o, s->dx);
+    FN(upd3e_)(s, 0, 0, s->Sxx, s->L2M, s->Syy, s->L, s->Szz, s->L, 6);
+    FN(orc_deriv)(s->Vy, s->df, nx, ny, nz, 1, 0, o, s->dy);
+    FN(upd3e_)(s, 1, 0, s->Sxx, s->L, s->Syy, s->L2M, s->Szz, s->L, 18);
+    FN(orc_deriv)(s->Vz, s->df, nx, ny, nz, 2, 0, o, s->dz);
+    /* the free surface row lpml lies in no z strip: zeroing Szz there after
+     * the strips equals the reference's zeroing before them */
+    FN(upd3e_)(s, 2, 0, s->Sxx, s->L, s->Syy, s->L, s->Szz, s->L2M, 30);
+    if (s->fs)
+        for (long p = 0; p < nxy; p++) s->Szz[(long)s->lpml * nxy + p] = 0.0;
+    FN(orc_deriv)(s->Vz, s->df, nx, ny, nz, 0, 1, o, s->dx);
+    FN(upd3e_)(s, 0, 1, s->Sxz, s->Mxz, 0, 0, 0, 0, 8);
+    FN(orc_deriv)(s->Vx, s->df, nx, ny, nz, 2, 1, o, s->dz);
+    FN(upd3e_)(s, 2, 1, s->Sxz, s->Mxz, 0, 0, 0, 0, 32);
+    FN(orc_deriv)(s->Vy, s->df, nx, ny, nz, 2, 1, o, s->dz);
+    FN(upd3e_)(s, 2, 1, s->Syz, s->Myz, 0, 0, 0, 0, 34);
+    FN(orc_deriv)(s->Vz, s->df, nx, ny, nz, 1, 1, o, s->dy);
+    FN(upd3e_)(s, 1, 1, s->Syz, s->Myz, 0, 0, 0, 0, 22);
+    FN(orc_deriv)(s->Vx, s->df, nx, ny, nz, 1, 1, o, s->dy);
+    FN(upd3e_)(s, 1, 1, s->Sxy, s->Mxy, 0, 0, 0, 0, 20);
+    FN(orc_deriv)(s->Vy, s->df, nx, ny, nz, 0, 1, o, s->dx);
+    FN(upd3e_)(s, 0, 1, s->Sxy, s->Mxy, 0, 0, 0, 0, 10);
+}
+
+/* Explosive source of WavesElastic3D::insertSource (waves.cpp:4137-4160):
+ * Sxx, Syy, Szz -= dt*wav*xtri*ytri*ztri at each in-bounds trace's 8 cells. */
+void FN(orc_inject3d_explosive)(REAL *Sxx, REAL *Syy, REAL *Szz, int nx, int ny, int lpml, REAL dt, const int *map,
+                                const REAL *shift, const REAL *wav, int ntrace, int nt, int it)
+{
+    long nxy = (long)nx * ny;
+    for (int i = 0; i < ntrace; i++) {
+        const int *m = map + 3 * i;
+        const REAL *sh = shift + 3 * i;
+        if (m[0] < 0 || m[1] < 0 || m[2] < 0) continue;
+        REAL xt[2] = {1 - sh[0], sh[0]}, yt[2] = {1 - sh[1], sh[1]}, zt[2] = {1 - sh[2], sh[2]};
+        for (int i1 = 0; i1 < 2; i1++)
+            for (int i2 = 0; i2 < 2; i2++)
+                for (int i3 = 0; i3 < 2; i3++) {
+                    long p = (long)(lpml + m[2] + i1) * nxy + (long)(lpml + m[1] + i2) * nx + (lpml + m[0] + i3);
+                    REAL v = dt * wav[(long)i * nt + it] * xt[i3] * yt[i2] * zt[i1];
+                    Sxx[p] -= v;
+                    Syy[p] -= v;
+                    Szz[p] -= v;
+                }
+    }
+}
+
+/* Pressure gather of WavesElastic3D::recordData (waves.cpp:4283-4311): per
+ * trace factor = (3 rho vp^2 - 4 rho vs^2) / 3 at the trace's model cell
+ * (in double, rounded to REAL), then data += (1/3) w F / factor for F =
+ * Sxx, Syy, Szz at each corner, in double. */
+void FN(orc_record3d_pressure_el)(const REAL *Sxx, const REAL *Syy, const REAL *Szz, int nx, int ny, int lpml,
+                                  const REAL *vp, const REAL *vs, const REAL *rho, int mnx, int mny, const int *map,
+                                  const REAL *shift, REAL *data, int ntrace, int nt, int it)
+{
+    long nxy = (long)nx * ny;
+    const REAL *F[3] = {Sxx, Syy, Szz};
+    for (int i = 0; i < ntrace; i++) {
+        const int *m = map + 3 * i;
+        const REAL *sh = shift + 3 * i;
+        if (m[0] < 0 || m[1] < 0 || m[2] < 0) continue;
+        long c = ((long)m[2] * mny + m[1]) * mnx + m[0];
+        REAL factor = (REAL)((3. * (double)rho[c] * (double)vp[c] * (double)vp[c] -
+                              4. * (double)rho[c] * (double)vs[c] * (double)vs[c]) / 3.);
+        REAL xt[2] = {1 - sh[0], sh[0]}, yt[2] = {1 - sh[1], sh[1]}, zt[2] = {1 - sh[2], sh[2]};
+        long q = (long)i * nt + it;
+        for (int i1 = 0; i1 < 2; i1++)
+            for (int i2 = 0; i2 < 2; i2++)
+                for (int i3 = 0; i3 < 2; i3++) {
+                    long p = (long)(lpml + m[2] + i1) * nxy + (long)(lpml + m[1] + i2) * nx + (lpml + m[0] + i3);
+                    for (int k = 0; k < 3; k++)
+                        data[q] = (REAL)((double)data[q] + (1. / 3.) * (double)xt[i3] * (double)yt[i2] *
+                                                               (double)zt[i1] * (double)F[k][p] / (double)factor);
+                }
     }
 }
 
@@ -750,6 +1126,68 @@ int FN(orc_stagger_e2d)(const REAL *vp, const REAL *vs, const REAL *rho, int nx,
     if (FN(invert_)(Rx, n) || FN(invert_)(Rz, n)) return -1;
     if (fs)
         for (int ix = 0; ix < nxp; ix++) Rx[(long)lpml * nxp + ix] *= 2.0;
+    return 0;
+}
+
+/* Model::padmodel3d (model.cpp:130-182): the model in the interior, every
+ * padding cell a copy of the nearest model cell (edge replication). */
+static void FN(pad3d_)(REAL *out, const REAL *in, int nx, int ny, int nz, int L)
+{
+    int nxp = nx + 2 * L, nyp = ny + 2 * L, nzp = nz + 2 * L;
+    for (int iz = 0; iz < nzp; iz++)
+        for (int iy = 0; iy < nyp; iy++)
+            for (int ix = 0; ix < nxp; ix++) {
+                int jx = ix - L < 0 ? 0 : (ix - L >= nx ? nx - 1 : ix - L);
+                int jy = iy - L < 0 ? 0 : (iy - L >= ny ? ny - 1 : iy - L);
+                int jz = iz - L < 0 ? 0 : (iz - L >= nz ? nz - 1 : iz - L);
+                out[((long)iz * nyp + iy) * nxp + ix] = in[((long)jz * ny + jy) * nx + jx];
+            }
+}
+
+/* ModelElastic3D::staggerModels (model.cpp:3371-3473): pad (padmodel3d),
+ * L2M = rho vp^2, L = rho vp^2 - 2 rho vs^2 (in double), M = rho vs^2;
+ * free surface plane lpml: L = 0, L2M = M; Rx/Ry/Rz staggered along x/y/z,
+ * M_xz along x then z, M_yz y then z, M_xy x then y; densities inverted;
+ * free surface: Rx, Ry doubled on plane lpml.  out: L, L2M, Mxz, Myz, Mxy,
+ * Rx, Ry, Rz, each (nx+2L)(ny+2L)(nz+2L). */
+int FN(orc_stagger_e3d)(const REAL *vp, const REAL *vs, const REAL *rho, int nx, int ny, int nz, int lpml, int fs,
+                        REAL *L, REAL *L2M, REAL *Mxz, REAL *Myz, REAL *Mxy, REAL *Rx, REAL *Ry, REAL *Rz)
+{
+    int nxp = nx + 2 * lpml, nyp = ny + 2 * lpml, nzp = nz + 2 * lpml;
+    long nxy = (long)nxp * nyp, n = nxy * nzp;
+    FN(pad3d_)(Rx, rho, nx, ny, nz, lpml);
+    FN(pad3d_)(Ry, rho, nx, ny, nz, lpml);
+    FN(pad3d_)(Rz, rho, nx, ny, nz, lpml);
+    FN(pad3d_)(L, vp, nx, ny, nz, lpml);
+    FN(pad3d_)(Mxz, vs, nx, ny, nz, lpml);
+    for (long p = 0; p < n; p++) {
+        REAL v = L[p], w = Mxz[p], r = Rx[p];
+        L2M[p] = r * v * v;
+        L[p] = (REAL)((double)(r * v * v) - 2.0 * (double)r * (double)w * (double)w);
+        Mxz[p] = r * w * w;
+        Myz[p] = r * w * w;
+        Mxy[p] = r * w * w;
+    }
+    if (fs)
+        for (long p = 0; p < nxy; p++) {
+            L[(long)lpml * nxy + p] = 0.0;
+            L2M[(long)lpml * nxy + p] = Mxz[(long)lpml * nxy + p];
+        }
+    FN(stagger_)(Rx, nxp, nyp, nzp, 0);
+    FN(stagger_)(Ry, nxp, nyp, nzp, 1);
+    FN(stagger_)(Rz, nxp, nyp, nzp, 2);
+    FN(stagger_)(Mxz, nxp, nyp, nzp, 0);
+    FN(stagger_)(Mxz, nxp, nyp, nzp, 2);
+    FN(stagger_)(Myz, nxp, nyp, nzp, 1);
+    FN(stagger_)(Myz, nxp, nyp, nzp, 2);
+    FN(stagger_)(Mxy, nxp, nyp, nzp, 0);
+    FN(stagger_)(Mxy, nxp, nyp, nzp, 1);
+    if (FN(invert_)(Rx, n) || FN(invert_)(Ry, n) || FN(invert_)(Rz, n)) return -1;
+    if (fs)
+        for (long p = 0; p < nxy; p++) {
+            Rx[(long)lpml * nxy + p] *= 2.0;
+            Ry[(long)lpml * nxy + p] *= 2.0;
+        }
     return 0;
 }
 

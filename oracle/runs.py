@@ -392,3 +392,117 @@ def ref_a3d_run(vp, rho, lpml, d3, dt, nt, order, fs, srcfield, scoords, wav, re
                       np.ascontiguousarray(wav, np.float32).ravel(), FIELD[recfield], gc.shape[0],
                       gc.ravel(), rec.reshape(-1), fields.reshape(-1))
     return dict(rec=rec, P=fields[0], Vx=fields[1], Vy=fields[2], Vz=fields[3])
+
+
+# --------------------------------------------------------------------------
+# Elastic 3D
+# --------------------------------------------------------------------------
+
+E3D_FIELDS = ("Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy")
+# strip slots of orc_e3d_t: (name, face axis), left/front/top first
+E3D_STRIPS = [(n, 0) for n in ("Sxx", "Sxzx", "Sxyx", "Vxx", "Vzx", "Vyx")] + \
+             [(n, 1) for n in ("Syy", "Sxyy", "Syzy", "Vyy", "Vxy", "Vzy")] + \
+             [(n, 2) for n in ("Szz", "Sxzz", "Syzz", "Vzz", "Vxz", "Vyz")]
+
+
+def stagger_e3d(vp, vs, rho, lpml, fs, real=np.float32):
+    nz, ny, nx = vp.shape
+    shp = (nz + 2 * lpml, ny + 2 * lpml, nx + 2 * lpml)
+    out = [np.zeros(shp, real) for _ in range(8)]
+    rc = fn("stagger_e3d", real)(*(np.ascontiguousarray(a, real).ravel() for a in (vp, vs, rho)), nx, ny, nz, lpml,
+                                 int(fs), *(a.reshape(-1) for a in out))
+    if rc:
+        raise ValueError("zero density")
+    return out  # L, L2M, Mxz, Myz, Mxy, Rx, Ry, Rz
+
+
+class E3DState:
+    """Padded-grid state of WavesElastic3D plus its 36 PML strips."""
+
+    def __init__(self, model, lpml, order, d3, dt, fs, real=np.float32):
+        self.real = real
+        L = model[0]
+        nzp, nyp, nxp = L.shape
+        self.nx, self.ny, self.nz, self.lpml = nxp, nyp, nzp, lpml
+        (self.L, self.L2M, self.Mxz, self.Myz, self.Mxy, self.Rx, self.Ry,
+         self.Rz) = (np.ascontiguousarray(a, real) for a in model)
+        for k in E3D_FIELDS + ("df",):
+            setattr(self, k, np.zeros((nzp, nyp, nxp), real))
+        shp = {0: (nzp, nyp, lpml), 1: (nzp, lpml, nxp), 2: (lpml, nyp, nxp)}
+        self.strips = []
+        for name, ax in E3D_STRIPS:
+            self.strips += [np.zeros(shp[ax], real), np.zeros(shp[ax], real)]
+        self.tab = pml_tables(lpml, dt, real)
+        ct = C.c_float if real == np.float32 else C.c_double
+        st = STRUCTS[real][3]()
+        st.nx, st.ny, st.nz, st.lpml, st.order, st.fs = nxp, nyp, nzp, lpml, order, int(fs)
+        st.dx, st.dy, st.dz, st.dt = d3[0], d3[1], d3[2], dt
+        for k in E3D_FIELDS + ("L", "L2M", "Mxz", "Myz", "Mxy", "Rx", "Ry", "Rz", "df", "tab"):
+            setattr(st, k, _p(getattr(self, k), ct))
+        for i, a in enumerate(self.strips):
+            st.psi[i] = _p(a, ct)
+        self.st = st
+
+    def velocity(self):
+        fn("e3d_velocity", self.real)(C.byref(self.st))
+
+    def stress(self):
+        fn("e3d_stress", self.real)(C.byref(self.st))
+
+
+def e3d_run(vp, vs, rho, lpml, d3, dt, nt, order, fs, srcfield, scoords, wav, recfield, gcoords, real=np.float32):
+    """Elastic 3D modelling with the oracle; returns dict(rec, state, fields)."""
+    nz, ny, nx = vp.shape
+    smap, sshift = makemap3d(scoords, (nx, ny, nz), d3, srcfield)
+    gmap, gshift = makemap3d(gcoords, (nx, ny, nz), d3, recfield)
+    s = E3DState(stagger_e3d(vp, vs, rho, lpml, fs, real), lpml, order, d3, dt, fs, real)
+    wav = np.ascontiguousarray(wav, real).reshape(smap.shape[0], nt)
+    rec = np.zeros((gmap.shape[0], nt), real)
+    sh32 = np.ascontiguousarray(sshift, real).reshape(-1)
+    gh32 = np.ascontiguousarray(gshift, real).reshape(-1)
+    mod = [np.ascontiguousarray(a, real).ravel() for a in (vp, vs, rho)]
+    for it in range(nt):
+        s.velocity()
+        s.stress()
+        if srcfield == "P":
+            fn("inject3d_explosive", real)(s.Sxx.reshape(-1), s.Syy.reshape(-1), s.Szz.reshape(-1), s.nx, s.ny,
+                                           lpml, real(dt), smap.reshape(-1), sh32, wav.reshape(-1), smap.shape[0],
+                                           nt, it)
+        else:
+            F, M = {"VX": (s.Vx, s.Rx), "VY": (s.Vy, s.Ry), "VZ": (s.Vz, s.Rz)}[srcfield]
+            fn("inject3d", real)(F.reshape(-1), M.reshape(-1), s.nx, s.ny, lpml, real(dt), smap.reshape(-1), sh32,
+                                 wav.reshape(-1), smap.shape[0], nt, it)
+        if recfield == "P":
+            fn("record3d_pressure_el", real)(s.Sxx.reshape(-1), s.Syy.reshape(-1), s.Szz.reshape(-1), s.nx, s.ny,
+                                             lpml, *mod, nx, ny, gmap.reshape(-1), gh32, rec.reshape(-1),
+                                             gmap.shape[0], nt, it)
+        else:
+            F = {"VX": s.Vx, "VY": s.Vy, "VZ": s.Vz}[recfield]
+            fn("record3d", real)(F.reshape(-1), s.nx, s.ny, lpml, gmap.reshape(-1), gh32, rec.reshape(-1),
+                                 gmap.shape[0], nt, it)
+    return dict(rec=rec, state=s, **{k: getattr(s, k) for k in E3D_FIELDS})
+
+
+_FIELD_CODE = {"P": 0, "VX": 1, "VY": 2, "VZ": 3}
+
+
+def ref_e3d_run(vp, vs, rho, lpml, d3, dt, nt, order, fs, srcfield, scoords, wav, recfield, gcoords):
+    """The compiled reference's WavesElastic3D modelling run."""
+    nz, ny, nx = vp.shape
+    sc = np.ascontiguousarray(scoords, np.float32).reshape(-1, 3)
+    gc = np.ascontiguousarray(gcoords, np.float32).reshape(-1, 3)
+    rec = np.zeros((gc.shape[0], nt), np.float32)
+    fields = np.zeros((9, nz + 2 * lpml, ny + 2 * lpml, nx + 2 * lpml), np.float32)
+    ref().ref_e3d_run(*(np.ascontiguousarray(a, np.float32).ravel() for a in (vp, vs, rho)), nx, ny, nz, lpml,
+                      *(np.float32(v) for v in d3), np.float32(dt), nt, order, int(fs), _FIELD_CODE[srcfield],
+                      sc.shape[0], sc.ravel(), np.ascontiguousarray(wav, np.float32).ravel(), _FIELD_CODE[recfield],
+                      gc.shape[0], gc.ravel(), rec.reshape(-1), fields.reshape(-1))
+    return dict(rec=rec, **{k: fields[i] for i, k in enumerate(E3D_FIELDS)})
+
+
+def ref_stagger_e3d(vp, vs, rho, lpml, fs):
+    nz, ny, nx = vp.shape
+    out = np.zeros((8, nz + 2 * lpml, ny + 2 * lpml, nx + 2 * lpml), np.float32)
+    ref().ref_stagger_e3d(*(np.ascontiguousarray(a, np.float32).ravel() for a in (vp, vs, rho)), nx, ny, nz, lpml,
+                          int(fs), out.reshape(-1))
+    return list(out)

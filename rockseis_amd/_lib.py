@@ -61,7 +61,7 @@ class FwiA3D(C.Structure):
                 + [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("src", Traces), ("obs", Traces)]
                 + [(n, C.c_void_p) for n in ("dmod", "dres", "wavgrad", "vpgrad", "rhograd")]
                 + [("misfit", C.POINTER(C.c_double)), ("ncheck", C.c_int), ("fp16", C.c_int),
-                   ("misfit_type", C.c_int), ("weight", C.c_void_p)])
+                   ("misfit_type", C.c_int), ("weight", C.c_void_p), ("adjoint", C.c_int)])
 
 
 class RtmA2D(C.Structure):
@@ -94,9 +94,34 @@ class E2DRecords(C.Structure):
 class FwiE2D(C.Structure):
     _fields_ = [("tab", C.c_void_p), ("ncheck", C.c_int), ("fp16", C.c_int), ("scale", C.c_int),
                 ("nx", C.c_int), ("nz", C.c_int), ("grad", C.c_void_p * 3), ("misfit", C.POINTER(C.c_double)),
-                ("misfit_type", C.c_int)]
+                ("misfit_type", C.c_int), ("adjoint", C.c_int)]
 
 
+E3D_NSTRIP = 36
+E3D_STRIPS = tuple(f"{n}_{s}" for n, sides in
+                   ((n, ("left", "right")) for n in ("Sxx", "Sxzx", "Sxyx", "Vxx", "Vzx", "Vyx")) for s in sides) + \
+    tuple(f"{n}_{s}" for n in ("Syy", "Sxyy", "Syzy", "Vyy", "Vxy", "Vzy") for s in ("front", "back")) + \
+    tuple(f"{n}_{s}" for n in ("Szz", "Sxzz", "Syzz", "Vzz", "Vxz", "Vyz") for s in ("top", "bottom"))
+
+
+class StateE3D(C.Structure):
+    _fields_ = ([(n, C.c_void_p) for n in ("Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy",
+                                          "L", "L2M", "Mxz", "Myz", "Mxy", "Rx", "Ry", "Rz", "tab")]
+                + [("psi", C.c_void_p * E3D_NSTRIP)])
+
+
+class RssHeader(C.Structure):
+    _fields_ = [("data_format", C.c_int), ("header_format", C.c_int), ("type", C.c_int), ("nheader", C.c_size_t),
+                ("ndims", C.c_size_t), ("n", C.c_size_t * 9), ("d", C.c_double * 9), ("o", C.c_double * 9),
+                ("start", C.c_size_t)]
+
+
+class Image2DDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("nhx", C.c_int), ("nhz", C.c_int), ("dx", C.c_float),
+                ("dz", C.c_float), ("ox", C.c_float), ("oz", C.c_float)]
+
+
+RSS_GENERIC, RSS_REGULAR, RSS_DATA2D, RSS_DATA3D = 0, 1, 2, 3
 FWI_REVOLVE, FWI_STORE, FWI_FULL = 0, 1, 2
 ADJOINT_REFERENCE, ADJOINT_EXACT = 0, 1
 
@@ -195,6 +220,19 @@ def _declare(L):
         "rsg_stagger_e2d_dev": ([vp, vp, vp, ip, ip, ip, ip, ip, vp, vp, vp, vp, vp, vp], ip),
         "rsg_misfit_residuals_noalloc": ([ip, vp, vp, vp, vp, vp, ip, ip, ip, ip, ip, C.POINTER(C.c_double), vp],
                                          ip),
+        "rsg_stagger_e3d": ([vp, vp, vp, ip, ip, ip, ip, ip] + [vp] * 8, ip),
+        "rsg_e3d_forward_velocity": ([GP, C.POINTER(StateE3D), vp], ip),
+        "rsg_e3d_forward_stress": ([GP, C.POINTER(StateE3D), vp], ip),
+        "rsg_e3d_insert_source": ([GP, C.POINTER(StateE3D), TP, ip, vp], ip),
+        "rsg_e3d_record_data": ([GP, C.POINTER(StateE3D), TP, vp, vp, vp, ip, vp], ip),
+        "rsg_rss_read_header": ([C.c_char_p, C.POINTER(RssHeader)], ip),
+        "rsg_rss_read_data2d": ([C.c_char_p, vp, vp, ip, ip, vp], ip),
+        "rsg_rss_read_data3d": ([C.c_char_p, vp, vp, ip, ip, vp], ip),
+        "rsg_rss_write_data2d": ([C.c_char_p, vp, vp, ip, ip, fp, fp, vp], ip),
+        "rsg_rss_write_data3d": ([C.c_char_p, vp, vp, ip, ip, fp, vp], ip),
+        "rsg_rss_read_regular": ([C.c_char_p, vp, C.c_size_t, vp], ip),
+        "rsg_rss_write_regular": ([C.c_char_p, C.POINTER(RssHeader), vp, vp], ip),
+        "rsg_stack_image_2d": ([vp, C.POINTER(Image2DDesc), vp, C.POINTER(Image2DDesc), ip, ip, ip, ip, vp], ip),
     }
     for name in ("a2d_forward_velocity", "a2d_forward_stress", "e2d_forward_velocity",
                  "e2d_forward_stress", "e2d_backward_velocity", "e2d_backward_stress",
@@ -226,6 +264,10 @@ EXPORTS = (
     "rsg_revolve_adjust", "rsg_revolve_init", "rsg_revolve_next",
     "rsg_modelling_plan_create", "rsg_modelling_plan_run", "rsg_modelling_plan_destroy",
     "rsg_stagger_a2d_dev", "rsg_stagger_e2d_dev", "rsg_misfit_residuals_noalloc",
+    "rsg_rss_read_header", "rsg_rss_read_data2d", "rsg_rss_read_data3d", "rsg_rss_write_data2d",
+    "rsg_rss_write_data3d", "rsg_rss_read_regular", "rsg_rss_write_regular", "rsg_stack_image_2d",
+    "rsg_stagger_e3d", "rsg_e3d_forward_velocity", "rsg_e3d_forward_stress", "rsg_e3d_insert_source",
+    "rsg_e3d_record_data",
 )
 
 

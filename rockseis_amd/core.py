@@ -253,6 +253,28 @@ class ModelAcoustic3D(_Model):
         self._upload(("L", "Rx", "Ry", "Rz"))
 
 
+class ModelElastic3D(ModelAcoustic3D):
+    """ModelElastic3D (model.h:441-470): Vp, Vs, R [nz][ny][nx];
+    staggerModels gives L, L2M, M_xz, M_yz, M_xy, Rx, Ry, Rz
+    (model.cpp:3371-3473, bit-exact, rsg_stagger_e3d)."""
+
+    def __init__(self, nx, ny, nz, lpml, dx, dy, dz, ox=0.0, oy=0.0, oz=0.0, fs=False):
+        super().__init__(nx, ny, nz, lpml, dx, dy, dz, ox, oy, oz, fs)
+        self.Vs = np.zeros((nz, ny, nx), np.float32)
+
+    _E3D_MODEL = ("L", "L2M", "Mxz", "Myz", "Mxy", "Rx", "Ry", "Rz")
+
+    def staggerModels(self):
+        shp = (self.getNz_pml(), self.getNy_pml(), self.getNx_pml())
+        for n in self._E3D_MODEL:
+            setattr(self, n, np.zeros(shp, np.float32))
+        vp, vs, rho = (np.ascontiguousarray(a, np.float32) for a in (self.Vp, self.Vs, self.R))
+        R.check(R.lib().rsg_stagger_e3d(_np_ptr(vp), _np_ptr(vs), _np_ptr(rho), self.nx, self.ny, self.nz,
+                                        self.lpml, int(self.fs), *(_np_ptr(getattr(self, n))
+                                                                   for n in self._E3D_MODEL)), "staggerModels")
+        self._upload(self._E3D_MODEL + ("Vp", "Vs", "R"))
+
+
 # --------------------------------------------------------------------------
 # Data (traces + maps)
 # --------------------------------------------------------------------------
@@ -319,6 +341,49 @@ class Data2D:
             keep.append(f)
             t.factor = f.data_ptr()
         return t, keep
+
+
+    # File input / output (Data2D::read / write, data.cpp:463-634; the RSS
+    # layout of file.cpp) through the device: rockseis_amd.rss.
+    def setFile(self, path):
+        self.file = str(path)
+
+    def getFile(self):
+        return getattr(self, "file", "")
+
+    @classmethod
+    def fromFile(cls, path, field="P"):
+        """Data2D(datafile): sizes, sampling and traces from the file."""
+        from . import rss
+        coords, data, h = rss.read_traces_device(path, cls.dim)
+        d = cls(data.shape[0], data.shape[1], h["d"][0], h["o"][0], field)
+        d._take(coords, data)
+        d.setFile(path)
+        return d
+
+    def _take(self, coords, data):
+        k = self.dim
+        self.scoords[:] = coords[:, :k]
+        self.gcoords[:] = coords[:, k:]
+        self.device_data = data
+        self.data[:] = data.cpu().numpy()
+
+    def read(self):
+        """Traces and coordinates from the file set by setFile (sizes must
+        match, as in the reference); the samples also stay on the device as
+        device_data."""
+        from . import rss
+        coords, data, h = rss.read_traces_device(self.getFile(), self.dim)
+        if data.shape != (self.ntrace, self.nt):
+            raise R.RsgError(f"Data{self.dim}D::read: trace count or nt differs from the file's")
+        self._take(coords, data)
+        return True
+
+    def write(self):
+        from . import rss
+        coords = np.concatenate([self.scoords, self.gcoords], axis=1)
+        rss.write_traces_device(self.getFile(), coords, self.data, self.dt, self.ot, self.dim)
+        return True
 
 
 class Data3D(Data2D):
@@ -543,6 +608,166 @@ class WavesAcoustic3D(_Waves):
 
     def getNz_pml(self):
         return self.nz_pml
+
+
+class WavesElastic3D:
+    """WavesElastic3D (waves.h:215-250): the nine device fields Vx, Vy, Vz,
+    Sxx, Syy, Szz, Syz, Sxz, Sxy and the 36 PML strips of PmlElastic3D on
+    the padded grid, stepped by rsg_e3d_forward_velocity / _stress."""
+
+    FIELDS = ("Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy")
+
+    def __init__(self, model: ModelElastic3D, nt, dt, ot=0.0):
+        self.nt, self.dt, self.ot = int(nt), float(dt), float(ot)
+        self.lpml, self.fs = model.lpml, model.fs
+        self.nx_pml, self.ny_pml, self.nz_pml = model.getNx_pml(), model.getNy_pml(), model.getNz_pml()
+        self.dx, self.dy, self.dz = model.dx, model.dy, model.dz
+        self._model = model
+        dev = _dev()
+        nxp, nyp, nzp, L = self.nx_pml, self.ny_pml, self.nz_pml, self.lpml
+        for f in self.FIELDS:
+            setattr(self, f, torch.zeros((nzp, nyp, nxp), dtype=torch.float32, device=dev))
+        face = {0: (nzp, nyp, L), 1: (nzp, L, nxp), 2: (L, nyp, nxp)}
+        self.psi = [torch.zeros(face[i // 12], dtype=torch.float32, device=dev) for i in range(R.E3D_NSTRIP)]
+        self.Pml = Pml(L, dt)
+
+    def getPml(self):
+        return self.Pml
+
+    def getNx_pml(self):
+        return self.nx_pml
+
+    def getNy_pml(self):
+        return self.ny_pml
+
+    def getNz_pml(self):
+        return self.nz_pml
+
+    def _grid(self, order):
+        g = R.Grid()
+        g.nx, g.ny, g.nz = self.nx_pml, self.ny_pml, self.nz_pml
+        g.lpml, g.order, g.fs = self.lpml, int(order), int(self.fs)
+        g.dx, g.dy, g.dz, g.dt = self.dx, self.dy, self.dz, self.dt
+        return g
+
+    def _state(self, model):
+        s = R.StateE3D()
+        for f in self.FIELDS:
+            setattr(s, f, getattr(self, f).data_ptr())
+        for n in ModelElastic3D._E3D_MODEL:
+            setattr(s, n, getattr(model, "_d" + n).data_ptr())
+        if self.Pml.tab is None:
+            self.Pml.computeABC()
+        s.tab = self.Pml.tab.data_ptr()
+        for i, t in enumerate(self.psi):
+            s.psi[i] = t.data_ptr()
+        return s
+
+    def forwardstepVelocity(self, model, der):
+        R.check(R.lib().rsg_e3d_forward_velocity(C.byref(self._grid(der.order)), C.byref(self._state(model)),
+                                                 _stream()), "rsg_e3d_forward_velocity")
+
+    def forwardstepStress(self, model, der):
+        R.check(R.lib().rsg_e3d_forward_stress(C.byref(self._grid(der.order)), C.byref(self._state(model)),
+                                               _stream()), "rsg_e3d_forward_stress")
+
+    def getVx(self):
+        return self.Vx
+
+    def getVy(self):
+        return self.Vy
+
+    def getVz(self):
+        return self.Vz
+
+    def getSxx(self):
+        return self.Sxx
+
+    def getSyy(self):
+        return self.Syy
+
+    def getSzz(self):
+        return self.Szz
+
+    def getSyz(self):
+        return self.Syz
+
+    def getSxz(self):
+        return self.Sxz
+
+    def getSxy(self):
+        return self.Sxy
+
+
+class ModellingElastic3D:
+    """ModellingElastic3D::run (modelling.cpp:727-): the elastic 3D time loop,
+    velocity step, stress step, insertSource(it), recordData(it) of every
+    record set, on the device; records are copied to the host Data3D
+    objects at the end."""
+
+    def __init__(self, model: ModelElastic3D, source: Data3D, order=4, snapinc=1):
+        self.model, self.source = model, source
+        self.order = int(order) if 1 < int(order) < 9 else 4
+        self.snapinc = max(int(snapinc), 1)
+        self.recs = {}
+        self.waves = None
+
+    def getOrder(self):
+        return self.order
+
+    def setRecP(self, d):
+        self.recs["P"] = d
+
+    def setRecVx(self, d):
+        self.recs["VX"] = d
+
+    def setRecVy(self, d):
+        self.recs["VY"] = d
+
+    def setRecVz(self, d):
+        self.recs["VZ"] = d
+
+    def checkStability(self):
+        # modelling.cpp:710-724
+        m = self.model
+        dx, dy, dz = np.float32(m.dx), np.float32(m.dy), np.float32(m.dz)
+        dt = np.float32(self.source.dt)
+        vpmax = np.float32(np.max(m.Vp))
+        dt_stab = 2.0 / (3.1415 * np.sqrt((1.0 / (dx * dx)) + (1 / (dy * dy)) + (1 / (dz * dz))) * vpmax)
+        return bool(dt < np.float32(dt_stab))
+
+    def run(self):
+        if not self.checkStability():
+            raise R.RsgError("Wavelet sampling interval (dt) does not match the stability criteria.")
+        m, src = self.model, self.source
+        if not hasattr(m, "_dL2M"):
+            m.staggerModels()
+        w = WavesElastic3D(m, src.nt, src.dt, src.ot)
+        self.waves = w
+        w.getPml().setSmax(SMAX)
+        w.getPml().computeABC()
+        g = w._grid(self.order)
+        s = w._state(m)
+        t_src, keep = src._device(SMAP)
+        recs = []
+        for f, d in self.recs.items():
+            if d.field != _field(f):
+                d.setField(f)
+            t, k = d._device(GMAP, zero=True)
+            recs.append((d, t, k))
+        L = R.lib()
+        st = _stream()
+        vp, vs, rho = (getattr(m, "_d" + n).data_ptr() for n in ("Vp", "Vs", "R"))
+        for it in range(src.nt):
+            R.check(L.rsg_e3d_forward_velocity(C.byref(g), C.byref(s), st), "rsg_e3d_forward_velocity")
+            R.check(L.rsg_e3d_forward_stress(C.byref(g), C.byref(s), st), "rsg_e3d_forward_stress")
+            R.check(L.rsg_e3d_insert_source(C.byref(g), C.byref(s), C.byref(t_src), it, st), "insertSource")
+            for d, t, k in recs:
+                R.check(L.rsg_e3d_record_data(C.byref(g), C.byref(s), C.byref(t), vp, vs, rho, it, st),
+                        "recordData")
+        for d, t, k in recs:
+            d.data[...] = k[2].cpu().numpy()
+        return 1
 
 
 LINEAR, BSPLINE, SINC = 0, 1, 2  # rs_interpmode (utils.h:25)

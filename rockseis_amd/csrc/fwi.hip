@@ -193,7 +193,14 @@ __global__ void crosscorr_a2d_kernel(const void *__restrict__ Pf_, const void *_
     const float vpscale = (float)(-2.0 / (double)Vp[m]);
     const float rhoscale = (float)(-1.0 / (double)Rho[m]);
     const double rx0 = Rx[q], rx1 = Rx[q - 1], rz0 = Rz[q], rz1 = Rz[q - nxp];
+    // the reference's x / (double)dx as x * (1 / (double)dx): within one
+    // double ulp, rounded to float afterwards
+    const double idx = 1.0 / (double)dx, idz = 1.0 / (double)dz;
     float vg = vpgrad[m], rg = rhograd[m], si = srcilum ? srcilum[m] : 0.f;
+    // unrolled so that several shots' loads are in flight at once (the S16
+    // decode would spill if unrolled); the gradients still add shot by shot
+    constexpr int kUnroll = STORE == kStoreS16 ? 1 : 4;
+#pragma unroll kUnroll
     for (int k = 0; k < nshot; k++) {
         float pc, pe, pw, ps, pn;
         if constexpr (STORE == kStoreS16) {
@@ -226,10 +233,10 @@ __global__ void crosscorr_a2d_kernel(const void *__restrict__ Pf_, const void *_
         const float mrzz = (Vz[q] - Vz[q - nxp]) / dz;
         vg -= vpscale * pc * (mrxx + mrzz);
         rg -= rhoscale * pc * (mrxx + mrzz);
-        float ux = (float)(0.5 * (double)Vx[q] * rx0 * (double)(pe - pc) / (double)dx);
-        ux = (float)((double)ux + 0.5 * (double)Vx[q - 1] * rx1 * (double)(pc - pw) / (double)dx);
-        float uz = (float)(0.5 * (double)Vz[q] * rz0 * (double)(ps - pc) / (double)dz);
-        uz = (float)((double)uz + 0.5 * (double)Vz[q - nxp] * rz1 * (double)(pc - pn) / (double)dz);
+        float ux = (float)(0.5 * (double)Vx[q] * rx0 * (double)(pe - pc) * idx);
+        ux = (float)((double)ux + 0.5 * (double)Vx[q - 1] * rx1 * (double)(pc - pw) * idx);
+        float uz = (float)(0.5 * (double)Vz[q] * rz0 * (double)(ps - pc) * idz);
+        uz = (float)((double)uz + 0.5 * (double)Vz[q - nxp] * rz1 * (double)(pc - pn) * idz);
         rg += ux + uz;
         if (srcilum) si -= vpscale * pc * pc;
     }

@@ -14,6 +14,7 @@
 #include <utility>
 #include <vector>
 
+#include "rsg_adjoint.h"
 #include "rsg_common.h"
 #include "rsg_fwi.h"
 #include "rsg_inject_ordered.h"
@@ -112,11 +113,19 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
     if (store != kStoreF32 && store != kStoreF16 && store != kStoreS16)
         return fail(RSG_EINVAL, "fwi_a3d: checkpoint storage must be 0 (FP32), 1 (FP16) or 2 (S16)");
     const bool f16 = store != kStoreF32;  // 2 bytes a value
+    const bool exact = f->adjoint == RSG_ADJOINT_EXACT;
+    if (f->adjoint != RSG_ADJOINT_REFERENCE && !exact)
+        return fail(RSG_EINVAL, "fwi_a3d: adjoint must be RSG_ADJOINT_REFERENCE or RSG_ADJOINT_EXACT");
+    if (exact && f->misfit_type != 0 && f->misfit_type != 4)
+        return fail(RSG_EINVAL, "fwi_a3d: the exact adjoint takes the DIFFERENCE misfit");
+    if (exact && g->domdec)
+        return fail(RSG_EINVAL, "fwi_a3d: the exact adjoint takes models padded by the library (no domain model)");
     Revolve rev(nt, f->ncheck);
     {
         size_t free_b = 0, total_b = 0;
         RSG_HIP(hipMemGetInfo(&free_b, &total_b));
-        const size_t need = 2 * total * 4 + total * (f16 ? 2 : 4) * (size_t)rev.snaps;
+        const size_t need = 2 * total * 4 + total * (f16 ? 2 : 4) * (size_t)rev.snaps +
+                            (exact ? n * (7 * 4 + 2 * 8) + total * 4 : 0);
         if (need + ((size_t)1 << 30) > free_b)
             return fail(RSG_ENOMEM, "fwi_a3d: wavefields and checkpoints do not fit in device memory");
     }
@@ -194,6 +203,33 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
     if (f->obs.ntrace > 0)
         if (int rc = oi.build(dm, f->obs.map, f->obs.ntrace, g->nx, g->ny, g->nz, map_pad(g), st)) return rc;
     const dim3 xb(64, 4), xg((f->nx - 1 + 63) / 64, (f->ny - 1 + 3) / 4, std::max(f->nz - 1, 1));
+    // exact adjoint: work arrays, and the state before the first turn's
+    // forward step (which runs first for the records)
+    A3DExactBufs ex{};
+    float *turn_state = nullptr;
+    if (exact) {
+        float *w = nullptr;
+        if (int rc = dm.alloc(&w, 7 * n)) return rc;
+        if (int rc = launch_zero(w, 7 * n * sizeof(float), st)) return rc;
+        for (int a = 0; a < 3; a++) {
+            ex.W[a] = w + a * n;
+            ex.gR[a] = w + (4 + a) * n;
+        }
+        ex.gL = w + 3 * n;
+        if (int rc = dm.alloc(&ex.gvp, n)) return rc;
+        if (int rc = dm.alloc(&ex.grho, n)) return rc;
+        if (int rc = dm.alloc(&turn_state, total)) return rc;
+    }
+    auto exact_step = [&](int capo) -> int {
+        if (int rc = rsg_a3d_forward_velocity(g, &sf, st)) return rc;
+        if (oi.ncells > 0) {  // the transposed gather: weight 1
+            inject3d_ordered_kernel<<<(oi.ncells + 63) / 64, 64, 0, st>>>(sb.P, nullptr, 1.f, oi.cells, oi.starts,
+                                                                          oi.ents, oi.ncells, res.shift, res.data,
+                                                                          nt, capo);
+            RSG_LAUNCH_CHECK("inject3d_ordered_kernel");
+        }
+        return a3d_exact_reverse(g, &sf, &sb, ex, &f->src, f->wavgrad, capo, map_pad(g), st);
+    };
 
     auto fstep = [&](int it, bool rec) -> int {
         if (int rc = rsg_a3d_forward_velocity(g, &sf, st)) return rc;
@@ -226,12 +262,22 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
             for (int it = oldcapo; it < capo; it++)
                 if (int rc = fstep(it, !reverse)) return rc;
         } else if (a == Revolve::FIRSTURN) {
+            if (exact)
+                RSG_HIP(hipMemcpyAsync(turn_state, fwb, total * sizeof(float), hipMemcpyDeviceToDevice, st));
             if (int rc = fstep(capo, true)) return rc;
             if (int rc = acoustic_misfit_residuals(f->misfit_type, f->dmod, f->obs.data, f->weight, f->dres,
                                                    f->obs.map, 3, f->obs.ntrace, nt, 1, 1, dmis, st))
                 return rc;
-            if (int rc = reverse_tail(capo)) return rc;
+            if (exact) {
+                RSG_HIP(hipMemcpyAsync(fwb, turn_state, total * sizeof(float), hipMemcpyDeviceToDevice, st));
+                if (int rc = exact_step(capo)) return rc;
+            } else if (int rc = reverse_tail(capo)) {
+                return rc;
+            }
             reverse = true;
+        } else if (a == Revolve::YOUTURN && exact) {
+            // fw is left half stepped: Revolve restores before the next step
+            if (int rc = exact_step(capo)) return rc;
         } else if (a == Revolve::YOUTURN) {
             if (int rc = rsg_a3d_forward_velocity(g, &sb, st)) return rc;
             if (int rc = rsg_a3d_forward_stress(g, &sb, st)) return rc;
@@ -266,6 +312,9 @@ static int fwi_a3d_impl(const rsg_grid *g, const rsg_fwi_a3d *f, hipStream_t st)
             return fail(RSG_EINVAL, "fwi_a3d: Revolve returned an error");
         }
     }
+    if (exact)
+        if (int rc = a3d_exact_model_grad(g, f->nx, f->ny, f->nz, f->Vp, f->Rho, ex, f->vpgrad, f->rhograd, st))
+            return rc;
     double m = 0.0;
     RSG_HIP(hipMemcpyAsync(&m, dmis, sizeof(double), hipMemcpyDeviceToHost, st));
     RSG_HIP(hipStreamSynchronize(st));

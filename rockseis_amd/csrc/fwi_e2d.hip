@@ -20,6 +20,7 @@
 #include <memory>
 #include <vector>
 
+#include "rsg_adjoint.h"
 #include "rsg_common.h"
 #include "rsg_fwi.h"
 #include "rsg_ragged.h"
@@ -39,6 +40,7 @@ struct ShotDev {
     float *fVx, *fVz, *fSxx, *fSzz, *fSxz;
     float *bVx, *bVz, *bSxx, *bSzz, *bSxz;
     const float *Rx, *Rz, *Vp, *Vs, *Rho;
+    float *gRx, *gRz;  // exact adjoint: buoyancy gradients (force sources add to them)
     int sfield, nsrc, nrec;
     const int *smap;
     const float *sshift, *wav;
@@ -96,7 +98,10 @@ __global__ void e2d_record_kernel(const ShotDev *__restrict__ sd, int it, int nt
 // cells.  Neighbouring receivers share corner cells (the staggered Ux / Uz
 // positions fall between grid nodes), so one thread per touched cell
 // (blockIdx.z = component) adds the cell's contributions in trace order, the
-// reference's order, and the result does not depend on scheduling.
+// reference's order, and the result does not depend on scheduling.  EXACT:
+// the transpose of the gathers instead (the exact adjoint), P with 1/2 into
+// Sxx and Szz, Ux and Uz with weight 1.
+template <bool EXACT>
 __global__ void e2d_residual_kernel(const ShotDev *__restrict__ sd, int it, int nt, float dt) {
     const ShotDev &s = sd[blockIdx.y];
     const int c = blockIdx.z;
@@ -107,7 +112,7 @@ __global__ void e2d_residual_kernel(const ShotDev *__restrict__ sd, int it, int 
     float *F = c == 0 ? s.bSxx : (c == 1 ? s.bVx : s.bVz);
     const float *Mod = c == 0 ? nullptr : (c == 1 ? s.Rx : s.Rz);
     const float scale = c == 0 ? -dt : dt;
-    const float m = Mod ? scale * Mod[p] : scale;
+    const float m = EXACT ? (c == 0 ? 0.5f : 1.f) : (Mod ? scale * Mod[p] : scale);
     float a = F[p], b = c == 0 ? s.bSzz[p] : 0.f;
     for (int q = s.rstart[c][j]; q < s.rstart[c][j + 1]; q++) {
         const int e = s.rent[c][q], i = e >> 2, i1 = (e >> 1) & 1, i2 = e & 1;
@@ -119,6 +124,36 @@ __global__ void e2d_residual_kernel(const ShotDev *__restrict__ sd, int it, int 
     }
     F[p] = a;
     if (c == 0) s.bSzz[p] = b;
+}
+
+// Exact adjoint of insertSource at step it: the wavelet gradient (PRESSURE:
+// -dt sum w (lSxx + lSzz); VX / VZ: dt sum w R lV) and, for a force source,
+// dt wav w lV into the buoyancy gradient (atomic: traces may share cells).
+__global__ void e2d_exact_source_kernel(const ShotDev *__restrict__ sd, int it, int nt, float dt) {
+    const ShotDev &s = sd[blockIdx.y];
+    const int i = blockIdx.x * kIoThreads + threadIdx.x;
+    if (i >= s.nsrc) return;
+    const int mx = s.smap[2 * i], mz = s.smap[2 * i + 1];
+    if (mx < 0 || mz < 0) return;
+    const float sx = s.sshift[2 * i], sz = s.sshift[2 * i + 1];
+    const float xtri[2] = {1 - sx, sx}, ytri[2] = {1 - sz, sz};
+    const float w = s.wav[(size_t)i * nt + it];
+    float acc = 0.f;
+    for (int i1 = 0; i1 < 2; i1++)
+        for (int i2 = 0; i2 < 2; i2++) {
+            const size_t q = (size_t)(s.lpml + mz + i1) * s.nxp + (s.lpml + mx + i2);
+            if (q >= (size_t)s.nxp * s.nzp) continue;  // as the forward injection skips it
+            const float wt = xtri[i2] * ytri[i1];
+            if (s.sfield == RSG_PRESSURE) {
+                acc -= dt * wt * (s.bSxx[q] + s.bSzz[q]);
+            } else {
+                const bool vx = s.sfield == RSG_VX;
+                const float l = vx ? s.bVx[q] : s.bVz[q];
+                acc += dt * wt * (vx ? s.Rx[q] : s.Rz[q]) * l;
+                unsafeAtomicAdd((vx ? s.gRx : s.gRz) + q, dt * w * wt * l);
+            }
+        }
+    if (s.wavgrad) s.wavgrad[(size_t)i * nt + it] += acc;
 }
 
 // Wavelet gradient of crossCorr: the adjoint field of the source type
@@ -236,6 +271,11 @@ struct PlanGuard {
     ~PlanGuard() { ragged2d_destroy(&p); }
 };
 
+struct ExactGuard {
+    E2DExact p;
+    ~ExactGuard() { e2d_exact_destroy(&p); }
+};
+
 // Is strip slot i of the elastic state an x-side strip (lpml x nz)?
 inline bool e2d_xstrip(int i) { return (i / 4) % 2 == 0; }
 
@@ -282,6 +322,15 @@ struct E2DFwi {
     std::vector<E2DRecs> recs;
     PlanGuard pf, pb;
     dim3 iob_s, iob_r, iob_c, xcg;
+    // exact adjoint: per-shot work arrays (in wbase), reverse-step kernels,
+    // the state before the first turn's forward step, FP64 model-gradient
+    // scratch of the largest padded window
+    bool exact = false;
+    std::vector<E2DExactWork> work;
+    float *wbase = nullptr, *turn_state = nullptr;
+    size_t wtotal = 0;
+    double *pad3 = nullptr;
+    ExactGuard ex;
 
     int init(const rsg_fwi_e2d *f_, const rsg_fwi_e2d_shot *sh_, int ns_, hipStream_t st) {
         f = f_;
@@ -337,6 +386,16 @@ struct E2DFwi {
         segs.push_back(off[ns]);
         total = off[ns];
         nseg = (int)segs.size() - 1;
+        exact = f->adjoint == RSG_ADJOINT_EXACT;
+        if (f->adjoint != RSG_ADJOINT_REFERENCE && !exact)
+            return fail(RSG_EINVAL, "fwi_e2d: adjoint must be RSG_ADJOINT_REFERENCE or RSG_ADJOINT_EXACT");
+        if (exact) {
+            if (f->misfit_type != 0) return fail(RSG_EINVAL, "fwi_e2d: the exact adjoint takes the DIFFERENCE misfit");
+            for (int k = 0; k < ns; k++)
+                if (sh[k].g.domdec)
+                    return fail(RSG_EINVAL, "fwi_e2d: the exact adjoint takes windows padded by the library "
+                                            "(no domain models)");
+        }
         rev.reset(new Revolve(nt, f->ncheck));
         store = f->fp16;
         if (store != kStoreF32 && store != kStoreF16 && store != kStoreS16)
@@ -352,7 +411,12 @@ struct E2DFwi {
                 const int pad = shot_pad(sh[k].g);
                 img += 3 * (size_t)(sh[k].g.nx - 2 * pad) * (sh[k].g.nz - 2 * pad);
             }
-            const size_t need = total * 4 * 2 + img * 4 + total * (f16 ? 2 : 4) * (size_t)rev->snaps;
+            size_t need = total * 4 * 2 + img * 4 + total * (f16 ? 2 : 4) * (size_t)rev->snaps;
+            if (exact) {
+                size_t nf = 0;
+                for (int k = 0; k < ns; k++) nf += (size_t)sh[k].g.nx * sh[k].g.nz;
+                need += nf * 9 * 4 + total * 4;
+            }
             if (need + ((size_t)1 << 30) > free_b)
                 return fail(RSG_ENOMEM, "fwi_e2d: states and checkpoints do not fit in device memory");
         }
@@ -364,6 +428,30 @@ struct E2DFwi {
             img_off[k + 1] = img_off[k] + 3 * (size_t)(sh[k].g.nx - 2 * pad) * (sh[k].g.nz - 2 * pad);
         }
         if (int rc = dm.alloc(&imgs, img_off[ns])) return rc;
+        if (exact) {
+            size_t maxp = 0;
+            for (int k = 0; k < ns; k++) {
+                const size_t n = (size_t)sh[k].g.nx * sh[k].g.nz;
+                wtotal += 9 * n;
+                maxp = std::max(maxp, n);
+            }
+            if (int rc = dm.alloc(&wbase, wtotal)) return rc;
+            if (int rc = dm.alloc(&turn_state, total)) return rc;
+            if (int rc = dm.alloc(&pad3, 3 * maxp)) return rc;
+            work.resize(ns);
+            float *q = wbase;
+            for (int k = 0; k < ns; k++) {
+                const size_t n = (size_t)sh[k].g.nx * sh[k].g.nz;
+                E2DExactWork &w = work[k];
+                for (int i = 0; i < 4; i++, q += n) w.W[i] = q;
+                w.gL = q;
+                w.gL2M = q + n;
+                w.gM = q + 2 * n;
+                w.gRx = q + 3 * n;
+                w.gRz = q + 4 * n;
+                q += 5 * n;
+            }
+        }
         // ---- per-shot states, ragged plans, kernel descriptors
         std::vector<rsg_grid> grids(ns);
         std::vector<rsg_state> sf(ns), sb(ns);
@@ -414,6 +502,8 @@ struct E2DFwi {
             d.bSxz = sb[k].Sxz;
             d.Rx = s.Rx;
             d.Rz = s.Rz;
+            d.gRx = exact ? work[k].gRx : nullptr;
+            d.gRz = exact ? work[k].gRz : nullptr;
             d.Vp = s.Vp;
             d.Vs = s.Vs;
             d.Rho = s.Rho;
@@ -514,6 +604,8 @@ struct E2DFwi {
         }
         if (int rc = ragged2d_create(RSG_KIND_E2D, ns, grids.data(), sf.data(), &pf.p)) return rc;
         if (int rc = ragged2d_create(RSG_KIND_E2D, ns, grids.data(), sb.data(), &pb.p)) return rc;
+        if (exact)
+            if (int rc = e2d_exact_create(ns, grids.data(), sf.data(), sb.data(), work.data(), &ex.p, st)) return rc;
         if (int rc = dm.alloc(&dsd, ns)) return rc;
         if (int rc = dm.alloc(&dmis, 1)) return rc;
         RSG_HIP(hipMemcpyAsync(dsd, hd.data(), sizeof(ShotDev) * ns, hipMemcpyHostToDevice, st));
@@ -568,9 +660,25 @@ struct E2DFwi {
         return RSG_OK;
     }
 
+    // One exact reverse step at time it: fw holds the forward state before
+    // step it (advanced here by its velocity half step), bw the adjoint
+    // state after it.
+    int exact_step(int it, hipStream_t st) {
+        if (int rc = ragged2d_step(&pf.p, 0, st)) return rc;
+        if (maxrec) {
+            e2d_residual_kernel<true><<<iob_c, kIoThreads, 0, st>>>(dsd, it, nt, dt);
+            RSG_LAUNCH_CHECK("e2d_residual_kernel");
+        }
+        if (maxsrc) {
+            e2d_exact_source_kernel<<<iob_s, kIoThreads, 0, st>>>(dsd, it, nt, dt);
+            RSG_LAUNCH_CHECK("e2d_exact_source_kernel");
+        }
+        return e2d_exact_reverse(ex.p, st);
+    }
+
     int reverse_tail(int it, hipStream_t st) {
         if (maxrec) {
-            e2d_residual_kernel<<<iob_c, kIoThreads, 0, st>>>(dsd, it, nt, dt);
+            e2d_residual_kernel<false><<<iob_c, kIoThreads, 0, st>>>(dsd, it, nt, dt);
             RSG_LAUNCH_CHECK("e2d_residual_kernel");
         }
         e2d_crosscorr_kernel<<<xcg, dim3(64, 4), 0, st>>>(dsd, sh[0].g.dx, sh[0].g.dz);
@@ -590,6 +698,8 @@ struct E2DFwi {
         if (int rc = launch_zero(bwb, total * sizeof(float), st)) return rc;
         if (int rc = launch_zero(imgs, img_off[ns] * sizeof(float), st)) return rc;
         if (int rc = launch_zero(dmis, sizeof(double), st)) return rc;
+        if (exact)
+            if (int rc = launch_zero(wbase, wtotal * sizeof(float), st)) return rc;
         for (int k = 0; k < ns; k++) {
             for (int c = 0; c < 3; c++)
                 if (int rc = launch_zero(sh[k].dmod[c], sizeof(float) * sh[k].obs[0].ntrace * nt, st)) return rc;
@@ -606,6 +716,8 @@ struct E2DFwi {
                 for (int it = oldcapo; it < capo; it++)
                     if (int rc = fstep(it, true, st)) return rc;
             } else if (a == Revolve::FIRSTURN) {
+                if (exact)  // the exact reverse step starts from the state before this step
+                    RSG_HIP(hipMemcpyAsync(turn_state, fwb, total * sizeof(float), hipMemcpyDeviceToDevice, st));
                 if (int rc = fstep(capo, true, st)) return rc;
                 *turn = capo;
                 return RSG_OK;
@@ -636,7 +748,12 @@ struct E2DFwi {
     // Phase B: the first turn's adjoint terms, the rest of the Revolve loop,
     // then every window stacked into the caller's images.
     int phase_b(int turn, hipStream_t st) {
-        if (int rc = reverse_tail(turn, st)) return rc;
+        if (exact) {
+            RSG_HIP(hipMemcpyAsync(fwb, turn_state, total * sizeof(float), hipMemcpyDeviceToDevice, st));
+            if (int rc = exact_step(turn, st)) return rc;
+        } else if (int rc = reverse_tail(turn, st)) {
+            return rc;
+        }
         reverse = true;
         for (;;) {
             const int oldcapo = rev->capo;
@@ -645,6 +762,9 @@ struct E2DFwi {
             if (a == Revolve::ADVANCE) {
                 for (int it = oldcapo; it < capo; it++)
                     if (int rc = fstep(it, !reverse, st)) return rc;
+            } else if (a == Revolve::YOUTURN && exact) {
+                // fw is left half stepped: Revolve restores before the next step
+                if (int rc = exact_step(capo, st)) return rc;
             } else if (a == Revolve::YOUTURN) {
                 if (int rc = ragged2d_step(&pb.p, 2, st)) return rc;
                 if (int rc = ragged2d_step(&pb.p, 3, st)) return rc;
@@ -670,8 +790,18 @@ struct E2DFwi {
                 return fail(RSG_EINVAL, "fwi_e2d: Revolve returned an error");
             }
         }
+        if (exact)  // each window's gradients into its images, already in the requested parameters
+            for (int k = 0; k < ns; k++) {
+                const int nx = sh[k].g.nx - 2 * lpml, nz = sh[k].g.nz - 2 * lpml;
+                float *o = imgs + img_off[k];
+                const size_t ni = (size_t)nx * nz;
+                if (int rc = e2d_exact_model_grad(work[k], nx, nz, lpml, sh[k].g.fs, f->scale, sh[k].Vp, sh[k].Vs,
+                                                  sh[k].Rho, pad3, o, o + ni, o + 2 * ni, st))
+                    return rc;
+            }
         e2d_stack_kernel<<<dim3((f->nx + 255) / 256, f->nz), 256, 0, st>>>(dsd, ns, f->nx, f->nz, f->grad[0],
-                                                                             f->grad[1], f->grad[2], f->scale);
+                                                                             f->grad[1], f->grad[2],
+                                                                             exact ? 0 : f->scale);
         RSG_LAUNCH_CHECK("e2d_stack_kernel");
         return RSG_OK;
     }

@@ -5,6 +5,7 @@
 // promotions follow the reference's expressions so that results match its
 // T=float build to the bit.
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -206,6 +207,73 @@ int rsg_stagger_a3d(const float *vp, const float *rho, int nx, int ny, int nz, i
             Rx[p] *= 2.0f;
             Ry[p] *= 2.0f;
             L[p] *= 0.0f;
+        }
+    return RSG_OK;
+}
+
+int rsg_stagger_e3d(const float *vp, const float *vs, const float *rho, int nx, int ny, int nz, int lpml, int fs,
+                    float *L, float *L2M, float *Mxz, float *Myz, float *Mxy, float *Rx, float *Ry, float *Rz) {
+    if (!vp || !vs || !rho || !L || !L2M || !Mxz || !Myz || !Mxy || !Rx || !Ry || !Rz || nx < 2 || ny < 2 || nz < 2 ||
+        lpml < 0)
+        return rsg::fail(RSG_EINVAL, "rsg_stagger_e3d: bad arguments");
+    const int nxp = nx + 2 * lpml, nyp = ny + 2 * lpml, nzp = nz + 2 * lpml;
+    const size_t nxy = (size_t)nxp * nyp, n = nxy * nzp;
+    // padmodel3d: every padding cell a copy of the nearest model cell
+    auto pad3 = [&](float *o, const float *m) {
+        for (int z = 0; z < nzp; z++) {
+            const int mz = std::min(std::max(z - lpml, 0), nz - 1);
+            for (int y = 0; y < nyp; y++) {
+                const int my = std::min(std::max(y - lpml, 0), ny - 1);
+                const float *row = m + ((size_t)mz * ny + my) * nx;
+                float *orow = o + z * nxy + (size_t)y * nxp;
+                for (int x = 0; x < nxp; x++) orow[x] = row[std::min(std::max(x - lpml, 0), nx - 1)];
+            }
+        }
+    };
+    pad3(Rx, rho);
+    pad3(Ry, rho);
+    pad3(Rz, rho);
+    pad3(L, vp);
+    pad3(Mxz, vs);
+    for (size_t p = 0; p < n; p++) {
+        const float v = L[p], w = Mxz[p], r = Rx[p];
+        L2M[p] = r * v * v;
+        L[p] = (float)((double)(r * v * v) - 2.0 * (double)r * (double)w * (double)w);
+        Mxz[p] = r * w * w;
+        Myz[p] = r * w * w;
+        Mxy[p] = r * w * w;
+    }
+    if (fs)
+        for (size_t xy = 0; xy < nxy; xy++) {
+            L[(size_t)lpml * nxy + xy] = 0.0f;
+            L2M[(size_t)lpml * nxy + xy] = Mxz[(size_t)lpml * nxy + xy];
+        }
+    auto sx = [&](float *m) {
+        for (int z = 0; z < nzp; z++)
+            for (int y = 0; y < nyp; y++) stagger_line(m + z * nxy + (size_t)y * nxp, 1, nxp);
+    };
+    auto sy = [&](float *m) {
+        for (int z = 0; z < nzp; z++)
+            for (int x = 0; x < nxp; x++) stagger_line(m + z * nxy + x, nxp, nyp);
+    };
+    auto sz = [&](float *m) {
+        for (size_t xy = 0; xy < nxy; xy++) stagger_line(m + xy, nxy, nzp);
+    };
+    sx(Rx);
+    sy(Ry);
+    sz(Rz);
+    sx(Mxz);
+    sz(Mxz);
+    sy(Myz);
+    sz(Myz);
+    sx(Mxy);
+    sy(Mxy);
+    if (!invert(Rx, n) || !invert(Ry, n) || !invert(Rz, n))
+        return rsg::fail(RSG_EINVAL, "staggerModels: Zero density found.");
+    if (fs)
+        for (size_t xy = 0; xy < nxy; xy++) {
+            Rx[(size_t)lpml * nxy + xy] *= 2.0f;
+            Ry[(size_t)lpml * nxy + xy] *= 2.0f;
         }
     return RSG_OK;
 }

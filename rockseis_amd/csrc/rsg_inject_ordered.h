@@ -171,7 +171,8 @@ struct OrderedInject3D {
     }
 };
 
-// P += dt * L * res * xtri * ytri * ztri, cell by cell in trace order.
+// P += dt * L * res * xtri * ytri * ztri (L null: dt * res * ...), cell by
+// cell in trace order.
 __global__ void inject3d_ordered_kernel(float *P, const float *__restrict__ L, float dt,
                                         const size_t *__restrict__ cells, const int *__restrict__ starts,
                                         const int *__restrict__ ents, int ncells, const float *__restrict__ shift,
@@ -179,7 +180,7 @@ __global__ void inject3d_ordered_kernel(float *P, const float *__restrict__ L, f
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ncells) return;
     const size_t p = cells[j];
-    const float m = dt * L[p];
+    const float m = L ? dt * L[p] : dt;
     float a = P[p];
     for (int q = starts[j]; q < starts[j + 1]; q++) {
         const int e = ents[q], i = e >> 3, i1 = (e >> 2) & 1, i2 = (e >> 1) & 1, i3 = e & 1;

@@ -515,12 +515,18 @@ class ElasticShot:
 
 class ElasticSurvey:
     """Shots of one elastic model plus the gradient settings (order, PML,
-    Revolve snapshots, FP16 checkpoints, misfit type 0..4 as rs_fwimisfit)."""
+    Revolve snapshots, FP16 checkpoints, misfit type 0..4 as rs_fwimisfit).
+    adjoint "reference" (the reference's adjoint-state steps and crossCorr)
+    or "exact" (the exact discrete adjoint of the forward step and of the
+    model staggering: the gradient of the DIFFERENCE misfit, with scale=False
+    the gradients of lambda / mu / rho as independent parameters)."""
 
     def __init__(self, shots, lpml, dx, dz, dt, order=8, fs=False, ncheck=50, fp16=False, ox=0.0, oz=0.0,
-                 misfit_type=DIFFERENCE):
+                 misfit_type=DIFFERENCE, adjoint="reference"):
         if int(misfit_type) not in _ELASTIC_MISFITS:
             raise R.RsgError(f"ElasticSurvey: misfit type {misfit_type} is not one of rs_fwimisfit's 0..4")
+        _adjoint_kind(adjoint)
+        self.adjoint = adjoint
         self.misfit_type = int(misfit_type)
         self.shots = list(shots)
         self.lpml, self.dx, self.dz, self.dt = int(lpml), float(dx), float(dz), float(dt)
@@ -616,6 +622,7 @@ class _E2DBatch:
         for c in range(3):
             f.grad[c] = self.grads[c].data_ptr()
         f.misfit_type = survey.misfit_type
+        f.adjoint = _adjoint_kind(getattr(survey, "adjoint", "reference"))
         self.f = f
 
     def tune(self, reps=5):
@@ -848,7 +855,10 @@ def elastic_misfit(vp, vs, rho, survey):
 class FwiAcoustic3D:
     """FwiAcoustic3D(model, source, dataP, order, snapinc) with the
     reference's setters and run_optimal() (Revolve checkpoints in HBM, FP32
-    or, with setFp16Storage(True), FP16 with one scale per array)."""
+    or, with setFp16Storage(True), FP16 with one scale per array).
+    setAdjoint("exact") replaces the reference's adjoint by the exact
+    discrete adjoint of the step and of the model staggering (DIFFERENCE
+    misfit only)."""
 
     def __init__(self, model: ModelAcoustic3D, source: Data3D, dataP: Data3D, order=4, snapinc=1):
         self.model, self.source, self.dataP = model, source, dataP
@@ -858,6 +868,7 @@ class FwiAcoustic3D:
         self.datamodP = self.dataresP = self.vpgrad = self.rhograd = self.wavgrad = None
         self.misfit = 0.0
         self.misfit_type, self.dataweight = DIFFERENCE, None
+        self.adjoint = "reference"
 
     def setDatamodP(self, d):
         self.datamodP = d
@@ -890,6 +901,12 @@ class FwiAcoustic3D:
 
     def setDataweight(self, d):
         self.dataweight = d
+
+    def setAdjoint(self, kind):
+        """"reference" (the reference's gradient) or "exact" (the exact
+        discrete adjoint; DIFFERENCE only)."""
+        _adjoint_kind(kind)
+        self.adjoint = kind
 
     def getMisfit(self):
         return self.misfit
@@ -929,6 +946,7 @@ class FwiAcoustic3D:
         f.misfit = C.pointer(misfit)
         f.ncheck, f.fp16 = self.ncheck, int(self.fp16)
         f.misfit_type = self.misfit_type
+        f.adjoint = _adjoint_kind(self.adjoint)
         if self.dataweight is not None:
             wt = torch.from_numpy(np.ascontiguousarray(self.dataweight.data, np.float32)).to(dev)
             f.weight = wt.data_ptr()

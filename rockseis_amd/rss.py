@@ -19,6 +19,14 @@ Snapshots are SNAPSHOT with n4 = the number of snapshots (snap.cpp:26-42).
 
 Reading returns numpy memmaps of the sample block (no copy), so a large
 record or snapshot file is paged in only where it is used.
+
+The device path (read_traces_device, write_traces_device,
+read_regular_device, write_regular_device) goes through the C-ABI
+(rsg_rss_*): one read or write of the whole file through pinned host memory
+and one strided copy, the samples landing on or leaving the GPU.
+stack_image_2d replaces the inversion driver's file stacking
+(Image2D::stackImage_parallel, image.cpp:533-600) with images in device
+memory (rsg_stack_image_2d).
 """
 from __future__ import annotations
 
@@ -191,3 +199,126 @@ def read_image2d(path: str) -> tuple[np.ndarray, Header]:
     h, s = read(path)
     nx, _, nz, nhx, _, nhz = (max(int(v), 1) for v in h.n[:6])
     return np.array(s[: nx * nz * nhx * nhz]).reshape(nhz, nhx, nz, nx), h
+
+
+# --------------------------------------------------------------------------
+# Device path (C-ABI)
+# --------------------------------------------------------------------------
+
+# (the torch and librsgpu imports stay inside these functions, so that the
+# numpy reader above needs neither)
+_KIND = {2: (DATA2D, 4, "rsg_rss_read_data2d", "rsg_rss_write_data2d"),
+         3: (DATA3D, 6, "rsg_rss_read_data3d", "rsg_rss_write_data3d")}
+
+
+def _rt():
+    import ctypes as C
+
+    import torch
+
+    from . import _lib as R
+    return C, torch, R
+
+
+def _stream():
+    C, torch, _ = _rt()
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev():
+    _, torch, _ = _rt()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _header_c(path: str) -> dict:
+    """The file's header: data_format, header_format, type, nheader, ndims,
+    n, d, o (nine each) and the byte offset of the data."""
+    C, torch, R = _rt()
+    h = R.RssHeader()
+    R.check(R.lib().rsg_rss_read_header(str(path).encode(), C.byref(h)), "rsg_rss_read_header")
+    return dict(data_format=h.data_format, header_format=h.header_format, type=h.type, nheader=h.nheader,
+                ndims=h.ndims, n=list(h.n), d=list(h.d), o=list(h.o), start=h.start)
+
+
+def read_traces_device(path: str, dim: int = 2):
+    """A DATA2D (dim 2) or DATA3D (dim 3) file through rsg_rss_read_data2d/3d:
+    (coords [ntrace, 4 or 6] on the host: sx, (sy,) sz, gx, (gy,) gz;
+    samples [ntrace, nt] as a device tensor; header dict)."""
+    C, torch, R = _rt()
+    kind, nhead, fread, _ = _KIND[dim]
+    h = _header_c(path)
+    if h["type"] != kind:
+        raise R.RsgError(f"{path}: not a DATA{dim}D file")
+    nt, ntrace = int(h["n"][0]), int(h["n"][1])
+    coords = np.zeros((ntrace, nhead), np.float32)
+    data = torch.empty((ntrace, nt), dtype=torch.float32, device=_dev())
+    R.check(getattr(R.lib(), fread)(str(path).encode(), coords.ctypes.data, data.data_ptr(), ntrace, nt, _stream()),
+            fread)
+    return coords, data, h
+
+
+def write_traces_device(path: str, coords, data, dt: float, ot: float = 0.0, dim: int = 2):
+    """Data2D<float>::write / Data3D<float>::write through
+    rsg_rss_write_data2d/3d from device samples (the 3D file records o1 = 0,
+    as the reference's)."""
+    C, torch, R = _rt()
+    _, nhead, _, fwrite = _KIND[dim]
+    coords = np.ascontiguousarray(coords, np.float32).reshape(-1, nhead)
+    if not (isinstance(data, torch.Tensor) and data.is_cuda):
+        data = torch.from_numpy(np.ascontiguousarray(data, np.float32)).to(_dev())
+    data = data.to(torch.float32).contiguous()
+    ntrace, nt = data.shape
+    if coords.shape[0] != ntrace:
+        raise R.RsgError("write: one coordinate row per trace")
+    args = [str(path).encode(), coords.ctypes.data, data.data_ptr(), ntrace, nt, float(dt)]
+    if dim == 2:
+        args.append(float(ot))
+    R.check(getattr(R.lib(), fwrite)(*args, _stream()), fwrite)
+
+
+def read_regular_device(path: str):
+    """A REGULAR float file (model, image) as a device tensor, dimensions
+    slowest first (n9 .. n1) with the size-0 dimensions dropped."""
+    C, torch, R = _rt()
+    h = _header_c(path)
+    shape = [int(n) for n in reversed(h["n"]) if n > 0]
+    t = torch.empty(shape or [1], dtype=torch.float32, device=_dev())
+    R.check(R.lib().rsg_rss_read_regular(str(path).encode(), t.data_ptr(), C.c_size_t(t.numel()), _stream()),
+            "rsg_rss_read_regular")
+    return t
+
+
+def write_regular_device(path: str, data, n, d, o, type_=REGULAR):
+    """A REGULAR float file with sizes n, steps d and origins o (up to nine
+    each, n1 first) from a device tensor."""
+    C, torch, R = _rt()
+    h = R.RssHeader()
+    h.data_format = h.header_format = 4
+    h.type, h.nheader, h.ndims = int(type_), 0, 1
+    for i, (a, b, c) in enumerate(zip(n, d, o)):
+        h.n[i], h.d[i], h.o[i] = int(a), float(b), float(c)
+    if not (isinstance(data, torch.Tensor) and data.is_cuda):
+        data = torch.from_numpy(np.ascontiguousarray(data, np.float32)).to(_dev())
+    data = data.to(torch.float32).contiguous()
+    R.check(R.lib().rsg_rss_write_regular(str(path).encode(), C.byref(h), data.data_ptr(), _stream()),
+            "rsg_rss_write_regular")
+
+
+def image_desc(nx, nz, dx, dz, ox, oz, nhx=1, nhz=1):
+    """The rsg_image2d description of an image (sizes, steps, origins)."""
+    C, torch, R = _rt()
+    return R.Image2DDesc(nx=int(nx), nz=int(nz), nhx=int(nhx), nhz=int(nhz), dx=float(dx), dz=float(dz),
+                         ox=float(ox), oz=float(oz))
+
+
+def stack_image_2d(gimg, g, limg, l, padlx=0, padhx=0, padlz=0, padhz=0):
+    """Image2D::stackImage_parallel in device memory: limg added into gimg
+    at the offset its origin gives (both [nhz][nhx][nz][nx] device tensors)."""
+    C, torch, R = _rt()
+    for t, desc in ((gimg, g), (limg, l)):
+        if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+            raise R.RsgError("stack_image_2d: contiguous float32 device tensors")
+        if t.numel() != desc.nx * desc.nz * desc.nhx * desc.nhz:
+            raise R.RsgError("stack_image_2d: tensor size differs from its description")
+    R.check(R.lib().rsg_stack_image_2d(gimg.data_ptr(), C.byref(g), limg.data_ptr(), C.byref(l), int(padlx),
+                                       int(padhx), int(padlz), int(padhz), _stream()), "rsg_stack_image_2d")

@@ -385,6 +385,32 @@ INTERP_CASES = [  # ntr, n1, d1, o1, n2, d2, o2 (Interp::interp for every mode)
 ]
 
 
+E3D_CASES = [  # order, fs, source field, record field (WavesElastic3D modelling)
+    (4, 0, "P", "P"), (8, 1, "VZ", "VX"), (2, 1, "VY", "P"), (6, 0, "VX", "VY"),
+]
+
+
+def e3d_fixtures(R):
+    """WavesElastic3D modelling runs of the compiled reference: records and
+    four of the nine final fields (Vx, Vz, Szz, Sxy), plus the staggered model
+    of ModelElastic3D::staggerModels with and without free surface."""
+    rng = np.random.default_rng(21)
+    nx, ny, nz, lpml, nt, dt, dx = 16, 14, 12, 5, 60, 1e-3, 10.0
+    vp = (2500 + 400 * rng.random((nz, ny, nx))).astype(np.float32)
+    vs = (vp / (1.7 + 0.2 * rng.random((nz, ny, nx)))).astype(np.float32)
+    rho = synth.gardner(vp)
+    sc = np.float32([[72.5, 61.0, 10.0]])
+    wav = synth.ricker(nt, dt, 25.0)[None].astype(np.float32)
+    g = np.stack(np.meshgrid(np.arange(0, 150, 20.0), np.arange(0, 130, 30.0), indexing="ij"), -1).reshape(-1, 2)
+    gc = np.concatenate([g, np.full((g.shape[0], 1), 15.0)], 1).astype(np.float32)
+    stag = {f"stag_fs{fs}": np.stack(runs.ref_stagger_e3d(vp, vs, rho, lpml, fs)) for fs in (0, 1)}
+    save("e3d_stagger", vp=vp, vs=vs, rho=rho, lpml=np.int32(lpml), **stag)
+    for k, (order, fs, sf, rf) in enumerate(E3D_CASES):
+        r = runs.ref_e3d_run(vp, vs, rho, lpml, (dx, dx, dx), dt, nt, order, fs, sf, sc, wav, rf, gc)
+        save(f"e3d_case{k}", rec=r["rec"], Vx=r["Vx"], Vz=r["Vz"], Szz=r["Szz"], Sxy=r["Sxy"], vp=vp, vs=vs,
+             rho=rho, scoords=sc, wav=wav, gcoords=gc, grid=np.float32([nx, ny, nz, lpml, dx, dt, nt]))
+
+
 def interp_fixture(R):
     rng = np.random.default_rng(17)
     out = {}
@@ -478,6 +504,7 @@ def domdec_fixture(R):
 
 
 GROUPS["domdec"] = domdec_fixture
+GROUPS["e3d"] = e3d_fixtures
 
 
 if __name__ == "__main__":

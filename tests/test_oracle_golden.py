@@ -8,7 +8,7 @@ import oracle
 from oracle import revolve as orv
 from oracle import runs
 
-from tests.golden.make_golden import A2D_CASES, E2D_CASES, REVOLVE_CASES
+from tests.golden.make_golden import A2D_CASES, E2D_CASES, E3D_CASES, REVOLVE_CASES
 
 
 def test_der_all_orders_and_axes(golden):
@@ -90,6 +90,27 @@ def test_a3d_run(golden, case):
     r = runs.a3d_run(g["vp"], g["rho"], int(lpml), (dx, dx, dx), dt, int(nt), int(order), 0, sf,
                      g["scoords"], g["wav"], rf, g["gcoords"])
     for k in ("rec", "P", "Vz"):
+        assert np.array_equal(r[k], g[k]), k
+
+
+@pytest.mark.parametrize("fs", [0, 1])
+def test_e3d_stagger(golden, fs):
+    g = golden("e3d_stagger")
+    out = runs.stagger_e3d(g["vp"], g["vs"], g["rho"], int(g["lpml"]), fs)
+    for i, (a, b) in enumerate(zip(out, g[f"stag_fs{fs}"])):
+        assert np.array_equal(a, b), ("L", "L2M", "Mxz", "Myz", "Mxy", "Rx", "Ry", "Rz")[i]
+
+
+@pytest.mark.parametrize("case", range(len(E3D_CASES)))
+def test_e3d_run(golden, case):
+    """WavesElastic3D modelling (explosive and force sources, pressure and
+    velocity gathers, free surface on and off, orders 2-8) bit for bit."""
+    g = golden(f"e3d_case{case}")
+    nx, ny, nz, lpml, dx, dt, nt = g["grid"]
+    order, fs, sf, rf = E3D_CASES[case]
+    r = runs.e3d_run(g["vp"], g["vs"], g["rho"], int(lpml), (dx, dx, dx), dt, int(nt), order, fs, sf, g["scoords"],
+                     g["wav"], rf, g["gcoords"])
+    for k in ("rec", "Vx", "Vz", "Szz", "Sxy"):
         assert np.array_equal(r[k], g[k]), k
 
 
