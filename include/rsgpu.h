@@ -314,6 +314,13 @@ int rsg_e3d_insert_source(const rsg_grid *g, const rsg_state_e3d *s, const rsg_t
  * trace's cell of the unpadded model Vp, Vs, Rho (device, [nz][ny][nx]). */
 int rsg_e3d_record_data(const rsg_grid *g, const rsg_state_e3d *s, const rsg_traces *rec, const float *Vp,
                         const float *Vs, const float *Rho, int it, void *stream);
+/* backwardstepVelocity / backwardstepStress (waves.cpp:3260-4103), the
+ * adjoint-state steps of FwiElastic3D: velocity from the derivatives of the
+ * moduli-weighted stresses (L2M Sxx + L Syy + L Szz, M_xy Sxy, M_xz Sxz, ...
+ * with dt R), stress from bare velocity derivatives (dt); the forward
+ * steps' strips and free surface. */
+int rsg_e3d_backward_velocity(const rsg_grid *g, const rsg_state_e3d *s, void *stream);
+int rsg_e3d_backward_stress(const rsg_grid *g, const rsg_state_e3d *s, void *stream);
 
 /* ---- storage of FWI snapshots and Revolve checkpoints (the fp16 member of
  * the FWI settings below; the reference keeps them as T = float, revolve.cpp
@@ -466,6 +473,40 @@ typedef struct {
 } rsg_fwi_a3d;
 
 int rsg_fwi_a3d_gradient(const rsg_grid *g, const rsg_fwi_a3d *f, void *stream);
+
+/* ---- Elastic 3D FWI gradient of one shot (FwiElastic3D::run_optimal,
+ * fwi.cpp:6770-6991): Revolve checkpoints of the nine fields and 36 strips
+ * in HBM (RSG_STORE_*), Ux / Uy / Uz records, misfit and residuals
+ * (computeMisfit / computeResiduals, fwi.cpp:5632-6196: DIFFERENCE,
+ * CORRELATION, ADAPTIVE_GAUSS, other types the reference's default branch),
+ * residuals injected cell by cell in trace order into the adjoint state of
+ * backwardstepVelocity / Stress, crossCorr (fwi.cpp:5046-5291, the
+ * reference's stencils and indices kept) into lambda / mu / rho kernels and
+ * scaleGrad (fwi.cpp:5553-5630) at the end; the scaled Vp, Vs, rho gradients
+ * are ADDED to vpgrad / vsgrad / rhograd (zero them for one shot's
+ * gradient).  Residual sample 0 of CORRELATION and of the default branch is
+ * left as the caller's array holds it, as the reference does. */
+typedef struct {
+    const float *L, *L2M, *Mxz, *Myz, *Mxy; /* staggered moduli (padded, device)    */
+    const float *Rx, *Ry, *Rz, *tab;     /* staggered buoyancies, PML tables        */
+    const float *Vp, *Vs, *Rho;          /* unpadded model nz*ny*nx (device)        */
+    int nx, ny, nz;                      /* unpadded sizes                          */
+    rsg_traces src;                      /* PRESSURE, VX, VY or VZ source            */
+    rsg_traces obs[3];                   /* observed Ux, Uy, Uz (fields VX, VY, VZ,
+                                            maps of makeMap for each; equal ntrace, nt) */
+    float *dmod[3], *dres[3];            /* out: modelled data, residuals [ntrace][nt] */
+    const float *weight[3];              /* data weights like obs, or NULL          */
+    float *wavgrad;                      /* out (optional): wavelet gradient, src layout,
+                                            of the source's field                   */
+    float *vpgrad, *vsgrad, *rhograd;    /* in/out: gradients added, nz*ny*nx      */
+    double *misfit;                      /* out (host, optional)                    */
+    int ncheck;                          /* Revolve snapshots                       */
+    int fp16;                            /* checkpoint storage, RSG_STORE_*         */
+    int misfit_type;                     /* rs_fwimisfit: 0 DIFFERENCE, 1 CORRELATION,
+                                            2 ADAPTIVE_GAUSS, 3 and 4 the default branch */
+} rsg_fwi_e3d;
+
+int rsg_fwi_e3d_gradient(const rsg_grid *g, const rsg_fwi_e3d *f, void *stream);
 
 /* ---- Acoustic 2D reverse-time migration of one shot
  * (RtmAcoustic2D::run_optimal, rtm.cpp:423-544): forward steps with the

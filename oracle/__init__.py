@@ -100,7 +100,7 @@ def _make_structs(real):
                      ("fs", C.c_int), ("dx", ct), ("dy", ct), ("dz", ct), ("dt", ct)]
                     + _ptr_fields(["tab", "Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy"], ct)
                     + [("psi", C.POINTER(ct) * 36)]
-                    + _ptr_fields(["L", "L2M", "Mxz", "Myz", "Mxy", "Rx", "Ry", "Rz", "df"], ct))
+                    + _ptr_fields(["L", "L2M", "Mxz", "Myz", "Mxy", "Rx", "Ry", "Rz", "df", "wrk"], ct))
 
     return A2D, E2D, A3D, E3D
 
@@ -158,9 +158,18 @@ def _declare(L):
         f.argtypes = [C.POINTER(E2D), C.POINTER(E2D), rp, rp, rp]
         f = getattr(L, "orc_e2d_adj_velocity" + suf)
         f.argtypes = [C.POINTER(E2D), C.POINTER(E2D), rp, rp]
-        for nm in ("e3d_velocity", "e3d_stress"):
+        for nm in ("e3d_velocity", "e3d_stress", "e3d_velocity_bw", "e3d_stress_bw"):
             f = getattr(L, "orc_" + nm + suf)
             f.argtypes = [C.POINTER(E3D)]
+        f = getattr(L, "orc_e3d_crosscorr" + suf)
+        f.argtypes = [rp] * 21 + [C.c_int] * 4 + [ct] * 3 + [rp] * 3
+        f = getattr(L, "orc_e3d_scalegrad" + suf)
+        f.argtypes = [rp, rp, rp, C.c_long, rp, rp, rp]
+        f = getattr(L, "orc_misfit_e3" + suf)
+        f.argtypes = [C.c_int] + [rp] * 6 + [C.c_void_p] * 3 + [C.c_int, C.c_int]
+        f.restype = ct
+        f = getattr(L, "orc_residual_e3" + suf)
+        f.argtypes = [C.c_int] + [rp] * 6 + [C.c_void_p] * 3 + [rp] * 3 + [C.c_int, C.c_int]
         f = getattr(L, "orc_inject3d_explosive" + suf)
         f.argtypes = [rp, rp, rp, C.c_int, C.c_int, C.c_int, ct, _i32p, rp, rp, C.c_int, C.c_int, C.c_int]
         f = getattr(L, "orc_record3d_pressure_el" + suf)
@@ -214,6 +223,9 @@ def _declare_ref(R):
     R.ref_e3d_run.argtypes = ([_f32p, _f32p, _f32p] + [C.c_int] * 4 + [C.c_float] * 4 + [C.c_int] * 5
                               + [_f32p, _f32p, C.c_int, C.c_int, _f32p, _f32p, _f32p])
     R.ref_stagger_e3d.argtypes = [_f32p, _f32p, _f32p] + [C.c_int] * 5 + [_f32p]
+    R.ref_fwi_e3d.argtypes = ([_f32p] * 3 + [C.c_int] * 4 + [C.c_float] * 4 + [C.c_int] * 6 + [_f32p, _f32p, C.c_int,
+                              _f32p, _f32p] + [_f32p] * 6 + [C.c_int] + [C.c_void_p] * 3)
+    R.ref_fwi_e3d.restype = C.c_float
     R.ref_der.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                           C.c_int, C.c_int]
     R.ref_pml_abc.argtypes = [C.c_int, C.c_float, C.c_float, C.c_int, _f32p]

@@ -15,11 +15,13 @@ import numpy as np
 
 from . import fn, ref
 from . import revolve as orv
-from .runs import (A2DState, A3DState, E2DState, e2d_record_factor, inject2d, makemap2d, makemap3d,
-                   record2d, stagger_a2d, stagger_a3d, stagger_e2d)
+from .runs import (E3D_FIELDS, A2DState, A3DState, E2DState, E3DState, e2d_record_factor, inject2d,
+                   makemap2d, makemap3d, record2d, stagger_a2d, stagger_a3d, stagger_e2d, stagger_e3d)
 
 
 def _fields(s):
+    if isinstance(s, E3DState):
+        return [getattr(s, k) for k in E3D_FIELDS] + s.strips
     if isinstance(s, A3DState):
         return [s.P, s.Vx, s.Vy, s.Vz] + list(s.strips.values())
     if isinstance(s, E2DState):
@@ -496,3 +498,140 @@ def ref_fwi_a3d(vp, rho, lpml, d3, dt, nt, order, fs, ncheck, scoords, wav, gcoo
                           dres.reshape(-1), wg.reshape(-1), int(misfit_type),
                           None if wts is None else wts.ctypes.data)
     return dict(misfit=np.float32(m), vpgrad=vg, rhograd=rg, dmod=dmod, dres=dres, wavgrad=wg)
+
+
+_E3_REC = ("VX", "VY", "VZ")
+
+
+def fwi_e3d(vp, vs, rho, lpml, d3, dt, nt, order, fs, ncheck, srcfield, scoords, wav, gcoords, dobs,
+            real=np.float32, misfit_type=0, weights=None):
+    """FwiElastic3D::run_optimal (fwi.cpp:6770-6991): forward steps with the
+    source (PRESSURE explosive or VX / VY / VZ force) and the Ux, Uy, Uz
+    records (each with the makeMap of its field); at the first turn the
+    misfit (computeMisfit, fwi.cpp:5632-5899) and residuals
+    (computeResiduals, fwi.cpp:5901-6196); the adjoint state of
+    backwardstepVelocity / Stress with the residuals injected as VX, VY, VZ
+    sources (insertSource in trace order); crossCorr (fwi.cpp:5046-5291) and
+    the wavelet gradient of the source's field at every reverse step;
+    scaleGrad (fwi.cpp:5553-5630) at the end.  dobs: [3][nrec][nt];
+    weights: None or three arrays / Nones."""
+    nz, ny, nx = vp.shape
+    smap, sshift = makemap3d(scoords, (nx, ny, nz), d3, srcfield)
+    gmaps = [makemap3d(gcoords, (nx, ny, nz), d3, f) for f in _E3_REC]
+    fw = E3DState(stagger_e3d(vp, vs, rho, lpml, fs, real), lpml, order, d3, dt, fs, real)
+    bw = E3DState(stagger_e3d(vp, vs, rho, lpml, fs, real), lpml, order, d3, dt, fs, real)
+    nsrc, nrec = smap.shape[0], gmaps[0][0].shape[0]
+    wav = np.ascontiguousarray(wav, real).reshape(nsrc, nt)
+    dobs = np.ascontiguousarray(dobs, real).reshape(3, nrec, nt)
+    dmod = np.zeros_like(dobs)
+    dres = np.zeros_like(dobs)
+    wgrad = np.zeros_like(wav)
+    lam, mu, rg = (np.zeros((nz, ny, nx), real) for _ in range(3))
+    wts = [None] * 3 if weights is None else [None if w is None else np.ascontiguousarray(w, real).reshape(nrec, nt)
+                                               for w in weights]
+    sh = np.ascontiguousarray(sshift, real).reshape(-1)
+    gh = [np.ascontiguousarray(g[1], real).reshape(-1) for g in gmaps]
+    model = [np.ascontiguousarray(a, real).ravel() for a in (vp, vs, rho)]
+
+    def src_inject(s, it):
+        if srcfield == "P":
+            fn("inject3d_explosive", real)(s.Sxx.reshape(-1), s.Syy.reshape(-1), s.Szz.reshape(-1), s.nx, s.ny,
+                                           lpml, real(dt), smap.reshape(-1), sh, wav.reshape(-1), nsrc, nt, it)
+        else:
+            F, M = {"VX": (s.Vx, s.Rx), "VY": (s.Vy, s.Ry), "VZ": (s.Vz, s.Rz)}[srcfield]
+            fn("inject3d", real)(F.reshape(-1), M.reshape(-1), s.nx, s.ny, lpml, real(dt), smap.reshape(-1), sh,
+                                 wav.reshape(-1), nsrc, nt, it)
+
+    def record(it):
+        for c, F in enumerate((fw.Vx, fw.Vy, fw.Vz)):
+            fn("record3d", real)(F.reshape(-1), fw.nx, fw.ny, lpml, gmaps[c][0].reshape(-1), gh[c], dmod[c].reshape(-1),
+                                 nrec, nt, it)
+
+    def res_inject(it):
+        for c, (F, M) in enumerate(((bw.Vx, bw.Rx), (bw.Vy, bw.Ry), (bw.Vz, bw.Rz))):
+            fn("inject3d", real)(F.reshape(-1), M.reshape(-1), bw.nx, bw.ny, lpml, real(dt),
+                                 gmaps[c][0].reshape(-1), gh[c], dres[c].reshape(-1), nrec, nt, it)
+
+    xc = fn("e3d_crosscorr", real)
+    nxp, nxyp = fw.nx, fw.nx * fw.ny
+
+    def corr(it):
+        xc(*(getattr(fw, k).reshape(-1) for k in ("Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy")),
+           *(getattr(bw, k).reshape(-1) for k in ("Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy")),
+           fw.Rx.reshape(-1), fw.Ry.reshape(-1), fw.Rz.reshape(-1), nx, ny, nz, lpml, real(d3[0]), real(d3[1]),
+           real(d3[2]), lam.reshape(-1), mu.reshape(-1), rg.reshape(-1))
+        # wavelet gradient (fwi.cpp:5143-5185): traces mapped to interior cells
+        for i in range(nsrc):
+            m = smap[i]
+            if not (1 <= m[0] < nx - 1 and 1 <= m[1] < ny - 1 and 1 <= m[2] < nz - 1):
+                continue
+            c = ((m[2] + lpml) * fw.ny + m[1] + lpml) * nxp + m[0] + lpml
+            if srcfield == "P":
+                v = bw.Sxx.reshape(-1)[c] + bw.Syy.reshape(-1)[c] + bw.Szz.reshape(-1)[c]
+            else:
+                F, st = {"VX": (bw.Vx, 1), "VY": (bw.Vy, nxp), "VZ": (bw.Vz, nxyp)}[srcfield]
+                v = real(0.5) * (F.reshape(-1)[c] + F.reshape(-1)[c - st])
+            wgrad[i, it] = v
+
+    def fstep(it, rec):
+        fw.velocity()
+        fw.stress()
+        src_inject(fw, it)
+        if rec:
+            record(it)
+
+    misfit = None
+    checks = {}
+    rev = orv.Revolve(nt, ncheck)
+    reverse = False
+    wp = [None if w is None else w.ctypes.data for w in wts]
+    while True:
+        oldcapo = rev.capo
+        a = rev.step()
+        capo = rev.capo
+        if a == orv.ADVANCE:
+            for it in range(oldcapo, capo):
+                fstep(it, not reverse)
+        elif a == orv.FIRSTURN:
+            fstep(capo, True)
+            args = [misfit_type] + [x for c in range(3) for x in (dmod[c].reshape(-1), dobs[c].reshape(-1))] + wp
+            misfit = fn("misfit_e3", real)(*args, nrec, nt)
+            fn("residual_e3", real)(*args, *(dres[c].reshape(-1) for c in range(3)), nrec, nt)
+            res_inject(capo)
+            corr(capo)
+            reverse = True
+        elif a == orv.YOUTURN:
+            bw.velocity_bw()
+            bw.stress_bw()
+            res_inject(capo)
+            corr(capo)
+        elif a == orv.TAKESHOT:
+            checks[rev.check] = _snapshot(fw)
+        elif a == orv.RESTORE:
+            _restore(fw, checks[rev.check])
+        if a in (orv.TERMINATE, orv.ERROR):
+            break
+    fn("e3d_scalegrad", real)(*model, nx * ny * nz, lam.reshape(-1), mu.reshape(-1), rg.reshape(-1))
+    return dict(misfit=misfit, vpgrad=lam, vsgrad=mu, rhograd=rg, dmod=dmod, dres=dres, wavgrad=wgrad)
+
+
+def ref_fwi_e3d(vp, vs, rho, lpml, d3, dt, nt, order, fs, ncheck, srcfield, scoords, wav, gcoords, dobs,
+                misfit_type=0, weights=None):
+    """FwiElastic3D::run_optimal of the compiled reference, the same outputs."""
+    nz, ny, nx = vp.shape
+    sc = np.ascontiguousarray(scoords, np.float32).reshape(-1, 3)
+    gc = np.ascontiguousarray(gcoords, np.float32).reshape(-1, 3)
+    nsrc, nrec = sc.shape[0], gc.shape[0]
+    g = [np.zeros((nz, ny, nx), np.float32) for _ in range(3)]
+    dmod = np.zeros((3, nrec, nt), np.float32)
+    dres = np.zeros((3, nrec, nt), np.float32)
+    wg = np.zeros((nsrc, nt), np.float32)
+    f32 = lambda a: np.ascontiguousarray(a, np.float32).ravel()  # noqa: E731
+    wts = [None] * 3 if weights is None else [None if w is None else f32(w) for w in weights]
+    from .runs import _FIELD_CODE
+    m = ref().ref_fwi_e3d(f32(vp), f32(vs), f32(rho), nx, ny, nz, lpml, *(np.float32(v) for v in d3),
+                          np.float32(dt), nt, order, int(fs), ncheck, _FIELD_CODE[srcfield], nsrc, sc.ravel(),
+                          f32(wav), nrec, gc.ravel(), f32(dobs), *(a.reshape(-1) for a in g), dmod.reshape(-1),
+                          dres.reshape(-1), wg.reshape(-1), int(misfit_type),
+                          *(None if w is None else w.ctypes.data for w in wts))
+    return dict(misfit=np.float32(m), vpgrad=g[0], vsgrad=g[1], rhograd=g[2], dmod=dmod, dres=dres, wavgrad=wg)

@@ -684,6 +684,79 @@ float ref_fwi_a3d(const float *vp, const float *rho, int nx, int ny, int nz, int
     return fwi->getMisfit();
 }
 
+// FwiElastic3D::run_optimal with in-core Revolve (ncheck snapshots): source
+// of field srcfield (0 P, 1 VX, 2 VY, 3 VZ), observed Ux, Uy, Uz records
+// dobs[3][nrec][nt]; misfit_type as rs_fwimisfit; weix/weiy/weiz optional
+// data weights.  Returns the misfit; outputs the Vp, Vs, rho gradients, the
+// modelled data and residuals [3][nrec][nt] and the wavelet gradient.
+float ref_fwi_e3d(const float *vp, const float *vs, const float *rho, int nx, int ny, int nz, int lpml, float dx,
+                  float dy, float dz, float dt, int nt, int order, int fs, int ncheck, int srcfield, int nsrc,
+                  const float *scoords, const float *wav, int nrec, const float *gcoords, const float *dobs,
+                  float *vpgrad, float *vsgrad, float *rhograd, float *dmod, float *dres, float *wgrad,
+                  int misfit_type, const float *weix, const float *weiy, const float *weiz) {
+    auto model = std::make_shared<ModelElastic3D<float>>(nx, ny, nz, lpml, dx, dy, dz, 0.f, 0.f, 0.f, fs != 0);
+    model->createModel();
+    const size_t nm = (size_t)nx * ny * nz, nd = (size_t)nrec * nt;
+    std::memcpy(model->getVp(), vp, sizeof(float) * nm);
+    std::memcpy(model->getVs(), vs, sizeof(float) * nm);
+    std::memcpy(model->getR(), rho, sizeof(float) * nm);
+    auto mk = [&](int ntr, const float *coords, bool src, rs_field f, const float *d) {
+        auto x = std::make_shared<Data3D<float>>(ntr, nt, dt, 0.f);
+        x->setField(f);
+        Point3D<float> *c = src ? x->getGeom()->getScoords() : x->getGeom()->getGcoords();
+        for (int i = 0; i < ntr; i++) {
+            c[i].x = coords[3 * i];
+            c[i].y = coords[3 * i + 1];
+            c[i].z = coords[3 * i + 2];
+        }
+        if (d) std::memcpy(x->getData(), d, sizeof(float) * ntr * nt);
+        x->makeMap(model->getGeom(), src ? SMAP : GMAP);
+        return x;
+    };
+    auto src = mk(nsrc, scoords, true, field_of(srcfield), wav);
+    auto wg = mk(nsrc, scoords, true, field_of(srcfield), nullptr);
+    const rs_field fields[3] = {VX, VY, VZ};
+    std::shared_ptr<Data3D<float>> obs[3], mod[3], res[3], wd[3];
+    const float *wei[3] = {weix, weiy, weiz};
+    for (int k = 0; k < 3; k++) {
+        obs[k] = mk(nrec, gcoords, false, fields[k], dobs + nd * k);
+        mod[k] = mk(nrec, gcoords, false, fields[k], nullptr);
+        res[k] = mk(nrec, gcoords, false, fields[k], nullptr);
+        if (wei[k]) wd[k] = mk(nrec, gcoords, false, fields[k], wei[k]);
+    }
+    model->staggerModels();
+    auto fwi = std::make_shared<FwiElastic3D<float>>(model, src, obs[0], obs[1], obs[2], order, 1);
+    fwi->setDatamodUx(mod[0]);
+    fwi->setDatamodUy(mod[1]);
+    fwi->setDatamodUz(mod[2]);
+    fwi->setDataresUx(res[0]);
+    fwi->setDataresUy(res[1]);
+    fwi->setDataresUz(res[2]);
+    if (wd[0]) fwi->setDataweightx(wd[0]);
+    if (wd[1]) fwi->setDataweighty(wd[1]);
+    if (wd[2]) fwi->setDataweightz(wd[2]);
+    auto vg = std::make_shared<Image3D<float>>("vpgrad_unused", model, 1, 1, 1);
+    auto sg = std::make_shared<Image3D<float>>("vsgrad_unused", model, 1, 1, 1);
+    auto rg = std::make_shared<Image3D<float>>("rhograd_unused", model, 1, 1, 1);
+    fwi->setVpgrad(vg);
+    fwi->setVsgrad(sg);
+    fwi->setRhograd(rg);
+    fwi->setWavgrad(wg);
+    fwi->setNcheck(ncheck);
+    fwi->setIncore(true);
+    fwi->setMisfit_type((rs_fwimisfit)misfit_type);
+    fwi->run_optimal();
+    std::memcpy(vpgrad, vg->getImagedata(), sizeof(float) * nm);
+    std::memcpy(vsgrad, sg->getImagedata(), sizeof(float) * nm);
+    std::memcpy(rhograd, rg->getImagedata(), sizeof(float) * nm);
+    for (int k = 0; k < 3; k++) {
+        std::memcpy(dmod + nd * k, mod[k]->getData(), sizeof(float) * nd);
+        std::memcpy(dres + nd * k, res[k]->getData(), sizeof(float) * nd);
+    }
+    std::memcpy(wgrad, wg->getData(), sizeof(float) * nsrc * nt);
+    return fwi->getMisfit();
+}
+
 // Interp<float>(mode)::interp between two Data2D of ntr traces (mode 0
 // LINEAR, 1 BSPLINE, 2 SINC).
 void ref_interp(int mode, const float *from, int ntr, int n1, float d1, float o1, float *to, int n2, float d2,

@@ -866,10 +866,12 @@ typedef struct {
     REAL *psi[36];
     const REAL *L, *L2M, *Mxz, *Myz, *Mxy, *Rx, *Ry, *Rz;
     REAL *df;
+    REAL *wrk; /* backward velocity step: the moduli-weighted stresses */
 } FN(orc_e3d_t);
 
-/* F_k += dt*M_k*df over the grid (k = 1..3, F2/F3 optional), then the two
- * strips of `axis`: psi = B*psi + A*df, F_k -= dt*M_k*(psi + C*df). */
+/* F_k += dt*M_k*df over the grid (k = 1..3, F2/F3 optional; M1 NULL: the
+ * backward stress step's bare dt*df), then the two strips of `axis`:
+ * psi = B*psi + A*df, F_k -= dt*M_k*(psi + C*df). */
 static void FN(upd3e_)(FN(orc_e3d_t) *s, int axis, int stag, REAL *F1, const REAL *M1, REAL *F2, const REAL *M2,
                        REAL *F3, const REAL *M3, int slot)
 {
@@ -879,7 +881,7 @@ static void FN(upd3e_)(FN(orc_e3d_t) *s, int axis, int stag, REAL *F1, const REA
     REAL *psiLo = s->psi[slot], *psiHi = s->psi[slot + 1];
     long nxy = (long)nx * ny, n = nxy * nz;
     for (long p = 0; p < n; p++) {
-        F1[p] += dt * M1[p] * df[p];
+        F1[p] += M1 ? dt * M1[p] * df[p] : dt * df[p];
         if (F2) F2[p] += dt * M2[p] * df[p];
         if (F3) F3[p] += dt * M3[p] * df[p];
     }
@@ -912,7 +914,7 @@ static void FN(upd3e_)(FN(orc_e3d_t) *s, int axis, int stag, REAL *F1, const REA
                     REAL *psi = side ? psiHi : psiLo;
                     REAL A = side ? Ar[t] : Al[t], B = side ? Br[t] : Bl[t], C = side ? Cr[t] : Cl[t];
                     psi[q] = B * psi[q] + A * df[c];
-                    F1[c] -= dt * M1[c] * (psi[q] + C * df[c]);
+                    F1[c] -= M1 ? dt * M1[c] * (psi[q] + C * df[c]) : dt * (psi[q] + C * df[c]);
                     if (F2) F2[c] -= dt * M2[c] * (psi[q] + C * df[c]);
                     if (F3) F3[c] -= dt * M3[c] * (psi[q] + C * df[c]);
                 }
@@ -968,6 +970,79 @@ void FN(orc_e3d_stress)(FN(orc_e3d_t) *s)
     FN(upd3e_)(s, 1, 1, s->Sxy, s->Mxy, 0, 0, 0, 0, 20);
     FN(orc_deriv)(s->Vy, s->df, nx, ny, nz, 0, 1, o, s->dx);
     FN(upd3e_)(s, 0, 1, s->Sxy, s->Mxy, 0, 0, 0, 0, 10);
+}
+
+/* wrk = M1*F1 (+ M2*F2 + M3*F3) cell by cell, the reference's wrk arrays of
+ * backwardstepVelocity. */
+static void FN(wrk3_)(FN(orc_e3d_t) *s, const REAL *M1, const REAL *F1, const REAL *M2, const REAL *F2,
+                      const REAL *M3, const REAL *F3)
+{
+    long n = (long)s->nx * s->ny * s->nz;
+    for (long p = 0; p < n; p++)
+        s->wrk[p] = F2 ? M1[p] * F1[p] + M2[p] * F2[p] + M3[p] * F3[p] : M1[p] * F1[p];
+}
+
+/* WavesElastic3D::backwardstepVelocity (waves.cpp:3260-3713): the forward
+ * velocity terms and strips on wrk = L2M Sxx + L Syy + L Szz (x), M_xy Sxy,
+ * M_xz Sxz, ... */
+void FN(orc_e3d_velocity_bw)(FN(orc_e3d_t) *s)
+{
+    int nx = s->nx, ny = s->ny, nz = s->nz, o = s->order;
+    FN(wrk3_)(s, s->L2M, s->Sxx, s->L, s->Syy, s->L, s->Szz);
+    FN(orc_deriv)(s->wrk, s->df, nx, ny, nz, 0, 1, o, s->dx);
+    FN(upd3e_)(s, 0, 1, s->Vx, s->Rx, 0, 0, 0, 0, 0);
+    FN(wrk3_)(s, s->Mxy, s->Sxy, 0, 0, 0, 0);
+    FN(orc_deriv)(s->wrk, s->df, nx, ny, nz, 1, 0, o, s->dy);
+    FN(upd3e_)(s, 1, 0, s->Vx, s->Rx, 0, 0, 0, 0, 14);
+    FN(wrk3_)(s, s->Mxz, s->Sxz, 0, 0, 0, 0);
+    FN(orc_deriv)(s->wrk, s->df, nx, ny, nz, 2, 0, o, s->dz);
+    FN(upd3e_)(s, 2, 0, s->Vx, s->Rx, 0, 0, 0, 0, 26);
+    FN(wrk3_)(s, s->Mxy, s->Sxy, 0, 0, 0, 0);
+    FN(orc_deriv)(s->wrk, s->df, nx, ny, nz, 0, 0, o, s->dx);
+    FN(upd3e_)(s, 0, 0, s->Vy, s->Ry, 0, 0, 0, 0, 4);
+    FN(wrk3_)(s, s->L, s->Sxx, s->L2M, s->Syy, s->L, s->Szz);
+    FN(orc_deriv)(s->wrk, s->df, nx, ny, nz, 1, 1, o, s->dy);
+    FN(upd3e_)(s, 1, 1, s->Vy, s->Ry, 0, 0, 0, 0, 12);
+    FN(wrk3_)(s, s->Myz, s->Syz, 0, 0, 0, 0);
+    FN(orc_deriv)(s->wrk, s->df, nx, ny, nz, 2, 0, o, s->dz);
+    FN(upd3e_)(s, 2, 0, s->Vy, s->Ry, 0, 0, 0, 0, 28);
+    FN(wrk3_)(s, s->Mxz, s->Sxz, 0, 0, 0, 0);
+    FN(orc_deriv)(s->wrk, s->df, nx, ny, nz, 0, 0, o, s->dx);
+    FN(upd3e_)(s, 0, 0, s->Vz, s->Rz, 0, 0, 0, 0, 2);
+    FN(wrk3_)(s, s->Myz, s->Syz, 0, 0, 0, 0);
+    FN(orc_deriv)(s->wrk, s->df, nx, ny, nz, 1, 0, o, s->dy);
+    FN(upd3e_)(s, 1, 0, s->Vz, s->Rz, 0, 0, 0, 0, 16);
+    FN(wrk3_)(s, s->L, s->Sxx, s->L, s->Syy, s->L2M, s->Szz);
+    FN(orc_deriv)(s->wrk, s->df, nx, ny, nz, 2, 1, o, s->dz);
+    FN(upd3e_)(s, 2, 1, s->Vz, s->Rz, 0, 0, 0, 0, 24);
+}
+
+/* WavesElastic3D::backwardstepStress (waves.cpp:3715-4103): each stress
+ * gains dt times its velocity derivatives, with the forward strips. */
+void FN(orc_e3d_stress_bw)(FN(orc_e3d_t) *s)
+{
+    int nx = s->nx, ny = s->ny, nz = s->nz, o = s->order;
+    long nxy = (long)nx * ny;
+    FN(orc_deriv)(s->Vx, s->df, nx, ny, nz, 0, 0, o, s->dx);
+    FN(upd3e_)(s, 0, 0, s->Sxx, 0, 0, 0, 0, 0, 6);
+    FN(orc_deriv)(s->Vy, s->df, nx, ny, nz, 1, 0, o, s->dy);
+    FN(upd3e_)(s, 1, 0, s->Syy, 0, 0, 0, 0, 0, 18);
+    FN(orc_deriv)(s->Vz, s->df, nx, ny, nz, 2, 0, o, s->dz);
+    FN(upd3e_)(s, 2, 0, s->Szz, 0, 0, 0, 0, 0, 30);
+    if (s->fs) /* plane lpml lies in no z strip */
+        for (long p = 0; p < nxy; p++) s->Szz[(long)s->lpml * nxy + p] = 0.0;
+    FN(orc_deriv)(s->Vz, s->df, nx, ny, nz, 0, 1, o, s->dx);
+    FN(upd3e_)(s, 0, 1, s->Sxz, 0, 0, 0, 0, 0, 8);
+    FN(orc_deriv)(s->Vx, s->df, nx, ny, nz, 2, 1, o, s->dz);
+    FN(upd3e_)(s, 2, 1, s->Sxz, 0, 0, 0, 0, 0, 32);
+    FN(orc_deriv)(s->Vy, s->df, nx, ny, nz, 2, 1, o, s->dz);
+    FN(upd3e_)(s, 2, 1, s->Syz, 0, 0, 0, 0, 0, 34);
+    FN(orc_deriv)(s->Vz, s->df, nx, ny, nz, 1, 1, o, s->dy);
+    FN(upd3e_)(s, 1, 1, s->Syz, 0, 0, 0, 0, 0, 22);
+    FN(orc_deriv)(s->Vx, s->df, nx, ny, nz, 1, 1, o, s->dy);
+    FN(upd3e_)(s, 1, 1, s->Sxy, 0, 0, 0, 0, 0, 20);
+    FN(orc_deriv)(s->Vy, s->df, nx, ny, nz, 0, 1, o, s->dx);
+    FN(upd3e_)(s, 0, 1, s->Sxy, 0, 0, 0, 0, 0, 10);
 }
 
 /* Explosive source of WavesElastic3D::insertSource (waves.cpp:4137-4160):
@@ -2160,6 +2235,270 @@ void FN(orc_residual_diff)(const REAL *mod, const REAL *rec, REAL *res, int ntr,
 {
     long n = (long)ntr * nt;
     for (long p = 0; p < n; p++) res[p] = scaling * (mod[p] - rec[p]);
+}
+
+/* ---------------------------------------------------------------------- */
+/* FwiElastic3D (fwi.cpp:5046-5291, 5553-6196)                              */
+/* ---------------------------------------------------------------------- */
+
+/* crossCorr of run_optimal (fwi.cpp:5046-5291), no domain decomposition:
+ * source fields ws* / fs* and receiver fields wr* / rs* on the padded grid
+ * (pad lpml), Rx/Ry/Rz padded; lambda (vpgrad), mu (vsgrad) and rho kernels
+ * over cells 1..n-2 of the unpadded grid, with the reference's operand order
+ * and T / double promotions.  Kept as the reference has them: all four shear
+ * terms of each pair read the receiver stress at the cell itself, and the
+ * second, fourth and sixth z terms of the rho kernel read wrz one cell
+ * back in y with Rz one cell back in z. */
+void FN(orc_e3d_crosscorr)(const REAL *wsx, const REAL *wsy, const REAL *wsz, const REAL *fsxx, const REAL *fsyy,
+                           const REAL *fszz, const REAL *fsyz, const REAL *fsxz, const REAL *fsxy, const REAL *wrx,
+                           const REAL *wry, const REAL *wrz, const REAL *rsxx, const REAL *rsyy, const REAL *rszz,
+                           const REAL *rsyz, const REAL *rsxz, const REAL *rsxy, const REAL *Rx, const REAL *Ry,
+                           const REAL *Rz, int nx, int ny, int nz, int lpml, REAL dx, REAL dy, REAL dz, REAL *lam,
+                           REAL *mu, REAL *rho)
+{
+    const long nxp = nx + 2 * lpml, nxyp = nxp * (ny + 2 * lpml);
+#define K(x, y, z) ((long)(z) * nxyp + (long)(y) * nxp + (x))
+#define SUBD(v, t) v = (REAL)((double)v - 0.25 * (double)(t) * (double)rs_c)
+#define ADDU(u, w, r, d, h) u = (REAL)((double)u + 0.5 * (double)(w) * (double)(r) * (double)(d) / (double)(h))
+    for (int iz = 1; iz < nz - 1; iz++)
+        for (int iy = 1; iy < ny - 1; iy++)
+            for (int ix = 1; ix < nx - 1; ix++) {
+                const int x = ix + lpml, y = iy + lpml, z = iz + lpml;
+                const long c = K(x, y, z), m = ((long)iz * ny + iy) * nx + ix;
+                REAL msxx = (wsx[c] - wsx[K(x - 1, y, z)]) / dx;
+                REAL msyy = (wsy[c] - wsy[K(x, y - 1, z)]) / dy;
+                REAL mszz = (wsz[c] - wsz[K(x, y, z - 1)]) / dz;
+                lam[m] -= (msxx + msyy + mszz) * (rsxx[c] + rsyy[c] + rszz[c]);
+                REAL v = mu[m], t, rs_c;
+                rs_c = rsyz[c];
+                t = (wsy[K(x, y - 1, z)] - wsy[K(x, y - 1, z - 1)]) / dz;
+                t += (wsz[K(x, y, z - 1)] - wsz[K(x, y - 1, z - 1)]) / dy;
+                SUBD(v, t);
+                t = (wsy[c] - wsy[K(x, y, z - 1)]) / dz;
+                t += (wsz[K(x, y + 1, z - 1)] - wsz[K(x, y, z - 1)]) / dy;
+                SUBD(v, t);
+                t = (wsy[K(x, y - 1, z + 1)] - wsy[K(x, y - 1, z)]) / dz;
+                t += (wsz[c] - wsz[K(x, y - 1, z)]) / dy;
+                SUBD(v, t);
+                t = (wsy[K(x, y, z + 1)] - wsy[c]) / dz;
+                t += (wsz[K(x, y + 1, z)] - wsz[c]) / dy;
+                SUBD(v, t);
+                rs_c = rsxz[c];
+                t = (wsx[K(x - 1, y, z)] - wsx[K(x - 1, y, z - 1)]) / dz;
+                t += (wsz[K(x, y, z - 1)] - wsz[K(x - 1, y, z - 1)]) / dx;
+                SUBD(v, t);
+                t = (wsx[c] - wsx[K(x, y, z - 1)]) / dz;
+                t += (wsz[K(x + 1, y, z - 1)] - wsz[K(x, y, z - 1)]) / dx;
+                SUBD(v, t);
+                t = (wsx[K(x - 1, y, z + 1)] - wsx[K(x - 1, y, z)]) / dz;
+                t += (wsz[c] - wsz[K(x - 1, y, z)]) / dx;
+                SUBD(v, t);
+                t = (wsx[K(x, y, z + 1)] - wsx[c]) / dz;
+                t += (wsz[K(x + 1, y, z)] - wsz[c]) / dx;
+                SUBD(v, t);
+                rs_c = rsxy[c];
+                t = (wsy[K(x, y - 1, z)] - wsy[K(x - 1, y - 1, z)]) / dx;
+                t += (wsx[K(x - 1, y, z)] - wsx[K(x - 1, y - 1, z)]) / dy;
+                SUBD(v, t);
+                t = (wsy[K(x + 1, y - 1, z)] - wsy[K(x, y - 1, z)]) / dx;
+                t += (wsx[c] - wsx[K(x, y - 1, z)]) / dy;
+                SUBD(v, t);
+                t = (wsy[c] - wsy[K(x - 1, y, z)]) / dx;
+                t += (wsx[K(x - 1, y + 1, z)] - wsx[K(x - 1, y, z)]) / dy;
+                SUBD(v, t);
+                t = (wsy[K(x + 1, y, z)] - wsy[c]) / dx;
+                t += (wsx[K(x, y + 1, z)] - wsx[c]) / dy;
+                SUBD(v, t);
+                mu[m] = (REAL)((double)v - (2.0 * (double)msxx * (double)rsxx[c] + 2.0 * (double)msyy * (double)rsyy[c] +
+                                            2.0 * (double)mszz * (double)rszz[c]));
+                const long xm = K(x - 1, y, z), ym = K(x, y - 1, z), zm = K(x, y, z - 1);
+                REAL ux = (REAL)(0.5 * (double)wrx[c] * (double)Rx[c] * (double)(fsxx[K(x + 1, y, z)] - fsxx[c]) /
+                                 (double)dx);
+                ADDU(ux, wrx[xm], Rx[xm], fsxx[c] - fsxx[xm], dx);
+                ADDU(ux, wrx[c], Rx[c], fsxy[c] - fsxy[ym], dy);
+                ADDU(ux, wrx[xm], Rx[xm], fsxy[xm] - fsxy[K(x - 1, y - 1, z)], dy);
+                ADDU(ux, wrx[c], Rx[c], fsxz[c] - fsxz[zm], dz);
+                ADDU(ux, wrx[xm], Rx[xm], fsxz[xm] - fsxz[K(x - 1, y, z - 1)], dz);
+                REAL uy = (REAL)(0.5 * (double)wry[c] * (double)Ry[c] * (double)(fsxy[c] - fsxy[xm]) / (double)dx);
+                ADDU(uy, wry[ym], Ry[ym], fsxy[ym] - fsxy[K(x - 1, y - 1, z)], dx);
+                ADDU(uy, wry[c], Ry[c], fsyy[K(x, y + 1, z)] - fsyy[c], dy);
+                ADDU(uy, wry[ym], Ry[ym], fsyy[c] - fsyy[ym], dy);
+                ADDU(uy, wry[c], Ry[c], fsyz[c] - fsyz[zm], dz);
+                ADDU(uy, wry[ym], Ry[ym], fsyz[ym] - fsyz[K(x, y - 1, z - 1)], dz);
+                REAL uz = (REAL)(0.5 * (double)wrz[c] * (double)Rz[c] * (double)(fsxz[c] - fsxz[xm]) / (double)dx);
+                ADDU(uz, wrz[ym], Rz[zm], fsxz[zm] - fsxz[K(x - 1, y, z - 1)], dx);
+                ADDU(uz, wrz[c], Rz[c], fsyz[c] - fsyz[ym], dy);
+                ADDU(uz, wrz[ym], Rz[zm], fsyz[zm] - fsyz[K(x, y - 1, z - 1)], dy);
+                ADDU(uz, wrz[c], Rz[c], fszz[K(x, y, z + 1)] - fszz[c], dz);
+                ADDU(uz, wrz[ym], Rz[zm], fszz[c] - fszz[zm], dz);
+                rho[m] -= (ux + uy + uz);
+            }
+#undef ADDU
+#undef SUBD
+#undef K
+}
+
+/* FwiElastic3D::scaleGrad (fwi.cpp:5553-5630): lambda/mu/rho kernels ->
+ * Vp/Vs/rho gradients in place.  The rho gradient is vp^2 lambda + vs^2 mu
+ * + rho as the reference forms it. */
+void FN(orc_e3d_scalegrad)(const REAL *Vp, const REAL *Vs, const REAL *Rho, long n, REAL *vpgrad, REAL *vsgrad,
+                           REAL *rhograd)
+{
+    for (long k = 0; k < n; k++) {
+        REAL vpscale = (REAL)(2.0 * (double)Rho[k] * (double)Vp[k]);
+        REAL vsscale = (REAL)(2.0 * (double)Rho[k] * (double)Vs[k]);
+        REAL rhoscale1 = Vp[k] * Vp[k], rhoscale2 = Vs[k] * Vs[k];
+        REAL lambda = vpgrad[k], mu = vsgrad[k], rho = rhograd[k];
+        vpgrad[k] = vpscale * lambda;
+        vsgrad[k] = (REAL)((double)(vsscale * mu) - 2.0 * (double)vsscale * (double)lambda);
+        rhograd[k] = rhoscale1 * lambda + rhoscale2 * mu + rho;
+    }
+}
+
+/* Shapers of the three components of one trace for ADAPTIVE_GAUSS
+ * (fwi.cpp:5773-5830, 6047-6100), with the reference's weighted inputs:
+ * weighted x leaves the record input zero, weighted y the model input zero
+ * (both are only ever assigned the other one), z takes both weighted.
+ * w: 3 * 8 nt (per component 6 nt of adaptive_shaper_, the shaper, and a
+ * weighted-input copy); returns the squared norms in nrm. */
+static void FN(e3_shapers_)(const REAL *const *m, const REAL *const *r, const REAL *const *wt, int nt, REAL *w,
+                            REAL *nrm)
+{
+    for (int c = 0; c < 3; c++) {
+        REAL *wc = w + (long)c * 8 * nt, *sh = wc + 6 * nt, *in = wc + 7 * nt;
+        if (wt[c] && c < 2) {
+            /* one input weighted, the other zero */
+            for (int it = 0; it < nt; it++) in[it] = c == 0 ? m[c][it] * wt[c][it] : r[c][it] * wt[c][it];
+            REAL *zero = (REAL *)calloc(nt > 0 ? nt : 1, sizeof(REAL));
+            nrm[c] = c == 0 ? FN(adaptive_shaper_)(in, zero, NULL, nt, wc, sh)
+                            : FN(adaptive_shaper_)(zero, in, NULL, nt, wc, sh);
+            free(zero);
+        } else {
+            nrm[c] = FN(adaptive_shaper_)(m[c], r[c], wt[c], nt, wc, sh);
+        }
+    }
+}
+
+/* computeMisfit (fwi.cpp:5632-5899) of the Ux, Uy, Uz records, every trace:
+ * DIFFERENCE and the default branch 0.5 (rx^2 + ry^2 + rz^2) of the weighted
+ * differences, CORRELATION -sum mod rec / (|mod||rec|) weighted (added in
+ * T), ADAPTIVE_GAUSS 0.5 sum (H s)^2 / |s|^2 over the three shapers. */
+REAL FN(orc_misfit_e3)(int type, const REAL *modx, const REAL *recx, const REAL *mody, const REAL *recy,
+                       const REAL *modz, const REAL *recz, const REAL *weix, const REAL *weiy, const REAL *weiz, int ntr,
+                       int nt)
+{
+    REAL misfit = 0.0;
+    REAL *w = (REAL *)malloc(sizeof(REAL) * 24 * (nt > 0 ? nt : 1));
+    for (int i = 0; i < ntr; i++) {
+        const long o = (long)i * nt;
+        const REAL *m[3] = {modx + o, mody + o, modz + o}, *r[3] = {recx + o, recy + o, recz + o};
+        const REAL *wt[3] = {weix ? weix + o : NULL, weiy ? weiy + o : NULL, weiz ? weiz + o : NULL};
+        if (type == 1) {
+            REAL n1[3], n2[3];
+            for (int c = 0; c < 3; c++) {
+                REAL a = 0.0, b = 0.0;
+                for (int it = 0; it < nt; it++) {
+                    a += m[c][it] * m[c][it];
+                    b += r[c][it] * r[c][it];
+                }
+                n1[c] = (REAL)sqrt((double)a);
+                n2[c] = (REAL)sqrt((double)b);
+                if (n1[c] == 0) n1[c] = 1.0;
+                if (n2[c] == 0) n2[c] = 1.0;
+            }
+            for (int it = 0; it < nt; it++) {
+                REAL v[3];
+                for (int c = 0; c < 3; c++) {
+                    v[c] = (REAL)((-1.0) * (double)(m[c][it] * r[c][it] / (n1[c] * n2[c])));
+                    if (wt[c]) v[c] *= wt[c][it];
+                }
+                misfit += (v[0] + v[1] + v[2]);
+            }
+        } else if (type == 2) {
+            REAL nrm[3];
+            FN(e3_shapers_)(m, r, wt, nt, w, nrm);
+            const REAL *sx = w + 6 * nt, *sy = w + 14 * nt, *sz = w + 22 * nt;
+            for (int it = 0; it < nt; it++) {
+                REAL h = FN(adaptive_h_)(2, it, nt);
+                REAL a = h * sx[it], b = h * sy[it], c = h * sz[it];
+                misfit = (REAL)((double)misfit + 0.5 * (double)(a * a / nrm[0] + b * b / nrm[1] + c * c / nrm[2]));
+            }
+        } else {
+            for (int it = 0; it < nt; it++) {
+                REAL v[3];
+                for (int c = 0; c < 3; c++) {
+                    v[c] = m[c][it] - r[c][it];
+                    if (wt[c]) v[c] *= wt[c][it];
+                }
+                misfit = (REAL)((double)misfit + 0.5 * (double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+            }
+        }
+    }
+    free(w);
+    return misfit;
+}
+
+/* computeResiduals (fwi.cpp:5901-6196): DIFFERENCE (mod - rec) w^2 from
+ * sample 0; CORRELATION -(rec/(n1 n2) - mod/(n1 n1) n3) w^2 and the
+ * default branch (mod - rec) w^2 from sample 1 (sample 0 left as it is);
+ * ADAPTIVE_GAUSS per component -(H^2 - 2 misfit) s / |s|^2, solved with the
+ * record's autocorrelation, convolved with the unweighted record, not
+ * weighted. */
+void FN(orc_residual_e3)(int type, const REAL *modx, const REAL *recx, const REAL *mody, const REAL *recy,
+                         const REAL *modz, const REAL *recz, const REAL *weix, const REAL *weiy, const REAL *weiz,
+                         REAL *resx, REAL *resy, REAL *resz, int ntr, int nt)
+{
+    REAL *w = (REAL *)malloc(sizeof(REAL) * 24 * (nt > 0 ? nt : 1));
+    for (int i = 0; i < ntr; i++) {
+        const long o = (long)i * nt;
+        const REAL *m[3] = {modx + o, mody + o, modz + o}, *r[3] = {recx + o, recy + o, recz + o};
+        const REAL *wt[3] = {weix ? weix + o : NULL, weiy ? weiy + o : NULL, weiz ? weiz + o : NULL};
+        REAL *rs[3] = {resx + o, resy + o, resz + o};
+        if (type == 2) {
+            REAL nrm[3];
+            FN(e3_shapers_)(m, r, wt, nt, w, nrm);
+            for (int c = 0; c < 3; c++) {
+                REAL *wc = w + (long)c * 8 * nt, *sh = wc + 6 * nt, mis = 0.0;
+                for (int it = 0; it < nt; it++) {
+                    REAL h = FN(adaptive_h_)(2, it, nt);
+                    mis = (REAL)((double)mis + 0.5 * (double)(h * sh[it]) * (double)(h * sh[it]) / (double)nrm[c]);
+                }
+                for (int it = 0; it < nt; it++) {
+                    REAL h = FN(adaptive_h_)(2, it, nt);
+                    rs[c][it] = (REAL)(-1.0 * ((double)(h * h) - 2.0 * (double)mis) * (double)sh[it] / (double)nrm[c]);
+                }
+                FN(stoep_)(nt, wc + 2 * nt, rs[c], sh, wc + 4 * nt);
+                FN(conv_)(nt, 0, sh, nt, 0, r[c], nt, 0, rs[c]);
+                /* no weighting: the reference's weight multiply (fwi.cpp:6143-6154)
+                 * sits after the sample loop and hits index nt, the next trace's
+                 * sample 0, which that trace then rewrites */
+            }
+            continue;
+        }
+        REAL n1[3] = {1, 1, 1}, n3[3] = {0, 0, 0}, n2[3] = {1, 1, 1};
+        if (type == 1)
+            for (int c = 0; c < 3; c++) {
+                REAL a = 0.0, b = 0.0, x = 0.0;
+                for (int it = 0; it < nt; it++) {
+                    a += m[c][it] * m[c][it];
+                    b += r[c][it] * r[c][it];
+                    x += m[c][it] * r[c][it];
+                }
+                n1[c] = (REAL)sqrt((double)a);
+                n2[c] = (REAL)sqrt((double)b);
+                if (n1[c] == 0) n1[c] = 1.0;
+                if (n2[c] == 0) n2[c] = 1.0;
+                n3[c] = x / (n1[c] * n2[c]);
+            }
+        for (int it = type == 0 ? 0 : 1; it < nt; it++)
+            for (int c = 0; c < 3; c++) {
+                REAL v = type == 1 ? (REAL)((-1.0) * (double)((r[c][it] / (n1[c] * n2[c])) - (m[c][it] / (n1[c] * n1[c])) * n3[c]))
+                                   : m[c][it] - r[c][it];
+                if (wt[c]) v *= wt[c][it] * wt[c][it];
+                rs[c][it] = v;
+            }
+    }
+    free(w);
 }
 
 #undef FN

@@ -426,7 +426,7 @@ class E3DState:
         self.nx, self.ny, self.nz, self.lpml = nxp, nyp, nzp, lpml
         (self.L, self.L2M, self.Mxz, self.Myz, self.Mxy, self.Rx, self.Ry,
          self.Rz) = (np.ascontiguousarray(a, real) for a in model)
-        for k in E3D_FIELDS + ("df",):
+        for k in E3D_FIELDS + ("df", "wrk"):
             setattr(self, k, np.zeros((nzp, nyp, nxp), real))
         shp = {0: (nzp, nyp, lpml), 1: (nzp, lpml, nxp), 2: (lpml, nyp, nxp)}
         self.strips = []
@@ -437,7 +437,7 @@ class E3DState:
         st = STRUCTS[real][3]()
         st.nx, st.ny, st.nz, st.lpml, st.order, st.fs = nxp, nyp, nzp, lpml, order, int(fs)
         st.dx, st.dy, st.dz, st.dt = d3[0], d3[1], d3[2], dt
-        for k in E3D_FIELDS + ("L", "L2M", "Mxz", "Myz", "Mxy", "Rx", "Ry", "Rz", "df", "tab"):
+        for k in E3D_FIELDS + ("L", "L2M", "Mxz", "Myz", "Mxy", "Rx", "Ry", "Rz", "df", "wrk", "tab"):
             setattr(st, k, _p(getattr(self, k), ct))
         for i, a in enumerate(self.strips):
             st.psi[i] = _p(a, ct)
@@ -448,6 +448,12 @@ class E3DState:
 
     def stress(self):
         fn("e3d_stress", self.real)(C.byref(self.st))
+
+    def velocity_bw(self):
+        fn("e3d_velocity_bw", self.real)(C.byref(self.st))
+
+    def stress_bw(self):
+        fn("e3d_stress_bw", self.real)(C.byref(self.st))
 
 
 def e3d_run(vp, vs, rho, lpml, d3, dt, nt, order, fs, srcfield, scoords, wav, recfield, gcoords, real=np.float32):

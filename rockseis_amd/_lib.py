@@ -64,6 +64,16 @@ class FwiA3D(C.Structure):
                    ("misfit_type", C.c_int), ("weight", C.c_void_p), ("adjoint", C.c_int)])
 
 
+class FwiE3D(C.Structure):
+    _fields_ = ([(n, C.c_void_p) for n in ("L", "L2M", "Mxz", "Myz", "Mxy", "Rx", "Ry", "Rz", "tab", "Vp", "Vs",
+                                          "Rho")]
+                + [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("src", Traces), ("obs", Traces * 3),
+                   ("dmod", C.c_void_p * 3), ("dres", C.c_void_p * 3), ("weight", C.c_void_p * 3)]
+                + [(n, C.c_void_p) for n in ("wavgrad", "vpgrad", "vsgrad", "rhograd")]
+                + [("misfit", C.POINTER(C.c_double)), ("ncheck", C.c_int), ("fp16", C.c_int),
+                   ("misfit_type", C.c_int)])
+
+
 class RtmA2D(C.Structure):
     _fields_ = ([(n, C.c_void_p) for n in ("L", "Rx", "Rz", "tab")]
                 + [("nx", C.c_int), ("nz", C.c_int), ("src", Traces), ("data", Traces), ("image", C.c_void_p),
@@ -223,6 +233,9 @@ def _declare(L):
         "rsg_stagger_e3d": ([vp, vp, vp, ip, ip, ip, ip, ip] + [vp] * 8, ip),
         "rsg_e3d_forward_velocity": ([GP, C.POINTER(StateE3D), vp], ip),
         "rsg_e3d_forward_stress": ([GP, C.POINTER(StateE3D), vp], ip),
+        "rsg_e3d_backward_velocity": ([GP, C.POINTER(StateE3D), vp], ip),
+        "rsg_e3d_backward_stress": ([GP, C.POINTER(StateE3D), vp], ip),
+        "rsg_fwi_e3d_gradient": ([GP, C.POINTER(FwiE3D), vp], ip),
         "rsg_e3d_insert_source": ([GP, C.POINTER(StateE3D), TP, ip, vp], ip),
         "rsg_e3d_record_data": ([GP, C.POINTER(StateE3D), TP, vp, vp, vp, ip, vp], ip),
         "rsg_rss_read_header": ([C.c_char_p, C.POINTER(RssHeader)], ip),
@@ -266,7 +279,8 @@ EXPORTS = (
     "rsg_stagger_a2d_dev", "rsg_stagger_e2d_dev", "rsg_misfit_residuals_noalloc",
     "rsg_rss_read_header", "rsg_rss_read_data2d", "rsg_rss_read_data3d", "rsg_rss_write_data2d",
     "rsg_rss_write_data3d", "rsg_rss_read_regular", "rsg_rss_write_regular", "rsg_stack_image_2d",
-    "rsg_stagger_e3d", "rsg_e3d_forward_velocity", "rsg_e3d_forward_stress", "rsg_e3d_insert_source",
+    "rsg_stagger_e3d", "rsg_e3d_forward_velocity", "rsg_e3d_forward_stress", "rsg_e3d_backward_velocity",
+    "rsg_e3d_backward_stress", "rsg_fwi_e3d_gradient", "rsg_e3d_insert_source",
     "rsg_e3d_record_data",
 )
 

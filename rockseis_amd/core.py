@@ -671,6 +671,16 @@ class WavesElastic3D:
         R.check(R.lib().rsg_e3d_forward_stress(C.byref(self._grid(der.order)), C.byref(self._state(model)),
                                                _stream()), "rsg_e3d_forward_stress")
 
+    def backwardstepVelocity(self, model, der):
+        """The adjoint-state velocity step of FwiElastic3D (waves.cpp:3260-3713)."""
+        R.check(R.lib().rsg_e3d_backward_velocity(C.byref(self._grid(der.order)), C.byref(self._state(model)),
+                                                  _stream()), "rsg_e3d_backward_velocity")
+
+    def backwardstepStress(self, model, der):
+        """The adjoint-state stress step of FwiElastic3D (waves.cpp:3715-4103)."""
+        R.check(R.lib().rsg_e3d_backward_stress(C.byref(self._grid(der.order)), C.byref(self._state(model)),
+                                                _stream()), "rsg_e3d_backward_stress")
+
     def getVx(self):
         return self.Vx
 

@@ -14,6 +14,12 @@
 // reduced orders.  Each derivative term is applied as the reference applies
 // it: F += dt M d, then in a strip psi = B psi + A d, F -= dt M (psi + C d).
 // Results agree with the reference to rounding (contraction is on here).
+//
+// The adjoint-state steps of FwiElastic3D (backwardstepVelocity / Stress,
+// waves.cpp:3260-4103) share the kernels: the velocity step differentiates
+// the moduli-weighted stresses (L2M Sxx + L Syy + L Szz, M_xy Sxy, ...),
+// formed at each stencil point where the reference first fills its wrk
+// array, and the stress step adds bare velocity derivatives.
 #include <vector>
 
 #include "rsg_common.h"
@@ -85,6 +91,41 @@ __device__ __forceinline__ float deriv(const K3E &k, const Cell &cl, const float
     return FWD ? dfw_k(s, st, kk, k.invd[a]) : dbw_k(s, st, kk, k.invd[a]);
 }
 
+// Moduli-weighted sum of N stress fields at a cell: the wrk array of
+// backwardstepVelocity, m[0] f[0] + m[1] f[1] + m[2] f[2] in that order.
+template <int N>
+struct Mix {
+    const float *f[N], *m[N];
+    __device__ __forceinline__ float operator()(long q) const {
+        float v = m[0][q] * f[0][q];
+#pragma unroll
+        for (int i = 1; i < N; i++) v += m[i][q] * f[i][q];
+        return v;
+    }
+};
+
+// deriv for a field given by a functor of the cell index.
+template <int ORDER, bool FWD, class G>
+__device__ __forceinline__ float derivm(const K3E &k, const Cell &cl, const G &g, int a) {
+    const int n = a == 0 ? k.nx : (a == 1 ? k.ny : k.nz);
+    const int kk = FWD ? kfw(cl.c[a], n, ORDER) : kbw(cl.c[a], n, ORDER);
+    if (kk <= 0) return 0.f;
+    const long p = cl.p, st = cl.st[a];
+    if (kk == ORDER) {  // coefficients pre-divided by the step
+        const float *c = k.c[a].c + cind(ORDER);
+        float acc = 0.f;
+#pragma unroll
+        for (int io = 0; io < ORDER; io++)
+            acc += FWD ? c[io] * (g(p + (io + 1) * st) - g(p - io * st)) : c[io] * (g(p + io * st) - g(p - (io + 1) * st));
+        return acc;
+    }
+    const float *c = kCoefC + cind(kk);
+    float acc = 0.f;
+    for (int io = 0; io < kk; io++)
+        acc += FWD ? c[io] * (g(p + (io + 1) * st) - g(p - io * st)) : c[io] * (g(p + io * st) - g(p - (io + 1) * st));
+    return acc * k.invd[a];
+}
+
 // One derivative term into up to three targets sharing one strip.
 __device__ __forceinline__ void term3(const K3E &k, const Cell &cl, int a, int stag, int slot, float d, float &f1,
                                       float m1, float *f2, float m2, float *f3, float m3) {
@@ -128,23 +169,59 @@ __global__ __launch_bounds__(256) void e3d_velocity_kernel(const K3E k) {
     k.V[2][p] = vz;
 }
 
+// backwardstepVelocity (waves.cpp:3260-3713): the forward velocity step's
+// terms and strips on the moduli-weighted stresses.
 template <int ORDER>
+__global__ __launch_bounds__(256) void e3d_velocity_bw_kernel(const K3E k) {
+    Cell cl;
+    if (!cell_of(k, cl)) return;
+    const long p = cl.p;
+    const float rx = k.R[0][p], ry = k.R[1][p], rz = k.R[2][p];
+    float vx = k.V[0][p], vy = k.V[1][p], vz = k.V[2][p];
+    const Mix<3> nx{{k.Sxx, k.Syy, k.Szz}, {k.L2M, k.L, k.L}};
+    const Mix<3> ny{{k.Sxx, k.Syy, k.Szz}, {k.L, k.L2M, k.L}};
+    const Mix<3> nz{{k.Sxx, k.Syy, k.Szz}, {k.L, k.L, k.L2M}};
+    const Mix<1> sxy{{k.Sxy}, {k.Mxy}}, sxz{{k.Sxz}, {k.Mxz}}, syz{{k.Syz}, {k.Myz}};
+    term3(k, cl, 0, 1, RSG_E3D_SXX_LEFT, derivm<ORDER, true>(k, cl, nx, 0), vx, rx, nullptr, 0.f, nullptr, 0.f);
+    term3(k, cl, 1, 0, RSG_E3D_SXYY_FRONT, derivm<ORDER, false>(k, cl, sxy, 1), vx, rx, nullptr, 0.f, nullptr, 0.f);
+    term3(k, cl, 2, 0, RSG_E3D_SXZZ_TOP, derivm<ORDER, false>(k, cl, sxz, 2), vx, rx, nullptr, 0.f, nullptr, 0.f);
+    term3(k, cl, 0, 0, RSG_E3D_SXYX_LEFT, derivm<ORDER, false>(k, cl, sxy, 0), vy, ry, nullptr, 0.f, nullptr, 0.f);
+    term3(k, cl, 1, 1, RSG_E3D_SYY_FRONT, derivm<ORDER, true>(k, cl, ny, 1), vy, ry, nullptr, 0.f, nullptr, 0.f);
+    term3(k, cl, 2, 0, RSG_E3D_SYZZ_TOP, derivm<ORDER, false>(k, cl, syz, 2), vy, ry, nullptr, 0.f, nullptr, 0.f);
+    term3(k, cl, 0, 0, RSG_E3D_SXZX_LEFT, derivm<ORDER, false>(k, cl, sxz, 0), vz, rz, nullptr, 0.f, nullptr, 0.f);
+    term3(k, cl, 1, 0, RSG_E3D_SYZY_FRONT, derivm<ORDER, false>(k, cl, syz, 1), vz, rz, nullptr, 0.f, nullptr, 0.f);
+    term3(k, cl, 2, 1, RSG_E3D_SZZ_TOP, derivm<ORDER, true>(k, cl, nz, 2), vz, rz, nullptr, 0.f, nullptr, 0.f);
+    k.V[0][p] = vx;
+    k.V[1][p] = vy;
+    k.V[2][p] = vz;
+}
+
+// forwardstepStress, or with BW backwardstepStress (waves.cpp:3715-4103):
+// the same terms and strips with unit moduli.
+template <int ORDER, bool BW>
 __global__ __launch_bounds__(256) void e3d_stress_kernel(const K3E k) {
     Cell cl;
     if (!cell_of(k, cl)) return;
     const long p = cl.p;
-    const float l = k.L[p], l2m = k.L2M[p];
     float sxx = k.Sxx[p], syy = k.Syy[p], szz = k.Szz[p];
-    term3(k, cl, 0, 0, RSG_E3D_VXX_LEFT, deriv<ORDER, false>(k, cl, k.V[0], 0), sxx, l2m, &syy, l, &szz, l);
-    term3(k, cl, 1, 0, RSG_E3D_VYY_FRONT, deriv<ORDER, false>(k, cl, k.V[1], 1), sxx, l, &syy, l2m, &szz, l);
-    term3(k, cl, 2, 0, RSG_E3D_VZZ_TOP, deriv<ORDER, false>(k, cl, k.V[2], 2), sxx, l, &syy, l, &szz, l2m);
+    if (BW) {
+        term3(k, cl, 0, 0, RSG_E3D_VXX_LEFT, deriv<ORDER, false>(k, cl, k.V[0], 0), sxx, 1.f, nullptr, 0.f, nullptr, 0.f);
+        term3(k, cl, 1, 0, RSG_E3D_VYY_FRONT, deriv<ORDER, false>(k, cl, k.V[1], 1), syy, 1.f, nullptr, 0.f, nullptr,
+              0.f);
+        term3(k, cl, 2, 0, RSG_E3D_VZZ_TOP, deriv<ORDER, false>(k, cl, k.V[2], 2), szz, 1.f, nullptr, 0.f, nullptr, 0.f);
+    } else {
+        const float l = k.L[p], l2m = k.L2M[p];
+        term3(k, cl, 0, 0, RSG_E3D_VXX_LEFT, deriv<ORDER, false>(k, cl, k.V[0], 0), sxx, l2m, &syy, l, &szz, l);
+        term3(k, cl, 1, 0, RSG_E3D_VYY_FRONT, deriv<ORDER, false>(k, cl, k.V[1], 1), sxx, l, &syy, l2m, &szz, l);
+        term3(k, cl, 2, 0, RSG_E3D_VZZ_TOP, deriv<ORDER, false>(k, cl, k.V[2], 2), sxx, l, &syy, l, &szz, l2m);
+    }
     // free surface: the plane lpml lies in no z strip, so zeroing after the
     // strips is the reference's zeroing before them
     if (k.fs && cl.c[2] == k.lpml) szz = 0.f;
     k.Sxx[p] = sxx;
     k.Syy[p] = syy;
     k.Szz[p] = szz;
-    const float mxz = k.Mxz[p], myz = k.Myz[p], mxy = k.Mxy[p];
+    const float mxz = BW ? 1.f : k.Mxz[p], myz = BW ? 1.f : k.Myz[p], mxy = BW ? 1.f : k.Mxy[p];
     float sxz = k.Sxz[p], syz = k.Syz[p], sxy = k.Sxy[p];
     term3(k, cl, 0, 1, RSG_E3D_VZX_LEFT, deriv<ORDER, true>(k, cl, k.V[2], 0), sxz, mxz, nullptr, 0.f, nullptr, 0.f);
     term3(k, cl, 2, 1, RSG_E3D_VXZ_TOP, deriv<ORDER, true>(k, cl, k.V[0], 2), sxz, mxz, nullptr, 0.f, nullptr, 0.f);
@@ -307,7 +384,29 @@ int rsg_e3d_forward_stress(const rsg_grid *g, const rsg_state_e3d *s, void *stre
     if (int rc = rsg::check_e3d(g, s)) return rc;
     const rsg::K3E k = rsg::make_k3e(g, s);
     const dim3 grid((g->nx + 63) / 64, (g->ny + 3) / 4, g->nz), block(64, 4);
-#define L_(O) rsg::e3d_stress_kernel<O><<<grid, block, 0, (hipStream_t)stream>>>(k)
+#define L_(O) rsg::e3d_stress_kernel<O, false><<<grid, block, 0, (hipStream_t)stream>>>(k)
+    E3D_ORDER_CASES(L_);
+#undef L_
+    RSG_LAUNCH_CHECK("e3d_stress_kernel");
+    return RSG_OK;
+}
+
+int rsg_e3d_backward_velocity(const rsg_grid *g, const rsg_state_e3d *s, void *stream) {
+    if (int rc = rsg::check_e3d(g, s)) return rc;
+    const rsg::K3E k = rsg::make_k3e(g, s);
+    const dim3 grid((g->nx + 63) / 64, (g->ny + 3) / 4, g->nz), block(64, 4);
+#define L_(O) rsg::e3d_velocity_bw_kernel<O><<<grid, block, 0, (hipStream_t)stream>>>(k)
+    E3D_ORDER_CASES(L_);
+#undef L_
+    RSG_LAUNCH_CHECK("e3d_velocity_bw_kernel");
+    return RSG_OK;
+}
+
+int rsg_e3d_backward_stress(const rsg_grid *g, const rsg_state_e3d *s, void *stream) {
+    if (int rc = rsg::check_e3d(g, s)) return rc;
+    const rsg::K3E k = rsg::make_k3e(g, s);
+    const dim3 grid((g->nx + 63) / 64, (g->ny + 3) / 4, g->nz), block(64, 4);
+#define L_(O) rsg::e3d_stress_kernel<O, true><<<grid, block, 0, (hipStream_t)stream>>>(k)
     E3D_ORDER_CASES(L_);
 #undef L_
     RSG_LAUNCH_CHECK("e3d_stress_kernel");

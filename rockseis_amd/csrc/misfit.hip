@@ -291,19 +291,16 @@ __device__ void stoep_dev(int n, const float *r, const float *g, float *f, float
     }
 }
 
-// Shaping filter of one trace: autocorrelation of the weighted record and
-// its crosscorrelation with the weighted model trace (the record reversed,
-// convolved), 1% white noise on the zero lag, Levinson solve.  w: 6 nt
-// scratch (weighted model, weighted record, autocorrelation,
+// Shaping filter of one trace (shaper_dev): autocorrelation of the weighted
+// record and its crosscorrelation with the weighted model trace (the record
+// reversed, convolved), 1% white noise on the zero lag, Levinson solve.
+// w: 6 nt scratch (weighted model, weighted record, autocorrelation,
 // crosscorrelation, Levinson scratch, reversed copy); returns the shaper's
-// squared norm (1 when zero).
-__device__ float shaper_dev(const float *m, const float *r, const float *wt, int nt, float *w, float *shaper) {
+// squared norm (1 when zero).  shaper_weighted_dev starts from the weighted
+// inputs already in w[0, nt) (model) and w[nt, 2 nt) (record).
+__device__ float shaper_weighted_dev(int nt, float *w, float *shaper) {
     float *modwei = w, *recwei = w + nt, *autoc = w + 2 * nt, *cross = w + 3 * nt, *spiker = w + 4 * nt,
           *xr = w + 5 * nt;
-    for (int it = 0; it < nt; it++) {
-        modwei[it] = wt ? m[it] * wt[it] : m[it];
-        recwei[it] = wt ? r[it] * wt[it] : r[it];
-    }
     for (int i = 0, j = nt - 1; i < nt; ++i, --j) xr[i] = recwei[j];
     conv_dev(nt, 1 - nt, xr, nt, 0, recwei, nt, 0, autoc);
     conv_dev(nt, 1 - nt, xr, nt, 0, modwei, nt, 0, cross);
@@ -317,11 +314,19 @@ __device__ float shaper_dev(const float *m, const float *r, const float *wt, int
     return norm;
 }
 
+__device__ float shaper_dev(const float *m, const float *r, const float *wt, int nt, float *w, float *shaper) {
+    for (int it = 0; it < nt; it++) {
+        w[it] = wt ? m[it] * wt[it] : m[it];
+        w[nt + it] = wt ? r[it] * wt[it] : r[it];
+    }
+    return shaper_weighted_dev(nt, w, shaper);
+}
+
 // Residual of one trace after shaper_dev on the same scratch: the shaper's
 // own misfit in float, rs = sgn (H^2 - 2 misfit) shaper / norm, solved again
 // with the autocorrelation, convolved with the weighted record, weighted.
 __device__ void adaptive_res_dev(int type, int nt, const float *wt, const float *H, float *w, float *shaper,
-                                 float norm, int scaling, float *rs) {
+                                 float norm, int scaling, float *rs, const float *conv_rec = nullptr) {
     float mis = 0.f;
     for (int it = 0; it < nt; it++) {
         const float v = H[it] * shaper[it];
@@ -331,7 +336,7 @@ __device__ void adaptive_res_dev(int type, int nt, const float *wt, const float 
     for (int it = 0; it < nt; it++)
         rs[it] = (float)(sgn * ((double)(H[it] * H[it]) - 2.0 * (double)mis) * (double)shaper[it] / (double)norm);
     stoep_dev(nt, w + 2 * nt, rs, shaper, w + 4 * nt);
-    conv_dev(nt, 0, shaper, nt, 0, w + nt, nt, 0, rs);
+    conv_dev(nt, 0, shaper, nt, 0, conv_rec ? conv_rec : w + nt, nt, 0, rs);
     if (wt)
         for (int it = 0; it < nt; it++) rs[it] *= wt[it];
 }
@@ -905,6 +910,164 @@ int elastic_misfit_residuals(int type, const std::vector<E2DRecs> &shots_in, int
     e2d_clip_kernel<<<dim3(blocks, ns), kThreads, 0, st>>>(dsh, nt, dpc, type == 1);
     RSG_LAUNCH_CHECK("e2d_clip_kernel");
     RSG_HIP(hipStreamSynchronize(st));  // the scratch is freed on return
+    return RSG_OK;
+}
+
+// ---- Elastic 3D records (FwiElastic3D::computeMisfit / computeResiduals,
+// fwi.cpp:5632-6196): Ux, Uy, Uz, every trace counts.
+
+namespace {
+
+__device__ inline double block_sum(double acc) {
+    __shared__ double part[kThreads];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    return part[0];
+}
+
+// DIFFERENCE and the default branch: misfit 0.5 (a^2 + b^2 + c^2) of the
+// weighted differences; residuals (mod - rec) w^2, from sample 1 in the
+// default branch.  CORRELATION (norms given): misfit -mod rec / (n1 n2)
+// weighted, summed over the components; residuals from sample 1,
+// -(rec / (n1 n2) - mod / (n1 n1) n3) w^2.
+__global__ __launch_bounds__(kThreads) void e3d_misfit_res_kernel(int type, const E3DRecs r, int nt,
+                                                                   const float *__restrict__ norms, double *misfit) {
+    const size_t n = (size_t)r.nrec * nt;
+    double acc = 0.0;
+    for (size_t p = blockIdx.x * (size_t)kThreads + threadIdx.x; p < n; p += (size_t)gridDim.x * kThreads) {
+        const int it = (int)(p % nt), tr = (int)(p / nt);
+        float v[3];
+        for (int c = 0; c < 3; c++) {
+            const float m = r.mod[c][p], o = r.obs[c][p];
+            const float w = r.wei[c] ? r.wei[c][p] : 1.f;
+            if (type == 1) {
+                const float *nn = norms + 3 * ((size_t)c * r.nrec + tr);
+                const float n1 = nn[0], n2 = nn[1], n3 = nn[2];
+                v[c] = -(m * o / (n1 * n2));
+                if (r.wei[c]) v[c] *= w;
+                if (it >= 1) {
+                    float q = -((o / (n1 * n2)) - (m / (n1 * n1)) * n3);
+                    if (r.wei[c]) q *= w * w;
+                    r.res[c][p] = q;
+                }
+            } else {
+                const float d = m - o;
+                v[c] = r.wei[c] ? d * w : d;
+                if (type == 0 || it >= 1) r.res[c][p] = r.wei[c] ? d * (w * w) : d;
+            }
+        }
+        acc += type == 1 ? (double)(v[0] + v[1] + v[2]) : 0.5 * (double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) atomicAdd(misfit, t);
+}
+
+// CORRELATION norms, one lane per (trace, component): |mod|, |rec| (zero
+// replaced by 1), sum(mod rec) / (|mod| |rec|).
+__global__ __launch_bounds__(kThreads) void e3d_corr_norms_kernel(const E3DRecs r, int nt, float *norms) {
+    const int tr = blockIdx.x * kThreads + threadIdx.x, c = blockIdx.y;
+    if (tr >= r.nrec) return;
+    const float *m = r.mod[c] + (size_t)tr * nt, *o = r.obs[c] + (size_t)tr * nt;
+    float a = 0.f, b = 0.f, x = 0.f;
+    for (int it = 0; it < nt; it++) {
+        a += m[it] * m[it];
+        b += o[it] * o[it];
+        x += m[it] * o[it];
+    }
+    a = (float)sqrt((double)a);
+    b = (float)sqrt((double)b);
+    if (a == 0.f) a = 1.f;
+    if (b == 0.f) b = 1.f;
+    float *q = norms + 3 * ((size_t)c * r.nrec + tr);
+    q[0] = a;
+    q[1] = b;
+    q[2] = x / (a * b);
+}
+
+// ADAPTIVE_GAUSS, one lane per trace (traces [j0, j0 + nj)), scratch 21 nt
+// per lane (per component shaper_dev's 6 nt and the shaper).  The shaper
+// inputs are the reference's (fwi.cpp:5765-5790): a weighted Ux keeps only
+// the weighted model (the record input stays zero), a weighted Uy only the
+// weighted record, Uz both.  Misfit +0.5 sum_t (H sx)^2/|sx|^2 + (H sy)^2/|sy|^2
+// + (H sz)^2/|sz|^2; residuals as adaptive_res_dev, convolved with the
+// unweighted record and not weighted (the reference's weight multiply falls
+// on the next trace's sample 0, which that trace rewrites).
+__global__ __launch_bounds__(64) void e3d_adaptive_kernel(const E3DRecs r, int j0, int nj, int nt,
+                                                          const float *__restrict__ H, float *__restrict__ scratch,
+                                                          double *misfit) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nj) return;
+    const size_t o = (size_t)(j0 + j) * nt;
+    float *w = scratch + (size_t)j * 21 * nt;
+    float nrm[3];
+    for (int c = 0; c < 3; c++) {
+        float *wc = w + (size_t)c * 7 * nt;
+        const float *m = r.mod[c] + o, *rc = r.obs[c] + o, *wt = r.wei[c] ? r.wei[c] + o : nullptr;
+        for (int it = 0; it < nt; it++) {
+            float a = m[it], b = rc[it];
+            if (wt) {
+                a = c == 1 ? 0.f : a * wt[it];
+                b = c == 0 ? 0.f : b * wt[it];
+            }
+            wc[it] = a;
+            wc[nt + it] = b;
+        }
+        nrm[c] = shaper_weighted_dev(nt, wc, wc + 6 * nt);
+    }
+    const float *sx = w + 6 * nt, *sy = w + 13 * nt, *sz = w + 20 * nt;
+    double acc = 0.0;
+    for (int it = 0; it < nt; it++) {
+        const float a = H[it] * sx[it], b = H[it] * sy[it], c = H[it] * sz[it];
+        acc += 0.5 * (double)(a * a / nrm[0] + b * b / nrm[1] + c * c / nrm[2]);
+    }
+    atomicAdd(misfit, acc);
+    for (int c = 0; c < 3; c++) {
+        float *wc = w + (size_t)c * 7 * nt;
+        adaptive_res_dev(2, nt, nullptr, H, wc, wc + 6 * nt, nrm[c], 1, r.res[c] + o, r.obs[c] + o);
+    }
+}
+
+}  // namespace
+
+int elastic3d_misfit_residuals(int type, const E3DRecs &r, int nt, double *misfit, hipStream_t st) {
+    if (type < 0 || type > 4)
+        return fail(RSG_EINVAL,
+                    "fwi_e3d: misfit type must be DIFFERENCE (0), CORRELATION (1), ADAPTIVE_GAUSS (2), "
+                    "ADAPTIVE_LINEAR (3) or ADAPTIVE_SINGLE (4)");
+    const size_t n = (size_t)r.nrec * nt;
+    if (n == 0) return RSG_OK;
+    FreeList fr;
+    const char *oom = "fwi_e3d: misfit scratch allocation failed";
+    if (type == 2) {
+        const std::vector<float> h = adaptive_h(2, nt);
+        const size_t per = 21 * (size_t)nt;
+        const size_t lanes = std::max<size_t>(64, std::min<size_t>(r.nrec, ((size_t)1 << 28) / per));
+        float *hd = nullptr, *scr = nullptr;
+        if (int rc = fr.alloc(&hd, nt, oom)) return rc;
+        if (int rc = fr.alloc(&scr, per * lanes, oom)) return rc;
+        RSG_HIP(hipMemcpyAsync(hd, h.data(), sizeof(float) * nt, hipMemcpyHostToDevice, st));
+        for (size_t j0 = 0; j0 < (size_t)r.nrec; j0 += lanes) {
+            const int nj = (int)std::min(lanes, (size_t)r.nrec - j0);
+            e3d_adaptive_kernel<<<(nj + 63) / 64, 64, 0, st>>>(r, (int)j0, nj, nt, hd, scr, misfit);
+            RSG_LAUNCH_CHECK("e3d_adaptive_kernel");
+        }
+        RSG_HIP(hipStreamSynchronize(st));  // the scratch is freed on return
+        return RSG_OK;
+    }
+    float *norms = nullptr;
+    if (type == 1) {
+        if (int rc = fr.alloc(&norms, 9 * (size_t)r.nrec, oom)) return rc;
+        e3d_corr_norms_kernel<<<dim3((r.nrec + kThreads - 1) / kThreads, 3), kThreads, 0, st>>>(r, nt, norms);
+        RSG_LAUNCH_CHECK("e3d_corr_norms_kernel");
+    }
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>((n + kThreads - 1) / kThreads, 1024));
+    e3d_misfit_res_kernel<<<blocks, kThreads, 0, st>>>(type, r, nt, norms, misfit);
+    RSG_LAUNCH_CHECK("e3d_misfit_res_kernel");
+    if (norms) RSG_HIP(hipStreamSynchronize(st));  // the scratch is freed on return
     return RSG_OK;
 }
 

@@ -55,4 +55,18 @@ struct E2DRecs {
 int elastic_misfit_residuals(int type, const std::vector<E2DRecs> &shots, int nt, double *misfit,
                              hipStream_t st);
 
+// The Ux (0), Uy (1) and Uz (2) records of one elastic 3D shot, [nrec][nt]
+// each on the device; wei[c] may be null.
+struct E3DRecs {
+    int nrec;
+    const float *mod[3], *obs[3], *wei[3];
+    float *res[3];
+};
+
+// Elastic 3D misfit (added to *misfit in double) and residuals
+// (FwiElastic3D::computeMisfit / computeResiduals, fwi.cpp:5632-6196), type
+// 0..4 as rs_fwimisfit (3 and 4: the default branch).  Types 1 and 2
+// allocate scratch and synchronize the stream; the others only enqueue.
+int elastic3d_misfit_residuals(int type, const E3DRecs &r, int nt, double *misfit, hipStream_t st);
+
 }  // namespace rsg

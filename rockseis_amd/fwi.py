@@ -28,8 +28,8 @@ import numpy as np
 import torch
 
 from . import _lib as R
-from .core import (GMAP, SMAP, Data2D, Data3D, ModelAcoustic2D, ModelAcoustic3D, ModelElastic2D, Pml, SMAX, _dev,
-                   _stream)
+from .core import (GMAP, SMAP, Data2D, Data3D, ModelAcoustic2D, ModelAcoustic3D, ModelElastic2D, ModelElastic3D, Pml,
+                   SMAX, _dev, _stream)
 
 DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS, ADAPTIVE_LINEAR, ADAPTIVE_SINGLE = 0, 1, 2, 3, 4  # rs_fwimisfit (utils.h:26)
 # the acoustic classes have no ADAPTIVE_SINGLE branch: it takes their default one, DIFFERENCE
@@ -959,6 +959,144 @@ class FwiAcoustic3D:
             self.datamodP.data[...] = dmod.cpu().numpy()
         if self.dataresP is not None:
             self.dataresP.data[...] = dres.cpu().numpy()
+        if self.wavgrad is not None:
+            self.wavgrad.data[...] = wg.cpu().numpy()
+        return 1
+
+
+class FwiElastic3D:
+    """FwiElastic3D(model, source, dataUx, dataUy, dataUz, order, snapinc)
+    with the reference's setters and run_optimal() (fwi.cpp:6770-6991):
+    Revolve checkpoints of the nine fields and 36 strips in HBM (FP32, or
+    16-bit with setFp16Storage), misfit types DIFFERENCE, CORRELATION,
+    ADAPTIVE_GAUSS and the reference's default branch, Vp / Vs / rho and
+    wavelet gradients.  The observed records are VX, VY, VZ traces with
+    their makeMap(GMAP) maps."""
+
+    def __init__(self, model: ModelElastic3D, source: Data3D, dataUx: Data3D, dataUy: Data3D, dataUz: Data3D,
+                 order=4, snapinc=1):
+        self.model, self.source = model, source
+        self.dataUx, self.dataUy, self.dataUz = dataUx, dataUy, dataUz
+        self.order = int(order) if 1 < int(order) < 9 else 4      # fwi.cpp:28-33
+        self.snapinc = max(int(snapinc), 1)
+        self.ncheck, self.incore, self.fp16 = 0, True, False
+        self.datamodUx = self.datamodUy = self.datamodUz = None
+        self.dataresUx = self.dataresUy = self.dataresUz = None
+        self.vpgrad = self.vsgrad = self.rhograd = self.wavgrad = None
+        self.dataweightx = self.dataweighty = self.dataweightz = None
+        self.misfit = 0.0
+        self.misfit_type = DIFFERENCE
+
+    def setDatamodUx(self, d):
+        self.datamodUx = d
+
+    def setDatamodUy(self, d):
+        self.datamodUy = d
+
+    def setDatamodUz(self, d):
+        self.datamodUz = d
+
+    def setDataresUx(self, d):
+        self.dataresUx = d
+
+    def setDataresUy(self, d):
+        self.dataresUy = d
+
+    def setDataresUz(self, d):
+        self.dataresUz = d
+
+    def setDataweightx(self, d):
+        self.dataweightx = d
+
+    def setDataweighty(self, d):
+        self.dataweighty = d
+
+    def setDataweightz(self, d):
+        self.dataweightz = d
+
+    def setVpgrad(self, img):
+        self.vpgrad = img
+
+    def setVsgrad(self, img):
+        self.vsgrad = img
+
+    def setRhograd(self, img):
+        self.rhograd = img
+
+    def setWavgrad(self, d):
+        self.wavgrad = d
+
+    def setNcheck(self, n):
+        self.ncheck = int(n)
+
+    def setIncore(self, v):
+        self.incore = bool(v)
+
+    def setFp16Storage(self, v):
+        self.fp16 = storage_kind(v)
+
+    def setMisfit_type(self, t):
+        if int(t) not in range(5):
+            raise R.RsgError("the elastic misfit type is one of rs_fwimisfit's 0..4")
+        self.misfit_type = int(t)
+
+    def getMisfit(self):
+        return self.misfit
+
+    def run_optimal(self):
+        dev = _dev()
+        m, src = self.model, self.source
+        obs = (self.dataUx, self.dataUy, self.dataUz)
+        if tuple(d.field for d in obs) != (R.VX, R.VY, R.VZ):
+            raise R.RsgError("FwiElastic3D takes Ux, Uy, Uz data as VX, VY, VZ traces")
+        if self.vpgrad is None and self.vsgrad is None and self.rhograd is None and self.wavgrad is None:
+            raise R.RsgError("FwiElastic3D::crossCorr: No gradient set for computation.")
+        if getattr(m, "_dL2M", None) is None or tuple(m._dL2M.shape) != (m.getNz_pml(), m.getNy_pml(), m.getNx_pml()):
+            m.staggerModels()
+        pml = Pml(m.lpml, src.dt)
+        pml.setSmax(SMAX)
+        pml.computeABC()
+        g = R.Grid(nx=m.getNx_pml(), ny=m.getNy_pml(), nz=m.getNz_pml(), lpml=m.lpml, order=self.order,
+                   fs=int(m.fs), dx=m.dx, dy=m.dy, dz=m.dz, dt=src.dt)
+        t_src, keep = src._device(SMAP)
+        f = R.FwiE3D()
+        for n in ("L", "L2M", "Mxz", "Myz", "Mxy", "Rx", "Ry", "Rz", "Vp", "Vs"):
+            setattr(f, n, getattr(m, "_d" + n).data_ptr())
+        f.Rho, f.tab = m._dR.data_ptr(), pml.tab.data_ptr()
+        f.nx, f.ny, f.nz = m.nx, m.ny, m.nz
+        f.src = t_src
+        ntr, nt = obs[0].ntrace, obs[0].nt
+        dmod = torch.zeros((3, ntr, nt), dtype=torch.float32, device=dev)
+        dres = torch.zeros_like(dmod)
+        for c, d in enumerate(obs):
+            t, k = d._device(GMAP)
+            keep.append(k)
+            f.obs[c] = t
+            f.dmod[c], f.dres[c] = dmod[c].data_ptr(), dres[c].data_ptr()
+        for c, w in enumerate((self.dataweightx, self.dataweighty, self.dataweightz)):
+            if w is not None:
+                wt = torch.from_numpy(np.ascontiguousarray(w.data, np.float32)).to(dev)
+                keep.append(wt)
+                f.weight[c] = wt.data_ptr()
+        grads = [torch.zeros((m.nz, m.ny, m.nx), dtype=torch.float32, device=dev) for _ in range(3)]
+        f.vpgrad, f.vsgrad, f.rhograd = (t.data_ptr() for t in grads)
+        wg = torch.zeros((src.ntrace, src.nt), dtype=torch.float32, device=dev)
+        f.wavgrad = wg.data_ptr()
+        misfit = C.c_double(0.0)
+        f.misfit = C.pointer(misfit)
+        f.ncheck, f.fp16, f.misfit_type = self.ncheck, int(self.fp16), self.misfit_type
+        R.check(R.lib().rsg_fwi_e3d_gradient(C.byref(g), C.byref(f), _stream()), "rsg_fwi_e3d_gradient")
+        self.misfit64 = misfit.value
+        self.misfit = float(np.float32(misfit.value))
+        for img, t in zip((self.vpgrad, self.vsgrad, self.rhograd), grads):
+            if img is not None:
+                img.imagedata[...] = t.cpu().numpy()
+        for c, (dm, dr) in enumerate(((self.datamodUx, self.dataresUx), (self.datamodUy, self.dataresUy),
+                                      (self.datamodUz, self.dataresUz))):
+            if dm is not None:
+                dm.data[...] = dmod[c].cpu().numpy()
+            if dr is not None:
+                dr.data[...] = dres[c].cpu().numpy()
         if self.wavgrad is not None:
             self.wavgrad.data[...] = wg.cpu().numpy()
         return 1

@@ -506,6 +506,43 @@ def domdec_fixture(R):
 GROUPS["domdec"] = domdec_fixture
 GROUPS["e3d"] = e3d_fixtures
 
+FWI_E3D_CASES = [  # order, fs, source field, misfit type, weights (x, y, z), ncheck
+    (4, 0, "P", 0, "", 7), (8, 1, "VZ", 1, "", 5), (4, 1, "VX", 2, "", 9), (6, 0, "P", 3, "xz", 6),
+    (2, 0, "VY", 0, "xyz", 60),
+]
+
+
+def fwi_e3d_fixtures(R):
+    """FwiElastic3D::run_optimal of the compiled reference: Vp, Vs, rho and
+    wavelet gradients, modelled records and residuals of the three
+    components for five cases (orders 2-8, free surface, P / VX / VY / VZ
+    sources, DIFFERENCE, CORRELATION, ADAPTIVE_GAUSS and the default branch,
+    data weights, Revolve with few and with enough snapshots).  The observed
+    records are the reference's own modelling in a 4% faster model."""
+    from oracle import fwi
+    rng = np.random.default_rng(23)
+    nx, ny, nz, lpml, nt, dt, dx = 14, 12, 13, 4, 60, 1e-3, 10.0
+    vp = (2300 + 300 * rng.random((nz, ny, nx))).astype(np.float32)
+    vs = (vp / (1.7 + 0.2 * rng.random((nz, ny, nx)))).astype(np.float32)
+    rho = synth.gardner(vp)
+    sc = np.float32([[63.0, 55.0, 42.0]])
+    wav = (synth.ricker(nt, dt, 40.0) * 1e8)[None].astype(np.float32)
+    gc = np.float32([[20 + 13 * i, 30 + 7 * i, 30.0 + 3 * i] for i in range(6)])
+    for k, (order, fs, sf, mt, wsel, ncheck) in enumerate(FWI_E3D_CASES):
+        obs = np.stack([runs.ref_e3d_run(vp * np.float32(1.04), vs, rho, lpml, (dx, dx, dx), dt, nt, order, fs, sf,
+                                         sc, wav, f, gc)["rec"] for f in ("VX", "VY", "VZ")])
+        wts = [(0.5 + rng.random(obs[0].shape)).astype(np.float32) if a in wsel else None for a in "xyz"]
+        r = fwi.ref_fwi_e3d(vp, vs, rho, lpml, (dx, dx, dx), dt, nt, order, fs, ncheck, sf, sc, wav, gc, obs,
+                            misfit_type=mt, weights=wts)
+        extra = {f"wei{a}": w for a, w in zip("xyz", wts) if w is not None}
+        save(f"fwi_e3d_case{k}", vp=vp, vs=vs, rho=rho, scoords=sc, wav=wav, gcoords=gc, dobs=obs,
+             grid=np.float32([nx, ny, nz, lpml, dx, dt, nt, order, fs, ncheck, mt]), srcfield=np.str_(sf),
+             misfit=np.float32(r["misfit"]), vpgrad=r["vpgrad"], vsgrad=r["vsgrad"], rhograd=r["rhograd"],
+             dmod=r["dmod"], dres=r["dres"], wavgrad=r["wavgrad"], **extra)
+
+
+GROUPS["fwi_e3d"] = fwi_e3d_fixtures
+
 
 if __name__ == "__main__":
     # python -m tests.golden.make_golden [group ...]   (default: every group)

@@ -161,7 +161,7 @@ def test_struct_layouts_match_header(tmp_path):
                "rsg_fwi_a2d": R.Fwi, "rsg_fwi_e2d_shot": R.FwiE2DShot, "rsg_fwi_e2d": R.FwiE2D,
                "rsg_fwi_a3d": R.FwiA3D, "rsg_e2d_records": R.E2DRecords, "rsg_rtm_a2d": R.RtmA2D,
                "rsg_rtm_a3d": R.RtmA3D, "rsg_rss_header": R.RssHeader, "rsg_image2d": R.Image2DDesc,
-               "rsg_state_e3d": R.StateE3D}
+               "rsg_state_e3d": R.StateE3D, "rsg_fwi_e3d": R.FwiE3D}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rsgpu.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'    printf("{cname} %zu\\n", sizeof({cname}));')

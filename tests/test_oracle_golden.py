@@ -8,7 +8,7 @@ import oracle
 from oracle import revolve as orv
 from oracle import runs
 
-from tests.golden.make_golden import A2D_CASES, E2D_CASES, E3D_CASES, REVOLVE_CASES
+from tests.golden.make_golden import A2D_CASES, E2D_CASES, E3D_CASES, FWI_E3D_CASES, REVOLVE_CASES
 
 
 def test_der_all_orders_and_axes(golden):
@@ -112,6 +112,24 @@ def test_e3d_run(golden, case):
                      g["wav"], rf, g["gcoords"])
     for k in ("rec", "Vx", "Vz", "Szz", "Sxy"):
         assert np.array_equal(r[k], g[k]), k
+
+
+@pytest.mark.parametrize("case", range(len(FWI_E3D_CASES)))
+def test_fwi_e3d_gradient(golden, case):
+    """FwiElastic3D::run_optimal restated (oracle/fwi.py fwi_e3d: backward
+    steps, crossCorr, scaleGrad, every misfit branch, data weights) vs the
+    compiled reference, bit for bit: gradients, records, residuals, the
+    wavelet gradient and the misfit."""
+    from oracle.fwi import fwi_e3d
+    g = golden(f"fwi_e3d_case{case}")
+    nx, ny, nz, lpml, dx, dt, nt, order, fs, ncheck, mt = g["grid"]
+    wts = [g[f"wei{a}"] if f"wei{a}" in g.files else None for a in "xyz"]
+    r = fwi_e3d(g["vp"], g["vs"], g["rho"], int(lpml), (dx, dx, dx), dt, int(nt), int(order), int(fs), int(ncheck),
+                str(g["srcfield"]), g["scoords"], g["wav"], g["gcoords"], g["dobs"], misfit_type=int(mt), weights=wts)
+    for k in ("vpgrad", "vsgrad", "rhograd", "dmod", "dres", "wavgrad"):
+        assert np.array_equal(r[k], g[k]), k
+        assert np.abs(g[k]).max() > 0, k
+    assert np.float32(r["misfit"]) == g["misfit"]
 
 
 def test_revolve(golden):
