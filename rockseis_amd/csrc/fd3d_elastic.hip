@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "rsg_common.h"
+#include "rsg_fwi.h"
 
 namespace rsg {
 
@@ -196,6 +197,26 @@ __global__ __launch_bounds__(256) void e3d_velocity_bw_kernel(const K3E k) {
     k.V[2][p] = vz;
 }
 
+// The moduli-weighted stresses of backwardstepVelocity as six arrays: the
+// three normal combinations for Dx+, Dy+, Dz+ and M_xy Sxy, M_xz Sxz,
+// M_yz Syz, formed as e3d_velocity_bw_kernel forms them per stencil point.
+// The forward velocity kernel on these arrays is the backward velocity step.
+__global__ __launch_bounds__(256) void e3d_premul_kernel(const K3E k, float *w0, float *w1, float *w2, float *w3,
+                                                         float *w4, float *w5, size_t n) {
+    const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const Mix<3> nx{{k.Sxx, k.Syy, k.Szz}, {k.L2M, k.L, k.L}};
+    const Mix<3> ny{{k.Sxx, k.Syy, k.Szz}, {k.L, k.L2M, k.L}};
+    const Mix<3> nz{{k.Sxx, k.Syy, k.Szz}, {k.L, k.L, k.L2M}};
+    const Mix<1> sxy{{k.Sxy}, {k.Mxy}}, sxz{{k.Sxz}, {k.Mxz}}, syz{{k.Syz}, {k.Myz}};
+    w0[p] = nx((long)p);
+    w1[p] = ny((long)p);
+    w2[p] = nz((long)p);
+    w3[p] = sxy((long)p);
+    w4[p] = sxz((long)p);
+    w5[p] = syz((long)p);
+}
+
 // forwardstepStress, or with BW backwardstepStress (waves.cpp:3715-4103):
 // the same terms and strips with unit moduli.
 template <int ORDER, bool BW>
@@ -364,6 +385,37 @@ K3E make_k3e(const rsg_grid *g, const rsg_state_e3d *s) {
     }
 
 }  // namespace
+
+}  // namespace rsg
+
+namespace rsg {
+
+// backwardstepVelocity with a scratch of six padded arrays: the weighted
+// stresses once per cell, then the forward velocity kernel on them.  The
+// same values as rsg_e3d_backward_velocity, which forms them per stencil
+// point; this path reads 6 instead of up to 96 values per cell's taps.
+int e3d_backward_velocity_premul(const rsg_grid *g, const rsg_state_e3d *s, float *scratch, hipStream_t st) {
+    if (int rc = check_e3d(g, s)) return rc;
+    if (!scratch) return fail(RSG_EINVAL, "e3d: null scratch");
+    K3E k = make_k3e(g, s);
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    float *w[6];
+    for (int i = 0; i < 6; i++) w[i] = scratch + i * n;
+    e3d_premul_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(k, w[0], w[1], w[2], w[3], w[4], w[5], n);
+    RSG_LAUNCH_CHECK("e3d_premul_kernel");
+    k.Sxx = w[0];
+    k.Syy = w[1];
+    k.Szz = w[2];
+    k.Sxy = w[3];
+    k.Sxz = w[4];
+    k.Syz = w[5];
+    const dim3 grid((g->nx + 63) / 64, (g->ny + 3) / 4, g->nz), block(64, 4);
+#define L_(O) e3d_velocity_kernel<O><<<grid, block, 0, st>>>(k)
+    E3D_ORDER_CASES(L_);
+#undef L_
+    RSG_LAUNCH_CHECK("e3d_velocity_kernel");
+    return RSG_OK;
+}
 
 }  // namespace rsg
 

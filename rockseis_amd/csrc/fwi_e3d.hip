@@ -279,7 +279,7 @@ static int fwi_e3d_impl(const rsg_grid *g, const rsg_fwi_e3d *f, hipStream_t st)
         RSG_HIP(hipMemGetInfo(&free_b, &total_b));
         size_t nblk = 0;
         for (size_t s = 0; s + 1 < segs.size(); s++) nblk += s16_blocks(segs[s + 1] - segs[s]);
-        const size_t need = 2 * total * 4 + CheckStore::bytes(f->fp16, total, nblk, rev.snaps) + 3 * nm * 4;
+        const size_t need = 2 * total * 4 + CheckStore::bytes(f->fp16, total, nblk, rev.snaps) + 3 * nm * 4 + 6 * n * 4;
         if (need + ((size_t)1 << 30) > free_b)
             return fail(RSG_ENOMEM, "fwi_e3d: wavefields and checkpoints do not fit in device memory");
     }
@@ -289,6 +289,8 @@ static int fwi_e3d_impl(const rsg_grid *g, const rsg_fwi_e3d *f, hipStream_t st)
     if (int rc = dm.alloc(&fwb, total)) return rc;
     if (int rc = dm.alloc(&bwb, total)) return rc;
     if (int rc = dm.alloc(&acc, 3 * nm)) return rc;
+    float *wrk = nullptr;  // the backward velocity step's weighted stresses
+    if (int rc = dm.alloc(&wrk, 6 * n)) return rc;
     if (int rc = dm.alloc(&dmis, 1)) return rc;
     if (int rc = launch_zero(fwb, total * sizeof(float), st)) return rc;
     if (int rc = launch_zero(bwb, total * sizeof(float), st)) return rc;
@@ -425,7 +427,7 @@ static int fwi_e3d_impl(const rsg_grid *g, const rsg_fwi_e3d *f, hipStream_t st)
             if (int rc = reverse_tail(capo)) return rc;
             reverse = true;
         } else if (a == Revolve::YOUTURN) {
-            if (int rc = rsg_e3d_backward_velocity(g, &sb, st)) return rc;
+            if (int rc = e3d_backward_velocity_premul(g, &sb, wrk, st)) return rc;
             if (int rc = rsg_e3d_backward_stress(g, &sb, st)) return rc;
             if (int rc = reverse_tail(capo)) return rc;
         } else if (a == Revolve::TAKESHOT) {

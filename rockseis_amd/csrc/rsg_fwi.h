@@ -55,6 +55,11 @@ struct E2DRecs {
 int elastic_misfit_residuals(int type, const std::vector<E2DRecs> &shots, int nt, double *misfit,
                              hipStream_t st);
 
+// backwardstepVelocity of the elastic 3D state with a scratch of six padded
+// grids (fd3d_elastic.hip): the moduli-weighted stresses formed once per
+// cell, then the forward velocity kernel on them.
+int e3d_backward_velocity_premul(const rsg_grid *g, const rsg_state_e3d *s, float *scratch, hipStream_t st);
+
 // The Ux (0), Uy (1) and Uz (2) records of one elastic 3D shot, [nrec][nt]
 // each on the device; wei[c] may be null.
 struct E3DRecs {
