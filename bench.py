@@ -54,10 +54,11 @@ def parse():
                     help="time steps of the single-thread CPU samples of configs 3 and 5")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fwi", action="store_true", help="skip the FWI shots/s leg")
-    ap.add_argument("--legs", default="a3d,e2dfwi",
+    ap.add_argument("--legs", default="a3d,e2dfwi,e3d",
                     help="extra legs, comma separated (empty: none): a3d (acoustic 3D modelling + FWI "
-                         "gradient, 440^3 padded, nt 2500) and e2dfwi (128 ragged elastic FWI windows, "
-                         "nt 4000); about 25 s and 60 s")
+                         "gradient, 440^3 padded, nt 2500), e2dfwi (128 ragged elastic FWI windows, "
+                         "nt 4000) and e3d (elastic 3D kernels and one FwiElastic3D gradient); about 25 s, "
+                         "60 s and 10 s")
     ap.add_argument("--e2dfwi-shots", type=int, default=128)
     ap.add_argument("--e2dfwi-eager", action="store_true",
                     help="also time the elastic batch with eager launches (about 50 s more)")
@@ -479,6 +480,9 @@ def main():
     if "e2dfwi" in legs:
         torch.cuda.empty_cache()
         out["e2dfwi"] = e2dfwi_leg(args.e2dfwi_shots, eager=args.e2dfwi_eager)
+    if "e3d" in legs:
+        torch.cuda.empty_cache()
+        out["e3d"] = e3d_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, inputs, cpu_state, cpu_it0, args.cpu_steps, args.cpu_threads)
         _progress("cpu baseline timed")
@@ -607,6 +611,96 @@ def _rtm_shot(vp0, rho0, lpml, dx, dt, nt, order, shot, nhx):
     t0 = time.perf_counter()
     r.run_optimal()
     return time.perf_counter() - t0
+
+
+def e3d_leg():
+    """Elastic 3D: the four half-step kernels (forward and adjoint-state
+    velocity / stress) at 200^3 (240^3 padded, lpml 20), order 8, free
+    surface, timed with HIP events over 20 back-to-back launches on the
+    launch stream; then one FwiElastic3D gradient of 120^3 (160^3 padded),
+    order 8, nt 500, explosive 15 Hz Ricker, 22 x 22 three-component
+    receivers, 20 Revolve checkpoints in 16 bits (S16), after a warm-up
+    gradient.  Algorithmic bytes per cell (each array touched once):
+    forward velocity 60 (six stresses, three buoyancies, three velocities
+    read; three velocities written), forward stress 80 (three velocities,
+    six stresses, five moduli read; six stresses written), adjoint velocity
+    80 (the forward velocity's arrays and the five moduli), adjoint stress
+    60 (three velocities and six stresses read; six stresses written)."""
+    import numpy as np
+    import torch
+    from rockseis_amd import _lib as R, core, fwi, synth
+    n, L = 200, 20
+    m = core.ModelElastic3D(n, n, n, L, 10.0, 10.0, 10.0, fs=True)
+    m.Vp[:], m.Vs[:], m.R[:] = 3000.0, 1700.0, 2000.0
+    m.staggerModels()
+    w = core.WavesElastic3D(m, 10, 1e-3)
+    w.getPml().setSmax(core.SMAX)
+    w.getPml().computeABC()
+    g, st = w._grid(8), w._state(m)
+    s_ = torch.cuda.current_stream()
+    sp = C.c_void_p(s_.cuda_stream)
+    cells = w.nx_pml * w.ny_pml * w.nz_pml
+    lib = R.lib()
+    kt = {}
+    for name, f, b in (("velocity", lib.rsg_e3d_forward_velocity, 60), ("stress", lib.rsg_e3d_forward_stress, 80),
+                       ("velocity_adjoint_state", lib.rsg_e3d_backward_velocity, 80),
+                       ("stress_adjoint_state", lib.rsg_e3d_backward_stress, 60)):
+        for _ in range(3):
+            R.check(f(C.byref(g), C.byref(st), sp), name)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s_)
+        for _ in range(20):
+            f(C.byref(g), C.byref(st), sp)
+        e1.record(s_)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / 20 * 1e3
+        gbs = cells * b / us / 1e3
+        kt[name] = {"avg_launch_us": round(us, 1), "gcell_per_s": round(cells / us / 1e3, 2),
+                    "bytes_per_cell": b, "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    _progress("e3d: half-step kernels timed")
+    del w, st
+    n, nt, dt = 120, 500, 1e-3
+    m = core.ModelElastic3D(n, n, n, L, 10.0, 10.0, 10.0, fs=True)
+    m.Vp[:], m.Vs[:], m.R[:] = 3000.0, 1700.0, 2000.0
+
+    def d3(coords, field, data=None):
+        c = np.asarray(coords, np.float32).reshape(-1, 3)
+        d = core.Data3D(c.shape[0], nt, dt, 0.0, field)
+        d.scoords[:] = c
+        d.gcoords[:] = c
+        if data is not None:
+            d.data[:] = data
+        return d
+
+    src = d3([[600.0, 600.0, 50.0]], "P", synth.ricker(nt, dt, 15.0)[None] * 1e8)
+    src.makeMap(m, core.SMAP)
+    xy = np.stack(np.meshgrid(np.arange(50, 1150, 50.0), np.arange(50, 1150, 50.0), indexing="ij"), -1).reshape(-1, 2)
+    gc = np.concatenate([xy, np.full((xy.shape[0], 1), 20.0)], 1)
+    rng = np.random.default_rng(0)
+    obs = []
+    for f in ("VX", "VY", "VZ"):
+        d = d3(gc, f, (rng.standard_normal((gc.shape[0], nt)) * 1e-3).astype(np.float32))
+        d.makeMap(m, core.GMAP)
+        obs.append(d)
+    m.staggerModels()
+    fw = fwi.FwiElastic3D(m, src, *obs, 8, 1)
+    imgs = [fwi.Image3D(m) for _ in range(3)]
+    fw.setVpgrad(imgs[0])
+    fw.setVsgrad(imgs[1])
+    fw.setRhograd(imgs[2])
+    fw.setNcheck(20)
+    fw.setFp16Storage("s16")
+    fw.run_optimal()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fw.run_optimal()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    _progress(f"e3d: FWI gradient {wall:.2f} s")
+    return {"workload": "elastic 3D, 200^3 at 10 m (240^3 padded), order 8, free surface: half-step kernels; "
+                        "FwiElastic3D::run_optimal of 120^3 (160^3 padded), order 8, nt 500, 484 x 3 receivers, "
+                        "20 S16 Revolve checkpoints",
+            "kernels": kt, "fwi_gradient_s": round(wall, 3), "fwi_misfit": fw.misfit64}
 
 
 def a3d_leg():
