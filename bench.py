@@ -624,8 +624,10 @@ def e3d_leg():
     forward velocity 60 (six stresses, three buoyancies, three velocities
     read; three velocities written), forward stress 80 (three velocities,
     six stresses, five moduli read; six stresses written), adjoint velocity
-    80 (the forward velocity's arrays and the five moduli), adjoint stress
-    60 (three velocities and six stresses read; six stresses written)."""
+    80 (the forward velocity's arrays and the five moduli; also for the
+    workspace entry, whose premultiplied stresses add 68 B per cell of
+    their own), adjoint stress 60 (three velocities and six stresses read;
+    six stresses written)."""
     import numpy as np
     import torch
     from rockseis_amd import _lib as R, core, fwi, synth
@@ -642,8 +644,11 @@ def e3d_leg():
     cells = w.nx_pml * w.ny_pml * w.nz_pml
     lib = R.lib()
     kt = {}
+    wsp = torch.empty(6 * cells, dtype=torch.float32, device="cuda")
+    bw_ws = lambda g_, s_, p_: lib.rsg_e3d_backward_velocity_ws(g_, s_, C.c_void_p(wsp.data_ptr()), p_)  # noqa: E731
     for name, f, b in (("velocity", lib.rsg_e3d_forward_velocity, 60), ("stress", lib.rsg_e3d_forward_stress, 80),
                        ("velocity_adjoint_state", lib.rsg_e3d_backward_velocity, 80),
+                       ("velocity_adjoint_state_workspace", bw_ws, 80),
                        ("stress_adjoint_state", lib.rsg_e3d_backward_stress, 60)):
         for _ in range(3):
             R.check(f(C.byref(g), C.byref(st), sp), name)
@@ -658,7 +663,7 @@ def e3d_leg():
         kt[name] = {"avg_launch_us": round(us, 1), "gcell_per_s": round(cells / us / 1e3, 2),
                     "bytes_per_cell": b, "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     _progress("e3d: half-step kernels timed")
-    del w, st
+    del w, st, wsp
     n, nt, dt = 120, 500, 1e-3
     m = core.ModelElastic3D(n, n, n, L, 10.0, 10.0, 10.0, fs=True)
     m.Vp[:], m.Vs[:], m.R[:] = 3000.0, 1700.0, 2000.0

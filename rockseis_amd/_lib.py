@@ -235,6 +235,7 @@ def _declare(L):
         "rsg_e3d_forward_stress": ([GP, C.POINTER(StateE3D), vp], ip),
         "rsg_e3d_backward_velocity": ([GP, C.POINTER(StateE3D), vp], ip),
         "rsg_e3d_backward_stress": ([GP, C.POINTER(StateE3D), vp], ip),
+        "rsg_e3d_backward_velocity_ws": ([GP, C.POINTER(StateE3D), vp, vp], ip),
         "rsg_fwi_e3d_gradient": ([GP, C.POINTER(FwiE3D), vp], ip),
         "rsg_e3d_insert_source": ([GP, C.POINTER(StateE3D), TP, ip, vp], ip),
         "rsg_e3d_record_data": ([GP, C.POINTER(StateE3D), TP, vp, vp, vp, ip, vp], ip),
@@ -280,7 +281,7 @@ EXPORTS = (
     "rsg_rss_read_header", "rsg_rss_read_data2d", "rsg_rss_read_data3d", "rsg_rss_write_data2d",
     "rsg_rss_write_data3d", "rsg_rss_read_regular", "rsg_rss_write_regular", "rsg_stack_image_2d",
     "rsg_stagger_e3d", "rsg_e3d_forward_velocity", "rsg_e3d_forward_stress", "rsg_e3d_backward_velocity",
-    "rsg_e3d_backward_stress", "rsg_fwi_e3d_gradient", "rsg_e3d_insert_source",
+    "rsg_e3d_backward_stress", "rsg_e3d_backward_velocity_ws", "rsg_fwi_e3d_gradient", "rsg_e3d_insert_source",
     "rsg_e3d_record_data",
 )
 

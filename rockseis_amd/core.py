@@ -672,9 +672,16 @@ class WavesElastic3D:
                                                _stream()), "rsg_e3d_forward_stress")
 
     def backwardstepVelocity(self, model, der):
-        """The adjoint-state velocity step of FwiElastic3D (waves.cpp:3260-3713)."""
-        R.check(R.lib().rsg_e3d_backward_velocity(C.byref(self._grid(der.order)), C.byref(self._state(model)),
-                                                  _stream()), "rsg_e3d_backward_velocity")
+        """The adjoint-state velocity step of FwiElastic3D (waves.cpp:3260-3713),
+        with a device workspace of six padded grids kept by this object (the
+        reference's wrk array)."""
+        ws = getattr(self, "_wrk", None)
+        if ws is None:
+            ws = self._wrk = torch.empty((6, self.nz_pml, self.ny_pml, self.nx_pml), dtype=torch.float32,
+                                         device=self.Vx.device)
+        R.check(R.lib().rsg_e3d_backward_velocity_ws(C.byref(self._grid(der.order)), C.byref(self._state(model)),
+                                                     C.c_void_p(ws.data_ptr()), _stream()),
+                "rsg_e3d_backward_velocity_ws")
 
     def backwardstepStress(self, model, der):
         """The adjoint-state stress step of FwiElastic3D (waves.cpp:3715-4103)."""

@@ -454,6 +454,10 @@ int rsg_e3d_backward_velocity(const rsg_grid *g, const rsg_state_e3d *s, void *s
     return RSG_OK;
 }
 
+int rsg_e3d_backward_velocity_ws(const rsg_grid *g, const rsg_state_e3d *s, float *workspace, void *stream) {
+    return rsg::e3d_backward_velocity_premul(g, s, workspace, (hipStream_t)stream);
+}
+
 int rsg_e3d_backward_stress(const rsg_grid *g, const rsg_state_e3d *s, void *stream) {
     if (int rc = rsg::check_e3d(g, s)) return rc;
     const rsg::K3E k = rsg::make_k3e(g, s);

@@ -136,6 +136,43 @@ def test_e3d_backward_steps_vs_oracle():
     assert worst <= 1e-6
 
 
+def test_e3d_backward_velocity_entries_agree():
+    """rsg_e3d_backward_velocity (weighted stresses formed at every tap) and
+    rsg_e3d_backward_velocity_ws (formed once into a workspace) give the same
+    step from a random state, order 6, free surface off."""
+    import ctypes as C
+    import torch
+    from rockseis_amd import _lib as R
+    from rockseis_amd import core
+    rng = np.random.default_rng(41)
+    nx, ny, nz, lpml, dx, dt = 19, 17, 21, 4, 10.0, 1e-3
+    m = core.ModelElastic3D(nx, ny, nz, lpml, dx, dx, dx, 0.0, 0.0, 0.0, False)
+    m.Vp[:] = 2400 + 500 * rng.random((nz, ny, nx))
+    m.Vs[:] = m.Vp / 1.8
+    m.R[:] = 1900 + 300 * rng.random((nz, ny, nx))
+    m.staggerModels()
+    ws = [core.WavesElastic3D(m, 10, dt) for _ in range(2)]
+    for k in core.WavesElastic3D.FIELDS:
+        a = torch.from_numpy((rng.standard_normal(getattr(ws[0], k).shape) * 1e-3).astype(np.float32))
+        for w in ws:
+            getattr(w, k).copy_(a)
+    for i in range(R.E3D_NSTRIP):
+        a = torch.from_numpy((rng.standard_normal(ws[0].psi[i].shape) * 1e-4).astype(np.float32))
+        for w in ws:
+            w.psi[i].copy_(a)
+    der = core.Der(ws[0].nx_pml, ws[0].ny_pml, ws[0].nz_pml, dx, dx, dx, 6)
+    R.check(R.lib().rsg_e3d_backward_velocity(C.byref(ws[0]._grid(6)), C.byref(ws[0]._state(m)), core._stream()),
+            "rsg_e3d_backward_velocity")
+    ws[1].backwardstepVelocity(m, der)
+    torch.cuda.synchronize()
+    for k in ("Vx", "Vy", "Vz"):
+        e = rel_l2(getattr(ws[1], k).cpu().numpy(), getattr(ws[0], k).cpu().numpy())
+        print(f"  {k}: rel L2 {e:.2e}")
+        assert e <= 1e-6, k
+    worst = max(rel_l2(ws[1].psi[i].cpu().numpy(), ws[0].psi[i].cpu().numpy()) for i in range(R.E3D_NSTRIP))
+    assert worst <= 1e-6
+
+
 def test_fwi_e3d_revolve_schedule_invariant(golden):
     g = golden("fwi_e3d_case0")
     a = _gpu(g, ncheck=3)
