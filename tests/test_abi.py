@@ -283,3 +283,50 @@ def test_modelling_plan_rejects_bad_arguments():
     assert h.value is None
     assert L.rsg_modelling_plan_run(None, 0, 1, None) != 0
     assert L.rsg_modelling_plan_destroy(None) == 0
+
+
+def test_fwi_e3d_rejects_bad_arguments():
+    """rsg_fwi_e3d_gradient and the elastic 3D adjoint-state entry points
+    validate their arguments before any device work: null arguments, a
+    source of another field, Ux / Uy / Uz records of the wrong fields or
+    trace counts, a domain model, an image of the wrong size, an unknown
+    storage kind or misfit type."""
+    L = R.lib()
+    g, f = R.Grid(), R.FwiE3D()
+    assert L.rsg_fwi_e3d_gradient(None, None, None) != 0
+    assert b"null" in L.rsg_last_error()
+    for n in ("L", "L2M", "Mxz", "Myz", "Mxy", "Rx", "Ry", "Rz", "tab", "Vp", "Vs", "Rho", "vpgrad", "vsgrad",
+              "rhograd"):
+        setattr(f, n, 64)
+    f.src.data, f.src.map, f.src.shift = 64, 64, 64
+    f.src.ntrace, f.src.nt, f.src.field = 1, 50, 5
+    assert L.rsg_fwi_e3d_gradient(C.byref(g), C.byref(f), None) != 0
+    assert b"source field" in L.rsg_last_error()
+    f.src.field = R.PRESSURE
+    for c, fld in enumerate((R.VX, R.VY, R.VZ)):
+        f.obs[c].data, f.obs[c].map, f.obs[c].shift = 64, 64, 64
+        f.obs[c].ntrace, f.obs[c].nt, f.obs[c].field = 4, 50, fld
+        f.dmod[c], f.dres[c] = 64, 64
+    f.obs[1].field = R.VX
+    assert L.rsg_fwi_e3d_gradient(C.byref(g), C.byref(f), None) != 0
+    assert b"VX, VY, VZ" in L.rsg_last_error()
+    f.obs[1].field, f.obs[2].ntrace = R.VY, 3
+    assert L.rsg_fwi_e3d_gradient(C.byref(g), C.byref(f), None) != 0
+    assert b"equal count" in L.rsg_last_error()
+    f.obs[2].ntrace = 4
+    g.nx, g.ny, g.nz, g.lpml, g.domdec = 20, 20, 20, 4, 1
+    assert L.rsg_fwi_e3d_gradient(C.byref(g), C.byref(f), None) != 0
+    assert b"domain" in L.rsg_last_error()
+    g.domdec = 0
+    f.nx, f.ny, f.nz = 12, 12, 11
+    assert L.rsg_fwi_e3d_gradient(C.byref(g), C.byref(f), None) != 0
+    assert b"unpadded" in L.rsg_last_error()
+    f.nz, f.fp16 = 12, 7
+    assert L.rsg_fwi_e3d_gradient(C.byref(g), C.byref(f), None) != 0
+    assert b"storage" in L.rsg_last_error()
+    f.fp16, f.misfit_type = 0, 9
+    assert L.rsg_fwi_e3d_gradient(C.byref(g), C.byref(f), None) != 0
+    assert b"misfit type" in L.rsg_last_error()
+    for fn in (L.rsg_e3d_backward_velocity, L.rsg_e3d_backward_stress):
+        assert fn(None, None, None) != 0
+    assert L.rsg_e3d_backward_velocity_ws(None, None, None, None) != 0
